@@ -1,0 +1,144 @@
+#!/usr/bin/env python3
+"""Benchmark of the EarthBox per-timestep hot path on one MI355X.
+
+Workload (1001 x 501 nodes over 1500 x 600 km, 6 x 6 markers per cell =
+18,000,000 markers, FP64): one step = one fused RK4 gather-advect pass of all
+markers (velocity + spin at 4 stages, position update) with inputs resident in
+HBM.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import platform
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "earthbox.jl_amd"))
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+RK4_BYTES_PER_MARKER = 32.0  # read x, y; write x, y (FP64)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--xnum", type=int, default=1001)
+    ap.add_argument("--ynum", type=int, default=501)
+    ap.add_argument("--markers-per-cell", type=int, default=6)
+    return ap.parse_args()
+
+
+def cpu_baseline(g, x, y, vx1, vy1, esp, dt, sample_markers: int):
+    """Oracle RK4 restatement timed on this host (test-infrastructure checker
+    used only as the reported CPU baseline)."""
+    import numpy as np
+    import oracle
+    xs, ys = x[:sample_markers].copy(), y[:sample_markers].copy()
+    flags = oracle.inside_flags(xs, ys, g.limits)
+    xn, yn = oracle.cell_indices_basic(xs, ys, flags, g)
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        oracle.advect_markers_and_rotate_stress_tensor(4, dt, g, xs, ys, xn, yn, flags,
+                                                       vx1, vy1, esp)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > 10.0:
+            break
+    rate = reps * sample_markers / el / 1e6
+    return {"value": rate, "unit": "Mmarkers/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} x RK4 advect of the first {sample_markers} markers of the same "
+                      f"workload (serial C restatement, {platform.processor() or 'host'} core)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1 "
+                          "(multi-GPU sharding is design-only, see DESIGN.md)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import earthbox_gpu
+    from earthbox_gpu import workloads
+    from earthbox_gpu.device import empty, to_device
+    from earthbox_gpu.grid import uniform_grid
+
+    nx, ny, nm = args.xnum, args.ynum, args.markers_per_cell
+    g = uniform_grid(nx, ny, 1.5e6 * (nx - 1) / 1000, 6.0e5 * (ny - 1) / 500)
+    x, y = workloads.marker_positions(g, nm, nm, 2)
+    vx1, vy1 = workloads.random_velocity_fields(g, 12, vmax=1e-9)
+    ctx = earthbox_gpu.GPUContext(0)
+    ctx.set_grid(g)
+    dvx1, dvy1 = to_device(vx1), to_device(vy1)
+    desp = empty((ny, nx), np.float64)
+    ctx.calculate_spin(dvx1, dvy1, desp)
+    dx, dy = to_device(x), to_device(y)
+    dt = 0.5 * g.xstpavg / 1e-9
+    n = x.size
+
+    def step():
+        ctx.advect_markers_and_rotate_stress_tensor(4, dt, None, dx, dy, None, None,
+                                                    dvx1, dvy1, desp)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s, e in evs:
+        s.record(stream)
+        step()
+        e.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([s.elapsed_time(e) for s, e in evs]))
+    ms_per_step = wall * 1e3 / args.steps
+    value = n / (ms_per_step * 1e-3) / 1e6
+    achieved = RK4_BYTES_PER_MARKER * n / (kern_ms * 1e-3) / 1e9
+    out = {
+        "metric": "RK4 marker advection throughput (Mmarkers/s); Stokes time-to-1e-8 reported separately",
+        "value": value,
+        "unit": "Mmarkers/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"rk4_advect {nx}x{ny} nodes, {nm}x{nm} markers/cell ({n} markers)",
+                   "markers": n, "xnum": nx, "ynum": ny},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "rk4_advect_kernel", "kernel_ms": kern_ms,
+                     "bytes_per_unit": RK4_BYTES_PER_MARKER},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(g, x, y, vx1, vy1, to_host_np(desp), dt, 2_000_000)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def to_host_np(t):
+    from earthbox_gpu.device import to_host
+    return to_host(t)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,68 @@
+// Shared plumbing for the EarthBox MI355X hot-path library: error capture,
+// the per-context stream, and device copies of the staggered-grid axes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "earthbox_gpu.h"
+
+namespace ebg {
+
+// Thread-local last error text for ebg_last_error(); the library never aborts.
+void set_error(const std::string& msg);
+
+#define EBG_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t e_ = (call);                                              \
+        if (e_ != hipSuccess) {                                              \
+            ::ebg::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return EBG_ERR_HIP;                                              \
+        }                                                                    \
+    } while (0)
+
+#define EBG_CHECK(cond, code, msg)        \
+    do {                                  \
+        if (!(cond)) {                    \
+            ::ebg::set_error(msg);        \
+            return (code);                \
+        }                                 \
+    } while (0)
+
+// Device copies of one 2-D staggered grid (all arrays FP64, lengths as in the
+// reference: basic gridx_b[xnum], xstp_b[xnum-1], vy-grid gridx_vy[xnum+1],
+// xstp_vy[xnum], vx-grid gridy_vx[ynum+1], ystp_vx[ynum]).
+struct Grid2D {
+    int64_t xnum = 0, ynum = 0;
+    double xstpavg = 0, ystpavg = 0;
+    double xmin = 0, xmax = 0, ymin = 0, ymax = 0;
+    // uniform-axis hints for O(1) cell search (exactness restored by fix-up)
+    double x0 = 0, y0 = 0, inv_hx = 0, inv_hy = 0;
+    int uniform_x = 0, uniform_y = 0;
+    double *gridx_b = nullptr, *gridy_b = nullptr;
+    double *xstp_b = nullptr, *ystp_b = nullptr;
+    double *gridx_vy = nullptr, *xstp_vy = nullptr;
+    double *gridy_vx = nullptr, *ystp_vx = nullptr;
+    std::vector<double> h_gridx_b, h_gridy_b, h_xstp_b, h_ystp_b;
+    std::vector<double> h_gridx_vy, h_xstp_vy, h_gridy_vx, h_ystp_vx;
+};
+
+}  // namespace ebg
+
+struct ebg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    ebg::Grid2D grid;
+    bool has_grid = false;
+    // solver state lives in a separate object created on demand
+    void* stokes = nullptr;
+    void* scatter = nullptr;
+    void* mg_ref = nullptr;
+};
+
+namespace ebg {
+int grid_release(Grid2D& g);
+}

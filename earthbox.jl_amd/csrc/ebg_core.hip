@@ -1,0 +1,184 @@
+// Context, memory helpers, error text and grid upload for the C ABI.
+#include "ebg_common.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace ebg {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int grid_release(Grid2D& g)
+{
+    double** ptrs[] = {&g.gridx_b, &g.gridy_b, &g.xstp_b, &g.ystp_b,
+                       &g.gridx_vy, &g.xstp_vy, &g.gridy_vx, &g.ystp_vx};
+    for (double** p : ptrs) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    return EBG_OK;
+}
+
+static int upload(double** dst, const double* src, int64_t n, std::vector<double>& host)
+{
+    host.assign(src, src + n);
+    EBG_HIP(hipMalloc(dst, sizeof(double) * (size_t)n));
+    EBG_HIP(hipMemcpy(*dst, src, sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+    return EBG_OK;
+}
+
+// An axis is "uniform" when every node lies within a quarter cell of
+// x0 + (i-1)*h; the O(1) cell guess is then off by at most one and the exact
+// comparison fix-up restores searchsortedlast semantics.
+static int axis_uniform(const double* g, int64_t n, double* x0, double* inv_h)
+{
+    double h = (g[n - 1] - g[0]) / (double)(n - 1);
+    *x0 = g[0];
+    *inv_h = 1.0 / h;
+    if (!(h > 0)) return 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (std::fabs(g[i] - (g[0] + (double)i * h)) > 0.25 * h) return 0;
+    return 1;
+}
+
+}  // namespace ebg
+
+using namespace ebg;
+
+extern "C" {
+
+const char* ebg_last_error(void) { return g_last_error.c_str(); }
+
+int ebg_ctx_create(ebg_ctx** out, int device)
+{
+    EBG_CHECK(out != nullptr, EBG_ERR_ARG, "ebg_ctx_create: out is NULL");
+    int ndev = 0;
+    EBG_HIP(hipGetDeviceCount(&ndev));
+    EBG_CHECK(device >= 0 && device < ndev, EBG_ERR_ARG, "ebg_ctx_create: bad device index");
+    EBG_HIP(hipSetDevice(device));
+    ebg_ctx* c = new ebg_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        set_error(std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        return EBG_ERR_HIP;
+    }
+    c->own_stream = true;
+    *out = c;
+    return EBG_OK;
+}
+
+int ebg_stokes_release(ebg_ctx* ctx);
+int ebg_scatter_release(ebg_ctx* ctx);
+int ebg_mgref_release(ebg_ctx* ctx);
+
+int ebg_ctx_destroy(ebg_ctx* ctx)
+{
+    if (!ctx) return EBG_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ebg_stokes_release(ctx);
+    ebg_scatter_release(ctx);
+    ebg_mgref_release(ctx);
+    grid_release(ctx->grid);
+    if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return EBG_OK;
+}
+
+int ebg_ctx_set_stream(ebg_ctx* ctx, void* hip_stream)
+{
+    EBG_CHECK(ctx != nullptr, EBG_ERR_ARG, "ebg_ctx_set_stream: ctx is NULL");
+    if (hip_stream) {
+        if (ctx->own_stream && ctx->stream) {
+            EBG_HIP(hipStreamSynchronize(ctx->stream));
+            EBG_HIP(hipStreamDestroy(ctx->stream));
+        }
+        ctx->stream = (hipStream_t)hip_stream;
+        ctx->own_stream = false;
+    } else if (!ctx->own_stream) {
+        EBG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        ctx->own_stream = true;
+    }
+    return EBG_OK;
+}
+
+int ebg_ctx_synchronize(ebg_ctx* ctx)
+{
+    EBG_CHECK(ctx != nullptr, EBG_ERR_ARG, "ebg_ctx_synchronize: ctx is NULL");
+    EBG_HIP(hipStreamSynchronize(ctx->stream));
+    return EBG_OK;
+}
+
+int ebg_device_alloc(void** ptr, size_t bytes)
+{
+    EBG_CHECK(ptr != nullptr, EBG_ERR_ARG, "ebg_device_alloc: ptr is NULL");
+    EBG_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+    return EBG_OK;
+}
+
+int ebg_device_free(void* ptr)
+{
+    if (ptr) EBG_HIP(hipFree(ptr));
+    return EBG_OK;
+}
+
+int ebg_memcpy_h2d(void* dst, const void* src, size_t bytes)
+{
+    EBG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return EBG_OK;
+}
+
+int ebg_memcpy_d2h(void* dst, const void* src, size_t bytes)
+{
+    EBG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return EBG_OK;
+}
+
+int ebg_grid2d_set(ebg_ctx* ctx, int64_t xnum, int64_t ynum,
+                   const double* gridx_b, const double* gridy_b,
+                   const double* xstp_b, const double* ystp_b,
+                   const double* gridx_vy, const double* xstp_vy,
+                   const double* gridy_vx, const double* ystp_vx,
+                   double xstpavg, double ystpavg)
+{
+    EBG_CHECK(ctx != nullptr, EBG_ERR_ARG, "ebg_grid2d_set: ctx is NULL");
+    EBG_CHECK(xnum >= 3 && ynum >= 3, EBG_ERR_ARG, "ebg_grid2d_set: need xnum, ynum >= 3");
+    EBG_CHECK(xnum < (1 << 30) && ynum < (1 << 30), EBG_ERR_ARG, "ebg_grid2d_set: grid too large");
+    EBG_CHECK(gridx_b && gridy_b && xstp_b && ystp_b && gridx_vy && xstp_vy && gridy_vx && ystp_vx,
+              EBG_ERR_ARG, "ebg_grid2d_set: NULL grid array");
+    for (int64_t i = 1; i < xnum; ++i)
+        EBG_CHECK(gridx_b[i] > gridx_b[i - 1], EBG_ERR_ARG, "ebg_grid2d_set: gridx_b not increasing");
+    for (int64_t i = 1; i < ynum; ++i)
+        EBG_CHECK(gridy_b[i] > gridy_b[i - 1], EBG_ERR_ARG, "ebg_grid2d_set: gridy_b not increasing");
+    EBG_HIP(hipSetDevice(ctx->device));
+    EBG_HIP(hipStreamSynchronize(ctx->stream));
+    Grid2D& g = ctx->grid;
+    grid_release(g);
+    g.xnum = xnum;
+    g.ynum = ynum;
+    g.xstpavg = xstpavg;
+    g.ystpavg = ystpavg;
+    int rc;
+    if ((rc = upload(&g.gridx_b, gridx_b, xnum, g.h_gridx_b))) return rc;
+    if ((rc = upload(&g.gridy_b, gridy_b, ynum, g.h_gridy_b))) return rc;
+    if ((rc = upload(&g.xstp_b, xstp_b, xnum - 1, g.h_xstp_b))) return rc;
+    if ((rc = upload(&g.ystp_b, ystp_b, ynum - 1, g.h_ystp_b))) return rc;
+    if ((rc = upload(&g.gridx_vy, gridx_vy, xnum + 1, g.h_gridx_vy))) return rc;
+    if ((rc = upload(&g.xstp_vy, xstp_vy, xnum, g.h_xstp_vy))) return rc;
+    if ((rc = upload(&g.gridy_vx, gridy_vx, ynum + 1, g.h_gridy_vx))) return rc;
+    if ((rc = upload(&g.ystp_vx, ystp_vx, ynum, g.h_ystp_vx))) return rc;
+    g.xmin = gridx_b[0];
+    g.xmax = gridx_b[xnum - 1];
+    g.ymin = gridy_b[0];
+    g.ymax = gridy_b[ynum - 1];
+    g.uniform_x = axis_uniform(gridx_b, xnum, &g.x0, &g.inv_hx);
+    g.uniform_y = axis_uniform(gridy_b, ynum, &g.y0, &g.inv_hy);
+    ctx->has_grid = true;
+    return EBG_OK;
+}
+
+}  // extern "C"

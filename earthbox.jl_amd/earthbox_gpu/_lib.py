@@ -1,0 +1,68 @@
+"""ctypes binding of ``libearthbox_gpu.so`` (the C ABI in include/earthbox_gpu.h).
+
+The library is loaded from this directory.  There is no fallback: if the
+shared library is missing or a symbol is absent, importing raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+_HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libearthbox_gpu.so"
+
+EBG_OK = 0
+EBG_NOT_CONVERGED = 1
+
+c_int64 = C.c_int64
+c_int = C.c_int
+c_double = C.c_double
+c_void_p = C.c_void_p
+c_size_t = C.c_size_t
+c_char_p = C.c_char_p
+
+# name -> (restype, argtypes).  Pointers to device memory are passed as c_void_p.
+SIGNATURES: dict[str, tuple] = {
+    "ebg_ctx_create": (c_int, [C.POINTER(c_void_p), c_int]),
+    "ebg_ctx_destroy": (c_int, [c_void_p]),
+    "ebg_ctx_set_stream": (c_int, [c_void_p, c_void_p]),
+    "ebg_ctx_synchronize": (c_int, [c_void_p]),
+    "ebg_last_error": (c_char_p, []),
+    "ebg_device_alloc": (c_int, [C.POINTER(c_void_p), c_size_t]),
+    "ebg_device_free": (c_int, [c_void_p]),
+    "ebg_memcpy_h2d": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "ebg_memcpy_d2h": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "ebg_grid2d_set": (c_int, [c_void_p, c_int64, c_int64] + [c_void_p] * 8 + [c_double, c_double]),
+    "ebg_marker_inside_flags": (c_int, [c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    "ebg_marker_cell_indices": (c_int, [c_void_p, c_int64] + [c_void_p] * 5),
+    "ebg_rk4_advect": (c_int, [c_void_p, c_int64, c_int, c_double] + [c_void_p] * 13 + [c_int]),
+    "ebg_calculate_spin": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+}
+
+
+class EBGError(RuntimeError):
+    """A hard error reported by the library (negative return code)."""
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is not built; run __graft_entry__.build() "
+            "(or python earthbox.jl_amd/build_lib.py) first")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)  # AttributeError if the export is missing
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(rc: int, what: str) -> int:
+    """Raise EBGError for rc < 0; return rc otherwise (0 ok, >0 not converged)."""
+    if rc < 0:
+        msg = lib.ebg_last_error().decode(errors="replace")
+        raise EBGError(f"{what} failed (rc={rc}): {msg}")
+    return rc

@@ -1,0 +1,128 @@
+"""CPU oracle for the EarthBox hot path -- TEST INFRASTRUCTURE ONLY.
+
+A plain-C restatement (liboracle.so, built by oracle/Makefile) of the reference
+Julia loops plus a scipy direct solve of the restated assembled system.  Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use it, and
+only as the checker or the timed CPU baseline -- never as a product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_P = C.c_void_p
+_I64 = C.c_int64
+_D = C.c_double
+
+_SIGS = {
+    "ebo_grid2d_uniform": (None, [_I64, _I64, _D, _D] + [_P] * 14),
+    "ebo_index_of_left_node": (C.c_int32, [_D, _P, _I64]),
+    "ebo_inside_flags": (None, [_I64, _P, _P, _D, _D, _D, _D, _P]),
+    "ebo_cell_indices_basic": (None, [_I64, _P, _P, _P, _P, _I64, _P, _I64, _P, _P]),
+    "ebo_rk4_velocity_and_spin": (None, [_I64, C.c_int, _D, _I64, _I64] + [_P] * 19),
+    "ebo_update_marker_location": (None, [_I64, _D, _P, _P, _P, _P, _P]),
+    "ebo_rotate_marker_stress": (None, [_I64, _D, _P, _P, _P, _P]),
+    "ebo_calculate_spin": (None, [_I64, _I64, _P, _P, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def build() -> None:
+    """Compile liboracle.so from the C sources in this directory."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def p(a: np.ndarray | None):
+    """Pointer of a numpy array (Fortran order for matrices), or NULL."""
+    if a is None:
+        return None
+    if a.ndim == 2 and not a.flags.f_contiguous:
+        raise ValueError("matrices must be Fortran-ordered (Julia layout)")
+    if a.ndim == 1 and not a.flags.c_contiguous:
+        raise ValueError("vectors must be contiguous")
+    return a.ctypes.data
+
+
+def grid2d_uniform(xnum, ynum, xsize, ysize) -> dict:
+    out = {
+        "gridx_b": np.empty(xnum), "gridy_b": np.empty(ynum),
+        "xstp_b": np.empty(xnum - 1), "ystp_b": np.empty(ynum - 1),
+        "gridx_vy": np.empty(xnum + 1), "xstp_vy": np.empty(xnum),
+        "gridy_vx": np.empty(ynum + 1), "ystp_vx": np.empty(ynum),
+        "gridx_pr": np.empty(xnum - 1), "gridy_pr": np.empty(ynum - 1),
+        "xstp_pr": np.empty(xnum - 2), "ystp_pr": np.empty(ynum - 2),
+    }
+    xa, ya = C.c_double(), C.c_double()
+    lib().ebo_grid2d_uniform(xnum, ynum, xsize, ysize, C.addressof(xa), C.addressof(ya),
+                             *[p(out[k]) for k in ("gridx_b", "gridy_b", "xstp_b", "ystp_b",
+                                                   "gridx_vy", "xstp_vy", "gridy_vx", "ystp_vx",
+                                                   "gridx_pr", "gridy_pr", "xstp_pr", "ystp_pr")])
+    out["xstpavg"], out["ystpavg"] = xa.value, ya.value
+    return out
+
+
+def index_of_left_node(x: float, grid: np.ndarray) -> int:
+    return int(lib().ebo_index_of_left_node(float(x), p(grid), grid.size))
+
+
+def inside_flags(x, y, limits) -> np.ndarray:
+    f = np.empty(x.size, dtype=np.int8)
+    lib().ebo_inside_flags(x.size, p(x), p(y), *limits, p(f))
+    return f
+
+
+def cell_indices_basic(x, y, flags, g) -> tuple[np.ndarray, np.ndarray]:
+    xn = np.zeros(x.size, dtype=np.int32)
+    yn = np.zeros(x.size, dtype=np.int32)
+    lib().ebo_cell_indices_basic(x.size, p(x), p(y), p(flags), p(g.gridx_b), g.xnum,
+                                 p(g.gridy_b), g.ynum, p(xn), p(yn))
+    return xn, yn
+
+
+def rk4_velocity_and_spin(order, timestep, g, x, y, xn, yn, flags, vx1, vy1, esp):
+    n = x.size
+    mvx, mvy, msp = np.empty(n), np.empty(n), np.empty(n)
+    lib().ebo_rk4_velocity_and_spin(
+        n, order, timestep, g.xnum, g.ynum, p(x), p(y), p(xn), p(yn), p(flags),
+        p(g.gridx_b), p(g.gridy_b), p(g.gridx_vy), p(g.gridy_vx), p(g.xstp_b), p(g.ystp_b),
+        p(g.xstp_vy), p(g.ystp_vx), p(vx1), p(vy1), p(esp), p(mvx), p(mvy), p(msp))
+    return mvx, mvy, msp
+
+
+def advect_markers_and_rotate_stress_tensor(order, timestep, g, x, y, xn, yn, flags,
+                                            vx1, vy1, esp, sxx=None, sxy=None):
+    """Advection.jl:109-114 on copies: returns (x, y, vx, vy, spin, sxx, sxy)."""
+    mvx, mvy, msp = rk4_velocity_and_spin(order, timestep, g, x, y, xn, yn, flags, vx1, vy1, esp)
+    x2, y2 = x.copy(), y.copy()
+    lib().ebo_update_marker_location(x.size, timestep, p(flags), p(mvx), p(mvy), p(x2), p(y2))
+    if sxx is not None:
+        sxx, sxy = sxx.copy(), sxy.copy()
+        lib().ebo_rotate_marker_stress(x.size, timestep, p(flags), p(msp), p(sxx), p(sxy))
+    return x2, y2, mvx, mvy, msp, sxx, sxy
+
+
+def calculate_spin(g, vx1, vy1) -> np.ndarray:
+    esp = np.zeros((g.ynum, g.xnum), order="F")
+    lib().ebo_calculate_spin(g.xnum, g.ynum, p(g.xstp_vy), p(g.ystp_vx), p(vx1), p(vy1), p(esp))
+    return esp
