@@ -1,0 +1,352 @@
+// Host side of the multigrid hierarchy: level planning, uniform level grids,
+// fine->coarse maps with gather ranges, device allocation and V-cycle driver.
+#include "ebg_common.hpp"
+#include "ebg_mg.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace ebg {
+
+// multigrid/level_manager/LevelManager.jl:441-457
+std::vector<std::pair<int, int>> mg_level_sizes(int xnum, int ynum, int max_levels)
+{
+    std::vector<std::pair<int, int>> s{{xnum, ynum}};
+    while (std::min(s.back().first, s.back().second) > 4) {
+        const int xn = (s.back().first + 1) / 2, yn = (s.back().second + 1) / 2;
+        if (std::min(xn, yn) < 4 || (int)s.size() >= max_levels) break;
+        s.push_back({xn, yn});
+    }
+    return s;
+}
+
+// multigrid_data/MultigridDataManager.jl:223-241
+std::vector<int> mg_smoothing_counts(int nlev, int finest, int multiplier, int cap)
+{
+    std::vector<int> s(nlev);
+    s[0] = finest;
+    for (int i = 1; i < nlev; ++i) s[i] = s[i - 1] * multiplier;
+    if (cap > 0)
+        for (int i = 1; i < nlev; ++i) s[i] = std::min(s[i], cap);
+    return s;
+}
+
+static void basic_axis(int n, double step, std::vector<double>& g, std::vector<double>& stp)
+{
+    g.assign(n, 0.0);
+    for (int i = 1; i < n; ++i) g[i] = g[i - 1] + step;
+    stp.assign(n - 1, 0.0);
+    for (int i = 0; i < n - 1; ++i) stp[i] = g[i + 1] - g[i];
+}
+static void staggered_axis(int n, const std::vector<double>& g, const std::vector<double>& stp,
+                           std::vector<double>& gs, std::vector<double>& stps)
+{
+    gs.assign(n + 1, 0.0);
+    gs[0] = g[0] - stp[0] / 2.0;
+    gs[n] = g[n - 1] + stp[n - 2] / 2.0;
+    for (int i = 1; i < n; ++i) gs[i] = (g[i] + g[i - 1]) / 2.0;
+    stps.assign(n, 0.0);
+    stps[0] = stp[0];
+    stps[n - 1] = stp[n - 2];
+    for (int i = 1; i < n - 1; ++i) stps[i] = (g[i + 1] - g[i - 1]) / 2.0;
+}
+static void pressure_axis(int n, const std::vector<double>& g, std::vector<double>& gp)
+{
+    std::vector<double> stpp(n - 2);
+    for (int i = 0; i < n - 2; ++i) stpp[i] = (g[i + 2] - g[i]) / 2.0;
+    gp.assign(n - 1, 0.0);
+    gp[0] = (g[0] + g[1]) / 2.0;
+    for (int i = 1; i < n - 1; ++i) gp[i] = gp[i - 1] + stpp[i - 1];
+}
+
+// Uniform level grid as staggered_grid/types_manager/uniform/core/Initialization.jl:27-43
+// and the staggered-grid tools build it.
+HostLevelGrid make_uniform_level_grid(int xnum, int ynum, double xsize, double ysize)
+{
+    HostLevelGrid h;
+    h.xnum = xnum;
+    h.ynum = ynum;
+    basic_axis(xnum, xsize / (double)(xnum - 1), h.gridx_b, h.xstp_b);
+    basic_axis(ynum, ysize / (double)(ynum - 1), h.gridy_b, h.ystp_b);
+    staggered_axis(xnum, h.gridx_b, h.xstp_b, h.gridx_vy, h.xstp_vy);
+    staggered_axis(ynum, h.gridy_b, h.ystp_b, h.gridy_vx, h.ystp_vx);
+    pressure_axis(xnum, h.gridx_b, h.gridx_pr);
+    pressure_axis(ynum, h.gridy_b, h.gridy_pr);
+    return h;
+}
+
+// clamp(searchsortedlast(g, x), 1, n-1) (MarkerGridMapping.jl:78-86)
+static int left_index_host(double x, const std::vector<double>& g)
+{
+    const int n = (int)g.size();
+    int lo = 0, hi = n + 1;
+    while (lo < hi - 1) {
+        const int m = lo + (hi - lo) / 2;
+        if (x < g[m - 1]) hi = m;
+        else lo = m;
+    }
+    return std::max(1, std::min(lo, n - 1));
+}
+
+template <typename T>
+struct HostAxisMap {
+    std::vector<int> idx, lo, hi;
+    std::vector<T> d;
+};
+
+// GridMappingManager.jl:289-338 for one axis, plus the contiguous fine range
+// [lo, hi] within [rlo, rhi] that feeds each coarse node.
+template <typename T>
+static HostAxisMap<T> axis_map(const std::vector<double>& ga, const std::vector<double>& gb,
+                               int rlo, int rhi)
+{
+    HostAxisMap<T> m;
+    const int na = (int)ga.size(), nb = (int)gb.size();
+    m.idx.resize(na);
+    m.d.resize(na);
+    for (int k = 0; k < na; ++k) {
+        const int ib = left_index_host(ga[k], gb);
+        const double x0 = gb[ib - 1];
+        const double w = gb[ib] - x0;
+        m.idx[k] = ib;
+        m.d[k] = (T)((ga[k] - x0) / w);
+    }
+    m.lo.assign(nb, 1);
+    m.hi.assign(nb, 0);
+    for (int kc = 1; kc <= nb; ++kc) {
+        int lo = -1, hi = -2;
+        for (int i = rlo; i <= rhi; ++i) {
+            const int ix = m.idx[i - 1];
+            if (ix == kc - 1 || ix == kc) {
+                if (lo < 0) lo = i;
+                hi = i;
+            }
+        }
+        if (lo > 0) {
+            m.lo[kc - 1] = lo;
+            m.hi[kc - 1] = hi;
+        }
+    }
+    return m;
+}
+
+template <typename T>
+static int dev_upload(MGHierarchy<T>& H, const void* src, size_t bytes, void** out)
+{
+    void* p = nullptr;
+    EBG_HIP(hipMalloc(&p, bytes ? bytes : 4));
+    H.allocations.push_back(p);
+    if (bytes) EBG_HIP(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
+    *out = p;
+    return EBG_OK;
+}
+
+template <typename T>
+static int dev_zeros(MGHierarchy<T>& H, size_t n, T** out)
+{
+    void* p = nullptr;
+    EBG_HIP(hipMalloc(&p, sizeof(T) * (n ? n : 1)));
+    H.allocations.push_back(p);
+    EBG_HIP(hipMemset(p, 0, sizeof(T) * (n ? n : 1)));
+    *out = (T*)p;
+    return EBG_OK;
+}
+
+template <typename T>
+static int upload_axis(MGHierarchy<T>& H, const HostAxisMap<T>& m, AxisMapDev<T>& d)
+{
+    int rc;
+    void* p;
+    if ((rc = dev_upload(H, m.idx.data(), m.idx.size() * sizeof(int), &p))) return rc;
+    d.idx = (const int*)p;
+    if ((rc = dev_upload(H, m.d.data(), m.d.size() * sizeof(T), &p))) return rc;
+    d.d = (const T*)p;
+    if ((rc = dev_upload(H, m.lo.data(), m.lo.size() * sizeof(int), &p))) return rc;
+    d.lo = (const int*)p;
+    if ((rc = dev_upload(H, m.hi.data(), m.hi.size() * sizeof(int), &p))) return rc;
+    d.hi = (const int*)p;
+    return EBG_OK;
+}
+
+template <typename T>
+static int upload_vec(MGHierarchy<T>& H, const std::vector<double>& v, const T** out)
+{
+    std::vector<T> t(v.begin(), v.end());
+    void* p;
+    int rc = dev_upload(H, t.data(), t.size() * sizeof(T), &p);
+    *out = (const T*)p;
+    return rc;
+}
+
+template <typename T>
+MGHierarchy<T>::~MGHierarchy()
+{
+    for (void* p : allocations) (void)hipFree(p);
+}
+
+template <typename T>
+int mg_build(MGHierarchy<T>& H, int xnum, int ynum, double xsize, double ysize, int nlevels,
+             const std::vector<HostLevelGrid>* override_grids)
+{
+    auto sizes = mg_level_sizes(xnum, ynum, 10);
+    if (nlevels > 0 && nlevels < (int)sizes.size()) sizes.resize(nlevels);
+    H.hgrid.clear();
+    for (size_t k = 0; k < sizes.size(); ++k) {
+        if (k == 0 && override_grids && !override_grids->empty()) H.hgrid.push_back((*override_grids)[0]);
+        else H.hgrid.push_back(make_uniform_level_grid(sizes[k].first, sizes[k].second, xsize, ysize));
+    }
+    const int nl = (int)H.hgrid.size();
+    H.lev.resize(nl);
+    int rc;
+    for (int k = 0; k < nl; ++k) {
+        const HostLevelGrid& g = H.hgrid[k];
+        MGLevelDev<T>& l = H.lev[k];
+        l.xnum = g.xnum;
+        l.ynum = g.ynum;
+        const size_t ny = g.ynum, nx = g.xnum;
+        if ((rc = upload_vec(H, g.xstp_b, &l.xstp_b))) return rc;
+        if ((rc = upload_vec(H, g.ystp_b, &l.ystp_b))) return rc;
+        if ((rc = upload_vec(H, g.xstp_vy, &l.xstp_vy))) return rc;
+        if ((rc = upload_vec(H, g.ystp_vx, &l.ystp_vx))) return rc;
+        if ((rc = dev_zeros(H, ny * nx, &l.etas))) return rc;
+        if ((rc = dev_zeros(H, (ny - 1) * (nx - 1), &l.etan))) return rc;
+        if ((rc = dev_zeros(H, (ny + 1) * nx, &l.vx))) return rc;
+        if ((rc = dev_zeros(H, ny * (nx + 1), &l.vy))) return rc;
+        if ((rc = dev_zeros(H, (ny - 1) * (nx - 1), &l.pr))) return rc;
+        if ((rc = dev_zeros(H, (ny + 1) * nx, &l.RX))) return rc;
+        if ((rc = dev_zeros(H, ny * (nx + 1), &l.RY))) return rc;
+        if ((rc = dev_zeros(H, (ny - 1) * (nx - 1), &l.RC))) return rc;
+        if ((rc = dev_zeros(H, (ny + 1) * nx, &l.rx))) return rc;
+        if ((rc = dev_zeros(H, ny * (nx + 1), &l.ry))) return rc;
+        if ((rc = dev_zeros(H, (ny - 1) * (nx - 1), &l.rc))) return rc;
+    }
+    H.xfer.resize(nl > 0 ? nl - 1 : 0);
+    for (int k = 0; k + 1 < nl; ++k) {
+        const HostLevelGrid& f = H.hgrid[k];
+        const HostLevelGrid& c = H.hgrid[k + 1];
+        const int yf = f.ynum, xf = f.xnum;
+        TransferDev<T>& t = H.xfer[k];
+        // residual restriction loop ranges, Restriction.jl:444-463
+        if ((rc = upload_axis(H, axis_map<T>(f.gridy_vx, c.gridy_vx, 2, yf), t.vx_y))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridx_b, c.gridx_b, 2, xf - 1), t.vx_x))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridy_b, c.gridy_b, 2, yf - 1), t.vy_y))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridx_vy, c.gridx_vy, 2, xf), t.vy_x))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridy_pr, c.gridy_pr, 2, yf - 1), t.pr_y))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridx_pr, c.gridx_pr, 2, xf - 1), t.pr_x))) return rc;
+        // viscosity restriction ranges, ViscosityRestriction.jl:202-215
+        if ((rc = upload_axis(H, axis_map<T>(f.gridy_b, c.gridy_b, 1, yf), t.b_y))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridx_b, c.gridx_b, 1, xf), t.b_x))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridy_pr, c.gridy_pr, 1, yf - 1), t.e_y))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridx_pr, c.gridx_pr, 1, xf - 1), t.e_x))) return rc;
+    }
+    if ((rc = dev_zeros(H, 4, &H.scratch))) return rc;
+    return EBG_OK;
+}
+
+static inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename T>
+void mg_restrict_viscosity(MGHierarchy<T>& H, hipStream_t s)
+{
+    for (int k = 0; k + 1 < H.nlev(); ++k) {
+        const MGLevelDev<T>& f = H.lev[k];
+        const MGLevelDev<T>& c = H.lev[k + 1];
+        const TransferDev<T>& t = H.xfer[k];
+        hipLaunchKernelGGL((mg_visc_restrict_kernel<T, 0>), dim3(nblk((int64_t)c.ynum * c.xnum)),
+                           dim3(256), 0, s, f, c, t.b_y, t.b_x);
+        hipLaunchKernelGGL((mg_visc_restrict_kernel<T, 1>),
+                           dim3(nblk((int64_t)(c.ynum - 1) * (c.xnum - 1))), dim3(256), 0, s, f, c,
+                           t.e_y, t.e_x);
+    }
+}
+
+template <typename T>
+void mg_zero_solution(MGHierarchy<T>& H, int k, hipStream_t s)
+{
+    const MGLevelDev<T>& l = H.lev[k];
+    const int64_t nvx = (int64_t)(l.ynum + 1) * l.xnum, nvy = (int64_t)l.ynum * (l.xnum + 1),
+                  np = (int64_t)(l.ynum - 1) * (l.xnum - 1);
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(nblk(nvx)), dim3(256), 0, s, l.vx, nvx, T(0));
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(nblk(nvy)), dim3(256), 0, s, l.vy, nvy, T(0));
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(nblk(np)), dim3(256), 0, s, l.pr, np, T(0));
+}
+
+// smoother/Smoother.jl:66-99 with StokesSolverGaussSeidel.jl:33-54
+template <typename T>
+void mg_smoother(MGHierarchy<T>& H, int k, int nsweeps, T theta_s, T theta_c, T prnorm,
+                 const T* prnorm_dev, const T* bc_d, hipStream_t s)
+{
+    const MGLevelDev<T>& l = H.lev[k];
+    const int nbc = mg_bc_threads(l.xnum, l.ynum);
+    const int64_t nvx = (int64_t)(l.ynum - 1) * (l.xnum - 2);
+    const int64_t nvy = (int64_t)(l.ynum - 2) * (l.xnum - 1);
+    const int64_t np = (int64_t)(l.ynum - 1) * (l.xnum - 1);
+    for (int it = 0; it < nsweeps; ++it) {
+        hipLaunchKernelGGL(mg_bc_kernel<T>, dim3(nblk(nbc)), dim3(256), 0, s, l, bc_d[0], bc_d[1],
+                           bc_d[2], bc_d[3]);
+        for (int color = 0; color <= 1; ++color) {
+            hipLaunchKernelGGL((mg_sweep_kernel<T, 0>), dim3(nblk(nvx)), dim3(256), 0, s, l, color,
+                               theta_s, theta_c);
+            hipLaunchKernelGGL((mg_sweep_kernel<T, 1>), dim3(nblk(nvy)), dim3(256), 0, s, l, color,
+                               theta_s, theta_c);
+            hipLaunchKernelGGL((mg_sweep_kernel<T, 2>), dim3(nblk(np)), dim3(256), 0, s, l, color,
+                               theta_s, theta_c);
+        }
+    }
+    hipLaunchKernelGGL(mg_pressure_shift_kernel<T>, dim3(1), dim3(64), 0, s, l.pr, prnorm_dev, prnorm,
+                       H.scratch);
+    hipLaunchKernelGGL(mg_pressure_add_kernel<T>, dim3(nblk(np)), dim3(256), 0, s, l.pr, (int)np,
+                       H.scratch);
+    hipLaunchKernelGGL(mg_residual_kernel<T>, dim3(nblk((int64_t)(l.ynum + 1) * (l.xnum + 1))),
+                       dim3(256), 0, s, l);
+}
+
+// vcycle/core/SmoothAndRestrict.jl:44-71 and SmoothAndProlongate.jl:60-115
+template <typename T>
+void mg_vcycle(MGHierarchy<T>& H, const std::vector<int>& sm, T theta_s, T theta_c, T relax_v,
+               T relax_p, T prnorm, const T* prnorm_dev, const T* bc_d, hipStream_t s)
+{
+    const int nl = H.nlev();
+    for (int n = 0; n < nl; ++n) {
+        if (n > 0) mg_zero_solution(H, n, s);
+        mg_smoother(H, n, sm[n], theta_s, theta_c, prnorm, prnorm_dev, bc_d, s);
+        if (n < nl - 1) {
+            const MGLevelDev<T>& f = H.lev[n];
+            const MGLevelDev<T>& c = H.lev[n + 1];
+            const TransferDev<T>& t = H.xfer[n];
+            hipLaunchKernelGGL((mg_restrict_kernel<T, 0>), dim3(nblk((int64_t)(c.ynum + 1) * c.xnum)),
+                               dim3(256), 0, s, f, c, t.vx_y, t.vx_x);
+            hipLaunchKernelGGL((mg_restrict_kernel<T, 1>), dim3(nblk((int64_t)c.ynum * (c.xnum + 1))),
+                               dim3(256), 0, s, f, c, t.vy_y, t.vy_x);
+            hipLaunchKernelGGL((mg_restrict_kernel<T, 2>),
+                               dim3(nblk((int64_t)(c.ynum - 1) * (c.xnum - 1))), dim3(256), 0, s, f, c,
+                               t.pr_y, t.pr_x);
+        }
+    }
+    for (int n = nl - 1; n >= 0; --n) {
+        if (n == 0) {
+            mg_smoother(H, 0, sm[0], theta_s, theta_c, prnorm, prnorm_dev, bc_d, s);
+        } else {
+            mg_smoother(H, n, sm[n], theta_s, theta_c, T(0), (const T*)nullptr, bc_d, s);
+            const MGLevelDev<T>& c = H.lev[n];
+            const MGLevelDev<T>& f = H.lev[n - 1];
+            hipLaunchKernelGGL(mg_prolong_kernel<T>, dim3(nblk((int64_t)(f.ynum + 1) * (f.xnum + 1))),
+                               dim3(256), 0, s, c, f, H.xfer[n - 1], relax_v, relax_p);
+        }
+    }
+}
+
+#define EBG_INST(T)                                                                              \
+    template struct MGHierarchy<T>;                                                              \
+    template int mg_build<T>(MGHierarchy<T>&, int, int, double, double, int,                     \
+                             const std::vector<HostLevelGrid>*);                                 \
+    template void mg_restrict_viscosity<T>(MGHierarchy<T>&, hipStream_t);                        \
+    template void mg_smoother<T>(MGHierarchy<T>&, int, int, T, T, T, const T*, const T*,         \
+                                 hipStream_t);                                                   \
+    template void mg_vcycle<T>(MGHierarchy<T>&, const std::vector<int>&, T, T, T, T, T, const T*, \
+                               const T*, hipStream_t);                                           \
+    template void mg_zero_solution<T>(MGHierarchy<T>&, int, hipStream_t);
+EBG_INST(float)
+EBG_INST(double)
+
+}  // namespace ebg

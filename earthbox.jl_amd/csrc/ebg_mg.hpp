@@ -1,0 +1,401 @@
+// Geometric multigrid for the 2-D staggered Stokes-continuity equations
+// (the reference's experimental solver, src/stokes_continuity_solver/multigrid/),
+// as HIP kernels templated on the working precision T (double: plain FP64;
+// float: the mixed-precision preconditioner).
+//
+//  - smoother: Gauss-Seidel velocity updates v += dR/diag*theta_s and the
+//    pressure update p += etan*dRc*theta_c (SolveStokes2dOpt.jl:9-76), coloured
+//    by (i+j) parity and phased per colour as vx, then vy, then p so every
+//    launch is race-free and deterministic;
+//  - residuals (Residuals.jl:101-591, 2-D variants), ghost-node boundary
+//    conditions (BoundaryConditions.jl:22-53), pressure normalisation
+//    (PressureUpdate.jl:24-37);
+//  - restriction (Restriction.jl:377-505), viscosity restriction
+//    (ViscosityRestriction.jl:68-298) and prolongation (Prolongation.jl:150-185)
+//    in gather form: every coarse node sums its fine contributors in the order
+//    the reference's serial scatter would add them, so no atomics are needed and
+//    FP64 results equal the serial loops.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <vector>
+
+namespace ebg {
+
+template <typename T>
+struct MGLevelDev {
+    int xnum, ynum;
+    const T *xstp_b, *ystp_b, *xstp_vy, *ystp_vx;
+    T *etas, *etan;
+    T *vx, *vy, *pr;
+    T *RX, *RY, *RC;
+    T *rx, *ry, *rc;
+};
+
+// Fine->coarse transfer description for one axis of one grid type: for each
+// fine node its left coarse node index (1-based) and normalised distance, and
+// for each coarse node the contiguous fine range [lo, hi] that contributes.
+template <typename T>
+struct AxisMapDev {
+    const int* idx;   // [n_fine]
+    const T* d;       // [n_fine]
+    const int* lo;    // [n_coarse]
+    const int* hi;    // [n_coarse]
+};
+
+template <typename T>
+struct TransferDev {
+    AxisMapDev<T> vx_y, vx_x, vy_y, vy_x, pr_y, pr_x;  // residual restriction ranges
+    AxisMapDev<T> b_y, b_x, e_y, e_x;                  // viscosity restriction ranges
+};
+
+#define MG_VX(l, i, j) (l).vx[(size_t)((j) - 1) * ((l).ynum + 1) + ((i) - 1)]
+#define MG_VY(l, i, j) (l).vy[(size_t)((j) - 1) * (l).ynum + ((i) - 1)]
+#define MG_PR(l, i, j) (l).pr[(size_t)((j) - 1) * ((l).ynum - 1) + ((i) - 1)]
+#define MG_ES(l, i, j) (l).etas[(size_t)((j) - 1) * (l).ynum + ((i) - 1)]
+#define MG_EN(l, i, j) (l).etan[(size_t)((j) - 1) * ((l).ynum - 1) + ((i) - 1)]
+
+template <typename T>
+__device__ __forceinline__ T mg_vx_residual(const MGLevelDev<T>& l, int i, int j, T* coef)
+{
+    const T dxL = l.xstp_b[j - 2], dxR = l.xstp_b[j - 1], dxC = l.xstp_vy[j - 1];
+    const T dyU = l.ystp_vx[i - 2], dyD = l.ystp_vx[i - 1], dyC = l.ystp_b[i - 2];
+    const T nR = MG_EN(l, i - 1, j), nL = MG_EN(l, i - 1, j - 1);
+    const T sU = MG_ES(l, i - 1, j), sD = MG_ES(l, i, j);
+    const T pL = MG_PR(l, i - 1, j - 1), pR = MG_PR(l, i - 1, j);
+    const T vxL = MG_VX(l, i, j - 1), vxC = MG_VX(l, i, j), vxR = MG_VX(l, i, j + 1);
+    const T vxU = MG_VX(l, i - 1, j), vxD = MG_VX(l, i + 1, j);
+    const T vyUL = MG_VY(l, i - 1, j), vyUR = MG_VY(l, i - 1, j + 1);
+    const T vyLL = MG_VY(l, i, j), vyLR = MG_VY(l, i, j + 1);
+    const T dp_dx = (pR - pL) / dxC;
+    const T dsxx = T(2) * (nR * (vxR - vxC) / dxR - nL * (vxC - vxL) / dxL) / dxC;
+    const T dsxy = (sD * ((vxD - vxC) / dyD + (vyLR - vyLL) / dxC) -
+                    sU * ((vxC - vxU) / dyU + (vyUR - vyUL) / dxC)) / dyC;
+    const T dR = l.RX[(size_t)(j - 1) * (l.ynum + 1) + (i - 1)] + dp_dx - dsxx - dsxy;
+    if (coef) *coef = T(-2) * (nR / dxR / dxC + nL / dxL / dxC) - (sD / dyD / dyC + sU / dyU / dyC);
+    return dR;
+}
+
+template <typename T>
+__device__ __forceinline__ T mg_vy_residual(const MGLevelDev<T>& l, int i, int j, T* coef)
+{
+    const T dxL = l.xstp_b[j - 2], dxR = l.xstp_vy[j - 1], dxC = l.xstp_vy[j - 2];
+    const T dyU = l.ystp_b[i - 2], dyD = l.ystp_b[i - 1], dyC = l.ystp_vx[i - 1];
+    const T pD = MG_PR(l, i, j - 1), pU = MG_PR(l, i - 1, j - 1);
+    const T nD = MG_EN(l, i, j - 1), nU = MG_EN(l, i - 1, j - 1);
+    const T sR = MG_ES(l, i, j), sL = MG_ES(l, i, j - 1);
+    const T vyU = MG_VY(l, i - 1, j), vyC = MG_VY(l, i, j), vyD = MG_VY(l, i + 1, j);
+    const T vyR = MG_VY(l, i, j + 1), vyL = MG_VY(l, i, j - 1);
+    const T vxLR = MG_VX(l, i + 1, j), vxUR = MG_VX(l, i, j);
+    const T vxLL = MG_VX(l, i + 1, j - 1), vxUL = MG_VX(l, i, j - 1);
+    const T dp_dy = (pD - pU) / dyC;
+    const T dsyy = T(2) * (nD * (vyD - vyC) / dyD - nU * (vyC - vyU) / dyU) / dyC;
+    const T dsyx = (sR * ((vyR - vyC) / dxR + (vxLR - vxUR) / dyC) -
+                    sL * ((vyC - vyL) / dxL + (vxLL - vxUL) / dyC)) / dxC;
+    const T dR = l.RY[(size_t)(j - 1) * l.ynum + (i - 1)] + dp_dy - dsyy - dsyx;
+    if (coef) *coef = T(-2) * (nD / dyD / dyC + nU / dyU / dyC) - (sR / dxR / dxC + sL / dxL / dxC);
+    return dR;
+}
+
+template <typename T>
+__device__ __forceinline__ T mg_pr_residual(const MGLevelDev<T>& l, int i, int j)
+{
+    const T dxC = l.xstp_b[j - 1], dyC = l.ystp_b[i - 1];
+    return l.RC[(size_t)(j - 1) * (l.ynum - 1) + (i - 1)] -
+           ((MG_VX(l, i + 1, j + 1) - MG_VX(l, i + 1, j)) / dxC +
+            (MG_VY(l, i + 1, j + 1) - MG_VY(l, i, j + 1)) / dyC);
+}
+
+// ---------------------------------------------------------------- kernels
+template <typename T>
+__global__ void mg_bc_kernel(MGLevelDev<T> l, T d_top, T d_bottom, T d_left, T d_right)
+{
+    // One thread per boundary node.  The serial reference pass (columns outer,
+    // rows inner) lets the top corner ghosts read the pre-pass value of the
+    // node below them and the later corners the already-zeroed neighbour; the
+    // threads that own those corners also own the neighbour they read, so the
+    // result equals the serial pass and no two threads touch the same node.
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int xnum = l.xnum, ynum = l.ynum;
+    if (t < xnum) {  // vx top/bottom ghosts of column j
+        const int j = t + 1;
+        if (j == 1 || j == xnum) {
+            const T old = MG_VX(l, 2, j);
+            MG_VX(l, 1, j) = d_top * old;
+            MG_VX(l, 2, j) = T(0);
+            MG_VX(l, ynum + 1, j) = d_bottom * T(0);
+        } else {
+            MG_VX(l, 1, j) = d_top * MG_VX(l, 2, j);
+            MG_VX(l, ynum + 1, j) = d_bottom * MG_VX(l, ynum, j);
+        }
+        return;
+    }
+    int u = t - xnum;
+    if (u < ynum - 2) {  // vx left/right boundary columns, rows 3..ynum
+        const int i = u + 3;
+        MG_VX(l, i, 1) = T(0);
+        MG_VX(l, i, xnum) = T(0);
+        return;
+    }
+    u -= (ynum - 2);
+    if (u < ynum) {  // vy left/right ghosts of row i
+        const int i = u + 1;
+        if (i == 1 || i == ynum) {
+            const T old = MG_VY(l, i, 2);
+            MG_VY(l, i, 1) = d_left * old;
+            MG_VY(l, i, 2) = T(0);
+            MG_VY(l, i, xnum + 1) = d_right * T(0);
+        } else {
+            MG_VY(l, i, 1) = d_left * MG_VY(l, i, 2);
+            MG_VY(l, i, xnum + 1) = d_right * MG_VY(l, i, xnum);
+        }
+        return;
+    }
+    u -= ynum;
+    if (u < xnum - 2) {  // vy top/bottom boundary rows, columns 3..xnum
+        const int j = u + 3;
+        MG_VY(l, 1, j) = T(0);
+        MG_VY(l, ynum, j) = T(0);
+    }
+}
+inline int mg_bc_threads(int xnum, int ynum) { return xnum + (ynum - 2) + ynum + (xnum - 2); }
+
+// PHASE 0: vx, 1: vy, 2: p; interior nodes of colour (i+j)&1 == color
+template <typename T, int PHASE>
+__global__ void __launch_bounds__(256) mg_sweep_kernel(MGLevelDev<T> l, int color, T theta_s, T theta_c)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (PHASE == 0) {  // i in 2..ynum, j in 2..xnum-1
+        const int ni = l.ynum - 1;
+        if (t >= ni * (l.xnum - 2)) return;
+        const int i = 2 + t % ni, j = 2 + t / ni;
+        if (((i + j) & 1) != color) return;
+        T coef;
+        const T dR = mg_vx_residual(l, i, j, &coef);
+        MG_VX(l, i, j) += dR / coef * theta_s;
+    } else if (PHASE == 1) {  // i in 2..ynum-1, j in 2..xnum
+        const int ni = l.ynum - 2;
+        if (t >= ni * (l.xnum - 1)) return;
+        const int i = 2 + t % ni, j = 2 + t / ni;
+        if (((i + j) & 1) != color) return;
+        T coef;
+        const T dR = mg_vy_residual(l, i, j, &coef);
+        MG_VY(l, i, j) += dR / coef * theta_s;
+    } else {
+        const int ni = l.ynum - 1;
+        if (t >= ni * (l.xnum - 1)) return;
+        const int i = 1 + t % ni, j = 1 + t / ni;
+        if (((i + j) & 1) != color) return;
+        const T dR = mg_pr_residual(l, i, j);
+        MG_PR(l, i, j) += MG_EN(l, i, j) * dR * theta_c;
+    }
+}
+
+// dp = prnorm - pr[1,1] computed once into *scratch, then added everywhere
+template <typename T>
+__global__ void mg_pressure_shift_kernel(const T* pr, const T* prnorm_dev, T prnorm, T* scratch)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) scratch[0] = (prnorm_dev ? prnorm_dev[0] : prnorm) - pr[0];
+}
+template <typename T>
+__global__ void mg_pressure_add_kernel(T* pr, int n, const T* scratch)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) pr[t] += scratch[0];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) mg_residual_kernel(MGLevelDev<T> l)
+{
+    // covers the vx grid (ynum+1) x xnum; vy and p handled by the same thread
+    // where their index ranges overlap
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int xnum = l.xnum, ynum = l.ynum;
+    const int ni = ynum + 1;
+    if (t >= ni * (xnum + 1)) return;
+    const int i = 1 + t % ni, j = 1 + t / ni;
+    if (j <= xnum) {
+        T r = T(0);
+        if (i >= 2 && i <= ynum && j >= 2 && j <= xnum - 1) r = mg_vx_residual(l, i, j, (T*)nullptr);
+        l.rx[(size_t)(j - 1) * (ynum + 1) + (i - 1)] = r;
+    }
+    if (i <= ynum) {
+        T r = T(0);
+        if (i >= 2 && i <= ynum - 1 && j >= 2 && j <= xnum) r = mg_vy_residual(l, i, j, (T*)nullptr);
+        l.ry[(size_t)(j - 1) * ynum + (i - 1)] = r;
+    }
+    if (i <= ynum - 1 && j <= xnum - 1)
+        l.rc[(size_t)(j - 1) * (ynum - 1) + (i - 1)] = mg_pr_residual(l, i, j);
+}
+
+// Gather restriction of one grid type.  val(i,j) is read from fine array f
+// (scaled by etan for the continuity residual).  Loop order j-outer, i-inner
+// matches Restriction.jl:444-463.
+template <typename T, int KIND>  // 0 vx, 1 vy, 2 pr
+__global__ void __launch_bounds__(256) mg_restrict_kernel(MGLevelDev<T> f, MGLevelDev<T> c,
+                                                           AxisMapDev<T> my, AxisMapDev<T> mx)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int yc = c.ynum, xc = c.xnum;
+    int nyc, nxc;
+    if (KIND == 0) { nyc = yc + 1; nxc = xc; }
+    else if (KIND == 1) { nyc = yc; nxc = xc + 1; }
+    else { nyc = yc - 1; nxc = xc - 1; }
+    if (t >= nyc * nxc) return;
+    const int ic = 1 + t % nyc, jc = 1 + t / nyc;
+    bool interior;
+    if (KIND == 0) interior = (ic > 1 && ic < yc + 1 && jc > 1 && jc < xc);
+    else if (KIND == 1) interior = (ic > 1 && ic < yc && jc > 1 && jc < xc + 1);
+    else interior = true;
+    T num = T(0), den = T(0);
+    if (interior) {
+        const int ilo = my.lo[ic - 1], ihi = my.hi[ic - 1];
+        const int jlo = mx.lo[jc - 1], jhi = mx.hi[jc - 1];
+        for (int j = jlo; j <= jhi; ++j) {
+            const int jx = mx.idx[j - 1];
+            const T dx = mx.d[j - 1];
+            const T wx = (jx == jc) ? (T(1) - dx) : dx;
+            for (int i = ilo; i <= ihi; ++i) {
+                const int iy = my.idx[i - 1];
+                const T dy = my.d[i - 1];
+                const T wy = (iy == ic) ? (T(1) - dy) : dy;
+                const T w = wx * wy;
+                T val;
+                if (KIND == 0) val = f.rx[(size_t)(j - 1) * (f.ynum + 1) + (i - 1)];
+                else if (KIND == 1) val = f.ry[(size_t)(j - 1) * f.ynum + (i - 1)];
+                else val = MG_EN(f, i, j) * f.rc[(size_t)(j - 1) * (f.ynum - 1) + (i - 1)];
+                num += w * val;
+                den += w;
+            }
+        }
+    }
+    if (KIND == 0) c.RX[(size_t)(jc - 1) * (yc + 1) + (ic - 1)] = (interior && den != T(0)) ? num / den : T(0);
+    else if (KIND == 1) c.RY[(size_t)(jc - 1) * yc + (ic - 1)] = (interior && den != T(0)) ? num / den : T(0);
+    else c.RC[(size_t)(jc - 1) * (yc - 1) + (ic - 1)] = (den != T(0)) ? num / den / MG_EN(c, ic, jc) : T(0);
+}
+
+// Gather viscosity restriction; loop order i-outer, j-inner as
+// ViscosityRestriction.jl:202-215.  KIND 0: etas (basic), 1: etan (pressure).
+template <typename T, int KIND>
+__global__ void mg_visc_restrict_kernel(MGLevelDev<T> f, MGLevelDev<T> c, AxisMapDev<T> my,
+                                        AxisMapDev<T> mx)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nyc = KIND == 0 ? c.ynum : c.ynum - 1;
+    const int nxc = KIND == 0 ? c.xnum : c.xnum - 1;
+    if (t >= nyc * nxc) return;
+    const int ic = 1 + t % nyc, jc = 1 + t / nyc;
+    const int ilo = my.lo[ic - 1], ihi = my.hi[ic - 1];
+    const int jlo = mx.lo[jc - 1], jhi = mx.hi[jc - 1];
+    T num = T(0), den = T(0);
+    for (int i = ilo; i <= ihi; ++i) {
+        const int iy = my.idx[i - 1];
+        const T dy = my.d[i - 1];
+        const T wy = (iy == ic) ? (T(1) - dy) : dy;
+        for (int j = jlo; j <= jhi; ++j) {
+            const int jx = mx.idx[j - 1];
+            const T dx = mx.d[j - 1];
+            const T wx = (jx == jc) ? (T(1) - dx) : dx;
+            const T w = wx * wy;
+            const T val = KIND == 0 ? MG_ES(f, i, j) : MG_EN(f, i, j);
+            num += w * val;
+            den += w;
+        }
+    }
+    const T out = (den != T(0)) ? num / den : T(0);
+    if (KIND == 0) MG_ES(c, ic, jc) = out;
+    else MG_EN(c, ic, jc) = out;
+}
+
+template <typename T>
+__device__ __forceinline__ T mg_c2f(const T* a, int ld, int iy, int jx, T dy, T dx)
+{
+    const T omx = T(1) - dx, omy = T(1) - dy;
+    const T w00 = omx * omy, w01 = omx * dy, w10 = dx * omy, w11 = dx * dy;
+    const T* p = a + (size_t)(jx - 1) * ld + (iy - 1);
+    T v = T(0);
+    v += w00 * p[0];
+    v += w01 * p[1];
+    v += w10 * p[ld];
+    v += w11 * p[ld + 1];
+    return v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) mg_prolong_kernel(MGLevelDev<T> c, MGLevelDev<T> f,
+                                                          TransferDev<T> m, T relax_v, T relax_p)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int xf = f.xnum, yf = f.ynum;
+    const int ni = yf + 1;
+    if (t >= ni * (xf + 1)) return;
+    const int i = 1 + t % ni, j = 1 + t / ni;
+    if (j < xf + 1) {
+        const T dv = mg_c2f(c.vx, c.ynum + 1, m.vx_y.idx[i - 1], m.vx_x.idx[j - 1],
+                            m.vx_y.d[i - 1], m.vx_x.d[j - 1]);
+        MG_VX(f, i, j) += dv * relax_v;
+    }
+    if (i < yf + 1) {
+        const T dv = mg_c2f(c.vy, c.ynum, m.vy_y.idx[i - 1], m.vy_x.idx[j - 1],
+                            m.vy_y.d[i - 1], m.vy_x.d[j - 1]);
+        MG_VY(f, i, j) += dv * relax_v;
+    }
+    if (i < yf && j < xf) {
+        const T dp = mg_c2f(c.pr, c.ynum - 1, m.pr_y.idx[i - 1], m.pr_x.idx[j - 1],
+                            m.pr_y.d[i - 1], m.pr_x.d[j - 1]);
+        MG_PR(f, i, j) += dp * relax_p;
+    }
+}
+
+template <typename T>
+__global__ void fill_kernel(T* a, int64_t n, T v)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] = v;
+}
+
+// ------------------------------------------------------- host-side hierarchy
+struct HostAxis {
+    std::vector<double> g;  // node coordinates of one axis of one grid type
+};
+
+// Host description of one uniform level (cumulative-sum coordinates as the
+// reference's Initialization.jl:27-43 builds them).
+struct HostLevelGrid {
+    int xnum, ynum;
+    std::vector<double> gridx_b, gridy_b, xstp_b, ystp_b, gridx_vy, xstp_vy, gridy_vx,
+        ystp_vx, gridx_pr, gridy_pr;
+};
+
+HostLevelGrid make_uniform_level_grid(int xnum, int ynum, double xsize, double ysize);
+std::vector<std::pair<int, int>> mg_level_sizes(int xnum, int ynum, int max_levels);
+
+// One device-resident hierarchy of precision T.
+template <typename T>
+struct MGHierarchy {
+    std::vector<HostLevelGrid> hgrid;
+    std::vector<MGLevelDev<T>> lev;
+    std::vector<TransferDev<T>> xfer;   // xfer[k]: level k -> k+1 maps
+    std::vector<void*> allocations;
+    T* scratch = nullptr;               // pressure-shift scalar
+    int nlev() const { return (int)lev.size(); }
+    ~MGHierarchy();
+};
+
+template <typename T>
+int mg_build(MGHierarchy<T>& H, int xnum, int ynum, double xsize, double ysize, int nlevels,
+             const std::vector<HostLevelGrid>* finest_override);
+template <typename T>
+void mg_restrict_viscosity(MGHierarchy<T>& H, hipStream_t s);
+template <typename T>
+void mg_smoother(MGHierarchy<T>& H, int k, int nsweeps, T theta_s, T theta_c, T prnorm,
+                 const T* prnorm_dev, const T* bc_d, hipStream_t s);
+template <typename T>
+void mg_vcycle(MGHierarchy<T>& H, const std::vector<int>& smoothing, T theta_s, T theta_c,
+               T relax_v, T relax_p, T prnorm, const T* prnorm_dev, const T* bc_d, hipStream_t s);
+template <typename T>
+void mg_zero_solution(MGHierarchy<T>& H, int k, hipStream_t s);
+
+std::vector<int> mg_smoothing_counts(int nlev, int finest, int multiplier, int cap);
+
+}  // namespace ebg

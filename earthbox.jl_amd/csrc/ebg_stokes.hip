@@ -1,3 +1,753 @@
-// placeholder, filled in by the stokes implementation
+// Stokes-continuity solve on the assembled-operator conventions.
+//
+// Replaces SystemSolver.solve_system_optimized / ParallelSolver.parallel_direct_solver
+// (solver/system_solver/SystemSolver.jl:67-91, parallel_solver/ParallelSolver.jl:52-126):
+// FGMRES(m) in FP64 on the matrix-free assembled operator (ebg_stokes_op.hpp),
+// right-preconditioned by one V-cycle of the reference's multigrid
+// (ebg_mg.hpp) run in FP32 (mixed precision) or FP64 (plain).  Also the
+// companions K10 (unpack + velocity BCs, SystemSolver.jl:141-271) and K11
+// (|L*S - R|, StokesResiduals.jl:63-82).
 #include "ebg_common.hpp"
-extern "C" int ebg_stokes_release(ebg_ctx* ctx) { (void)ctx; return EBG_OK; }
+#include "ebg_mg.hpp"
+#include "ebg_stokes_op.hpp"
+#include "ebg_learned.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+namespace ebg {
+
+static inline unsigned nb(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// ------------------------------------------------------------------ kernels
+template <bool WANT_R, class Lay>
+__global__ void __launch_bounds__(256) stokes_rows_kernel(StokesOpDev s, const double* __restrict__ x,
+                                                          double* __restrict__ y, Lay L)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ny1 = s.ynum - 1;
+    if (c >= L.nc) return;
+    const int i = c % ny1 + 1, j = c / ny1 + 1;
+    y[L.vx(c)] = x_row<WANT_R>(s, i, j, x, L);
+    y[L.vy(c)] = y_row<WANT_R>(s, i, j, x, L);
+    y[L.p(c)] = p_row<WANT_R>(s, i, j, x, L);
+}
+
+// y = b - L*x (planar)
+__global__ void __launch_bounds__(256) residual_planar_kernel(StokesOpDev s, const double* __restrict__ x,
+                                                              const double* __restrict__ b,
+                                                              double* __restrict__ r, LayoutPlanar L)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ny1 = s.ynum - 1;
+    if (c >= L.nc) return;
+    const int i = c % ny1 + 1, j = c / ny1 + 1;
+    r[L.vx(c)] = b[L.vx(c)] - x_row<false>(s, i, j, x, L);
+    r[L.vy(c)] = b[L.vy(c)] - y_row<false>(s, i, j, x, L);
+    r[L.p(c)] = b[L.p(c)] - p_row<false>(s, i, j, x, L);
+}
+
+__global__ void interleaved_to_planar_kernel(const double* __restrict__ a, double* __restrict__ p, int nc)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    p[c] = a[3 * (int64_t)c];
+    p[nc + c] = a[3 * (int64_t)c + 1];
+    p[2 * (int64_t)nc + c] = a[3 * (int64_t)c + 2];
+}
+__global__ void planar_to_interleaved_kernel(const double* __restrict__ p, double* __restrict__ a, int nc)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    a[3 * (int64_t)c] = p[c];
+    a[3 * (int64_t)c + 1] = p[nc + c];
+    a[3 * (int64_t)c + 2] = p[2 * (int64_t)nc + c];
+}
+
+// K11: |L*S - R| in the reference's interleaved order
+__global__ void abs_diff_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                double* __restrict__ out, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = fabs(a[t] - b[t]);
+}
+
+// K10: reload_solu_vel! (SystemSolver.jl:188-221), interleaved S
+__global__ void unpack_kernel(StokesOpDev s, const double* __restrict__ S, double* __restrict__ vx1,
+                              double* __restrict__ vy1, double* __restrict__ pr1)
+{
+    // one thread per vx-grid node (covers vy and pr nodes too)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int xnum = s.xnum, ynum = s.ynum;
+    const int ni = ynum + 1;
+    if (t >= ni * (xnum + 1)) return;
+    const int I = t % ni + 1, J = t / ni + 1;  // node indices on the vx/vy grids
+    if (J <= xnum) {  // vx1[I, J]: cell (I-1, J-1) for J-1 < xnum-1
+        double v = 0.0;
+        const int i = I - 1, j = J - 1;
+        if (i >= 1 && i <= ynum - 1 && j >= 1 && j < xnum - 1) v = S[3 * (int64_t)((j - 1) * (ynum - 1) + (i - 1))];
+        vx1[(size_t)(J - 1) * (ynum + 1) + (I - 1)] = v;
+    }
+    if (I <= ynum) {  // vy1[I, J]
+        double v = 0.0;
+        const int i = I - 1, j = J - 1;
+        if (i >= 1 && i < ynum - 1 && j >= 1 && j <= xnum - 1) v = S[3 * (int64_t)((j - 1) * (ynum - 1) + (i - 1)) + 1];
+        vy1[(size_t)(J - 1) * ynum + (I - 1)] = v;
+    }
+    if (I <= ynum - 1 && J <= xnum - 1)
+        pr1[(size_t)(J - 1) * (ynum - 1) + (I - 1)] = S[3 * (int64_t)((J - 1) * (ynum - 1) + (I - 1)) + 2] * s.pscale;
+}
+
+// apply_vel_bc! (SystemSolver.jl:237-271): top/bottom rows first, then
+// left/right columns (the column pass overwrites the corners, as the reference).
+__global__ void vel_bc_rows_kernel(StokesOpDev s, double* vx1, double* vy1)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int xnum = s.xnum, ynum = s.ynum;
+    if (j <= xnum) {
+        double* col = vx1 + (size_t)(j - 1) * (ynum + 1);
+        col[0] = EBG_BX(btopx, j, 1) + EBG_BX(btopx, j, 2) * col[1];
+        col[ynum] = EBG_BX(bbottomx, j, 1) + EBG_BX(bbottomx, j, 2) * col[ynum - 1];
+    }
+    if (j <= xnum + 1) {
+        double* col = vy1 + (size_t)(j - 1) * ynum;
+        col[0] = EBG_BX(btopy, j, 1) + EBG_BX(btopy, j, 2) * col[1];
+        col[ynum - 1] = EBG_BX(bbottomy, j, 1) + EBG_BX(bbottomy, j, 2) * col[ynum - 2];
+    }
+}
+__global__ void vel_bc_cols_kernel(StokesOpDev s, double* vx1, double* vy1)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+    const int xnum = s.xnum, ynum = s.ynum;
+    if (i <= ynum + 1) {
+        double* row = vx1 + (i - 1);
+        const size_t ld = ynum + 1;
+        row[0] = EBG_BY(bleftx, i, 1) + EBG_BY(bleftx, i, 2) * row[ld];
+        row[(size_t)(xnum - 1) * ld] = EBG_BY(brightx, i, 1) + EBG_BY(brightx, i, 2) * row[(size_t)(xnum - 2) * ld];
+    }
+    if (i <= ynum) {
+        double* row = vy1 + (i - 1);
+        const size_t ld = ynum;
+        row[0] = EBG_BY(blefty, i, 1) + EBG_BY(blefty, i, 2) * row[ld];
+        row[(size_t)xnum * ld] = EBG_BY(brighty, i, 1) + EBG_BY(brighty, i, 2) * row[(size_t)(xnum - 1) * ld];
+    }
+}
+
+// ---- deterministic reductions ----
+constexpr int RED_BLOCKS = 512;
+constexpr int MAXK = 64;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// partial[b*(k)+q] = sum over this block's grid-stride slice of w * V[q]
+__global__ void __launch_bounds__(256) multidot_kernel(const double* __restrict__ w,
+                                                      const double* __restrict__ V, int64_t ldv,
+                                                      int k, int64_t n, double* __restrict__ partial)
+{
+    __shared__ double sh[4][MAXK];
+    double acc[MAXK];
+    for (int q = 0; q < k; ++q) acc[q] = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const double wv = w[t];
+        for (int q = 0; q < k; ++q) acc[q] += wv * V[(int64_t)q * ldv + t];
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int q = 0; q < k; ++q) {
+        const double s = wave_sum(acc[q]);
+        if (lane == 0) sh[wid][q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < k) {
+        const int q = threadIdx.x;
+        partial[(int64_t)blockIdx.x * k + q] = ((sh[0][q] + sh[1][q]) + sh[2][q]) + sh[3][q];
+    }
+}
+
+// out[q] = sum_b partial[b*k+q] (one block, fixed order)
+__global__ void reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int k,
+                                       double* __restrict__ out, int accumulate)
+{
+    __shared__ double sh[256];
+    for (int q = 0; q < k; ++q) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partial[(int64_t)b * k + q];
+        sh[threadIdx.x] = s;
+        __syncthreads();
+        for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+            if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[q] = accumulate ? out[q] + sh[0] : sh[0];
+        __syncthreads();
+    }
+}
+
+// w -= sum_q h[q] V[q]
+__global__ void __launch_bounds__(256) update_kernel(double* __restrict__ w, const double* __restrict__ V,
+                                                    int64_t ldv, const double* __restrict__ h, int k,
+                                                    int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double v = w[t];
+    for (int q = 0; q < k; ++q) v -= h[q] * V[(int64_t)q * ldv + t];
+    w[t] = v;
+}
+
+// x += sum_q y[q] Z[q]
+__global__ void __launch_bounds__(256) combine_kernel(double* __restrict__ x, const double* __restrict__ Z,
+                                                     int64_t ldz, const double* __restrict__ y, int k,
+                                                     int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double v = x[t];
+    for (int q = 0; q < k; ++q) v += y[q] * Z[(int64_t)q * ldz + t];
+    x[t] = v;
+}
+
+__global__ void scale_kernel(double* __restrict__ out, const double* __restrict__ in, const double* nrm,
+                             int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] = in[t] / nrm[0];
+}
+
+// ---- preconditioner packing ----
+template <typename T>
+__global__ void pack_rhs_kernel(StokesOpDev s, const double* __restrict__ r, MGLevelDev<T> l,
+                                T* prnorm_dev, int nc)
+{
+    // covers the vx grid; RX/RY/RC of level 0 (MG scaling: RC = r_p / pscale)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int xnum = s.xnum, ynum = s.ynum, ni = ynum + 1;
+    if (t >= ni * (xnum + 1)) return;
+    const int I = t % ni + 1, J = t / ni + 1;
+    const int i = I - 1, j = J - 1;
+    const bool cellv = (i >= 1 && i <= ynum - 1 && j >= 1 && j <= xnum - 1);
+    const int c = (j - 1) * (ynum - 1) + (i - 1);
+    if (J <= xnum) l.RX[(size_t)(J - 1) * (ynum + 1) + (I - 1)] = cellv ? (T)r[c] : T(0);
+    if (I <= ynum) l.RY[(size_t)(J - 1) * ynum + (I - 1)] = cellv ? (T)r[nc + c] : T(0);
+    if (I <= ynum - 1 && J <= xnum - 1) {
+        const int cp = (J - 1) * (ynum - 1) + (I - 1);
+        l.RC[cp] = (T)(r[2 * (int64_t)nc + cp] / s.pscale);
+        if (cp == 0)
+            prnorm_dev[0] = (s.pressure_bc_mode == 0) ? (T)(s.pscale * (r[2 * (int64_t)nc] / kbond(s))) : T(0);
+    }
+}
+
+template <typename T>
+__global__ void unpack_corr_kernel(StokesOpDev s, const double* __restrict__ r, MGLevelDev<T> l,
+                                   double* __restrict__ z, int nc)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int ny1 = s.ynum - 1;
+    const int i = c % ny1 + 1, j = c / ny1 + 1;
+    const double kb = kbond(s);
+    // vx unknown of cell (i,j) is vx1[i+1, j+1]
+    if (j < s.xnum - 1 && out_zone_x(s, i, j))
+        z[c] = (double)l.vx[(size_t)j * (s.ynum + 1) + i];
+    else
+        z[c] = r[c] / kb;
+    if (i < s.ynum - 1 && out_zone_y(s, i, j))
+        z[nc + c] = (double)l.vy[(size_t)j * s.ynum + i];
+    else
+        z[nc + c] = r[nc + c] / kb;
+    if (internal_continuity(s, i, j))
+        z[2 * (int64_t)nc + c] = (double)l.pr[c] / s.pscale;
+    else
+        z[2 * (int64_t)nc + c] = r[2 * (int64_t)nc + c] / kb;
+}
+
+template <typename T>
+__global__ void to_precision_kernel(const double* __restrict__ a, T* __restrict__ b, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = (T)a[t];
+}
+
+// ------------------------------------------------------------------ state
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int ensure(size_t b)
+    {
+        if (bytes >= b && p) return EBG_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        EBG_HIP(hipMalloc(&p, b ? b : 8));
+        bytes = b;
+        return EBG_OK;
+    }
+    template <typename T> T* as() { return (T*)p; }
+};
+
+struct StokesState {
+    int xnum = 0, ynum = 0, nc = 0;
+    StokesOpDev op{};
+    DevBuf etas, etan, rho, RX, RY, RC, bcs;
+    // solver workspace
+    DevBuf b, x, r, w, V, Z, partial, hdev, ydev, nrm, stage;
+    int m_alloc = 0;
+    std::unique_ptr<MGHierarchy<float>> Hf;
+    std::unique_ptr<MGHierarchy<double>> Hd;
+    DevBuf prnorm_f, prnorm_d;
+    int mg_key_nlev = -1;
+    bool visc_dirty = true;
+    LearnedCorrection* learned = nullptr;
+};
+
+static StokesState* state(ebg_ctx* ctx) { return ctx ? (StokesState*)ctx->stokes : nullptr; }
+
+static int copy_in(DevBuf& d, const double* src, size_t n, hipStream_t s)
+{
+    int rc = d.ensure(n * sizeof(double));
+    if (rc) return rc;
+    EBG_HIP(hipMemcpyAsync(d.p, src, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    return EBG_OK;
+}
+
+// run one V-cycle on r (planar FP64) into z (planar FP64)
+template <typename T>
+static void precondition(StokesState& st, MGHierarchy<T>& H, T* prn, const ebg_solve_opts& o,
+                         const std::vector<int>& sm, const double* r, double* z, hipStream_t s)
+{
+    MGLevelDev<T>& l0 = H.lev[0];
+    const int nvx = (st.ynum + 1) * (st.xnum + 1);
+    hipLaunchKernelGGL(pack_rhs_kernel<T>, dim3(nb(nvx)), dim3(256), 0, s, st.op, r, l0, prn, st.nc);
+    mg_zero_solution(H, 0, s);
+    const T bcd[4] = {(T)o.bc_d[0], (T)o.bc_d[1], (T)o.bc_d[2], (T)o.bc_d[3]};
+    mg_vcycle(H, sm, (T)o.theta_s, (T)o.theta_c, T(1), T(1), T(0), (const T*)prn, bcd, s);
+    hipLaunchKernelGGL(unpack_corr_kernel<T>, dim3(nb(st.nc)), dim3(256), 0, s, st.op, r, l0, z, st.nc);
+}
+
+template <typename T>
+static int ensure_hierarchy(ebg_ctx* ctx, StokesState& st, std::unique_ptr<MGHierarchy<T>>& H,
+                            DevBuf& prn, const ebg_solve_opts& o)
+{
+    if (!H || st.mg_key_nlev != o.nlevels) {
+        H.reset(new MGHierarchy<T>());
+        std::vector<HostLevelGrid> fine{HostLevelGrid{}};
+        const Grid2D& g = ctx->grid;
+        HostLevelGrid& h = fine[0];
+        h.xnum = (int)g.xnum;
+        h.ynum = (int)g.ynum;
+        h.gridx_b = g.h_gridx_b;
+        h.gridy_b = g.h_gridy_b;
+        h.xstp_b = g.h_xstp_b;
+        h.ystp_b = g.h_ystp_b;
+        h.gridx_vy = g.h_gridx_vy;
+        h.xstp_vy = g.h_xstp_vy;
+        h.gridy_vx = g.h_gridy_vx;
+        h.ystp_vx = g.h_ystp_vx;
+        h.gridx_pr.resize(g.xnum - 1);
+        h.gridy_pr.resize(g.ynum - 1);
+        // pressure nodes (PressureGridTools.jl:34-86)
+        h.gridx_pr[0] = (g.h_gridx_b[0] + g.h_gridx_b[1]) / 2.0;
+        for (int64_t k = 1; k < g.xnum - 1; ++k)
+            h.gridx_pr[k] = h.gridx_pr[k - 1] + (g.h_gridx_b[k + 1] - g.h_gridx_b[k - 1]) / 2.0;
+        h.gridy_pr[0] = (g.h_gridy_b[0] + g.h_gridy_b[1]) / 2.0;
+        for (int64_t k = 1; k < g.ynum - 1; ++k)
+            h.gridy_pr[k] = h.gridy_pr[k - 1] + (g.h_gridy_b[k + 1] - g.h_gridy_b[k - 1]) / 2.0;
+        const double xsize = g.h_gridx_b.back() - g.h_gridx_b.front();
+        const double ysize = g.h_gridy_b.back() - g.h_gridy_b.front();
+        int rc = mg_build(*H, (int)g.xnum, (int)g.ynum, xsize, ysize, o.nlevels, &fine);
+        if (rc) return rc;
+        if ((rc = prn.ensure(sizeof(T) * 4))) return rc;
+        st.mg_key_nlev = o.nlevels;
+        st.visc_dirty = true;
+    }
+    return EBG_OK;
+}
+
+template <typename T>
+static void load_viscosity(StokesState& st, MGHierarchy<T>& H, hipStream_t s)
+{
+    MGLevelDev<T>& l0 = H.lev[0];
+    const int64_t ns = (int64_t)st.ynum * st.xnum, nn = (int64_t)(st.ynum - 1) * (st.xnum - 1);
+    hipLaunchKernelGGL(to_precision_kernel<T>, dim3(nb(ns)), dim3(256), 0, s, st.etas.as<double>(), l0.etas, ns);
+    hipLaunchKernelGGL(to_precision_kernel<T>, dim3(nb(nn)), dim3(256), 0, s, st.etan.as<double>(), l0.etan, nn);
+    mg_restrict_viscosity(H, s);
+}
+
+// deterministic dot products of w against V[0..k-1] -> hdev[0..k-1]
+static void multidot(StokesState& st, const double* w, const double* V, int64_t ldv, int k, int64_t n,
+                     double* out, int accumulate, hipStream_t s)
+{
+    hipLaunchKernelGGL(multidot_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, w, V, ldv, k, n,
+                       st.partial.as<double>());
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, st.partial.as<double>(),
+                       RED_BLOCKS, k, out, accumulate);
+}
+
+static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_solve_stats* stats)
+{
+    hipStream_t s = ctx->stream;
+    const int nc = st.nc;
+    const int64_t n = 3 * (int64_t)nc;
+    const int m = o.restart;
+    int rc;
+    if ((rc = st.b.ensure(n * 8)) || (rc = st.x.ensure(n * 8)) || (rc = st.r.ensure(n * 8)) ||
+        (rc = st.w.ensure(n * 8)) || (rc = st.V.ensure((size_t)(m + 1) * n * 8)) ||
+        (rc = st.Z.ensure((size_t)m * n * 8)) || (rc = st.partial.ensure((size_t)RED_BLOCKS * MAXK * 8)) ||
+        (rc = st.hdev.ensure(MAXK * 8 * 2)) || (rc = st.ydev.ensure(MAXK * 8)) || (rc = st.nrm.ensure(64)))
+        return rc;
+    std::vector<int> sm;
+    const bool f32 = (o.precision == 1);
+    if (f32) {
+        if ((rc = ensure_hierarchy(ctx, st, st.Hf, st.prnorm_f, o))) return rc;
+        sm = mg_smoothing_counts(st.Hf->nlev(), o.finest_sweeps, o.sweep_multiplier, o.sweep_cap);
+        if (st.visc_dirty) load_viscosity(st, *st.Hf, s);
+    } else {
+        if ((rc = ensure_hierarchy(ctx, st, st.Hd, st.prnorm_d, o))) return rc;
+        sm = mg_smoothing_counts(st.Hd->nlev(), o.finest_sweeps, o.sweep_multiplier, o.sweep_cap);
+        if (st.visc_dirty) load_viscosity(st, *st.Hd, s);
+    }
+    st.visc_dirty = false;
+    LayoutPlanar LP{nc};
+    double* b = st.b.as<double>();
+    double* x = st.x.as<double>();
+    double* r = st.r.as<double>();
+    double* w = st.w.as<double>();
+    double* V = st.V.as<double>();
+    double* Z = st.Z.as<double>();
+    double* hd = st.hdev.as<double>();
+    hipLaunchKernelGGL((stokes_rows_kernel<true, LayoutPlanar>), dim3(nb(nc)), dim3(256), 0, s, st.op,
+                       (const double*)nullptr, b, LP);
+    auto norm = [&](const double* v, double* out) {
+        multidot(st, v, v, 0, 1, n, out, 0, s);
+    };
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hh(MAXK + 1);
+    double bnorm2 = 0;
+    norm(b, st.nrm.as<double>());
+    EBG_HIP(hipMemcpyAsync(&bnorm2, st.nrm.p, 8, hipMemcpyDeviceToHost, s));
+    EBG_HIP(hipStreamSynchronize(s));
+    const double bnorm = std::sqrt(bnorm2);
+    if (!(bnorm > 0) || !std::isfinite(bnorm)) {
+        if (stats) { stats->iterations = 0; stats->relres = 0; stats->relres_initial = 0; }
+        hipLaunchKernelGGL(fill_kernel<double>, dim3(nb(n)), dim3(256), 0, s, x, n, 0.0);
+        return bnorm == 0 ? EBG_OK : EBG_ERR_NUMERIC;
+    }
+    int total = 0, restarts = 0;
+    double relres = 1.0, relres0 = -1.0;
+    while (true) {
+        hipLaunchKernelGGL(residual_planar_kernel, dim3(nb(nc)), dim3(256), 0, s, st.op, x, b, r, LP);
+        norm(r, st.nrm.as<double>());
+        double beta2;
+        EBG_HIP(hipMemcpyAsync(&beta2, st.nrm.p, 8, hipMemcpyDeviceToHost, s));
+        EBG_HIP(hipStreamSynchronize(s));
+        const double beta = std::sqrt(beta2);
+        relres = beta / bnorm;
+        if (relres0 < 0) relres0 = relres;
+        if (!std::isfinite(relres)) return EBG_ERR_NUMERIC;
+        if (relres <= o.rtol || total >= o.max_iterations) break;
+        EBG_HIP(hipMemcpyAsync(st.nrm.p, &beta, 8, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(scale_kernel, dim3(nb(n)), dim3(256), 0, s, V, r, st.nrm.as<double>(), n);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int k = 0;
+        bool done = false;
+        for (; k < m && total < o.max_iterations; ++k) {
+            double* vk = V + (int64_t)k * n;
+            double* zk = Z + (int64_t)k * n;
+            if (f32) precondition(st, *st.Hf, st.prnorm_f.as<float>(), o, sm, vk, zk, s);
+            else precondition(st, *st.Hd, st.prnorm_d.as<double>(), o, sm, vk, zk, s);
+            if (st.learned && o.learned) learned_apply(st.learned, ctx, vk, zk, s);
+            hipLaunchKernelGGL((stokes_rows_kernel<false, LayoutPlanar>), dim3(nb(nc)), dim3(256), 0, s,
+                               st.op, zk, w, LP);
+            // classical Gram-Schmidt, twice (CGS2)
+            multidot(st, w, V, n, k + 1, n, hd, 0, s);
+            hipLaunchKernelGGL(update_kernel, dim3(nb(n)), dim3(256), 0, s, w, V, n, hd, k + 1, n);
+            multidot(st, w, V, n, k + 1, n, hd + MAXK, 0, s);
+            hipLaunchKernelGGL(update_kernel, dim3(nb(n)), dim3(256), 0, s, w, V, n, hd + MAXK, k + 1, n);
+            norm(w, st.nrm.as<double>());
+            double hbuf[2 * MAXK], wn2;
+            EBG_HIP(hipMemcpyAsync(hbuf, hd, sizeof(double) * 2 * MAXK, hipMemcpyDeviceToHost, s));
+            EBG_HIP(hipMemcpyAsync(&wn2, st.nrm.p, 8, hipMemcpyDeviceToHost, s));
+            EBG_HIP(hipStreamSynchronize(s));
+            for (int q = 0; q <= k; ++q) H[(size_t)k * (m + 1) + q] = hbuf[q] + hbuf[MAXK + q];
+            const double hn = std::sqrt(wn2);
+            H[(size_t)k * (m + 1) + k + 1] = hn;
+            if (!std::isfinite(hn)) return EBG_ERR_NUMERIC;
+            if (hn > 0) {
+                EBG_HIP(hipMemcpyAsync(st.nrm.p, &hn, 8, hipMemcpyHostToDevice, s));
+                hipLaunchKernelGGL(scale_kernel, dim3(nb(n)), dim3(256), 0, s, V + (int64_t)(k + 1) * n, w,
+                                   st.nrm.as<double>(), n);
+            }
+            // Givens rotations on column k
+            double* col = &H[(size_t)k * (m + 1)];
+            for (int q = 0; q < k; ++q) {
+                const double t = cs[q] * col[q] + sn[q] * col[q + 1];
+                col[q + 1] = -sn[q] * col[q] + cs[q] * col[q + 1];
+                col[q] = t;
+            }
+            const double d = std::hypot(col[k], col[k + 1]);
+            cs[k] = col[k] / d;
+            sn[k] = col[k + 1] / d;
+            col[k] = d;
+            col[k + 1] = 0.0;
+            g[k + 1] = -sn[k] * g[k];
+            g[k] = cs[k] * g[k];
+            ++total;
+            if (std::fabs(g[k + 1]) / bnorm <= o.rtol || hn == 0) {
+                ++k;
+                done = true;
+                break;
+            }
+        }
+        // back substitution and x += Z y
+        std::vector<double> y(k);
+        for (int q = k - 1; q >= 0; --q) {
+            double v = g[q];
+            for (int p = q + 1; p < k; ++p) v -= H[(size_t)p * (m + 1) + q] * y[p];
+            y[q] = v / H[(size_t)q * (m + 1) + q];
+        }
+        EBG_HIP(hipMemcpyAsync(st.ydev.p, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(combine_kernel, dim3(nb(n)), dim3(256), 0, s, x, Z, n, st.ydev.as<double>(), k, n);
+        ++restarts;
+        (void)done;
+    }
+    if (stats) {
+        stats->iterations = total;
+        stats->restarts = restarts;
+        stats->relres = relres;
+        stats->relres_initial = relres0;
+    }
+    return relres <= o.rtol ? EBG_OK : EBG_NOT_CONVERGED;
+}
+
+}  // namespace ebg
+
+using namespace ebg;
+
+extern "C" {
+
+int ebg_stokes_release(ebg_ctx* ctx)
+{
+    if (ctx && ctx->stokes) {
+        StokesState* st = (StokesState*)ctx->stokes;
+        if (st->learned) learned_release(st->learned);
+        delete st;
+        ctx->stokes = nullptr;
+    }
+    return EBG_OK;
+}
+
+int ebg_stokes_set_system(ebg_ctx* ctx, const double* etas0, const double* etan0, const double* rho1_vy,
+                          const double* RX1, const double* RY1, const double* RC1,
+                          const double* btopx, const double* btopy, const double* bbottomx,
+                          const double* bbottomy, const double* bleftx, const double* blefty,
+                          const double* brightx, const double* brighty, const int64_t* bintern_zone,
+                          const double* bintern_velocity, int pressure_bc_mode, double pressure_bc,
+                          double pscale, double timestep, double gravity_y,
+                          int iuse_interface_stabilization)
+{
+    EBG_CHECK(ctx && ctx->has_grid, EBG_ERR_STATE, "ebg_stokes_set_system: no grid set");
+    EBG_CHECK(etas0 && etan0 && RX1 && RY1 && RC1, EBG_ERR_ARG, "ebg_stokes_set_system: NULL field");
+    EBG_CHECK(rho1_vy || iuse_interface_stabilization != 1, EBG_ERR_ARG,
+              "ebg_stokes_set_system: rho1_vy required with interface stabilization");
+    EBG_CHECK(btopx && btopy && bbottomx && bbottomy && bleftx && blefty && brightx && brighty,
+              EBG_ERR_ARG, "ebg_stokes_set_system: NULL boundary array");
+    EBG_CHECK(pressure_bc_mode >= 0 && pressure_bc_mode <= 2, EBG_ERR_ARG,
+              "ebg_stokes_set_system: pressure_bc_mode must be 0, 1 or 2");
+    EBG_CHECK(std::isfinite(pscale) && pscale > 0, EBG_ERR_ARG, "ebg_stokes_set_system: pscale must be > 0");
+    if (!ctx->stokes) ctx->stokes = new StokesState();
+    StokesState& st = *state(ctx);
+    const Grid2D& g = ctx->grid;
+    hipStream_t s = ctx->stream;
+    const int xnum = (int)g.xnum, ynum = (int)g.ynum;
+    if (st.xnum != xnum || st.ynum != ynum) {
+        st.Hf.reset();
+        st.Hd.reset();
+        st.mg_key_nlev = -1;
+    }
+    st.xnum = xnum;
+    st.ynum = ynum;
+    st.nc = (xnum - 1) * (ynum - 1);
+    int rc;
+    const size_t ny = ynum, nx = xnum;
+    if ((rc = copy_in(st.etas, etas0, ny * nx, s))) return rc;
+    if ((rc = copy_in(st.etan, etan0, (ny - 1) * (nx - 1), s))) return rc;
+    if (rho1_vy) { if ((rc = copy_in(st.rho, rho1_vy, ny * (nx + 1), s))) return rc; }
+    else if ((rc = st.rho.ensure(8))) return rc;
+    if ((rc = copy_in(st.RX, RX1, (ny + 1) * nx, s))) return rc;
+    if ((rc = copy_in(st.RY, RY1, ny * (nx + 1), s))) return rc;
+    if ((rc = copy_in(st.RC, RC1, (ny - 1) * (nx - 1), s))) return rc;
+    // boundary arrays: 4 x (nx+1)x2 then 4 x (ny+1)x2, host -> device
+    const size_t bx = (nx + 1) * 2, by = (ny + 1) * 2;
+    std::vector<double> hb(4 * bx + 4 * by);
+    std::memcpy(&hb[0], btopx, bx * 8);
+    std::memcpy(&hb[bx], btopy, bx * 8);
+    std::memcpy(&hb[2 * bx], bbottomx, bx * 8);
+    std::memcpy(&hb[3 * bx], bbottomy, bx * 8);
+    std::memcpy(&hb[4 * bx], bleftx, by * 8);
+    std::memcpy(&hb[4 * bx + by], blefty, by * 8);
+    std::memcpy(&hb[4 * bx + 2 * by], brightx, by * 8);
+    std::memcpy(&hb[4 * bx + 3 * by], brighty, by * 8);
+    if ((rc = st.bcs.ensure(hb.size() * 8))) return rc;
+    EBG_HIP(hipMemcpyAsync(st.bcs.p, hb.data(), hb.size() * 8, hipMemcpyHostToDevice, s));
+    EBG_HIP(hipStreamSynchronize(s));
+    double* B = st.bcs.as<double>();
+    StokesOpDev& op = st.op;
+    op.xnum = xnum;
+    op.ynum = ynum;
+    op.xstp = g.xstp_b;
+    op.ystp = g.ystp_b;
+    op.xstpc = g.xstp_vy;
+    op.ystpc = g.ystp_vx;
+    op.xstpavg = g.xstpavg;
+    op.ystpavg = g.ystpavg;
+    op.etan = st.etan.as<double>();
+    op.etas = st.etas.as<double>();
+    op.rho1_vy = st.rho.as<double>();
+    op.RX = st.RX.as<double>();
+    op.RY = st.RY.as<double>();
+    op.RC = st.RC.as<double>();
+    op.btopx = B;
+    op.btopy = B + bx;
+    op.bbottomx = B + 2 * bx;
+    op.bbottomy = B + 3 * bx;
+    op.bleftx = B + 4 * bx;
+    op.blefty = B + 4 * bx + by;
+    op.brightx = B + 4 * bx + 2 * by;
+    op.brighty = B + 4 * bx + 3 * by;
+    const int64_t nz[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+    const int64_t* z = bintern_zone ? bintern_zone : nz;
+    op.zx_j = (int)z[0];
+    op.zx_i0 = (int)z[1];
+    op.zx_i1 = (int)z[2];
+    op.zy_j = (int)z[4];
+    op.zy_i0 = (int)z[5];
+    op.zy_i1 = (int)z[6];
+    op.zvx = bintern_velocity ? bintern_velocity[3] : 0.0;
+    op.zvy = bintern_velocity ? bintern_velocity[7] : 0.0;
+    op.pressure_bc_mode = pressure_bc_mode;
+    op.pressure_bc = pressure_bc;
+    op.pscale = pscale;
+    op.timestep = timestep;
+    op.gravity_y = gravity_y;
+    op.stab = iuse_interface_stabilization;
+    st.visc_dirty = true;
+    return EBG_OK;
+}
+
+int ebg_stokes_apply(ebg_ctx* ctx, const double* S, double* LS)
+{
+    StokesState* st = state(ctx);
+    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_apply: no system set");
+    EBG_CHECK(S && LS && S != LS, EBG_ERR_ARG, "ebg_stokes_apply: bad vectors");
+    hipLaunchKernelGGL((stokes_rows_kernel<false, LayoutInterleaved>), dim3(nb(st->nc)), dim3(256), 0,
+                       ctx->stream, st->op, S, LS, LayoutInterleaved{st->nc});
+    EBG_HIP(hipGetLastError());
+    return EBG_OK;
+}
+
+int ebg_stokes_rhs(ebg_ctx* ctx, double* R)
+{
+    StokesState* st = state(ctx);
+    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_rhs: no system set");
+    EBG_CHECK(R, EBG_ERR_ARG, "ebg_stokes_rhs: NULL R");
+    hipLaunchKernelGGL((stokes_rows_kernel<true, LayoutInterleaved>), dim3(nb(st->nc)), dim3(256), 0,
+                       ctx->stream, st->op, (const double*)nullptr, R, LayoutInterleaved{st->nc});
+    EBG_HIP(hipGetLastError());
+    return EBG_OK;
+}
+
+int ebg_stokes_residual(ebg_ctx* ctx, const double* S, double* resnl)
+{
+    StokesState* st = state(ctx);
+    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_residual: no system set");
+    EBG_CHECK(S && resnl && S != resnl, EBG_ERR_ARG, "ebg_stokes_residual: bad vectors");
+    const int64_t n = 3 * (int64_t)st->nc;
+    int rc;
+    if ((rc = st->stage.ensure(2 * n * 8))) return rc;
+    double* LS = st->stage.as<double>();
+    double* R = LS + n;
+    LayoutInterleaved L{st->nc};
+    hipLaunchKernelGGL((stokes_rows_kernel<false, LayoutInterleaved>), dim3(nb(st->nc)), dim3(256), 0,
+                       ctx->stream, st->op, S, LS, L);
+    hipLaunchKernelGGL((stokes_rows_kernel<true, LayoutInterleaved>), dim3(nb(st->nc)), dim3(256), 0,
+                       ctx->stream, st->op, (const double*)nullptr, R, L);
+    hipLaunchKernelGGL(abs_diff_kernel, dim3(nb(n)), dim3(256), 0, ctx->stream, LS, R, resnl, n);
+    EBG_HIP(hipGetLastError());
+    return EBG_OK;
+}
+
+int ebg_stokes_unpack(ebg_ctx* ctx, const double* S, double* vx1, double* vy1, double* pr1)
+{
+    StokesState* st = state(ctx);
+    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_unpack: no system set");
+    EBG_CHECK(S && vx1 && vy1 && pr1, EBG_ERR_ARG, "ebg_stokes_unpack: NULL array");
+    const int xnum = st->xnum, ynum = st->ynum;
+    hipLaunchKernelGGL(unpack_kernel, dim3(nb((int64_t)(ynum + 1) * (xnum + 1))), dim3(256), 0,
+                       ctx->stream, st->op, S, vx1, vy1, pr1);
+    hipLaunchKernelGGL(vel_bc_rows_kernel, dim3(nb(xnum + 1)), dim3(256), 0, ctx->stream, st->op, vx1, vy1);
+    hipLaunchKernelGGL(vel_bc_cols_kernel, dim3(nb(ynum + 1)), dim3(256), 0, ctx->stream, st->op, vx1, vy1);
+    EBG_HIP(hipGetLastError());
+    return EBG_OK;
+}
+
+int ebg_stokes_solve_opts_default(ebg_solve_opts* o)
+{
+    EBG_CHECK(o, EBG_ERR_ARG, "ebg_stokes_solve_opts_default: NULL");
+    o->precision = 1;
+    o->restart = 30;
+    o->max_iterations = 2000;
+    o->rtol = 1e-8;
+    o->nlevels = 0;
+    o->finest_sweeps = 4;
+    o->sweep_multiplier = 2;
+    o->sweep_cap = 48;
+    o->theta_s = 0.9;
+    o->theta_c = 1.0;
+    o->use_initial_guess = 0;
+    o->learned = 0;
+    for (int k = 0; k < 4; ++k) o->bc_d[k] = 1.0;
+    return EBG_OK;
+}
+
+int ebg_stokes_solve(ebg_ctx* ctx, double* S, const ebg_solve_opts* opts, ebg_solve_stats* stats)
+{
+    StokesState* st = state(ctx);
+    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_solve: no system set");
+    EBG_CHECK(S, EBG_ERR_ARG, "ebg_stokes_solve: NULL S");
+    ebg_solve_opts o;
+    if (opts) o = *opts;
+    else ebg_stokes_solve_opts_default(&o);
+    EBG_CHECK(o.restart >= 1 && o.restart < MAXK - 1, EBG_ERR_ARG, "ebg_stokes_solve: restart must be in [1, 62]");
+    EBG_CHECK(o.precision == 0 || o.precision == 1, EBG_ERR_ARG, "ebg_stokes_solve: precision must be 0 or 1");
+    EBG_CHECK(o.rtol > 0 && o.max_iterations >= 0, EBG_ERR_ARG, "ebg_stokes_solve: bad tolerance");
+    EBG_CHECK(o.finest_sweeps >= 1 && o.sweep_multiplier >= 1, EBG_ERR_ARG, "ebg_stokes_solve: bad sweeps");
+    EBG_CHECK(o.learned == 0 || st->learned, EBG_ERR_STATE, "ebg_stokes_solve: learned correction not loaded");
+    const auto t0 = std::chrono::steady_clock::now();
+    const int64_t n = 3 * (int64_t)st->nc;
+    int rc;
+    if ((rc = st->x.ensure(n * 8))) return rc;
+    if (o.use_initial_guess)
+        hipLaunchKernelGGL(interleaved_to_planar_kernel, dim3(nb(st->nc)), dim3(256), 0, ctx->stream, S,
+                           st->x.as<double>(), st->nc);
+    else
+        hipLaunchKernelGGL(fill_kernel<double>, dim3(nb(n)), dim3(256), 0, ctx->stream, st->x.as<double>(), n, 0.0);
+    rc = fgmres(ctx, *st, o, stats);
+    if (rc < 0) return rc;
+    hipLaunchKernelGGL(planar_to_interleaved_kernel, dim3(nb(st->nc)), dim3(256), 0, ctx->stream,
+                       st->x.as<double>(), S, st->nc);
+    EBG_HIP(hipStreamSynchronize(ctx->stream));
+    if (stats) {
+        stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        stats->status = rc;
+    }
+    return rc;
+}
+
+}  // extern "C"

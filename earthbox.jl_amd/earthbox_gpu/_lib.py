@@ -37,7 +37,37 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_marker_cell_indices": (c_int, [c_void_p, c_int64] + [c_void_p] * 5),
     "ebg_rk4_advect": (c_int, [c_void_p, c_int64, c_int, c_double] + [c_void_p] * 13 + [c_int]),
     "ebg_calculate_spin": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_stokes_set_system": (c_int, [c_void_p] + [c_void_p] * 16 + [c_int, c_double, c_double,
+                                                                     c_double, c_double, c_int]),
+    "ebg_stokes_apply": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ebg_stokes_rhs": (c_int, [c_void_p, c_void_p]),
+    "ebg_stokes_residual": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ebg_stokes_unpack": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_stokes_solve_opts_default": (c_int, [c_void_p]),
+    "ebg_stokes_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_mg2d_create": (c_int, [c_void_p, c_int64, c_int64, c_double, c_double, c_int, c_void_p]),
+    "ebg_mg2d_level_shape": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "ebg_mg2d_copy_field": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "ebg_mg2d_restrict_viscosity": (c_int, [c_void_p]),
+    "ebg_mg2d_smoother": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_double, c_void_p]),
+    "ebg_mg2d_vcycles": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
+                                 c_double, c_double, c_double, c_void_p]),
 }
+
+
+class SolveOpts(C.Structure):
+    """ebg_solve_opts (include/earthbox_gpu.h)."""
+    _fields_ = [("precision", c_int), ("restart", c_int), ("max_iterations", c_int),
+                ("rtol", c_double), ("nlevels", c_int), ("finest_sweeps", c_int),
+                ("sweep_multiplier", c_int), ("sweep_cap", c_int), ("theta_s", c_double),
+                ("theta_c", c_double), ("use_initial_guess", c_int), ("learned", c_int),
+                ("bc_d", c_double * 4)]
+
+
+class SolveStats(C.Structure):
+    """ebg_solve_stats (include/earthbox_gpu.h)."""
+    _fields_ = [("iterations", c_int), ("restarts", c_int), ("relres", c_double),
+                ("relres_initial", c_double), ("seconds", c_double), ("status", c_int)]
 
 
 class EBGError(RuntimeError):

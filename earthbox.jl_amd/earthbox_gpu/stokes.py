@@ -1,0 +1,163 @@
+"""Host-side mirror of the reference's Stokes solve seam and multigrid driver.
+
+StokesSolverGPU replaces, per Picard iteration, the reference's
+  SystemSolver.solve_system_optimized (solver/system_solver/SystemSolver.jl:67-91),
+  SystemSolver.process_stokes_solution! (:141-151) and
+  StokesResiduals.stokes_calc_nonlinear_system_residual_optimized!
+  (solver/residuals/StokesResiduals.jl:63-82),
+on device-resident fields.  MultigridGPU drives the reference's experimental
+2-D multigrid (multigrid/multigrid_solver/MultigridSolver.jl:113-135).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import SolveOpts, SolveStats, check, lib
+from .context import GPUContext
+from .device import empty, ptr, to_device
+from .stokes_problem import StokesProblem
+
+MG_FIELDS = {"etas": 0, "etan": 1, "vx": 2, "vy": 3, "pr": 4, "RX": 5, "RY": 6, "RC": 7,
+             "rx": 8, "ry": 9, "rc": 10}
+
+
+def default_opts(**kw) -> SolveOpts:
+    o = SolveOpts()
+    check(lib.ebg_stokes_solve_opts_default(C.byref(o)), "ebg_stokes_solve_opts_default")
+    for k, v in kw.items():
+        if k == "bc_d":
+            for q in range(4):
+                o.bc_d[q] = float(v[q])
+        else:
+            setattr(o, k, v)
+    return o
+
+
+def bc_mirror_factors(prob: StokesProblem) -> tuple[float, float, float, float]:
+    """Tangential ghost factors D of the walls (top/bottom for vx, left/right
+    for vy) for the preconditioner's homogeneous boundary conditions."""
+    b = prob.bcs
+    return (float(b.btopx[1, 1]), float(b.bbottomx[1, 1]), float(b.blefty[1, 1]),
+            float(b.brighty[1, 1]))
+
+
+class StokesSolverGPU:
+    """One assembled Stokes-continuity system resident on the GPU."""
+
+    def __init__(self, ctx: GPUContext):
+        self.ctx = ctx
+        self.prob: StokesProblem | None = None
+
+    def set_system(self, prob: StokesProblem, fields: dict | None = None) -> None:
+        """Upload the fields (numpy, or device tensors in `fields`)."""
+        if self.ctx.grid is None or self.ctx.grid.xnum != prob.grid.xnum:
+            self.ctx.set_grid(prob.grid)
+        f = fields or {}
+        self._dev = {k: f.get(k) if f.get(k) is not None else to_device(getattr(prob, k))
+                     for k in ("etas0", "etan0", "rho1_vy", "RX1", "RY1", "RC1")}
+        b = prob.bcs
+        self._host = [np.asfortranarray(a, dtype=np.float64) for a in
+                      (b.btopx, b.btopy, b.bbottomx, b.bbottomy, b.bleftx, b.blefty,
+                       b.brightx, b.brighty)]
+        self._zone = np.ascontiguousarray(b.bintern_zone, dtype=np.int64)
+        self._zvel = np.ascontiguousarray(b.bintern_velocity, dtype=np.float64)
+        d = self._dev
+        check(lib.ebg_stokes_set_system(
+            self.ctx.handle, ptr(d["etas0"]), ptr(d["etan0"]), ptr(d["rho1_vy"]),
+            ptr(d["RX1"]), ptr(d["RY1"]), ptr(d["RC1"]),
+            *[a.ctypes.data for a in self._host], self._zone.ctypes.data, self._zvel.ctypes.data,
+            int(prob.pressure_bc_mode), float(prob.pressure_bc), float(prob.pscale),
+            float(prob.timestep), float(prob.gravity_y), int(prob.iuse_interface_stabilization)),
+            "ebg_stokes_set_system")
+        self.prob = prob
+
+    def new_vector(self) -> torch.Tensor:
+        return torch.zeros(self.prob.N, dtype=torch.float64, device=self.ctx.device)
+
+    def apply(self, S: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        out = self.new_vector() if out is None else out
+        check(lib.ebg_stokes_apply(self.ctx.handle, ptr(S, np.float64, self.prob.N),
+                                   ptr(out, np.float64, self.prob.N)), "ebg_stokes_apply")
+        return out
+
+    def rhs(self) -> torch.Tensor:
+        out = self.new_vector()
+        check(lib.ebg_stokes_rhs(self.ctx.handle, ptr(out, np.float64, self.prob.N)), "ebg_stokes_rhs")
+        return out
+
+    def stokes_calc_nonlinear_system_residual(self, S: torch.Tensor) -> torch.Tensor:
+        out = self.new_vector()
+        check(lib.ebg_stokes_residual(self.ctx.handle, ptr(S, np.float64, self.prob.N),
+                                      ptr(out, np.float64, self.prob.N)), "ebg_stokes_residual")
+        return out
+
+    def solve_system(self, S: torch.Tensor | None = None, **opts) -> tuple[torch.Tensor, dict]:
+        """FGMRES + multigrid; returns (S, stats dict).  Raises on hard errors;
+        stats["converged"] is False when the iteration budget ran out."""
+        S = self.new_vector() if S is None else S
+        o = default_opts(**opts)
+        st = SolveStats()
+        rc = check(lib.ebg_stokes_solve(self.ctx.handle, ptr(S, np.float64, self.prob.N),
+                                        C.byref(o), C.byref(st)), "ebg_stokes_solve")
+        stats = {k: getattr(st, k) for k, _ in SolveStats._fields_}
+        stats["converged"] = (rc == 0)
+        return S, stats
+
+    def process_stokes_solution(self, S: torch.Tensor):
+        g = self.prob.grid
+        vx1 = empty((g.ynum + 1, g.xnum), np.float64)
+        vy1 = empty((g.ynum, g.xnum + 1), np.float64)
+        pr1 = empty((g.ynum - 1, g.xnum - 1), np.float64)
+        check(lib.ebg_stokes_unpack(self.ctx.handle, ptr(S, np.float64, self.prob.N),
+                                    ptr(vx1), ptr(vy1), ptr(pr1)), "ebg_stokes_unpack")
+        return vx1, vy1, pr1
+
+
+class MultigridGPU:
+    """The reference's experimental 2-D multigrid hierarchy on the device (FP64)."""
+
+    def __init__(self, ctx: GPUContext, xnum, ynum, xsize, ysize, nlevels=0):
+        self.ctx = ctx
+        n = C.c_int()
+        check(lib.ebg_mg2d_create(ctx.handle, xnum, ynum, xsize, ysize, nlevels, C.byref(n)),
+              "ebg_mg2d_create")
+        self.nlevels = n.value
+        self.shapes = []
+        for k in range(self.nlevels):
+            x, y = C.c_int64(), C.c_int64()
+            check(lib.ebg_mg2d_level_shape(ctx.handle, k, C.byref(x), C.byref(y)), "level_shape")
+            self.shapes.append((x.value, y.value))
+
+    def field_shape(self, level, name):
+        x, y = self.shapes[level]
+        return {"etas": (y, x), "etan": (y - 1, x - 1), "vx": (y + 1, x), "vy": (y, x + 1),
+                "pr": (y - 1, x - 1), "RX": (y + 1, x), "RY": (y, x + 1), "RC": (y - 1, x - 1),
+                "rx": (y + 1, x), "ry": (y, x + 1), "rc": (y - 1, x - 1)}[name]
+
+    def get(self, level, name) -> torch.Tensor:
+        t = empty(self.field_shape(level, name), np.float64)
+        check(lib.ebg_mg2d_copy_field(self.ctx.handle, level, MG_FIELDS[name], ptr(t), None),
+              "ebg_mg2d_copy_field")
+        return t
+
+    def set(self, level, name, t: torch.Tensor) -> None:
+        check(lib.ebg_mg2d_copy_field(self.ctx.handle, level, MG_FIELDS[name], None, ptr(t)),
+              "ebg_mg2d_copy_field")
+
+    def restrict_viscosity(self):
+        check(lib.ebg_mg2d_restrict_viscosity(self.ctx.handle), "ebg_mg2d_restrict_viscosity")
+
+    def smoother(self, level, nsweeps, theta_s=0.9, theta_c=0.3, prnorm=0.0, bc_d=(1, 1, 1, 1)):
+        d = (C.c_double * 4)(*bc_d)
+        check(lib.ebg_mg2d_smoother(self.ctx.handle, level, nsweeps, theta_s, theta_c, prnorm, d),
+              "ebg_mg2d_smoother")
+
+    def vcycles(self, ncycles, finest_sweeps=2, multiplier=2, cap=48, theta_s=0.9, theta_c=0.3,
+                relax_v=1.0, relax_p=1.0, prnorm=0.0, bc_d=(1, 1, 1, 1)):
+        d = (C.c_double * 4)(*bc_d)
+        check(lib.ebg_mg2d_vcycles(self.ctx.handle, ncycles, finest_sweeps, multiplier, cap,
+                                   theta_s, theta_c, relax_v, relax_p, prnorm, d),
+              "ebg_mg2d_vcycles")

@@ -60,3 +60,87 @@ def solid_body_rotation_fields(g: StaggeredGrid2D, omega: float):
     vx1 = np.asfortranarray(-omega * (g.gridy_vx[:, None] - yc) * np.ones((1, g.xnum)))
     vy1 = np.asfortranarray(omega * (g.gridx_vy[None, :] - xc) * np.ones((g.ynum, 1)))
     return vx1, vy1
+
+
+def _rhs_viscous(g: StaggeredGrid2D, rho1: np.ndarray, rho1_vy: np.ndarray, gravity_y: float,
+                 stabilization: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """RX1, RY1, RC1 for a purely viscous, stress-free state (gravity only),
+    i.e. solver/rhs/StokesContinuityRhs.jl:30-190 with sxx0 = sxy0 = 0 and
+    gravity_x = 0."""
+    ny, nx = g.ynum, g.xnum
+    RX1 = np.zeros((ny + 1, nx), order="F")
+    RY1 = np.zeros((ny, nx + 1), order="F")
+    RC1 = np.zeros((ny - 1, nx - 1), order="F")
+    # RY1[i, j] for i in 2..ynum-1, j in 2..xnum (1-based)
+    if stabilization == 1:
+        RY1[1:ny - 1, 1:nx] = -gravity_y * rho1_vy[1:ny - 1, 1:nx]
+    else:
+        RY1[1:ny - 1, 1:nx] = -gravity_y * (rho1[1:ny - 1, 1:nx] + rho1[1:ny - 1, 0:nx - 1]) / 2.0
+    # RX1 = -gravity_x*(...) = 0 written as -0.0*(...) by the reference; zeros suffice.
+    return RX1, RY1, RC1
+
+
+def falling_block(xnum: int = 101, ynum: int = 101, size: float = 1.0e6,
+                  block: float = 1.0e5, eta_bg: float = 1e21, eta_block: float = 1e27,
+                  rho_bg: float = 3200.0, rho_block: float = 3300.0, gravity_y: float = 10.0,
+                  stabilization: int = 0, timestep: float = 0.0):
+    """Nodal falling-block Stokes problem (free-slip box, centred square block):
+    viscosity and density sampled at the staggered nodes."""
+    from .grid import uniform_grid
+    from .stokes_problem import StokesProblem, free_slip_bcs
+    g = uniform_grid(xnum, ynum, size, size)
+    xc = yc = 0.5 * size
+    def inside(X, Y):
+        return (np.abs(X - xc) <= block / 2) & (np.abs(Y - yc) <= block / 2)
+    Xb, Yb = np.meshgrid(g.gridx_b, g.gridy_b)
+    Xp, Yp = np.meshgrid(g.gridx_pr, g.gridy_pr)
+    Xv, Yv = np.meshgrid(g.gridx_vy, g.gridy_b)
+    etas = np.where(inside(Xb, Yb), eta_block, eta_bg)
+    etan = np.where(inside(Xp, Yp), eta_block, eta_bg)
+    rho1 = np.where(inside(Xb, Yb), rho_block, rho_bg)
+    rho1_vy = np.where(inside(Xv, Yv), rho_block, rho_bg)
+    RX1, RY1, RC1 = _rhs_viscous(g, rho1, rho1_vy, gravity_y, stabilization)
+    return StokesProblem(grid=g, etas0=etas, etan0=etan, rho1_vy=rho1_vy, RX1=RX1, RY1=RY1,
+                         RC1=RC1, bcs=free_slip_bcs(g), pressure_bc_mode=0, pressure_bc=0.0,
+                         timestep=timestep, gravity_y=gravity_y,
+                         iuse_interface_stabilization=stabilization)
+
+
+def layered_random(xnum: int = 1001, ynum: int = 501, xsize: float = 1.5e6, ysize: float = 6.0e5,
+                   seed: int = 2, corr_len: float = 2.0e4, gravity_y: float = 10.0,
+                   log_eta_range=(18.0, 25.0)):
+    """Layered log10-viscosity with a Gaussian random perturbation (correlation
+    length corr_len, FFT-filtered) clipped to log_eta_range, density
+    2700-3300 tied to the layers; free-slip walls."""
+    from .grid import uniform_grid
+    from .stokes_problem import StokesProblem, free_slip_bcs
+    g = uniform_grid(xnum, ynum, xsize, ysize)
+    rng = np.random.default_rng(seed)
+    # perturbation on a fine uniform lattice covering the domain
+    nxf, nyf = 2 * xnum, 2 * ynum
+    noise = rng.standard_normal((nyf, nxf))
+    kx = np.fft.fftfreq(nxf, d=xsize / nxf)
+    ky = np.fft.fftfreq(nyf, d=ysize / nyf)
+    K2 = ky[:, None] ** 2 + kx[None, :] ** 2
+    filt = np.exp(-0.5 * (2 * np.pi) ** 2 * K2 * corr_len ** 2)
+    field = np.real(np.fft.ifft2(np.fft.fft2(noise) * filt))
+    field /= field.std()
+    layer_depths = np.array([0.0, 0.05, 0.20, 0.45, 1.01]) * ysize
+    layer_logeta = np.array([23.0, 21.5, 24.0, 19.5])
+    layer_rho = np.array([2700.0, 2900.0, 3300.0, 3250.0])
+    def sample(X, Y):
+        ix = np.clip((X / xsize * nxf).astype(int), 0, nxf - 1)
+        iy = np.clip((Y / ysize * nyf).astype(int), 0, nyf - 1)
+        lay = np.clip(np.searchsorted(layer_depths, Y, side="right") - 1, 0, 3)
+        loge = np.clip(layer_logeta[lay] + 1.5 * field[iy, ix], *log_eta_range)
+        return 10.0 ** loge, layer_rho[lay] + 30.0 * field[iy, ix]
+    Xb, Yb = np.meshgrid(g.gridx_b, g.gridy_b)
+    Xp, Yp = np.meshgrid(g.gridx_pr, g.gridy_pr)
+    Xv, Yv = np.meshgrid(g.gridx_vy, g.gridy_b)
+    etas, rho1 = sample(Xb, Yb)
+    etan, _ = sample(Xp, Yp)
+    _, rho1_vy = sample(Xv, Yv)
+    RX1, RY1, RC1 = _rhs_viscous(g, rho1, rho1_vy, gravity_y, 0)
+    return StokesProblem(grid=g, etas0=etas, etan0=etan, rho1_vy=rho1_vy, RX1=RX1, RY1=RY1,
+                         RC1=RC1, bcs=free_slip_bcs(g), pressure_bc_mode=0, pressure_bc=0.0,
+                         timestep=0.0, gravity_y=gravity_y, iuse_interface_stabilization=0)

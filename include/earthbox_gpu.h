@@ -107,6 +107,97 @@ int ebg_rk4_advect(ebg_ctx *ctx, int64_t marknum, int order, double timestep,
  * (stokes_continuity_solver/utils/strain_rate/GridStrainRate.jl:161-182). */
 int ebg_calculate_spin(ebg_ctx *ctx, const double *vx1, const double *vy1, double *esp);
 
+/* ---- Stokes-continuity system (assembled-operator conventions) ----
+ * Replaces the inputs StokesBuildData gathers (solver/build_system/utils/
+ * BuildStructs.jl:37-177).  Fields are DEVICE pointers with the reference
+ * shapes: etas0 (ynum,xnum), etan0 (ynum-1,xnum-1), rho1_vy (ynum,xnum+1; may be
+ * NULL without interface stabilization), RX1 (ynum+1,xnum), RY1 (ynum,xnum+1),
+ * RC1 (ynum-1,xnum-1).  The b* arrays are HOST pointers, column-major
+ * (xnum+1)x2 for the top and bottom arrays and (ynum+1)x2 for left and right
+ * (v_ghost = b[:,1] + b[:,2]*v_inner); bintern_zone (Int64[8]) and
+ * bintern_velocity (Float64[8]) are HOST pointers or NULL.  Fields are copied. */
+int ebg_stokes_set_system(ebg_ctx *ctx, const double *etas0, const double *etan0,
+                          const double *rho1_vy, const double *RX1, const double *RY1,
+                          const double *RC1, const double *btopx, const double *btopy,
+                          const double *bbottomx, const double *bbottomy, const double *bleftx,
+                          const double *blefty, const double *brightx, const double *brighty,
+                          const int64_t *bintern_zone, const double *bintern_velocity,
+                          int pressure_bc_mode, double pressure_bc, double pscale,
+                          double timestep, double gravity_y, int iuse_interface_stabilization);
+
+/* L*S for S in the reference's unknown order (per cell [vx, vy, p'], cells
+ * column-major): bit-identical to summing the reference's COO triplets of
+ * StokesBuildManager.build_system_of_equations (StokesBuildManager.jl:661-706). */
+int ebg_stokes_apply(ebg_ctx *ctx, const double *S, double *LS);
+/* The right-hand side R the reference's build produces (same order). */
+int ebg_stokes_rhs(ebg_ctx *ctx, double *R);
+/* K11: |L*S - R| (StokesResiduals.stokes_calc_nonlinear_system_residual_optimized!,
+ * solver/residuals/StokesResiduals.jl:63-82). */
+int ebg_stokes_residual(ebg_ctx *ctx, const double *S, double *resnl);
+/* K10: SystemSolver.reload_solu_vel! + apply_vel_bc! (SystemSolver.jl:188-271):
+ * vx1 (ynum+1,xnum), vy1 (ynum,xnum+1), pr1 = S[p']*pscale (ynum-1,xnum-1). */
+int ebg_stokes_unpack(ebg_ctx *ctx, const double *S, double *vx1, double *vy1, double *pr1);
+
+typedef struct {
+    int precision;        /* 1: FP32 V-cycle preconditioner (mixed), 0: FP64 (plain) */
+    int restart;          /* FGMRES restart length (1..62), default 30 */
+    int max_iterations;   /* total FGMRES iterations */
+    double rtol;          /* stop at ||R - L*S||_2 / ||R||_2 <= rtol (FP64) */
+    int nlevels;          /* multigrid levels, 0 = reference rule (LevelManager.jl:441-457) */
+    int finest_sweeps;    /* smoothing sweeps on the finest level */
+    int sweep_multiplier; /* sweeps multiply by this per coarser level ... */
+    int sweep_cap;        /* ... capped at this (MultigridDataManager.jl:223-241) */
+    double theta_s;       /* velocity relaxation (relax_stokes) */
+    double theta_c;       /* pressure relaxation (relax_continuity) */
+    int use_initial_guess;/* 1: start from S, 0: start from zero */
+    int learned;          /* 1: add the learned correction (needs ebg_learned_load) */
+    double bc_d[4];       /* preconditioner ghost mirror factors top, bottom (vx), left, right (vy):
+                             +1 free slip, -1 no slip */
+} ebg_solve_opts;
+
+typedef struct {
+    int iterations;
+    int restarts;
+    double relres;         /* final ||R - L*S|| / ||R|| (FP64, recomputed) */
+    double relres_initial;
+    double seconds;
+    int status;
+} ebg_solve_stats;
+
+int ebg_stokes_solve_opts_default(ebg_solve_opts *opts);
+/* Replaces SystemSolver.solve_system_optimized / ParallelSolver.parallel_direct_solver
+ * (solver/system_solver/SystemSolver.jl:67-91, parallel_solver/ParallelSolver.jl:52-126).
+ * S (device, reference order, length 3*(xnum-1)*(ynum-1)) receives the solution.
+ * Returns EBG_OK, EBG_NOT_CONVERGED (S holds the last iterate) or an error. */
+int ebg_stokes_solve(ebg_ctx *ctx, double *S, const ebg_solve_opts *opts, ebg_solve_stats *stats);
+
+/* ---- the reference's experimental 2-D multigrid (FP64) ----
+ * Replaces multigrid/multigrid_solver/MultigridSolver.jl:113-135 and the
+ * level, smoother, restriction and prolongation modules it drives. */
+enum {
+    EBG_MG_ETAS = 0, EBG_MG_ETAN = 1, EBG_MG_VX = 2, EBG_MG_VY = 3, EBG_MG_PR = 4,
+    EBG_MG_RX = 5, EBG_MG_RY = 6, EBG_MG_RC = 7, EBG_MG_RESX = 8, EBG_MG_RESY = 9,
+    EBG_MG_RESC = 10
+};
+/* Level hierarchy with uniform grids (LevelManager.jl:441-519); nlevels 0 =
+ * reference rule.  levels_out may be NULL. */
+int ebg_mg2d_create(ebg_ctx *ctx, int64_t xnum, int64_t ynum, double xsize, double ysize,
+                    int nlevels, int *levels_out);
+int ebg_mg2d_level_shape(ebg_ctx *ctx, int level, int64_t *xnum, int64_t *ynum);
+/* Copy one level field (EBG_MG_*) to dst or from src (device pointers, one NULL). */
+int ebg_mg2d_copy_field(ebg_ctx *ctx, int level, int which, double *dst, const double *src);
+/* LevelInterpolation.interpolate_viscosity_to_coarser_levels!
+ * (multigrid/interpolation/ViscosityRestriction.jl:68-298). */
+int ebg_mg2d_restrict_viscosity(ebg_ctx *ctx);
+/* Smoother.stokes_continuity2d_viscous_smoother! (multigrid/smoother/Smoother.jl:66-99). */
+int ebg_mg2d_smoother(ebg_ctx *ctx, int level, int nsweeps, double theta_s, double theta_c,
+                      double prnorm, const double *bc_d4);
+/* ncycles V-cycles (vcycle/MultigridVCycle.jl:197-267 without viscosity
+ * continuation): SmoothAndRestrict.jl:44-71 then SmoothAndProlongate.jl:60-115. */
+int ebg_mg2d_vcycles(ebg_ctx *ctx, int ncycles, int finest_sweeps, int multiplier, int cap,
+                     double theta_s, double theta_c, double relax_v, double relax_p,
+                     double prnorm, const double *bc_d4);
+
 #ifdef __cplusplus
 }
 #endif

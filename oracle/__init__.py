@@ -126,3 +126,98 @@ def calculate_spin(g, vx1, vy1) -> np.ndarray:
     esp = np.zeros((g.ynum, g.xnum), order="F")
     lib().ebo_calculate_spin(g.xnum, g.ynum, p(g.xstp_vy), p(g.ystp_vx), p(vx1), p(vy1), p(esp))
     return esp
+
+
+# ---------------------------------------------------------------- Stokes system
+class _StokesIn(C.Structure):
+    _fields_ = [("xnum", _I64), ("ynum", _I64),
+                ("xstp", _P), ("ystp", _P), ("xstpc", _P), ("ystpc", _P),
+                ("xstpavg", _D), ("ystpavg", _D),
+                ("etan", _P), ("etas", _P), ("rho1_vy", _P),
+                ("RX", _P), ("RY", _P), ("RC", _P),
+                ("btopx", _P), ("btopy", _P), ("bbottomx", _P), ("bbottomy", _P),
+                ("bleftx", _P), ("blefty", _P), ("brightx", _P), ("brighty", _P),
+                ("bintern_zone", _P), ("bintern_velocity", _P),
+                ("pressure_bc_mode", _I64), ("pressure_bc", _D),
+                ("pscale", _D), ("timestep", _D), ("gravity_y", _D),
+                ("iuse_interface_stabilization", _I64)]
+
+
+def _stokes_in(prob):
+    g, b = prob.grid, prob.bcs
+    keep = [np.ascontiguousarray(b.bintern_zone, dtype=np.int64),
+            np.ascontiguousarray(b.bintern_velocity, dtype=np.float64)]
+    s = _StokesIn(g.xnum, g.ynum, p(g.xstp_b), p(g.ystp_b), p(g.xstp_vy), p(g.ystp_vx),
+                  g.xstpavg, g.ystpavg, p(prob.etan0), p(prob.etas0), p(prob.rho1_vy),
+                  p(prob.RX1), p(prob.RY1), p(prob.RC1),
+                  p(b.btopx), p(b.btopy), p(b.bbottomx), p(b.bbottomy),
+                  p(b.bleftx), p(b.blefty), p(b.brightx), p(b.brighty),
+                  p(keep[0]), p(keep[1]), prob.pressure_bc_mode, prob.pressure_bc,
+                  prob.pscale, prob.timestep, prob.gravity_y, prob.iuse_interface_stabilization)
+    return s, keep
+
+
+def stokes_build(prob):
+    """K8: COO (Li, Lj 1-based int64; Lv) and R exactly as the reference builds them."""
+    L = lib()
+    L.ebo_stokes_build.restype = _I64
+    L.ebo_stokes_build.argtypes = [C.POINTER(_StokesIn), _P, _P, _P, _P]
+    s, keep = _stokes_in(prob)
+    N = prob.N
+    cap = 9 * N
+    Li = np.empty(cap, dtype=np.int64)
+    Lj = np.empty(cap, dtype=np.int64)
+    Lv = np.empty(cap)
+    R = np.empty(N)
+    nnz = L.ebo_stokes_build(C.byref(s), p(Li), p(Lj), p(Lv), p(R))
+    assert nnz <= cap
+    return Li[:nnz].copy(), Lj[:nnz].copy(), Lv[:nnz].copy(), R
+
+
+def stokes_unpack(prob, S):
+    """K10: (vx1, vy1, pr1) from the solution vector with ghost BCs applied."""
+    L = lib()
+    L.ebo_stokes_unpack.restype = None
+    L.ebo_stokes_unpack.argtypes = [C.POINTER(_StokesIn), _P, _P, _P, _P]
+    g = prob.grid
+    s, keep = _stokes_in(prob)
+    vx1 = np.zeros((g.ynum + 1, g.xnum), order="F")
+    vy1 = np.zeros((g.ynum, g.xnum + 1), order="F")
+    pr1 = np.zeros((g.ynum - 1, g.xnum - 1), order="F")
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    L.ebo_stokes_unpack(C.byref(s), p(S), p(vx1), p(vy1), p(pr1))
+    return vx1, vy1, pr1
+
+
+def coo_spmv(N, Li, Lj, Lv, x):
+    L = lib()
+    L.ebo_coo_spmv.restype = None
+    L.ebo_coo_spmv.argtypes = [_I64, _I64, _P, _P, _P, _P, _P]
+    y = np.empty(N)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    L.ebo_coo_spmv(N, Li.size, p(Li), p(Lj), p(Lv), p(x), p(y))
+    return y
+
+
+def coo_residual(N, Li, Lj, Lv, x, R):
+    """K11: |L*x - R| in COO triplet order (StokesResiduals.jl:63-82)."""
+    L = lib()
+    L.ebo_coo_residual.restype = None
+    L.ebo_coo_residual.argtypes = [_I64, _I64, _P, _P, _P, _P, _P, _P]
+    res = np.empty(N)
+    L.ebo_coo_residual(N, Li.size, p(Li), p(Lj), p(Lv), p(np.ascontiguousarray(x)), p(R), p(res))
+    return res
+
+
+def direct_solve(N, Li, Lj, Lv, R, refine: int = 3):
+    """FP64 sparse LU (scipy SuperLU) standing in for MUMPS
+    (SystemSolver.jl:67-91), plus FP64 iterative refinement steps."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    A = sp.csc_matrix((Lv, (Li - 1, Lj - 1)), shape=(N, N))
+    lu = spla.splu(A, permc_spec="COLAMD", options={"SymmetricMode": False})
+    S = lu.solve(R)
+    for _ in range(refine):
+        r = R - A @ S
+        S = S + lu.solve(r)
+    return S

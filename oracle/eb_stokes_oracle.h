@@ -1,0 +1,58 @@
+/* Oracle (test infrastructure only): assembled Stokes-continuity system. */
+#ifndef EB_STOKES_ORACLE_H
+#define EB_STOKES_ORACLE_H
+#include "eb_oracle.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Inputs of the K8 build, mirroring StokesBuildData
+ * (solver/build_system/utils/BuildStructs.jl:37-177). Matrices are
+ * column-major with the reference's shapes:
+ *   etan (ynum-1, xnum-1), etas (ynum, xnum), rho1_vy (ynum, xnum+1),
+ *   RX (ynum+1, xnum), RY (ynum, xnum+1), RC (ynum-1, xnum-1),
+ *   btopx/btopy/bbottomx/bbottomy (xnum+1, 2), bleftx/blefty/brightx/brighty (ynum+1, 2).
+ * bintern_zone[8] and bintern_velocity[8] are 0-based copies of the 1-based arrays. */
+typedef struct {
+    int64_t xnum, ynum;
+    const double *xstp, *ystp, *xstpc, *ystpc;
+    double xstpavg, ystpavg;
+    const double *etan, *etas, *rho1_vy;
+    const double *RX, *RY, *RC;
+    const double *btopx, *btopy, *bbottomx, *bbottomy;
+    const double *bleftx, *blefty, *brightx, *brighty;
+    const int64_t *bintern_zone;
+    const double *bintern_velocity;
+    int64_t pressure_bc_mode;
+    double pressure_bc;
+    double pscale, timestep, gravity_y;
+    int64_t iuse_interface_stabilization;
+} ebo_stokes_in;
+
+/* Inputs of K7 (viscoelastic terms and right-hand sides). */
+typedef struct {
+    int64_t xnum, ynum;
+    const double *xstp_b, *ystp_b, *xstp_vy, *ystp_vx;
+    const double *etas1, *mus1, *sxy1;      /* (ynum, xnum) */
+    const double *etan1, *mun1, *sxx1;      /* (ynum-1, xnum-1) */
+    const double *rho1, *rho1_vy;           /* (ynum, xnum), (ynum, xnum+1) */
+    const double *dilatation, *eii_plastic_pressure; /* (ynum-1, xnum-1), may be NULL */
+    double timestep, gravity_x, gravity_y;
+    int64_t iuse_interface_stabilization;
+} ebo_rhs_in;
+
+int64_t ebo_stokes_build(const ebo_stokes_in *s, int64_t *Li, int64_t *Lj, double *Lv, double *R);
+double ebo_pressure_scaling_coeff(double etan11, double xstpavg, double ystpavg);
+void ebo_stokes_rhs(const ebo_rhs_in *r, double *etas0, double *sxy0, double *etan0,
+                    double *sxx0, double *RX1, double *RY1, double *RC1);
+void ebo_stokes_unpack(const ebo_stokes_in *s, const double *S, double *vx1, double *vy1, double *pr1);
+void ebo_coo_residual(int64_t N, int64_t nnz, const int64_t *Li, const int64_t *Lj,
+                      const double *Lv, const double *x, const double *R, double *res);
+void ebo_coo_spmv(int64_t N, int64_t nnz, const int64_t *Li, const int64_t *Lj,
+                  const double *Lv, const double *x, double *y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

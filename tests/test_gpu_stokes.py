@@ -1,0 +1,119 @@
+"""GPU parity of the Stokes-continuity operator, RHS, unpack, residual and
+solve, and of the reference multigrid, against the oracle (through the C ABI).
+
+Bars: operator apply, RHS, unpack and |L S - R| bit-exact (same coefficients,
+same COO summation order); solve: ||R - L S||/||R|| <= 1e-8 at the default
+tolerance and velocity/pressure within 1e-8 relative L2 of the FP64 SuperLU
+solve when run to rtol 1e-13; reference multigrid bit-exact in FP64.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import oracle.mg as omg
+from earthbox_gpu import workloads
+from earthbox_gpu.context import GPUContext
+from earthbox_gpu.device import to_device, to_host
+from earthbox_gpu.stokes import MultigridGPU, StokesSolverGPU, bc_mirror_factors
+from stokes_cases import CASES, random_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_apply_rhs_residual_unpack_bit_exact(gpu_ctx, case):
+    print(f"stokes operator case {case}", flush=True)
+    pr = random_case(**case)
+    Li, Lj, Lv, R = oracle.stokes_build(pr)
+    rng = np.random.default_rng(case["seed"])
+    S = rng.standard_normal(pr.N) * np.tile([1e-9, 1e-9, 1e3], pr.N // 3)
+    sol = StokesSolverGPU(gpu_ctx)
+    sol.set_system(pr)
+    dS = to_device(S)
+    LS = to_host(sol.apply(dS))
+    Rg = to_host(sol.rhs())
+    res = to_host(sol.stokes_calc_nonlinear_system_residual(dS))
+    vx1, vy1, pr1 = sol.process_stokes_solution(dS)
+    gpu_ctx.synchronize()
+    assert np.array_equal(LS, oracle.coo_spmv(pr.N, Li, Lj, Lv, S))
+    assert np.array_equal(Rg, R)
+    assert np.array_equal(res, oracle.coo_residual(pr.N, Li, Lj, Lv, S, R))
+    ex, ey, ep = oracle.stokes_unpack(pr, S)
+    assert np.array_equal(to_host(vx1), ex)
+    assert np.array_equal(to_host(vy1), ey)
+    assert np.array_equal(to_host(pr1), ep)
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+@pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[3], CASES[6]])
+def test_solve_matches_direct(gpu_ctx, case, precision):
+    print(f"stokes solve case {case} precision {precision}", flush=True)
+    pr = random_case(**case)
+    Li, Lj, Lv, R = oracle.stokes_build(pr)
+    Sd = oracle.direct_solve(pr.N, Li, Lj, Lv, R)
+    sol = StokesSolverGPU(gpu_ctx)
+    sol.set_system(pr)
+    S, st = sol.solve_system(precision=precision, rtol=1e-13, max_iterations=3000,
+                             bc_d=bc_mirror_factors(pr))
+    assert st["converged"], st
+    Sg = to_host(S)
+    rel = np.linalg.norm(R - oracle.coo_spmv(pr.N, Li, Lj, Lv, Sg)) / np.linalg.norm(R)
+    assert rel <= 1e-12
+    vx, vy, p = oracle.stokes_unpack(pr, Sg)
+    ex, ey, ep = oracle.stokes_unpack(pr, Sd)
+    v = np.concatenate([vx.ravel(), vy.ravel()])
+    ev = np.concatenate([ex.ravel(), ey.ravel()])
+    assert np.linalg.norm(v - ev) <= 1e-8 * np.linalg.norm(ev)
+    assert np.linalg.norm(p - ep) <= 1e-8 * np.linalg.norm(ep)
+
+
+@pytest.mark.parametrize("precision", [1, 0])
+def test_falling_block_time_to_tolerance(gpu_ctx, precision):
+    """The 101 x 101 falling block (1e6 viscosity contrast): FGMRES reaches
+    1e-8 relative residual; the forward error of that solve is reported."""
+    pr = workloads.falling_block(101, 101)
+    Li, Lj, Lv, R = oracle.stokes_build(pr)
+    sol = StokesSolverGPU(gpu_ctx)
+    sol.set_system(pr)
+    S, st = sol.solve_system(precision=precision, rtol=1e-8)
+    print(f"falling block precision={precision}: {st}", flush=True)
+    assert st["converged"] and st["relres"] <= 1e-8
+    Sg = to_host(S)
+    rel = np.linalg.norm(R - oracle.coo_spmv(pr.N, Li, Lj, Lv, Sg)) / np.linalg.norm(R)
+    assert rel <= 1.0001e-8
+
+
+def test_reference_multigrid_bit_exact(gpu_ctx):
+    """The device multigrid reproduces the oracle restatement of the
+    reference's V-cycle bit for bit in FP64: coarse viscosities, two V-cycles
+    and the final residuals."""
+    pr = workloads.falling_block(45, 37, size=1.0e5, eta_block=1e23)
+    g = pr.grid
+    H = omg.Hierarchy(g.xnum, g.ynum, g.xsize, g.ysize)
+    f = H.levels[0]
+    f.etas[...] = pr.etas0
+    f.etan[...] = pr.etan0
+    f.RX[...] = pr.RX1
+    f.RY[...] = pr.RY1
+    f.RC[...] = pr.RC1
+    H.restrict_viscosity()
+    M = MultigridGPU(gpu_ctx, g.xnum, g.ynum, g.xsize, g.ysize)
+    assert M.nlevels == len(H.levels)
+    for name in ("etas", "etan", "RX", "RY", "RC"):
+        M.set(0, name, to_device(getattr(f, name)))
+    M.restrict_viscosity()
+    gpu_ctx.synchronize()
+    for k, lev in enumerate(H.levels):
+        assert np.array_equal(to_host(M.get(k, "etas")), lev.etas), k
+        assert np.array_equal(to_host(M.get(k, "etan")), lev.etan), k
+    sm = omg.smoothing_counts(len(H.levels), 2, cap=48)
+    for _ in range(2):
+        H.vcycle(sm, 0.9, 0.3)
+    M.vcycles(2, finest_sweeps=2, multiplier=2, cap=48, theta_s=0.9, theta_c=0.3)
+    gpu_ctx.synchronize()
+    for name in ("vx", "vy", "pr", "rx", "ry", "rc"):
+        assert np.array_equal(to_host(M.get(0, name)), getattr(f, name)), name
+    for k in range(1, len(H.levels)):
+        for name in ("vx", "vy", "pr", "RX", "RY", "RC"):
+            assert np.array_equal(to_host(M.get(k, name)), getattr(H.levels[k], name)), (k, name)
