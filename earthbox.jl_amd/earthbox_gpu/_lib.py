@@ -37,6 +37,7 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_marker_cell_indices": (c_int, [c_void_p, c_int64] + [c_void_p] * 5),
     "ebg_rk4_advect": (c_int, [c_void_p, c_int64, c_int, c_double] + [c_void_p] * 13 + [c_int]),
     "ebg_calculate_spin": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_interpolate_marker_stokes_parameters_to_grid": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_stokes_set_system": (c_int, [c_void_p] + [c_void_p] * 16 + [c_int, c_double, c_double,
                                                                      c_double, c_double, c_int]),
     "ebg_stokes_apply": (c_int, [c_void_p, c_void_p, c_void_p]),
@@ -53,6 +54,26 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_mg2d_vcycles": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
                                  c_double, c_double, c_double, c_void_p]),
 }
+
+
+class MarkerStokesProps(C.Structure):
+    """ebg_marker_stokes_props (include/earthbox_gpu.h)."""
+    _fields_ = [("marknum", c_int64), ("x", c_void_p), ("y", c_void_p), ("inside_flags", c_void_p),
+                ("rho", c_void_p), ("cohesion", c_void_p), ("fric", c_void_p), ("eta_flow", c_void_p),
+                ("eta", c_void_p), ("sxy", c_void_p), ("sxx", c_void_p), ("meltfrac", c_void_p),
+                ("pfailure", c_void_p), ("dilatation", c_void_p), ("matid", c_void_p),
+                ("mat_mu", c_void_p), ("nmat", c_int64)]
+
+
+GRID_FIELD_NAMES = ("rho1", "rho1_vy", "etas1", "etan1", "eta_flow", "mus1", "mun1", "sxy1", "sxx1",
+                    "plastics", "plasticn", "cohesion_grid", "fric_degrees_grid", "dilatation_grid",
+                    "extractable_meltfrac_grid", "wtnodes", "wtetas", "wtetan", "wtnodes_vy",
+                    "marker_xn", "marker_yn")
+
+
+class StokesGridFields(C.Structure):
+    """ebg_stokes_grid_fields (include/earthbox_gpu.h)."""
+    _fields_ = [(k, c_void_p) for k in GRID_FIELD_NAMES]
 
 
 class SolveOpts(C.Structure):

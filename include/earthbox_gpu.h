@@ -107,6 +107,49 @@ int ebg_rk4_advect(ebg_ctx *ctx, int64_t marknum, int order, double timestep,
  * (stokes_continuity_solver/utils/strain_rate/GridStrainRate.jl:161-182). */
 int ebg_calculate_spin(ebg_ctx *ctx, const double *vx1, const double *vy1, double *esp);
 
+/* ---- marker -> grid interpolation of the Stokes transport properties ----
+ * Per-marker DEVICE arrays (length marknum); any property pointer may be NULL
+ * (its contribution is then 0).  matid (Int16, 1-based) and mat_mu (DEVICE,
+ * nmat shear moduli, the reference's materials.arrays.mat_mu) go together. */
+typedef struct {
+    int64_t marknum;
+    const double *x, *y;
+    const int8_t *inside_flags;      /* NULL: computed from the grid limits */
+    const double *rho, *cohesion, *fric, *eta_flow, *eta, *sxy, *sxx, *meltfrac;
+    const float *pfailure, *dilatation;  /* Float32 as in the reference */
+    const int16_t *matid;
+    const double *mat_mu;
+    int64_t nmat;
+} ebg_marker_stokes_props;
+
+/* DEVICE grid arrays, in/out: on entry the previous values (the backup used
+ * where a node receives no marker weight, BackupTransportArrays.jl:33-104),
+ * on exit the new averages.  Any may be NULL (not written).  Shapes: basic
+ * (ynum,xnum): rho1 etas1 eta_flow mus1 sxy1 plastics cohesion_grid
+ * fric_degrees_grid; pressure (ynum-1,xnum-1): etan1 mun1 sxx1 plasticn
+ * dilatation_grid extractable_meltfrac_grid; vy (ynum,xnum+1): rho1_vy.
+ * Weight sums (optional outputs): wtnodes, wtetas (ynum,xnum), wtetan
+ * (ynum-1,xnum-1), wtnodes_vy (ynum,xnum+1).  marker_xn/yn (optional) receive
+ * the bit-exact 1-based cell indices of inside markers. */
+typedef struct {
+    double *rho1, *rho1_vy, *etas1, *etan1, *eta_flow, *mus1, *mun1, *sxy1, *sxx1;
+    double *plastics, *plasticn, *cohesion_grid, *fric_degrees_grid, *dilatation_grid;
+    double *extractable_meltfrac_grid;
+    double *wtnodes, *wtetas, *wtetan, *wtnodes_vy;
+    int32_t *marker_xn, *marker_yn;
+} ebg_stokes_grid_fields;
+
+/* Replaces Interpolation.initialize_bilinear_interpolation! +
+ * interpolate_marker_stokes_parameters_to_grid! (interpolation/Interpolation.jl:61-95):
+ * MarkerGridMapping.jl:99-265, MarkerGridNodeWeights.jl:68-150,
+ * BilinearDenominator.jl:34-182, BilinearNumerator.jl:205-454, WeightFuncs.jl:67-190,
+ * BilinearAverage.jl:57-178.  Every node sum is accumulated in the reference's
+ * serial marker order (no atomics): results equal the serial loops bit for bit
+ * (the friction angle's asin aside). */
+int ebg_interpolate_marker_stokes_parameters_to_grid(ebg_ctx *ctx,
+                                                     const ebg_marker_stokes_props *props,
+                                                     ebg_stokes_grid_fields *fields);
+
 /* ---- Stokes-continuity system (assembled-operator conventions) ----
  * Replaces the inputs StokesBuildData gathers (solver/build_system/utils/
  * BuildStructs.jl:37-177).  Fields are DEVICE pointers with the reference

@@ -221,3 +221,66 @@ def direct_solve(N, Li, Lj, Lv, R, refine: int = 3):
         r = R - A @ S
         S = S + lu.solve(r)
     return S
+
+
+# ---------------------------------------------------------------- scatter
+class _ScatterIn(C.Structure):
+    _fields_ = [("marknum", _I64), ("xnum", _I64), ("ynum", _I64), ("x", _P), ("y", _P),
+                ("flags", _P), ("gridx_b", _P), ("gridy_b", _P), ("xstp_b", _P), ("ystp_b", _P),
+                ("gridx_vy", _P), ("xstp_vy", _P), ("rho", _P), ("cohesion", _P), ("fric", _P),
+                ("eta_flow", _P), ("eta", _P), ("sxy", _P), ("sxx", _P), ("meltfrac", _P),
+                ("pfailure", _P), ("dilatation", _P), ("matid", _P), ("mat_mu", _P)]
+
+
+_SC_OUT = ("rho1", "rho1_vy", "etas1", "etan1", "eta_flow", "mus1", "mun1", "sxy1", "sxx1",
+           "plastics", "plasticn", "cohesion", "fric_deg", "dilatation", "meltfrac",
+           "wtnodes", "wtetas", "wtetan", "wtnodes_vy", "xn", "yn")
+
+
+class _ScatterOut(C.Structure):
+    _fields_ = [(k, _P) for k in _SC_OUT]
+
+
+def scatter_stokes(g, m: dict, prev: dict):
+    """Serial restatement of the Stokes scatter.  m: marker arrays (numpy);
+    prev: previous grid values keyed like the GPU field names.  Returns the
+    new fields (same keys) plus weights and indices."""
+    L = lib()
+    L.ebo_scatter_stokes.restype = C.c_int
+    L.ebo_scatter_stokes.argtypes = [C.POINTER(_ScatterIn), C.POINTER(_ScatterOut)]
+    n = m["x"].size
+    flags = m.get("inside_flags")
+    if flags is None:
+        flags = inside_flags(m["x"], m["y"], g.limits)
+    zeros = {k: np.zeros(n) for k in ("rho", "cohesion", "fric", "eta_flow", "eta", "sxy", "sxx", "meltfrac")}
+    arr = {k: np.ascontiguousarray(m.get(k, zeros.get(k)), dtype=np.float64) for k in zeros}
+    pf = np.ascontiguousarray(m.get("pfailure", np.zeros(n, np.float32)), dtype=np.float32)
+    dl = np.ascontiguousarray(m.get("dilatation", np.zeros(n, np.float32)), dtype=np.float32)
+    mid = np.ascontiguousarray(m.get("matid", np.ones(n, np.int16)), dtype=np.int16)
+    mmu = np.ascontiguousarray(m.get("mat_mu", np.ones(1)), dtype=np.float64)
+    sin = _ScatterIn(n, g.xnum, g.ynum, p(m["x"]), p(m["y"]), p(flags), p(g.gridx_b), p(g.gridy_b),
+                     p(g.xstp_b), p(g.ystp_b), p(g.gridx_vy), p(g.xstp_vy),
+                     *[p(arr[k]) for k in ("rho", "cohesion", "fric", "eta_flow", "eta", "sxy", "sxx",
+                                           "meltfrac")], p(pf), p(dl), p(mid), p(mmu))
+    names = {"rho1": "rho1", "rho1_vy": "rho1_vy", "etas1": "etas1", "etan1": "etan1",
+             "eta_flow": "eta_flow", "mus1": "mus1", "mun1": "mun1", "sxy1": "sxy1", "sxx1": "sxx1",
+             "plastics": "plastics", "plasticn": "plasticn", "cohesion": "cohesion_grid",
+             "fric_deg": "fric_degrees_grid", "dilatation": "dilatation_grid",
+             "meltfrac": "extractable_meltfrac_grid", "wtnodes": "wtnodes", "wtetas": "wtetas",
+             "wtetan": "wtetan", "wtnodes_vy": "wtnodes_vy"}
+    out = {}
+    for ok, gk in names.items():
+        a = prev.get(gk)
+        out[gk] = np.asfortranarray(a.copy()) if a is not None else None
+    ny, nx = g.ynum, g.xnum
+    shapes = {"wtnodes": (ny, nx), "wtetas": (ny, nx), "wtetan": (ny - 1, nx - 1), "wtnodes_vy": (ny, nx + 1)}
+    for k, s in shapes.items():
+        out[k] = np.zeros(s, order="F")
+    xn = np.zeros(n, np.int32)
+    yn = np.zeros(n, np.int32)
+    so = _ScatterOut(*[p(out[names[k]]) for k in _SC_OUT[:19]], p(xn), p(yn))
+    err = L.ebo_scatter_stokes(C.byref(sin), C.byref(so))
+    if err:
+        raise ValueError("zero shear modulus")
+    out["marker_xn"], out["marker_yn"] = xn, yn
+    return out
