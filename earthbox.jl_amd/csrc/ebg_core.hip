@@ -92,17 +92,14 @@ int ebg_ctx_destroy(ebg_ctx* ctx)
 int ebg_ctx_set_stream(ebg_ctx* ctx, void* hip_stream)
 {
     EBG_CHECK(ctx != nullptr, EBG_ERR_ARG, "ebg_ctx_set_stream: ctx is NULL");
-    if (hip_stream) {
-        if (ctx->own_stream && ctx->stream) {
-            EBG_HIP(hipStreamSynchronize(ctx->stream));
-            EBG_HIP(hipStreamDestroy(ctx->stream));
-        }
-        ctx->stream = (hipStream_t)hip_stream;
-        ctx->own_stream = false;
-    } else if (!ctx->own_stream) {
-        EBG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-        ctx->own_stream = true;
+    // NULL selects the default (null) stream, the one an external runtime such
+    // as torch uses unless told otherwise
+    if (ctx->own_stream && ctx->stream) {
+        EBG_HIP(hipStreamSynchronize(ctx->stream));
+        EBG_HIP(hipStreamDestroy(ctx->stream));
     }
+    ctx->stream = (hipStream_t)hip_stream;
+    ctx->own_stream = false;
     return EBG_OK;
 }
 

@@ -276,6 +276,7 @@ template <typename T>
 void mg_smoother(MGHierarchy<T>& H, int k, int nsweeps, T theta_s, T theta_c, T prnorm,
                  const T* prnorm_dev, const T* bc_d, hipStream_t s)
 {
+    const bool normalize = (k > 0) || H.normalize_finest;
     const MGLevelDev<T>& l = H.lev[k];
     const int nbc = mg_bc_threads(l.xnum, l.ynum);
     const int64_t nvx = (int64_t)(l.ynum - 1) * (l.xnum - 2);
@@ -293,10 +294,12 @@ void mg_smoother(MGHierarchy<T>& H, int k, int nsweeps, T theta_s, T theta_c, T 
                                theta_s, theta_c);
         }
     }
-    hipLaunchKernelGGL(mg_pressure_shift_kernel<T>, dim3(1), dim3(64), 0, s, l.pr, prnorm_dev, prnorm,
-                       H.scratch);
-    hipLaunchKernelGGL(mg_pressure_add_kernel<T>, dim3(nblk(np)), dim3(256), 0, s, l.pr, (int)np,
-                       H.scratch);
+    if (normalize) {
+        hipLaunchKernelGGL(mg_pressure_shift_kernel<T>, dim3(1), dim3(64), 0, s, l.pr, prnorm_dev, prnorm,
+                           H.scratch);
+        hipLaunchKernelGGL(mg_pressure_add_kernel<T>, dim3(nblk(np)), dim3(256), 0, s, l.pr, (int)np,
+                           H.scratch);
+    }
     hipLaunchKernelGGL(mg_residual_kernel<T>, dim3(nblk((int64_t)(l.ynum + 1) * (l.xnum + 1))),
                        dim3(256), 0, s, l);
 }

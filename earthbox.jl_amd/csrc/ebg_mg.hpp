@@ -30,6 +30,9 @@ struct MGLevelDev {
     T *vx, *vy, *pr;
     T *RX, *RY, *RC;
     T *rx, *ry, *rc;
+    // optional fixed-node masks (finest level of the Stokes preconditioner):
+    // nodes whose correction is known are neither smoothed nor corrected
+    const int8_t *fix_vx = nullptr, *fix_vy = nullptr, *fix_p = nullptr;
 };
 
 // Fine->coarse transfer description for one axis of one grid type: for each
@@ -170,6 +173,7 @@ __global__ void __launch_bounds__(256) mg_sweep_kernel(MGLevelDev<T> l, int colo
         if (t >= ni * (l.xnum - 2)) return;
         const int i = 2 + t % ni, j = 2 + t / ni;
         if (((i + j) & 1) != color) return;
+        if (l.fix_vx && l.fix_vx[(size_t)(j - 1) * (l.ynum + 1) + (i - 1)]) return;
         T coef;
         const T dR = mg_vx_residual(l, i, j, &coef);
         MG_VX(l, i, j) += dR / coef * theta_s;
@@ -178,6 +182,7 @@ __global__ void __launch_bounds__(256) mg_sweep_kernel(MGLevelDev<T> l, int colo
         if (t >= ni * (l.xnum - 1)) return;
         const int i = 2 + t % ni, j = 2 + t / ni;
         if (((i + j) & 1) != color) return;
+        if (l.fix_vy && l.fix_vy[(size_t)(j - 1) * l.ynum + (i - 1)]) return;
         T coef;
         const T dR = mg_vy_residual(l, i, j, &coef);
         MG_VY(l, i, j) += dR / coef * theta_s;
@@ -186,6 +191,7 @@ __global__ void __launch_bounds__(256) mg_sweep_kernel(MGLevelDev<T> l, int colo
         if (t >= ni * (l.xnum - 1)) return;
         const int i = 1 + t % ni, j = 1 + t / ni;
         if (((i + j) & 1) != color) return;
+        if (l.fix_p && l.fix_p[(size_t)(j - 1) * (l.ynum - 1) + (i - 1)]) return;
         const T dR = mg_pr_residual(l, i, j);
         MG_PR(l, i, j) += MG_EN(l, i, j) * dR * theta_c;
     }
@@ -217,15 +223,19 @@ __global__ void __launch_bounds__(256) mg_residual_kernel(MGLevelDev<T> l)
     if (j <= xnum) {
         T r = T(0);
         if (i >= 2 && i <= ynum && j >= 2 && j <= xnum - 1) r = mg_vx_residual(l, i, j, (T*)nullptr);
+        if (l.fix_vx && l.fix_vx[(size_t)(j - 1) * (ynum + 1) + (i - 1)]) r = T(0);
         l.rx[(size_t)(j - 1) * (ynum + 1) + (i - 1)] = r;
     }
     if (i <= ynum) {
         T r = T(0);
         if (i >= 2 && i <= ynum - 1 && j >= 2 && j <= xnum) r = mg_vy_residual(l, i, j, (T*)nullptr);
+        if (l.fix_vy && l.fix_vy[(size_t)(j - 1) * ynum + (i - 1)]) r = T(0);
         l.ry[(size_t)(j - 1) * ynum + (i - 1)] = r;
     }
-    if (i <= ynum - 1 && j <= xnum - 1)
-        l.rc[(size_t)(j - 1) * (ynum - 1) + (i - 1)] = mg_pr_residual(l, i, j);
+    if (i <= ynum - 1 && j <= xnum - 1) {
+        const size_t k = (size_t)(j - 1) * (ynum - 1) + (i - 1);
+        l.rc[k] = (l.fix_p && l.fix_p[k]) ? T(0) : mg_pr_residual(l, i, j);
+    }
 }
 
 // Gather restriction of one grid type.  val(i,j) is read from fine array f
@@ -333,17 +343,17 @@ __global__ void __launch_bounds__(256) mg_prolong_kernel(MGLevelDev<T> c, MGLeve
     if (j < xf + 1) {
         const T dv = mg_c2f(c.vx, c.ynum + 1, m.vx_y.idx[i - 1], m.vx_x.idx[j - 1],
                             m.vx_y.d[i - 1], m.vx_x.d[j - 1]);
-        MG_VX(f, i, j) += dv * relax_v;
+        if (!(f.fix_vx && f.fix_vx[(size_t)(j - 1) * (f.ynum + 1) + (i - 1)])) MG_VX(f, i, j) += dv * relax_v;
     }
     if (i < yf + 1) {
         const T dv = mg_c2f(c.vy, c.ynum, m.vy_y.idx[i - 1], m.vy_x.idx[j - 1],
                             m.vy_y.d[i - 1], m.vy_x.d[j - 1]);
-        MG_VY(f, i, j) += dv * relax_v;
+        if (!(f.fix_vy && f.fix_vy[(size_t)(j - 1) * f.ynum + (i - 1)])) MG_VY(f, i, j) += dv * relax_v;
     }
     if (i < yf && j < xf) {
         const T dp = mg_c2f(c.pr, c.ynum - 1, m.pr_y.idx[i - 1], m.pr_x.idx[j - 1],
                             m.pr_y.d[i - 1], m.pr_x.d[j - 1]);
-        MG_PR(f, i, j) += dp * relax_p;
+        if (!(f.fix_p && f.fix_p[(size_t)(j - 1) * (f.ynum - 1) + (i - 1)])) MG_PR(f, i, j) += dp * relax_p;
     }
 }
 
@@ -378,6 +388,7 @@ struct MGHierarchy {
     std::vector<TransferDev<T>> xfer;   // xfer[k]: level k -> k+1 maps
     std::vector<void*> allocations;
     T* scratch = nullptr;               // pressure-shift scalar
+    bool normalize_finest = true;       // pressure gauge p[1,1] = prnorm on level 0
     int nlev() const { return (int)lev.size(); }
     ~MGHierarchy();
 };

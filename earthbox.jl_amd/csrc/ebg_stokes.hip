@@ -267,6 +267,24 @@ __global__ void unpack_corr_kernel(StokesOpDev s, const double* __restrict__ r, 
         z[2 * (int64_t)nc + c] = r[2 * (int64_t)nc + c] / kb;
 }
 
+// finest-level nodes whose correction is known exactly (identity rows):
+// internal velocity zones and the pressure rows of BC modes 1 and 2
+template <typename T>
+__global__ void set_fixed_kernel(StokesOpDev s, const double* __restrict__ r, MGLevelDev<T> l, int nc)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int ny1 = s.ynum - 1;
+    const int i = c % ny1 + 1, j = c / ny1 + 1;
+    const double kb = kbond(s);
+    if (l.fix_vx && j < s.xnum - 1 && !out_zone_x(s, i, j))
+        l.vx[(size_t)j * (s.ynum + 1) + i] = (T)(r[c] / kb);
+    if (l.fix_vy && i < s.ynum - 1 && !out_zone_y(s, i, j))
+        l.vy[(size_t)j * s.ynum + i] = (T)(r[nc + c] / kb);
+    if (l.fix_p && !internal_continuity(s, i, j))
+        l.pr[c] = (T)(s.pscale * (r[2 * (int64_t)nc + c] / kb));
+}
+
 template <typename T>
 __global__ void to_precision_kernel(const double* __restrict__ a, T* __restrict__ b, int64_t n)
 {
@@ -302,6 +320,8 @@ struct StokesState {
     std::unique_ptr<MGHierarchy<float>> Hf;
     std::unique_ptr<MGHierarchy<double>> Hd;
     DevBuf prnorm_f, prnorm_d;
+    DevBuf fix_vx, fix_vy, fix_p;
+    bool any_fix_v = false, any_fix_p = false;
     int mg_key_nlev = -1;
     bool visc_dirty = true;
     LearnedCorrection* learned = nullptr;
@@ -326,6 +346,8 @@ static void precondition(StokesState& st, MGHierarchy<T>& H, T* prn, const ebg_s
     const int nvx = (st.ynum + 1) * (st.xnum + 1);
     hipLaunchKernelGGL(pack_rhs_kernel<T>, dim3(nb(nvx)), dim3(256), 0, s, st.op, r, l0, prn, st.nc);
     mg_zero_solution(H, 0, s);
+    if (l0.fix_vx || l0.fix_p)
+        hipLaunchKernelGGL(set_fixed_kernel<T>, dim3(nb(st.nc)), dim3(256), 0, s, st.op, r, l0, st.nc);
     const T bcd[4] = {(T)o.bc_d[0], (T)o.bc_d[1], (T)o.bc_d[2], (T)o.bc_d[3]};
     mg_vcycle(H, sm, (T)o.theta_s, (T)o.theta_c, T(1), T(1), T(0), (const T*)prn, bcd, s);
     hipLaunchKernelGGL(unpack_corr_kernel<T>, dim3(nb(st.nc)), dim3(256), 0, s, st.op, r, l0, z, st.nc);
@@ -370,6 +392,47 @@ static int ensure_hierarchy(ebg_ctx* ctx, StokesState& st, std::unique_ptr<MGHie
     return EBG_OK;
 }
 
+// Fixed-node masks of the finest level (host-built from the zone indices and
+// the pressure BC mode) attached to the hierarchy's level 0.
+static int build_masks(StokesState& st)
+{
+    const StokesOpDev& op = st.op;
+    const int nx = st.xnum, ny = st.ynum;
+    std::vector<int8_t> mx((size_t)(ny + 1) * nx, 0), my((size_t)ny * (nx + 1), 0), mp((size_t)(ny - 1) * (nx - 1), 0);
+    bool fv = false, fp = false;
+    for (int j = 1; j <= nx - 1; ++j)
+        for (int i = 1; i <= ny - 1; ++i) {
+            const bool zx = !(j != op.zx_j || i < op.zx_i0 || i > op.zx_i1);
+            const bool zy = !(j != op.zy_j || i < op.zy_i0 || i > op.zy_i1);
+            if (zx && j < nx - 1) { mx[(size_t)j * (ny + 1) + i] = 1; fv = true; }
+            if (zy && i < ny - 1) { my[(size_t)j * ny + i] = 1; fv = true; }
+            bool internal;
+            if (op.pressure_bc_mode == 1) internal = (1 < i && i < ny - 1);
+            else if (op.pressure_bc_mode == 2) internal = (1 < j && j < nx - 1);
+            else internal = true;  // mode 0 keeps the gauge through normalisation
+            if (!internal) { mp[(size_t)(j - 1) * (ny - 1) + (i - 1)] = 1; fp = true; }
+        }
+    int rc;
+    if ((rc = st.fix_vx.ensure(mx.size())) || (rc = st.fix_vy.ensure(my.size())) || (rc = st.fix_p.ensure(mp.size())))
+        return rc;
+    EBG_HIP(hipMemcpy(st.fix_vx.p, mx.data(), mx.size(), hipMemcpyHostToDevice));
+    EBG_HIP(hipMemcpy(st.fix_vy.p, my.data(), my.size(), hipMemcpyHostToDevice));
+    EBG_HIP(hipMemcpy(st.fix_p.p, mp.data(), mp.size(), hipMemcpyHostToDevice));
+    st.any_fix_v = fv;
+    st.any_fix_p = fp;
+    return EBG_OK;
+}
+
+template <typename T>
+static void attach_masks(StokesState& st, MGHierarchy<T>& H)
+{
+    MGLevelDev<T>& l0 = H.lev[0];
+    l0.fix_vx = st.any_fix_v ? st.fix_vx.as<int8_t>() : nullptr;
+    l0.fix_vy = st.any_fix_v ? st.fix_vy.as<int8_t>() : nullptr;
+    l0.fix_p = st.any_fix_p ? st.fix_p.as<int8_t>() : nullptr;
+    H.normalize_finest = (st.op.pressure_bc_mode == 0);
+}
+
 template <typename T>
 static void load_viscosity(StokesState& st, MGHierarchy<T>& H, hipStream_t s)
 {
@@ -408,10 +471,12 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
         if ((rc = ensure_hierarchy(ctx, st, st.Hf, st.prnorm_f, o))) return rc;
         sm = mg_smoothing_counts(st.Hf->nlev(), o.finest_sweeps, o.sweep_multiplier, o.sweep_cap);
         if (st.visc_dirty) load_viscosity(st, *st.Hf, s);
+        attach_masks(st, *st.Hf);
     } else {
         if ((rc = ensure_hierarchy(ctx, st, st.Hd, st.prnorm_d, o))) return rc;
         sm = mg_smoothing_counts(st.Hd->nlev(), o.finest_sweeps, o.sweep_multiplier, o.sweep_cap);
         if (st.visc_dirty) load_viscosity(st, *st.Hd, s);
+        attach_masks(st, *st.Hd);
     }
     st.visc_dirty = false;
     LayoutPlanar LP{nc};
@@ -638,7 +703,7 @@ int ebg_stokes_set_system(ebg_ctx* ctx, const double* etas0, const double* etan0
     op.gravity_y = gravity_y;
     op.stab = iuse_interface_stabilization;
     st.visc_dirty = true;
-    return EBG_OK;
+    return build_masks(st);
 }
 
 int ebg_stokes_apply(ebg_ctx* ctx, const double* S, double* LS)

@@ -31,9 +31,11 @@ class GPUContext:
         self._h = handle
         self.device = torch.device("cuda", device_index)
         self.grid: StaggeredGrid2D | None = None
-        # Run on torch's current stream so tensor ops and kernels are ordered.
+        # Run on torch's current stream (the null stream unless the caller
+        # selected another) so tensor copies and library kernels are ordered.
         stream = torch.cuda.current_stream(self.device).cuda_stream
-        check(lib.ebg_ctx_set_stream(self._h, C.c_void_p(stream)), "ebg_ctx_set_stream")
+        check(lib.ebg_ctx_set_stream(self._h, C.c_void_p(stream) if stream else None),
+              "ebg_ctx_set_stream")
 
     @property
     def handle(self) -> C.c_void_p:

@@ -41,8 +41,9 @@ enum {
 /* ---- context, memory, errors (no reference counterpart: plumbing) ---- */
 int ebg_ctx_create(ebg_ctx **out, int device);
 int ebg_ctx_destroy(ebg_ctx *ctx);
-/* Use an external hipStream_t (e.g. the caller's current stream); NULL restores
- * the context's own stream. */
+/* Enqueue all later work on an external hipStream_t (e.g. the caller's current
+ * stream); NULL selects the default (null) stream.  The context's private
+ * non-blocking stream, created by ebg_ctx_create, is released. */
 int ebg_ctx_set_stream(ebg_ctx *ctx, void *hip_stream);
 int ebg_ctx_synchronize(ebg_ctx *ctx);
 const char *ebg_last_error(void);
