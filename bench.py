@@ -35,7 +35,63 @@ def parse():
     ap.add_argument("--xnum", type=int, default=1001)
     ap.add_argument("--ynum", type=int, default=501)
     ap.add_argument("--markers-per-cell", type=int, default=6)
+    ap.add_argument("--no-stokes", action="store_true")
+    ap.add_argument("--no-scatter", action="store_true")
+    ap.add_argument("--stokes-max-iter", type=int, default=3000)
     return ap.parse_args()
+
+
+def bench_stokes(ctx, nx, ny, max_iter):
+    """Time-to-1e-8 of the config-2 style system (layered random log-viscosity
+    18-25, 1500 x 600 km) for the plain FP64 V-cycle preconditioner and the
+    FP32 mixed-precision one."""
+    import torch
+    from earthbox_gpu import workloads
+    from earthbox_gpu.stokes import StokesSolverGPU, bc_mirror_factors
+    prob = workloads.layered_random(nx, ny, 1.5e6 * (nx - 1) / 1000, 6.0e5 * (ny - 1) / 500, seed=2)
+    sol = StokesSolverGPU(ctx)
+    sol.set_system(prob)
+    bcd = bc_mirror_factors(prob)
+    out = {"system": f"layered random log10(eta) in [18,25], {nx}x{ny} nodes, N={prob.N}",
+           "rtol": 1e-8}
+    for name, prec in (("plain_fp64_mg", 0), ("mixed_fp32_mg", 1)):
+        sol.solve_system(precision=prec, rtol=1e-8, max_iterations=3, bc_d=bcd)  # build + warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        S, st = sol.solve_system(precision=prec, rtol=1e-8, max_iterations=max_iter, bc_d=bcd)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[name] = {"seconds": el, "iterations": st["iterations"], "relres": st["relres"],
+                     "converged": st["converged"]}
+    p, m = out["plain_fp64_mg"], out["mixed_fp32_mg"]
+    if p["converged"] and m["converged"]:
+        out["speedup_mixed_vs_plain"] = p["seconds"] / m["seconds"]
+    return out
+
+
+def bench_scatter(ctx, g, x, y, reps=3):
+    """One marker->grid Stokes scatter of all markers (material-table shear
+    modulus, per-marker density and viscosity)."""
+    import numpy as np
+    import torch
+    from earthbox_gpu.device import to_device
+    from earthbox_gpu.scatter import interpolate_marker_stokes_parameters_to_grid, new_grid_fields
+    n = x.size
+    rng = np.random.default_rng(5)
+    m = {"x": to_device(x), "y": to_device(y),
+         "rho": to_device(3000 + 300 * rng.random(n)),
+         "eta": to_device(10 ** rng.uniform(19, 23, n)),
+         "matid": to_device(rng.integers(1, 4, n).astype(np.int16)),
+         "mat_mu": to_device(np.array([3e10, 6e10, 1e10]))}
+    f = new_grid_fields(g.xnum, g.ynum, 1.0)
+    interpolate_marker_stokes_parameters_to_grid(ctx, m, f)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        interpolate_marker_stokes_parameters_to_grid(ctx, m, f)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / reps
+    return {"ms": el * 1e3, "Mmarkers_per_s": n / el / 1e6, "markers": n}
 
 
 def cpu_baseline(g, x, y, vx1, vy1, esp, dt, sample_markers: int):
@@ -129,6 +185,10 @@ def main():
                      "kernel": "rk4_advect_kernel", "kernel_ms": kern_ms,
                      "bytes_per_unit": RK4_BYTES_PER_MARKER},
     }
+    if not args.no_stokes:
+        out["stokes"] = bench_stokes(ctx, nx, ny, args.stokes_max_iter)
+    if not args.no_scatter:
+        out["scatter"] = bench_scatter(ctx, g, x, y)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, x, y, vx1, vy1, to_host_np(desp), dt, 2_000_000)
     print(json.dumps(out), flush=True)
