@@ -98,7 +98,7 @@ struct HostAxisMap {
 // [lo, hi] within [rlo, rhi] that feeds each coarse node.
 template <typename T>
 static HostAxisMap<T> axis_map(const std::vector<double>& ga, const std::vector<double>& gb,
-                               int rlo, int rhi)
+                               int rlo, int rhi, bool clamp)
 {
     HostAxisMap<T> m;
     const int na = (int)ga.size(), nb = (int)gb.size();
@@ -109,7 +109,9 @@ static HostAxisMap<T> axis_map(const std::vector<double>& ga, const std::vector<
         const double x0 = gb[ib - 1];
         const double w = gb[ib] - x0;
         m.idx[k] = ib;
-        m.d[k] = (T)((ga[k] - x0) / w);
+        double d = (ga[k] - x0) / w;
+        if (clamp) d = std::min(1.0, std::max(0.0, d));
+        m.d[k] = (T)d;
     }
     m.lo.assign(nb, 1);
     m.hi.assign(nb, 0);
@@ -186,7 +188,7 @@ MGHierarchy<T>::~MGHierarchy()
 
 template <typename T>
 int mg_build(MGHierarchy<T>& H, int xnum, int ynum, double xsize, double ysize, int nlevels,
-             const std::vector<HostLevelGrid>* override_grids)
+             const std::vector<HostLevelGrid>* override_grids, bool cw)
 {
     auto sizes = mg_level_sizes(xnum, ynum, 10);
     if (nlevels > 0 && nlevels < (int)sizes.size()) sizes.resize(nlevels);
@@ -227,17 +229,17 @@ int mg_build(MGHierarchy<T>& H, int xnum, int ynum, double xsize, double ysize, 
         const int yf = f.ynum, xf = f.xnum;
         TransferDev<T>& t = H.xfer[k];
         // residual restriction loop ranges, Restriction.jl:444-463
-        if ((rc = upload_axis(H, axis_map<T>(f.gridy_vx, c.gridy_vx, 2, yf), t.vx_y))) return rc;
-        if ((rc = upload_axis(H, axis_map<T>(f.gridx_b, c.gridx_b, 2, xf - 1), t.vx_x))) return rc;
-        if ((rc = upload_axis(H, axis_map<T>(f.gridy_b, c.gridy_b, 2, yf - 1), t.vy_y))) return rc;
-        if ((rc = upload_axis(H, axis_map<T>(f.gridx_vy, c.gridx_vy, 2, xf), t.vy_x))) return rc;
-        if ((rc = upload_axis(H, axis_map<T>(f.gridy_pr, c.gridy_pr, 2, yf - 1), t.pr_y))) return rc;
-        if ((rc = upload_axis(H, axis_map<T>(f.gridx_pr, c.gridx_pr, 2, xf - 1), t.pr_x))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridy_vx, c.gridy_vx, 2, yf, cw), t.vx_y))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridx_b, c.gridx_b, 2, xf - 1, cw), t.vx_x))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridy_b, c.gridy_b, 2, yf - 1, cw), t.vy_y))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridx_vy, c.gridx_vy, 2, xf, cw), t.vy_x))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridy_pr, c.gridy_pr, 2, yf - 1, cw), t.pr_y))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridx_pr, c.gridx_pr, 2, xf - 1, cw), t.pr_x))) return rc;
         // viscosity restriction ranges, ViscosityRestriction.jl:202-215
-        if ((rc = upload_axis(H, axis_map<T>(f.gridy_b, c.gridy_b, 1, yf), t.b_y))) return rc;
-        if ((rc = upload_axis(H, axis_map<T>(f.gridx_b, c.gridx_b, 1, xf), t.b_x))) return rc;
-        if ((rc = upload_axis(H, axis_map<T>(f.gridy_pr, c.gridy_pr, 1, yf - 1), t.e_y))) return rc;
-        if ((rc = upload_axis(H, axis_map<T>(f.gridx_pr, c.gridx_pr, 1, xf - 1), t.e_x))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridy_b, c.gridy_b, 1, yf, cw), t.b_y))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridx_b, c.gridx_b, 1, xf, cw), t.b_x))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridy_pr, c.gridy_pr, 1, yf - 1, cw), t.e_y))) return rc;
+        if ((rc = upload_axis(H, axis_map<T>(f.gridx_pr, c.gridx_pr, 1, xf - 1, cw), t.e_x))) return rc;
     }
     if ((rc = dev_zeros(H, 4, &H.scratch))) return rc;
     return EBG_OK;
@@ -342,7 +344,7 @@ void mg_vcycle(MGHierarchy<T>& H, const std::vector<int>& sm, T theta_s, T theta
 #define EBG_INST(T)                                                                              \
     template struct MGHierarchy<T>;                                                              \
     template int mg_build<T>(MGHierarchy<T>&, int, int, double, double, int,                     \
-                             const std::vector<HostLevelGrid>*);                                 \
+                             const std::vector<HostLevelGrid>*, bool);                           \
     template void mg_restrict_viscosity<T>(MGHierarchy<T>&, hipStream_t);                        \
     template void mg_smoother<T>(MGHierarchy<T>&, int, int, T, T, T, const T*, const T*,         \
                                  hipStream_t);                                                   \

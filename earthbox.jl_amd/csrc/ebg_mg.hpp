@@ -393,9 +393,12 @@ struct MGHierarchy {
     ~MGHierarchy();
 };
 
+// clamp_weights: clamp normalised transfer distances to [0, 1] so boundary
+// nodes of non-nested pressure grids interpolate instead of extrapolating
+// (the reference's maps extrapolate; the preconditioner clamps).
 template <typename T>
 int mg_build(MGHierarchy<T>& H, int xnum, int ynum, double xsize, double ysize, int nlevels,
-             const std::vector<HostLevelGrid>* finest_override);
+             const std::vector<HostLevelGrid>* finest_override, bool clamp_weights = false);
 template <typename T>
 void mg_restrict_viscosity(MGHierarchy<T>& H, hipStream_t s);
 template <typename T>

@@ -285,6 +285,30 @@ __global__ void set_fixed_kernel(StokesOpDev s, const double* __restrict__ r, MG
         l.pr[c] = (T)(s.pscale * (r[2 * (int64_t)nc + c] / kb));
 }
 
+// Pressure mode 0 replaces the continuity equation of cell (1,1) by the gauge
+// row.  With homogeneous boundary velocities the correction's divergence sums
+// to zero over all cells, so the multigrid's continuity right-hand side at
+// (1,1) is set to the value that makes the coarse problems compatible:
+// RC(1,1) = -sum_{c != (1,1)} RC_c dx dy / (dx_1 dy_1).
+template <typename T>
+__global__ void __launch_bounds__(1024) compat_rc_kernel(MGLevelDev<T> l)
+{
+    __shared__ double sh[1024];
+    const int ny1 = l.ynum - 1, n = ny1 * (l.xnum - 1);
+    double acc = 0.0;
+    for (int c = 1 + threadIdx.x; c < n; c += blockDim.x) {
+        const int i = c % ny1, j = c / ny1;
+        acc += (double)l.RC[c] * (double)l.xstp_b[j] * (double)l.ystp_b[i];
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) l.RC[0] = (T)(-sh[0] / ((double)l.xstp_b[0] * (double)l.ystp_b[0]));
+}
+
 template <typename T>
 __global__ void to_precision_kernel(const double* __restrict__ a, T* __restrict__ b, int64_t n)
 {
@@ -345,6 +369,8 @@ static void precondition(StokesState& st, MGHierarchy<T>& H, T* prn, const ebg_s
     MGLevelDev<T>& l0 = H.lev[0];
     const int nvx = (st.ynum + 1) * (st.xnum + 1);
     hipLaunchKernelGGL(pack_rhs_kernel<T>, dim3(nb(nvx)), dim3(256), 0, s, st.op, r, l0, prn, st.nc);
+    if (st.op.pressure_bc_mode == 0)
+        hipLaunchKernelGGL(compat_rc_kernel<T>, dim3(1), dim3(1024), 0, s, l0);
     mg_zero_solution(H, 0, s);
     if (l0.fix_vx || l0.fix_p)
         hipLaunchKernelGGL(set_fixed_kernel<T>, dim3(nb(st.nc)), dim3(256), 0, s, st.op, r, l0, st.nc);
@@ -383,7 +409,7 @@ static int ensure_hierarchy(ebg_ctx* ctx, StokesState& st, std::unique_ptr<MGHie
             h.gridy_pr[k] = h.gridy_pr[k - 1] + (g.h_gridy_b[k + 1] - g.h_gridy_b[k - 1]) / 2.0;
         const double xsize = g.h_gridx_b.back() - g.h_gridx_b.front();
         const double ysize = g.h_gridy_b.back() - g.h_gridy_b.front();
-        int rc = mg_build(*H, (int)g.xnum, (int)g.ynum, xsize, ysize, o.nlevels, &fine);
+        int rc = mg_build(*H, (int)g.xnum, (int)g.ynum, xsize, ysize, o.nlevels, &fine, true);
         if (rc) return rc;
         if ((rc = prn.ensure(sizeof(T) * 4))) return rc;
         st.mg_key_nlev = o.nlevels;
@@ -501,7 +527,9 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
     if (!(bnorm > 0) || !std::isfinite(bnorm)) {
         if (stats) { stats->iterations = 0; stats->relres = 0; stats->relres_initial = 0; }
         hipLaunchKernelGGL(fill_kernel<double>, dim3(nb(n)), dim3(256), 0, s, x, n, 0.0);
-        return bnorm == 0 ? EBG_OK : EBG_ERR_NUMERIC;
+        if (bnorm == 0) return EBG_OK;
+        set_error("ebg_stokes_solve: right-hand side is not finite");
+        return EBG_ERR_NUMERIC;
     }
     int total = 0, restarts = 0;
     double relres = 1.0, relres0 = -1.0;
@@ -514,7 +542,10 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
         const double beta = std::sqrt(beta2);
         relres = beta / bnorm;
         if (relres0 < 0) relres0 = relres;
-        if (!std::isfinite(relres)) return EBG_ERR_NUMERIC;
+        if (!std::isfinite(relres)) {
+            set_error("ebg_stokes_solve: residual became non-finite");
+            return EBG_ERR_NUMERIC;
+        }
         if (relres <= o.rtol || total >= o.max_iterations) break;
         EBG_HIP(hipMemcpyAsync(st.nrm.p, &beta, 8, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(scale_kernel, dim3(nb(n)), dim3(256), 0, s, V, r, st.nrm.as<double>(), n);
@@ -543,7 +574,11 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
             for (int q = 0; q <= k; ++q) H[(size_t)k * (m + 1) + q] = hbuf[q] + hbuf[MAXK + q];
             const double hn = std::sqrt(wn2);
             H[(size_t)k * (m + 1) + k + 1] = hn;
-            if (!std::isfinite(hn)) return EBG_ERR_NUMERIC;
+            if (!std::isfinite(hn)) {
+                set_error("ebg_stokes_solve: preconditioned Krylov vector is non-finite "
+                          "(multigrid preconditioner diverged)");
+                return EBG_ERR_NUMERIC;
+            }
             if (hn > 0) {
                 EBG_HIP(hipMemcpyAsync(st.nrm.p, &hn, 8, hipMemcpyHostToDevice, s));
                 hipLaunchKernelGGL(scale_kernel, dim3(nb(n)), dim3(256), 0, s, V + (int64_t)(k + 1) * n, w,

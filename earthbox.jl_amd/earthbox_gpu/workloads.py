@@ -108,9 +108,10 @@ def falling_block(xnum: int = 101, ynum: int = 101, size: float = 1.0e6,
 
 def layered_random(xnum: int = 1001, ynum: int = 501, xsize: float = 1.5e6, ysize: float = 6.0e5,
                    seed: int = 2, corr_len: float = 2.0e4, gravity_y: float = 10.0,
-                   log_eta_range=(18.0, 25.0)):
+                   log_eta_range=(18.0, 25.0), amplitude: float = 0.5):
     """Layered log10-viscosity with a Gaussian random perturbation (correlation
-    length corr_len, FFT-filtered) clipped to log_eta_range, density
+    length corr_len, FFT-filtered, standard deviation `amplitude` decades)
+    clipped to log_eta_range, density
     2700-3300 tied to the layers; free-slip walls."""
     from .grid import uniform_grid
     from .stokes_problem import StokesProblem, free_slip_bcs
@@ -132,7 +133,7 @@ def layered_random(xnum: int = 1001, ynum: int = 501, xsize: float = 1.5e6, ysiz
         ix = np.clip((X / xsize * nxf).astype(int), 0, nxf - 1)
         iy = np.clip((Y / ysize * nyf).astype(int), 0, nyf - 1)
         lay = np.clip(np.searchsorted(layer_depths, Y, side="right") - 1, 0, 3)
-        loge = np.clip(layer_logeta[lay] + 1.5 * field[iy, ix], *log_eta_range)
+        loge = np.clip(layer_logeta[lay] + amplitude * field[iy, ix], *log_eta_range)
         return 10.0 ** loge, layer_rho[lay] + 30.0 * field[iy, ix]
     Xb, Yb = np.meshgrid(g.gridx_b, g.gridy_b)
     Xp, Yp = np.meshgrid(g.gridx_pr, g.gridy_pr)

@@ -1,0 +1,192 @@
+"""
+    EarthBoxGPU
+
+Thin `ccall` binding of `libearthbox_gpu.so` (C ABI: include/earthbox_gpu.h) for
+EarthBox.jl.  It keeps the reference's entry points as the seams:
+
+- `solve_system_gpu!(model, S)`        beside SystemSolver.solve_system_optimized
+                                        (solver/system_solver/SystemSolver.jl:67-91)
+- `interpolate_markers_gpu!(model)`    beside Interpolation.interpolate_marker_stokes_parameters_to_grid!
+                                        (interpolation/Interpolation.jl:87-95)
+- `advect_markers_gpu!(model)`         beside Advection.advect_markers_and_rotate_stress_tensor!
+                                        (advection/Advection.jl:109-114)
+
+Every call copies the needed ModelData arrays to device buffers owned by a
+`GPUState`, runs the kernels and copies results back, so ModelData stays the
+authoritative state (checkpoints need nothing from the device).
+"""
+module EarthBoxGPU
+
+const LIB = get(ENV, "EARTHBOX_GPU_LIB",
+                joinpath(@__DIR__, "..", "earthbox_gpu", "libearthbox_gpu.so"))
+
+struct SolveOpts
+    precision::Cint
+    restart::Cint
+    max_iterations::Cint
+    rtol::Cdouble
+    nlevels::Cint
+    finest_sweeps::Cint
+    sweep_multiplier::Cint
+    sweep_cap::Cint
+    theta_s::Cdouble
+    theta_c::Cdouble
+    use_initial_guess::Cint
+    learned::Cint
+    bc_d::NTuple{4,Cdouble}
+end
+
+struct SolveStats
+    iterations::Cint
+    restarts::Cint
+    relres::Cdouble
+    relres_initial::Cdouble
+    seconds::Cdouble
+    status::Cint
+end
+
+check(rc, what) = rc < 0 ? error("$what failed: ",
+    unsafe_string(ccall((:ebg_last_error, LIB), Cstring, ()))) : rc
+
+mutable struct GPUState
+    ctx::Ptr{Cvoid}
+    buffers::Dict{Symbol,Ptr{Cvoid}}
+end
+
+function GPUState(device::Integer=0)
+    ctx = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:ebg_ctx_create, LIB), Cint, (Ptr{Ptr{Cvoid}}, Cint), ctx, device), "ebg_ctx_create")
+    st = GPUState(ctx[], Dict{Symbol,Ptr{Cvoid}}())
+    finalizer(st) do s
+        for p in values(s.buffers)
+            ccall((:ebg_device_free, LIB), Cint, (Ptr{Cvoid},), p)
+        end
+        ccall((:ebg_ctx_destroy, LIB), Cint, (Ptr{Cvoid},), s.ctx)
+    end
+    return st
+end
+
+"Device mirror of a host array (reallocated when the size changes)."
+function dev!(st::GPUState, key::Symbol, a::AbstractArray)
+    nbytes = sizeof(a)
+    p = get(st.buffers, key, C_NULL)
+    if p == C_NULL || get(st.buffers, Symbol(key, :_n), C_NULL) != Ptr{Cvoid}(nbytes)
+        p != C_NULL && ccall((:ebg_device_free, LIB), Cint, (Ptr{Cvoid},), p)
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:ebg_device_alloc, LIB), Cint, (Ptr{Ptr{Cvoid}}, Csize_t), r, nbytes), "alloc")
+        p = r[]
+        st.buffers[key] = p
+        st.buffers[Symbol(key, :_n)] = Ptr{Cvoid}(nbytes)
+    end
+    return p
+end
+h2d!(st, key, a) = (p = dev!(st, key, a);
+                    GC.@preserve a check(ccall((:ebg_memcpy_h2d, LIB), Cint,
+                        (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t), p, pointer(a), sizeof(a)), "h2d"); p)
+d2h!(a, p) = GC.@preserve a check(ccall((:ebg_memcpy_d2h, LIB), Cint,
+                 (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t), pointer(a), p, sizeof(a)), "d2h")
+
+function set_grid!(st::GPUState, model)
+    g = model.grids.arrays
+    geo = model.grids.parameters.geometry
+    check(ccall((:ebg_grid2d_set, LIB), Cint,
+        (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+         Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Cdouble, Cdouble),
+        st.ctx, geo.xnum.value, geo.ynum.value,
+        g.basic.gridx_b.array, g.basic.gridy_b.array, g.basic.xstp_b.array, g.basic.ystp_b.array,
+        g.staggered_vy.gridx_vy.array, g.staggered_vy.xstp_vy.array,
+        g.staggered_vx.gridy_vx.array, g.staggered_vx.ystp_vx.array,
+        geo.xstpavg.value, geo.ystpavg.value), "ebg_grid2d_set")
+end
+
+"""
+    solve_system_gpu!(st, model; mixed=true, rtol=1e-8) -> (S, stats)
+
+GPU replacement of SystemSolver.solve_system_optimized: solves the assembled
+system without building the COO, returning S in the reference's unknown order.
+"""
+function solve_system_gpu!(st::GPUState, model; mixed::Bool=true, rtol::Float64=1e-8)
+    set_grid!(st, model)
+    sc = model.stokes_continuity
+    bcs = model.bcs.arrays
+    p = Dict(k => h2d!(st, k, a) for (k, a) in (
+        :etas0 => sc.arrays.viscosity.etas0.array, :etan0 => sc.arrays.viscosity.etan0.array,
+        :rho1_vy => sc.arrays.density.rho1_vy.array, :RX1 => sc.arrays.rhs.RX1.array,
+        :RY1 => sc.arrays.rhs.RY1.array, :RC1 => sc.arrays.rhs.RC1.array))
+    vc = bcs.vel_comp
+    check(ccall((:ebg_stokes_set_system, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+         Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+         Ptr{Float64}, Ptr{Float64}, Ptr{Int64}, Ptr{Float64}, Cint, Cdouble, Cdouble, Cdouble,
+         Cdouble, Cint),
+        st.ctx, p[:etas0], p[:etan0], p[:rho1_vy], p[:RX1], p[:RY1], p[:RC1],
+        vc.btopx.array, vc.btopy.array, vc.bbottomx.array, vc.bbottomy.array,
+        vc.bleftx.array, vc.blefty.array, vc.brightx.array, vc.brighty.array,
+        bcs.internal.bintern_zone.array, bcs.internal.bintern_velocity.array,
+        model.bcs.parameters.bc_options.pressure_bc_mode.value,
+        model.bcs.parameters.pressure.pressure_bc.value, sc.parameters.build.pscale.value,
+        model.timestep.parameters.main_time_loop.timestep.value,
+        model.gravity.parameters.gravity_y.value,
+        sc.parameters.build.iuse_interface_stabilization.value), "ebg_stokes_set_system")
+    N = sc.parameters.build.N.value
+    S = zeros(Float64, N)
+    dS = h2d!(st, :S, S)
+    o = Ref{SolveOpts}()
+    ccall((:ebg_stokes_solve_opts_default, LIB), Cint, (Ref{SolveOpts},), o)
+    o[] = SolveOpts(mixed ? 1 : 0, o[].restart, o[].max_iterations, rtol, o[].nlevels,
+                    o[].finest_sweeps, o[].sweep_multiplier, o[].sweep_cap, o[].theta_s,
+                    o[].theta_c, 0, 0,
+                    (vc.btopx.array[2, 2], vc.bbottomx.array[2, 2], vc.blefty.array[2, 2],
+                     vc.brighty.array[2, 2]))
+    stats = Ref{SolveStats}()
+    rc = check(ccall((:ebg_stokes_solve, LIB), Cint,
+        (Ptr{Cvoid}, Ptr{Cvoid}, Ref{SolveOpts}, Ref{SolveStats}), st.ctx, dS, o, stats),
+        "ebg_stokes_solve")
+    d2h!(S, dS)
+    # rc > 0: not converged; the caller falls back to the CPU path, as the
+    # reference does after MUMPS retries (ParallelSolver.jl:75-124)
+    return S, stats[], rc
+end
+
+"""
+    advect_markers_gpu!(st, model, inside_flags)
+
+GPU replacement of Advection.advect_markers_and_rotate_stress_tensor!.
+"""
+function advect_markers_gpu!(st::GPUState, model, inside_flags::Vector{Int8})
+    set_grid!(st, model)
+    m = model.markers.arrays
+    sc = model.stokes_continuity.arrays
+    n = model.markers.parameters.distribution.marknum.value
+    dx = h2d!(st, :mx, m.location.marker_x.array)
+    dy = h2d!(st, :my, m.location.marker_y.array)
+    dxn = h2d!(st, :xn, m.grid_marker_relationship.marker_xn.array)
+    dyn = h2d!(st, :yn, m.grid_marker_relationship.marker_yn.array)
+    dfl = h2d!(st, :flags, inside_flags)
+    dvx1 = h2d!(st, :vx1, sc.staggered_grid_velocity.vx1.array)
+    dvy1 = h2d!(st, :vy1, sc.staggered_grid_velocity.vy1.array)
+    desp = h2d!(st, :esp, sc.strain_rate_and_spin.esp.array)
+    dmvx = dev!(st, :mvx, m.advection.marker_vx.array)
+    dmvy = dev!(st, :mvy, m.advection.marker_vy.array)
+    dmsp = dev!(st, :msp, m.advection.marker_spin.array)
+    dsxx = h2d!(st, :sxx, m.stress.marker_sxx.array)
+    dsxy = h2d!(st, :sxy, m.stress.marker_sxy.array)
+    order = 4
+    check(ccall((:ebg_rk4_advect, LIB), Cint,
+        (Ptr{Cvoid}, Int64, Cint, Cdouble, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+         Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+         Ptr{Cvoid}, Ptr{Cvoid}, Cint),
+        st.ctx, n, order, model.timestep.parameters.main_time_loop.timestep.value,
+        dx, dy, dxn, dyn, dfl, dvx1, dvy1, desp, dmvx, dmvy, dmsp, dsxx, dsxy, 1), "ebg_rk4_advect")
+    check(ccall((:ebg_ctx_synchronize, LIB), Cint, (Ptr{Cvoid},), st.ctx), "sync")
+    d2h!(m.location.marker_x.array, dx)
+    d2h!(m.location.marker_y.array, dy)
+    d2h!(m.advection.marker_vx.array, dmvx)
+    d2h!(m.advection.marker_vy.array, dmvy)
+    d2h!(m.advection.marker_spin.array, dmsp)
+    d2h!(m.stress.marker_sxx.array, dsxx)
+    d2h!(m.stress.marker_sxy.array, dsxy)
+    return nothing
+end
+
+end # module
