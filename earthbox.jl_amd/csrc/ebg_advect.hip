@@ -2,25 +2,27 @@
 // gather-advect (velocity + spin + location + stress rotation) and grid spin.
 //
 // Compiled with -ffp-contract=off: every FP64 expression is evaluated in the
-// reference's order without fused multiply-adds, and IEEE division is correctly
-// rounded, so results are bit-identical to the reference loops (only the
-// libm sin/cos of the stress rotation may differ in the last bit).
+// reference's order without fused multiply-adds, and every division returns
+// the correctly rounded quotient (rn_div: reciprocal plus Markstein
+// correction, which equals IEEE division), so results are bit-identical to
+// the reference loops (only the libm sin/cos of the stress rotation may differ
+// in the last bit).
 #include "ebg_common.hpp"
 #include "ebg_device.hpp"
 
 namespace ebg {
 
 struct AxisDesc {
-    const double* g;   // node coordinates, 0-based storage of a 1-based axis
-    int n;             // number of nodes
-    double x0, inv_h;  // uniform-axis guess parameters
+    const double* g;         // node coordinates, 0-based storage of a 1-based axis
+    int n;                   // number of nodes
+    double x0, inv_h;        // uniform-axis guess parameters
     int uniform;
+    const AxisCell* cells;   // n-1 per-cell records (lo, hi, steps, reciprocals)
 };
 
 struct RK4Args {
     int64_t marknum;
-    int order;
-    double dt;
+    double dt, half_dt;
     double *x, *y;
     const int32_t *xn, *yn;
     const int8_t* flags;
@@ -30,9 +32,36 @@ struct RK4Args {
     int update_location;
     int xnum, ynum;
     AxisDesc ax, ay;
-    const double *gridx_vy, *gridy_vx, *xstp_b, *ystp_b, *xstp_vy, *ystp_vx;
     double xmin, xmax, ymin, ymax;
 };
+
+// Correctly rounded a/b given rb = RN(1/b): q0 = RN(a*rb) is within 1.5 ulp
+// of a/b, the first remainder step makes it faithful, and Markstein's theorem
+// (P. Markstein, IBM J. Res. Dev. 34(1), 1990; Muller et al., Handbook of
+// Floating-Point Arithmetic, "Newton-Raphson-based division with an FMA") makes
+// the second step return RN(a/b): with rb within half an ulp of 1/b and q
+// within one ulp of a/b, r = a - b*q is exact and RN(q + r*rb) = RN(a/b).
+// The theorem needs no underflow or overflow in r and q: callers use it only
+// for numerators that pass div_in_range() (a zero numerator keeps its sign by
+// the final select) and take the plain IEEE division otherwise, so the result
+// always equals a / b bit for bit.
+__device__ __forceinline__ double rn_div(double a, double b, double rb)
+{
+    double q = a * rb;
+    double r = __builtin_fma(-b, q, a);
+    q = __builtin_fma(r, rb, q);
+    r = __builtin_fma(-b, q, a);
+    q = __builtin_fma(r, rb, q);
+    return (a == 0.0) ? a : q;
+}
+
+// Numerators for which rn_div is exact with divisors and reciprocals between
+// 2^-100 and 2^100 (grid steps): zero, or magnitude in [2^-900, 2^900].
+__device__ __forceinline__ bool div_in_range(double a)
+{
+    const double m = fabs(a);
+    return m == 0.0 || (m >= 0x1p-900 && m <= 0x1p900);
+}
 
 // GridToMarker.get_marker_value (interpolation/utils/GridToMarker.jl:190-226):
 // bilinear weights, summed UL, LL, UR, LR starting from 0.0.
@@ -43,7 +72,7 @@ __device__ __forceinline__ double marker_value(int yn, int xn, double dy, double
     const double wll = (1.0 - dx) * dy;
     const double wur = dx * (1.0 - dy);
     const double wlr = dx * dy;
-    const double* p = a + (size_t)(xn - 1) * ld + (yn - 1);
+    const double* p = a + (unsigned)((xn - 1) * ld + (yn - 1));
     double v = 0.0;
     v += wul * p[0];
     v += wll * p[1];
@@ -57,8 +86,79 @@ __device__ __forceinline__ int cell_index(double x, const AxisDesc& a)
     return left_node_index(x, a.g, a.n, a.x0, a.inv_h, a.uniform);
 }
 
+// clamp(searchsortedlast(g, x), 1, n-1) (MarkerGridMapping.jl:78-86) for the
+// RK stages.  Uniform axes: every node lies within a quarter cell of
+// x0 + (i-1)*h (checked when the grid is set), so for x in the true cell k,
+// t = (x - x0)/h is in [k-1.25, k+0.25) and the guess floor(t)+1 is k-1, k or
+// k+1; one exact comparison against the guessed cell's stored bounds
+// (lo = g[c-1], hi = g[c]) then gives k.  Non-uniform axes: binary search.
+template <bool UNIFORM>
+__device__ __forceinline__ int stage_cell(double x, const AxisDesc& a)
+{
+    if (!UNIFORM) return left_node_index(x, a.g, a.n, 0.0, 0.0, 0);
+    if (x != x) return a.n - 1;
+    const double t = (x - a.x0) * a.inv_h;
+    int c = (t < 1.0) ? 1 : ((t >= (double)(a.n - 2)) ? a.n - 1 : (int)t + 1);
+    const double2 lh = *reinterpret_cast<const double2*>(&a.cells[c - 1].lo);
+    if (c > 1 && x < lh.x) --c;
+    else if (c < a.n - 1 && x >= lh.y) ++c;
+    return c;
+}
+
+// Velocity and spin at one point (VelocityAndSpin.jl:343-428 with the
+// MarkerGridMapping.jl:354-441 index remaps): vx on the vx grid (basic x
+// cell, y remapped with strict '>'), vy on the vy grid (x remapped with
+// strict '>'), spin on the basic grid.  The reference's clamps of the remapped
+// indices (iyv <= ynum, jxv <= xnum) cannot bind: the basic cell is at most
+// n-1, so the remap gives at most n.
+__device__ __forceinline__ void point_values(const RK4Args& a, double xc, double yc, int xi,
+                                             int yi, double& vx, double& vy, double& sp)
+{
+    const AxisCell cx = a.ax.cells[xi - 1];
+    const AxisCell cy = a.ay.cells[yi - 1];
+    const bool ry = yc > cy.sb;
+    const bool rx = xc > cx.sb;
+    const double nxb = xc - cx.lo, nyb = yc - cy.lo;
+    const double nyv = yc - (ry ? cy.sb : cy.sa), nxy = xc - (rx ? cx.sb : cx.sa);
+    const double hyv = ry ? cy.shb : cy.sha, hxy = rx ? cx.shb : cx.sha;
+    double dxb, dyb, dyv, dxy;
+    if (div_in_range(nxb) && div_in_range(nyb) && div_in_range(nyv) && div_in_range(nxy)) {
+        dxb = rn_div(nxb, cx.h, cx.rh);
+        dyb = rn_div(nyb, cy.h, cy.rh);
+        dyv = rn_div(nyv, hyv, ry ? cy.rshb : cy.rsha);
+        dxy = rn_div(nxy, hxy, rx ? cx.rshb : cx.rsha);
+    } else {
+        dxb = nxb / cx.h;
+        dyb = nyb / cy.h;
+        dyv = nyv / hyv;
+        dxy = nxy / hxy;
+    }
+    vx = marker_value(yi + (ry ? 1 : 0), xi, dyv, dxb, a.vx1, a.ynum + 1);
+    vy = marker_value(yi, xi + (rx ? 1 : 0), dyb, dxy, a.vy1, a.ynum);
+    sp = marker_value(yi, xi, dyb, dxb, a.esp, a.ynum);
+}
+
+// (v1 + 2 v2 + 2 v3 + v4)/6 in the reference's order (RungeKutta.jl:36-47).
+__device__ __forceinline__ double rk4_combine(double v1, double v2, double v3, double v4)
+{
+    const double s = v1 + 2.0 * v2 + 2.0 * v3 + v4;
+    constexpr double r6 = 1.0 / 6.0;
+    return div_in_range(s) ? rn_div(s, 6.0, r6) : s / 6.0;
+}
+
+// RK stages (RungeKutta.jl:9-47, VelocityAndSpin.jl:45-154), location update
+// (MarkersLocation.jl:36-41) and stress rotation (MarkersStressRotation.jl:60-74)
+// fused per marker.  F: RK_RK4 (runge_kutta_order_max 4, else one stage),
+// RK_XN (stage-1 cells given), RK_STRESS (rotate sxx, sxy), RK_UNIFORM (both
+// axes uniform).
+enum { RK_RK4 = 1, RK_XN = 2, RK_STRESS = 4, RK_UNIFORM = 8 };
+
+template <int F>
 __global__ void __launch_bounds__(256) rk4_advect_kernel(RK4Args a)
 {
+    constexpr int NST = (F & RK_RK4) ? 4 : 1;
+    constexpr bool XN = (F & RK_XN) != 0, STRESS = (F & RK_STRESS) != 0;
+    constexpr bool UNI = (F & RK_UNIFORM) != 0;
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= a.marknum) return;
     const double x0 = a.x[m];
@@ -72,70 +172,44 @@ __global__ void __launch_bounds__(256) rk4_advect_kernel(RK4Args a)
         if (a.mspin) a.mspin[m] = 0.0;
         return;
     }
-    const int nst = (a.order == 4) ? 4 : 1;
-    double xc = x0, yc = y0;
-    double svx = 0.0, svy = 0.0, ssp = 0.0;
-    double v1x = 0.0, v1y = 0.0, v1s = 0.0;
-    double v2x = 0.0, v2y = 0.0, v2s = 0.0;
-    double v3x = 0.0, v3y = 0.0, v3s = 0.0;
-    for (int k = 1; k <= nst; ++k) {
-        int xi, yi;
-        if (k == 1 && a.xn) {
-            xi = a.xn[m];
-            yi = a.yn[m];
-        } else {
-            xi = cell_index(xc, a.ax);
-            yi = cell_index(yc, a.ay);
-        }
-        // vx grid: x index = basic index, y remapped with strict '>'
-        // (MarkerGridMapping.jl:378-409)
-        const double gxb = a.ax.g[xi - 1];
-        const double hxb = a.xstp_b[xi - 1];
-        const double dxv = (xc - gxb) / hxb;
-        int iyv = yi;
-        if (yc > a.gridy_vx[iyv]) iyv += 1;  // gridy_vx[iyv+1] in 1-based terms
-        if (iyv > a.ynum) iyv -= 1;
-        const double dyv = (yc - a.gridy_vx[iyv - 1]) / a.ystp_vx[iyv - 1];
-        const double vxm = marker_value(iyv, xi, dyv, dxv, a.vx1, a.ynum + 1);
-        // vy grid: x remapped with strict '>', then min(xnum) (:413-428)
-        int jxv = xi;
-        if (xc > a.gridx_vy[jxv]) jxv += 1;
-        if (jxv > a.xnum) jxv = a.xnum;
-        const double dxy = (xc - a.gridx_vy[jxv - 1]) / a.xstp_vy[jxv - 1];
-        const double gyb = a.ay.g[yi - 1];
-        const double hyb = a.ystp_b[yi - 1];
-        const double dyy = (yc - gyb) / hyb;
-        const double vym = marker_value(yi, jxv, dyy, dxy, a.vy1, a.ynum);
-        // spin on the basic grid (:402-428)
-        const double dxs = (xc - gxb) / hxb;
-        const double dys = (yc - gyb) / hyb;
-        const double spm = marker_value(yi, xi, dys, dxs, a.esp, a.ynum);
-        // RungeKutta.jl:9-47
-        if (k == 1) { v1x = vxm; v1y = vym; v1s = spm; }
-        else if (k == 2) { v2x = vxm; v2y = vym; v2s = spm; }
-        else if (k == 3) { v3x = vxm; v3y = vym; v3s = spm; }
-        else {
-            svx = (v1x + 2.0 * v2x + 2.0 * v3x + vxm) / 6.0;
-            svy = (v1y + 2.0 * v2y + 2.0 * v3y + vym) / 6.0;
-            ssp = (v1s + 2.0 * v2s + 2.0 * v3s + spm) / 6.0;
-        }
-        if (k < 3) {
-            xc = x0 + a.dt / 2.0 * vxm;
-            yc = y0 + a.dt / 2.0 * vym;
-        } else if (k == 3) {
-            xc = x0 + a.dt * vxm;
-            yc = y0 + a.dt * vym;
-        }
+    double svx, svy, ssp;
+    int xi, yi;
+    if (XN) {
+        xi = a.xn[m];
+        yi = a.yn[m];
+    } else {
+        xi = stage_cell<UNI>(x0, a.ax);
+        yi = stage_cell<UNI>(y0, a.ay);
     }
-    if (nst == 1) { svx = v1x; svy = v1y; ssp = v1s; }
+    double v1x, v1y, v1s;
+    point_values(a, x0, y0, xi, yi, v1x, v1y, v1s);
+    if (NST == 4) {
+        double v2x, v2y, v2s, v3x, v3y, v3s, v4x, v4y, v4s;
+        double xc = x0 + a.half_dt * v1x;
+        double yc = y0 + a.half_dt * v1y;
+        point_values(a, xc, yc, stage_cell<UNI>(xc, a.ax), stage_cell<UNI>(yc, a.ay), v2x, v2y, v2s);
+        xc = x0 + a.half_dt * v2x;
+        yc = y0 + a.half_dt * v2y;
+        point_values(a, xc, yc, stage_cell<UNI>(xc, a.ax), stage_cell<UNI>(yc, a.ay), v3x, v3y, v3s);
+        xc = x0 + a.dt * v3x;
+        yc = y0 + a.dt * v3y;
+        point_values(a, xc, yc, stage_cell<UNI>(xc, a.ax), stage_cell<UNI>(yc, a.ay), v4x, v4y, v4s);
+        svx = rk4_combine(v1x, v2x, v3x, v4x);
+        svy = rk4_combine(v1y, v2y, v3y, v4y);
+        ssp = rk4_combine(v1s, v2s, v3s, v4s);
+    } else {
+        svx = v1x;
+        svy = v1y;
+        ssp = v1s;
+    }
     if (a.mvx) a.mvx[m] = svx;
     if (a.mvy) a.mvy[m] = svy;
     if (a.mspin) a.mspin[m] = ssp;
-    if (a.update_location) {  // MarkersLocation.jl:36-41
+    if (a.update_location) {
         a.x[m] = x0 + a.dt * svx;
         a.y[m] = y0 + a.dt * svy;
     }
-    if (a.sxx) {  // MarkersStressRotation.jl:60-74
+    if (STRESS) {
         const double t1 = 2.0 * ssp * a.dt;
         const double t2 = ssp * a.dt;
         double s1, c1, s2, c2;
@@ -184,14 +258,24 @@ __global__ void spin_kernel(int xnum, int ynum, const double* __restrict__ xstp_
 
 static AxisDesc axis_x(const Grid2D& g)
 {
-    return AxisDesc{g.gridx_b, (int)g.xnum, g.x0, g.inv_hx, g.uniform_x};
+    return AxisDesc{g.gridx_b, (int)g.xnum, g.x0, g.inv_hx, g.uniform_x, g.cells_x};
 }
 static AxisDesc axis_y(const Grid2D& g)
 {
-    return AxisDesc{g.gridy_b, (int)g.ynum, g.y0, g.inv_hy, g.uniform_y};
+    return AxisDesc{g.gridy_b, (int)g.ynum, g.y0, g.inv_hy, g.uniform_y, g.cells_y};
 }
 
 static unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+static const void* const rk4_kernels[16] = {
+    (const void*)rk4_advect_kernel<0>,  (const void*)rk4_advect_kernel<1>,
+    (const void*)rk4_advect_kernel<2>,  (const void*)rk4_advect_kernel<3>,
+    (const void*)rk4_advect_kernel<4>,  (const void*)rk4_advect_kernel<5>,
+    (const void*)rk4_advect_kernel<6>,  (const void*)rk4_advect_kernel<7>,
+    (const void*)rk4_advect_kernel<8>,  (const void*)rk4_advect_kernel<9>,
+    (const void*)rk4_advect_kernel<10>, (const void*)rk4_advect_kernel<11>,
+    (const void*)rk4_advect_kernel<12>, (const void*)rk4_advect_kernel<13>,
+    (const void*)rk4_advect_kernel<14>, (const void*)rk4_advect_kernel<15>};
 
 }  // namespace ebg
 
@@ -248,8 +332,8 @@ int ebg_rk4_advect(ebg_ctx* ctx, int64_t marknum, int order, double timestep,
     const Grid2D& g = ctx->grid;
     RK4Args a;
     a.marknum = marknum;
-    a.order = order;
     a.dt = timestep;
+    a.half_dt = timestep / 2.0;
     a.x = marker_x;
     a.y = marker_y;
     a.xn = marker_xn;
@@ -268,18 +352,15 @@ int ebg_rk4_advect(ebg_ctx* ctx, int64_t marknum, int order, double timestep,
     a.ynum = (int)g.ynum;
     a.ax = axis_x(g);
     a.ay = axis_y(g);
-    a.gridx_vy = g.gridx_vy;
-    a.gridy_vx = g.gridy_vx;
-    a.xstp_b = g.xstp_b;
-    a.ystp_b = g.ystp_b;
-    a.xstp_vy = g.xstp_vy;
-    a.ystp_vx = g.ystp_vx;
     a.xmin = g.xmin;
     a.xmax = g.xmax;
     a.ymin = g.ymin;
     a.ymax = g.ymax;
-    hipLaunchKernelGGL(rk4_advect_kernel, dim3(blocks_for(marknum, 256)), dim3(256), 0,
-                       ctx->stream, a);
+    const int f = (order == 4 ? RK_RK4 : 0) | (marker_xn ? RK_XN : 0) |
+                  (marker_sxx ? RK_STRESS : 0) | (g.uniform_x && g.uniform_y ? RK_UNIFORM : 0);
+    void* args[] = {&a};
+    EBG_HIP(hipLaunchKernel(rk4_kernels[f], dim3(blocks_for(marknum, 256)), dim3(256), args, 0,
+                            ctx->stream));
     EBG_HIP(hipGetLastError());
     return EBG_OK;
 }

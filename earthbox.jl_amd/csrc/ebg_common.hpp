@@ -31,6 +31,19 @@ void set_error(const std::string& msg);
         }                                 \
     } while (0)
 
+// One basic cell [lo, hi) of a grid axis together with everything the marker
+// gathers read for a point inside it: the basic step h and its correctly
+// rounded reciprocal, and the two staggered nodes that the velocity-grid remap
+// chooses between (staggered index c-1 when x <= sb, else c), with their steps
+// and reciprocals.  Record k (0-based) describes the 1-based basic cell k+1.
+struct alignas(16) AxisCell {
+    double lo, hi;
+    double h, rh;
+    double sa, sha;
+    double rsha, sb;
+    double shb, rshb;
+};
+
 // Device copies of one 2-D staggered grid (all arrays FP64, lengths as in the
 // reference: basic gridx_b[xnum], xstp_b[xnum-1], vy-grid gridx_vy[xnum+1],
 // xstp_vy[xnum], vx-grid gridy_vx[ynum+1], ystp_vx[ynum]).
@@ -45,6 +58,7 @@ struct Grid2D {
     double *xstp_b = nullptr, *ystp_b = nullptr;
     double *gridx_vy = nullptr, *xstp_vy = nullptr;
     double *gridy_vx = nullptr, *ystp_vx = nullptr;
+    AxisCell *cells_x = nullptr, *cells_y = nullptr;  // xnum-1 and ynum-1 records
     std::vector<double> h_gridx_b, h_gridy_b, h_xstp_b, h_ystp_b;
     std::vector<double> h_gridx_vy, h_xstp_vy, h_gridy_vx, h_ystp_vx;
 };

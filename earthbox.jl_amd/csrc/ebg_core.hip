@@ -18,6 +18,37 @@ int grid_release(Grid2D& g)
         if (*p) (void)hipFree(*p);
         *p = nullptr;
     }
+    AxisCell** cells[] = {&g.cells_x, &g.cells_y};
+    for (AxisCell** p : cells) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    return EBG_OK;
+}
+
+// Per-cell axis records for the marker gathers.  The reciprocals are IEEE
+// divisions on the host, i.e. correctly rounded, which the device division
+// rn_div() requires.  stag/stag_stp are the staggered axis the velocity remap
+// chooses from (gridx_vy/xstp_vy for x, gridy_vx/ystp_vx for y).
+static int upload_cells(AxisCell** dst, int64_t n, const double* g, const double* stp,
+                        const double* stag, const double* stag_stp)
+{
+    std::vector<AxisCell> h((size_t)(n - 1));
+    for (int64_t k = 0; k + 1 < n; ++k) {
+        AxisCell& c = h[(size_t)k];
+        c.lo = g[k];
+        c.hi = g[k + 1];
+        c.h = stp[k];
+        c.rh = 1.0 / stp[k];
+        c.sa = stag[k];
+        c.sha = stag_stp[k];
+        c.rsha = 1.0 / stag_stp[k];
+        c.sb = stag[k + 1];
+        c.shb = stag_stp[k + 1];
+        c.rshb = 1.0 / stag_stp[k + 1];
+    }
+    EBG_HIP(hipMalloc(dst, sizeof(AxisCell) * h.size()));
+    EBG_HIP(hipMemcpy(*dst, h.data(), sizeof(AxisCell) * h.size(), hipMemcpyHostToDevice));
     return EBG_OK;
 }
 
@@ -168,6 +199,8 @@ int ebg_grid2d_set(ebg_ctx* ctx, int64_t xnum, int64_t ynum,
     if ((rc = upload(&g.xstp_vy, xstp_vy, xnum, g.h_xstp_vy))) return rc;
     if ((rc = upload(&g.gridy_vx, gridy_vx, ynum + 1, g.h_gridy_vx))) return rc;
     if ((rc = upload(&g.ystp_vx, ystp_vx, ynum, g.h_ystp_vx))) return rc;
+    if ((rc = upload_cells(&g.cells_x, xnum, gridx_b, xstp_b, gridx_vy, xstp_vy))) return rc;
+    if ((rc = upload_cells(&g.cells_y, ynum, gridy_b, ystp_b, gridy_vx, ystp_vx))) return rc;
     g.xmin = gridx_b[0];
     g.xmax = gridx_b[xnum - 1];
     g.ymin = gridy_b[0];

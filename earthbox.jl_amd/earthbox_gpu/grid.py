@@ -1,4 +1,4 @@
-"""Host-side staggered-grid construction (uniform basic grid).
+"""Host-side staggered-grid construction (uniform and graded basic grids).
 
 Restates the reference's coordinate and spacing loops so the arrays handed to
 the GPU are the same bits the reference would hold in ModelData:
@@ -16,12 +16,14 @@ from dataclasses import dataclass
 import numpy as np
 
 
-def _basic_axis(n: int, step: float) -> tuple[np.ndarray, np.ndarray]:
+def _basic_axis(n: int, step) -> tuple[np.ndarray, np.ndarray]:
+    """Cumulative-sum coordinates; step is one value or n-1 per-cell steps."""
+    steps = np.broadcast_to(np.asarray(step, dtype=np.float64), (n - 1,))
     g = np.empty(n)
     acc = 0.0
     g[0] = acc
     for i in range(1, n):  # sequential FP64 accumulation, as the reference
-        acc = acc + step
+        acc = acc + float(steps[i - 1])
         g[i] = acc
     stp = g[1:] - g[:-1]
     return g, stp
@@ -78,17 +80,36 @@ class StaggeredGrid2D:
                 float(self.gridy_b[0]), float(self.gridy_b[-1]))
 
 
-def uniform_grid(xnum: int, ynum: int, xsize: float, ysize: float) -> StaggeredGrid2D:
-    """Uniform staggered grid as the reference builds it."""
-    if xnum < 3 or ynum < 3:
-        raise ValueError("xnum and ynum must be >= 3")
+def _assemble(xnum, ynum, xsize, ysize, xstep, ystep) -> StaggeredGrid2D:
     xstpavg = xsize / (xnum - 1)
     ystpavg = ysize / (ynum - 1)
-    gx, sx = _basic_axis(xnum, xstpavg)
-    gy, sy = _basic_axis(ynum, ystpavg)
+    gx, sx = _basic_axis(xnum, xstep)
+    gy, sy = _basic_axis(ynum, ystep)
     gxv, sxv = _staggered_axis(xnum, gx, sx)
     gyv, syv = _staggered_axis(ynum, gy, sy)
     gxp, sxp = _pressure_axis(xnum, gx)
     gyp, syp = _pressure_axis(ynum, gy)
     return StaggeredGrid2D(xnum, ynum, xsize, ysize, xstpavg, ystpavg,
                            gx, gy, sx, sy, gxv, sxv, gyv, syv, gxp, gyp, sxp, syp)
+
+
+def graded_grid(xnum: int, ynum: int, xsize: float, ysize: float, seed: int,
+                spread: float = 0.6) -> StaggeredGrid2D:
+    """Non-uniform staggered grid: per-cell steps drawn in
+    [(1-spread), (1+spread)] x the mean step and rescaled to the box size, then
+    accumulated as the reference accumulates a refined grid's steps."""
+    if xnum < 3 or ynum < 3:
+        raise ValueError("xnum and ynum must be >= 3")
+    rng = np.random.default_rng(seed)
+
+    def steps(n, size):
+        w = 1.0 + spread * (2.0 * rng.random(n - 1) - 1.0)
+        return w * (size / w.sum())
+    return _assemble(xnum, ynum, xsize, ysize, steps(xnum, xsize), steps(ynum, ysize))
+
+
+def uniform_grid(xnum: int, ynum: int, xsize: float, ysize: float) -> StaggeredGrid2D:
+    """Uniform staggered grid as the reference builds it."""
+    if xnum < 3 or ynum < 3:
+        raise ValueError("xnum and ynum must be >= 3")
+    return _assemble(xnum, ynum, xsize, ysize, xsize / (xnum - 1), ysize / (ynum - 1))

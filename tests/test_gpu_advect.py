@@ -11,13 +11,14 @@ import torch
 import oracle
 from earthbox_gpu import workloads
 from earthbox_gpu.device import to_device, to_host
-from earthbox_gpu.grid import uniform_grid
+from earthbox_gpu.grid import graded_grid, uniform_grid
 
 pytestmark = pytest.mark.gpu
 
 
-def _case(nx, ny, nm, seed, outside=True):
-    g = uniform_grid(nx, ny, 1.5e6 * (nx - 1) / 1000, 6.0e5 * (ny - 1) / 500)
+def _case(nx, ny, nm, seed, outside=True, graded=False):
+    make = (lambda *a: graded_grid(*a, seed=seed + 100)) if graded else uniform_grid
+    g = make(nx, ny, 1.5e6 * (nx - 1) / 1000, 6.0e5 * (ny - 1) / 500)
     x, y = workloads.marker_positions(g, nm, nm, seed)
     if outside:
         x[:3] = [-1.0, g.gridx_b[-1] + 1.0, np.nan]
@@ -29,11 +30,13 @@ def _case(nx, ny, nm, seed, outside=True):
     return g, x, y, vx1, vy1, esp, dt
 
 
-@pytest.mark.parametrize("nx,ny,nm,seed", [(11, 7, 3, 1), (101, 101, 4, 2), (203, 61, 5, 3)])
+@pytest.mark.parametrize("nx,ny,nm,seed,graded", [(11, 7, 3, 1, False), (101, 101, 4, 2, False),
+                                                  (203, 61, 5, 3, False), (11, 7, 3, 4, True),
+                                                  (97, 53, 4, 5, True)])
 @pytest.mark.parametrize("mode", ["reference", "minimal"])
-def test_rk4_advect_bit_exact(gpu_ctx, nx, ny, nm, seed, mode):
-    g, x, y, vx1, vy1, esp, dt = _case(nx, ny, nm, seed)
-    print(f"rk4 case nx={nx} ny={ny} markers={x.size} mode={mode}", flush=True)
+def test_rk4_advect_bit_exact(gpu_ctx, nx, ny, nm, seed, graded, mode):
+    g, x, y, vx1, vy1, esp, dt = _case(nx, ny, nm, seed, graded=graded)
+    print(f"rk4 case nx={nx} ny={ny} markers={x.size} graded={graded} mode={mode}", flush=True)
     flags = oracle.inside_flags(x, y, g.limits)
     xn, yn = oracle.cell_indices_basic(x, y, flags, g)
     rng = np.random.default_rng(seed)
@@ -64,8 +67,30 @@ def test_rk4_advect_bit_exact(gpu_ctx, nx, ny, nm, seed, mode):
     np.testing.assert_allclose(to_host(dsxy), esxy, rtol=1e-14, atol=1e-7)
 
 
-def test_cell_indices_and_flags_bit_exact(gpu_ctx):
-    g = uniform_grid(41, 23, 4.0e5 / 3.0, 2.2e5)
+@pytest.mark.parametrize("graded", [False, True])
+def test_rk_order1_velocity_bit_exact(gpu_ctx, graded):
+    """runge_kutta_order_max != 4: the stage-1 velocity and spin only."""
+    g, x, y, vx1, vy1, esp, dt = _case(61, 41, 4, 7, graded=graded)
+    print(f"rk order-1 case graded={graded} markers={x.size}", flush=True)
+    flags = oracle.inside_flags(x, y, g.limits)
+    xn, yn = oracle.cell_indices_basic(x, y, flags, g)
+    evx, evy, esp_m = oracle.rk4_velocity_and_spin(1, dt, g, x, y, xn, yn, flags, vx1, vy1, esp)
+    gpu_ctx.set_grid(g)
+    n = x.size
+    out = [torch.full((n,), 7.0, dtype=torch.float64, device="cuda") for _ in range(3)]
+    gpu_ctx.update_marker_velocity_and_spin_using_runge_kutta(
+        1, dt, to_device(flags), to_device(x), to_device(y), to_device(xn), to_device(yn),
+        to_device(vx1), to_device(vy1), to_device(esp), *out)
+    gpu_ctx.synchronize()
+    assert np.array_equal(to_host(out[0]), evx)
+    assert np.array_equal(to_host(out[1]), evy)
+    assert np.array_equal(to_host(out[2]), esp_m)
+
+
+@pytest.mark.parametrize("graded", [False, True])
+def test_cell_indices_and_flags_bit_exact(gpu_ctx, graded):
+    g = (graded_grid(41, 23, 4.0e5 / 3.0, 2.2e5, seed=8) if graded
+         else uniform_grid(41, 23, 4.0e5 / 3.0, 2.2e5))
     # every node coordinate, its neighbours one ulp away, and outside probes
     xs = np.concatenate([g.gridx_b, np.nextafter(g.gridx_b, -np.inf), np.nextafter(g.gridx_b, np.inf),
                          [-0.0, -1e-300, np.inf, -np.inf, g.gridx_b[-1] * 2]])
