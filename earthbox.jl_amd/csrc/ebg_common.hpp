@@ -59,6 +59,7 @@ struct Grid2D {
     double *gridx_vy = nullptr, *xstp_vy = nullptr;
     double *gridy_vx = nullptr, *ystp_vx = nullptr;
     AxisCell *cells_x = nullptr, *cells_y = nullptr;  // xnum-1 and ynum-1 records
+    uint64_t generation = 0;  // changes on every ebg_grid2d_set
     std::vector<double> h_gridx_b, h_gridy_b, h_xstp_b, h_ystp_b;
     std::vector<double> h_gridx_vy, h_xstp_vy, h_gridy_vx, h_ystp_vx;
 };

@@ -207,6 +207,8 @@ int ebg_grid2d_set(ebg_ctx* ctx, int64_t xnum, int64_t ynum,
     g.ymax = gridy_b[ynum - 1];
     g.uniform_x = axis_uniform(gridx_b, xnum, &g.x0, &g.inv_hx);
     g.uniform_y = axis_uniform(gridy_b, ynum, &g.y0, &g.inv_hy);
+    static uint64_t generations = 0;
+    g.generation = ++generations;
     ctx->has_grid = true;
     return EBG_OK;
 }

@@ -2,6 +2,7 @@
 // fine->coarse maps with gather ranges, device allocation and V-cycle driver.
 #include "ebg_common.hpp"
 #include "ebg_mg.hpp"
+#include "ebg_mg_tile.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -341,6 +342,130 @@ void mg_vcycle(MGHierarchy<T>& H, const std::vector<int>& sm, T theta_s, T theta
     }
 }
 
+template <typename T>
+int mg_plan_tiles(MGHierarchy<T>& H, int tile, bool allow_single, bool fast)
+{
+    int dev = 0, max_lds = 65536;
+    EBG_HIP(hipGetDevice(&dev));
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        max_lds = 65536;
+    size_t need = 0;
+    H.tiles.clear();
+    for (int k = 0; k < H.nlev(); ++k) {
+        const MGLevelDev<T>& l = H.lev[k];
+        typename MGHierarchy<T>::TilePlan p;
+        const size_t whole = tile_lds_bytes<T>(l.ynum + 1, l.xnum + 1);
+        if (allow_single && whole <= (size_t)max_lds) {
+            p.ti = l.ynum + 1;
+            p.tj = l.xnum + 1;
+            p.nti = p.ntj = 1;
+            p.colours = 1;
+            p.single = true;
+            p.lds = whole;
+        } else {
+            p.ti = p.tj = tile;
+            p.nti = (l.ynum + 1 + tile - 1) / tile;
+            p.ntj = (l.xnum + 1 + tile - 1) / tile;
+            p.colours = 4;
+            p.single = false;
+            p.lds = tile_lds_bytes<T>(tile, tile);
+        }
+        need = std::max(need, p.lds);
+        H.tiles.push_back(p);
+    }
+    H.fast_div = fast;
+    if (need > 65536) {
+        const void* k = fast ? (const void*)mg_tile_smooth_kernel<T, true>
+                             : (const void*)mg_tile_smooth_kernel<T, false>;
+        EBG_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need));
+    }
+    return EBG_OK;
+}
+
+template <typename T>
+void mg_smoother_tiled(MGHierarchy<T>& H, int k, int nsweeps, int local_sweeps, T theta_s,
+                       T theta_c, T prnorm, const T* prnorm_dev, const T* bc_d, hipStream_t s)
+{
+    const bool normalize = (k > 0) || H.normalize_finest;
+    const MGLevelDev<T>& l = H.lev[k];
+    const auto& p = H.tiles[k];
+    const TileSpec ts{p.ti, p.tj, p.nti, p.ntj, p.colours};
+    const int np = (l.ynum - 1) * (l.xnum - 1);
+    if (p.single) {
+        const TileSmoothArgs a{0, nsweeps, 1, normalize ? 1 : 0};
+        if (H.fast_div)
+            hipLaunchKernelGGL((mg_tile_smooth_kernel<T, true>), dim3(1), dim3(1024), p.lds, s, l, ts, a,
+                               theta_s, theta_c, bc_d[0], bc_d[1], bc_d[2], bc_d[3], prnorm, prnorm_dev);
+        else
+            hipLaunchKernelGGL((mg_tile_smooth_kernel<T, false>), dim3(1), dim3(1024), p.lds, s, l, ts, a,
+                               theta_s, theta_c, bc_d[0], bc_d[1], bc_d[2], bc_d[3], prnorm, prnorm_dev);
+        return;
+    }
+    const int nthreads = std::min(1024, ((p.ti * p.tj + 63) / 64) * 64);
+    for (int done = 0; done < nsweeps; done += local_sweeps) {
+        const int m = std::min(local_sweeps, nsweeps - done);
+        for (int c = 0; c < 4; ++c) {
+            const int ci = c & 1, cj = c >> 1;
+            const int ni = (p.nti - ci + 1) / 2, nj = (p.ntj - cj + 1) / 2;
+            if (ni <= 0 || nj <= 0) continue;
+            const TileSmoothArgs a{c, m, 0, 0};
+            if (H.fast_div)
+                hipLaunchKernelGGL((mg_tile_smooth_kernel<T, true>), dim3(ni * nj), dim3(nthreads), p.lds,
+                                   s, l, ts, a, theta_s, theta_c, bc_d[0], bc_d[1], bc_d[2], bc_d[3],
+                                   prnorm, prnorm_dev);
+            else
+                hipLaunchKernelGGL((mg_tile_smooth_kernel<T, false>), dim3(ni * nj), dim3(nthreads), p.lds,
+                                   s, l, ts, a, theta_s, theta_c, bc_d[0], bc_d[1], bc_d[2], bc_d[3],
+                                   prnorm, prnorm_dev);
+        }
+    }
+    if (normalize) {
+        hipLaunchKernelGGL(mg_pressure_shift_kernel<T>, dim3(1), dim3(64), 0, s, l.pr, prnorm_dev, prnorm,
+                           H.scratch);
+        hipLaunchKernelGGL(mg_pressure_add_kernel<T>, dim3(nblk(np)), dim3(256), 0, s, l.pr, np,
+                           H.scratch);
+    }
+    hipLaunchKernelGGL(mg_residual_kernel<T>, dim3(nblk((int64_t)(l.ynum + 1) * (l.xnum + 1))),
+                       dim3(256), 0, s, l);
+}
+
+// Same level order and transfers as mg_vcycle; the smoothing calls use the
+// tiled smoother.
+template <typename T>
+void mg_vcycle_tiled(MGHierarchy<T>& H, const std::vector<int>& sm, int local_sweeps, T theta_s,
+                     T theta_c, T relax_v, T relax_p, T prnorm, const T* prnorm_dev, const T* bc_d,
+                     hipStream_t s)
+{
+    const int nl = H.nlev();
+    for (int n = 0; n < nl; ++n) {
+        if (n > 0) mg_zero_solution(H, n, s);
+        mg_smoother_tiled(H, n, sm[n], local_sweeps, theta_s, theta_c, prnorm, prnorm_dev, bc_d, s);
+        if (n < nl - 1) {
+            const MGLevelDev<T>& f = H.lev[n];
+            const MGLevelDev<T>& c = H.lev[n + 1];
+            const TransferDev<T>& t = H.xfer[n];
+            hipLaunchKernelGGL((mg_restrict_kernel<T, 0>), dim3(nblk((int64_t)(c.ynum + 1) * c.xnum)),
+                               dim3(256), 0, s, f, c, t.vx_y, t.vx_x);
+            hipLaunchKernelGGL((mg_restrict_kernel<T, 1>), dim3(nblk((int64_t)c.ynum * (c.xnum + 1))),
+                               dim3(256), 0, s, f, c, t.vy_y, t.vy_x);
+            hipLaunchKernelGGL((mg_restrict_kernel<T, 2>),
+                               dim3(nblk((int64_t)(c.ynum - 1) * (c.xnum - 1))), dim3(256), 0, s, f, c,
+                               t.pr_y, t.pr_x);
+        }
+    }
+    for (int n = nl - 1; n >= 0; --n) {
+        if (n == 0) {
+            mg_smoother_tiled(H, 0, sm[0], local_sweeps, theta_s, theta_c, prnorm, prnorm_dev, bc_d, s);
+        } else {
+            mg_smoother_tiled(H, n, sm[n], local_sweeps, theta_s, theta_c, T(0), (const T*)nullptr, bc_d, s);
+            const MGLevelDev<T>& c = H.lev[n];
+            const MGLevelDev<T>& f = H.lev[n - 1];
+            hipLaunchKernelGGL(mg_prolong_kernel<T>, dim3(nblk((int64_t)(f.ynum + 1) * (f.xnum + 1))),
+                               dim3(256), 0, s, c, f, H.xfer[n - 1], relax_v, relax_p);
+        }
+    }
+}
+
 #define EBG_INST(T)                                                                              \
     template struct MGHierarchy<T>;                                                              \
     template int mg_build<T>(MGHierarchy<T>&, int, int, double, double, int,                     \
@@ -350,7 +475,12 @@ void mg_vcycle(MGHierarchy<T>& H, const std::vector<int>& sm, T theta_s, T theta
                                  hipStream_t);                                                   \
     template void mg_vcycle<T>(MGHierarchy<T>&, const std::vector<int>&, T, T, T, T, T, const T*, \
                                const T*, hipStream_t);                                           \
-    template void mg_zero_solution<T>(MGHierarchy<T>&, int, hipStream_t);
+    template void mg_zero_solution<T>(MGHierarchy<T>&, int, hipStream_t);                        \
+    template int mg_plan_tiles<T>(MGHierarchy<T>&, int, bool, bool);                             \
+    template void mg_smoother_tiled<T>(MGHierarchy<T>&, int, int, int, T, T, T, const T*, const T*, \
+                                       hipStream_t);                                             \
+    template void mg_vcycle_tiled<T>(MGHierarchy<T>&, const std::vector<int>&, int, T, T, T, T, T, \
+                                     const T*, const T*, hipStream_t);
 EBG_INST(float)
 EBG_INST(double)
 

@@ -58,55 +58,107 @@ struct TransferDev {
 #define MG_ES(l, i, j) (l).etas[(size_t)((j) - 1) * (l).ynum + ((i) - 1)]
 #define MG_EN(l, i, j) (l).etan[(size_t)((j) - 1) * ((l).ynum - 1) + ((i) - 1)]
 
+// Field access through a view, so the same residual arithmetic runs on the
+// level arrays in HBM (GlobalView) and on LDS tiles (TileView, ebg_mg_tile.hpp).
+// Step accessors take the 0-based index into the reference's 1-based arrays.
+template <typename T>
+struct GlobalView {
+    static constexpr bool fast_div = false;
+    const MGLevelDev<T>& l;
+    __device__ T vx(int i, int j) const { return MG_VX(l, i, j); }
+    __device__ T vy(int i, int j) const { return MG_VY(l, i, j); }
+    __device__ T pr(int i, int j) const { return MG_PR(l, i, j); }
+    __device__ T es(int i, int j) const { return MG_ES(l, i, j); }
+    __device__ T en(int i, int j) const { return MG_EN(l, i, j); }
+    __device__ T RX(int i, int j) const { return l.RX[(size_t)(j - 1) * (l.ynum + 1) + (i - 1)]; }
+    __device__ T RY(int i, int j) const { return l.RY[(size_t)(j - 1) * l.ynum + (i - 1)]; }
+    __device__ T RC(int i, int j) const { return l.RC[(size_t)(j - 1) * (l.ynum - 1) + (i - 1)]; }
+    __device__ T xb(int k) const { return l.xstp_b[k]; }
+    __device__ T yb(int k) const { return l.ystp_b[k]; }
+    __device__ T xv(int k) const { return l.xstp_vy[k]; }
+    __device__ T yv(int k) const { return l.ystp_vx[k]; }
+};
+
+// Division policy of the residual arithmetic: V::fast_div selects the
+// hardware reciprocal for FP32 (the mixed-precision preconditioner); every
+// other view divides with IEEE rounding, as the reference.
+template <class V, typename T>
+__device__ __forceinline__ T vdiv(T a, T b)
+{
+    if constexpr (V::fast_div && sizeof(T) == 4) return __fdividef(a, b);
+    else return a / b;
+}
+
+// x-Stokes residual and diagonal at vx(i, j) (Residuals.jl:227-287, 2-D)
+template <typename T, class V>
+__device__ __forceinline__ T vx_residual_v(const V& v, int i, int j, T* coef)
+{
+    auto D = [](T a, T b) { return vdiv<V, T>(a, b); };
+    const T dxL = v.xb(j - 2), dxR = v.xb(j - 1), dxC = v.xv(j - 1);
+    const T dyU = v.yv(i - 2), dyD = v.yv(i - 1), dyC = v.yb(i - 2);
+    const T nR = v.en(i - 1, j), nL = v.en(i - 1, j - 1);
+    const T sU = v.es(i - 1, j), sD = v.es(i, j);
+    const T pL = v.pr(i - 1, j - 1), pR = v.pr(i - 1, j);
+    const T vxL = v.vx(i, j - 1), vxC = v.vx(i, j), vxR = v.vx(i, j + 1);
+    const T vxU = v.vx(i - 1, j), vxD = v.vx(i + 1, j);
+    const T vyUL = v.vy(i - 1, j), vyUR = v.vy(i - 1, j + 1);
+    const T vyLL = v.vy(i, j), vyLR = v.vy(i, j + 1);
+    const T dp_dx = D(pR - pL, dxC);
+    const T dsxx = D(T(2) * (D(nR * (vxR - vxC), dxR) - D(nL * (vxC - vxL), dxL)), dxC);
+    const T dsxy = D(sD * (D(vxD - vxC, dyD) + D(vyLR - vyLL, dxC)) -
+                     sU * (D(vxC - vxU, dyU) + D(vyUR - vyUL, dxC)), dyC);
+    const T dR = v.RX(i, j) + dp_dx - dsxx - dsxy;
+    if (coef) *coef = T(-2) * (D(D(nR, dxR), dxC) + D(D(nL, dxL), dxC)) - (D(D(sD, dyD), dyC) + D(D(sU, dyU), dyC));
+    return dR;
+}
+
+// y-Stokes residual and diagonal at vy(i, j) (Residuals.jl:373-433, 2-D)
+template <typename T, class V>
+__device__ __forceinline__ T vy_residual_v(const V& v, int i, int j, T* coef)
+{
+    auto D = [](T a, T b) { return vdiv<V, T>(a, b); };
+    const T dxL = v.xb(j - 2), dxR = v.xv(j - 1), dxC = v.xv(j - 2);
+    const T dyU = v.yb(i - 2), dyD = v.yb(i - 1), dyC = v.yv(i - 1);
+    const T pD = v.pr(i, j - 1), pU = v.pr(i - 1, j - 1);
+    const T nD = v.en(i, j - 1), nU = v.en(i - 1, j - 1);
+    const T sR = v.es(i, j), sL = v.es(i, j - 1);
+    const T vyU = v.vy(i - 1, j), vyC = v.vy(i, j), vyD = v.vy(i + 1, j);
+    const T vyR = v.vy(i, j + 1), vyL = v.vy(i, j - 1);
+    const T vxLR = v.vx(i + 1, j), vxUR = v.vx(i, j);
+    const T vxLL = v.vx(i + 1, j - 1), vxUL = v.vx(i, j - 1);
+    const T dp_dy = D(pD - pU, dyC);
+    const T dsyy = D(T(2) * (D(nD * (vyD - vyC), dyD) - D(nU * (vyC - vyU), dyU)), dyC);
+    const T dsyx = D(sR * (D(vyR - vyC, dxR) + D(vxLR - vxUR, dyC)) -
+                     sL * (D(vyC - vyL, dxL) + D(vxLL - vxUL, dyC)), dxC);
+    const T dR = v.RY(i, j) + dp_dy - dsyy - dsyx;
+    if (coef) *coef = T(-2) * (D(D(nD, dyD), dyC) + D(D(nU, dyU), dyC)) - (D(D(sR, dxR), dxC) + D(D(sL, dxL), dxC));
+    return dR;
+}
+
+// continuity residual at pr(i, j) (Residuals.jl:562-591, 2-D)
+template <typename T, class V>
+__device__ __forceinline__ T pr_residual_v(const V& v, int i, int j)
+{
+    auto D = [](T a, T b) { return vdiv<V, T>(a, b); };
+    const T dxC = v.xb(j - 1), dyC = v.yb(i - 1);
+    return v.RC(i, j) - (D(v.vx(i + 1, j + 1) - v.vx(i + 1, j), dxC) +
+                         D(v.vy(i + 1, j + 1) - v.vy(i, j + 1), dyC));
+}
+
 template <typename T>
 __device__ __forceinline__ T mg_vx_residual(const MGLevelDev<T>& l, int i, int j, T* coef)
 {
-    const T dxL = l.xstp_b[j - 2], dxR = l.xstp_b[j - 1], dxC = l.xstp_vy[j - 1];
-    const T dyU = l.ystp_vx[i - 2], dyD = l.ystp_vx[i - 1], dyC = l.ystp_b[i - 2];
-    const T nR = MG_EN(l, i - 1, j), nL = MG_EN(l, i - 1, j - 1);
-    const T sU = MG_ES(l, i - 1, j), sD = MG_ES(l, i, j);
-    const T pL = MG_PR(l, i - 1, j - 1), pR = MG_PR(l, i - 1, j);
-    const T vxL = MG_VX(l, i, j - 1), vxC = MG_VX(l, i, j), vxR = MG_VX(l, i, j + 1);
-    const T vxU = MG_VX(l, i - 1, j), vxD = MG_VX(l, i + 1, j);
-    const T vyUL = MG_VY(l, i - 1, j), vyUR = MG_VY(l, i - 1, j + 1);
-    const T vyLL = MG_VY(l, i, j), vyLR = MG_VY(l, i, j + 1);
-    const T dp_dx = (pR - pL) / dxC;
-    const T dsxx = T(2) * (nR * (vxR - vxC) / dxR - nL * (vxC - vxL) / dxL) / dxC;
-    const T dsxy = (sD * ((vxD - vxC) / dyD + (vyLR - vyLL) / dxC) -
-                    sU * ((vxC - vxU) / dyU + (vyUR - vyUL) / dxC)) / dyC;
-    const T dR = l.RX[(size_t)(j - 1) * (l.ynum + 1) + (i - 1)] + dp_dx - dsxx - dsxy;
-    if (coef) *coef = T(-2) * (nR / dxR / dxC + nL / dxL / dxC) - (sD / dyD / dyC + sU / dyU / dyC);
-    return dR;
+    return vx_residual_v<T>(GlobalView<T>{l}, i, j, coef);
 }
-
 template <typename T>
 __device__ __forceinline__ T mg_vy_residual(const MGLevelDev<T>& l, int i, int j, T* coef)
 {
-    const T dxL = l.xstp_b[j - 2], dxR = l.xstp_vy[j - 1], dxC = l.xstp_vy[j - 2];
-    const T dyU = l.ystp_b[i - 2], dyD = l.ystp_b[i - 1], dyC = l.ystp_vx[i - 1];
-    const T pD = MG_PR(l, i, j - 1), pU = MG_PR(l, i - 1, j - 1);
-    const T nD = MG_EN(l, i, j - 1), nU = MG_EN(l, i - 1, j - 1);
-    const T sR = MG_ES(l, i, j), sL = MG_ES(l, i, j - 1);
-    const T vyU = MG_VY(l, i - 1, j), vyC = MG_VY(l, i, j), vyD = MG_VY(l, i + 1, j);
-    const T vyR = MG_VY(l, i, j + 1), vyL = MG_VY(l, i, j - 1);
-    const T vxLR = MG_VX(l, i + 1, j), vxUR = MG_VX(l, i, j);
-    const T vxLL = MG_VX(l, i + 1, j - 1), vxUL = MG_VX(l, i, j - 1);
-    const T dp_dy = (pD - pU) / dyC;
-    const T dsyy = T(2) * (nD * (vyD - vyC) / dyD - nU * (vyC - vyU) / dyU) / dyC;
-    const T dsyx = (sR * ((vyR - vyC) / dxR + (vxLR - vxUR) / dyC) -
-                    sL * ((vyC - vyL) / dxL + (vxLL - vxUL) / dyC)) / dxC;
-    const T dR = l.RY[(size_t)(j - 1) * l.ynum + (i - 1)] + dp_dy - dsyy - dsyx;
-    if (coef) *coef = T(-2) * (nD / dyD / dyC + nU / dyU / dyC) - (sR / dxR / dxC + sL / dxL / dxC);
-    return dR;
+    return vy_residual_v<T>(GlobalView<T>{l}, i, j, coef);
 }
-
 template <typename T>
 __device__ __forceinline__ T mg_pr_residual(const MGLevelDev<T>& l, int i, int j)
 {
-    const T dxC = l.xstp_b[j - 1], dyC = l.ystp_b[i - 1];
-    return l.RC[(size_t)(j - 1) * (l.ynum - 1) + (i - 1)] -
-           ((MG_VX(l, i + 1, j + 1) - MG_VX(l, i + 1, j)) / dxC +
-            (MG_VY(l, i + 1, j + 1) - MG_VY(l, i, j + 1)) / dyC);
+    return pr_residual_v<T>(GlobalView<T>{l}, i, j);
 }
 
 // ---------------------------------------------------------------- kernels
@@ -389,6 +441,14 @@ struct MGHierarchy {
     std::vector<void*> allocations;
     T* scratch = nullptr;               // pressure-shift scalar
     bool normalize_finest = true;       // pressure gauge p[1,1] = prnorm on level 0
+    // LDS-tiled smoother plan per level (mg_plan_tiles)
+    struct TilePlan {
+        int ti, tj, nti, ntj, colours;
+        size_t lds;
+        bool single;
+    };
+    std::vector<TilePlan> tiles;
+    bool fast_div = false;              // tiled smoother: FP32 hardware-reciprocal division
     int nlev() const { return (int)lev.size(); }
     ~MGHierarchy();
 };
@@ -411,5 +471,19 @@ template <typename T>
 void mg_zero_solution(MGHierarchy<T>& H, int k, hipStream_t s);
 
 std::vector<int> mg_smoothing_counts(int nlev, int finest, int multiplier, int cap);
+
+// Tiled smoother (ebg_mg_tile.hpp): plan the tiles of every level (a level
+// whose whole window fits the workgroup LDS limit becomes one tile), then a
+// V-cycle with the same structure as mg_vcycle whose smoothing calls run
+// local_sweeps sweeps per tile launch.
+template <typename T>
+int mg_plan_tiles(MGHierarchy<T>& H, int tile, bool allow_single, bool fast_div);
+template <typename T>
+void mg_smoother_tiled(MGHierarchy<T>& H, int k, int nsweeps, int local_sweeps, T theta_s,
+                       T theta_c, T prnorm, const T* prnorm_dev, const T* bc_d, hipStream_t s);
+template <typename T>
+void mg_vcycle_tiled(MGHierarchy<T>& H, const std::vector<int>& smoothing, int local_sweeps,
+                     T theta_s, T theta_c, T relax_v, T relax_p, T prnorm, const T* prnorm_dev,
+                     const T* bc_d, hipStream_t s);
 
 }  // namespace ebg

@@ -112,6 +112,26 @@ int ebg_mg2d_smoother(ebg_ctx* ctx, int level, int nsweeps, double theta_s, doub
     return EBG_OK;
 }
 
+int ebg_mg2d_smoother_tiled(ebg_ctx* ctx, int level, int nsweeps, int local_sweeps, int tile,
+                            double theta_s, double theta_c, double prnorm, const double* bc_d4)
+{
+    MGRef* m = get(ctx);
+    EBG_CHECK(m, EBG_ERR_STATE, "ebg_mg2d_smoother_tiled: no hierarchy");
+    EBG_CHECK(level >= 0 && level < m->H.nlev() && nsweeps >= 0 && local_sweeps >= 1, EBG_ERR_ARG,
+              "ebg_mg2d_smoother_tiled: bad level or sweep count");
+    EBG_CHECK(tile == 0 || (tile >= 4 && tile <= 32), EBG_ERR_ARG,
+              "ebg_mg2d_smoother_tiled: tile must be 0 (whole level) or in [4, 32]");
+    int rc = mg_plan_tiles(m->H, tile == 0 ? 16 : tile, tile == 0, false);
+    if (rc) return rc;
+    EBG_CHECK(tile != 0 || m->H.tiles[level].single, EBG_ERR_ARG,
+              "ebg_mg2d_smoother_tiled: level does not fit one LDS tile");
+    const double one[4] = {1.0, 1.0, 1.0, 1.0};
+    mg_smoother_tiled(m->H, level, nsweeps, local_sweeps, theta_s, theta_c, prnorm,
+                      (const double*)nullptr, bc_d4 ? bc_d4 : one, ctx->stream);
+    EBG_HIP(hipGetLastError());
+    return EBG_OK;
+}
+
 int ebg_mg2d_vcycles(ebg_ctx* ctx, int ncycles, int finest_sweeps, int multiplier, int cap,
                      double theta_s, double theta_c, double relax_v, double relax_p,
                      double prnorm, const double* bc_d4)

@@ -138,6 +138,7 @@ __global__ void vel_bc_cols_kernel(StokesOpDev s, double* vx1, double* vy1)
 // ---- deterministic reductions ----
 constexpr int RED_BLOCKS = 512;
 constexpr int MAXK = 64;
+constexpr int COMPAT_BLOCKS = 256;  // partial sums of the mode-0 compatibility RHS
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -291,15 +292,31 @@ __global__ void set_fixed_kernel(StokesOpDev s, const double* __restrict__ r, MG
 // (1,1) is set to the value that makes the coarse problems compatible:
 // RC(1,1) = -sum_{c != (1,1)} RC_c dx dy / (dx_1 dy_1).
 template <typename T>
-__global__ void __launch_bounds__(1024) compat_rc_kernel(MGLevelDev<T> l)
+__global__ void __launch_bounds__(256) compat_rc_partial_kernel(MGLevelDev<T> l, double* partial)
 {
-    __shared__ double sh[1024];
+    __shared__ double sh[256];
     const int ny1 = l.ynum - 1, n = ny1 * (l.xnum - 1);
     double acc = 0.0;
-    for (int c = 1 + threadIdx.x; c < n; c += blockDim.x) {
+    for (int c = 1 + blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
         const int i = c % ny1, j = c / ny1;
         acc += (double)l.RC[c] * (double)l.xstp_b[j] * (double)l.ystp_b[i];
     }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) compat_rc_finish_kernel(MGLevelDev<T> l, const double* partial,
+                                                               int nparts)
+{
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) acc += partial[k];
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (int off = blockDim.x / 2; off > 0; off >>= 1) {
@@ -346,8 +363,13 @@ struct StokesState {
     DevBuf prnorm_f, prnorm_d;
     DevBuf fix_vx, fix_vy, fix_p;
     bool any_fix_v = false, any_fix_p = false;
-    int mg_key_nlev = -1;
-    bool visc_dirty = true;
+    // per-precision hierarchy keys: levels, tile edge and grid generation it
+    // was built for, and whether the viscosity must be restricted again
+    struct HKey {
+        int nlev = -1, tile = -1;
+        uint64_t grid = 0;
+        bool visc_dirty = true;
+    } key_f, key_d;
     LearnedCorrection* learned = nullptr;
 };
 
@@ -369,21 +391,29 @@ static void precondition(StokesState& st, MGHierarchy<T>& H, T* prn, const ebg_s
     MGLevelDev<T>& l0 = H.lev[0];
     const int nvx = (st.ynum + 1) * (st.xnum + 1);
     hipLaunchKernelGGL(pack_rhs_kernel<T>, dim3(nb(nvx)), dim3(256), 0, s, st.op, r, l0, prn, st.nc);
-    if (st.op.pressure_bc_mode == 0)
-        hipLaunchKernelGGL(compat_rc_kernel<T>, dim3(1), dim3(1024), 0, s, l0);
+    if (st.op.pressure_bc_mode == 0) {  // deterministic two-stage sum
+        hipLaunchKernelGGL(compat_rc_partial_kernel<T>, dim3(COMPAT_BLOCKS), dim3(256), 0, s, l0,
+                           st.partial.as<double>());
+        hipLaunchKernelGGL(compat_rc_finish_kernel<T>, dim3(1), dim3(256), 0, s, l0,
+                           (const double*)st.partial.as<double>(), COMPAT_BLOCKS);
+    }
     mg_zero_solution(H, 0, s);
     if (l0.fix_vx || l0.fix_p)
         hipLaunchKernelGGL(set_fixed_kernel<T>, dim3(nb(st.nc)), dim3(256), 0, s, st.op, r, l0, st.nc);
     const T bcd[4] = {(T)o.bc_d[0], (T)o.bc_d[1], (T)o.bc_d[2], (T)o.bc_d[3]};
-    mg_vcycle(H, sm, (T)o.theta_s, (T)o.theta_c, T(1), T(1), T(0), (const T*)prn, bcd, s);
+    if (o.smoother == 1)
+        mg_vcycle_tiled(H, sm, o.local_sweeps, (T)o.theta_s, (T)o.theta_c, T(1), T(1), T(0),
+                        (const T*)prn, bcd, s);
+    else
+        mg_vcycle(H, sm, (T)o.theta_s, (T)o.theta_c, T(1), T(1), T(0), (const T*)prn, bcd, s);
     hipLaunchKernelGGL(unpack_corr_kernel<T>, dim3(nb(st.nc)), dim3(256), 0, s, st.op, r, l0, z, st.nc);
 }
 
 template <typename T>
 static int ensure_hierarchy(ebg_ctx* ctx, StokesState& st, std::unique_ptr<MGHierarchy<T>>& H,
-                            DevBuf& prn, const ebg_solve_opts& o)
+                            DevBuf& prn, StokesState::HKey& key, const ebg_solve_opts& o)
 {
-    if (!H || st.mg_key_nlev != o.nlevels) {
+    if (!H || key.nlev != o.nlevels || key.grid != ctx->grid.generation) {
         H.reset(new MGHierarchy<T>());
         std::vector<HostLevelGrid> fine{HostLevelGrid{}};
         const Grid2D& g = ctx->grid;
@@ -412,8 +442,15 @@ static int ensure_hierarchy(ebg_ctx* ctx, StokesState& st, std::unique_ptr<MGHie
         int rc = mg_build(*H, (int)g.xnum, (int)g.ynum, xsize, ysize, o.nlevels, &fine, true);
         if (rc) return rc;
         if ((rc = prn.ensure(sizeof(T) * 4))) return rc;
-        st.mg_key_nlev = o.nlevels;
-        st.visc_dirty = true;
+        key.nlev = o.nlevels;
+        key.grid = ctx->grid.generation;
+        key.tile = -1;
+        key.visc_dirty = true;
+    }
+    if (o.smoother == 1 && key.tile != o.tile) {
+        int rc = mg_plan_tiles(*H, o.tile, true, sizeof(T) == 4);
+        if (rc) return rc;
+        key.tile = o.tile;
     }
     return EBG_OK;
 }
@@ -494,17 +531,18 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
     std::vector<int> sm;
     const bool f32 = (o.precision == 1);
     if (f32) {
-        if ((rc = ensure_hierarchy(ctx, st, st.Hf, st.prnorm_f, o))) return rc;
+        if ((rc = ensure_hierarchy(ctx, st, st.Hf, st.prnorm_f, st.key_f, o))) return rc;
         sm = mg_smoothing_counts(st.Hf->nlev(), o.finest_sweeps, o.sweep_multiplier, o.sweep_cap);
-        if (st.visc_dirty) load_viscosity(st, *st.Hf, s);
+        if (st.key_f.visc_dirty) load_viscosity(st, *st.Hf, s);
+        st.key_f.visc_dirty = false;
         attach_masks(st, *st.Hf);
     } else {
-        if ((rc = ensure_hierarchy(ctx, st, st.Hd, st.prnorm_d, o))) return rc;
+        if ((rc = ensure_hierarchy(ctx, st, st.Hd, st.prnorm_d, st.key_d, o))) return rc;
         sm = mg_smoothing_counts(st.Hd->nlev(), o.finest_sweeps, o.sweep_multiplier, o.sweep_cap);
-        if (st.visc_dirty) load_viscosity(st, *st.Hd, s);
+        if (st.key_d.visc_dirty) load_viscosity(st, *st.Hd, s);
+        st.key_d.visc_dirty = false;
         attach_masks(st, *st.Hd);
     }
-    st.visc_dirty = false;
     LayoutPlanar LP{nc};
     double* b = st.b.as<double>();
     double* x = st.x.as<double>();
@@ -669,7 +707,6 @@ int ebg_stokes_set_system(ebg_ctx* ctx, const double* etas0, const double* etan0
     if (st.xnum != xnum || st.ynum != ynum) {
         st.Hf.reset();
         st.Hd.reset();
-        st.mg_key_nlev = -1;
     }
     st.xnum = xnum;
     st.ynum = ynum;
@@ -737,7 +774,8 @@ int ebg_stokes_set_system(ebg_ctx* ctx, const double* etas0, const double* etan0
     op.timestep = timestep;
     op.gravity_y = gravity_y;
     op.stab = iuse_interface_stabilization;
-    st.visc_dirty = true;
+    st.key_f.visc_dirty = true;
+    st.key_d.visc_dirty = true;
     return build_masks(st);
 }
 
@@ -813,6 +851,9 @@ int ebg_stokes_solve_opts_default(ebg_solve_opts* o)
     o->use_initial_guess = 0;
     o->learned = 0;
     for (int k = 0; k < 4; ++k) o->bc_d[k] = 1.0;
+    o->smoother = 1;
+    o->local_sweeps = 2;  // more sweeps per halo refresh let the tile pressures drift
+    o->tile = 16;
     return EBG_OK;
 }
 
@@ -828,6 +869,9 @@ int ebg_stokes_solve(ebg_ctx* ctx, double* S, const ebg_solve_opts* opts, ebg_so
     EBG_CHECK(o.precision == 0 || o.precision == 1, EBG_ERR_ARG, "ebg_stokes_solve: precision must be 0 or 1");
     EBG_CHECK(o.rtol > 0 && o.max_iterations >= 0, EBG_ERR_ARG, "ebg_stokes_solve: bad tolerance");
     EBG_CHECK(o.finest_sweeps >= 1 && o.sweep_multiplier >= 1, EBG_ERR_ARG, "ebg_stokes_solve: bad sweeps");
+    EBG_CHECK(o.smoother == 0 || o.smoother == 1, EBG_ERR_ARG, "ebg_stokes_solve: smoother must be 0 or 1");
+    EBG_CHECK(o.smoother == 0 || (o.local_sweeps >= 1 && o.tile >= 4 && o.tile <= 32), EBG_ERR_ARG,
+              "ebg_stokes_solve: tiled smoother needs local_sweeps >= 1 and tile in [4, 32]");
     EBG_CHECK(o.learned == 0 || st->learned, EBG_ERR_STATE, "ebg_stokes_solve: learned correction not loaded");
     const auto t0 = std::chrono::steady_clock::now();
     const int64_t n = 3 * (int64_t)st->nc;

@@ -51,6 +51,8 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_mg2d_copy_field": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "ebg_mg2d_restrict_viscosity": (c_int, [c_void_p]),
     "ebg_mg2d_smoother": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_double, c_void_p]),
+    "ebg_mg2d_smoother_tiled": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
+                                        c_double, c_void_p]),
     "ebg_mg2d_vcycles": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
                                  c_double, c_double, c_double, c_void_p]),
 }
@@ -82,7 +84,8 @@ class SolveOpts(C.Structure):
                 ("rtol", c_double), ("nlevels", c_int), ("finest_sweeps", c_int),
                 ("sweep_multiplier", c_int), ("sweep_cap", c_int), ("theta_s", c_double),
                 ("theta_c", c_double), ("use_initial_guess", c_int), ("learned", c_int),
-                ("bc_d", c_double * 4)]
+                ("bc_d", c_double * 4), ("smoother", c_int), ("local_sweeps", c_int),
+                ("tile", c_int)]
 
 
 class SolveStats(C.Structure):

@@ -155,6 +155,13 @@ class MultigridGPU:
         check(lib.ebg_mg2d_smoother(self.ctx.handle, level, nsweeps, theta_s, theta_c, prnorm, d),
               "ebg_mg2d_smoother")
 
+    def smoother_tiled(self, level, nsweeps, local_sweeps=1, tile=0, theta_s=0.9, theta_c=0.3,
+                       prnorm=0.0, bc_d=(1, 1, 1, 1)):
+        """The smoother on LDS tiles; tile=0 keeps the whole level in one tile."""
+        d = (C.c_double * 4)(*bc_d)
+        check(lib.ebg_mg2d_smoother_tiled(self.ctx.handle, level, nsweeps, local_sweeps, tile,
+                                          theta_s, theta_c, prnorm, d), "ebg_mg2d_smoother_tiled")
+
     def vcycles(self, ncycles, finest_sweeps=2, multiplier=2, cap=48, theta_s=0.9, theta_c=0.3,
                 relax_v=1.0, relax_p=1.0, prnorm=0.0, bc_d=(1, 1, 1, 1)):
         d = (C.c_double * 4)(*bc_d)

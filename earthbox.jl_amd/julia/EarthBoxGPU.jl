@@ -34,6 +34,9 @@ struct SolveOpts
     use_initial_guess::Cint
     learned::Cint
     bc_d::NTuple{4,Cdouble}
+    smoother::Cint
+    local_sweeps::Cint
+    tile::Cint
 end
 
 struct SolveStats
@@ -137,7 +140,7 @@ function solve_system_gpu!(st::GPUState, model; mixed::Bool=true, rtol::Float64=
                     o[].finest_sweeps, o[].sweep_multiplier, o[].sweep_cap, o[].theta_s,
                     o[].theta_c, 0, 0,
                     (vc.btopx.array[2, 2], vc.bbottomx.array[2, 2], vc.blefty.array[2, 2],
-                     vc.brighty.array[2, 2]))
+                     vc.brighty.array[2, 2]), o[].smoother, o[].local_sweeps, o[].tile)
     stats = Ref{SolveStats}()
     rc = check(ccall((:ebg_stokes_solve, LIB), Cint,
         (Ptr{Cvoid}, Ptr{Cvoid}, Ref{SolveOpts}, Ref{SolveStats}), st.ctx, dS, o, stats),

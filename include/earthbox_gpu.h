@@ -197,6 +197,10 @@ typedef struct {
     int learned;          /* 1: add the learned correction (needs ebg_learned_load) */
     double bc_d[4];       /* preconditioner ghost mirror factors top, bottom (vx), left, right (vy):
                              +1 free slip, -1 no slip */
+    int smoother;         /* 1: LDS-tiled smoother (default), 0: global phased sweeps in the
+                             reference multigrid's order */
+    int local_sweeps;     /* tiled smoother: sweeps per tile launch between halo refreshes */
+    int tile;             /* tiled smoother: tile edge (nodes) on levels that exceed one LDS tile */
 } ebg_solve_opts;
 
 typedef struct {
@@ -236,6 +240,13 @@ int ebg_mg2d_restrict_viscosity(ebg_ctx *ctx);
 /* Smoother.stokes_continuity2d_viscous_smoother! (multigrid/smoother/Smoother.jl:66-99). */
 int ebg_mg2d_smoother(ebg_ctx *ctx, int level, int nsweeps, double theta_s, double theta_c,
                       double prnorm, const double *bc_d4);
+/* The same smoother on LDS tiles (the preconditioner's smoother).  tile 0:
+ * the whole level is one tile and the result equals ebg_mg2d_smoother bit for
+ * bit (error if the level exceeds the LDS limit); tile 4..32: tiles of that
+ * edge in four colours, local_sweeps sweeps per launch with the halo held at
+ * its launch-start values (a block Gauss-Seidel / Schwarz smoother). */
+int ebg_mg2d_smoother_tiled(ebg_ctx *ctx, int level, int nsweeps, int local_sweeps, int tile,
+                            double theta_s, double theta_c, double prnorm, const double *bc_d4);
 /* ncycles V-cycles (vcycle/MultigridVCycle.jl:197-267 without viscosity
  * continuation): SmoothAndRestrict.jl:44-71 then SmoothAndProlongate.jl:60-115. */
 int ebg_mg2d_vcycles(ebg_ctx *ctx, int ncycles, int finest_sweeps, int multiplier, int cap,
