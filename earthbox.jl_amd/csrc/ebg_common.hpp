@@ -76,6 +76,7 @@ struct ebg_ctx {
     void* stokes = nullptr;
     void* scatter = nullptr;
     void* mg_ref = nullptr;
+    void* cnn = nullptr;  // learned preconditioner correction (ebg_cnn_create)
 };
 
 namespace ebg {

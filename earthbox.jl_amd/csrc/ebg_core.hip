@@ -1,5 +1,6 @@
 // Context, memory helpers, error text and grid upload for the C ABI.
 #include "ebg_common.hpp"
+#include "ebg_learned.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -114,6 +115,7 @@ int ebg_ctx_destroy(ebg_ctx* ctx)
     ebg_stokes_release(ctx);
     ebg_scatter_release(ctx);
     ebg_mgref_release(ctx);
+    cnn_release_ctx(ctx);
     grid_release(ctx->grid);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

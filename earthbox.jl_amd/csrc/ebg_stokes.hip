@@ -370,7 +370,8 @@ struct StokesState {
         uint64_t grid = 0;
         bool visc_dirty = true;
     } key_f, key_d;
-    LearnedCorrection* learned = nullptr;
+    // learned correction: candidate vector, judge residuals and acceptance count
+    DevBuf zc, jr0, jr1, judge, pair_e, pair_r, pair_z;
 };
 
 static StokesState* state(ebg_ctx* ctx) { return ctx ? (StokesState*)ctx->stokes : nullptr; }
@@ -516,6 +517,70 @@ static void multidot(StokesState& st, const double* w, const double* V, int64_t 
                        RED_BLOCKS, k, out, accumulate);
 }
 
+// hierarchy of the requested precision, smoothing counts, viscosity and masks
+static int prepare_preconditioner(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, std::vector<int>& sm)
+{
+    hipStream_t s = ctx->stream;
+    int rc;
+    if ((rc = st.partial.ensure((size_t)RED_BLOCKS * MAXK * 8))) return rc;
+    if (o.precision == 1) {
+        if ((rc = ensure_hierarchy(ctx, st, st.Hf, st.prnorm_f, st.key_f, o))) return rc;
+        sm = mg_smoothing_counts(st.Hf->nlev(), o.finest_sweeps, o.sweep_multiplier, o.sweep_cap);
+        if (st.key_f.visc_dirty) load_viscosity(st, *st.Hf, s);
+        st.key_f.visc_dirty = false;
+        attach_masks(st, *st.Hf);
+    } else {
+        if ((rc = ensure_hierarchy(ctx, st, st.Hd, st.prnorm_d, st.key_d, o))) return rc;
+        sm = mg_smoothing_counts(st.Hd->nlev(), o.finest_sweeps, o.sweep_multiplier, o.sweep_cap);
+        if (st.key_d.visc_dirty) load_viscosity(st, *st.Hd, s);
+        st.key_d.visc_dirty = false;
+        attach_masks(st, *st.Hd);
+    }
+    return EBG_OK;
+}
+
+static void apply_preconditioner(StokesState& st, const ebg_solve_opts& o, const std::vector<int>& sm,
+                                 const double* r, double* z, hipStream_t s)
+{
+    if (o.precision == 1) precondition(st, *st.Hf, st.prnorm_f.as<float>(), o, sm, r, z, s);
+    else precondition(st, *st.Hd, st.prnorm_d.as<double>(), o, sm, r, z, s);
+}
+
+// judge[1], judge[2] = ||v - L z||^2, ||v - L zc||^2; keep zc in z when it is
+// smaller and count it in judge[0]
+__global__ void learned_select_kernel(double* __restrict__ z, const double* __restrict__ zc, int64_t n,
+                                      double* __restrict__ judge)
+{
+    const bool take = judge[2] < judge[1];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (take && t < n) z[t] = zc[t];
+    if (t == 0 && take) judge[0] += 1.0;
+}
+
+// The learned correction as a candidate preconditioner output: zc = z + N(...),
+// kept only where it lowers the FP64 residual of the preconditioned direction
+// (otherwise the plain V-cycle output stays).
+static int learned_step(ebg_ctx* ctx, StokesState& st, const double* v, double* z, hipStream_t s)
+{
+    const int nc = st.nc;
+    const int64_t n = 3 * (int64_t)nc;
+    LayoutPlanar LP{nc};
+    int rc = cnn_correct((CnnNet*)ctx->cnn, st.xnum - 1, st.ynum - 1, st.etan.as<double>(), v, z,
+                         st.zc.as<double>(), s);
+    if (rc) return rc;
+    double* jd = st.judge.as<double>();
+    hipLaunchKernelGGL(residual_planar_kernel, dim3(nb(nc)), dim3(256), 0, s, st.op, (const double*)z, v,
+                       st.jr0.as<double>(), LP);
+    hipLaunchKernelGGL(residual_planar_kernel, dim3(nb(nc)), dim3(256), 0, s, st.op,
+                       (const double*)st.zc.as<double>(), v, st.jr1.as<double>(), LP);
+    multidot(st, st.jr0.as<double>(), st.jr0.as<double>(), 0, 1, n, jd + 1, 0, s);
+    multidot(st, st.jr1.as<double>(), st.jr1.as<double>(), 0, 1, n, jd + 2, 0, s);
+    hipLaunchKernelGGL(learned_select_kernel, dim3(nb(n)), dim3(256), 0, s, z, (const double*)st.zc.as<double>(),
+                       n, jd);
+    EBG_HIP(hipGetLastError());
+    return EBG_OK;
+}
+
 static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_solve_stats* stats)
 {
     hipStream_t s = ctx->stream;
@@ -530,19 +595,11 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
         return rc;
     std::vector<int> sm;
     const bool f32 = (o.precision == 1);
-    if (f32) {
-        if ((rc = ensure_hierarchy(ctx, st, st.Hf, st.prnorm_f, st.key_f, o))) return rc;
-        sm = mg_smoothing_counts(st.Hf->nlev(), o.finest_sweeps, o.sweep_multiplier, o.sweep_cap);
-        if (st.key_f.visc_dirty) load_viscosity(st, *st.Hf, s);
-        st.key_f.visc_dirty = false;
-        attach_masks(st, *st.Hf);
-    } else {
-        if ((rc = ensure_hierarchy(ctx, st, st.Hd, st.prnorm_d, st.key_d, o))) return rc;
-        sm = mg_smoothing_counts(st.Hd->nlev(), o.finest_sweeps, o.sweep_multiplier, o.sweep_cap);
-        if (st.key_d.visc_dirty) load_viscosity(st, *st.Hd, s);
-        st.key_d.visc_dirty = false;
-        attach_masks(st, *st.Hd);
-    }
+    if ((rc = prepare_preconditioner(ctx, st, o, sm))) return rc;
+    if (o.learned && ((rc = st.zc.ensure(n * 8)) || (rc = st.jr0.ensure(n * 8)) || (rc = st.jr1.ensure(n * 8)) ||
+                      (rc = st.judge.ensure(64))))
+        return rc;
+    if (o.learned) hipLaunchKernelGGL(fill_kernel<double>, dim3(1), dim3(64), 0, s, st.judge.as<double>(), 8, 0.0);
     LayoutPlanar LP{nc};
     double* b = st.b.as<double>();
     double* x = st.x.as<double>();
@@ -594,9 +651,8 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
         for (; k < m && total < o.max_iterations; ++k) {
             double* vk = V + (int64_t)k * n;
             double* zk = Z + (int64_t)k * n;
-            if (f32) precondition(st, *st.Hf, st.prnorm_f.as<float>(), o, sm, vk, zk, s);
-            else precondition(st, *st.Hd, st.prnorm_d.as<double>(), o, sm, vk, zk, s);
-            if (st.learned && o.learned) learned_apply(st.learned, ctx, vk, zk, s);
+            apply_preconditioner(st, o, sm, vk, zk, s);
+            if (o.learned && (rc = learned_step(ctx, st, vk, zk, s))) return rc;
             hipLaunchKernelGGL((stokes_rows_kernel<false, LayoutPlanar>), dim3(nb(nc)), dim3(256), 0, s,
                                st.op, zk, w, LP);
             // classical Gram-Schmidt, twice (CGS2)
@@ -660,7 +716,15 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
         stats->restarts = restarts;
         stats->relres = relres;
         stats->relres_initial = relres0;
+        stats->learned_accepted = 0;
+        if (o.learned) {
+            double acc = 0;
+            EBG_HIP(hipMemcpyAsync(&acc, st.judge.p, 8, hipMemcpyDeviceToHost, s));
+            EBG_HIP(hipStreamSynchronize(s));
+            stats->learned_accepted = (int)acc;
+        }
     }
+    (void)f32;
     return relres <= o.rtol ? EBG_OK : EBG_NOT_CONVERGED;
 }
 
@@ -674,7 +738,6 @@ int ebg_stokes_release(ebg_ctx* ctx)
 {
     if (ctx && ctx->stokes) {
         StokesState* st = (StokesState*)ctx->stokes;
-        if (st->learned) learned_release(st->learned);
         delete st;
         ctx->stokes = nullptr;
     }
@@ -872,7 +935,9 @@ int ebg_stokes_solve(ebg_ctx* ctx, double* S, const ebg_solve_opts* opts, ebg_so
     EBG_CHECK(o.smoother == 0 || o.smoother == 1, EBG_ERR_ARG, "ebg_stokes_solve: smoother must be 0 or 1");
     EBG_CHECK(o.smoother == 0 || (o.local_sweeps >= 1 && o.tile >= 4 && o.tile <= 32), EBG_ERR_ARG,
               "ebg_stokes_solve: tiled smoother needs local_sweeps >= 1 and tile in [4, 32]");
-    EBG_CHECK(o.learned == 0 || st->learned, EBG_ERR_STATE, "ebg_stokes_solve: learned correction not loaded");
+    EBG_CHECK(o.learned == 0 || o.learned == 1, EBG_ERR_ARG, "ebg_stokes_solve: learned must be 0 or 1");
+    EBG_CHECK(o.learned == 0 || ctx->cnn, EBG_ERR_STATE,
+              "ebg_stokes_solve: learned correction requested but no network (ebg_cnn_create)");
     const auto t0 = std::chrono::steady_clock::now();
     const int64_t n = 3 * (int64_t)st->nc;
     int rc;
@@ -892,6 +957,64 @@ int ebg_stokes_solve(ebg_ctx* ctx, double* S, const ebg_solve_opts* opts, ebg_so
         stats->status = rc;
     }
     return rc;
+}
+
+int ebg_stokes_precondition(ebg_ctx* ctx, const double* r, double* z, const ebg_solve_opts* opts)
+{
+    StokesState* st = state(ctx);
+    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_precondition: no system set");
+    EBG_CHECK(r && z, EBG_ERR_ARG, "ebg_stokes_precondition: NULL vector");
+    ebg_solve_opts o;
+    if (opts) o = *opts;
+    else ebg_stokes_solve_opts_default(&o);
+    EBG_CHECK(o.precision == 0 || o.precision == 1, EBG_ERR_ARG, "ebg_stokes_precondition: bad precision");
+    hipStream_t s = ctx->stream;
+    const int nc = st->nc;
+    const int64_t n = 3 * (int64_t)nc;
+    std::vector<int> sm;
+    int rc;
+    if ((rc = prepare_preconditioner(ctx, *st, o, sm))) return rc;
+    if ((rc = st->pair_r.ensure(n * 8)) || (rc = st->pair_z.ensure(n * 8))) return rc;
+    hipLaunchKernelGGL(interleaved_to_planar_kernel, dim3(nb(nc)), dim3(256), 0, s, r, st->pair_r.as<double>(), nc);
+    apply_preconditioner(*st, o, sm, st->pair_r.as<double>(), st->pair_z.as<double>(), s);
+    hipLaunchKernelGGL(planar_to_interleaved_kernel, dim3(nb(nc)), dim3(256), 0, s,
+                       (const double*)st->pair_z.as<double>(), z, nc);
+    EBG_HIP(hipGetLastError());
+    return EBG_OK;
+}
+
+int ebg_cnn_make_sample(ebg_ctx* ctx, const double* e, const ebg_solve_opts* opts, float* in_img,
+                        float* tgt_img)
+{
+    StokesState* st = state(ctx);
+    EBG_CHECK(st, EBG_ERR_STATE, "ebg_cnn_make_sample: no system set");
+    EBG_CHECK(ctx->cnn, EBG_ERR_STATE, "ebg_cnn_make_sample: no network (ebg_cnn_create)");
+    EBG_CHECK(e && in_img && tgt_img, EBG_ERR_ARG, "ebg_cnn_make_sample: NULL pointer");
+    ebg_solve_opts o;
+    if (opts) o = *opts;
+    else ebg_stokes_solve_opts_default(&o);
+    EBG_CHECK(o.precision == 0 || o.precision == 1, EBG_ERR_ARG, "ebg_cnn_make_sample: bad precision");
+    hipStream_t s = ctx->stream;
+    const int nc = st->nc;
+    const int64_t n = 3 * (int64_t)nc;
+    std::vector<int> sm;
+    int rc;
+    if ((rc = prepare_preconditioner(ctx, *st, o, sm))) return rc;
+    if ((rc = st->pair_e.ensure(n * 8)) || (rc = st->pair_r.ensure(n * 8)) || (rc = st->pair_z.ensure(n * 8)))
+        return rc;
+    LayoutPlanar LP{nc};
+    double* pe = st->pair_e.as<double>();
+    double* pr = st->pair_r.as<double>();
+    double* pz = st->pair_z.as<double>();
+    hipLaunchKernelGGL(interleaved_to_planar_kernel, dim3(nb(nc)), dim3(256), 0, s, e, pe, nc);
+    hipLaunchKernelGGL((stokes_rows_kernel<false, LayoutPlanar>), dim3(nb(nc)), dim3(256), 0, s, st->op,
+                       (const double*)pe, pr, LP);
+    apply_preconditioner(*st, o, sm, pr, pz, s);
+    rc = cnn_pack_sample((CnnNet*)ctx->cnn, st->xnum - 1, st->ynum - 1, st->etan.as<double>(), pr, pz, pe,
+                         in_img, tgt_img, s);
+    if (rc) return rc;
+    EBG_HIP(hipGetLastError());
+    return EBG_OK;
 }
 
 }  // extern "C"

@@ -51,6 +51,17 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_mg2d_copy_field": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "ebg_mg2d_restrict_viscosity": (c_int, [c_void_p]),
     "ebg_mg2d_smoother": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_double, c_void_p]),
+    "ebg_stokes_precondition": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_cnn_create": (c_int, [c_void_p, C.c_uint64]),
+    "ebg_cnn_num_params": (c_int, [c_void_p, c_void_p]),
+    "ebg_cnn_get_params": (c_int, [c_void_p, c_void_p]),
+    "ebg_cnn_set_params": (c_int, [c_void_p, c_void_p]),
+    "ebg_cnn_forward": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "ebg_cnn_loss": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "ebg_cnn_loss_and_grad": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                      c_void_p]),
+    "ebg_cnn_adamw_step": (c_int, [c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]),
+    "ebg_cnn_make_sample": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ebg_mg2d_smoother_tiled": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
                                         c_double, c_void_p]),
     "ebg_mg2d_vcycles": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
@@ -91,7 +102,8 @@ class SolveOpts(C.Structure):
 class SolveStats(C.Structure):
     """ebg_solve_stats (include/earthbox_gpu.h)."""
     _fields_ = [("iterations", c_int), ("restarts", c_int), ("relres", c_double),
-                ("relres_initial", c_double), ("seconds", c_double), ("status", c_int)]
+                ("relres_initial", c_double), ("seconds", c_double), ("status", c_int),
+                ("learned_accepted", c_int)]
 
 
 class EBGError(RuntimeError):

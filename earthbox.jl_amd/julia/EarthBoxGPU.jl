@@ -46,6 +46,7 @@ struct SolveStats
     relres_initial::Cdouble
     seconds::Cdouble
     status::Cint
+    learned_accepted::Cint
 end
 
 check(rc, what) = rc < 0 ? error("$what failed: ",

@@ -210,6 +210,7 @@ typedef struct {
     double relres_initial;
     double seconds;
     int status;
+    int learned_accepted; /* iterations whose learned correction lowered the residual */
 } ebg_solve_stats;
 
 int ebg_stokes_solve_opts_default(ebg_solve_opts *opts);
@@ -218,6 +219,37 @@ int ebg_stokes_solve_opts_default(ebg_solve_opts *opts);
  * S (device, reference order, length 3*(xnum-1)*(ynum-1)) receives the solution.
  * Returns EBG_OK, EBG_NOT_CONVERGED (S holds the last iterate) or an error. */
 int ebg_stokes_solve(ebg_ctx *ctx, double *S, const ebg_solve_opts *opts, ebg_solve_stats *stats);
+
+/* One application of the solver's multigrid preconditioner z = M r (reference
+ * order, device).  No reference counterpart: exposes the preconditioner the
+ * solve uses, for testing and for generating training samples. */
+int ebg_stokes_precondition(ebg_ctx *ctx, const double *r, double *z, const ebg_solve_opts *opts);
+
+/* ---- learned preconditioner correction (no reference counterpart) ----
+ * A fully convolutional network on the cell grid (BF16 weights and
+ * activations, FP32 accumulation) maps (log10 viscosity, r, z = M r) to a
+ * correction of z; with opts.learned the solve uses z + correction wherever
+ * it lowers the FP64 residual ||r - L z||, else the plain V-cycle output.
+ * Images are [batch][H][W][channels] FP32 on the device with H = xnum-1 and
+ * W = ynum-1 (pixel h*W + w = reference cell index). */
+int ebg_cnn_create(ebg_ctx *ctx, uint64_t seed);
+int ebg_cnn_num_params(ebg_ctx *ctx, int64_t *n);
+int ebg_cnn_get_params(ebg_ctx *ctx, float *host);
+int ebg_cnn_set_params(ebg_ctx *ctx, const float *host);
+/* out [B][H][W][3] = network(in [B][H][W][7]) */
+int ebg_cnn_forward(ebg_ctx *ctx, int B, int H, int W, const float *in, float *out);
+/* mean squared error against target [B][H][W][3]; loss_and_grad also leaves
+ * the parameter gradients for ebg_cnn_adamw_step and copies them to grad_host
+ * when it is not NULL */
+int ebg_cnn_loss(ebg_ctx *ctx, int B, int H, int W, const float *in, const float *target, double *loss);
+int ebg_cnn_loss_and_grad(ebg_ctx *ctx, int B, int H, int W, const float *in, const float *target,
+                          double *loss, float *grad_host);
+int ebg_cnn_adamw_step(ebg_ctx *ctx, float lr, float beta1, float beta2, float eps, float weight_decay);
+/* Training sample of the current Stokes system from an error vector e
+ * (reference order, device): r = L e, z = M r, in_img = (log10 etan, r/s_r,
+ * z/s_z), tgt_img = (e - z)/s_z with s the per-component RMS of r and z. */
+int ebg_cnn_make_sample(ebg_ctx *ctx, const double *e, const ebg_solve_opts *opts, float *in_img,
+                        float *tgt_img);
 
 /* ---- the reference's experimental 2-D multigrid (FP64) ----
  * Replaces multigrid/multigrid_solver/MultigridSolver.jl:113-135 and the

@@ -1,0 +1,92 @@
+"""GPU learned correction against the CPU oracle (through the C ABI).
+
+Bars: forward within 2e-2 relative L2 of the float64 oracle with BF16
+emulation; loss within 1e-2 relative and the gradient within 5e-2 relative L2
+(cosine > 0.998); training lowers the loss; a solve with the correction
+converges to the direct solution (the FP64 residual judges every correction).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import oracle.cnn as ocnn
+from earthbox_gpu import learned, workloads
+from earthbox_gpu.device import to_device, to_host
+from earthbox_gpu.stokes import StokesSolverGPU, bc_mirror_factors
+from stokes_cases import CASES, random_case
+from test_oracle_cnn import _params
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("shape", [(2, 37, 21), (1, 64, 48)])
+def test_forward_and_gradient_match_oracle(gpu_ctx, shape):
+    B, H, W = shape
+    print(f"cnn forward/grad shape {shape}", flush=True)
+    net = learned.LearnedCorrection(gpu_ctx, seed=3)
+    p = _params(5).astype(np.float32)
+    net.set_params(p)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((B, H, W, 7)).astype(np.float32)
+    t = rng.standard_normal((B, H, W, 3)).astype(np.float32)
+    xd, td = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
+    out = net.forward(xd).cpu().numpy()
+    ref = ocnn.forward(p, x)
+    assert _rel(out, ref) < 2e-2, _rel(out, ref)
+    loss, g = net.loss_and_grad(xd, td, host_grad=True)
+    rloss, rg = ocnn.loss_and_grad(p, x, t)
+    print(f"loss {loss} oracle {rloss} grad rel {_rel(g, rg)}", flush=True)
+    assert abs(loss - rloss) <= 1e-2 * abs(rloss)
+    assert _rel(g, rg) < 5e-2
+    assert float(np.dot(g, rg) / (np.linalg.norm(g) * np.linalg.norm(rg))) > 0.998
+    # padded channels carry no gradient
+    lay, _ = learned.param_layout()
+    w0, _, cin, cout, cir, _, _, _ = lay[0]
+    G0 = g[w0:w0 + 9 * cin * cout].reshape(9, cout, cin)
+    assert np.all(G0[:, :, cir:] == 0)
+
+
+def test_training_lowers_loss(gpu_ctx):
+    print("cnn training on plain-multigrid samples", flush=True)
+    prob = workloads.layered_random(129, 65, 1.5e6 * 128 / 1000, 6.0e5 * 64 / 500, seed=40)
+    sol = StokesSolverGPU(gpu_ctx)
+    sol.set_system(prob)
+    net = learned.LearnedCorrection(gpu_ctx, seed=1)
+    rng = np.random.default_rng(0)
+    S, _ = sol.solve_system(precision=1, rtol=1e-6, bc_d=bc_mirror_factors(prob))
+    scales = learned.component_rms(to_host(S))
+    H, W = 128, 64
+    X = torch.empty((16, H, W, 7), dtype=torch.float32, device="cuda")
+    Y = torch.empty((16, H, W, 3), dtype=torch.float32, device="cuda")
+    for k in range(16):
+        e = to_device(learned.smooth_error(prob.grid, rng, scales))
+        net.make_sample(sol, e, X[k], Y[k], bc_d=bc_mirror_factors(prob))
+    assert torch.isfinite(X).all() and torch.isfinite(Y).all()
+    l0 = net.loss(X, Y)
+    hist = learned.train(net, X, Y, steps=80, batch=8, lr=2e-3, log=None)
+    l1 = net.loss(X, Y)
+    print(f"loss {l0} -> {l1}", flush=True)
+    assert l1 < 0.8 * l0
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[6]])
+def test_learned_solve_matches_direct(gpu_ctx, case):
+    print(f"learned solve case {case}", flush=True)
+    pr = random_case(**case)
+    Li, Lj, Lv, R = oracle.stokes_build(pr)
+    Sd = oracle.direct_solve(pr.N, Li, Lj, Lv, R)
+    sol = StokesSolverGPU(gpu_ctx)
+    sol.set_system(pr)
+    net = learned.LearnedCorrection(gpu_ctx, seed=7)
+    net.set_params(_params(8).astype(np.float32))  # untrained but non-zero correction
+    S, st = sol.solve_system(precision=1, rtol=1e-12, max_iterations=3000, learned=1,
+                             bc_d=bc_mirror_factors(pr))
+    print(st, flush=True)
+    assert st["converged"], st
+    Sg = to_host(S)
+    assert np.linalg.norm(Sg - Sd) <= 1e-8 * np.linalg.norm(Sd)
