@@ -416,14 +416,20 @@ __global__ void __launch_bounds__(256) rms6_partial_kernel(const double* __restr
     if (threadIdx.x < 6) part[(size_t)blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
+// One scale for all of r and one for all of z (RMS over the three
+// components): the components keep their relative sizes, so a component that
+// is zero (the x-momentum and continuity parts of a body-force right-hand
+// side) stays zero instead of being blown up to unit size.
 __global__ void rms6_finish_kernel(const double* __restrict__ part, int nb, int64_t n, double* __restrict__ sc)
 {
     const int k = threadIdx.x;
-    if (k >= 6) return;
+    if (k >= 2) return;
     double s = 0.0;
-    for (int b = 0; b < nb; ++b) s += part[(size_t)b * 6 + k];
-    const double rms = std::sqrt(s / (double)n);
-    sc[k] = (rms > 0.0 && rms < 1e300) ? rms : 1.0;
+    for (int b = 0; b < nb; ++b)
+        for (int c = 0; c < 3; ++c) s += part[(size_t)b * 6 + 3 * k + c];
+    const double rms = std::sqrt(s / (3.0 * (double)n));
+    const double v = (rms > 0.0 && rms < 1e300) ? rms : 1.0;
+    for (int c = 0; c < 3; ++c) sc[3 * k + c] = v;
 }
 
 // network input [p][7] (FP32) or [p][16] (BF16, zero padded)

@@ -3,6 +3,7 @@
 // activations with FP32 accumulation on the MFMA units) that maps
 // (log viscosity, residual r, V-cycle output z) on the cell grid to a
 // correction of z.  Training and inference kernels are in ebg_learned.hip.
+// Inputs are scaled by the RMS of all of r and of all of z (s_r, s_z).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -23,8 +24,8 @@ int64_t cnn_num_params(const CnnNet* net);
 
 // Preconditioner correction on one image of H x W cells (H = xnum-1 x-columns,
 // W = ynum-1 rows, pixel index = reference cell index): zc = z + s_z * N(...)
-// where N sees (log10 etan, r / s_r, z / s_z) per component and s are the
-// component RMS values.  r, z, zc are planar FP64 vectors (3 planes of H*W).
+// where N sees (log10 etan, r / s_r, z / s_z) and s_r, s_z are the RMS
+// values of r and z over all three components.  r, z, zc are planar FP64 vectors (3 planes of H*W).
 int cnn_correct(CnnNet* net, int H, int W, const double* etan, const double* r, const double* z,
                 double* zc, hipStream_t s);
 

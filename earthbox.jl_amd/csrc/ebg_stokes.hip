@@ -560,7 +560,8 @@ __global__ void learned_select_kernel(double* __restrict__ z, const double* __re
 // The learned correction as a candidate preconditioner output: zc = z + N(...),
 // kept only where it lowers the FP64 residual of the preconditioned direction
 // (otherwise the plain V-cycle output stays).
-static int learned_step(ebg_ctx* ctx, StokesState& st, const double* v, double* z, hipStream_t s)
+static int learned_step(ebg_ctx* ctx, StokesState& st, const double* v, double* z, bool judge,
+                        hipStream_t s)
 {
     const int nc = st.nc;
     const int64_t n = 3 * (int64_t)nc;
@@ -568,6 +569,10 @@ static int learned_step(ebg_ctx* ctx, StokesState& st, const double* v, double* 
     int rc = cnn_correct((CnnNet*)ctx->cnn, st.xnum - 1, st.ynum - 1, st.etan.as<double>(), v, z,
                          st.zc.as<double>(), s);
     if (rc) return rc;
+    if (!judge) {
+        EBG_HIP(hipMemcpyAsync(z, st.zc.p, n * 8, hipMemcpyDeviceToDevice, s));
+        return EBG_OK;
+    }
     double* jd = st.judge.as<double>();
     hipLaunchKernelGGL(residual_planar_kernel, dim3(nb(nc)), dim3(256), 0, s, st.op, (const double*)z, v,
                        st.jr0.as<double>(), LP);
@@ -581,7 +586,9 @@ static int learned_step(ebg_ctx* ctx, StokesState& st, const double* v, double* 
     return EBG_OK;
 }
 
-static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_solve_stats* stats)
+// rhs: right-hand side in planar layout, or NULL for the system's R
+static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_solve_stats* stats,
+                  const double* rhs)
 {
     hipStream_t s = ctx->stream;
     const int nc = st.nc;
@@ -608,8 +615,11 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
     double* V = st.V.as<double>();
     double* Z = st.Z.as<double>();
     double* hd = st.hdev.as<double>();
-    hipLaunchKernelGGL((stokes_rows_kernel<true, LayoutPlanar>), dim3(nb(nc)), dim3(256), 0, s, st.op,
-                       (const double*)nullptr, b, LP);
+    if (rhs)
+        EBG_HIP(hipMemcpyAsync(b, rhs, n * 8, hipMemcpyDeviceToDevice, s));
+    else
+        hipLaunchKernelGGL((stokes_rows_kernel<true, LayoutPlanar>), dim3(nb(nc)), dim3(256), 0, s, st.op,
+                           (const double*)nullptr, b, LP);
     auto norm = [&](const double* v, double* out) {
         multidot(st, v, v, 0, 1, n, out, 0, s);
     };
@@ -626,7 +636,7 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
         set_error("ebg_stokes_solve: right-hand side is not finite");
         return EBG_ERR_NUMERIC;
     }
-    int total = 0, restarts = 0;
+    int total = 0, restarts = 0, learned_dirs = 0;
     double relres = 1.0, relres0 = -1.0;
     while (true) {
         hipLaunchKernelGGL(residual_planar_kernel, dim3(nb(nc)), dim3(256), 0, s, st.op, x, b, r, LP);
@@ -647,24 +657,48 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
         std::fill(g.begin(), g.end(), 0.0);
         g[0] = beta;
         int k = 0;
-        bool done = false;
+        bool done = false, pending = false;
         for (; k < m && total < o.max_iterations; ++k) {
             double* vk = V + (int64_t)k * n;
             double* zk = Z + (int64_t)k * n;
-            apply_preconditioner(st, o, sm, vk, zk, s);
-            if (o.learned && (rc = learned_step(ctx, st, vk, zk, s))) return rc;
+            const bool learned_dir = pending;
+            if (pending) {
+                // learned 3: the correction of the previous V-cycle output is
+                // an extra flexible direction (any z_k keeps A Z = V H valid)
+                rc = cnn_correct((CnnNet*)ctx->cnn, st.xnum - 1, st.ynum - 1, st.etan.as<double>(),
+                                 vk - n, zk - n, zk, s);
+                if (rc) return rc;
+                pending = false;
+            } else {
+                apply_preconditioner(st, o, sm, vk, zk, s);
+                if ((o.learned == 1 || o.learned == 2) &&
+                    (rc = learned_step(ctx, st, vk, zk, o.learned == 1, s)))
+                    return rc;
+                pending = (o.learned == 3);
+            }
             hipLaunchKernelGGL((stokes_rows_kernel<false, LayoutPlanar>), dim3(nb(nc)), dim3(256), 0, s,
                                st.op, zk, w, LP);
+            if (learned_dir) norm(w, st.nrm.as<double>() + 2);
             // classical Gram-Schmidt, twice (CGS2)
             multidot(st, w, V, n, k + 1, n, hd, 0, s);
             hipLaunchKernelGGL(update_kernel, dim3(nb(n)), dim3(256), 0, s, w, V, n, hd, k + 1, n);
             multidot(st, w, V, n, k + 1, n, hd + MAXK, 0, s);
             hipLaunchKernelGGL(update_kernel, dim3(nb(n)), dim3(256), 0, s, w, V, n, hd + MAXK, k + 1, n);
             norm(w, st.nrm.as<double>());
-            double hbuf[2 * MAXK], wn2;
+            double hbuf[2 * MAXK], wn[3];
             EBG_HIP(hipMemcpyAsync(hbuf, hd, sizeof(double) * 2 * MAXK, hipMemcpyDeviceToHost, s));
-            EBG_HIP(hipMemcpyAsync(&wn2, st.nrm.p, 8, hipMemcpyDeviceToHost, s));
+            EBG_HIP(hipMemcpyAsync(wn, st.nrm.p, sizeof(wn), hipMemcpyDeviceToHost, s));
             EBG_HIP(hipStreamSynchronize(s));
+            const double wn2 = wn[0];
+            // a learned direction that lies (almost) in the current space would
+            // turn rounding noise into the next basis vector: drop it
+            if (learned_dir) {
+                if (!(wn2 > 1e-4 * wn[2])) {
+                    --k;
+                    continue;
+                }
+                ++learned_dirs;
+            }
             for (int q = 0; q <= k; ++q) H[(size_t)k * (m + 1) + q] = hbuf[q] + hbuf[MAXK + q];
             const double hn = std::sqrt(wn2);
             H[(size_t)k * (m + 1) + k + 1] = hn;
@@ -716,8 +750,8 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
         stats->restarts = restarts;
         stats->relres = relres;
         stats->relres_initial = relres0;
-        stats->learned_accepted = 0;
-        if (o.learned) {
+        stats->learned_accepted = learned_dirs;
+        if (o.learned == 1) {
             double acc = 0;
             EBG_HIP(hipMemcpyAsync(&acc, st.judge.p, 8, hipMemcpyDeviceToHost, s));
             EBG_HIP(hipStreamSynchronize(s));
@@ -920,7 +954,23 @@ int ebg_stokes_solve_opts_default(ebg_solve_opts* o)
     return EBG_OK;
 }
 
+static int solve_impl(ebg_ctx* ctx, const double* R, double* S, const ebg_solve_opts* opts,
+                      ebg_solve_stats* stats);
+
 int ebg_stokes_solve(ebg_ctx* ctx, double* S, const ebg_solve_opts* opts, ebg_solve_stats* stats)
+{
+    return solve_impl(ctx, nullptr, S, opts, stats);
+}
+
+int ebg_stokes_solve_rhs(ebg_ctx* ctx, const double* R, double* S, const ebg_solve_opts* opts,
+                         ebg_solve_stats* stats)
+{
+    EBG_CHECK(R, EBG_ERR_ARG, "ebg_stokes_solve_rhs: NULL R");
+    return solve_impl(ctx, R, S, opts, stats);
+}
+
+static int solve_impl(ebg_ctx* ctx, const double* R, double* S, const ebg_solve_opts* opts,
+                      ebg_solve_stats* stats)
 {
     StokesState* st = state(ctx);
     EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_solve: no system set");
@@ -935,7 +985,7 @@ int ebg_stokes_solve(ebg_ctx* ctx, double* S, const ebg_solve_opts* opts, ebg_so
     EBG_CHECK(o.smoother == 0 || o.smoother == 1, EBG_ERR_ARG, "ebg_stokes_solve: smoother must be 0 or 1");
     EBG_CHECK(o.smoother == 0 || (o.local_sweeps >= 1 && o.tile >= 4 && o.tile <= 32), EBG_ERR_ARG,
               "ebg_stokes_solve: tiled smoother needs local_sweeps >= 1 and tile in [4, 32]");
-    EBG_CHECK(o.learned == 0 || o.learned == 1, EBG_ERR_ARG, "ebg_stokes_solve: learned must be 0 or 1");
+    EBG_CHECK(o.learned >= 0 && o.learned <= 3, EBG_ERR_ARG, "ebg_stokes_solve: learned must be 0..3");
     EBG_CHECK(o.learned == 0 || ctx->cnn, EBG_ERR_STATE,
               "ebg_stokes_solve: learned correction requested but no network (ebg_cnn_create)");
     const auto t0 = std::chrono::steady_clock::now();
@@ -947,7 +997,14 @@ int ebg_stokes_solve(ebg_ctx* ctx, double* S, const ebg_solve_opts* opts, ebg_so
                            st->x.as<double>(), st->nc);
     else
         hipLaunchKernelGGL(fill_kernel<double>, dim3(nb(n)), dim3(256), 0, ctx->stream, st->x.as<double>(), n, 0.0);
-    rc = fgmres(ctx, *st, o, stats);
+    const double* rhs = nullptr;
+    if (R) {
+        if ((rc = st->pair_r.ensure(n * 8))) return rc;
+        hipLaunchKernelGGL(interleaved_to_planar_kernel, dim3(nb(st->nc)), dim3(256), 0, ctx->stream, R,
+                           st->pair_r.as<double>(), st->nc);
+        rhs = st->pair_r.as<double>();
+    }
+    rc = fgmres(ctx, *st, o, stats, rhs);
     if (rc < 0) return rc;
     hipLaunchKernelGGL(planar_to_interleaved_kernel, dim3(nb(st->nc)), dim3(256), 0, ctx->stream,
                        st->x.as<double>(), S, st->nc);

@@ -52,6 +52,7 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_mg2d_restrict_viscosity": (c_int, [c_void_p]),
     "ebg_mg2d_smoother": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_double, c_void_p]),
     "ebg_stokes_precondition": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_stokes_solve_rhs": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ebg_cnn_create": (c_int, [c_void_p, C.c_uint64]),
     "ebg_cnn_num_params": (c_int, [c_void_p, c_void_p]),
     "ebg_cnn_get_params": (c_int, [c_void_p, c_void_p]),
@@ -115,6 +116,11 @@ def _load() -> C.CDLL:
         raise ImportError(
             f"{LIB_PATH} is not built; run __graft_entry__.build() "
             "(or python earthbox.jl_amd/build_lib.py) first")
+    # torch ships its own HIP runtime.  Loading it first lets the library bind
+    # to that same runtime (same soname); loading the library first would map
+    # a second HIP/HSA runtime into the process, and the one initialised
+    # second sees no device.
+    import torch  # noqa: F401
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)  # AttributeError if the export is missing

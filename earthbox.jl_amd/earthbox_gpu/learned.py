@@ -9,7 +9,7 @@ learned=1), so the plain V-cycle is the fallback and the FP64 residual the judge
 
 Training pairs come from the plain preconditioner itself: for an error field e
 the library forms r = L e and z = M r on the device (ebg_cnn_make_sample), and
-the target is (e - z) scaled per component.  Training (forward, backward,
+the target is (e - z) / RMS(z).  Training (forward, backward,
 AdamW) runs entirely in the library's kernels; torch only holds device memory.
 """
 from __future__ import annotations
@@ -189,6 +189,53 @@ def build_dataset(ctx, net: LearnedCorrection, field_seeds, samples_per_field: i
             X.append(x)
             Y.append(y)
         log(f"field seed {fs}: solve iterations {st['iterations']}, samples {samples_per_field}")
+    return torch.cat(X), torch.cat(Y)
+
+
+def krylov_vectors(sol, nvec: int, **opts):
+    """The first nvec orthonormal Krylov vectors v_k of the flexible GMRES
+    iteration with the multigrid preconditioner (the vectors the
+    preconditioner is applied to), built from the library's operator and
+    preconditioner; torch only does the vector algebra."""
+    b = sol.rhs()
+    V = [b / torch.linalg.vector_norm(b)]
+    for _ in range(nvec - 1):
+        w = sol.apply(sol.precondition(V[-1], **opts))
+        for _ in range(2):  # classical Gram-Schmidt, twice
+            for v in V:
+                w -= torch.dot(w, v) * v
+        V.append(w / torch.linalg.vector_norm(w))
+    return V
+
+
+def build_krylov_dataset(ctx, net: LearnedCorrection, field_seeds, nvec: int, crops_per_sample: int,
+                         crop=(128, 64), xnum=1001, ynum=501, seed=0, log=print, **opts):
+    """Training crops whose inputs are the preconditioner's actual inputs: for
+    each layered random field, the first nvec Krylov vectors v_k; the exact
+    error e_k = L^-1 v_k comes from a tight solve with v_k as right-hand side,
+    and ebg_cnn_make_sample forms (L e_k ~ v_k, z = M v_k, e_k - z)."""
+    from . import workloads
+    from .stokes import StokesSolverGPU, bc_mirror_factors
+    rng = np.random.default_rng(seed)
+    X, Y = [], []
+    sol = StokesSolverGPU(ctx)
+    H, W = xnum - 1, ynum - 1
+    in_img = torch.empty((H, W, N_IN), dtype=torch.float32, device="cuda")
+    tgt_img = torch.empty((H, W, N_OUT), dtype=torch.float32, device="cuda")
+    for fs in field_seeds:
+        prob = workloads.layered_random(xnum, ynum, 1.5e6 * (xnum - 1) / 1000, 6.0e5 * (ynum - 1) / 500,
+                                        seed=int(fs))
+        sol.set_system(prob)
+        o = dict(opts, bc_d=bc_mirror_factors(prob))
+        its = []
+        for v in krylov_vectors(sol, nvec, **o):
+            e, st = sol.solve_rhs(v, rtol=1e-9, max_iterations=600, **o)
+            its.append(st["iterations"])
+            net.make_sample(sol, e, in_img, tgt_img, **o)
+            x, y = random_crops(in_img, tgt_img, crops_per_sample, crop[0], crop[1], rng)
+            X.append(x)
+            Y.append(y)
+        log(f"field seed {fs}: {nvec} Krylov samples, solve iterations {min(its)}-{max(its)}")
     return torch.cat(X), torch.cat(Y)
 
 

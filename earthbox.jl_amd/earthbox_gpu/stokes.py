@@ -106,6 +106,27 @@ class StokesSolverGPU:
         stats["converged"] = (rc == 0)
         return S, stats
 
+    def solve_rhs(self, R: torch.Tensor, S: torch.Tensor | None = None, **opts):
+        """solve_system with a caller right-hand side R (reference order)."""
+        S = self.new_vector() if S is None else S
+        o = default_opts(**opts)
+        st = SolveStats()
+        rc = check(lib.ebg_stokes_solve_rhs(self.ctx.handle, ptr(R, np.float64, self.prob.N),
+                                            ptr(S, np.float64, self.prob.N), C.byref(o), C.byref(st)),
+                   "ebg_stokes_solve_rhs")
+        stats = {k: getattr(st, k) for k, _ in SolveStats._fields_}
+        stats["converged"] = (rc == 0)
+        return S, stats
+
+    def precondition(self, r: torch.Tensor, out: torch.Tensor | None = None, **opts) -> torch.Tensor:
+        """One application z = M r of the solve's multigrid preconditioner."""
+        out = self.new_vector() if out is None else out
+        o = default_opts(**opts)
+        check(lib.ebg_stokes_precondition(self.ctx.handle, ptr(r, np.float64, self.prob.N),
+                                          ptr(out, np.float64, self.prob.N), C.byref(o)),
+              "ebg_stokes_precondition")
+        return out
+
     def process_stokes_solution(self, S: torch.Tensor):
         g = self.prob.grid
         vx1 = empty((g.ynum + 1, g.xnum), np.float64)

@@ -194,7 +194,9 @@ typedef struct {
     double theta_s;       /* velocity relaxation (relax_stokes) */
     double theta_c;       /* pressure relaxation (relax_continuity) */
     int use_initial_guess;/* 1: start from S, 0: start from zero */
-    int learned;          /* 1: add the learned correction (needs ebg_learned_load) */
+    int learned;          /* learned correction (needs ebg_cnn_create): 0 off; 1 use z + N where it
+                             lowers the FP64 residual, else the V-cycle z; 2 always z + N;
+                             3 z + N as an extra flexible Krylov direction after each V-cycle */
     double bc_d[4];       /* preconditioner ghost mirror factors top, bottom (vx), left, right (vy):
                              +1 free slip, -1 no slip */
     int smoother;         /* 1: LDS-tiled smoother (default), 0: global phased sweeps in the
@@ -210,7 +212,8 @@ typedef struct {
     double relres_initial;
     double seconds;
     int status;
-    int learned_accepted; /* iterations whose learned correction lowered the residual */
+    int learned_accepted; /* learned 1: corrections kept by the residual judge;
+                             learned 3: learned Krylov directions */
 } ebg_solve_stats;
 
 int ebg_stokes_solve_opts_default(ebg_solve_opts *opts);
@@ -219,6 +222,11 @@ int ebg_stokes_solve_opts_default(ebg_solve_opts *opts);
  * S (device, reference order, length 3*(xnum-1)*(ynum-1)) receives the solution.
  * Returns EBG_OK, EBG_NOT_CONVERGED (S holds the last iterate) or an error. */
 int ebg_stokes_solve(ebg_ctx *ctx, double *S, const ebg_solve_opts *opts, ebg_solve_stats *stats);
+/* The same solve with a caller right-hand side R (reference order, device)
+ * instead of the system's; no reference counterpart (used to build training
+ * samples e = L^-1 v for the learned correction). */
+int ebg_stokes_solve_rhs(ebg_ctx *ctx, const double *R, double *S, const ebg_solve_opts *opts,
+                         ebg_solve_stats *stats);
 
 /* One application of the solver's multigrid preconditioner z = M r (reference
  * order, device).  No reference counterpart: exposes the preconditioner the
