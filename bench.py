@@ -24,6 +24,11 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 RK4_BYTES_PER_MARKER = 32.0  # read x, y; write x, y (FP64)
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+# workload (18M markers, 1001 x 501 nodes), each counter scaled by its reading
+# for a plain 512 MiB copy: 385.4 MB read + 288.0 MB written.
+RK4_MEASURED_TRAFFIC_BYTES = 673.4e6
+RK4_TRAFFIC_WORKLOAD = (1001, 501, 6)
 
 
 def parse():
@@ -191,7 +196,8 @@ def main():
         "config": {"workload": f"rk4_advect {nx}x{ny} nodes, {nm}x{nm} markers/cell ({n} markers)",
                    "markers": n, "xnum": nx, "ynum": ny},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (RK4_MEASURED_TRAFFIC_BYTES if (nx, ny, nm) == RK4_TRAFFIC_WORKLOAD else None),
                      "kernel": "rk4_advect_kernel", "kernel_ms": kern_ms,
                      "bytes_per_unit": RK4_BYTES_PER_MARKER},
     }

@@ -333,6 +333,131 @@ __global__ void to_precision_kernel(const double* __restrict__ a, T* __restrict_
     if (t < n) b[t] = (T)a[t];
 }
 
+
+// compute_stokes_residuals! (solver/residuals/StokesResiduals.jl:94-137):
+// XStokes.jl:66-117, YStokes.jl:59-112, Continuity.jl:53-84 per node, same
+// expression order.  One thread per (i, j) of the (ynum+1) x (xnum+1) index
+// space; every node of each output array is written (0 where the reference's
+// zeroed array is not updated).
+#define GR_M(a, ld, i, j) (a)[((size_t)(j) - 1) * (size_t)(ld) + (size_t)((i) - 1)]
+__global__ void __launch_bounds__(256) stokes_grid_residuals_kernel(StokesOpDev s, const double* __restrict__ vx1,
+                                                                    const double* __restrict__ vy1,
+                                                                    const double* __restrict__ pr1,
+                                                                    double* __restrict__ resx,
+                                                                    double* __restrict__ resy,
+                                                                    double* __restrict__ resc)
+{
+    const int xnum = s.xnum, ynum = s.ynum;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)(ynum + 1) * (xnum + 1)) return;
+    const int i = 1 + (int)(t % (ynum + 1)), j = 1 + (int)(t / (ynum + 1));
+    const int lx = ynum + 1, ly = ynum, lp = ynum - 1, le = ynum, ln = ynum - 1;
+    if (j < xnum + 1) {
+        double r = 0.0;
+        if ((j != s.zx_j || i < s.zx_i0 || i > s.zx_i1) && !(i == 1 || i == ynum + 1 || j == 1 || j == xnum)) {
+            r = GR_M(s.RX, lx, i, j) -
+                (2.0 * (GR_M(s.etan, ln, i - 1, j) * (GR_M(vx1, lx, i, j + 1) - GR_M(vx1, lx, i, j)) / s.xstp[j - 1] -
+                        GR_M(s.etan, ln, i - 1, j - 1) * (GR_M(vx1, lx, i, j) - GR_M(vx1, lx, i, j - 1)) / s.xstp[j - 2]) -
+                 (GR_M(pr1, lp, i - 1, j) - GR_M(pr1, lp, i - 1, j - 1))) /
+                    s.xstpc[j - 1];
+            r = r - (GR_M(s.etas, le, i, j) * ((GR_M(vx1, lx, i + 1, j) - GR_M(vx1, lx, i, j)) / s.ystpc[i - 1] +
+                                               (GR_M(vy1, ly, i, j + 1) - GR_M(vy1, ly, i, j)) / s.xstpc[j - 1]) -
+                     GR_M(s.etas, le, i - 1, j) * ((GR_M(vx1, lx, i, j) - GR_M(vx1, lx, i - 1, j)) / s.ystpc[i - 2] +
+                                                   (GR_M(vy1, ly, i - 1, j + 1) - GR_M(vy1, ly, i - 1, j)) / s.xstpc[j - 1])) /
+                        s.ystp[i - 2];
+        }
+        GR_M(resx, lx, i, j) = r;
+    }
+    if (i < ynum + 1) {
+        double r = 0.0;
+        if ((j != s.zy_j || i < s.zy_i0 || i > s.zy_i1) && !(i == 1 || i == ynum || j == 1 || j == xnum + 1)) {
+            r = GR_M(s.RY, ly, i, j) -
+                (2.0 * (GR_M(s.etan, ln, i, j - 1) * (GR_M(vy1, ly, i + 1, j) - GR_M(vy1, ly, i, j)) / s.ystp[i - 1] -
+                        GR_M(s.etan, ln, i - 1, j - 1) * (GR_M(vy1, ly, i, j) - GR_M(vy1, ly, i - 1, j)) / s.ystp[i - 2]) -
+                 (GR_M(pr1, lp, i, j - 1) - GR_M(pr1, lp, i - 1, j - 1))) /
+                    s.ystpc[i - 1];
+            r = r - (GR_M(s.etas, le, i, j) * ((GR_M(vy1, ly, i, j + 1) - GR_M(vy1, ly, i, j)) / s.xstpc[j - 1] +
+                                               (GR_M(vx1, lx, i + 1, j) - GR_M(vx1, lx, i, j)) / s.ystpc[i - 1]) -
+                     GR_M(s.etas, le, i, j - 1) * ((GR_M(vy1, ly, i, j) - GR_M(vy1, ly, i, j - 1)) / s.xstpc[j - 2] +
+                                                   (GR_M(vx1, lx, i + 1, j - 1) - GR_M(vx1, lx, i, j - 1)) / s.ystpc[i - 1])) /
+                        s.xstp[j - 2];
+        }
+        GR_M(resy, ly, i, j) = r;
+    }
+    if (i < ynum && j < xnum)
+        GR_M(resc, lp, i, j) = GR_M(s.RC, lp, i, j) -
+                               ((GR_M(vx1, lx, i + 1, j + 1) - GR_M(vx1, lx, i + 1, j)) / s.xstp[j - 1] +
+                                (GR_M(vy1, ly, i + 1, j + 1) - GR_M(vy1, ly, i, j + 1)) / s.ystp[i - 1]);
+}
+
+// Change norms of (v, v_old) over up to 4 segments (MathTools.jl:64-84):
+// part[block][4] = sum (v_old - v)^2, sum v^2, max |v_old - v|, max |v|
+struct NormSegs {
+    int nseg;
+    int64_t n[4];
+    const double* v[4];
+    const double* vo[4];
+};
+__global__ void __launch_bounds__(256) change_norms_kernel(NormSegs g, double* __restrict__ part)
+{
+    __shared__ double sh[4][256];
+    double sd = 0.0, sv = 0.0, md = 0.0, mv = 0.0;
+    for (int q = 0; q < g.nseg; ++q)
+        for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < g.n[q];
+             k += (int64_t)gridDim.x * blockDim.x) {
+            const double v = g.v[q][k];
+            const double d = fabs((g.vo[q] ? g.vo[q][k] : 0.0) - v);
+            sd += d * d;
+            sv += v * v;
+            md = fmax(md, d);
+            mv = fmax(mv, fabs(v));
+        }
+    sh[0][threadIdx.x] = sd;
+    sh[1][threadIdx.x] = sv;
+    sh[2][threadIdx.x] = md;
+    sh[3][threadIdx.x] = mv;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+            sh[2][threadIdx.x] = fmax(sh[2][threadIdx.x], sh[2][threadIdx.x + o]);
+            sh[3][threadIdx.x] = fmax(sh[3][threadIdx.x], sh[3][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[(size_t)blockIdx.x * 4 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// max and min of two arrays: part[block][4] = max a, min a, max b, min b
+__global__ void __launch_bounds__(256) maxmin2_kernel(const double* __restrict__ a, int64_t na,
+                                                      const double* __restrict__ b, int64_t nb,
+                                                      double* __restrict__ part)
+{
+    __shared__ double sh[4][256];
+    double r[4] = {-INFINITY, INFINITY, -INFINITY, INFINITY};
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < na; k += (int64_t)gridDim.x * blockDim.x) {
+        r[0] = fmax(r[0], a[k]);
+        r[1] = fmin(r[1], a[k]);
+    }
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nb; k += (int64_t)gridDim.x * blockDim.x) {
+        r[2] = fmax(r[2], b[k]);
+        r[3] = fmin(r[3], b[k]);
+    }
+    for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] = r[q];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sh[0][threadIdx.x] = fmax(sh[0][threadIdx.x], sh[0][threadIdx.x + o]);
+            sh[1][threadIdx.x] = fmin(sh[1][threadIdx.x], sh[1][threadIdx.x + o]);
+            sh[2][threadIdx.x] = fmax(sh[2][threadIdx.x], sh[2][threadIdx.x + o]);
+            sh[3][threadIdx.x] = fmin(sh[3][threadIdx.x], sh[3][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[(size_t)blockIdx.x * 4 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
 // ------------------------------------------------------------------ state
 struct DevBuf {
     void* p = nullptr;
@@ -365,6 +490,7 @@ struct StokesState {
     bool any_fix_v = false, any_fix_p = false;
     // per-precision hierarchy keys: levels, tile edge and grid generation it
     // was built for, and whether the viscosity must be restricted again
+    uint64_t grid_gen = 0;  // grid generation the operator was built on
     struct HKey {
         int nlev = -1, tile = -1;
         uint64_t grid = 0;
@@ -374,7 +500,14 @@ struct StokesState {
     DevBuf zc, jr0, jr1, judge, pair_e, pair_r, pair_z;
 };
 
-static StokesState* state(ebg_ctx* ctx) { return ctx ? (StokesState*)ctx->stokes : nullptr; }
+// The system's operator points into the context's grid arrays, so a system
+// is usable only while the grid it was set with is current.
+static StokesState* state(ebg_ctx* ctx)
+{
+    if (!ctx || !ctx->stokes) return nullptr;
+    StokesState* st = (StokesState*)ctx->stokes;
+    return (ctx->has_grid && st->grid_gen == ctx->grid.generation) ? st : nullptr;
+}
 
 static int copy_in(DevBuf& d, const double* src, size_t n, hipStream_t s)
 {
@@ -797,7 +930,7 @@ int ebg_stokes_set_system(ebg_ctx* ctx, const double* etas0, const double* etan0
               "ebg_stokes_set_system: pressure_bc_mode must be 0, 1 or 2");
     EBG_CHECK(std::isfinite(pscale) && pscale > 0, EBG_ERR_ARG, "ebg_stokes_set_system: pscale must be > 0");
     if (!ctx->stokes) ctx->stokes = new StokesState();
-    StokesState& st = *state(ctx);
+    StokesState& st = *(StokesState*)ctx->stokes;
     const Grid2D& g = ctx->grid;
     hipStream_t s = ctx->stream;
     const int xnum = (int)g.xnum, ynum = (int)g.ynum;
@@ -807,6 +940,7 @@ int ebg_stokes_set_system(ebg_ctx* ctx, const double* etas0, const double* etan0
     }
     st.xnum = xnum;
     st.ynum = ynum;
+    st.grid_gen = g.generation;
     st.nc = (xnum - 1) * (ynum - 1);
     int rc;
     const size_t ny = ynum, nx = xnum;
@@ -879,7 +1013,7 @@ int ebg_stokes_set_system(ebg_ctx* ctx, const double* etas0, const double* etan0
 int ebg_stokes_apply(ebg_ctx* ctx, const double* S, double* LS)
 {
     StokesState* st = state(ctx);
-    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_apply: no system set");
+    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_apply: no system set (or the grid changed since ebg_stokes_set_system)");
     EBG_CHECK(S && LS && S != LS, EBG_ERR_ARG, "ebg_stokes_apply: bad vectors");
     hipLaunchKernelGGL((stokes_rows_kernel<false, LayoutInterleaved>), dim3(nb(st->nc)), dim3(256), 0,
                        ctx->stream, st->op, S, LS, LayoutInterleaved{st->nc});
@@ -890,7 +1024,7 @@ int ebg_stokes_apply(ebg_ctx* ctx, const double* S, double* LS)
 int ebg_stokes_rhs(ebg_ctx* ctx, double* R)
 {
     StokesState* st = state(ctx);
-    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_rhs: no system set");
+    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_rhs: no system set (or the grid changed since ebg_stokes_set_system)");
     EBG_CHECK(R, EBG_ERR_ARG, "ebg_stokes_rhs: NULL R");
     hipLaunchKernelGGL((stokes_rows_kernel<true, LayoutInterleaved>), dim3(nb(st->nc)), dim3(256), 0,
                        ctx->stream, st->op, (const double*)nullptr, R, LayoutInterleaved{st->nc});
@@ -901,7 +1035,7 @@ int ebg_stokes_rhs(ebg_ctx* ctx, double* R)
 int ebg_stokes_residual(ebg_ctx* ctx, const double* S, double* resnl)
 {
     StokesState* st = state(ctx);
-    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_residual: no system set");
+    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_residual: no system set (or the grid changed since ebg_stokes_set_system)");
     EBG_CHECK(S && resnl && S != resnl, EBG_ERR_ARG, "ebg_stokes_residual: bad vectors");
     const int64_t n = 3 * (int64_t)st->nc;
     int rc;
@@ -921,7 +1055,7 @@ int ebg_stokes_residual(ebg_ctx* ctx, const double* S, double* resnl)
 int ebg_stokes_unpack(ebg_ctx* ctx, const double* S, double* vx1, double* vy1, double* pr1)
 {
     StokesState* st = state(ctx);
-    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_unpack: no system set");
+    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_unpack: no system set (or the grid changed since ebg_stokes_set_system)");
     EBG_CHECK(S && vx1 && vy1 && pr1, EBG_ERR_ARG, "ebg_stokes_unpack: NULL array");
     const int xnum = st->xnum, ynum = st->ynum;
     hipLaunchKernelGGL(unpack_kernel, dim3(nb((int64_t)(ynum + 1) * (xnum + 1))), dim3(256), 0,
@@ -973,7 +1107,7 @@ static int solve_impl(ebg_ctx* ctx, const double* R, double* S, const ebg_solve_
                       ebg_solve_stats* stats)
 {
     StokesState* st = state(ctx);
-    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_solve: no system set");
+    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_solve: no system set (or the grid changed since ebg_stokes_set_system)");
     EBG_CHECK(S, EBG_ERR_ARG, "ebg_stokes_solve: NULL S");
     ebg_solve_opts o;
     if (opts) o = *opts;
@@ -1019,7 +1153,7 @@ static int solve_impl(ebg_ctx* ctx, const double* R, double* S, const ebg_solve_
 int ebg_stokes_precondition(ebg_ctx* ctx, const double* r, double* z, const ebg_solve_opts* opts)
 {
     StokesState* st = state(ctx);
-    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_precondition: no system set");
+    EBG_CHECK(st, EBG_ERR_STATE, "ebg_stokes_precondition: no system set (or the grid changed since ebg_stokes_set_system)");
     EBG_CHECK(r && z, EBG_ERR_ARG, "ebg_stokes_precondition: NULL vector");
     ebg_solve_opts o;
     if (opts) o = *opts;
@@ -1044,7 +1178,7 @@ int ebg_cnn_make_sample(ebg_ctx* ctx, const double* e, const ebg_solve_opts* opt
                         float* tgt_img)
 {
     StokesState* st = state(ctx);
-    EBG_CHECK(st, EBG_ERR_STATE, "ebg_cnn_make_sample: no system set");
+    EBG_CHECK(st, EBG_ERR_STATE, "ebg_cnn_make_sample: no system set (or the grid changed since ebg_stokes_set_system)");
     EBG_CHECK(ctx->cnn, EBG_ERR_STATE, "ebg_cnn_make_sample: no network (ebg_cnn_create)");
     EBG_CHECK(e && in_img && tgt_img, EBG_ERR_ARG, "ebg_cnn_make_sample: NULL pointer");
     ebg_solve_opts o;
@@ -1071,6 +1205,98 @@ int ebg_cnn_make_sample(ebg_ctx* ctx, const double* e, const ebg_solve_opts* opt
                          in_img, tgt_img, s);
     if (rc) return rc;
     EBG_HIP(hipGetLastError());
+    return EBG_OK;
+}
+
+
+int ebg_compute_stokes_residuals(ebg_ctx* ctx, const double* vx1, const double* vy1, const double* pr1,
+                                 double* resx1, double* resy1, double* resc1)
+{
+    StokesState* st = state(ctx);
+    EBG_CHECK(st, EBG_ERR_STATE, "ebg_compute_stokes_residuals: no system set (or the grid changed since ebg_stokes_set_system)");
+    EBG_CHECK(vx1 && vy1 && pr1 && resx1 && resy1 && resc1, EBG_ERR_ARG,
+              "ebg_compute_stokes_residuals: NULL array");
+    const int64_t n = (int64_t)(st->ynum + 1) * (st->xnum + 1);
+    hipLaunchKernelGGL(stokes_grid_residuals_kernel, dim3(nb(n)), dim3(256), 0, ctx->stream, st->op, vx1, vy1,
+                       pr1, resx1, resy1, resc1);
+    EBG_HIP(hipGetLastError());
+    return EBG_OK;
+}
+
+constexpr int NORM_BLOCKS = 256;
+
+// blocks' partials reduced on the host in block order (deterministic)
+static int device_partials(ebg_ctx* ctx, std::vector<double>& h, DevBuf& buf)
+{
+    EBG_HIP(hipMemcpyAsync(h.data(), buf.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    EBG_HIP(hipStreamSynchronize(ctx->stream));
+    return EBG_OK;
+}
+
+int ebg_change_norms(ebg_ctx* ctx, int nseg, const int64_t* n, const double* const* v,
+                     const double* const* v_old, double* out)
+{
+    EBG_CHECK(ctx && out && n && v, EBG_ERR_ARG, "ebg_change_norms: NULL argument");
+    EBG_CHECK(nseg >= 1 && nseg <= 4, EBG_ERR_ARG, "ebg_change_norms: nseg must be 1..4");
+    NormSegs g{};
+    g.nseg = nseg;
+    for (int q = 0; q < nseg; ++q) {
+        EBG_CHECK(n[q] >= 0 && (v[q] || n[q] == 0), EBG_ERR_ARG, "ebg_change_norms: bad segment");
+        g.n[q] = n[q];
+        g.v[q] = v[q];
+        g.vo[q] = v_old ? v_old[q] : nullptr;
+    }
+    static thread_local DevBuf buf;
+    int rc = buf.ensure(NORM_BLOCKS * 4 * sizeof(double));
+    if (rc) return rc;
+    hipLaunchKernelGGL(change_norms_kernel, dim3(NORM_BLOCKS), dim3(256), 0, ctx->stream, g, buf.as<double>());
+    EBG_HIP(hipGetLastError());
+    std::vector<double> h(NORM_BLOCKS * 4);
+    if ((rc = device_partials(ctx, h, buf))) return rc;
+    double sd = 0, sv = 0, md = 0, mv = 0;
+    for (int b = 0; b < NORM_BLOCKS; ++b) {
+        sd += h[4 * b];
+        sv += h[4 * b + 1];
+        md = std::max(md, h[4 * b + 2]);
+        mv = std::max(mv, h[4 * b + 3]);
+    }
+    const double l2d = std::sqrt(sd), l2v = std::sqrt(sv);
+    out[0] = l2d;
+    out[1] = l2v != 0.0 ? l2d / l2v : 1e38;
+    out[2] = md;
+    out[3] = mv != 0.0 ? md / mv : 1e38;
+    return EBG_OK;
+}
+
+int ebg_calculate_new_timestep(ebg_ctx* ctx, const double* vx1, const double* vy1, double timestep_old,
+                               double marker_cell_displ_max, double* timestep_new, int* updated)
+{
+    EBG_CHECK(ctx && ctx->has_grid, EBG_ERR_STATE, "ebg_calculate_new_timestep: no grid set");
+    EBG_CHECK(vx1 && vy1 && timestep_new, EBG_ERR_ARG, "ebg_calculate_new_timestep: NULL argument");
+    const Grid2D& gr = ctx->grid;
+    static thread_local DevBuf buf;
+    int rc = buf.ensure(NORM_BLOCKS * 4 * sizeof(double));
+    if (rc) return rc;
+    hipLaunchKernelGGL(maxmin2_kernel, dim3(NORM_BLOCKS), dim3(256), 0, ctx->stream, vx1, (gr.ynum + 1) * gr.xnum,
+                       vy1, gr.ynum * (gr.xnum + 1), buf.as<double>());
+    EBG_HIP(hipGetLastError());
+    std::vector<double> h(NORM_BLOCKS * 4);
+    if ((rc = device_partials(ctx, h, buf))) return rc;
+    double mxa = -INFINITY, mna = INFINITY, mxb = -INFINITY, mnb = INFINITY;
+    for (int b = 0; b < NORM_BLOCKS; ++b) {
+        mxa = std::max(mxa, h[4 * b]);
+        mna = std::min(mna, h[4 * b + 1]);
+        mxb = std::max(mxb, h[4 * b + 2]);
+        mnb = std::min(mnb, h[4 * b + 3]);
+    }
+    // time_step/update/Update.jl:71-98, 198-218
+    const double vxmax = std::max(std::fabs(mxa), std::fabs(mna));
+    const double vymax = std::max(std::fabs(mxb), std::fabs(mnb));
+    double dt = timestep_old;
+    if (vxmax > 0) dt = std::min(dt, marker_cell_displ_max * gr.xstpavg / vxmax);
+    if (vymax > 0) dt = std::min(dt, marker_cell_displ_max * gr.ystpavg / vymax);
+    *timestep_new = dt;
+    if (updated) *updated = dt != timestep_old;
     return EBG_OK;
 }
 

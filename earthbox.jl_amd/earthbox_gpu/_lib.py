@@ -45,6 +45,10 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_stokes_residual": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_stokes_unpack": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ebg_stokes_solve_opts_default": (c_int, [c_void_p]),
+    "ebg_compute_stokes_residuals": (c_int, [c_void_p] * 7),
+    "ebg_change_norms": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_calculate_new_timestep": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_double, c_void_p,
+                                           c_void_p]),
     "ebg_stokes_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "ebg_mg2d_create": (c_int, [c_void_p, c_int64, c_int64, c_double, c_double, c_int, c_void_p]),
     "ebg_mg2d_level_shape": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),

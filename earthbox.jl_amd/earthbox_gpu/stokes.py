@@ -53,7 +53,7 @@ class StokesSolverGPU:
 
     def set_system(self, prob: StokesProblem, fields: dict | None = None) -> None:
         """Upload the fields (numpy, or device tensors in `fields`)."""
-        if self.ctx.grid is None or self.ctx.grid.xnum != prob.grid.xnum:
+        if self.ctx.grid is not prob.grid:
             self.ctx.set_grid(prob.grid)
         f = fields or {}
         self._dev = {k: f.get(k) if f.get(k) is not None else to_device(getattr(prob, k))
@@ -127,6 +127,17 @@ class StokesSolverGPU:
               "ebg_stokes_precondition")
         return out
 
+    def compute_stokes_residuals(self, vx1, vy1, pr1):
+        """StokesResiduals.compute_stokes_residuals! (StokesResiduals.jl:94-137):
+        (resx1, resy1, resc1) on the vx, vy and pressure grids."""
+        g = self.prob.grid
+        rx = empty((g.ynum + 1, g.xnum), np.float64)
+        ry = empty((g.ynum, g.xnum + 1), np.float64)
+        rc = empty((g.ynum - 1, g.xnum - 1), np.float64)
+        check(lib.ebg_compute_stokes_residuals(self.ctx.handle, ptr(vx1), ptr(vy1), ptr(pr1), ptr(rx),
+                                               ptr(ry), ptr(rc)), "ebg_compute_stokes_residuals")
+        return rx, ry, rc
+
     def process_stokes_solution(self, S: torch.Tensor):
         g = self.prob.grid
         vx1 = empty((g.ynum + 1, g.xnum), np.float64)
@@ -189,3 +200,34 @@ class MultigridGPU:
         check(lib.ebg_mg2d_vcycles(self.ctx.handle, ncycles, finest_sweeps, multiplier, cap,
                                    theta_s, theta_c, relax_v, relax_p, prnorm, d),
               "ebg_mg2d_vcycles")
+
+
+def change_norms(ctx, v, v_old=None):
+    """MathTools.get_L2_norms / get_inf_norms (src/utils/MathTools.jl:64-84) over
+    the concatenation of the device arrays in v (and v_old; None = zeros):
+    (abs_L2, rel_L2, abs_inf, rel_inf)."""
+    v = list(v) if isinstance(v, (list, tuple)) else [v]
+    vo = list(v_old) if isinstance(v_old, (list, tuple)) else ([v_old] * len(v))
+    if not 1 <= len(v) <= 4 or len(vo) != len(v):
+        raise ValueError("1..4 segments, v and v_old of equal count")
+    k = len(v)
+    n = (C.c_int64 * k)(*[t.numel() for t in v])
+    pv = (C.c_void_p * k)(*[t.data_ptr() for t in v])
+    po = (C.c_void_p * k)(*[(t.data_ptr() if t is not None else None) for t in vo])
+    for a, b in zip(v, vo):
+        if b is not None and b.numel() != a.numel():
+            raise ValueError("v and v_old segments differ in size")
+    out = (C.c_double * 4)()
+    check(lib.ebg_change_norms(ctx.handle, k, n, pv, po, out), "ebg_change_norms")
+    return tuple(out)
+
+
+def calculate_new_timestep(ctx, vx1, vy1, timestep_old, marker_cell_displ_max):
+    """Update.calculate_new_time_step_using_average_grid_spacing
+    (time_step/update/Update.jl:43-61): (timestep_new, updated)."""
+    dt = C.c_double()
+    up = C.c_int()
+    check(lib.ebg_calculate_new_timestep(ctx.handle, ptr(vx1), ptr(vy1), float(timestep_old),
+                                         float(marker_cell_displ_max), C.byref(dt), C.byref(up)),
+          "ebg_calculate_new_timestep")
+    return dt.value, bool(up.value)

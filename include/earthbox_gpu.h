@@ -181,6 +181,24 @@ int ebg_stokes_residual(ebg_ctx *ctx, const double *S, double *resnl);
 /* K10: SystemSolver.reload_solu_vel! + apply_vel_bc! (SystemSolver.jl:188-271):
  * vx1 (ynum+1,xnum), vy1 (ynum,xnum+1), pr1 = S[p']*pscale (ynum-1,xnum-1). */
 int ebg_stokes_unpack(ebg_ctx *ctx, const double *S, double *vx1, double *vy1, double *pr1);
+/* StokesResiduals.compute_stokes_residuals! (solver/residuals/StokesResiduals.jl:94-137)
+ * with XStokes/YStokes/Continuity.calculate_residual!: per-equation residuals
+ * of (vx1, vy1, pr1) on the vx, vy and pressure grids for the current system
+ * (zone nodes and unwritten entries are 0). */
+int ebg_compute_stokes_residuals(ebg_ctx *ctx, const double *vx1, const double *vy1, const double *pr1,
+                                 double *resx1, double *resy1, double *resc1);
+/* MathTools.get_L2_norms / get_inf_norms (src/utils/MathTools.jl:64-84) as used by
+ * StokesNorms.calculate_stokes_norms! (solver/norms/StokesNorms.jl:8-160) on the
+ * concatenation of nseg (1..4) device segments: out = {||v_old - v||_2,
+ * relative L2 (1e38 if ||v|| = 0), ||v_old - v||_inf, relative inf}; v_old NULL
+ * (or a NULL entry) means zeros, so out[0] is then ||v||_2.  n, v, v_old are host
+ * arrays of nseg entries. */
+int ebg_change_norms(ebg_ctx *ctx, int nseg, const int64_t *n, const double *const *v,
+                     const double *const *v_old, double *out);
+/* TimeStep update (time_step/update/Update.jl:43-98, 198-218): the step
+ * limited by marker_cell_displ_max * x/ystpavg / max|vx1|, max|vy1|. */
+int ebg_calculate_new_timestep(ebg_ctx *ctx, const double *vx1, const double *vy1, double timestep_old,
+                               double marker_cell_displ_max, double *timestep_new, int *updated);
 
 typedef struct {
     int precision;        /* 1: FP32 V-cycle preconditioner (mixed), 0: FP64 (plain) */

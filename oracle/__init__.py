@@ -284,3 +284,49 @@ def scatter_stokes(g, m: dict, prev: dict):
         raise ValueError("zero shear modulus")
     out["marker_xn"], out["marker_yn"] = xn, yn
     return out
+
+
+def stokes_grid_residuals(prob, vx1, vy1, pr1):
+    """compute_stokes_residuals! (StokesResiduals.jl:94-137): resx1, resy1, resc1."""
+    L = lib()
+    L.ebo_stokes_grid_residuals.restype = None
+    L.ebo_stokes_grid_residuals.argtypes = [C.POINTER(_StokesIn), _P, _P, _P, _P, _P, _P]
+    s, keep = _stokes_in(prob)
+    g = prob.grid
+    vx1, vy1, pr1 = (np.asfortranarray(a, dtype=np.float64) for a in (vx1, vy1, pr1))
+    rx = np.zeros((g.ynum + 1, g.xnum), order="F")
+    ry = np.zeros((g.ynum, g.xnum + 1), order="F")
+    rc = np.zeros((g.ynum - 1, g.xnum - 1), order="F")
+    L.ebo_stokes_grid_residuals(C.byref(s), p(vx1), p(vy1), p(pr1), p(rx), p(ry), p(rc))
+    return rx, ry, rc
+
+
+def get_L2_norms(v, v_old):
+    """MathTools.get_L2_norms (src/utils/MathTools.jl:64-73)."""
+    v, v_old = np.asarray(v, np.float64).ravel(order="F"), np.asarray(v_old, np.float64).ravel(order="F")
+    d = np.linalg.norm(np.abs(v_old - v))
+    n = np.linalg.norm(v)
+    return d, (d / n if n != 0.0 else 1e38)
+
+
+def get_inf_norms(v, v_old):
+    """MathTools.get_inf_norms (src/utils/MathTools.jl:75-84)."""
+    v, v_old = np.asarray(v, np.float64).ravel(order="F"), np.asarray(v_old, np.float64).ravel(order="F")
+    d = float(np.max(np.abs(v_old - v))) if v.size else 0.0
+    n = float(np.max(np.abs(v))) if v.size else 0.0
+    return d, (d / n if n != 0.0 else 1e38)
+
+
+def calculate_new_timestep(timestep_old, marker_cell_displ_max, xstpavg, ystpavg, vx1, vy1):
+    """time_step/update/Update.jl:43-61, 71-98, 198-218: the velocity-limited
+    step and whether it changed."""
+    def vmax(a):
+        return max(abs(float(np.max(a))), abs(float(np.min(a))))
+
+    def limit(dt, size, v):
+        if v > 0:
+            dt = min(dt, marker_cell_displ_max * size / v)
+        return dt
+    dt = limit(timestep_old, xstpavg, vmax(vx1))
+    dt = limit(dt, ystpavg, vmax(vy1))
+    return dt, dt != timestep_old

@@ -47,6 +47,8 @@ double ebo_pressure_scaling_coeff(double etan11, double xstpavg, double ystpavg)
 void ebo_stokes_rhs(const ebo_rhs_in *r, double *etas0, double *sxy0, double *etan0,
                     double *sxx0, double *RX1, double *RY1, double *RC1);
 void ebo_stokes_unpack(const ebo_stokes_in *s, const double *S, double *vx1, double *vy1, double *pr1);
+void ebo_stokes_grid_residuals(const ebo_stokes_in *s, const double *vx1, const double *vy1,
+                               const double *pr1, double *resx, double *resy, double *resc);
 void ebo_coo_residual(int64_t N, int64_t nnz, const int64_t *Li, const int64_t *Lj,
                       const double *Lv, const double *x, const double *R, double *res);
 void ebo_coo_spmv(int64_t N, int64_t nnz, const int64_t *Li, const int64_t *Lj,

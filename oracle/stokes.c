@@ -418,3 +418,67 @@ void ebo_coo_spmv(int64_t N, int64_t nnz, const int64_t *Li, const int64_t *Lj,
     for (int64_t k = 0; k < N; ++k) y[k] = 0.0;
     for (int64_t k = 0; k < nnz; ++k) y[Li[k] - 1] += Lv[k] * x[Lj[k] - 1];
 }
+
+/* compute_stokes_residuals! (solver/residuals/StokesResiduals.jl:94-137) with
+ * XStokes.calculate_residual! (residuals/utils/XStokes.jl:66-117),
+ * YStokes.calculate_residual! (YStokes.jl:59-112) and
+ * Continuity.calculate_residual! (Continuity.jl:53-84): per-equation grid
+ * residuals of (vx1, vy1, pr1); nodes in an internal velocity zone keep 0. */
+void ebo_stokes_grid_residuals(const ebo_stokes_in *s, const double *vx1, const double *vy1,
+                               const double *pr1, double *resx, double *resy, double *resc)
+{
+    const int64_t xnum = s->xnum, ynum = s->ynum;
+    const int64_t lx = ynum + 1, ly = ynum, lp = ynum - 1, le = ynum, ln = ynum - 1;
+    const int64_t *z = s->bintern_zone;
+    for (int64_t k = 0; k < lx * xnum; ++k) resx[k] = 0.0;
+    for (int64_t k = 0; k < ly * (xnum + 1); ++k) resy[k] = 0.0;
+    for (int64_t k = 0; k < lp * (xnum - 1); ++k) resc[k] = 0.0;
+    for (int64_t j = 1; j <= xnum + 1; ++j)
+        for (int64_t i = 1; i <= ynum + 1; ++i) {
+            /* XStokes.jl:66-117 */
+            if (j < xnum + 1 && (j != z[0] || i < z[1] || i > z[2])) {
+                if (i == 1 || i == ynum + 1 || j == 1 || j == xnum) {
+                    EBO_M(resx, lx, i, j) = 0.0;
+                } else {
+                    double r = EBO_M(s->RX, lx, i, j) -
+                               (2.0 * (EBO_M(s->etan, ln, i - 1, j) * (EBO_M(vx1, lx, i, j + 1) - EBO_M(vx1, lx, i, j)) /
+                                           EBO_V(s->xstp, j) -
+                                       EBO_M(s->etan, ln, i - 1, j - 1) * (EBO_M(vx1, lx, i, j) - EBO_M(vx1, lx, i, j - 1)) /
+                                           EBO_V(s->xstp, j - 1)) -
+                                (EBO_M(pr1, lp, i - 1, j) - EBO_M(pr1, lp, i - 1, j - 1))) /
+                                   EBO_V(s->xstpc, j);
+                    r = r - (EBO_M(s->etas, le, i, j) * ((EBO_M(vx1, lx, i + 1, j) - EBO_M(vx1, lx, i, j)) / EBO_V(s->ystpc, i) +
+                                                        (EBO_M(vy1, ly, i, j + 1) - EBO_M(vy1, ly, i, j)) / EBO_V(s->xstpc, j)) -
+                             EBO_M(s->etas, le, i - 1, j) * ((EBO_M(vx1, lx, i, j) - EBO_M(vx1, lx, i - 1, j)) / EBO_V(s->ystpc, i - 1) +
+                                                            (EBO_M(vy1, ly, i - 1, j + 1) - EBO_M(vy1, ly, i - 1, j)) / EBO_V(s->xstpc, j))) /
+                                EBO_V(s->ystp, i - 1);
+                    EBO_M(resx, lx, i, j) = r;
+                }
+            }
+            /* YStokes.jl:59-112 */
+            if (i < ynum + 1 && (j != z[4] || i < z[5] || i > z[6])) {
+                if (i == 1 || i == ynum || j == 1 || j == xnum + 1) {
+                    EBO_M(resy, ly, i, j) = 0.0;
+                } else {
+                    double r = EBO_M(s->RY, ly, i, j) -
+                               (2.0 * (EBO_M(s->etan, ln, i, j - 1) * (EBO_M(vy1, ly, i + 1, j) - EBO_M(vy1, ly, i, j)) /
+                                           EBO_V(s->ystp, i) -
+                                       EBO_M(s->etan, ln, i - 1, j - 1) * (EBO_M(vy1, ly, i, j) - EBO_M(vy1, ly, i - 1, j)) /
+                                           EBO_V(s->ystp, i - 1)) -
+                                (EBO_M(pr1, lp, i, j - 1) - EBO_M(pr1, lp, i - 1, j - 1))) /
+                                   EBO_V(s->ystpc, i);
+                    r = r - (EBO_M(s->etas, le, i, j) * ((EBO_M(vy1, ly, i, j + 1) - EBO_M(vy1, ly, i, j)) / EBO_V(s->xstpc, j) +
+                                                        (EBO_M(vx1, lx, i + 1, j) - EBO_M(vx1, lx, i, j)) / EBO_V(s->ystpc, i)) -
+                             EBO_M(s->etas, le, i, j - 1) * ((EBO_M(vy1, ly, i, j) - EBO_M(vy1, ly, i, j - 1)) / EBO_V(s->xstpc, j - 1) +
+                                                            (EBO_M(vx1, lx, i + 1, j - 1) - EBO_M(vx1, lx, i, j - 1)) / EBO_V(s->ystpc, i))) /
+                                EBO_V(s->xstp, j - 1);
+                    EBO_M(resy, ly, i, j) = r;
+                }
+            }
+            /* Continuity.jl:53-84 */
+            if (i < ynum && j < xnum)
+                EBO_M(resc, lp, i, j) = EBO_M(s->RC, lp, i, j) -
+                                        ((EBO_M(vx1, lx, i + 1, j + 1) - EBO_M(vx1, lx, i + 1, j)) / EBO_V(s->xstp, j) +
+                                         (EBO_M(vy1, ly, i + 1, j + 1) - EBO_M(vy1, ly, i, j + 1)) / EBO_V(s->ystp, i));
+        }
+}

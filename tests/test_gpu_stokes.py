@@ -198,3 +198,64 @@ def test_solve_smoother_variants_match_direct(gpu_ctx, smoother):
     assert st["converged"], st
     Sg = to_host(S)
     assert np.linalg.norm(Sg - Sd) <= 1e-8 * np.linalg.norm(Sd)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_grid_residuals_bit_exact(gpu_ctx, case):
+    """compute_stokes_residuals! on random fields: bit-exact vs the oracle."""
+    print(f"grid residuals case {case}", flush=True)
+    pr = random_case(**case)
+    g = pr.grid
+    rng = np.random.default_rng(case["seed"] + 50)
+    vx = np.asfortranarray(rng.standard_normal((g.ynum + 1, g.xnum)) * 1e-9)
+    vy = np.asfortranarray(rng.standard_normal((g.ynum, g.xnum + 1)) * 1e-9)
+    p = np.asfortranarray(rng.standard_normal((g.ynum - 1, g.xnum - 1)) * 1e6)
+    ex, ey, ec = oracle.stokes_grid_residuals(pr, vx, vy, p)
+    sol = StokesSolverGPU(gpu_ctx)
+    sol.set_system(pr)
+    rx, ry, rc = sol.compute_stokes_residuals(to_device(vx), to_device(vy), to_device(p))
+    gpu_ctx.synchronize()
+    assert np.array_equal(to_host(rx), ex)
+    assert np.array_equal(to_host(ry), ey)
+    assert np.array_equal(to_host(rc), ec)
+
+
+def test_change_norms_and_timestep(gpu_ctx):
+    from earthbox_gpu.stokes import calculate_new_timestep, change_norms
+    from earthbox_gpu.grid import uniform_grid
+    print("change norms and time step", flush=True)
+    rng = np.random.default_rng(4)
+    a, ao = rng.standard_normal(100003), rng.standard_normal(100003)
+    b, bo = rng.standard_normal(777), rng.standard_normal(777)
+    da, dao, db, dbo = (to_device(t) for t in (a, ao, b, bo))
+    out = change_norms(gpu_ctx, [da, db], [dao, dbo])
+    v, vo = np.concatenate([a, b]), np.concatenate([ao, bo])
+    l2, rl2 = oracle.get_L2_norms(v, vo)
+    inf, rinf = oracle.get_inf_norms(v, vo)
+    assert out[0] == pytest.approx(l2, rel=1e-13) and out[1] == pytest.approx(rl2, rel=1e-13)
+    assert out[2] == inf and out[3] == rinf
+    assert change_norms(gpu_ctx, da)[0] == pytest.approx(np.linalg.norm(a), rel=1e-13)
+    g = uniform_grid(41, 23, 4.0e5, 2.2e5)
+    gpu_ctx.set_grid(g)
+    vx1 = np.asfortranarray(rng.standard_normal((g.ynum + 1, g.xnum)) * 1e-9)
+    vy1 = np.asfortranarray(rng.standard_normal((g.ynum, g.xnum + 1)) * 2e-9)
+    got = calculate_new_timestep(gpu_ctx, to_device(vx1), to_device(vy1), 1e16, 0.5)
+    assert got == oracle.calculate_new_timestep(1e16, 0.5, g.xstpavg, g.ystpavg, vx1, vy1)
+
+
+def test_system_invalidated_when_grid_changes(gpu_ctx):
+    """The operator reads the context's grid arrays, so setting another grid
+    invalidates the system instead of leaving it pointing at freed arrays."""
+    from earthbox_gpu._lib import EBGError
+    print("stale system after grid change", flush=True)
+    pr = random_case(**CASES[0])
+    sol = StokesSolverGPU(gpu_ctx)
+    sol.set_system(pr)
+    S = to_device(np.ones(pr.N))
+    sol.apply(S)
+    gpu_ctx.set_grid(random_case(**CASES[1]).grid)
+    with pytest.raises(EBGError, match="grid changed"):
+        sol.apply(S)
+    sol.set_system(pr)  # re-setting the system makes it usable again
+    sol.apply(S)
+    gpu_ctx.synchronize()
