@@ -77,6 +77,8 @@ struct ebg_ctx {
     void* scatter = nullptr;
     void* mg_ref = nullptr;
     void* cnn = nullptr;  // learned preconditioner correction (ebg_cnn_create)
+    void* mg3_ref = nullptr;   // reference-order 3-D multigrid (ebg_mg3d_create)
+    void* stokes3d = nullptr;  // 3-D Stokes solve state (ebg_stokes3d_set_system)
 };
 
 namespace ebg {

@@ -75,6 +75,9 @@ static int axis_uniform(const double* g, int64_t n, double* x0, double* inv_h)
     return 1;
 }
 
+int mg3ref_release(ebg_ctx* ctx);    // ebg_mg3d.hip
+int stokes3d_release(ebg_ctx* ctx);  // ebg_stokes3d.hip
+
 }  // namespace ebg
 
 using namespace ebg;
@@ -115,6 +118,8 @@ int ebg_ctx_destroy(ebg_ctx* ctx)
     ebg_stokes_release(ctx);
     ebg_scatter_release(ctx);
     ebg_mgref_release(ctx);
+    mg3ref_release(ctx);
+    stokes3d_release(ctx);
     cnn_release_ctx(ctx);
     grid_release(ctx->grid);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -1,0 +1,52 @@
+// FP64 Krylov building blocks shared by the 2-D and 3-D flexible GMRES
+// solvers: deterministic multi-dot products (fixed-order block partials, then
+// one fixed-order reduction), vector updates, and a growable device buffer.
+// The kernels are defined in ebg_stokes.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "ebg_common.hpp"
+
+namespace ebg {
+
+constexpr int RED_BLOCKS = 512;  // partial sums per reduction
+constexpr int MAXK = 64;         // longest Krylov basis a multi-dot handles
+
+// partial[b*k + q] = sum over block b's grid-stride slice of w * V[q]
+__global__ void multidot_kernel(const double* __restrict__ w, const double* __restrict__ V, int64_t ldv,
+                                int k, int64_t n, double* __restrict__ partial);
+// out[q] (+)= sum_b partial[b*k + q], one block, fixed order
+__global__ void reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int k,
+                                       double* __restrict__ out, int accumulate);
+// w -= sum_q h[q] V[q]
+__global__ void update_kernel(double* __restrict__ w, const double* __restrict__ V, int64_t ldv,
+                              const double* __restrict__ h, int k, int64_t n);
+// x += sum_q y[q] Z[q]
+__global__ void combine_kernel(double* __restrict__ x, const double* __restrict__ Z, int64_t ldz,
+                               const double* __restrict__ y, int k, int64_t n);
+// out = in / nrm[0]
+__global__ void scale_kernel(double* __restrict__ out, const double* __restrict__ in, const double* nrm,
+                             int64_t n);
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int ensure(size_t b)
+    {
+        if (bytes >= b && p) return EBG_OK;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        EBG_HIP(hipMalloc(&p, b ? b : 8));
+        bytes = b;
+        return EBG_OK;
+    }
+    template <typename T> T* as() { return (T*)p; }
+};
+
+}  // namespace ebg

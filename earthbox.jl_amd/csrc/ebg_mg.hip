@@ -32,15 +32,15 @@ std::vector<int> mg_smoothing_counts(int nlev, int finest, int multiplier, int c
     return s;
 }
 
-static void basic_axis(int n, double step, std::vector<double>& g, std::vector<double>& stp)
+void basic_axis(int n, double step, std::vector<double>& g, std::vector<double>& stp)
 {
     g.assign(n, 0.0);
     for (int i = 1; i < n; ++i) g[i] = g[i - 1] + step;
     stp.assign(n - 1, 0.0);
     for (int i = 0; i < n - 1; ++i) stp[i] = g[i + 1] - g[i];
 }
-static void staggered_axis(int n, const std::vector<double>& g, const std::vector<double>& stp,
-                           std::vector<double>& gs, std::vector<double>& stps)
+void staggered_axis(int n, const std::vector<double>& g, const std::vector<double>& stp,
+                    std::vector<double>& gs, std::vector<double>& stps)
 {
     gs.assign(n + 1, 0.0);
     gs[0] = g[0] - stp[0] / 2.0;
@@ -51,7 +51,7 @@ static void staggered_axis(int n, const std::vector<double>& g, const std::vecto
     stps[n - 1] = stp[n - 2];
     for (int i = 1; i < n - 1; ++i) stps[i] = (g[i + 1] - g[i - 1]) / 2.0;
 }
-static void pressure_axis(int n, const std::vector<double>& g, std::vector<double>& gp)
+void pressure_axis(int n, const std::vector<double>& g, std::vector<double>& gp)
 {
     std::vector<double> stpp(n - 2);
     for (int i = 0; i < n - 2; ++i) stpp[i] = (g[i + 2] - g[i]) / 2.0;
@@ -74,63 +74,6 @@ HostLevelGrid make_uniform_level_grid(int xnum, int ynum, double xsize, double y
     pressure_axis(xnum, h.gridx_b, h.gridx_pr);
     pressure_axis(ynum, h.gridy_b, h.gridy_pr);
     return h;
-}
-
-// clamp(searchsortedlast(g, x), 1, n-1) (MarkerGridMapping.jl:78-86)
-static int left_index_host(double x, const std::vector<double>& g)
-{
-    const int n = (int)g.size();
-    int lo = 0, hi = n + 1;
-    while (lo < hi - 1) {
-        const int m = lo + (hi - lo) / 2;
-        if (x < g[m - 1]) hi = m;
-        else lo = m;
-    }
-    return std::max(1, std::min(lo, n - 1));
-}
-
-template <typename T>
-struct HostAxisMap {
-    std::vector<int> idx, lo, hi;
-    std::vector<T> d;
-};
-
-// GridMappingManager.jl:289-338 for one axis, plus the contiguous fine range
-// [lo, hi] within [rlo, rhi] that feeds each coarse node.
-template <typename T>
-static HostAxisMap<T> axis_map(const std::vector<double>& ga, const std::vector<double>& gb,
-                               int rlo, int rhi, bool clamp)
-{
-    HostAxisMap<T> m;
-    const int na = (int)ga.size(), nb = (int)gb.size();
-    m.idx.resize(na);
-    m.d.resize(na);
-    for (int k = 0; k < na; ++k) {
-        const int ib = left_index_host(ga[k], gb);
-        const double x0 = gb[ib - 1];
-        const double w = gb[ib] - x0;
-        m.idx[k] = ib;
-        double d = (ga[k] - x0) / w;
-        if (clamp) d = std::min(1.0, std::max(0.0, d));
-        m.d[k] = (T)d;
-    }
-    m.lo.assign(nb, 1);
-    m.hi.assign(nb, 0);
-    for (int kc = 1; kc <= nb; ++kc) {
-        int lo = -1, hi = -2;
-        for (int i = rlo; i <= rhi; ++i) {
-            const int ix = m.idx[i - 1];
-            if (ix == kc - 1 || ix == kc) {
-                if (lo < 0) lo = i;
-                hi = i;
-            }
-        }
-        if (lo > 0) {
-            m.lo[kc - 1] = lo;
-            m.hi[kc - 1] = hi;
-        }
-    }
-    return m;
 }
 
 template <typename T>

@@ -430,6 +430,71 @@ struct HostLevelGrid {
 };
 
 HostLevelGrid make_uniform_level_grid(int xnum, int ynum, double xsize, double ysize);
+
+// One axis of a uniform level grid: basic nodes by cumulative sum
+// (Initialization.jl:27-43), staggered nodes with ghosts and their steps
+// (VxGridTools.jl:40-99), pressure nodes (PressureGridTools.jl:34-86).
+void basic_axis(int n, double step, std::vector<double>& g, std::vector<double>& stp);
+void staggered_axis(int n, const std::vector<double>& g, const std::vector<double>& stp,
+                    std::vector<double>& gs, std::vector<double>& stps);
+void pressure_axis(int n, const std::vector<double>& g, std::vector<double>& gp);
+
+// clamp(searchsortedlast(g, x), 1, n-1) (MarkerGridMapping.jl:78-86)
+inline int left_index_host(double x, const std::vector<double>& g)
+{
+    const int n = (int)g.size();
+    int lo = 0, hi = n + 1;
+    while (lo < hi - 1) {
+        const int m = lo + (hi - lo) / 2;
+        if (x < g[m - 1]) hi = m;
+        else lo = m;
+    }
+    return lo < 1 ? 1 : (lo > n - 1 ? n - 1 : lo);
+}
+
+template <typename T>
+struct HostAxisMap {
+    std::vector<int> idx, lo, hi;
+    std::vector<T> d;
+};
+
+// GridMappingManager.jl:289-338 for one axis, plus the contiguous fine range
+// [lo, hi] within [rlo, rhi] that feeds each coarse node.
+template <typename T>
+HostAxisMap<T> axis_map(const std::vector<double>& ga, const std::vector<double>& gb, int rlo, int rhi,
+                        bool clamp)
+{
+    HostAxisMap<T> m;
+    const int na = (int)ga.size(), nb = (int)gb.size();
+    m.idx.resize(na);
+    m.d.resize(na);
+    for (int k = 0; k < na; ++k) {
+        const int ib = left_index_host(ga[k], gb);
+        const double x0 = gb[ib - 1];
+        const double w = gb[ib] - x0;
+        m.idx[k] = ib;
+        double d = (ga[k] - x0) / w;
+        if (clamp) d = d < 0.0 ? 0.0 : (d > 1.0 ? 1.0 : d);
+        m.d[k] = (T)d;
+    }
+    m.lo.assign(nb, 1);
+    m.hi.assign(nb, 0);
+    for (int kc = 1; kc <= nb; ++kc) {
+        int lo = -1, hi = -2;
+        for (int i = rlo; i <= rhi; ++i) {
+            const int ix = m.idx[i - 1];
+            if (ix == kc - 1 || ix == kc) {
+                if (lo < 0) lo = i;
+                hi = i;
+            }
+        }
+        if (lo > 0) {
+            m.lo[kc - 1] = lo;
+            m.hi[kc - 1] = hi;
+        }
+    }
+    return m;
+}
 std::vector<std::pair<int, int>> mg_level_sizes(int xnum, int ynum, int max_levels);
 
 // One device-resident hierarchy of precision T.

@@ -8,6 +8,7 @@
 // companions K10 (unpack + velocity BCs, SystemSolver.jl:141-271) and K11
 // (|L*S - R|, StokesResiduals.jl:63-82).
 #include "ebg_common.hpp"
+#include "ebg_krylov.hpp"
 #include "ebg_mg.hpp"
 #include "ebg_stokes_op.hpp"
 #include "ebg_learned.hpp"
@@ -135,9 +136,7 @@ __global__ void vel_bc_cols_kernel(StokesOpDev s, double* vx1, double* vy1)
     }
 }
 
-// ---- deterministic reductions ----
-constexpr int RED_BLOCKS = 512;
-constexpr int MAXK = 64;
+// ---- deterministic reductions (declared in ebg_krylov.hpp) ----
 constexpr int COMPAT_BLOCKS = 256;  // partial sums of the mode-0 compatibility RHS
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -459,23 +458,6 @@ __global__ void __launch_bounds__(256) maxmin2_kernel(const double* __restrict__
 }
 
 // ------------------------------------------------------------------ state
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    int ensure(size_t b)
-    {
-        if (bytes >= b && p) return EBG_OK;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        EBG_HIP(hipMalloc(&p, b ? b : 8));
-        bytes = b;
-        return EBG_OK;
-    }
-    template <typename T> T* as() { return (T*)p; }
-};
-
 struct StokesState {
     int xnum = 0, ynum = 0, nc = 0;
     StokesOpDev op{};

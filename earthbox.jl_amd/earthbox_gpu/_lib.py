@@ -71,6 +71,23 @@ SIGNATURES: dict[str, tuple] = {
                                         c_double, c_void_p]),
     "ebg_mg2d_vcycles": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
                                  c_double, c_double, c_double, c_void_p]),
+    "ebg_mg3d_create": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_double, c_double, c_double,
+                                c_int, c_void_p]),
+    "ebg_mg3d_level_shape": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "ebg_mg3d_copy_field": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "ebg_mg3d_restrict_viscosity": (c_int, [c_void_p]),
+    "ebg_mg3d_set_bc": (c_int, [c_void_p, c_int]),
+    "ebg_mg3d_smoother": (c_int, [c_void_p, c_int, c_int, c_double, c_double, c_double]),
+    "ebg_mg3d_restrict": (c_int, [c_void_p, c_int]),
+    "ebg_mg3d_prolong": (c_int, [c_void_p, c_int, c_double, c_double]),
+    "ebg_mg3d_vcycles": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
+                                 c_double, c_double, c_double]),
+    "ebg_stokes3d_set_system": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_double, c_double,
+                                        c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_stokes3d_vector_length": (c_int, [c_void_p, c_void_p]),
+    "ebg_stokes3d_apply": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ebg_stokes3d_precondition": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_stokes3d_solve": (c_int, [c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
 }
 
 

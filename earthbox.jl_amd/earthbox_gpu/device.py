@@ -33,7 +33,10 @@ def to_device(a: np.ndarray, dev: torch.device | None = None) -> torch.Tensor:
     if a.ndim == 2:
         t = torch.from_numpy(np.ascontiguousarray(a.T)).to(dev)
         return t.T
-    raise ValueError("only vectors and matrices are supported")
+    if a.ndim == 3:
+        t = torch.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0))).to(dev)
+        return t.permute(2, 1, 0)
+    raise ValueError("only vectors, matrices and 3-D arrays are supported")
 
 
 def empty(shape, dtype, dev: torch.device | None = None) -> torch.Tensor:
@@ -41,6 +44,9 @@ def empty(shape, dtype, dev: torch.device | None = None) -> torch.Tensor:
     tdt = _DTYPES[np.dtype(dtype)]
     if isinstance(shape, int) or len(shape) == 1:
         return torch.empty(shape, dtype=tdt, device=dev)
+    if len(shape) == 3:
+        ny, nx, nz = shape
+        return torch.empty((nz, nx, ny), dtype=tdt, device=dev).permute(2, 1, 0)
     ny, nx = shape
     return torch.empty((nx, ny), dtype=tdt, device=dev).T
 
@@ -55,6 +61,8 @@ def to_host(t: torch.Tensor) -> np.ndarray:
     """Device tensor -> numpy array with the reference's (ny, nx) indexing."""
     if t.dim() == 1:
         return t.cpu().numpy()
+    if t.dim() == 3:
+        return t.permute(2, 1, 0).contiguous().cpu().numpy().transpose(2, 1, 0)
     return np.asfortranarray(t.T.contiguous().cpu().numpy().T)
 
 
@@ -69,6 +77,9 @@ def ptr(t: torch.Tensor | None, dtype=None, numel: int | None = None) -> int | N
     if t.dim() == 2:
         if not t.T.is_contiguous():
             raise ValueError("2-D fields must be column-major (ny, nx) views")
+    elif t.dim() == 3:
+        if not t.permute(2, 1, 0).is_contiguous():
+            raise ValueError("3-D fields must be column-major (ny, nx, nz) views")
     elif not t.is_contiguous():
         raise ValueError("vectors must be contiguous")
     if numel is not None and t.numel() != numel:

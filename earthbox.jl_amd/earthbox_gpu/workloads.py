@@ -145,3 +145,64 @@ def layered_random(xnum: int = 1001, ynum: int = 501, xsize: float = 1.5e6, ysiz
     return StokesProblem(grid=g, etas0=etas, etan0=etan, rho1_vy=rho1_vy, RX1=RX1, RY1=RY1,
                          RC1=RC1, bcs=free_slip_bcs(g), pressure_bc_mode=0, pressure_bc=0.0,
                          timestep=0.0, gravity_y=gravity_y, iuse_interface_stabilization=0)
+
+
+def _axis_kinds(n: int, size: float) -> dict:
+    """Node coordinates of one uniform axis by kind: basic, staggered with
+    ghosts, pressure (cell centres), shear centre."""
+    h = size / (n - 1)
+    b = np.concatenate([[0.0], np.cumsum(np.full(n - 1, h))])
+    mid = 0.5 * (b[1:] + b[:-1])
+    v = np.concatenate([[b[0] - h / 2], mid, [b[-1] + h / 2]])
+    return {"b": b, "v": v, "p": mid, "s": mid}
+
+
+def spheres3d(n: int = 257, size: float = 1.0, nspheres: int = 8, seed: int = 3,
+              radius=(0.05, 0.12), log_contrast=(2.0, 6.0), drho: float = 0.01,
+              gravity: float = 1.0, device=None) -> dict:
+    """Seeded 3-D inclusion problem on an n^3-node unit cube: nspheres spheres
+    of radius in `radius`, viscosity 10^U(log_contrast) times the background
+    of 1, density 1 +- drho on a background of 1, body force -gravity*rho in z
+    (RZ), free-slip walls.  Fields are column-major torch tensors with the
+    3-D multigrid's level shapes (on `device`, default the CPU)."""
+    import torch
+    rng = np.random.default_rng(seed)
+    cen = rng.uniform(0.2, 0.8, size=(nspheres, 3)) * size
+    rad = rng.uniform(*radius, size=nspheres) * size
+    eta_s = 10.0 ** rng.uniform(*log_contrast, size=nspheres)
+    sgn = rng.choice([-1.0, 1.0], size=nspheres)
+    ax = _axis_kinds(n, size)
+    dev = torch.device("cpu") if device is None else device
+    kinds = {"vx": "vbv", "vy": "bvv", "vz": "vvb", "pr": "ppp", "etaxy": "bbs",
+             "etaxz": "sbb", "etayz": "bsb"}
+
+    def coords(field):
+        ky, kx, kz = kinds[field]
+        t = lambda k: torch.as_tensor(ax[k], dtype=torch.float64, device=dev)
+        return t(ky), t(kx), t(kz)
+
+    def inside(field, s):
+        y, x, z = coords(field)
+        cy, cx, cz = cen[s][1], cen[s][0], cen[s][2]
+        d2 = ((z - cz) ** 2)[:, None, None] + ((x - cx) ** 2)[None, :, None] + ((y - cy) ** 2)[None, None, :]
+        return (d2 <= rad[s] ** 2).permute(2, 1, 0)
+
+    def julia_zeros(field):
+        y, x, z = coords(field)
+        return torch.zeros((z.numel(), x.numel(), y.numel()), dtype=torch.float64, device=dev).permute(2, 1, 0)
+
+    out = {"dims": (n, n, n, size, size, size), "spheres": {"center": cen, "radius": rad,
+                                                             "eta": eta_s, "drho_sign": sgn}}
+    for f, name in (("pr", "etan"), ("etaxy", "etaxy"), ("etaxz", "etaxz"), ("etayz", "etayz")):
+        eta = julia_zeros(f) + 1.0
+        for s in range(nspheres):
+            eta = torch.where(inside(f, s), torch.clamp(eta, min=float(eta_s[s])), eta)
+        out[name] = eta
+    rho = julia_zeros("vz") + 1.0
+    for s in range(nspheres):
+        rho = rho + inside("vz", s).to(torch.float64) * (sgn[s] * drho)
+    out["RZ"] = -gravity * rho
+    out["RX"] = julia_zeros("vx")
+    out["RY"] = julia_zeros("vy")
+    out["RC"] = julia_zeros("pr")
+    return out

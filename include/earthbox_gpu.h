@@ -311,6 +311,79 @@ int ebg_mg2d_vcycles(ebg_ctx *ctx, int ncycles, int finest_sweeps, int multiplie
                      double theta_s, double theta_c, double relax_v, double relax_p,
                      double prnorm, const double *bc_d4);
 
+/* ---- the reference's experimental 3-D multigrid (FP64) ----
+ * Replaces multigrid/multigrid_solver/MultigridSolver.jl:113-135 for
+ * MultigridData3d and the level, smoother, restriction and prolongation
+ * modules it drives (free-slip walls, the only boundary type the 3-D solver
+ * uses).  Arrays are column-major (y fastest, then x, then z) with the
+ * reference's LevelData shapes: vx (ynum+1, xnum, znum+1), vy (ynum, xnum+1,
+ * znum+1), vz (ynum+1, xnum+1, znum), pr/etan (ynum-1, xnum-1, znum-1),
+ * etaxy (ynum, xnum, znum-1), etaxz (ynum-1, xnum, znum), etayz (ynum,
+ * xnum-1, znum). */
+enum {
+    EBG_MG3_ETAN = 0, EBG_MG3_ETAXY = 1, EBG_MG3_ETAXZ = 2, EBG_MG3_ETAYZ = 3,
+    EBG_MG3_VX = 4, EBG_MG3_VY = 5, EBG_MG3_VZ = 6, EBG_MG3_PR = 7,
+    EBG_MG3_RX = 8, EBG_MG3_RY = 9, EBG_MG3_RZ = 10, EBG_MG3_RC = 11,
+    EBG_MG3_RESX = 12, EBG_MG3_RESY = 13, EBG_MG3_RESZ = 14, EBG_MG3_RESC = 15
+};
+/* Level hierarchy with uniform grids (LevelManager.make_level_vector_3d,
+ * LevelManager.jl:411-435, 459-482); nlevels 0 = reference rule. */
+int ebg_mg3d_create(ebg_ctx *ctx, int64_t xnum, int64_t ynum, int64_t znum, double xsize, double ysize,
+                    double zsize, int nlevels, int *levels_out);
+int ebg_mg3d_level_shape(ebg_ctx *ctx, int level, int64_t *xnum, int64_t *ynum, int64_t *znum);
+/* Copy one level field (EBG_MG3_*) to dst or from src (device pointers, one NULL). */
+int ebg_mg3d_copy_field(ebg_ctx *ctx, int level, int which, double *dst, const double *src);
+/* ViscosityRestriction.viscosity_restriction3d! on every level
+ * (multigrid/interpolation/ViscosityRestriction.jl:55-230). */
+int ebg_mg3d_restrict_viscosity(ebg_ctx *ctx);
+/* BoundaryConditions.set_boundary_conditions3d! (BoundaryConditions.jl:92-174). */
+int ebg_mg3d_set_bc(ebg_ctx *ctx, int level);
+/* Smoother.stokes_continuity3d_viscous_smoother! (multigrid/smoother/Smoother.jl:50-64):
+ * nsweeps x [ghost pass; red-black sweep], pressure normalisation, residuals. */
+int ebg_mg3d_smoother(ebg_ctx *ctx, int level, int nsweeps, double theta_s, double theta_c, double prnorm);
+/* Restriction.restrict_stokes3d_residuals! (restriction/Restriction.jl:64-376):
+ * residuals of `level` to the right-hand sides of level+1. */
+int ebg_mg3d_restrict(ebg_ctx *ctx, int level);
+/* Prolongation.prolongate_stokes3d_solution plus the correction
+ * (Prolongation.jl:60-140, SmoothAndProlongate.jl:39-56): level's solution
+ * added to level-1. */
+int ebg_mg3d_prolong(ebg_ctx *ctx, int level, double relax_v, double relax_p);
+/* ncycles V-cycles (vcycle/MultigridVCycle.jl:79-135 without viscosity
+ * continuation): SmoothAndRestrict.jl:15-46 then SmoothAndProlongate.jl:14-60. */
+int ebg_mg3d_vcycles(ebg_ctx *ctx, int ncycles, int finest_sweeps, int multiplier, int cap,
+                     double theta_s, double theta_c, double relax_v, double relax_p, double prnorm);
+
+/* ---- 3-D Stokes solve on the 3-D multigrid's system ----
+ * No assembled 3-D system exists in the reference: its 3-D Stokes path is the
+ * multigrid of MultigridSolver.run_multigrid_solver (MultigridSolver.jl:113-135)
+ * iterating V-cycles on the residual equations of Residuals.jl:10-560.  These
+ * entry points solve that same system with FGMRES in FP64, preconditioned by
+ * one V-cycle in FP32 (opts.precision 1) or FP64 (0).
+ * Vectors are [vx | vy | vz | pr] with the level-0 shapes above
+ * (ebg_stokes3d_vector_length).  Unknowns are the interior velocity nodes and
+ * all pressure cells; the continuity equation of cell (1,1,1) is replaced by
+ * the gauge p(1,1,1) = prnorm (PressureUpdate.jl:5-21) scaled by
+ * 1/etan(1,1,1). */
+/* Viscosity arrays are device FP64 with the shapes above; the grid is the
+ * uniform level grid of LevelManager.initialize_grid! (LevelManager.jl:354-366). */
+int ebg_stokes3d_set_system(ebg_ctx *ctx, int64_t xnum, int64_t ynum, int64_t znum, double xsize,
+                            double ysize, double zsize, const double *etan, const double *etaxy,
+                            const double *etaxz, const double *etayz);
+int ebg_stokes3d_vector_length(ebg_ctx *ctx, int64_t *n);
+/* y = A x; the ghost slots of x are overwritten with its free-slip ghost
+ * values first, the ghost slots of y are zero. */
+int ebg_stokes3d_apply(ebg_ctx *ctx, double *x, double *y);
+/* z = M r: one V-cycle with right-hand side r (ghost slots of r ignored). */
+int ebg_stokes3d_precondition(ebg_ctx *ctx, const double *r, double *z, const ebg_solve_opts *opts);
+/* Solve A x = b where b is rhs = [RX | RY | RZ | RC] on the unknowns (ghost
+ * entries ignored) with the gauge entry prnorm/etan(1,1,1).  Uses opts.precision,
+ * restart, max_iterations, rtol, nlevels, finest_sweeps, sweep_multiplier,
+ * sweep_cap, theta_s, theta_c, use_initial_guess; the 2-D-only fields are
+ * ignored.  x (device, vector layout) receives the solution with its ghost
+ * nodes set. */
+int ebg_stokes3d_solve(ebg_ctx *ctx, const double *rhs, double prnorm, double *x,
+                       const ebg_solve_opts *opts, ebg_solve_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,7 +57,7 @@ def p(a: np.ndarray | None):
     """Pointer of a numpy array (Fortran order for matrices), or NULL."""
     if a is None:
         return None
-    if a.ndim == 2 and not a.flags.f_contiguous:
+    if a.ndim >= 2 and not a.flags.f_contiguous:
         raise ValueError("matrices must be Fortran-ordered (Julia layout)")
     if a.ndim == 1 and not a.flags.c_contiguous:
         raise ValueError("vectors must be contiguous")

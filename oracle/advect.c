@@ -5,6 +5,7 @@
  * file:line it follows.  Build with -ffp-contract=off.
  */
 #include "eb_oracle.h"
+#include "eb_mg3d_oracle.h"
 #include <math.h>
 
 /* Initialization.jl:27-43, Spacing.jl:6-18 and :124-133,
@@ -231,4 +232,29 @@ void ebo_calculate_spin(int64_t xnum, int64_t ynum, const double *xstp_vy,
             EBO_M(esp, ynum, i, j) = 0.5 * (
                 (EBO_M(vy1, ynum, i, j + 1) - EBO_M(vy1, ynum, i, j)) / EBO_V(xstp_vy, j) -
                 (EBO_M(vx1, ynum + 1, i + 1, j) - EBO_M(vx1, ynum + 1, i, j)) / EBO_V(ystp_vx, i));
+}
+
+/* One axis of a uniform level grid (Initialization.jl:47-74 for the basic
+ * nodes, VxGridTools.jl:40-99 for the staggered nodes with ghosts,
+ * PressureGridTools.jl:34-86 for the pressure nodes, SxyGridTools.jl:30-40 for
+ * the shear-centre nodes).  Buffers: g_b[n], stp_b[n-1], g_v[n+1], stp_v[n],
+ * g_p[n-1], stp_p[n-2], g_s[n-1]. */
+void ebo_axis_uniform(int64_t n, double size, double *g_b, double *stp_b, double *g_v,
+                      double *stp_v, double *g_p, double *stp_p, double *g_s)
+{
+    basic_axis(n, size / (double)(n - 1), g_b, stp_b);
+    ebo_axis_from_basic(n, g_b, stp_b, g_v, stp_v, g_p, stp_p, g_s);
+}
+
+/* The same for given basic node coordinates g_b (Spacing.jl:124-133 for the
+ * basic steps). */
+void ebo_axis_from_basic(int64_t n, const double *g_b, double *stp_b, double *g_v, double *stp_v,
+                         double *g_p, double *stp_p, double *g_s)
+{
+    for (int64_t i = 1; i <= n - 1; ++i) EBO_V(stp_b, i) = EBO_V(g_b, i + 1) - EBO_V(g_b, i);
+    staggered_axis(n, g_b, stp_b, g_v, stp_v);
+    pressure_axis(n, g_b, g_p, stp_p);
+    EBO_V(g_s, 1) = EBO_V(g_b, 1) + EBO_V(stp_b, 1) / 2.0;
+    EBO_V(g_s, n - 1) = EBO_V(g_b, n) - EBO_V(stp_b, n - 1) / 2.0;
+    for (int64_t i = 2; i <= n - 2; ++i) EBO_V(g_s, i) = EBO_V(g_b, i) + EBO_V(stp_b, i) / 2.0;
 }
