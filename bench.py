@@ -43,7 +43,47 @@ def parse():
     ap.add_argument("--no-stokes", action="store_true")
     ap.add_argument("--no-scatter", action="store_true")
     ap.add_argument("--stokes-max-iter", type=int, default=3000)
+    ap.add_argument("--no-stokes3d", action="store_true")
+    ap.add_argument("--stokes3d-n", type=int, default=257)
+    ap.add_argument("--stokes3d-max-iter", type=int, default=300)
     return ap.parse_args()
+
+
+def bench_stokes3d(ctx, n, max_iter):
+    """3-D inclusion problem on n^3 nodes (unit cube, 8 seeded spheres of
+    radius 0.05-0.12, density 1 +- 1 %, body force in z, free slip): time to
+    ||b - A x|| / ||b|| <= 1e-8 (FP64) with the FP32 and the FP64 V-cycle as
+    preconditioner, at the inclusion contrast range 1e2-1e6 and at contrast 1e2."""
+    import torch
+    from earthbox_gpu import workloads
+    from earthbox_gpu.stokes3d import Stokes3DSolverGPU
+    out = {"rtol": 1e-8, "n": n}
+    # contrast 1e2: FGMRES(30); contrast 1e2-1e6: GMRES(200) keeping only the
+    # basis (the stiff spheres' near-rigid modes need the long restart)
+    for tag, lc, cap, kry in (("contrast_1e2", (2.0, 2.0 + 1e-9), max_iter, dict(restart=30, flexible=1)),
+                              ("contrast_1e2_1e6", (2.0, 6.0), 2 * max_iter, dict(restart=200, flexible=0))):
+        w = workloads.spheres3d(n=n, seed=3, log_contrast=lc, device=ctx.device)
+        sol = Stokes3DSolverGPU(ctx)
+        sol.set_system(*w["dims"], w["etan"], w["etaxy"], w["etaxz"], w["etayz"])
+        rhs = sol.pack({"vx": w["RX"], "vy": w["RY"], "vz": w["RZ"], "pr": w["RC"]})
+        import math
+        leg = {"sphere_log10_contrast": [round(math.log10(float(v)), 3) for v in w["spheres"]["eta"]],
+               "krylov": kry}
+        for name, prec in (("mixed_fp32_mg", 1), ("plain_fp64_mg", 0)):
+            sol.solve(rhs, precision=prec, rtol=1e-8, max_iterations=2, **kry)   # build + warm
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            x, st = sol.solve(rhs, precision=prec, rtol=1e-8, max_iterations=cap, **kry)
+            torch.cuda.synchronize()
+            leg[name] = {"seconds": time.perf_counter() - t0, "iterations": st["iterations"],
+                         "relres": st["relres"], "converged": st["converged"]}
+            del x
+        if leg["mixed_fp32_mg"]["converged"] and leg["plain_fp64_mg"]["converged"]:
+            leg["speedup_mixed_vs_plain"] = leg["plain_fp64_mg"]["seconds"] / leg["mixed_fp32_mg"]["seconds"]
+        out[tag] = leg
+        del sol, rhs, w
+        torch.cuda.empty_cache()
+    return out
 
 
 def bench_stokes(ctx, nx, ny, max_iter):
@@ -205,6 +245,8 @@ def main():
         out["stokes"] = bench_stokes(ctx, nx, ny, args.stokes_max_iter)
     if not args.no_scatter:
         out["scatter"] = bench_scatter(ctx, g, x, y)
+    if not args.no_stokes3d:
+        out["stokes3d"] = bench_stokes3d(ctx, args.stokes3d_n, args.stokes3d_max_iter)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, x, y, vx1, vy1, to_host_np(desp), dt, 2_000_000)
     print(json.dumps(out), flush=True)

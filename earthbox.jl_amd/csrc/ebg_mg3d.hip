@@ -242,11 +242,30 @@ static void residuals(const MG3LevelDev<T>& l, hipStream_t s)
     hipLaunchKernelGGL((mg3_residual_kernel<T, false>), dim3(nb3(n)), dim3(256), 0, s, l);
 }
 
-// smoother/Smoother.jl:50-64 with StokesSolverGaussSeidel.jl:16-28
 template <typename T>
-void mg3_smoother(MG3Hierarchy<T>& H, int k, int nsweeps, T theta_s, T theta_c, T prnorm, hipStream_t s)
+void mg3_zero_solution(MG3Hierarchy<T>& H, int k, hipStream_t s)
 {
     const MG3LevelDev<T>& l = H.lev[k];
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(nb3((int64_t)l.nvx())), dim3(256), 0, s, l.vx, (int64_t)l.nvx(), T(0));
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(nb3((int64_t)l.nvy())), dim3(256), 0, s, l.vy, (int64_t)l.nvy(), T(0));
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(nb3((int64_t)l.nvz())), dim3(256), 0, s, l.vz, (int64_t)l.nvz(), T(0));
+    hipLaunchKernelGGL(fill_kernel<T>, dim3(nb3((int64_t)l.npr())), dim3(256), 0, s, l.pr, (int64_t)l.npr(), T(0));
+}
+
+// smoother/Smoother.jl:50-64 with StokesSolverGaussSeidel.jl:16-28; levels
+// of at most H.fused_nodes nodes run as one single-workgroup launch
+template <typename T>
+static void smoother_impl(MG3Hierarchy<T>& H, int k, int nsweeps, T theta_s, T theta_c, T prnorm, bool zero_first,
+                          hipStream_t s)
+{
+    const MG3LevelDev<T>& l = H.lev[k];
+    const int64_t nodes = (int64_t)(l.ynum + 1) * (l.xnum + 1) * (l.znum + 1);
+    if (nodes <= H.fused_nodes) {
+        hipLaunchKernelGGL(mg3_smoother_small_kernel<T>, dim3(1), dim3(1024), 0, s, l, nsweeps, theta_s, theta_c,
+                           prnorm, zero_first ? 1 : 0);
+        return;
+    }
+    if (zero_first) mg3_zero_solution(H, k, s);
     for (int sw = 0; sw < nsweeps; ++sw) {
         mg3_set_bc(l, s);
         for (int color = 0; color <= 1; ++color) {
@@ -260,6 +279,12 @@ void mg3_smoother(MG3Hierarchy<T>& H, int k, int nsweeps, T theta_s, T theta_c, 
     hipLaunchKernelGGL(mg3_pressure_shift_kernel<T>, dim3(1), dim3(64), 0, s, l.pr, prnorm, H.scratch);
     hipLaunchKernelGGL(mg3_pressure_add_kernel<T>, dim3(nb3(np)), dim3(256), 0, s, l.pr, np, (const T*)H.scratch);
     residuals(l, s);
+}
+
+template <typename T>
+void mg3_smoother(MG3Hierarchy<T>& H, int k, int nsweeps, T theta_s, T theta_c, T prnorm, hipStream_t s)
+{
+    smoother_impl(H, k, nsweeps, theta_s, theta_c, prnorm, false, s);
 }
 
 // restriction/Restriction.jl:64-376
@@ -283,16 +308,6 @@ void mg3_prolong(MG3Hierarchy<T>& H, int k, T relax_v, T relax_p, hipStream_t s)
     hipLaunchKernelGGL(mg3_prolong_kernel<T>, dim3(nb3(n)), dim3(256), 0, s, c, f, H.xfer[k - 1], relax_v, relax_p);
 }
 
-template <typename T>
-void mg3_zero_solution(MG3Hierarchy<T>& H, int k, hipStream_t s)
-{
-    const MG3LevelDev<T>& l = H.lev[k];
-    hipLaunchKernelGGL(fill_kernel<T>, dim3(nb3((int64_t)l.nvx())), dim3(256), 0, s, l.vx, (int64_t)l.nvx(), T(0));
-    hipLaunchKernelGGL(fill_kernel<T>, dim3(nb3((int64_t)l.nvy())), dim3(256), 0, s, l.vy, (int64_t)l.nvy(), T(0));
-    hipLaunchKernelGGL(fill_kernel<T>, dim3(nb3((int64_t)l.nvz())), dim3(256), 0, s, l.vz, (int64_t)l.nvz(), T(0));
-    hipLaunchKernelGGL(fill_kernel<T>, dim3(nb3((int64_t)l.npr())), dim3(256), 0, s, l.pr, (int64_t)l.npr(), T(0));
-}
-
 // vcycle/core/SmoothAndRestrict.jl:15-46 then SmoothAndProlongate.jl:14-60
 template <typename T>
 void mg3_vcycle(MG3Hierarchy<T>& H, const std::vector<int>& sm, T theta_s, T theta_c, T relax_v, T relax_p,
@@ -300,8 +315,7 @@ void mg3_vcycle(MG3Hierarchy<T>& H, const std::vector<int>& sm, T theta_s, T the
 {
     const int nl = H.nlev();
     for (int n = 0; n < nl; ++n) {
-        if (n > 0) mg3_zero_solution(H, n, s);
-        mg3_smoother(H, n, sm[n], theta_s, theta_c, prnorm, s);
+        smoother_impl(H, n, sm[n], theta_s, theta_c, prnorm, n > 0, s);
         if (n < nl - 1) mg3_restrict(H, n, s);
     }
     for (int n = nl - 1; n >= 0; --n) {
@@ -401,6 +415,15 @@ int ebg_mg3d_create(ebg_ctx* ctx, int64_t xnum, int64_t ynum, int64_t znum, doub
     if (rc) return rc;
     if (levels_out) *levels_out = m->H.nlev();
     ctx->mg3_ref = m.release();
+    return EBG_OK;
+}
+
+int ebg_mg3d_set_fused_limit(ebg_ctx* ctx, int64_t nodes)
+{
+    MG3Ref* m = get3(ctx);
+    EBG_CHECK(m, EBG_ERR_STATE, "ebg_mg3d_set_fused_limit: no hierarchy");
+    EBG_CHECK(nodes >= 0, EBG_ERR_ARG, "ebg_mg3d_set_fused_limit: nodes must be >= 0");
+    m->H.fused_nodes = nodes;
     return EBG_OK;
 }
 

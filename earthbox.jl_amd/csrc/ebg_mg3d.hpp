@@ -305,6 +305,93 @@ __global__ void __launch_bounds__(256) mg3_residual_kernel(MG3LevelDev<T> l)
     }
 }
 
+// A whole smoother call (optional zeroing of the solution, nsweeps x [ghost
+// pass; four phases per colour], pressure normalisation, residuals) for a
+// level small enough that kernel launches would dominate: one workgroup,
+// a barrier between dependent steps.  Same arithmetic and order as the
+// multi-launch path, so results are identical.
+template <typename T>
+__global__ void __launch_bounds__(1024) mg3_smoother_small_kernel(MG3LevelDev<T> l, int nsweeps, T theta_s,
+                                                                  T theta_c, T prnorm, int zero_first)
+{
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (zero_first) {
+        for (size_t t = tid; t < l.nvx(); t += nt) l.vx[t] = T(0);
+        for (size_t t = tid; t < l.nvy(); t += nt) l.vy[t] = T(0);
+        for (size_t t = tid; t < l.nvz(); t += nt) l.vz[t] = T(0);
+        for (size_t t = tid; t < l.npr(); t += nt) l.pr[t] = T(0);
+        __syncthreads();
+    }
+    const int nb = l.nbx + l.nby + l.nbz;
+    for (int sw = 0; sw < nsweeps; ++sw) {
+        for (int t = tid; t < nb; t += nt) {
+            const T* base = t < l.nbx ? l.vx : (t < l.nbx + l.nby ? l.vy : l.vz);
+            const int s = l.bc_src[t];
+            l.bc_tmp[t] = s >= 0 ? base[s] : T(0);
+        }
+        __syncthreads();
+        for (int t = tid; t < nb; t += nt) {
+            T* base = t < l.nbx ? l.vx : (t < l.nbx + l.nby ? l.vy : l.vz);
+            base[l.bc_dst[t]] = l.bc_tmp[t];
+        }
+        __syncthreads();
+        for (int color = 0; color <= 1; ++color) {
+            for (int phase = 0; phase < 4; ++phase) {
+                const Box3 b = phase_box(l, phase);
+                const int nh = (b.ihi - b.ilo + 2) / 2, nj = b.jhi - b.jlo + 1;
+                const int total = nh * nj * (b.khi - b.klo + 1);
+                for (int t = tid; t < total; t += nt) {
+                    const int ii = t % nh, r = t / nh;
+                    const int j = b.jlo + r % nj, k = b.klo + r / nj;
+                    const int i = b.ilo + 2 * ii + ((color + b.ilo + j + k) & 1);
+                    if (i > b.ihi) continue;
+                    T coef;
+                    if (phase == 0) {
+                        const T dR = vx_res3<T, false>(l, i, j, k, &coef);
+                        l.VX(i, j, k) += dR / coef * theta_s;
+                    } else if (phase == 1) {
+                        const T dR = vy_res3<T, false>(l, i, j, k, &coef);
+                        l.VY(i, j, k) += dR / coef * theta_s;
+                    } else if (phase == 2) {
+                        const T dR = vz_res3<T, false>(l, i, j, k, &coef);
+                        l.VZ(i, j, k) += dR / coef * theta_s;
+                    } else {
+                        const T dR = pr_res3<T, false>(l, i, j, k);
+                        l.PR(i, j, k) += l.EN(i, j, k) * dR * theta_c;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    const T dp = prnorm - l.pr[0];
+    __syncthreads();
+    for (size_t t = tid; t < l.npr(); t += nt) l.pr[t] += dp;
+    __syncthreads();
+    const int X = l.xnum, Y = l.ynum, Z = l.znum;
+    const int ni = Y + 1, njj = X + 1, total = ni * njj * (Z + 1);
+    for (int t = tid; t < total; t += nt) {
+        const int i = 1 + t % ni, r = t / ni;
+        const int j = 1 + r % njj, k = 1 + r / njj;
+        if (j <= X) {
+            T v = T(0);
+            if (i >= 2 && i <= Y && j >= 2 && j <= X - 1 && k >= 2 && k <= Z) v = vx_res3<T, false>(l, i, j, k, nullptr);
+            l.rx[i3(Y + 1, X, i, j, k)] = v;
+        }
+        if (i <= Y) {
+            T v = T(0);
+            if (i >= 2 && i <= Y - 1 && j >= 2 && j <= X && k >= 2 && k <= Z) v = vy_res3<T, false>(l, i, j, k, nullptr);
+            l.ry[i3(Y, X + 1, i, j, k)] = v;
+        }
+        if (k <= Z) {
+            T v = T(0);
+            if (i >= 2 && i <= Y && j >= 2 && j <= X && k >= 2 && k <= Z - 1) v = vz_res3<T, false>(l, i, j, k, nullptr);
+            l.rz[i3(Y + 1, X + 1, i, j, k)] = v;
+        }
+        if (i <= Y - 1 && j <= X - 1 && k <= Z - 1) l.rc[i3(Y - 1, X - 1, i, j, k)] = pr_res3<T, false>(l, i, j, k);
+    }
+}
+
 template <typename T>
 __global__ void mg3_pressure_shift_kernel(const T* pr, T prnorm, T* scratch)
 {
@@ -484,6 +571,9 @@ struct MG3Hierarchy {
     std::vector<Transfer3Dev<T>> xfer;           // xfer[k]: level k -> k+1
     std::vector<void*> allocations;
     T* scratch = nullptr;
+    // levels with at most this many nodes ((ynum+1)(xnum+1)(znum+1)) smooth
+    // in one single-workgroup launch (launch overhead dominates there)
+    int64_t fused_nodes = 2000;
     int nlev() const { return (int)lev.size(); }
     ~MG3Hierarchy();
 };

@@ -1067,6 +1067,7 @@ int ebg_stokes_solve_opts_default(ebg_solve_opts* o)
     o->smoother = 1;
     o->local_sweeps = 2;  // more sweeps per halo refresh let the tile pressures drift
     o->tile = 16;
+    o->flexible = 1;
     return EBG_OK;
 }
 

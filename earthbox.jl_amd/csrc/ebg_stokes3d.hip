@@ -3,8 +3,13 @@
 // level grid with the free-slip ghost passes of BoundaryConditions.jl:92-174).
 //
 // The reference drives that system only with V-cycles (MultigridVCycle.jl:
-// 79-135).  Here FGMRES(m) in FP64 solves it, right-preconditioned by one
-// V-cycle of the same multigrid in FP32 (mixed precision) or FP64 (plain).
+// 79-135).  Here GMRES(m) in FP64 solves it, right-preconditioned by one
+// V-cycle of the same multigrid in FP32 (mixed precision) or FP64 (plain):
+// flexible (every preconditioned vector kept) or, to fit twice the restart
+// length in HBM, with only the basis kept and the V-cycle applied once more
+// to the combined correction at each restart.  Stiff inclusions leave a few
+// small eigenvalues (their near-rigid motions) that short restarts cannot
+// resolve; restart lengths up to 240 are supported for that reason.
 // Unknowns: interior velocity nodes and every pressure cell, stored in the
 // level layout [vx | vy | vz | pr] (ghost slots carried along, zero in every
 // Krylov basis vector).  Equations: the interior momentum and continuity
@@ -16,6 +21,7 @@
 #include "ebg_mg.hpp"
 #include "ebg_mg3d.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <memory>
@@ -184,11 +190,22 @@ static void precondition3(Stokes3State& st, MG3Hierarchy<T>& H, const ebg_solve_
                            z + off[c], (int64_t)len[c]);
 }
 
+// longest restart cycle of the 3-D solver: the Krylov basis lives in HBM,
+// (2m+1) vectors of the level layout (about 1.1 GB each at 257^3 in FP64)
+constexpr int MAX_RESTART3 = 240;
+constexpr int DOT_CHUNK = 32;  // basis vectors per multi-dot launch
+
+// out[q] = <w, V[q]> for q < k, in chunks of DOT_CHUNK vectors (fixed order)
 static void multidot3(Stokes3State& st, const double* w, const double* V, int64_t ldv, int k, int64_t n, double* out,
                       hipStream_t s)
 {
-    hipLaunchKernelGGL(multidot_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, w, V, ldv, k, n, st.partial.as<double>());
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, st.partial.as<double>(), RED_BLOCKS, k, out, 0);
+    for (int q0 = 0; q0 < k; q0 += DOT_CHUNK) {
+        const int kc = std::min(DOT_CHUNK, k - q0);
+        hipLaunchKernelGGL(multidot_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, w, V + (int64_t)q0 * ldv, ldv, kc, n,
+                           st.partial.as<double>());
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, st.partial.as<double>(), RED_BLOCKS, kc,
+                           out + q0, 0);
+    }
 }
 
 static int prepare3(ebg_ctx* ctx, Stokes3State& st, const ebg_solve_opts& o, std::vector<int>& sm)
@@ -219,11 +236,13 @@ static int fgmres3(ebg_ctx* ctx, Stokes3State& st, const ebg_solve_opts& o, cons
     hipStream_t s = ctx->stream;
     const int64_t n = (int64_t)st.n;
     const int m = o.restart;
+    const bool flexible = o.flexible != 0;
     int rc;
     if ((rc = st.b.ensure(n * 8)) || (rc = st.r.ensure(n * 8)) || (rc = st.w.ensure(n * 8)) ||
-        (rc = st.V.ensure((size_t)(m + 1) * n * 8)) || (rc = st.Z.ensure((size_t)m * n * 8)) ||
-        (rc = st.partial.ensure((size_t)RED_BLOCKS * MAXK * 8)) || (rc = st.hdev.ensure(MAXK * 8 * 2)) ||
-        (rc = st.ydev.ensure(MAXK * 8)) || (rc = st.nrm.ensure(64)))
+        (rc = st.V.ensure((size_t)(m + 1) * n * 8)) || (rc = st.Z.ensure((size_t)(flexible ? m : 1) * n * 8)) ||
+        (rc = st.partial.ensure((size_t)RED_BLOCKS * DOT_CHUNK * 8)) ||
+        (rc = st.hdev.ensure((size_t)(MAX_RESTART3 + 1) * 8 * 2)) ||
+        (rc = st.ydev.ensure((size_t)(MAX_RESTART3 + 1) * 8)) || (rc = st.nrm.ensure(64)))
         return rc;
     std::vector<int> sm;
     if ((rc = prepare3(ctx, st, o, sm))) return rc;
@@ -252,6 +271,8 @@ static int fgmres3(ebg_ctx* ctx, Stokes3State& st, const ebg_solve_opts& o, cons
                            prnorm);
     }
     auto norm2 = [&](const double* v, double* out) { multidot3(st, v, v, 0, 1, n, out, s); };
+    const int HOFF = MAX_RESTART3 + 1;  // second Gram-Schmidt pass coefficients
+    std::vector<double> hbuf(2 * (size_t)HOFF);
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1);
     double bnorm2 = 0;
     norm2(b, st.nrm.as<double>());
@@ -289,20 +310,20 @@ static int fgmres3(ebg_ctx* ctx, Stokes3State& st, const ebg_solve_opts& o, cons
         int k = 0;
         for (; k < m && total < o.max_iterations; ++k) {
             double* vk = V + (int64_t)k * n;
-            double* zk = Z + (int64_t)k * n;
+            double* zk = flexible ? Z + (int64_t)k * n : Z;
             apply_m(st, o, sm, vk, zk, s);
             mg3_apply(op_view(st, zk, w), s);
             // classical Gram-Schmidt, twice (CGS2)
             multidot3(st, w, V, n, k + 1, n, hd, s);
             hipLaunchKernelGGL(update_kernel, dim3(nbk(n)), dim3(256), 0, s, w, V, n, hd, k + 1, n);
-            multidot3(st, w, V, n, k + 1, n, hd + MAXK, s);
-            hipLaunchKernelGGL(update_kernel, dim3(nbk(n)), dim3(256), 0, s, w, V, n, hd + MAXK, k + 1, n);
+            multidot3(st, w, V, n, k + 1, n, hd + HOFF, s);
+            hipLaunchKernelGGL(update_kernel, dim3(nbk(n)), dim3(256), 0, s, w, V, n, hd + HOFF, k + 1, n);
             norm2(w, st.nrm.as<double>());
-            double hbuf[2 * MAXK], wn2;
-            EBG_HIP(hipMemcpyAsync(hbuf, hd, sizeof(double) * 2 * MAXK, hipMemcpyDeviceToHost, s));
+            double wn2;
+            EBG_HIP(hipMemcpyAsync(hbuf.data(), hd, sizeof(double) * 2 * HOFF, hipMemcpyDeviceToHost, s));
             EBG_HIP(hipMemcpyAsync(&wn2, st.nrm.p, 8, hipMemcpyDeviceToHost, s));
             EBG_HIP(hipStreamSynchronize(s));
-            for (int q = 0; q <= k; ++q) H[(size_t)k * (m + 1) + q] = hbuf[q] + hbuf[MAXK + q];
+            for (int q = 0; q <= k; ++q) H[(size_t)k * (m + 1) + q] = hbuf[q] + hbuf[HOFF + q];
             const double hn = std::sqrt(wn2);
             H[(size_t)k * (m + 1) + k + 1] = hn;
             if (!std::isfinite(hn)) {
@@ -341,7 +362,17 @@ static int fgmres3(ebg_ctx* ctx, Stokes3State& st, const ebg_solve_opts& o, cons
             y[q] = v / H[(size_t)q * (m + 1) + q];
         }
         EBG_HIP(hipMemcpyAsync(st.ydev.p, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(combine_kernel, dim3(nbk(n)), dim3(256), 0, s, x, Z, n, st.ydev.as<double>(), k, n);
+        if (flexible) {
+            hipLaunchKernelGGL(combine_kernel, dim3(nbk(n)), dim3(256), 0, s, x, Z, n, st.ydev.as<double>(), k, n);
+        } else {
+            // x += M (V y): the preconditioner applied once to the combination
+            static const double one = 1.0;
+            hipLaunchKernelGGL(fill_kernel<double>, dim3(nbk(n)), dim3(256), 0, s, r, n, 0.0);
+            hipLaunchKernelGGL(combine_kernel, dim3(nbk(n)), dim3(256), 0, s, r, V, n, st.ydev.as<double>(), k, n);
+            apply_m(st, o, sm, r, Z, s);
+            EBG_HIP(hipMemcpyAsync(st.ydev.p, &one, 8, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(combine_kernel, dim3(nbk(n)), dim3(256), 0, s, x, Z, n, st.ydev.as<double>(), 1, n);
+        }
         ++restarts;
     }
     // leave the ghost nodes of the solution consistent with its interior
@@ -463,7 +494,8 @@ int ebg_stokes3d_solve(ebg_ctx* ctx, const double* rhs, double prnorm, double* x
     ebg_solve_opts o;
     if (opts) o = *opts;
     else ebg_stokes_solve_opts_default(&o);
-    EBG_CHECK(o.restart >= 1 && o.restart < MAXK - 1, EBG_ERR_ARG, "ebg_stokes3d_solve: restart must be in 1..62");
+    EBG_CHECK(o.restart >= 1 && o.restart <= MAX_RESTART3, EBG_ERR_ARG,
+              "ebg_stokes3d_solve: restart must be in 1..240");
     EBG_CHECK(o.precision == 0 || o.precision == 1, EBG_ERR_ARG, "ebg_stokes3d_solve: precision must be 0 or 1");
     EBG_CHECK(o.finest_sweeps >= 0 && o.sweep_multiplier >= 1, EBG_ERR_ARG, "ebg_stokes3d_solve: bad sweep counts");
     EBG_HIP(hipSetDevice(ctx->device));

@@ -74,6 +74,7 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_mg3d_create": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_double, c_double, c_double,
                                 c_int, c_void_p]),
     "ebg_mg3d_level_shape": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "ebg_mg3d_set_fused_limit": (c_int, [c_void_p, c_int64]),
     "ebg_mg3d_copy_field": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "ebg_mg3d_restrict_viscosity": (c_int, [c_void_p]),
     "ebg_mg3d_set_bc": (c_int, [c_void_p, c_int]),
@@ -118,7 +119,7 @@ class SolveOpts(C.Structure):
                 ("sweep_multiplier", c_int), ("sweep_cap", c_int), ("theta_s", c_double),
                 ("theta_c", c_double), ("use_initial_guess", c_int), ("learned", c_int),
                 ("bc_d", c_double * 4), ("smoother", c_int), ("local_sweeps", c_int),
-                ("tile", c_int)]
+                ("tile", c_int), ("flexible", c_int)]
 
 
 class SolveStats(C.Structure):

@@ -69,6 +69,10 @@ class Multigrid3DGPU:
         check(lib.ebg_mg3d_copy_field(self.ctx.handle, level, MG3_FIELDS[name], None,
                                       ptr(t, np.float64)), "ebg_mg3d_copy_field")
 
+    def set_fused_limit(self, nodes: int):
+        """Levels of at most `nodes` nodes smooth in one single-workgroup launch."""
+        check(lib.ebg_mg3d_set_fused_limit(self.ctx.handle, int(nodes)), "ebg_mg3d_set_fused_limit")
+
     def restrict_viscosity(self):
         check(lib.ebg_mg3d_restrict_viscosity(self.ctx.handle), "ebg_mg3d_restrict_viscosity")
 

@@ -37,6 +37,7 @@ struct SolveOpts
     smoother::Cint
     local_sweeps::Cint
     tile::Cint
+    flexible::Cint
 end
 
 struct SolveStats
@@ -141,7 +142,8 @@ function solve_system_gpu!(st::GPUState, model; mixed::Bool=true, rtol::Float64=
                     o[].finest_sweeps, o[].sweep_multiplier, o[].sweep_cap, o[].theta_s,
                     o[].theta_c, 0, 0,
                     (vc.btopx.array[2, 2], vc.bbottomx.array[2, 2], vc.blefty.array[2, 2],
-                     vc.brighty.array[2, 2]), o[].smoother, o[].local_sweeps, o[].tile)
+                     vc.brighty.array[2, 2]), o[].smoother, o[].local_sweeps, o[].tile,
+                    o[].flexible)
     stats = Ref{SolveStats}()
     rc = check(ccall((:ebg_stokes_solve, LIB), Cint,
         (Ptr{Cvoid}, Ptr{Cvoid}, Ref{SolveOpts}, Ref{SolveStats}), st.ctx, dS, o, stats),

@@ -221,6 +221,10 @@ typedef struct {
                              reference multigrid's order */
     int local_sweeps;     /* tiled smoother: sweeps per tile launch between halo refreshes */
     int tile;             /* tiled smoother: tile edge (nodes) on levels that exceed one LDS tile */
+    int flexible;         /* 1: flexible GMRES, storing every preconditioned vector (default);
+                             0 (3-D solve only): right-preconditioned GMRES storing only the
+                             basis, the V-cycle applied once more per restart (half the memory,
+                             so twice the restart length fits) */
 } ebg_solve_opts;
 
 typedef struct {
@@ -331,6 +335,10 @@ enum {
 int ebg_mg3d_create(ebg_ctx *ctx, int64_t xnum, int64_t ynum, int64_t znum, double xsize, double ysize,
                     double zsize, int nlevels, int *levels_out);
 int ebg_mg3d_level_shape(ebg_ctx *ctx, int level, int64_t *xnum, int64_t *ynum, int64_t *znum);
+/* Levels of at most `nodes` nodes ((ynum+1)(xnum+1)(znum+1), default 2000)
+ * run each smoother call as one single-workgroup launch; results are
+ * identical either way (no reference counterpart: a launch-overhead knob). */
+int ebg_mg3d_set_fused_limit(ebg_ctx *ctx, int64_t nodes);
 /* Copy one level field (EBG_MG3_*) to dst or from src (device pointers, one NULL). */
 int ebg_mg3d_copy_field(ebg_ctx *ctx, int level, int which, double *dst, const double *src);
 /* ViscosityRestriction.viscosity_restriction3d! on every level
@@ -377,7 +385,7 @@ int ebg_stokes3d_apply(ebg_ctx *ctx, double *x, double *y);
 int ebg_stokes3d_precondition(ebg_ctx *ctx, const double *r, double *z, const ebg_solve_opts *opts);
 /* Solve A x = b where b is rhs = [RX | RY | RZ | RC] on the unknowns (ghost
  * entries ignored) with the gauge entry prnorm/etan(1,1,1).  Uses opts.precision,
- * restart, max_iterations, rtol, nlevels, finest_sweeps, sweep_multiplier,
+ * restart (1..240), max_iterations, rtol, nlevels, finest_sweeps, sweep_multiplier,
  * sweep_cap, theta_s, theta_c, use_initial_guess; the 2-D-only fields are
  * ignored.  x (device, vector layout) receives the solution with its ghost
  * nodes set. */

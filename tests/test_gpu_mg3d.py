@@ -82,10 +82,14 @@ def test_ghost_pass_bit_exact(gpu_ctx, case):
         _same(H, G, level, ("vx", "vy", "vz"))
 
 
+@pytest.mark.parametrize("fused", [0, 10 ** 9])
 @pytest.mark.parametrize("case", CASES)
-def test_smoother_bit_exact(gpu_ctx, case):
-    print(f"mg3d smoother {case}", flush=True)
+def test_smoother_bit_exact(gpu_ctx, case, fused):
+    """Both smoother paths: one launch per phase (fused limit 0) and the
+    single-workgroup launch per smoother call (every level fused)."""
+    print(f"mg3d smoother {case} fused limit {fused}", flush=True)
     H, G, rng = _pair(gpu_ctx, **case)
+    G.set_fused_limit(fused)
     for level in range(len(H.levels)):
         _fill(H, G, level, SOL + RHS, rng)
         H.smoother(level, 3, 0.9, 0.3, 0.37)
@@ -109,10 +113,12 @@ def test_restrict_and_prolong_bit_exact(gpu_ctx, case):
         _same(H, G, level, SOL)
 
 
+@pytest.mark.parametrize("fused", [0, 12000])
 @pytest.mark.parametrize("case", CASES[:2])
-def test_vcycles_bit_exact(gpu_ctx, case):
-    print(f"mg3d vcycles {case}", flush=True)
+def test_vcycles_bit_exact(gpu_ctx, case, fused):
+    print(f"mg3d vcycles {case} fused limit {fused}", flush=True)
     H, G, rng = _pair(gpu_ctx, **case)
+    G.set_fused_limit(fused)
     _fill(H, G, 0, RHS, rng)
     for k in SOL:
         G.set(0, k, to_device(getattr(H.levels[0], k)))
@@ -147,10 +153,10 @@ def test_operator_apply_bit_exact(gpu_ctx, case):
         assert np.array_equal(to_host(got[k]), y[k]), k
 
 
-@pytest.mark.parametrize("precision", [1, 0])
+@pytest.mark.parametrize("precision,flexible,restart", [(1, 1, 30), (0, 1, 30), (1, 0, 60), (0, 0, 60)])
 @pytest.mark.parametrize("case,prnorm", [(CASES[0], 0.0), (CASES[1], 1.5)])
-def test_solve_matches_direct(gpu_ctx, case, prnorm, precision):
-    print(f"stokes3d solve {case} precision {precision}", flush=True)
+def test_solve_matches_direct(gpu_ctx, case, prnorm, precision, flexible, restart):
+    print(f"stokes3d solve {case} precision {precision} flexible {flexible}", flush=True)
     lv, sol, rng = _system(gpu_ctx, **case)
     R = {k: np.asfortranarray(rng.normal(size=lv.sh[s])) for k, s in zip(RHS, SOL)}
     A, masks = mg3.assemble(lv)
@@ -158,7 +164,8 @@ def test_solve_matches_direct(gpu_ctx, case, prnorm, precision):
     b[b.size - int(masks["pr"].sum())] = prnorm / lv.etan.flat[0]
     xs = mg3.unpack(spla.spsolve(A.tocsc(), b), lv, masks)
     rhs = sol.pack({s: to_device(R[k]) for k, s in zip(RHS, SOL)})
-    x, st = sol.solve(rhs, prnorm=prnorm, precision=precision, rtol=1e-11, max_iterations=400)
+    x, st = sol.solve(rhs, prnorm=prnorm, precision=precision, rtol=1e-11, max_iterations=400,
+                      flexible=flexible, restart=restart)
     print(st, flush=True)
     assert st["converged"], st
     got = sol.split(x)
