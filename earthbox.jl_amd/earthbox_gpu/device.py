@@ -78,8 +78,9 @@ def ptr(t: torch.Tensor | None, dtype=None, numel: int | None = None) -> int | N
         if not t.T.is_contiguous():
             raise ValueError("2-D fields must be column-major (ny, nx) views")
     elif t.dim() == 3:
-        if not t.permute(2, 1, 0).is_contiguous():
-            raise ValueError("3-D fields must be column-major (ny, nx, nz) views")
+        # column-major grid fields (ny, nx, nz) or row-major images (H, W, C)
+        if not (t.permute(2, 1, 0).is_contiguous() or t.is_contiguous()):
+            raise ValueError("3-D arrays must be dense column-major fields or row-major images")
     elif not t.is_contiguous():
         raise ValueError("vectors must be contiguous")
     if numel is not None and t.numel() != numel:
