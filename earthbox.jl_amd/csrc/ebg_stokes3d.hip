@@ -191,9 +191,39 @@ static void precondition3(Stokes3State& st, MG3Hierarchy<T>& H, const ebg_solve_
 }
 
 // longest restart cycle of the 3-D solver: the Krylov basis lives in HBM,
-// (2m+1) vectors of the level layout (about 1.1 GB each at 257^3 in FP64)
+// (m+1) vectors of the level layout (545 MB each at 257^3 in FP64), plus m
+// preconditioned vectors in the flexible mode
 constexpr int MAX_RESTART3 = 240;
-constexpr int DOT_CHUNK = 32;  // basis vectors per multi-dot launch
+constexpr int DOT_CHUNK = 8;  // basis vectors per multi-dot pass (register accumulators)
+
+// partial[b*kc + q] = block b's grid-stride share of <w, V[q]>, q < kc <= 8;
+// accumulators stay in registers (a fixed-size, fully unrolled array)
+__global__ void __launch_bounds__(256) multidot8_kernel(const double* __restrict__ w, const double* __restrict__ V,
+                                                        int64_t ldv, int kc, int64_t n, double* __restrict__ partial)
+{
+    __shared__ double sh[4][DOT_CHUNK];
+    double acc[DOT_CHUNK];
+#pragma unroll
+    for (int q = 0; q < DOT_CHUNK; ++q) acc[q] = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const double wv = w[t];
+#pragma unroll
+        for (int q = 0; q < DOT_CHUNK; ++q)
+            if (q < kc) acc[q] += wv * V[(int64_t)q * ldv + t];
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < DOT_CHUNK; ++q) {
+        double v = acc[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) sh[wid][q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kc) {
+        const int q = threadIdx.x;
+        partial[(int64_t)blockIdx.x * kc + q] = ((sh[0][q] + sh[1][q]) + sh[2][q]) + sh[3][q];
+    }
+}
 
 // out[q] = <w, V[q]> for q < k, in chunks of DOT_CHUNK vectors (fixed order)
 static void multidot3(Stokes3State& st, const double* w, const double* V, int64_t ldv, int k, int64_t n, double* out,
@@ -201,7 +231,7 @@ static void multidot3(Stokes3State& st, const double* w, const double* V, int64_
 {
     for (int q0 = 0; q0 < k; q0 += DOT_CHUNK) {
         const int kc = std::min(DOT_CHUNK, k - q0);
-        hipLaunchKernelGGL(multidot_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, w, V + (int64_t)q0 * ldv, ldv, kc, n,
+        hipLaunchKernelGGL(multidot8_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, w, V + (int64_t)q0 * ldv, ldv, kc, n,
                            st.partial.as<double>());
         hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, st.partial.as<double>(), RED_BLOCKS, kc,
                            out + q0, 0);
@@ -313,17 +343,28 @@ static int fgmres3(ebg_ctx* ctx, Stokes3State& st, const ebg_solve_opts& o, cons
             double* zk = flexible ? Z + (int64_t)k * n : Z;
             apply_m(st, o, sm, vk, zk, s);
             mg3_apply(op_view(st, zk, w), s);
-            // classical Gram-Schmidt, twice (CGS2)
+            // classical Gram-Schmidt; a second pass only when the first one
+            // cancelled more than half of the norm ("twice is enough")
+            norm2(w, st.nrm.as<double>() + 1);
             multidot3(st, w, V, n, k + 1, n, hd, s);
             hipLaunchKernelGGL(update_kernel, dim3(nbk(n)), dim3(256), 0, s, w, V, n, hd, k + 1, n);
-            multidot3(st, w, V, n, k + 1, n, hd + HOFF, s);
-            hipLaunchKernelGGL(update_kernel, dim3(nbk(n)), dim3(256), 0, s, w, V, n, hd + HOFF, k + 1, n);
             norm2(w, st.nrm.as<double>());
-            double wn2;
-            EBG_HIP(hipMemcpyAsync(hbuf.data(), hd, sizeof(double) * 2 * HOFF, hipMemcpyDeviceToHost, s));
-            EBG_HIP(hipMemcpyAsync(&wn2, st.nrm.p, 8, hipMemcpyDeviceToHost, s));
+            double nn[2];
+            EBG_HIP(hipMemcpyAsync(hbuf.data(), hd, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
+            EBG_HIP(hipMemcpyAsync(nn, st.nrm.p, 16, hipMemcpyDeviceToHost, s));
             EBG_HIP(hipStreamSynchronize(s));
-            for (int q = 0; q <= k; ++q) H[(size_t)k * (m + 1) + q] = hbuf[q] + hbuf[HOFF + q];
+            double wn2 = nn[0];
+            for (int q = 0; q <= k; ++q) H[(size_t)k * (m + 1) + q] = hbuf[q];
+            if (!(wn2 > 0.5 * nn[1])) {
+                multidot3(st, w, V, n, k + 1, n, hd + HOFF, s);
+                hipLaunchKernelGGL(update_kernel, dim3(nbk(n)), dim3(256), 0, s, w, V, n, hd + HOFF, k + 1, n);
+                norm2(w, st.nrm.as<double>());
+                EBG_HIP(hipMemcpyAsync(hbuf.data() + HOFF, hd + HOFF, sizeof(double) * (k + 1),
+                                       hipMemcpyDeviceToHost, s));
+                EBG_HIP(hipMemcpyAsync(&wn2, st.nrm.p, 8, hipMemcpyDeviceToHost, s));
+                EBG_HIP(hipStreamSynchronize(s));
+                for (int q = 0; q <= k; ++q) H[(size_t)k * (m + 1) + q] += hbuf[HOFF + q];
+            }
             const double hn = std::sqrt(wn2);
             H[(size_t)k * (m + 1) + k + 1] = hn;
             if (!std::isfinite(hn)) {
