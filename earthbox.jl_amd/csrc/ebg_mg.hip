@@ -427,4 +427,90 @@ void mg_vcycle_tiled(MGHierarchy<T>& H, const std::vector<int>& sm, int local_sw
 EBG_INST(float)
 EBG_INST(double)
 
+void ViscRamp::init(double vmin, double vmax, double njumps)
+{
+    min_o = vmin;
+    max_o = vmax;
+    min_cur = vmin;
+    max_cur = vmin;
+    min_factor = 1.0;
+    max_factor = std::pow(10.0, std::log10(vmax / vmin) / njumps);
+}
+
+void ViscRamp::jump()
+{
+    min_cur = min_cur * min_factor;
+    if (min_cur < min_o) min_cur = min_o;
+    max_cur = max_cur * max_factor;
+    if (max_cur > max_o) max_cur = max_o;
+}
+
+double ViscRamp::coef() const { return std::log(max_cur / min_cur) / std::log(max_o / min_o); }
+
+// per-block minimum and maximum of a[0..n)
+__global__ void __launch_bounds__(256) mg_minmax_partial_kernel(const double* __restrict__ a, int64_t n,
+                                                                double* __restrict__ part)
+{
+    __shared__ double smin[256], smax[256];
+    double lo = INFINITY, hi = -INFINITY;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        lo = fmin(lo, a[t]);
+        hi = fmax(hi, a[t]);
+    }
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + off]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = smin[0];
+        part[2 * blockIdx.x + 1] = smax[0];
+    }
+}
+
+int mg_device_range(const double* a, int64_t n, double* partial, hipStream_t s, double& lo, double& hi)
+{
+    double host[2 * MG_RUN_BLOCKS];
+    hipLaunchKernelGGL(mg_minmax_partial_kernel, dim3(MG_RUN_BLOCKS), dim3(256), 0, s, a, n, partial);
+    EBG_HIP(hipMemcpyAsync(host, partial, sizeof(host), hipMemcpyDeviceToHost, s));
+    EBG_HIP(hipStreamSynchronize(s));
+    for (int b = 0; b < MG_RUN_BLOCKS; ++b) {
+        lo = std::min(lo, host[2 * b]);
+        hi = std::max(hi, host[2 * b + 1]);
+    }
+    return EBG_OK;
+}
+
+// ViscosityScaling.scale_viscosity_for_all_levels! (ViscosityScaling.jl:41-79), one array
+__global__ void __launch_bounds__(256) mg_visc_scale_kernel(const double* __restrict__ orig, double* __restrict__ eta,
+                                                            int64_t n, double min_cur, double inv_min_o, double coef)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) eta[t] = min_cur * pow(orig[t] * inv_min_o, coef);
+}
+
+void mg_scale_viscosity(const double* orig, double* eta, int64_t n, double min_cur, double inv_min_o, double coef,
+                        hipStream_t s)
+{
+    hipLaunchKernelGGL(mg_visc_scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, orig, eta, n,
+                       min_cur, inv_min_o, coef);
+}
+
+double mg_max_principal_residual(const double* res, int n, int ivcycle)
+{
+    if (ivcycle <= 2) return 1e38;
+    double m = 0.0;
+    for (int q = 0; q < n; ++q) {
+        const double v = std::fabs(std::pow(10.0, res[q]));
+        if (std::isnan(v)) return NAN;
+        m = std::max(m, v);
+    }
+    return m;
+}
+
 }  // namespace ebg

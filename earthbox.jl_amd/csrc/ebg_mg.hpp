@@ -551,4 +551,28 @@ void mg_vcycle_tiled(MGHierarchy<T>& H, const std::vector<int>& smoothing, int l
                      T theta_s, T theta_c, T relax_v, T relax_p, T prnorm, const T* prnorm_dev,
                      const T* bc_d, hipStream_t s);
 
+// ---- pieces of the reference's stand-alone multigrid driver (2-D and 3-D) ----
+// Viscosity continuation: the initial range and step factor of
+// MultigridDataManager.jl:349-373, the jump of ViscosityScaling.jl:267-309 and
+// the exponent of scale_viscosity_for_all_levels! (ViscosityScaling.jl:41-79).
+struct ViscRamp {
+    double min_o = 0.0, max_o = 0.0, min_cur = 0.0, max_cur = 0.0, min_factor = 1.0, max_factor = 1.0;
+    void init(double vmin, double vmax, double njumps);
+    bool below_full() const { return max_cur < max_o || min_cur > min_o; }
+    void jump();
+    double coef() const;
+};
+
+constexpr int MG_RUN_BLOCKS = 256;
+// Folds the minimum and maximum of device array a[0..n) into lo and hi
+// (synchronises the stream); partial holds 2 * MG_RUN_BLOCKS doubles.
+int mg_device_range(const double* a, int64_t n, double* partial, hipStream_t s, double& lo, double& hi);
+// eta = min_cur * (orig / min_o)^coef on n entries
+void mg_scale_viscosity(const double* orig, double* eta, int64_t n, double min_cur, double inv_min_o, double coef,
+                        hipStream_t s);
+// calculate_max_principle_residual_{2d,3d} (MultigridVCycle.jl:436-468): the
+// largest 10^res of the cycle's n log10 residuals from the third cycle on,
+// 1e38 before; NaN when any residual is NaN (Julia's max propagates NaN).
+double mg_max_principal_residual(const double* res, int n, int ivcycle);
+
 }  // namespace ebg

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <memory>
 #include <numeric>
 
@@ -356,7 +357,58 @@ EBG_MG3_INST(float)
 namespace {
 struct MG3Ref {
     MG3Hierarchy<double> H;
+    // original (unscaled) viscosity per level for the continuation and the
+    // reduction partials (4 * MG_RUN_BLOCKS); allocated by the first ebg_mg3d_run
+    std::vector<std::array<double*, 4>> orig;
+    double* partial = nullptr;
+    ~MG3Ref()
+    {
+        for (auto& a : orig)
+            for (double* p : a) (void)hipFree(p);
+        if (partial) (void)hipFree(partial);
+    }
 };
+
+// per-block sums of the squared scaled interior residuals of the finest level
+// (CalculateMeanResiduals.jl:14-70): part[4*b + {x, y, z, c}]
+__global__ void __launch_bounds__(256) mean_residual_partial_kernel(MG3LevelDev<double> l, double inv_ss,
+                                                                    double inv_cc, double* __restrict__ part)
+{
+    __shared__ double sh[4][256];
+    const int X = l.xnum, Y = l.ynum, Z = l.znum;
+    const int64_t total = (int64_t)Y * X * Z;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = 1 + (int)(t % Y);
+        const int64_t r = t / Y;
+        const int j = 1 + (int)(r % X), k = 1 + (int)(r / X);
+        if (i > 1 && j > 1 && k > 1 && j < X) {
+            const double v = l.rx[i3(Y + 1, X, i, j, k)] * inv_ss;
+            s[0] += v * v;
+        }
+        if (i > 1 && j > 1 && k > 1 && i < Y) {
+            const double v = l.ry[i3(Y, X + 1, i, j, k)] * inv_ss;
+            s[1] += v * v;
+        }
+        if (i > 1 && j > 1 && k > 1 && k < Z) {
+            const double v = l.rz[i3(Y + 1, X + 1, i, j, k)] * inv_ss;
+            s[2] += v * v;
+        }
+        if (j < X && i < Y && k < Z) {
+            const double v = l.rc[i3(Y - 1, X - 1, i, j, k)] * inv_cc;
+            s[3] += v * v;
+        }
+    }
+    for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
 MG3Ref* get3(ebg_ctx* ctx) { return ctx ? (MG3Ref*)ctx->mg3_ref : nullptr; }
 
 double* field3(MG3LevelDev<double>& l, int which, size_t* n)
@@ -513,6 +565,123 @@ int ebg_mg3d_vcycles(ebg_ctx* ctx, int ncycles, int finest_sweeps, int multiplie
     auto sm = mg_smoothing_counts(m->H.nlev(), finest_sweeps, multiplier, cap);
     for (int c = 0; c < ncycles; ++c) mg3_vcycle(m->H, sm, theta_s, theta_c, relax_v, relax_p, prnorm, ctx->stream);
     EBG_HIP(hipGetLastError());
+    return EBG_OK;
+}
+
+// MultigridSolver.run_multigrid_solver (MultigridSolver.jl:113-135) with
+// use_multimulti = false: viscosity restriction, saved originals, viscosity
+// continuation from constant viscosity (MultigridDataManager.jl:349-373,
+// ViscosityScaling.jl:11-24, 41-60, 81-110), then V-cycles with mean principal residuals
+// (CalculateMeanResiduals.jl:14-70) and the convergence test
+// (MultigridVCycle.jl:79-135, 390-412, 436-452).
+int ebg_mg3d_run(ebg_ctx* ctx, const ebg_mg_run_params* prm, int* cycles_done, int* converged, double* history)
+{
+    MG3Ref* m = get3(ctx);
+    EBG_CHECK(m, EBG_ERR_STATE, "ebg_mg3d_run: no hierarchy");
+    EBG_CHECK(prm, EBG_ERR_ARG, "ebg_mg3d_run: params is NULL");
+    const ebg_mg_run_params& p = *prm;
+    EBG_CHECK(p.nvcycles >= 0 && p.finest_sweeps >= 0 && p.sweep_multiplier >= 1 && p.nviscosity_jumps > 0 &&
+                  p.stokesscale != 0 && p.continscale != 0,
+              EBG_ERR_ARG, "ebg_mg3d_run: bad parameters");
+    hipStream_t s = ctx->stream;
+    MG3Hierarchy<double>& H = m->H;
+    const int nl = H.nlev();
+    auto nvisc = [&](int k, int f) -> int64_t {
+        const int64_t y = H.lev[k].ynum, x = H.lev[k].xnum, z = H.lev[k].znum;
+        switch (f) {
+            case 0: return (y - 1) * (x - 1) * (z - 1);
+            case 1: return y * x * (z - 1);
+            case 2: return (y - 1) * x * z;
+            default: return y * (x - 1) * z;
+        }
+    };
+    auto field = [&](int k, int f) {
+        const MG3LevelDev<double>& l = H.lev[k];
+        return f == 0 ? l.etan : f == 1 ? l.etaxy : f == 2 ? l.etaxz : l.etayz;
+    };
+    if (!m->partial) {   // first run (or a failed first allocation): allocate everything
+        for (auto& a : m->orig)
+            for (double*& q : a) {
+                if (q) (void)hipFree(q);
+                q = nullptr;
+            }
+        m->orig.assign(nl, {nullptr, nullptr, nullptr, nullptr});
+        for (int k = 0; k < nl; ++k)
+            for (int f = 0; f < 4; ++f) EBG_HIP(hipMalloc(&m->orig[k][f], (size_t)nvisc(k, f) * sizeof(double)));
+        EBG_HIP(hipMalloc(&m->partial, (size_t)MG_RUN_BLOCKS * 4 * sizeof(double)));
+    }
+    int rc;
+    // set_initial_viscosity_scaling_parameters: range of the finest level
+    double vmin = INFINITY, vmax = -INFINITY;
+    for (int f = 0; f < 4; ++f)
+        if ((rc = mg_device_range(field(0, f), nvisc(0, f), m->partial, s, vmin, vmax))) return rc;
+    EBG_CHECK(vmin > 0.0 && std::isfinite(vmax), EBG_ERR_ARG, "ebg_mg3d_run: viscosity must be positive and finite");
+    ViscRamp ramp;
+    ramp.init(vmin, vmax, p.nviscosity_jumps);
+    // interpolate_viscosity_to_coarser_levels! then save_original_viscosity_on_all_layers!
+    mg3_restrict_viscosity(H, s);
+    double omin = INFINITY, omax = -INFINITY;
+    for (int k = 0; k < nl; ++k)
+        for (int f = 0; f < 4; ++f) {
+            EBG_HIP(hipMemcpyAsync(m->orig[k][f], field(k, f), (size_t)nvisc(k, f) * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
+            if ((rc = mg_device_range(m->orig[k][f], nvisc(k, f), m->partial, s, omin, omax))) return rc;
+        }
+    // The restriction's boundary weights extrapolate, so a rough finest-level
+    // field can restrict to negative coarse viscosity; the reference's power
+    // law then raises a DomainError (negative base, fractional exponent),
+    // reported here as an error before that scaling.
+    auto scale_all = [&]() -> int {
+        const double coef = ramp.coef();
+        EBG_CHECK(omin >= 0.0 || coef == std::floor(coef), EBG_ERR_ARG,
+                  "ebg_mg3d_run: restricted viscosity is negative on a coarse level (non-integer power)");
+        for (int k = 0; k < nl; ++k)
+            for (int f = 0; f < 4; ++f)
+                mg_scale_viscosity(m->orig[k][f], field(k, f), nvisc(k, f), ramp.min_cur, 1.0 / ramp.min_o, coef, s);
+        return EBG_OK;
+    };
+    if (ramp.below_full() && (rc = scale_all())) return rc;   // rescale_viscosity!
+    const auto sm = mg_smoothing_counts(nl, p.finest_sweeps, p.sweep_multiplier, p.sweep_cap);
+    const MG3LevelDev<double>& l0 = H.lev[0];
+    const int64_t ny = l0.ynum, nx = l0.xnum, nz = l0.znum;
+    const double cnt[4] = {(double)((ny - 1) * (nx - 2) * (nz - 1)), (double)((ny - 2) * (nx - 1) * (nz - 1)),
+                           (double)((ny - 1) * (nx - 1) * (nz - 2)), (double)((ny - 1) * (nx - 1) * (nz - 1))};
+    double part[4 * MG_RUN_BLOCKS];
+    int jump = 1, done = 0, conv = 0;
+    for (int ivcycle = 1; ivcycle <= p.nvcycles; ++ivcycle) {
+        mg3_vcycle(H, sm, p.theta_s, p.theta_c, p.relax_v, p.relax_p, p.pressure_bc, s);
+        hipLaunchKernelGGL(mean_residual_partial_kernel, dim3(MG_RUN_BLOCKS), dim3(256), 0, s, l0,
+                           1.0 / p.stokesscale, 1.0 / p.continscale, m->partial);
+        EBG_HIP(hipMemcpyAsync(part, m->partial, sizeof(part), hipMemcpyDeviceToHost, s));
+        EBG_HIP(hipStreamSynchronize(s));
+        double res[4];
+        for (int q = 0; q < 4; ++q) {
+            double sum = 0.0;
+            for (int b = 0; b < MG_RUN_BLOCKS; ++b) sum += part[4 * b + q];
+            res[q] = std::log10(std::sqrt(sum / cnt[q]));
+            if (history) history[4 * (ivcycle - 1) + q] = res[q];
+        }
+        done = ivcycle;
+        // rescale_viscosity_for_levels! (3-D): restot sums the x-Stokes
+        // residual four times, as the reference does
+        ++jump;
+        const double rx10 = std::pow(10.0, res[0]);
+        const double restot = rx10 + rx10 + rx10 + rx10;
+        if ((jump > p.nvcycles_viscosity_jump || restot < p.resmax) && ramp.below_full()) {
+            jump = 1;
+            ramp.jump();
+            if ((rc = scale_all())) return rc;
+        }
+        // is_converged (non-multimulti)
+        if (mg_max_principal_residual(res, 4, ivcycle) < p.convergence_criterion &&
+            ivcycle > p.nvcycles_viscosity_jump * p.nviscosity_jumps) {
+            conv = 1;
+            break;
+        }
+    }
+    EBG_HIP(hipGetLastError());
+    if (cycles_done) *cycles_done = done;
+    if (converged) *converged = conv;
     return EBG_OK;
 }
 

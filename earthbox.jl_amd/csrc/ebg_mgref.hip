@@ -4,12 +4,61 @@
 #include "ebg_common.hpp"
 #include "ebg_mg.hpp"
 
+#include <array>
+#include <cmath>
+#include <vector>
+
 using namespace ebg;
 
 namespace {
 struct MGRef {
     MGHierarchy<double> H;
+    // original (unscaled) etas and etan per level for the viscosity
+    // continuation and the reduction partials (3 * MG_RUN_BLOCKS); allocated
+    // by the first ebg_mg2d_run
+    std::vector<std::array<double*, 2>> orig;
+    double* partial = nullptr;
+    ~MGRef()
+    {
+        for (auto& a : orig)
+            for (double* q : a) (void)hipFree(q);
+        if (partial) (void)hipFree(partial);
+    }
 };
+
+// per-block sums of the squared scaled interior residuals of the finest level
+// (CalculateMeanResiduals.jl:130-172): part[3*b + {x, y, c}]
+__global__ void __launch_bounds__(256) mean_residual2d_partial_kernel(MGLevelDev<double> l, double ss, double cs,
+                                                                      double* __restrict__ part)
+{
+    __shared__ double sh[3][256];
+    const int X = l.xnum, Y = l.ynum;
+    const int64_t total = (int64_t)Y * X;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = 1 + (int)(t % Y), j = 1 + (int)(t / Y);
+        if (i > 1 && j > 1 && j < X) {
+            const double v = l.rx[(size_t)(j - 1) * (Y + 1) + (i - 1)] / ss;
+            s[0] += v * v;
+        }
+        if (i > 1 && j > 1 && i < Y) {
+            const double v = l.ry[(size_t)(j - 1) * Y + (i - 1)] / ss;
+            s[1] += v * v;
+        }
+        if (j < X && i < Y) {
+            const double v = l.rc[(size_t)(j - 1) * (Y - 1) + (i - 1)] / cs;
+            s[2] += v * v;
+        }
+    }
+    for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] = s[q];
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
 MGRef* get(ebg_ctx* ctx) { return ctx ? (MGRef*)ctx->mg_ref : nullptr; }
 
 double* field_ptr(MGLevelDev<double>& l, int which, size_t* n)
@@ -146,6 +195,109 @@ int ebg_mg2d_vcycles(ebg_ctx* ctx, int ncycles, int finest_sweeps, int multiplie
         mg_vcycle(m->H, sm, theta_s, theta_c, relax_v, relax_p, prnorm, (const double*)nullptr,
                   bc_d4 ? bc_d4 : one, ctx->stream);
     EBG_HIP(hipGetLastError());
+    return EBG_OK;
+}
+
+// MultigridSolver.run_multigrid_solver for MultigridData2d with
+// use_multimulti = false (MultigridSolver.jl:113-135): viscosity restriction,
+// saved originals, viscosity continuation from constant viscosity
+// (MultigridDataManager.jl:321-373, ViscosityScaling.jl:26-39, 62-79,
+// 112-131: a jump every nvcycles_viscosity_jump cycles, no residual trigger),
+// V-cycles (MultigridVCycle.jl:197-267) with mean principal residuals
+// (CalculateMeanResiduals.jl:130-172) and the test of MultigridVCycle.jl:414-434, 454-468.
+int ebg_mg2d_run(ebg_ctx* ctx, const ebg_mg_run_params* prm, const double* bc_d4, int* cycles_done, int* converged,
+                 double* history)
+{
+    MGRef* m = get(ctx);
+    EBG_CHECK(m, EBG_ERR_STATE, "ebg_mg2d_run: no hierarchy");
+    EBG_CHECK(prm, EBG_ERR_ARG, "ebg_mg2d_run: params is NULL");
+    const ebg_mg_run_params& p = *prm;
+    EBG_CHECK(p.nvcycles >= 0 && p.finest_sweeps >= 0 && p.sweep_multiplier >= 1 && p.nviscosity_jumps > 0 &&
+                  p.stokesscale != 0 && p.continscale != 0,
+              EBG_ERR_ARG, "ebg_mg2d_run: bad parameters");
+    hipStream_t s = ctx->stream;
+    MGHierarchy<double>& H = m->H;
+    const int nl = H.nlev();
+    auto nvisc = [&](int k, int f) -> int64_t {
+        const int64_t y = H.lev[k].ynum, x = H.lev[k].xnum;
+        return f == 0 ? y * x : (y - 1) * (x - 1);
+    };
+    auto field = [&](int k, int f) { return f == 0 ? H.lev[k].etas : H.lev[k].etan; };
+    if (!m->partial) {   // first run (or a failed first allocation): allocate everything
+        for (auto& a : m->orig)
+            for (double*& q : a) {
+                if (q) (void)hipFree(q);
+                q = nullptr;
+            }
+        m->orig.assign(nl, {nullptr, nullptr});
+        for (int k = 0; k < nl; ++k)
+            for (int f = 0; f < 2; ++f) EBG_HIP(hipMalloc(&m->orig[k][f], (size_t)nvisc(k, f) * sizeof(double)));
+        EBG_HIP(hipMalloc(&m->partial, (size_t)MG_RUN_BLOCKS * 3 * sizeof(double)));
+    }
+    int rc;
+    double vmin = INFINITY, vmax = -INFINITY;
+    for (int f = 0; f < 2; ++f)
+        if ((rc = mg_device_range(field(0, f), nvisc(0, f), m->partial, s, vmin, vmax))) return rc;
+    EBG_CHECK(vmin > 0.0 && std::isfinite(vmax), EBG_ERR_ARG, "ebg_mg2d_run: viscosity must be positive and finite");
+    ViscRamp ramp;
+    ramp.init(vmin, vmax, p.nviscosity_jumps);
+    mg_restrict_viscosity(H, s);
+    double omin = INFINITY, omax = -INFINITY;
+    for (int k = 0; k < nl; ++k)
+        for (int f = 0; f < 2; ++f) {
+            EBG_HIP(hipMemcpyAsync(m->orig[k][f], field(k, f), (size_t)nvisc(k, f) * sizeof(double),
+                                   hipMemcpyDeviceToDevice, s));
+            if ((rc = mg_device_range(m->orig[k][f], nvisc(k, f), m->partial, s, omin, omax))) return rc;
+        }
+    // a negative restricted viscosity under a fractional power is the
+    // reference's DomainError
+    auto scale_all = [&]() -> int {
+        const double coef = ramp.coef();
+        EBG_CHECK(omin >= 0.0 || coef == std::floor(coef), EBG_ERR_ARG,
+                  "ebg_mg2d_run: restricted viscosity is negative on a coarse level (non-integer power)");
+        for (int k = 0; k < nl; ++k)
+            for (int f = 0; f < 2; ++f)
+                mg_scale_viscosity(m->orig[k][f], field(k, f), nvisc(k, f), ramp.min_cur, 1.0 / ramp.min_o, coef, s);
+        return EBG_OK;
+    };
+    if (ramp.below_full() && (rc = scale_all())) return rc;   // rescale_viscosity!
+    const double one[4] = {1.0, 1.0, 1.0, 1.0};
+    const auto sm = mg_smoothing_counts(nl, p.finest_sweeps, p.sweep_multiplier, p.sweep_cap);
+    const MGLevelDev<double>& l0 = H.lev[0];
+    const int64_t ny = l0.ynum, nx = l0.xnum;
+    const double cnt[3] = {(double)((ny - 1) * (nx - 2)), (double)((ny - 2) * (nx - 1)), (double)((ny - 1) * (nx - 1))};
+    double part[3 * MG_RUN_BLOCKS];
+    int jump = 1, done = 0, conv = 0;
+    for (int ivcycle = 1; ivcycle <= p.nvcycles; ++ivcycle) {
+        mg_vcycle(H, sm, p.theta_s, p.theta_c, p.relax_v, p.relax_p, p.pressure_bc, (const double*)nullptr,
+                  bc_d4 ? bc_d4 : one, s);
+        hipLaunchKernelGGL(mean_residual2d_partial_kernel, dim3(MG_RUN_BLOCKS), dim3(256), 0, s, l0, p.stokesscale,
+                           p.continscale, m->partial);
+        EBG_HIP(hipMemcpyAsync(part, m->partial, sizeof(part), hipMemcpyDeviceToHost, s));
+        EBG_HIP(hipStreamSynchronize(s));
+        double res[3];
+        for (int q = 0; q < 3; ++q) {
+            double sum = 0.0;
+            for (int b = 0; b < MG_RUN_BLOCKS; ++b) sum += part[3 * b + q];
+            res[q] = std::log10(std::sqrt(sum / cnt[q]));
+            if (history) history[3 * (ivcycle - 1) + q] = res[q];
+        }
+        done = ivcycle;
+        ++jump;   // rescale_viscosity_for_levels! (2-D)
+        if (jump > p.nvcycles_viscosity_jump && ramp.below_full()) {
+            jump = 1;
+            ramp.jump();
+            if ((rc = scale_all())) return rc;
+        }
+        if (mg_max_principal_residual(res, 3, ivcycle) < p.convergence_criterion &&
+            ivcycle > p.nvcycles_viscosity_jump * p.nviscosity_jumps) {
+            conv = 1;
+            break;
+        }
+    }
+    EBG_HIP(hipGetLastError());
+    if (cycles_done) *cycles_done = done;
+    if (converged) *converged = conv;
     return EBG_OK;
 }
 

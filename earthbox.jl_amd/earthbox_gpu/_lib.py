@@ -83,6 +83,8 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_mg3d_prolong": (c_int, [c_void_p, c_int, c_double, c_double]),
     "ebg_mg3d_vcycles": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
                                  c_double, c_double, c_double]),
+    "ebg_mg3d_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_mg2d_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ebg_stokes3d_set_system": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_double, c_double,
                                         c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ebg_stokes3d_vector_length": (c_int, [c_void_p, c_void_p]),
@@ -120,6 +122,15 @@ class SolveOpts(C.Structure):
                 ("theta_c", c_double), ("use_initial_guess", c_int), ("learned", c_int),
                 ("bc_d", c_double * 4), ("smoother", c_int), ("local_sweeps", c_int),
                 ("tile", c_int), ("flexible", c_int)]
+
+
+class MGRunParams(C.Structure):
+    """ebg_mg_run_params (include/earthbox_gpu.h)."""
+    _fields_ = [("nvcycles", c_int), ("nvcycles_viscosity_jump", c_int), ("finest_sweeps", c_int),
+                ("sweep_multiplier", c_int), ("sweep_cap", c_int), ("nviscosity_jumps", c_double),
+                ("convergence_criterion", c_double), ("resmax", c_double), ("stokesscale", c_double),
+                ("continscale", c_double), ("theta_s", c_double), ("theta_c", c_double),
+                ("relax_v", c_double), ("relax_p", c_double), ("pressure_bc", c_double)]
 
 
 class SolveStats(C.Structure):

@@ -15,7 +15,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from ._lib import SolveOpts, SolveStats, check, lib
+from ._lib import MGRunParams, SolveOpts, SolveStats, check, lib
 from .context import GPUContext
 from .device import empty, ptr, to_device
 from .stokes_problem import StokesProblem
@@ -200,6 +200,28 @@ class MultigridGPU:
         check(lib.ebg_mg2d_vcycles(self.ctx.handle, ncycles, finest_sweeps, multiplier, cap,
                                    theta_s, theta_c, relax_v, relax_p, prnorm, d),
               "ebg_mg2d_vcycles")
+
+    def run(self, nvcycles, convergence_criterion, nviscosity_jumps=12.0, nvcycles_viscosity_jump=15,
+            stokesscale=1.0, continscale=1.0, finest_sweeps=2, multiplier=2, cap=48, theta_s=0.9,
+            theta_c=0.3, relax_v=1.0, relax_p=1.0, pressure_bc=0.0, bc_d=(1, 1, 1, 1)):
+        """MultigridSolver.run_multigrid_solver (MultigridSolver.jl:113-135,
+        use_multimulti = false): viscosity continuation, V-cycles, mean
+        principal residuals and the reference's stopping test.  Level 0 must
+        hold the viscosity, right-hand sides and initial solution.  Returns
+        (cycles run, converged, history[cycles, 3] of log10 RMS residuals
+        x, y, continuity)."""
+        prm = MGRunParams(nvcycles=nvcycles, nvcycles_viscosity_jump=nvcycles_viscosity_jump,
+                          finest_sweeps=finest_sweeps, sweep_multiplier=multiplier, sweep_cap=cap,
+                          nviscosity_jumps=nviscosity_jumps,
+                          convergence_criterion=convergence_criterion, resmax=0.0,
+                          stokesscale=stokesscale, continscale=continscale, theta_s=theta_s,
+                          theta_c=theta_c, relax_v=relax_v, relax_p=relax_p, pressure_bc=pressure_bc)
+        d = (C.c_double * 4)(*bc_d)
+        hist = np.zeros((max(nvcycles, 1), 3))
+        done, conv = C.c_int(), C.c_int()
+        check(lib.ebg_mg2d_run(self.ctx.handle, C.byref(prm), d, C.byref(done), C.byref(conv),
+                               hist.ctypes.data), "ebg_mg2d_run")
+        return done.value, bool(conv.value), hist[:done.value].copy()
 
 
 def change_norms(ctx, v, v_old=None):

@@ -19,7 +19,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from ._lib import SolveStats, check, lib
+from ._lib import MGRunParams, SolveStats, check, lib
 from .context import GPUContext
 from .device import empty, ptr
 from .stokes import default_opts
@@ -93,6 +93,27 @@ class Multigrid3DGPU:
                 relax_v=1.0, relax_p=1.0, prnorm=0.0):
         check(lib.ebg_mg3d_vcycles(self.ctx.handle, ncycles, finest_sweeps, multiplier, cap,
                                    theta_s, theta_c, relax_v, relax_p, prnorm), "ebg_mg3d_vcycles")
+
+    def run(self, nvcycles, convergence_criterion, nviscosity_jumps=12.0, nvcycles_viscosity_jump=15,
+            resmax=1e-6, stokesscale=1.0, continscale=1.0, finest_sweeps=2, multiplier=2, cap=0,
+            theta_s=0.9, theta_c=0.3, relax_v=1.0, relax_p=1.0, pressure_bc=0.0):
+        """MultigridSolver.run_multigrid_solver (MultigridSolver.jl:113-135,
+        use_multimulti = false): viscosity continuation, V-cycles, mean
+        principal residuals and the reference's stopping test.  Level 0 must
+        hold the viscosity, right-hand sides and initial solution.  Returns
+        (cycles run, converged, history[cycles, 4] of log10 RMS residuals
+        x, y, z, continuity)."""
+        prm = MGRunParams(nvcycles=nvcycles, nvcycles_viscosity_jump=nvcycles_viscosity_jump,
+                          finest_sweeps=finest_sweeps, sweep_multiplier=multiplier, sweep_cap=cap,
+                          nviscosity_jumps=nviscosity_jumps,
+                          convergence_criterion=convergence_criterion, resmax=resmax,
+                          stokesscale=stokesscale, continscale=continscale, theta_s=theta_s,
+                          theta_c=theta_c, relax_v=relax_v, relax_p=relax_p, pressure_bc=pressure_bc)
+        hist = np.zeros((max(nvcycles, 1), 4))
+        done, conv = C.c_int(), C.c_int()
+        check(lib.ebg_mg3d_run(self.ctx.handle, C.byref(prm), C.byref(done), C.byref(conv),
+                               hist.ctypes.data), "ebg_mg3d_run")
+        return done.value, bool(conv.value), hist[:done.value].copy()
 
 
 class Stokes3DSolverGPU:

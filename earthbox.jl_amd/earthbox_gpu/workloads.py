@@ -206,3 +206,129 @@ def spheres3d(n: int = 257, size: float = 1.0, nspheres: int = 8, seed: int = 3,
     out["RY"] = julia_zeros("vy")
     out["RC"] = julia_zeros("pr")
     return out
+
+
+def _sinker_axis(n: int, size: float) -> dict:
+    """One uniform axis as the multigrid's level grid builds it: basic nodes
+    by running sum of the average step (Spacing.jl:12, Initialization.jl:27-43),
+    cell centres from the pressure steps (PressureGridTools.jl:34-86) and the
+    shear-centre nodes (SxyGridTools.jl:30-40)."""
+    h = size / (n - 1)
+    b = np.zeros(n)
+    for i in range(1, n):
+        b[i] = b[i - 1] + h
+    stp = b[1:] - b[:-1]
+    stp_p = (b[2:] - b[:-2]) / 2.0
+    p = np.zeros(n - 1)
+    p[0] = (b[0] + b[1]) / 2.0
+    for i in range(1, n - 1):
+        p[i] = p[i - 1] + stp_p[i - 1]
+    s = b[:-1] + stp / 2.0
+    s[0] = b[0] + stp[0] / 2.0
+    s[-1] = b[-1] - stp[-1] / 2.0
+    return {"b": b, "p": p, "s": s, "stp_p": stp_p}
+
+
+# The reference's own multigrid demo (multigrid_tests/stokes_sinker/StokesSinker.jl:29-160,
+# model_data/ModelParametersManager.jl): a dense, stiff cube (normalised
+# 0.4..0.6 in every direction) sinking in a 100 km box.
+SINKER = dict(density_block=3100.0, density_medium=3000.0, viscosity_block=1e26,
+              viscosity_medium=1e20, gravity=9.81, pressure_bc=0.0, block=(0.4, 0.6))
+# V-cycle and relaxation settings of the demo's runs (StokesSinker.jl:40-101)
+SINKER_RUNS = {
+    "3d_smoke": dict(n=17, nvcycles=30, nvcycles_viscosity_jump=15, nviscosity_jumps=11.475031046555408,
+                     finest_sweeps=2, convergence_criterion=1e-4, cap=12),
+    "3d": dict(n=97, nvcycles=250, nvcycles_viscosity_jump=15, nviscosity_jumps=11.475031046555408,
+               finest_sweeps=5, convergence_criterion=1e-20, cap=48),
+    "2d": dict(n=97, nvcycles=500, nvcycles_viscosity_jump=17, nviscosity_jumps=12.0, finest_sweeps=5,
+               convergence_criterion=1e-5, cap=48),
+}
+SINKER_RELAXATION = dict(theta_s=0.9, relax_v=1.0, theta_c=0.3, relax_p=1.0)
+
+
+def _in_block(*c):
+    lo, hi = SINKER["block"]
+    m = True
+    for a in c:
+        m = m & (lo <= a) & (a <= hi)
+    return m
+
+
+def stokes_sinker3d(n: int = 97, size: float = 1.0e5, viscosity_block: float | None = None) -> dict:
+    """Initial state of the 3-D Stokes sinker on an n^3-node cube
+    (StokesSinker.jl:111-139 with initial_conditions/Density.jl:11-25, 45-58,
+    Viscosity.jl:11-40, 99-160, RightParts.jl:10-36, 62-71, Pressure.jl:10-40):
+    basic-node density, the four viscosity arrays, RX = RZ = RC = 0,
+    RY = -g * (mean of the four basic densities around the vy node), the
+    hydrostatic pressure guess integrated down from pressure_bc, zero
+    velocity guesses, and the residual scales of ScalingFactors.jl:3-18.
+    Arrays are numpy column-major with the level shapes (y, x, z)."""
+    P = dict(SINKER)
+    if viscosity_block is not None:
+        P["viscosity_block"] = viscosity_block
+    a = _sinker_axis(n, size)
+    g = P["gravity"]
+    F = lambda sh: np.zeros(sh, order="F")
+
+    def field(ky, kx, kz, val_in, val_out):
+        y, x, z = a[ky] / size, a[kx] / size, a[kz] / size
+        m = _in_block(y[:, None, None], x[None, :, None], z[None, None, :])
+        return np.asfortranarray(np.where(m, val_in, val_out))
+
+    rho = field("b", "b", "b", P["density_block"], P["density_medium"])
+    vb, vm = P["viscosity_block"], P["viscosity_medium"]
+    out = {"rho": rho, "etan": field("p", "p", "p", vb, vm), "etaxy": field("b", "b", "s", vb, vm),
+           "etaxz": field("s", "b", "b", vb, vm), "etayz": field("b", "s", "b", vb, vm)}
+    ny = nx = nz = n
+    RY = F((ny, nx + 1, nz + 1))
+    avg = (((rho[:, 1:, 1:] + rho[:, :-1, 1:]) + rho[:, 1:, :-1]) + rho[:, :-1, :-1]) / 4.0
+    RY[1:ny - 1, 1:nx, 1:nz] = -g * avg[1:ny - 1]
+    out.update(RX=F((ny + 1, nx, nz + 1)), RY=RY, RZ=F((ny + 1, nx + 1, nz)), RC=F((ny - 1, nx - 1, nz - 1)))
+    pr = F((ny - 1, nx - 1, nz - 1))
+    pr[0] = P["pressure_bc"]
+    for i in range(1, ny - 1):
+        pr[i] = pr[i - 1] + a["stp_p"][i - 1] * g * avg[i]
+    out.update(pr=pr, vx=F((ny + 1, nx, nz + 1)), vy=F((ny, nx + 1, nz + 1)), vz=F((ny + 1, nx + 1, nz)))
+    xstpavg = size / (n - 1)
+    out["stokesscale"] = P["density_block"] * g
+    out["continscale"] = P["density_block"] * g / P["viscosity_medium"] * xstpavg
+    out["dims"] = (n, n, n, size, size, size)
+    out["pressure_bc"] = P["pressure_bc"]
+    return out
+
+
+def stokes_sinker2d(n: int = 97, size: float = 1.0e5, viscosity_block: float | None = None) -> dict:
+    """The 2-D Stokes sinker (StokesSinker.jl:84-101, 140-157 with
+    Density.jl:27-43, 60-72, Viscosity.jl:42-60, 75-97, 99-110,
+    RightParts.jl:38-60, 73-81, Pressure.jl:42-70): density and etas on basic
+    nodes, etan on cell centres, RX = RC = 0, RY = -g * (mean of the two basic
+    densities left and right of the vy node), hydrostatic pressure guess."""
+    P = dict(SINKER)
+    if viscosity_block is not None:
+        P["viscosity_block"] = viscosity_block
+    a = _sinker_axis(n, size)
+    g = P["gravity"]
+
+    def field(ky, kx, val_in, val_out):
+        y, x = a[ky] / size, a[kx] / size
+        return np.asfortranarray(np.where(_in_block(y[:, None], x[None, :]), val_in, val_out))
+
+    rho = field("b", "b", P["density_block"], P["density_medium"])
+    vb, vm = P["viscosity_block"], P["viscosity_medium"]
+    ny = nx = n
+    out = {"rho": rho, "etas": field("b", "b", vb, vm), "etan": field("p", "p", vb, vm)}
+    RY = np.zeros((ny, nx + 1), order="F")
+    avg = (rho[:, 1:] + rho[:, :-1]) / 2.0
+    RY[1:ny - 1, 1:nx] = -g * avg[1:ny - 1]
+    out.update(RX=np.zeros((ny + 1, nx), order="F"), RY=RY, RC=np.zeros((ny - 1, nx - 1), order="F"))
+    pr = np.zeros((ny - 1, nx - 1), order="F")
+    pr[0] = P["pressure_bc"]
+    for i in range(1, ny - 1):
+        pr[i] = pr[i - 1] + a["stp_p"][i - 1] * g * avg[i]
+    out.update(pr=pr, vx=np.zeros((ny + 1, nx), order="F"), vy=np.zeros((ny, nx + 1), order="F"))
+    xstpavg = size / (n - 1)
+    out["stokesscale"] = P["density_block"] * g
+    out["continscale"] = P["density_block"] * g / P["viscosity_medium"] * xstpavg
+    out["dims"] = (n, n, size, size)
+    out["pressure_bc"] = P["pressure_bc"]
+    return out

@@ -314,6 +314,37 @@ int ebg_mg2d_smoother_tiled(ebg_ctx *ctx, int level, int nsweeps, int local_swee
 int ebg_mg2d_vcycles(ebg_ctx *ctx, int ncycles, int finest_sweeps, int multiplier, int cap,
                      double theta_s, double theta_c, double relax_v, double relax_p,
                      double prnorm, const double *bc_d4);
+/* Parameters of the reference's stand-alone V-cycle driver: VcycleParameters,
+ * ViscosityScalingParameters, RelaxationParameters and ResidualScalingFactors
+ * (multigrid_data/structures/MultigridStructures.jl:8-90). */
+typedef struct ebg_mg_run_params {
+    int nvcycles;                /* vcycle.nvcycles */
+    int nvcycles_viscosity_jump; /* V-cycles between viscosity jumps (15) */
+    int finest_sweeps;           /* smoothing iterations on the finest level */
+    int sweep_multiplier;        /* per-level growth of the sweep count */
+    int sweep_cap;               /* largest sweep count on any level */
+    double nviscosity_jumps;     /* number of viscosity jumps (12.0) */
+    double convergence_criterion;
+    double resmax;               /* residual that triggers an early jump (1e-6) */
+    double stokesscale;          /* ScalingFactors.jl:3-18 */
+    double continscale;
+    double theta_s, theta_c;     /* relax_stokes, relax_continuity */
+    double relax_v, relax_p;     /* relax_velocity, relax_pressure */
+    double pressure_bc;          /* pressure of cell (1,1,1) on the finest level */
+} ebg_mg_run_params;
+/* MultigridSolver.run_multigrid_solver for MultigridData2d with
+ * use_multimulti = false (MultigridSolver.jl:113-135): the finest-level
+ * viscosity range (MultigridDataManager.jl:321-373), restriction to every
+ * level, saved originals, then V-cycles while the viscosity contrast rises
+ * from 1 to the full range, one jump every nvcycles_viscosity_jump cycles
+ * (ViscosityScaling.jl:26-39, 62-79, 112-131; resmax is not used in 2-D),
+ * recording the mean principal residuals (CalculateMeanResiduals.jl:130-172)
+ * and stopping on the reference's test (MultigridVCycle.jl:197-267, 414-434,
+ * 454-468).  bc_d4 as for ebg_mg2d_vcycles.  history (host, may be NULL)
+ * receives 3 doubles per cycle run: log10 RMS of the x-, y-Stokes and
+ * continuity residuals. */
+int ebg_mg2d_run(ebg_ctx *ctx, const ebg_mg_run_params *params, const double *bc_d4, int *cycles_done,
+                 int *converged, double *history);
 
 /* ---- the reference's experimental 3-D multigrid (FP64) ----
  * Replaces multigrid/multigrid_solver/MultigridSolver.jl:113-135 for
@@ -360,6 +391,19 @@ int ebg_mg3d_prolong(ebg_ctx *ctx, int level, double relax_v, double relax_p);
  * continuation): SmoothAndRestrict.jl:15-46 then SmoothAndProlongate.jl:14-60. */
 int ebg_mg3d_vcycles(ebg_ctx *ctx, int ncycles, int finest_sweeps, int multiplier, int cap,
                      double theta_s, double theta_c, double relax_v, double relax_p, double prnorm);
+/* MultigridSolver.run_multigrid_solver for MultigridData3d with
+ * use_multimulti = false (MultigridSolver.jl:113-135): takes the range of the
+ * finest-level viscosity (MultigridDataManager.jl:305-373), restricts it to
+ * every level, keeps the originals, then runs V-cycles while raising the
+ * viscosity contrast from 1 to the full range in nviscosity_jumps steps
+ * (ViscosityScaling.jl:11-24, 41-60, 81-110), recording the mean principal residuals
+ * (CalculateMeanResiduals.jl:14-70) and stopping on the reference's test
+ * (MultigridVCycle.jl:79-135, 390-412, 436-452).  The level-0 viscosity arrays hold the
+ * last scaled viscosity on return, as the reference's do.  history (host, may
+ * be NULL) receives 4 doubles per cycle run: log10 RMS of the x-, y-, z-Stokes
+ * and continuity residuals. */
+int ebg_mg3d_run(ebg_ctx *ctx, const ebg_mg_run_params *params, int *cycles_done, int *converged,
+                 double *history);
 
 /* ---- 3-D Stokes solve on the 3-D multigrid's system ----
  * No assembled 3-D system exists in the reference: its 3-D Stokes path is the

@@ -56,6 +56,7 @@ void ebo_mg3_prolong(const ebo_mg3level *c, ebo_mg3level *f, const ebo_mg3map *m
 void ebo_mg3_viscosity_restrict(const ebo_mg3level *f, ebo_mg3level *c, const ebo_mg3map *m,
                                 double *wxy, double *wxz, double *wyz, double *wn);
 void ebo_mg3_apply(ebo_mg3level *l);
+void ebo_mg3_mean_residuals(const ebo_mg3level *l, double stokesscale, double continscale, double *out);
 void ebo_axis_uniform(int64_t n, double size, double *g_b, double *stp_b, double *g_v,
                       double *stp_v, double *g_p, double *stp_p, double *g_s);
 void ebo_axis_from_basic(int64_t n, const double *g_b, double *stp_b, double *g_v, double *stp_v,

@@ -46,6 +46,7 @@ void ebo_mg_viscosity_restrict(const ebo_mglevel *f, ebo_mglevel *c, const ebo_m
                                double *wts, double *wtn);
 void ebo_mg_scale_viscosity(int64_t n, const double *eta_orig, double *eta, double eta_min_o,
                             double eta_max_o, double eta_min_cur, double eta_max_cur);
+void ebo_mg_mean_residuals(const ebo_mglevel *l, double stokesscale, double continscale, double *out);
 
 #ifdef __cplusplus
 }

@@ -325,3 +325,31 @@ void ebo_mg_scale_viscosity(int64_t n, const double *eta_orig, double *eta, doub
     const double inv_min_o = 1.0 / eta_min_o;
     for (int64_t k = 0; k < n; ++k) eta[k] = eta_min_cur * pow(eta_orig[k] * inv_min_o, coef);
 }
+
+/* utils/CalculateMeanResiduals.jl:130-172 (2-D calculate_scaled_and_mean_residuals!):
+ * out[0..2] = log10(sqrt(mean((r / scale)^2))) of the interior x- and y-Stokes
+ * residuals (scale stokesscale) and of the continuity residuals
+ * (continscale), summed serially in the order j, i. */
+void ebo_mg_mean_residuals(const ebo_mglevel *l, double stokesscale, double continscale, double *out)
+{
+    const int64_t xnum = l->xnum, ynum = l->ynum;
+    double sx = 0.0, sy = 0.0, sc = 0.0;
+    for (int64_t j = 1; j <= xnum; ++j)
+        for (int64_t i = 1; i <= ynum; ++i) {
+            if (i > 1 && j > 1 && j < xnum) {
+                const double r = EBO_M(l->rx, ynum + 1, i, j) / stokesscale;
+                sx += r * r;
+            }
+            if (i > 1 && j > 1 && i < ynum) {
+                const double r = EBO_M(l->ry, ynum, i, j) / stokesscale;
+                sy += r * r;
+            }
+            if (j < xnum && i < ynum) {
+                const double r = EBO_M(l->rc, ynum - 1, i, j) / continscale;
+                sc += r * r;
+            }
+        }
+    out[0] = log10(sqrt(sx / (double)((ynum - 1) * (xnum - 2))));
+    out[1] = log10(sqrt(sy / (double)((ynum - 2) * (xnum - 1))));
+    out[2] = log10(sqrt(sc / (double)((ynum - 1) * (xnum - 1))));
+}

@@ -56,6 +56,8 @@ def _sigs():
             "ebo_mg3_prolong": (None, [PL, PL, PM, _D, _D]),
             "ebo_mg3_viscosity_restrict": (None, [PL, PL, PM, _P, _P, _P, _P]),
             "ebo_mg3_apply": (None, [PL]),
+            "ebo_mg3_mean_residuals": (None, [PL, _D, _D, _P]),
+            "ebo_mg_scale_viscosity": (None, [_I64, _P, _P, _D, _D, _D, _D]),
         }
         for k, (r, a) in table.items():
             getattr(L, k).restype = r
@@ -234,6 +236,66 @@ class Hierarchy:
                 self.prolong(n, relax_v, relax_p)
         f = self.levels[0]
         return f.rx, f.ry, f.rz, f.rc
+
+    def mean_residuals(self, stokesscale, continscale):
+        out = np.zeros(4)
+        _sigs().ebo_mg3_mean_residuals(C.byref(self.levels[0].cs()), stokesscale, continscale, p(out))
+        return out
+
+    def run(self, nvcycles, convergence_criterion, nviscosity_jumps=12.0, nvcycles_viscosity_jump=15,
+            resmax=1e-6, stokesscale=1.0, continscale=1.0, finest_sweeps=2, multiplier=2, cap=None,
+            theta_s=0.9, theta_c=0.3, relax_v=1.0, relax_p=1.0, pressure_bc=0.0):
+        """MultigridSolver.run_multigrid_solver with use_multimulti = false
+        (MultigridSolver.jl:113-135).  Finest-level viscosity range
+        (MultigridDataManager.jl:305-373), viscosity restriction, saved
+        originals, continuation (ViscosityScaling.jl:11-24, 41-60, 81-110 with
+        the counters of ViscosityScaling.jl:267-309), V-cycles (MultigridVCycle.jl:79-135)
+        and the convergence test (MultigridVCycle.jl:390-412, 436-452).  Returns
+        (cycles run, converged, history[cycles, 4])."""
+        L = _sigs()
+        names = ("etan", "etaxy", "etaxz", "etayz")
+        f0 = self.levels[0]
+        vmin = min(float(getattr(f0, n).min()) for n in names)
+        vmax = max(float(getattr(f0, n).max()) for n in names)
+        min_o, max_o = vmin, vmax
+        min_cur = max_cur = vmin
+        min_factor = 1.0
+        max_factor = 10.0 ** (np.log10(vmax / vmin) / nviscosity_jumps)
+        self.restrict_viscosity()
+        orig = [{n: getattr(lv, n).copy(order="F") for n in names} for lv in self.levels]
+        omin = min(float(a.min()) for o in orig for a in o.values())
+
+        def scale_all():
+            coef = np.log(max_cur / min_cur) / np.log(max_o / min_o)
+            if omin < 0 and coef != np.floor(coef):
+                # Julia's ^ raises DomainError for a negative base and a
+                # fractional exponent
+                raise ValueError("restricted viscosity is negative on a coarse level")
+            for lv, o in zip(self.levels, orig):
+                for n in names:
+                    L.ebo_mg_scale_viscosity(o[n].size, p(o[n]), p(getattr(lv, n)), min_o, max_o,
+                                             min_cur, max_cur)
+        if max_cur < max_o or min_cur > min_o:
+            scale_all()
+        sm = smoothing_counts(len(self.levels), finest_sweeps, multiplier, cap)
+        jump, hist, converged = 1, [], False
+        for ivcycle in range(1, nvcycles + 1):
+            self.vcycle(sm, theta_s, theta_c, relax_v, relax_p, pressure_bc)
+            res = self.mean_residuals(stokesscale, continscale)
+            hist.append(res)
+            jump += 1
+            t = 10.0 ** res[0]
+            restot = t + t + t + t   # the reference sums the x residual four times
+            if (jump > nvcycles_viscosity_jump or restot < resmax) and (max_cur < max_o or min_cur > min_o):
+                jump = 1
+                min_cur = max(min_cur * min_factor, min_o)
+                max_cur = min(max_cur * max_factor, max_o)
+                scale_all()
+            rmax = float(np.max(np.abs(10.0 ** res))) if ivcycle > 2 else 1e38   # NaN propagates
+            if rmax < convergence_criterion and ivcycle > nvcycles_viscosity_jump * nviscosity_jumps:
+                converged = True
+                break
+        return len(hist), converged, np.array(hist).reshape(-1, 4)
 
 
 def interior_masks(lv: Level) -> dict:

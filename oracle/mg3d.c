@@ -17,6 +17,7 @@
  * BoundaryConditions.jl:92-174 in the reference's visiting order.
  */
 #include "eb_mg3d_oracle.h"
+#include <math.h>
 #include <stdlib.h>
 
 #define VX(l, i, j, k) EBO_A3((l)->vx, (l)->ynum + 1, (l)->xnum, i, j, k)
@@ -517,6 +518,41 @@ void ebo_mg3_viscosity_restrict(const ebo_mg3level *f, ebo_mg3level *c, const eb
     for (int64_t q = 0; q < nxz; ++q) if (wxz[q] != 0) c->etaxz[q] = c->etaxz[q] / wxz[q];
     for (int64_t q = 0; q < nyz; ++q) if (wyz[q] != 0) c->etayz[q] = c->etayz[q] / wyz[q];
     for (int64_t q = 0; q < nn; ++q) if (wn[q] != 0) c->etan[q] = c->etan[q] / wn[q];
+}
+
+/* utils/CalculateMeanResiduals.jl:14-70 (accumulate_principle_residuals_3d!):
+ * out[0..3] = log10(sqrt(mean((r / scale)^2))) of the interior x-, y-, z-Stokes
+ * residuals (scale stokesscale) and of the continuity residuals (continscale),
+ * summed serially in the order k, j, i. */
+void ebo_mg3_mean_residuals(const ebo_mg3level *l, double stokesscale, double continscale, double *out)
+{
+    const int64_t xnum = l->xnum, ynum = l->ynum, znum = l->znum;
+    double sx = 0.0, sy = 0.0, sz = 0.0, sc = 0.0;
+    const double inv_ss = 1.0 / stokesscale, inv_cc = 1.0 / continscale;
+    for (int64_t k = 1; k <= znum; ++k)
+        for (int64_t j = 1; j <= xnum; ++j)
+            for (int64_t i = 1; i <= ynum; ++i) {
+                if (i > 1 && j > 1 && k > 1 && j < xnum) {
+                    const double r = EBO_A3(l->rx, ynum + 1, xnum, i, j, k) * inv_ss;
+                    sx += r * r;
+                }
+                if (i > 1 && j > 1 && k > 1 && i < ynum) {
+                    const double r = EBO_A3(l->ry, ynum, xnum + 1, i, j, k) * inv_ss;
+                    sy += r * r;
+                }
+                if (i > 1 && j > 1 && k > 1 && k < znum) {
+                    const double r = EBO_A3(l->rz, ynum + 1, xnum + 1, i, j, k) * inv_ss;
+                    sz += r * r;
+                }
+                if (j < xnum && i < ynum && k < znum) {
+                    const double r = EBO_A3(l->rc, ynum - 1, xnum - 1, i, j, k) * inv_cc;
+                    sc += r * r;
+                }
+            }
+    out[0] = log10(sqrt(sx / (double)((ynum - 1) * (xnum - 2) * (znum - 1))));
+    out[1] = log10(sqrt(sy / (double)((ynum - 2) * (xnum - 1) * (znum - 1))));
+    out[2] = log10(sqrt(sz / (double)((ynum - 1) * (xnum - 1) * (znum - 2))));
+    out[3] = log10(sqrt(sc / (double)((ynum - 1) * (xnum - 1) * (znum - 1))));
 }
 
 /* Linear operator of the 3-D problem, as the outer Krylov solver sees it:

@@ -10,7 +10,12 @@ so these fixtures are the oracle's outputs on tiny seeded inputs, frozen:
   stokes_ext_17x11.npz  the same for prescribed boundary velocities with
                     interface stabilization
 They pin the oracle against silent change (tests/test_golden.py) and give the
-GPU tests fixed expected outputs."""
+GPU tests fixed expected outputs.
+
+stokes_sinker3d_97_stats.json is not written by this script: it holds the
+final-solution statistics the reference's authors recorded in the source of
+its 97^3 Stokes sinker demo (multigrid_tests/stokes_sinker/StokesSinker.jl:58-62),
+transcribed as data."""
 import pathlib
 import sys
 

@@ -273,3 +273,78 @@ def test_operator_apply_matches_residual_definition():
     expect = L0.RC - L0.rc
     expect.flat[0] = x["pr"].flat[0] / L0.etan.flat[0]
     np.testing.assert_allclose(y["pr"], expect, rtol=1e-9, atol=1e-9)
+
+
+def test_mean_residuals_closed_form():
+    """Interior RMS with ghost slots excluded: alternating +-c gives log10(c/scale)."""
+    H = _hierarchy(dims=(7, 9, 8))
+    L0 = H.levels[0]
+    ny, nx, nz = L0.ynum, L0.xnum, L0.znum
+    c = {"rx": 3.0, "ry": 0.25, "rz": 7.5, "rc": 0.04}
+    for k in ("rx", "ry", "rz", "rc"):
+        a = getattr(L0, k)
+        a[...] = 1e30  # ghost slots must not count
+    sl = {"rx": (slice(1, ny), slice(1, nx - 1), slice(1, nz)),
+          "ry": (slice(1, ny - 1), slice(1, nx), slice(1, nz)),
+          "rz": (slice(1, ny), slice(1, nx), slice(1, nz - 1)),
+          "rc": (slice(None), slice(None), slice(None))}
+    for k, s in sl.items():
+        a = getattr(L0, k)
+        sign = np.where(RNG.uniform(size=a[s].shape) < 0.5, -1.0, 1.0)
+        a[s] = c[k] * sign
+    ss, cs = 2.0, 0.5
+    out = H.mean_residuals(ss, cs)
+    expect = [np.log10(c["rx"] / ss), np.log10(c["ry"] / ss), np.log10(c["rz"] / ss),
+              np.log10(c["rc"] / cs)]
+    np.testing.assert_allclose(out, expect, rtol=0, atol=1e-14)
+
+
+def _originals(H):
+    return {k: getattr(H.levels[0], k).copy() for k in ("etan", "etaxy", "etaxz", "etayz")}
+
+
+def test_run_continuation_schedule_and_stopping_rule():
+    """Viscosity starts constant at the minimum, the maximum grows by
+    (max/min)^(1/njumps) every nvcycles_viscosity_jump cycles, the full field
+    returns after njumps jumps, and the stop comes at the first cycle past
+    nvcycles_viscosity_jump * njumps when the criterion is loose."""
+    H = _hierarchy(seed=3)
+    eta0 = _originals(H)
+    vmin = min(a.min() for a in eta0.values())
+    vmax = max(a.max() for a in eta0.values())
+    n, conv, hist = H.run(1, 1e300, nviscosity_jumps=3.0, nvcycles_viscosity_jump=2, resmax=0.0)
+    assert (n, conv) == (1, False) and hist.shape == (1, 4)
+    for k in eta0:
+        assert np.all(getattr(H.levels[0], k) == vmin)
+        assert np.all(getattr(H.levels[1], k) == vmin)
+    H = _hierarchy(seed=3)
+    H.run(2, 1e300, nviscosity_jumps=3.0, nvcycles_viscosity_jump=2, resmax=0.0)
+    step = 10 ** (np.log10(vmax / vmin) / 3.0)
+    top = max(getattr(H.levels[0], k).max() for k in eta0)
+    bottom = min(getattr(H.levels[0], k).min() for k in eta0)
+    np.testing.assert_allclose([bottom, top], [vmin, vmin * step], rtol=1e-13)
+    H = _hierarchy(seed=3)
+    n, conv, hist = H.run(20, 1e300, nviscosity_jumps=3.0, nvcycles_viscosity_jump=2, resmax=0.0)
+    assert (n, conv) == (7, True) and hist.shape == (7, 4)
+    for k in eta0:
+        np.testing.assert_allclose(getattr(H.levels[0], k), eta0[k], rtol=1e-12)
+    # resmax above every residual: a jump on every cycle, full field after njumps cycles
+    H = _hierarchy(seed=3)
+    n, conv, _ = H.run(3, 1e300, nviscosity_jumps=3.0, nvcycles_viscosity_jump=50, resmax=1e300)
+    assert (n, conv) == (3, False)
+    for k in eta0:
+        np.testing.assert_allclose(getattr(H.levels[0], k), eta0[k], rtol=1e-12)
+
+
+def test_run_converges_to_direct_solve():
+    H = _hierarchy(seed=5)
+    L0 = H.levels[0]
+    xs, masks = _direct_solution(L0)   # full-contrast system
+    n, conv, hist = H.run(300, 1e-11, nviscosity_jumps=2.0, nvcycles_viscosity_jump=5,
+                          finest_sweeps=3)
+    assert conv and 10 < n < 300
+    assert np.all(10.0 ** hist[-1] < 1e-11)
+    for k in ("vx", "vy", "vz", "pr"):
+        m = masks[k]
+        err = np.linalg.norm(getattr(L0, k)[m] - xs[k][m]) / np.linalg.norm(xs[k][m])
+        assert err < 1e-8, (k, err)
