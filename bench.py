@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--no-stokes3d", action="store_true")
     ap.add_argument("--stokes3d-n", type=int, default=257)
     ap.add_argument("--stokes3d-max-iter", type=int, default=300)
+    ap.add_argument("--no-sinker", action="store_true")
     return ap.parse_args()
 
 
@@ -84,6 +85,121 @@ def bench_stokes3d(ctx, n, max_iter):
         del sol, rhs, w
         torch.cuda.empty_cache()
     return out
+
+
+def _sinker_stats_error(parts):
+    """Largest relative deviation from the solution statistics recorded in the
+    reference's 97^3 sinker demo (pressure minimum relative to the maximum)."""
+    ref = json.loads((ROOT / "tests" / "golden" / "stokes_sinker3d_97_stats.json").read_text())
+    err = 0.0
+    for k in ("vx", "vy", "vz", "pr"):
+        a = parts[k]
+        scale = abs(ref["pr_max"]) if k == "pr" else abs(ref[k + "_min"])
+        err = max(err, abs(float(a.min()) - ref[k + "_min"]) / scale,
+                  abs(float(a.max()) - ref[k + "_max"]) / abs(ref[k + "_max"]))
+    return max(err, abs(float(parts["pr"].mean()) - ref["pr_avg"]) / ref["pr_avg"])
+
+
+def bench_sinker(ctx):
+    """The reference's own multigrid demo (Stokes sinker, contrast 1e6):
+    3-D 97^3 through the stand-alone V-cycle driver with the demo's settings
+    (250 cycles) and through GMRES(200) + V-cycle to 1e-10; 2-D 97^2 through
+    the driver with its settings (up to 500 cycles, criterion 1e-5).  Each
+    3-D result carries its largest deviation from the statistics the
+    reference recorded for that demo."""
+    import torch
+    from earthbox_gpu import workloads
+    from earthbox_gpu.device import to_device, to_host
+    from earthbox_gpu.stokes import MultigridGPU
+    from earthbox_gpu.stokes3d import Multigrid3DGPU, Stokes3DSolverGPU
+    out = {}
+    S = workloads.stokes_sinker3d(97)
+    run = dict(workloads.SINKER_RUNS["3d"])
+    run.pop("n")
+    nv, crit = run.pop("nvcycles"), run.pop("convergence_criterion")
+    kw = dict(stokesscale=S["stokesscale"], continscale=S["continscale"], pressure_bc=S["pressure_bc"],
+              **workloads.SINKER_RELAXATION, **run)
+    G = Multigrid3DGPU(ctx, *S["dims"])
+    names = ("etan", "etaxy", "etaxz", "etayz", "RX", "RY", "RZ", "RC", "vx", "vy", "vz", "pr")
+    for k in names:
+        G.set(0, k, to_device(S[k]))
+    G.run(2, crit, **kw)   # warm: first-run allocations and code paths
+    for k in names:
+        G.set(0, k, to_device(S[k]))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n, conv, hist = G.run(nv, crit, **kw)
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    out["mg3d_97_reference_driver"] = {
+        "seconds": sec, "cycles": n, "ms_per_cycle": 1e3 * sec / n, "converged": conv,
+        "final_log10_residuals": [float(v) for v in hist[-1]],
+        "max_rel_dev_from_reference_stats": _sinker_stats_error({k: to_host(G.get(0, k)) for k in
+                                                                 ("vx", "vy", "vz", "pr")})}
+    del G
+    sol = Stokes3DSolverGPU(ctx)
+    d = lambda a: to_device(a)
+    sol.set_system(*S["dims"], d(S["etan"]), d(S["etaxy"]), d(S["etaxz"]), d(S["etayz"]))
+    rhs = sol.pack({"vx": d(S["RX"]), "vy": d(S["RY"]), "vz": d(S["RZ"]), "pr": d(S["RC"])})
+    for name, prec in (("mixed_fp32_mg", 1), ("plain_fp64_mg", 0)):
+        kry = dict(restart=200, flexible=0, rtol=1e-10)
+        sol.solve(rhs, precision=prec, max_iterations=2, **kry)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, st = sol.solve(rhs, prnorm=S["pressure_bc"], precision=prec, max_iterations=600, **kry)
+        torch.cuda.synchronize()
+        out[f"stokes3d_97_gmres_{name}"] = {
+            "seconds": time.perf_counter() - t0, "iterations": st["iterations"], "relres": st["relres"],
+            "converged": st["converged"],
+            "max_rel_dev_from_reference_stats": _sinker_stats_error({k: to_host(v) for k, v in
+                                                                     sol.split(x).items()})}
+        del x
+    del sol, rhs
+    torch.cuda.empty_cache()
+    S2 = workloads.stokes_sinker2d(97)
+    run = dict(workloads.SINKER_RUNS["2d"])
+    run.pop("n")
+    nv, crit = run.pop("nvcycles"), run.pop("convergence_criterion")
+    kw2 = dict(stokesscale=S2["stokesscale"], continscale=S2["continscale"], pressure_bc=S2["pressure_bc"],
+               **workloads.SINKER_RELAXATION, **run)
+    M = MultigridGPU(ctx, 97, 97, 1e5, 1e5)
+    for rep in range(2):   # warm-up run, then the timed one
+        for k in ("etas", "etan", "RX", "RY", "RC", "vx", "vy", "pr"):
+            M.set(0, k, to_device(S2[k]))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n, conv, hist = M.run(nv if rep else 2, crit, **kw2)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+    out["mg2d_97_reference_driver"] = {"seconds": sec, "cycles": n, "ms_per_cycle": 1e3 * sec / n,
+                                       "converged": conv, "final_log10_residuals": [float(v) for v in hist[-1]]}
+    return out
+
+
+def cpu_sinker_driver(cycles: int = 3):
+    """CPU time per V-cycle of the oracle's serial C restatement of the
+    stand-alone 3-D driver on the 97^3 sinker (after the first, set-up
+    cycle), for the driver leg of the sinker benchmark."""
+    from earthbox_gpu import workloads
+    from oracle import mg3
+    S = workloads.stokes_sinker3d(97)
+    run = dict(workloads.SINKER_RUNS["3d"])
+    run.pop("n")
+    run.pop("nvcycles")
+    crit = run.pop("convergence_criterion")
+    kw = dict(stokesscale=S["stokesscale"], continscale=S["continscale"], pressure_bc=S["pressure_bc"],
+              **workloads.SINKER_RELAXATION, **run)
+
+    def timed(nv):
+        H = mg3.Hierarchy(97, 97, 97, 1e5, 1e5, 1e5)
+        for k in ("etan", "etaxy", "etaxz", "etayz", "RX", "RY", "RZ", "RC", "vx", "vy", "vz", "pr"):
+            getattr(H.levels[0], k)[...] = S[k]
+        t0 = time.perf_counter()
+        H.run(nv, crit, **kw)
+        return time.perf_counter() - t0
+    per = (timed(1 + cycles) - timed(1)) / cycles
+    return {"seconds_per_cycle": per, "seconds_250_cycles_extrapolated": 250 * per, "cores": 1,
+            "kind": "port", "sample": f"{cycles} V-cycles of the 97^3 sinker (oracle C restatement)"}
 
 
 def bench_stokes(ctx, nx, ny, max_iter):
@@ -247,8 +363,12 @@ def main():
         out["scatter"] = bench_scatter(ctx, g, x, y)
     if not args.no_stokes3d:
         out["stokes3d"] = bench_stokes3d(ctx, args.stokes3d_n, args.stokes3d_max_iter)
+    if not args.no_sinker:
+        out["sinker"] = bench_sinker(ctx)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, x, y, vx1, vy1, to_host_np(desp), dt, 2_000_000)
+        if not args.no_sinker:
+            out["cpu_baseline"]["mg3d_97_reference_driver"] = cpu_sinker_driver()
     print(json.dumps(out), flush=True)
     ctx.close()
 
