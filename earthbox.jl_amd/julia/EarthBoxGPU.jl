@@ -10,6 +10,8 @@ EarthBox.jl.  It keeps the reference's entry points as the seams:
                                         (interpolation/Interpolation.jl:87-95)
 - `advect_markers_gpu!(model)`         beside Advection.advect_markers_and_rotate_stress_tensor!
                                         (advection/Advection.jl:109-114)
+- `run_multigrid_solver_gpu!(multigrid_data)` beside MultigridSolver.run_multigrid_solver
+                                        (multigrid/multigrid_solver/MultigridSolver.jl:113-135, 3-D)
 
 Every call copies the needed ModelData arrays to device buffers owned by a
 `GPUState`, runs the kernels and copies results back, so ModelData stays the
@@ -48,6 +50,25 @@ struct SolveStats
     seconds::Cdouble
     status::Cint
     learned_accepted::Cint
+end
+
+"Field order of ebg_mg_run_params (include/earthbox_gpu.h)."
+struct MGRunParams
+    nvcycles::Cint
+    nvcycles_viscosity_jump::Cint
+    finest_sweeps::Cint
+    sweep_multiplier::Cint
+    sweep_cap::Cint
+    nviscosity_jumps::Cdouble
+    convergence_criterion::Cdouble
+    resmax::Cdouble
+    stokesscale::Cdouble
+    continscale::Cdouble
+    theta_s::Cdouble
+    theta_c::Cdouble
+    relax_v::Cdouble
+    relax_p::Cdouble
+    pressure_bc::Cdouble
 end
 
 check(rc, what) = rc < 0 ? error("$what failed: ",
@@ -193,6 +214,61 @@ function advect_markers_gpu!(st::GPUState, model, inside_flags::Vector{Int8})
     d2h!(m.stress.marker_sxx.array, dsxx)
     d2h!(m.stress.marker_sxy.array, dsxy)
     return nothing
+end
+
+"""
+    run_multigrid_solver_gpu!(st, multigrid_data) -> (cycles, converged)
+
+GPU replacement of MultigridSolver.run_multigrid_solver for MultigridData3d
+with use_multimulti = false (multigrid_solver/MultigridSolver.jl:113-135).
+Uploads level_vector[1] (viscosity, right-hand sides, initial solution), runs
+the continuation and V-cycles on the device, and writes back the solution, the
+scaled viscosity and the principal mean residuals, as the reference leaves them.
+"""
+function run_multigrid_solver_gpu!(st::GPUState, md)
+    md.vcycle.use_multimulti && error("run_multigrid_solver_gpu!: multimulti is not supported")
+    lv = md.level_vector[1]
+    geo = lv.grid.parameters.geometry
+    nlev = Ref{Cint}(0)
+    check(ccall((:ebg_mg3d_create, LIB), Cint,
+        (Ptr{Cvoid}, Int64, Int64, Int64, Cdouble, Cdouble, Cdouble, Cint, Ref{Cint}),
+        st.ctx, geo.xnum.value, geo.ynum.value, geo.znum.value, geo.xsize.value, geo.ysize.value,
+        geo.zsize.value, length(md.level_vector), nlev), "ebg_mg3d_create")
+    ids = (etan=0, etaxy=1, etaxz=2, etayz=3, vx=4, vy=5, vz=6, pr=7, RX=8, RY=9, RZ=10, RC=11)
+    copy_field(id, dst, src) = check(ccall((:ebg_mg3d_copy_field, LIB), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Ptr{Cvoid}, Ptr{Cvoid}), st.ctx, 0, id, dst, src), "ebg_mg3d_copy_field")
+    for (name, id) in pairs(ids)
+        hasproperty(lv, name) || continue
+        copy_field(id, C_NULL, h2d!(st, Symbol(:mg3_, name), getproperty(lv, name).array))
+    end
+    s = md.smoothing_iterations
+    cap = length(s) > 1 ? maximum(s[2:end]) : 0
+    mult = length(s) > 1 ? div(s[2], s[1]) : 2
+    vs, rx = md.viscosity_scaling, md.relaxation
+    p = Ref(MGRunParams(md.vcycle.nvcycles, vs.nvcycles_viscosity_jump, s[1], mult, cap,
+                        vs.nviscosity_jumps, md.vcycle.convergence_criterion, vs.resmax,
+                        md.residual_scaling_factors.stokesscale, md.residual_scaling_factors.continscale,
+                        rx.relax_stokes, rx.relax_continuity, rx.relax_velocity, rx.relax_pressure,
+                        md.pressure_bc))
+    cycles, conv = Ref{Cint}(0), Ref{Cint}(0)
+    hist = zeros(Float64, 4 * max(md.vcycle.nvcycles, 1))
+    check(ccall((:ebg_mg3d_run, LIB), Cint, (Ptr{Cvoid}, Ref{MGRunParams}, Ref{Cint}, Ref{Cint}, Ptr{Float64}),
+                st.ctx, p, cycles, conv, hist), "ebg_mg3d_run")
+    for name in (:vx, :vy, :vz, :pr, :etan, :etaxy, :etaxz, :etayz)
+        a = getproperty(lv, name).array
+        dp = dev!(st, Symbol(:mg3_, name), a)
+        copy_field(ids[name], dp, C_NULL)
+        check(ccall((:ebg_ctx_synchronize, LIB), Cint, (Ptr{Cvoid},), st.ctx), "sync")
+        d2h!(a, dp)
+    end
+    mr = md.mean_residuals
+    for c in 1:cycles[]
+        mr.resx_principle[c] = hist[4c - 3]
+        mr.resy_principle[c] = hist[4c - 2]
+        mr.resz_principle[c] = hist[4c - 1]
+        mr.resc_principle[c] = hist[4c]
+    end
+    return Int(cycles[]), conv[] != 0
 end
 
 end # module
