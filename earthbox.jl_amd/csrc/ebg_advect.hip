@@ -42,9 +42,11 @@ struct RK4Args {
 // the second step return RN(a/b): with rb within half an ulp of 1/b and q
 // within one ulp of a/b, r = a - b*q is exact and RN(q + r*rb) = RN(a/b).
 // The theorem needs no underflow or overflow in r and q: callers use it only
-// for numerators that pass div_in_range() (a zero numerator keeps its sign by
-// the final select) and take the plain IEEE division otherwise, so the result
-// always equals a / b bit for bit.
+// for numerators that pass div_in_range() and divisors in [2^-100, 2^100]
+// (ebg_grid2d_set rejects grid steps outside that range), and take the plain
+// IEEE division otherwise, so the result always equals a / b bit for bit.
+// The divisors are positive, so a / b has the sign of a; copying that sign
+// onto q also turns the +0 the steps return for a = -0 into -0.
 __device__ __forceinline__ double rn_div(double a, double b, double rb)
 {
     double q = a * rb;
@@ -52,7 +54,7 @@ __device__ __forceinline__ double rn_div(double a, double b, double rb)
     q = __builtin_fma(r, rb, q);
     r = __builtin_fma(-b, q, a);
     q = __builtin_fma(r, rb, q);
-    return (a == 0.0) ? a : q;
+    return __builtin_copysign(q, a);
 }
 
 // Numerators for which rn_div is exact with divisors and reciprocals between
@@ -64,21 +66,26 @@ __device__ __forceinline__ bool div_in_range(double a)
 }
 
 // GridToMarker.get_marker_value (interpolation/utils/GridToMarker.jl:190-226):
-// bilinear weights, summed UL, LL, UR, LR starting from 0.0.
-__device__ __forceinline__ double marker_value(int yn, int xn, double dy, double dx,
-                                               const double* __restrict__ a, int ld)
+// bilinear weights, summed UL, LL, UR, LR starting from 0.0; p points at the
+// upper-left node of a column-major array with leading dimension ld.
+__device__ __forceinline__ double bilinear(const double* p, int ld, double dy, double dx)
 {
     const double wul = (1.0 - dx) * (1.0 - dy);
     const double wll = (1.0 - dx) * dy;
     const double wur = dx * (1.0 - dy);
     const double wlr = dx * dy;
-    const double* p = a + (unsigned)((xn - 1) * ld + (yn - 1));
     double v = 0.0;
     v += wul * p[0];
     v += wll * p[1];
     v += wur * p[ld];
     v += wlr * p[ld + 1];
     return v;
+}
+
+__device__ __forceinline__ double marker_value(int yn, int xn, double dy, double dx,
+                                               const double* __restrict__ a, int ld)
+{
+    return bilinear(a + (unsigned)((xn - 1) * ld + (yn - 1)), ld, dy, dx);
 }
 
 __device__ __forceinline__ int cell_index(double x, const AxisDesc& a)
@@ -92,17 +99,30 @@ __device__ __forceinline__ int cell_index(double x, const AxisDesc& a)
 // t = (x - x0)/h is in [k-1.25, k+0.25) and the guess floor(t)+1 is k-1, k or
 // k+1; one exact comparison against the guessed cell's stored bounds
 // (lo = g[c-1], hi = g[c]) then gives k.  Non-uniform axes: binary search.
+// Written without branches: the guess clamps t to [0, n-2] (a NaN t becomes
+// 0), which gives 1 for t < 1, n-1 for t >= n-2 and floor(t)+1 otherwise;
+// a NaN x selects n-1 at the end.
+__device__ __forceinline__ int uniform_guess(double x, const AxisDesc& a)
+{
+    const double t = (x - a.x0) * a.inv_h;
+    return (int)fmin(fmax(t, 0.0), (double)(a.n - 2)) + 1;
+}
+
+__device__ __forceinline__ int uniform_fixup(double x, int c, double lo, double hi, int n)
+{
+    const int dn = (c > 1 && x < lo) ? 1 : 0;
+    const int up = (c < n - 1 && x >= hi) ? 1 : 0;
+    c = dn ? c - 1 : c + up;
+    return (x != x) ? n - 1 : c;
+}
+
 template <bool UNIFORM>
 __device__ __forceinline__ int stage_cell(double x, const AxisDesc& a)
 {
     if (!UNIFORM) return left_node_index(x, a.g, a.n, 0.0, 0.0, 0);
-    if (x != x) return a.n - 1;
-    const double t = (x - a.x0) * a.inv_h;
-    int c = (t < 1.0) ? 1 : ((t >= (double)(a.n - 2)) ? a.n - 1 : (int)t + 1);
+    const int c = uniform_guess(x, a);
     const double2 lh = *reinterpret_cast<const double2*>(&a.cells[c - 1].lo);
-    if (c > 1 && x < lh.x) --c;
-    else if (c < a.n - 1 && x >= lh.y) ++c;
-    return c;
+    return uniform_fixup(x, c, lh.x, lh.y, a.n);
 }
 
 // Velocity and spin at one point (VelocityAndSpin.jl:343-428 with the
@@ -111,37 +131,50 @@ __device__ __forceinline__ int stage_cell(double x, const AxisDesc& a)
 // strict '>'), spin on the basic grid.  The reference's clamps of the remapped
 // indices (iyv <= ynum, jxv <= xnum) cannot bind: the basic cell is at most
 // n-1, so the remap gives at most n.
+struct StageWeights {
+    double dxb, dyb, dyv, dxy;
+    bool rx, ry;
+};
+
+__device__ __forceinline__ StageWeights stage_weights(const AxisCell& cx, const AxisCell& cy, double xc,
+                                                      double yc)
+{
+    StageWeights w;
+    w.ry = yc > cy.sb;
+    w.rx = xc > cx.sb;
+    const double nxb = xc - cx.lo, nyb = yc - cy.lo;
+    const double nyv = yc - (w.ry ? cy.sb : cy.sa), nxy = xc - (w.rx ? cx.sb : cx.sa);
+    const double hyv = w.ry ? cy.shb : cy.sha, hxy = w.rx ? cx.shb : cx.sha;
+    if (div_in_range(nxb) && div_in_range(nyb) && div_in_range(nyv) && div_in_range(nxy)) {
+        w.dxb = rn_div(nxb, cx.h, cx.rh);
+        w.dyb = rn_div(nyb, cy.h, cy.rh);
+        w.dyv = rn_div(nyv, hyv, w.ry ? cy.rshb : cy.rsha);
+        w.dxy = rn_div(nxy, hxy, w.rx ? cx.rshb : cx.rsha);
+    } else {
+        w.dxb = nxb / cx.h;
+        w.dyb = nyb / cy.h;
+        w.dyv = nyv / hyv;
+        w.dxy = nxy / hxy;
+    }
+    return w;
+}
+
 __device__ __forceinline__ void point_values(const RK4Args& a, double xc, double yc, int xi,
                                              int yi, double& vx, double& vy, double& sp)
 {
     const AxisCell cx = a.ax.cells[xi - 1];
     const AxisCell cy = a.ay.cells[yi - 1];
-    const bool ry = yc > cy.sb;
-    const bool rx = xc > cx.sb;
-    const double nxb = xc - cx.lo, nyb = yc - cy.lo;
-    const double nyv = yc - (ry ? cy.sb : cy.sa), nxy = xc - (rx ? cx.sb : cx.sa);
-    const double hyv = ry ? cy.shb : cy.sha, hxy = rx ? cx.shb : cx.sha;
-    double dxb, dyb, dyv, dxy;
-    if (div_in_range(nxb) && div_in_range(nyb) && div_in_range(nyv) && div_in_range(nxy)) {
-        dxb = rn_div(nxb, cx.h, cx.rh);
-        dyb = rn_div(nyb, cy.h, cy.rh);
-        dyv = rn_div(nyv, hyv, ry ? cy.rshb : cy.rsha);
-        dxy = rn_div(nxy, hxy, rx ? cx.rshb : cx.rsha);
-    } else {
-        dxb = nxb / cx.h;
-        dyb = nyb / cy.h;
-        dyv = nyv / hyv;
-        dxy = nxy / hxy;
-    }
-    vx = marker_value(yi + (ry ? 1 : 0), xi, dyv, dxb, a.vx1, a.ynum + 1);
-    vy = marker_value(yi, xi + (rx ? 1 : 0), dyb, dxy, a.vy1, a.ynum);
-    sp = marker_value(yi, xi, dyb, dxb, a.esp, a.ynum);
+    const StageWeights w = stage_weights(cx, cy, xc, yc);
+    vx = marker_value(yi + (w.ry ? 1 : 0), xi, w.dyv, w.dxb, a.vx1, a.ynum + 1);
+    vy = marker_value(yi, xi + (w.rx ? 1 : 0), w.dyb, w.dxy, a.vy1, a.ynum);
+    sp = marker_value(yi, xi, w.dyb, w.dxb, a.esp, a.ynum);
 }
 
-// (v1 + 2 v2 + 2 v3 + v4)/6 in the reference's order (RungeKutta.jl:36-47).
-__device__ __forceinline__ double rk4_combine(double v1, double v2, double v3, double v4)
+// (v1 + 2 v2 + 2 v3 + v4)/6 in the reference's order (RungeKutta.jl:36-47),
+// i.e. ((v1 + 2 v2) + 2 v3) + v4: the kernel accumulates the partial sum
+// stage by stage, which is the same sequence of roundings, and divides here.
+__device__ __forceinline__ double rk4_sixth(double s)
 {
-    const double s = v1 + 2.0 * v2 + 2.0 * v3 + v4;
     constexpr double r6 = 1.0 / 6.0;
     return div_in_range(s) ? rn_div(s, 6.0, r6) : s / 6.0;
 }
@@ -181,26 +214,32 @@ __global__ void __launch_bounds__(256) rk4_advect_kernel(RK4Args a)
         xi = stage_cell<UNI>(x0, a.ax);
         yi = stage_cell<UNI>(y0, a.ay);
     }
-    double v1x, v1y, v1s;
-    point_values(a, x0, y0, xi, yi, v1x, v1y, v1s);
+    double vx, vy, vs;
+    point_values(a, x0, y0, xi, yi, vx, vy, vs);
     if (NST == 4) {
-        double v2x, v2y, v2s, v3x, v3y, v3s, v4x, v4y, v4s;
-        double xc = x0 + a.half_dt * v1x;
-        double yc = y0 + a.half_dt * v1y;
-        point_values(a, xc, yc, stage_cell<UNI>(xc, a.ax), stage_cell<UNI>(yc, a.ay), v2x, v2y, v2s);
-        xc = x0 + a.half_dt * v2x;
-        yc = y0 + a.half_dt * v2y;
-        point_values(a, xc, yc, stage_cell<UNI>(xc, a.ax), stage_cell<UNI>(yc, a.ay), v3x, v3y, v3s);
-        xc = x0 + a.dt * v3x;
-        yc = y0 + a.dt * v3y;
-        point_values(a, xc, yc, stage_cell<UNI>(xc, a.ax), stage_cell<UNI>(yc, a.ay), v4x, v4y, v4s);
-        svx = rk4_combine(v1x, v2x, v3x, v4x);
-        svy = rk4_combine(v1y, v2y, v3y, v4y);
-        ssp = rk4_combine(v1s, v2s, v3s, v4s);
+        double ax = vx, ay = vy, as = vs;   // running ((v1 + 2 v2) + 2 v3) + v4
+        double xc = x0 + a.half_dt * vx;
+        double yc = y0 + a.half_dt * vy;
+        point_values(a, xc, yc, stage_cell<UNI>(xc, a.ax), stage_cell<UNI>(yc, a.ay), vx, vy, vs);
+        ax = ax + 2.0 * vx;
+        ay = ay + 2.0 * vy;
+        as = as + 2.0 * vs;
+        xc = x0 + a.half_dt * vx;
+        yc = y0 + a.half_dt * vy;
+        point_values(a, xc, yc, stage_cell<UNI>(xc, a.ax), stage_cell<UNI>(yc, a.ay), vx, vy, vs);
+        ax = ax + 2.0 * vx;
+        ay = ay + 2.0 * vy;
+        as = as + 2.0 * vs;
+        xc = x0 + a.dt * vx;
+        yc = y0 + a.dt * vy;
+        point_values(a, xc, yc, stage_cell<UNI>(xc, a.ax), stage_cell<UNI>(yc, a.ay), vx, vy, vs);
+        svx = rk4_sixth(ax + vx);
+        svy = rk4_sixth(ay + vy);
+        ssp = rk4_sixth(as + vs);
     } else {
-        svx = v1x;
-        svy = v1y;
-        ssp = v1s;
+        svx = vx;
+        svy = vy;
+        ssp = vs;
     }
     if (a.mvx) a.mvx[m] = svx;
     if (a.mvy) a.mvy[m] = svy;

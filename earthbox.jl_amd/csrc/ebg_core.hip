@@ -189,6 +189,15 @@ int ebg_grid2d_set(ebg_ctx* ctx, int64_t xnum, int64_t ynum,
         EBG_CHECK(gridx_b[i] > gridx_b[i - 1], EBG_ERR_ARG, "ebg_grid2d_set: gridx_b not increasing");
     for (int64_t i = 1; i < ynum; ++i)
         EBG_CHECK(gridy_b[i] > gridy_b[i - 1], EBG_ERR_ARG, "ebg_grid2d_set: gridy_b not increasing");
+    // the marker kernels' exact division needs every step in [2^-100, 2^100]
+    auto steps_ok = [](const double* s, int64_t n) {
+        for (int64_t i = 0; i < n; ++i)
+            if (!(s[i] >= 0x1p-100 && s[i] <= 0x1p100)) return false;
+        return true;
+    };
+    EBG_CHECK(steps_ok(xstp_b, xnum - 1) && steps_ok(ystp_b, ynum - 1) && steps_ok(xstp_vy, xnum) &&
+                  steps_ok(ystp_vx, ynum),
+              EBG_ERR_ARG, "ebg_grid2d_set: grid steps must lie in [2^-100, 2^100]");
     EBG_HIP(hipSetDevice(ctx->device));
     EBG_HIP(hipStreamSynchronize(ctx->stream));
     Grid2D& g = ctx->grid;

@@ -59,7 +59,10 @@ int ebg_memcpy_d2h(void *dst_host, const void *src_dev, size_t bytes);
  * HOST pointers: gridx_b[xnum], gridy_b[ynum], xstp_b[xnum-1], ystp_b[ynum-1],
  * gridx_vy[xnum+1], xstp_vy[xnum], gridy_vx[ynum+1], ystp_vx[ynum].
  * The domain limits used for inside flags are gridx_b[1], gridx_b[xnum],
- * gridy_b[1], gridy_b[ynum] (staggered_grid/utils/Dimensions.jl:6-19). */
+ * gridy_b[1], gridy_b[ynum] (staggered_grid/utils/Dimensions.jl:6-19).
+ * Basic coordinates must increase and every step must lie in
+ * [2^-100, 2^100] (EBG_ERR_ARG otherwise): the marker kernels' correctly
+ * rounded division by the steps relies on it. */
 int ebg_grid2d_set(ebg_ctx *ctx, int64_t xnum, int64_t ynum,
                    const double *gridx_b, const double *gridy_b,
                    const double *xstp_b, const double *ystp_b,

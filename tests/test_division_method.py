@@ -3,7 +3,10 @@ remainder corrections with rb = RN(1/b) (rn_div in csrc/ebg_advect.hip).
 This test evaluates that exact sequence with exact rational arithmetic (each
 fma rounded once, as the hardware does) and checks it against the correctly
 rounded quotient on random, near-tie and extreme-significand operands, within
-the numerator range the kernel admits ([2^-900, 2^900] and zero)."""
+the numerator range the kernel admits ([2^-900, 2^900] and zero) and the
+positive divisors it admits ([2^-100, 2^100]); the sign of a zero numerator is
+restored by copying the numerator's sign onto the result."""
+import math
 from fractions import Fraction
 
 import numpy as np
@@ -19,7 +22,7 @@ def rn_div(a: float, b: float, rb: float) -> float:
     q = _fma(r, rb, q)
     r = _fma(-b, q, a)
     q = _fma(r, rb, q)
-    return a if a == 0.0 else q
+    return math.copysign(q, a)
 
 
 def _cases(rng):

@@ -37,6 +37,25 @@ def _case(nx, ny, nm, seed, outside=True, graded=False):
 def test_rk4_advect_bit_exact(gpu_ctx, nx, ny, nm, seed, graded, mode):
     g, x, y, vx1, vy1, esp, dt = _case(nx, ny, nm, seed, graded=graded)
     print(f"rk4 case nx={nx} ny={ny} markers={x.size} graded={graded} mode={mode}", flush=True)
+    _check_rk4(gpu_ctx, g, x, y, vx1, vy1, esp, dt, seed, mode)
+
+
+@pytest.mark.parametrize("order", ["reference", "shuffled"])
+@pytest.mark.parametrize("courant", [0.5, 3.0])
+def test_rk4_shuffled_and_large_steps_bit_exact(gpu_ctx, order, courant):
+    """Markers in random order and steps of several cells per stage (stage
+    points far from the stage-1 cell, some outside the grid) match the oracle
+    bit for bit."""
+    g, x, y, vx1, vy1, esp, _ = _case(203, 61, 5, 7)
+    dt = courant * g.xstpavg / 1e-9
+    if order == "shuffled":
+        p = np.random.default_rng(8).permutation(x.size)
+        x, y = x[p].copy(), y[p].copy()
+    print(f"rk4 window fallbacks order={order} courant={courant}", flush=True)
+    _check_rk4(gpu_ctx, g, x, y, vx1, vy1, esp, dt, 7, "minimal")
+
+
+def _check_rk4(gpu_ctx, g, x, y, vx1, vy1, esp, dt, seed, mode):
     flags = oracle.inside_flags(x, y, g.limits)
     xn, yn = oracle.cell_indices_basic(x, y, flags, g)
     rng = np.random.default_rng(seed)
