@@ -21,6 +21,7 @@ LIB_NAME = "libearthbox_gpu.so"
 OBJ_DIR = HERE / "_build"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM_LIB = os.environ.get("ROCM_LIB", "/opt/rocm/lib")
 ARCH = "gfx950"
 
 COMMON_FLAGS = [
@@ -58,7 +59,7 @@ def build(verbose: bool = False) -> pathlib.Path:
     out = OUT_DIR / LIB_NAME
     tmp = OUT_DIR / (LIB_NAME + ".tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
-           *map(str, objs)]
+           *map(str, objs), f"-L{ROCM_LIB}", "-lrocsolver", "-lrocblas"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")

@@ -256,7 +256,12 @@ void mg_vcycle(MGHierarchy<T>& H, const std::vector<int>& sm, T theta_s, T theta
                T relax_p, T prnorm, const T* prnorm_dev, const T* bc_d, hipStream_t s)
 {
     const int nl = H.nlev();
+    const bool direct = H.coarse_inv && nl > 1;   // exact solve replaces the coarsest smoothing
     for (int n = 0; n < nl; ++n) {
+        if (direct && n == nl - 1) {
+            mg_coarse_solve(H, bc_d, s);
+            break;
+        }
         if (n > 0) mg_zero_solution(H, n, s);
         mg_smoother(H, n, sm[n], theta_s, theta_c, prnorm, prnorm_dev, bc_d, s);
         if (n < nl - 1) {
@@ -276,7 +281,8 @@ void mg_vcycle(MGHierarchy<T>& H, const std::vector<int>& sm, T theta_s, T theta
         if (n == 0) {
             mg_smoother(H, 0, sm[0], theta_s, theta_c, prnorm, prnorm_dev, bc_d, s);
         } else {
-            mg_smoother(H, n, sm[n], theta_s, theta_c, T(0), (const T*)nullptr, bc_d, s);
+            if (!(direct && n == nl - 1))
+                mg_smoother(H, n, sm[n], theta_s, theta_c, T(0), (const T*)nullptr, bc_d, s);
             const MGLevelDev<T>& c = H.lev[n];
             const MGLevelDev<T>& f = H.lev[n - 1];
             hipLaunchKernelGGL(mg_prolong_kernel<T>, dim3(nblk((int64_t)(f.ynum + 1) * (f.xnum + 1))),
@@ -380,7 +386,12 @@ void mg_vcycle_tiled(MGHierarchy<T>& H, const std::vector<int>& sm, int local_sw
                      hipStream_t s)
 {
     const int nl = H.nlev();
+    const bool direct = H.coarse_inv && nl > 1;   // exact solve replaces the coarsest smoothing
     for (int n = 0; n < nl; ++n) {
+        if (direct && n == nl - 1) {
+            mg_coarse_solve(H, bc_d, s);
+            break;
+        }
         if (n > 0) mg_zero_solution(H, n, s);
         mg_smoother_tiled(H, n, sm[n], local_sweeps, theta_s, theta_c, prnorm, prnorm_dev, bc_d, s);
         if (n < nl - 1) {
@@ -400,7 +411,8 @@ void mg_vcycle_tiled(MGHierarchy<T>& H, const std::vector<int>& sm, int local_sw
         if (n == 0) {
             mg_smoother_tiled(H, 0, sm[0], local_sweeps, theta_s, theta_c, prnorm, prnorm_dev, bc_d, s);
         } else {
-            mg_smoother_tiled(H, n, sm[n], local_sweeps, theta_s, theta_c, T(0), (const T*)nullptr, bc_d, s);
+            if (!(direct && n == nl - 1))
+                mg_smoother_tiled(H, n, sm[n], local_sweeps, theta_s, theta_c, T(0), (const T*)nullptr, bc_d, s);
             const MGLevelDev<T>& c = H.lev[n];
             const MGLevelDev<T>& f = H.lev[n - 1];
             hipLaunchKernelGGL(mg_prolong_kernel<T>, dim3(nblk((int64_t)(f.ynum + 1) * (f.xnum + 1))),

@@ -514,6 +514,12 @@ struct MGHierarchy {
     };
     std::vector<TilePlan> tiles;
     bool fast_div = false;              // tiled smoother: FP32 hardware-reciprocal division
+    // optional direct solve of the coarsest level (mg_coarse_setup): the
+    // row-major inverse of its operator and two vectors of coarse_n entries;
+    // the memory belongs to the caller
+    const T* coarse_inv = nullptr;
+    T *coarse_b = nullptr, *coarse_x = nullptr;
+    int coarse_n = 0;
     int nlev() const { return (int)lev.size(); }
     ~MGHierarchy();
 };
@@ -550,6 +556,27 @@ template <typename T>
 void mg_vcycle_tiled(MGHierarchy<T>& H, const std::vector<int>& smoothing, int local_sweeps,
                      T theta_s, T theta_c, T relax_v, T relax_p, T prnorm, const T* prnorm_dev,
                      const T* bc_d, hipStream_t s);
+
+// ---- direct solve of the coarsest level (ebg_coarse.hip) ----
+// Unknowns of a level: interior vx (i 2..ynum, j 2..xnum-1), interior vy
+// (i 2..ynum-1, j 2..xnum), every pressure cell.
+inline int64_t mg_level_unknowns(int xnum, int ynum)
+{
+    return (int64_t)(ynum - 1) * (xnum - 2) + (int64_t)(ynum - 2) * (xnum - 1) + (int64_t)(ynum - 1) * (xnum - 1);
+}
+// Assembles the operator of level k in FP64 (the smoother's ghost rules with
+// the mirror factors bc_d, the pressure gauge p(1,1) = 0 in place of the
+// continuity row of cell (1,1)) into A (n x n, column-major, device), factors
+// it with rocSOLVER, solves for the inverse in work (n x n FP64) and stores it
+// row-major in precision T into inv.  Returns EBG_ERR_NUMERIC when the
+// operator is singular.
+template <typename T>
+int mg_coarse_setup(const MGHierarchy<T>& H, int k, const double* bc_d, double* A, double* work, int* ipiv,
+                    int* info, T* inv, void* solver_handle, hipStream_t s);
+// x = inverse * b on the coarsest level: gathers RX, RY, RC, writes vx, vy, pr
+// (zero elsewhere) and refreshes the ghosts.
+template <typename T>
+void mg_coarse_solve(MGHierarchy<T>& H, const T* bc_d, hipStream_t s);
 
 // ---- pieces of the reference's stand-alone multigrid driver (2-D and 3-D) ----
 // Viscosity continuation: the initial range and step factor of

@@ -2,7 +2,10 @@
 // (stokes_continuity_solver/multigrid/, driver MultigridSolver.run_multigrid_solver
 // at multigrid_solver/MultigridSolver.jl:113-135), FP64 on the device.
 #include "ebg_common.hpp"
+#include "ebg_krylov.hpp"
 #include "ebg_mg.hpp"
+
+#include <rocblas/rocblas.h>
 
 #include <array>
 #include <cmath>
@@ -196,6 +199,53 @@ int ebg_mg2d_vcycles(ebg_ctx* ctx, int ncycles, int finest_sweeps, int multiplie
                   bc_d4 ? bc_d4 : one, ctx->stream);
     EBG_HIP(hipGetLastError());
     return EBG_OK;
+}
+
+// Exact solve of one level's equations for its RX, RY, RC (the dense direct
+// solve the Stokes preconditioner uses on its coarsest level): assembles the
+// level's operator, inverts it, and writes vx, vy, pr with p(1,1) = 0 and
+// the ghost pass applied.
+int ebg_mg2d_direct_solve(ebg_ctx* ctx, int level, const double* bc_d4)
+{
+    MGRef* m = get(ctx);
+    EBG_CHECK(m, EBG_ERR_STATE, "ebg_mg2d_direct_solve: no hierarchy");
+    EBG_CHECK(level >= 0 && level < m->H.nlev(), EBG_ERR_ARG, "ebg_mg2d_direct_solve: bad level");
+    const MGLevelDev<double>& l = m->H.lev[level];
+    const int64_t n = mg_level_unknowns(l.xnum, l.ynum);
+    EBG_CHECK(n <= 20000, EBG_ERR_ARG, "ebg_mg2d_direct_solve: level has more than 20000 unknowns");
+    const double one[4] = {1.0, 1.0, 1.0, 1.0};
+    const double* bcd = bc_d4 ? bc_d4 : one;
+    hipStream_t s = ctx->stream;
+    DevBuf A, W, ipiv, info, inv, vec;
+    int rc;
+    if ((rc = A.ensure((size_t)n * n * sizeof(double))) || (rc = W.ensure((size_t)n * n * sizeof(double))) ||
+        (rc = ipiv.ensure((size_t)n * sizeof(int))) ||
+        (rc = info.ensure(sizeof(int))) || (rc = inv.ensure((size_t)n * n * sizeof(double))) ||
+        (rc = vec.ensure((size_t)2 * n * sizeof(double))))
+        return rc;
+    rocblas_handle h = nullptr;
+    EBG_CHECK(rocblas_create_handle(&h) == rocblas_status_success, EBG_ERR_HIP,
+              "ebg_mg2d_direct_solve: rocblas_create_handle failed");
+    rc = mg_coarse_setup(m->H, level, bcd, A.as<double>(), W.as<double>(), ipiv.as<int>(), info.as<int>(),
+                         inv.as<double>(), h, s);
+    if (rc == EBG_OK) {
+        // solve on a view of the hierarchy that ends at `level`
+        MGHierarchy<double>& H = m->H;
+        std::vector<MGLevelDev<double>> saved(H.lev.begin() + level + 1, H.lev.end());
+        H.lev.resize(level + 1);
+        H.coarse_inv = inv.as<double>();
+        H.coarse_b = vec.as<double>();
+        H.coarse_x = vec.as<double>() + n;
+        H.coarse_n = (int)n;
+        mg_coarse_solve(H, bcd, s);
+        H.coarse_inv = nullptr;
+        H.lev.insert(H.lev.end(), saved.begin(), saved.end());
+        EBG_HIP(hipStreamSynchronize(s));
+    } else {
+        (void)hipStreamSynchronize(s);
+    }
+    (void)rocblas_destroy_handle(h);
+    return rc;
 }
 
 // MultigridSolver.run_multigrid_solver for MultigridData2d with

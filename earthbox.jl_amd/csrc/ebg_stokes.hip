@@ -13,6 +13,8 @@
 #include "ebg_stokes_op.hpp"
 #include "ebg_learned.hpp"
 
+#include <rocblas/rocblas.h>
+
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -474,10 +476,20 @@ struct StokesState {
     // was built for, and whether the viscosity must be restricted again
     uint64_t grid_gen = 0;  // grid generation the operator was built on
     struct HKey {
-        int nlev = -1, tile = -1;
+        int nlev = -1, tile = -1, coarse = -1;
         uint64_t grid = 0;
         bool visc_dirty = true;
+        bool dense_dirty = true;   // the coarsest level's inverse must be rebuilt
+        double bcd[4] = {0, 0, 0, 0};
     } key_f, key_d;
+    // direct coarsest-level solve: FP64 assembly/LU workspace, pivots, info,
+    // the per-precision inverses and coarse vectors, and the solver handle
+    DevBuf cA, cW, cipiv, cinfo, cinv_f, cinv_d, cvec_f, cvec_d;
+    void* solver = nullptr;
+    ~StokesState()
+    {
+        if (solver) (void)rocblas_destroy_handle((rocblas_handle)solver);
+    }
     // learned correction: candidate vector, judge residuals and acceptance count
     DevBuf zc, jr0, jr1, judge, pair_e, pair_r, pair_z;
 };
@@ -529,7 +541,7 @@ template <typename T>
 static int ensure_hierarchy(ebg_ctx* ctx, StokesState& st, std::unique_ptr<MGHierarchy<T>>& H,
                             DevBuf& prn, StokesState::HKey& key, const ebg_solve_opts& o)
 {
-    if (!H || key.nlev != o.nlevels || key.grid != ctx->grid.generation) {
+    if (!H || key.nlev != o.nlevels || key.coarse != o.coarse_unknowns || key.grid != ctx->grid.generation) {
         H.reset(new MGHierarchy<T>());
         std::vector<HostLevelGrid> fine{HostLevelGrid{}};
         const Grid2D& g = ctx->grid;
@@ -555,10 +567,23 @@ static int ensure_hierarchy(ebg_ctx* ctx, StokesState& st, std::unique_ptr<MGHie
             h.gridy_pr[k] = h.gridy_pr[k - 1] + (g.h_gridy_b[k + 1] - g.h_gridy_b[k - 1]) / 2.0;
         const double xsize = g.h_gridx_b.back() - g.h_gridx_b.front();
         const double ysize = g.h_gridy_b.back() - g.h_gridy_b.front();
-        int rc = mg_build(*H, (int)g.xnum, (int)g.ynum, xsize, ysize, o.nlevels, &fine, true);
+        int nl = o.nlevels;
+        if (o.coarse_unknowns > 0) {   // stop at the first coarse level small enough for a direct solve
+            auto sizes = mg_level_sizes((int)g.xnum, (int)g.ynum, 10);
+            if (nl > 0 && nl < (int)sizes.size()) sizes.resize(nl);
+            nl = (int)sizes.size();
+            for (int k = 1; k < (int)sizes.size(); ++k)
+                if (mg_level_unknowns(sizes[k].first, sizes[k].second) <= o.coarse_unknowns) {
+                    nl = k + 1;
+                    break;
+                }
+        }
+        int rc = mg_build(*H, (int)g.xnum, (int)g.ynum, xsize, ysize, nl, &fine, true);
         if (rc) return rc;
         if ((rc = prn.ensure(sizeof(T) * 4))) return rc;
         key.nlev = o.nlevels;
+        key.coarse = o.coarse_unknowns;
+        key.dense_dirty = true;
         key.grid = ctx->grid.generation;
         key.tile = -1;
         key.visc_dirty = true;
@@ -632,6 +657,44 @@ static void multidot(StokesState& st, const double* w, const double* V, int64_t 
                        RED_BLOCKS, k, out, accumulate);
 }
 
+// Direct solve of the coarsest level when it is small enough: the inverse is
+// rebuilt after a new viscosity or different ghost mirror factors.
+template <typename T>
+static int prepare_coarse(StokesState& st, MGHierarchy<T>& H, StokesState::HKey& key, DevBuf& inv,
+                          DevBuf& vec, const ebg_solve_opts& o, hipStream_t s)
+{
+    H.coarse_inv = nullptr;
+    const int k = H.nlev() - 1;
+    if (o.coarse_unknowns <= 0 || k < 1) return EBG_OK;
+    const int64_t n = mg_level_unknowns(H.lev[k].xnum, H.lev[k].ynum);
+    if (n > o.coarse_unknowns) return EBG_OK;
+    const bool bc_changed = std::memcmp(key.bcd, o.bc_d, sizeof(key.bcd)) != 0;
+    int rc;
+    if (key.dense_dirty || bc_changed) {
+        if (!st.solver) {
+            rocblas_handle h = nullptr;
+            EBG_CHECK(rocblas_create_handle(&h) == rocblas_status_success, EBG_ERR_HIP,
+                      "coarse solve: rocblas_create_handle failed");
+            st.solver = h;
+        }
+        if ((rc = st.cA.ensure((size_t)n * n * sizeof(double))) || (rc = st.cW.ensure((size_t)n * n * sizeof(double))) ||
+            (rc = st.cipiv.ensure((size_t)n * sizeof(int))) ||
+            (rc = st.cinfo.ensure(sizeof(int))) || (rc = inv.ensure((size_t)n * n * sizeof(T))) ||
+            (rc = vec.ensure((size_t)2 * n * sizeof(T))))
+            return rc;
+        if ((rc = mg_coarse_setup(H, k, o.bc_d, st.cA.as<double>(), st.cW.as<double>(), st.cipiv.as<int>(),
+                                  st.cinfo.as<int>(), inv.as<T>(), st.solver, s)))
+            return rc;
+        std::memcpy(key.bcd, o.bc_d, sizeof(key.bcd));
+        key.dense_dirty = false;
+    }
+    H.coarse_inv = inv.as<T>();
+    H.coarse_b = vec.as<T>();
+    H.coarse_x = vec.as<T>() + n;
+    H.coarse_n = (int)n;
+    return EBG_OK;
+}
+
 // hierarchy of the requested precision, smoothing counts, viscosity and masks
 static int prepare_preconditioner(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, std::vector<int>& sm)
 {
@@ -641,15 +704,23 @@ static int prepare_preconditioner(ebg_ctx* ctx, StokesState& st, const ebg_solve
     if (o.precision == 1) {
         if ((rc = ensure_hierarchy(ctx, st, st.Hf, st.prnorm_f, st.key_f, o))) return rc;
         sm = mg_smoothing_counts(st.Hf->nlev(), o.finest_sweeps, o.sweep_multiplier, o.sweep_cap);
-        if (st.key_f.visc_dirty) load_viscosity(st, *st.Hf, s);
+        if (st.key_f.visc_dirty) {
+            load_viscosity(st, *st.Hf, s);
+            st.key_f.dense_dirty = true;
+        }
         st.key_f.visc_dirty = false;
         attach_masks(st, *st.Hf);
+        if ((rc = prepare_coarse(st, *st.Hf, st.key_f, st.cinv_f, st.cvec_f, o, s))) return rc;
     } else {
         if ((rc = ensure_hierarchy(ctx, st, st.Hd, st.prnorm_d, st.key_d, o))) return rc;
         sm = mg_smoothing_counts(st.Hd->nlev(), o.finest_sweeps, o.sweep_multiplier, o.sweep_cap);
-        if (st.key_d.visc_dirty) load_viscosity(st, *st.Hd, s);
+        if (st.key_d.visc_dirty) {
+            load_viscosity(st, *st.Hd, s);
+            st.key_d.dense_dirty = true;
+        }
         st.key_d.visc_dirty = false;
         attach_masks(st, *st.Hd);
+        if ((rc = prepare_coarse(st, *st.Hd, st.key_d, st.cinv_d, st.cvec_d, o, s))) return rc;
     }
     return EBG_OK;
 }
@@ -1056,9 +1127,11 @@ int ebg_stokes_solve_opts_default(ebg_solve_opts* o)
     o->max_iterations = 2000;
     o->rtol = 1e-8;
     o->nlevels = 0;
-    o->finest_sweeps = 4;
+    // with the coarsest level solved exactly, short smoothing is enough: two
+    // finest-level sweeps, doubling per level, at most eight
+    o->finest_sweeps = 2;
     o->sweep_multiplier = 2;
-    o->sweep_cap = 48;
+    o->sweep_cap = 8;
     o->theta_s = 0.9;
     o->theta_c = 1.0;
     o->use_initial_guess = 0;
@@ -1068,6 +1141,7 @@ int ebg_stokes_solve_opts_default(ebg_solve_opts* o)
     o->local_sweeps = 2;  // more sweeps per halo refresh let the tile pressures drift
     o->tile = 16;
     o->flexible = 1;
+    o->coarse_unknowns = 6000;
     return EBG_OK;
 }
 

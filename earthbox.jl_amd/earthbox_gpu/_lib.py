@@ -85,6 +85,7 @@ SIGNATURES: dict[str, tuple] = {
                                  c_double, c_double, c_double]),
     "ebg_mg3d_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ebg_mg2d_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_mg2d_direct_solve": (c_int, [c_void_p, c_int, c_void_p]),
     "ebg_stokes3d_set_system": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_double, c_double,
                                         c_double, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ebg_stokes3d_vector_length": (c_int, [c_void_p, c_void_p]),
@@ -121,7 +122,7 @@ class SolveOpts(C.Structure):
                 ("sweep_multiplier", c_int), ("sweep_cap", c_int), ("theta_s", c_double),
                 ("theta_c", c_double), ("use_initial_guess", c_int), ("learned", c_int),
                 ("bc_d", c_double * 4), ("smoother", c_int), ("local_sweeps", c_int),
-                ("tile", c_int), ("flexible", c_int)]
+                ("tile", c_int), ("flexible", c_int), ("coarse_unknowns", c_int)]
 
 
 class MGRunParams(C.Structure):

@@ -201,6 +201,11 @@ class MultigridGPU:
                                    theta_s, theta_c, relax_v, relax_p, prnorm, d),
               "ebg_mg2d_vcycles")
 
+    def direct_solve(self, level, bc_d=(1, 1, 1, 1)):
+        """Exact solve of the level's equations (gauge p(1,1) = 0) into vx, vy, pr."""
+        d = (C.c_double * 4)(*bc_d)
+        check(lib.ebg_mg2d_direct_solve(self.ctx.handle, level, d), "ebg_mg2d_direct_solve")
+
     def run(self, nvcycles, convergence_criterion, nviscosity_jumps=12.0, nvcycles_viscosity_jump=15,
             stokesscale=1.0, continscale=1.0, finest_sweeps=2, multiplier=2, cap=48, theta_s=0.9,
             theta_c=0.3, relax_v=1.0, relax_p=1.0, pressure_bc=0.0, bc_d=(1, 1, 1, 1)):

@@ -40,6 +40,7 @@ struct SolveOpts
     local_sweeps::Cint
     tile::Cint
     flexible::Cint
+    coarse_unknowns::Cint
 end
 
 struct SolveStats
@@ -164,7 +165,7 @@ function solve_system_gpu!(st::GPUState, model; mixed::Bool=true, rtol::Float64=
                     o[].theta_c, 0, 0,
                     (vc.btopx.array[2, 2], vc.bbottomx.array[2, 2], vc.blefty.array[2, 2],
                      vc.brighty.array[2, 2]), o[].smoother, o[].local_sweeps, o[].tile,
-                    o[].flexible)
+                    o[].flexible, o[].coarse_unknowns)
     stats = Ref{SolveStats}()
     rc = check(ccall((:ebg_stokes_solve, LIB), Cint,
         (Ptr{Cvoid}, Ptr{Cvoid}, Ref{SolveOpts}, Ref{SolveStats}), st.ctx, dS, o, stats),

@@ -228,6 +228,11 @@ typedef struct {
                              0 (3-D solve only): right-preconditioned GMRES storing only the
                              basis, the V-cycle applied once more per restart (half the memory,
                              so twice the restart length fits) */
+    int coarse_unknowns;  /* 2-D solve: > 0 (default 6000) ends the preconditioner hierarchy at
+                             the first level below the finest with at most this many unknowns
+                             and solves that level exactly (dense inverse of its operator, built
+                             once per system with rocSOLVER); 0: smooth every level (no
+                             reference counterpart) */
 } ebg_solve_opts;
 
 typedef struct {
@@ -317,6 +322,16 @@ int ebg_mg2d_smoother_tiled(ebg_ctx *ctx, int level, int nsweeps, int local_swee
 int ebg_mg2d_vcycles(ebg_ctx *ctx, int ncycles, int finest_sweeps, int multiplier, int cap,
                      double theta_s, double theta_c, double relax_v, double relax_p,
                      double prnorm, const double *bc_d4);
+/* Exact solve of one level's equations for its right-hand sides RX, RY, RC:
+ * the level's operator (the smoother's residual arithmetic and ghost rules
+ * with mirror factors bc_d4, host, NULL = free slip) with the continuity row
+ * of cell (1,1) replaced by p(1,1) = 0, assembled densely and inverted; vx,
+ * vy, pr receive the solution and the ghost pass is applied.  Levels up to
+ * 20000 unknowns.  No reference counterpart: it is the coarsest-level solve of
+ * the Stokes preconditioner (ebg_solve_opts.coarse_unknowns), exposed on the
+ * reference-order hierarchy so it can be checked against the reference's
+ * residuals. */
+int ebg_mg2d_direct_solve(ebg_ctx *ctx, int level, const double *bc_d4);
 /* Parameters of the reference's stand-alone V-cycle driver: VcycleParameters,
  * ViscosityScalingParameters, RelaxationParameters and ResidualScalingFactors
  * (multigrid_data/structures/MultigridStructures.jl:8-90). */

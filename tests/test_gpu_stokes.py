@@ -45,17 +45,23 @@ def test_apply_rhs_residual_unpack_bit_exact(gpu_ctx, case):
     assert np.array_equal(to_host(pr1), ep)
 
 
+# default preconditioner (coarsest level solved directly) and the all-smoothed
+# hierarchy of the reference's sweep rule
+PRECOND = [dict(), dict(coarse_unknowns=0, finest_sweeps=4, sweep_cap=48)]
+
+
+@pytest.mark.parametrize("pc", PRECOND)
 @pytest.mark.parametrize("precision", [1, 0])
 @pytest.mark.parametrize("case", [CASES[0], CASES[2], CASES[3], CASES[6]])
-def test_solve_matches_direct(gpu_ctx, case, precision):
-    print(f"stokes solve case {case} precision {precision}", flush=True)
+def test_solve_matches_direct(gpu_ctx, case, precision, pc):
+    print(f"stokes solve case {case} precision {precision} {pc}", flush=True)
     pr = random_case(**case)
     Li, Lj, Lv, R = oracle.stokes_build(pr)
     Sd = oracle.direct_solve(pr.N, Li, Lj, Lv, R)
     sol = StokesSolverGPU(gpu_ctx)
     sol.set_system(pr)
     S, st = sol.solve_system(precision=precision, rtol=1e-12, max_iterations=3000,
-                             bc_d=bc_mirror_factors(pr))
+                             bc_d=bc_mirror_factors(pr), **pc)
     print(st, flush=True)
     assert st["converged"], st
     Sg = to_host(S)
@@ -69,15 +75,16 @@ def test_solve_matches_direct(gpu_ctx, case, precision):
     assert np.linalg.norm(p - ep) <= 1e-8 * np.linalg.norm(ep)
 
 
+@pytest.mark.parametrize("pc", PRECOND)
 @pytest.mark.parametrize("precision", [1, 0])
-def test_falling_block_time_to_tolerance(gpu_ctx, precision):
+def test_falling_block_time_to_tolerance(gpu_ctx, precision, pc):
     """The 101 x 101 falling block (1e6 viscosity contrast): FGMRES reaches
     1e-8 relative residual; the forward error of that solve is reported."""
     pr = workloads.falling_block(101, 101)
     Li, Lj, Lv, R = oracle.stokes_build(pr)
     sol = StokesSolverGPU(gpu_ctx)
     sol.set_system(pr)
-    S, st = sol.solve_system(precision=precision, rtol=1e-8)
+    S, st = sol.solve_system(precision=precision, rtol=1e-8, **pc)
     print(f"falling block precision={precision}: {st}", flush=True)
     assert st["converged"] and st["relres"] <= 1e-8
     Sg = to_host(S)
@@ -297,3 +304,51 @@ def test_reference_multigrid_run_matches_oracle(gpu_ctx, eta_block, kw, hist_tol
         assert np.linalg.norm(got - exp) <= 1e-9 * np.linalg.norm(exp), k
     for k in ("etas", "etan"):
         np.testing.assert_allclose(to_host(M.get(0, k)), getattr(f, k), rtol=1e-13)
+
+
+@pytest.mark.parametrize("bc_d", [(1.0, 1.0, 1.0, 1.0), (-1.0, -1.0, -1.0, -1.0), (1.0, -1.0, -1.0, 1.0)])
+def test_direct_level_solve_zeroes_the_reference_residuals(gpu_ctx, bc_d):
+    """The dense direct solve the preconditioner uses on its coarsest level
+    solves the reference smoother's equations: after it, the oracle's ghost
+    pass and residuals (BoundaryConditions.jl, Residuals.jl restated) vanish
+    on every unknown except the continuity row of cell (1,1), which the gauge
+    p(1,1) = 0 replaces."""
+    print(f"mg2d direct level solve bc_d={bc_d}", flush=True)
+    pr = workloads.falling_block(45, 37, size=1.0e5, eta_block=1e23)
+    g = pr.grid
+    H = omg.Hierarchy(g.xnum, g.ynum, g.xsize, g.ysize, bc_d=bc_d)
+    f = H.levels[0]
+    f.etas[...] = pr.etas0
+    f.etan[...] = pr.etan0
+    H.restrict_viscosity()
+    M = MultigridGPU(gpu_ctx, g.xnum, g.ynum, g.xsize, g.ysize)
+    M.set(0, "etas", to_device(f.etas))
+    M.set(0, "etan", to_device(f.etan))
+    M.restrict_viscosity()
+    rng = np.random.default_rng(4)
+    L = omg._sigs()
+    for level in (1, 2):
+        lv = H.levels[level]
+        for name in ("RX", "RY", "RC"):
+            a = getattr(lv, name)
+            a[...] = rng.standard_normal(a.shape) * (1e-12 if name == "RC" else 1.0)
+            M.set(level, name, to_device(a))
+        M.direct_solve(level, bc_d)
+        gpu_ctx.synchronize()
+        for name in ("vx", "vy", "pr"):
+            getattr(lv, name)[...] = to_host(M.get(level, name))
+        assert lv.pr[0, 0] == 0.0
+        L.ebo_mg_set_bc(omg.C.byref(lv.cs()), *bc_d)
+        L.ebo_mg_residuals(omg.C.byref(lv.cs()))
+        ny, nx = lv.ynum, lv.xnum
+        rx = lv.rx[1:ny, 1:nx - 1]
+        ry = lv.ry[1:ny - 1, 1:nx]
+        rc = lv.rc.copy()
+        rc[0, 0] = 0.0
+        # the level operator is badly scaled (viscosity 1e21-1e23 against unit
+        # pressure; condition ~1e40): an FP64 LU solve of the same equations in
+        # numpy leaves residuals of ~1e-7 of the right-hand side; a wrong
+        # operator would leave residuals of its order
+        assert np.abs(rx).max() <= 1e-4 * np.abs(lv.RX).max(), (level, np.abs(rx).max())
+        assert np.abs(ry).max() <= 1e-4 * np.abs(lv.RY).max(), (level, np.abs(ry).max())
+        assert np.abs(rc).max() <= 1e-4 * np.abs(lv.RC).max(), (level, np.abs(rc).max())
