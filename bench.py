@@ -173,6 +173,41 @@ def bench_sinker(ctx):
         sec = time.perf_counter() - t0
     out["mg2d_97_reference_driver"] = {"seconds": sec, "cycles": n, "ms_per_cycle": 1e3 * sec / n,
                                        "converged": conv, "final_log10_residuals": [float(v) for v in hist[-1]]}
+    # the same 2-D system through the FGMRES solve (its equations are the
+    # multigrid's: free slip, no stabilization), both answers judged by the
+    # reference's grid residuals (StokesResiduals.jl:94-137)
+    import numpy as np
+    from earthbox_gpu.grid import uniform_grid
+    from earthbox_gpu.stokes import StokesSolverGPU
+    from earthbox_gpu.stokes_problem import StokesProblem, free_slip_bcs
+    g2 = uniform_grid(97, 97, 1e5, 1e5)
+    prob = StokesProblem(grid=g2, etas0=S2["etas"], etan0=S2["etan"], rho1_vy=np.zeros((97, 98), order="F"),
+                         RX1=S2["RX"], RY1=S2["RY"], RC1=S2["RC"], bcs=free_slip_bcs(g2), pressure_bc_mode=0,
+                         pressure_bc=S2["pressure_bc"], timestep=0.0, gravity_y=workloads.SINKER["gravity"],
+                         iuse_interface_stabilization=0)
+    ctx.set_grid(g2)
+    sol2 = StokesSolverGPU(ctx)
+    sol2.set_system(prob)
+    rnorm = float(np.sqrt((S2["RX"] ** 2).sum() + (S2["RY"] ** 2).sum()))
+
+    def momentum_relres(vx1, vy1, pr1):
+        rx, ry, _ = sol2.compute_stokes_residuals(vx1, vy1, pr1)
+        return float(torch.sqrt((rx ** 2).sum() + (ry ** 2).sum())) / rnorm
+
+    out["mg2d_97_reference_driver"]["momentum_relres"] = momentum_relres(M.get(0, "vx"), M.get(0, "vy"),
+                                                                         M.get(0, "pr"))
+    for name, prec in (("mixed_fp32_mg", 1), ("plain_fp64_mg", 0)):
+        sol2.solve_system(rtol=1e-8, max_iterations=3, precision=prec)   # build + warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sol2.set_system(prob)
+        S, st = sol2.solve_system(rtol=1e-8, max_iterations=2000, precision=prec)
+        torch.cuda.synchronize()
+        sec2 = time.perf_counter() - t0
+        out[f"stokes2d_97_fgmres_{name}"] = {
+            "seconds_with_setup": sec2, "iterations": st["iterations"], "relres": st["relres"],
+            "converged": st["converged"], "momentum_relres": momentum_relres(*sol2.process_stokes_solution(S)),
+            "speedup_vs_reference_driver": sec / sec2}
     return out
 
 
