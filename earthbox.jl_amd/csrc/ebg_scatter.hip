@@ -8,11 +8,14 @@
 // averages with backup fallback (BilinearAverage.jl:57-178, BackupTransportArrays.jl).
 //
 // GPU formulation: markers are binned by basic cell with a stable radix sort of
-// (cell, marker index) pairs; each grid node then gathers the markers of its
-// neighbouring cells, merging the cell lists by marker index, so every node
-// sum is accumulated in exactly the reference's serial marker order.  No
-// atomics, deterministic, bit-identical to the serial loops (asin of the
-// friction angle aside, which goes through the device math library).
+// (cell, marker index) pairs.  One pass then gathers every marker once into
+// per-cell lists laid out in 64-cell slices (entry t of 64 neighbouring cells
+// in adjacent words), holding its normalized distances and the property
+// values the requested outputs consume.  Each grid node walks the lists of
+// its neighbouring cells, merging them by marker index, so every node sum is
+// accumulated in exactly the reference's serial marker order.  No atomics,
+// deterministic, bit-identical to the serial loops (asin of the friction
+// angle aside, which goes through the device math library).
 #include "ebg_common.hpp"
 #include "ebg_device.hpp"
 
@@ -22,20 +25,36 @@
 
 namespace ebg {
 
+// Cell lists in a sliced layout: cells are taken in groups of 64 consecutive
+// cell indices (one wavefront of node lanes touches consecutive cells), and
+// group b owns rows rowbase[b] .. rowbase[b] + nrow[b] - 1, nrow[b] being the
+// largest marker count of its cells.  Entry t of cell c (its t-th marker in
+// increasing marker index) sits at (rowbase[c/64] + t) * 64 + c % 64, so the
+// lanes of a wave reading entry t of neighbouring cells read adjacent words.
+constexpr int CGROUP = 64;
+
 struct ScatterState {
     int64_t cap = 0;
     int ncell = 0;
     uint32_t *keys = nullptr, *keys_s = nullptr, *vals = nullptr, *vals_s = nullptr;
     int *cstart = nullptr, *cend = nullptr;
+    int* rowbase = nullptr;  // ngroup + 1 (exclusive scan of the group row counts)
     void* tmp = nullptr;
     size_t tmp_bytes = 0;
     double* wbuf = nullptr;  // internal weight arrays when the caller passes none
     int64_t wbuf_n = 0;
+    // per-entry columns: marker index, vy-grid column, and the FP64 slots
+    uint32_t* emark = nullptr;
+    int* exnv = nullptr;
+    double* eval = nullptr;
+    int64_t ecap = 0, evcap = 0;
+    int* h_rows = nullptr;  // pinned: total row count read back per call
     ~ScatterState()
     {
-        void* ps[] = {keys, keys_s, vals, vals_s, cstart, cend, tmp, wbuf};
+        void* ps[] = {keys, keys_s, vals, vals_s, cstart, cend, rowbase, tmp, wbuf, emark, exnv, eval};
         for (void* p : ps)
             if (p) (void)hipFree(p);
+        if (h_rows) (void)hipHostFree(h_rows);
     }
 };
 
@@ -44,6 +63,20 @@ struct AxisD {
     int n;
     double x0, inv_h;
     int uniform;
+};
+
+// FP64 slots of an entry: 0 dx, 1 dy (normalized distances from the
+// upper-left basic node of the marker's cell, MarkerGridMapping.jl:185-226),
+// 2 dx on the vy grid (MarkerGridMapping.jl:228-265), then the marker values
+// that some requested output consumes, at the slots in Layout::off (-1: not
+// stored).  Values are stored as the reference's numerator loops use them:
+// the friction angle in degrees and the reciprocal shear modulus
+// (BilinearNumerator.jl:296-388).
+enum { F_RHO, F_COH, F_FRIC, F_EFLOW, F_ETA, F_INVMU, F_SXY, F_PL, F_DIL, F_MELT, F_SXX, NFIELD };
+
+struct Layout {
+    int nslot;
+    int off[NFIELD];
 };
 
 struct ScatterArgs {
@@ -56,9 +89,15 @@ struct ScatterArgs {
     const double *xstp_b, *ystp_b, *gridx_vy, *xstp_vy;
     ebg_marker_stokes_props p;
     ebg_stokes_grid_fields f;
-    const uint32_t* vals_s;
+    const uint32_t *keys_s, *vals_s;
     const int *cstart, *cend;
+    const int* rowbase;
     uint32_t *keys, *vals;
+    uint32_t* emark;
+    int* exnv;
+    double* eval;
+    int64_t slot_stride;  // entries per FP64 slot (total rows * 64)
+    Layout L;
     double *wtnodes, *wtetas, *wtetan, *wtnodes_vy;
 };
 
@@ -102,43 +141,180 @@ __global__ void fill_int_kernel(int* a, int n, int v)
     if (t < n) a[t] = v;
 }
 
-// Merge-walk over up to NC cell lists in increasing marker index.
-template <int NC>
-struct Merger {
-    int cur[NC], end[NC];
-    uint32_t head[NC];
-    __device__ void init(const ScatterArgs& a, const int* cells)
-    {
-        for (int q = 0; q < NC; ++q) {
-            if (cells[q] >= 0) {
-                cur[q] = a.cstart[cells[q]];
-                end[q] = a.cend[cells[q]];
-            } else {
-                cur[q] = 0;
-                end[q] = 0;
-            }
-            head[q] = cur[q] < end[q] ? a.vals_s[cur[q]] : 0xffffffffu;
+// Row count of each 64-cell group (its largest cell list), written to
+// rows[b + 1] so that the inclusive scan below leaves rowbase[b] in place.
+__global__ void __launch_bounds__(256) group_rows_kernel(const int* __restrict__ cstart,
+                                                         const int* __restrict__ cend, int ncell,
+                                                         int ngroup, int* __restrict__ rows)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int cnt = c < ncell ? cend[c] - cstart[c] : 0;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) cnt = max(cnt, __shfl_xor(cnt, d, 64));
+    const int b = c / CGROUP;
+    if ((c & (CGROUP - 1)) == 0 && b < ngroup) rows[b + 1] = cnt;
+    if (c == 0) rows[0] = 0;
+}
+
+// In-place inclusive scan of rows[0..n] by one workgroup of 1024 threads.
+__global__ void __launch_bounds__(1024) scan_rows_kernel(int* __restrict__ rows, int n)
+{
+    __shared__ int part[1024];
+    const int tid = threadIdx.x;
+    const int per = (n + 1 + 1023) / 1024;
+    const int lo = tid * per, hi = min(lo + per, n + 1);
+    int sum = 0;
+    for (int k = lo; k < hi; ++k) sum += rows[k];
+    part[tid] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int v = tid >= d ? part[tid - d] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = tid ? part[tid - 1] : 0;
+    for (int k = lo; k < hi; ++k) {
+        run += rows[k];
+        rows[k] = run;
+    }
+}
+
+// Fill the sliced cell lists: lane = cell, loop over its entries in order,
+// four at a time so that the gathers of four markers are in flight together.
+// Each marker is gathered and divided once; the writes of a wave are rows of
+// 64 adjacent words.
+constexpr int EBATCH = 4;
+
+__global__ void __launch_bounds__(256) scatter_entries_kernel(ScatterArgs a)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = c / CGROUP, lane = c & (CGROUP - 1);
+    if (c >= a.ncell) return;
+    const int ks = a.cstart[c];
+    const int cnt = a.cend[c] - ks;
+    if (cnt == 0) return;
+    const int r0 = a.rowbase[b];
+    const int cj = c / (a.ynum - 1), ci = c - cj * (a.ynum - 1);
+    const double gx = a.ax.g[cj], hx = a.xstp_b[cj], gy = a.ay.g[ci], hy = a.ystp_b[ci];
+    const int* o = a.L.off;
+    const int64_t S = a.slot_stride;
+    const uint32_t* __restrict__ vs = a.vals_s + ks;
+    const double* __restrict__ X = a.x;
+    const double* __restrict__ Y = a.y;
+    for (int t0 = 0; t0 < cnt; t0 += EBATCH) {
+        uint32_t m[EBATCH];
+        double xm[EBATCH], ym[EBATCH];
+#pragma unroll
+        for (int u = 0; u < EBATCH; ++u) m[u] = vs[min(t0 + u, cnt - 1)];
+#pragma unroll
+        for (int u = 0; u < EBATCH; ++u) {
+            xm[u] = X[m[u]];
+            ym[u] = Y[m[u]];
+        }
+        double pv[NFIELD][EBATCH];
+#pragma unroll
+        for (int u = 0; u < EBATCH; ++u) {
+            const uint32_t mu_ = m[u];
+            pv[F_RHO][u] = o[F_RHO] >= 0 ? a.p.rho[mu_] : 0.0;
+            pv[F_COH][u] = o[F_COH] >= 0 ? a.p.cohesion[mu_] : 0.0;
+            pv[F_FRIC][u] = o[F_FRIC] >= 0 ? a.p.fric[mu_] : 0.0;
+            pv[F_EFLOW][u] = o[F_EFLOW] >= 0 ? a.p.eta_flow[mu_] : 0.0;
+            pv[F_ETA][u] = o[F_ETA] >= 0 ? a.p.eta[mu_] : 0.0;
+            pv[F_INVMU][u] = (o[F_INVMU] >= 0 && a.p.mat_mu) ? a.p.mat_mu[a.p.matid[mu_] - 1] : 1.0;
+            pv[F_SXY][u] = o[F_SXY] >= 0 ? a.p.sxy[mu_] : 0.0;
+            pv[F_PL][u] = o[F_PL] >= 0 ? (double)a.p.pfailure[mu_] : 0.0;
+            pv[F_DIL][u] = o[F_DIL] >= 0 ? (double)a.p.dilatation[mu_] : 0.0;
+            pv[F_MELT][u] = o[F_MELT] >= 0 ? a.p.meltfrac[mu_] : 0.0;
+            pv[F_SXX][u] = o[F_SXX] >= 0 ? a.p.sxx[mu_] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < EBATCH; ++u) {
+            if (t0 + u >= cnt) break;
+            const int64_t e = (int64_t)(r0 + t0 + u) * CGROUP + lane;
+            a.emark[e] = m[u];
+            double* v = a.eval + e;
+            v[0] = (xm[u] - gx) / hx;
+            v[S] = (ym[u] - gy) / hy;
+            const int xnv = left_node_index(xm[u], a.avx.g, a.avx.n, a.avx.x0, a.avx.inv_h, a.avx.uniform);
+            a.exnv[e] = xnv;
+            v[2 * S] = (xm[u] - a.gridx_vy[xnv - 1]) / a.xstp_vy[xnv - 1];
+            if (o[F_RHO] >= 0) v[o[F_RHO] * S] = pv[F_RHO][u];
+            if (o[F_COH] >= 0) v[o[F_COH] * S] = pv[F_COH][u];
+            if (o[F_FRIC] >= 0) v[o[F_FRIC] * S] = asin(pv[F_FRIC][u]) * 180.0 / M_PI;
+            if (o[F_EFLOW] >= 0) v[o[F_EFLOW] * S] = pv[F_EFLOW][u];
+            if (o[F_ETA] >= 0) v[o[F_ETA] * S] = pv[F_ETA][u];
+            if (o[F_INVMU] >= 0) v[o[F_INVMU] * S] = 1 / pv[F_INVMU][u];
+            if (o[F_SXY] >= 0) v[o[F_SXY] * S] = pv[F_SXY][u];
+            if (o[F_PL] >= 0) v[o[F_PL] * S] = pv[F_PL][u];
+            if (o[F_DIL] >= 0) v[o[F_DIL] * S] = pv[F_DIL][u];
+            if (o[F_MELT] >= 0) v[o[F_MELT] * S] = pv[F_MELT][u];
+            if (o[F_SXX] >= 0) v[o[F_SXX] * S] = pv[F_SXX][u];
         }
     }
-    // returns the list index of the next marker or -1 when all lists are empty
-    __device__ int next(const ScatterArgs& a, uint32_t* m)
+}
+
+// First entry and one-past-last entry of cell c's sliced list (stride 64).
+__device__ __forceinline__ void cell_list(const ScatterArgs& a, int c, int64_t* first, int64_t* last)
+{
+    if (c < 0) {
+        *first = *last = 0;
+        return;
+    }
+    *first = (int64_t)a.rowbase[c / CGROUP] * CGROUP + (c & (CGROUP - 1));
+    *last = *first + (int64_t)(a.cend[c] - a.cstart[c]) * CGROUP;
+}
+
+// Merge-walk over K cell lists (each in increasing marker index) in
+// increasing marker index: the serial loops' order restricted to the node's
+// markers.  Unrolled over K with constant indices, so the state stays in
+// registers.
+template <int K>
+struct Merge {
+    int64_t cur[K], end[K];
+    uint32_t head[K], nxt[K];  // marker index at cur and at cur + 64 (prefetched)
+    __device__ __forceinline__ void init(const ScatterArgs& a, const int (&cells)[K])
     {
-        int best = -1;
-        uint32_t bv = 0xffffffffu;
-        for (int q = 0; q < NC; ++q)
-            if (head[q] < bv) {
-                bv = head[q];
-                best = q;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            cell_list(a, cells[q], &cur[q], &end[q]);
+            head[q] = cur[q] < end[q] ? a.emark[cur[q]] : 0xffffffffu;
+            nxt[q] = cur[q] + CGROUP < end[q] ? a.emark[cur[q] + CGROUP] : 0xffffffffu;
+        }
+    }
+    // entry of the next marker and its list, or false when all are done
+    __device__ __forceinline__ bool next(const ScatterArgs& a, int64_t* pos, int* list)
+    {
+        uint32_t mn = head[0];
+        int q = 0;
+#pragma unroll
+        for (int t = 1; t < K; ++t)
+            if (head[t] < mn) {
+                mn = head[t];
+                q = t;
             }
-        if (best < 0) return -1;
-        *m = bv;
-        const int c = ++cur[best];
-        head[best] = c < end[best] ? a.vals_s[c] : 0xffffffffu;
-        return best;
+        if (mn == 0xffffffffu) return false;
+        int64_t k = cur[0], e = end[0];
+#pragma unroll
+        for (int t = 1; t < K; ++t) {
+            k = q == t ? cur[t] : k;
+            e = q == t ? end[t] : e;
+        }
+        // the list's new head was prefetched; fetch the one after it
+        const int64_t kf = k + 2 * CGROUP;
+        const uint32_t hf = kf < e ? a.emark[kf] : 0xffffffffu;
+#pragma unroll
+        for (int t = 0; t < K; ++t)
+            if (q == t) {
+                cur[t] = k + CGROUP;
+                head[t] = nxt[t];
+                nxt[t] = hf;
+            }
+        *pos = k;
+        *list = q;
+        return true;
     }
 };
-
-__device__ __forceinline__ double getp(const double* p, uint32_t m) { return p ? p[m] : 0.0; }
 
 // Basic nodes: inclusive (rho, wtnodes) and exclusive (wtetas and 7 fields).
 __global__ void __launch_bounds__(256) scatter_basic_kernel(ScatterArgs a)
@@ -147,42 +323,41 @@ __global__ void __launch_bounds__(256) scatter_basic_kernel(ScatterArgs a)
     const int ny = a.ynum, nx = a.xnum;
     if (t >= ny * nx) return;
     const int i = t % ny + 1, j = t / ny + 1;
-    // cells whose marker feeds node (i,j): role UL (i,j), LL (i-1,j), UR (i,j-1), LR (i-1,j-1)
-    int cells[4];
+    // cells whose markers feed node (i,j), by the node's role in the cell:
+    // 0 upper-left of (i,j), 1 lower-left of (i-1,j), 2 upper-right of (i,j-1),
+    // 3 lower-right of (i-1,j-1)
     const int ci[4] = {i, i - 1, i, i - 1}, cj[4] = {j, j, j - 1, j - 1};
+    int cells[4];
+#pragma unroll
     for (int q = 0; q < 4; ++q)
         cells[q] = (ci[q] >= 1 && ci[q] <= ny - 1 && cj[q] >= 1 && cj[q] <= nx - 1)
                        ? (cj[q] - 1) * (ny - 1) + (ci[q] - 1)
                        : -1;
-    Merger<4> mg;
+    Merge<4> mg;
     mg.init(a, cells);
+    const int* o = a.L.off;
+    const int64_t S = a.slot_stride;
     double w_in = 0.0, w_ex = 0.0, s_rho = 0.0, s_coh = 0.0, s_fric = 0.0, s_ef = 0.0, s_eta = 0.0,
            s_mu = 0.0, s_sxy = 0.0, s_pl = 0.0;
-    uint32_t m;
+    int64_t e;
     int q;
-    while ((q = mg.next(a, &m)) >= 0) {
-        const int yn = ci[q], xn = cj[q];
-        const double dx = (a.x[m] - a.ax.g[xn - 1]) / a.xstp_b[xn - 1];
-        const double dy = (a.y[m] - a.ay.g[yn - 1]) / a.ystp_b[yn - 1];
-        double w;
-        bool ex;
-        if (q == 0) { w = (1.0 - dx) * (1.0 - dy); ex = (dx <= 0.5 && dy <= 0.5); }
-        else if (q == 1) { w = (1.0 - dx) * dy; ex = (dx <= 0.5 && 0.5 <= dy); }
-        else if (q == 2) { w = dx * (1.0 - dy); ex = (dx >= 0.5 && 0.5 >= dy); }
-        else { w = dx * dy; ex = (dx >= 0.5 && dy >= 0.5); }
+    while (mg.next(a, &e, &q)) {
+        const double* v = a.eval + e;
+        const double dx = v[0], dy = v[S];
+        // WeightFuncs.jl:67-140: (1-dx)(1-dy), (1-dx)dy, dx(1-dy), dx dy by role
+        const double w = ((q & 2) ? dx : (1.0 - dx)) * ((q & 1) ? dy : (1.0 - dy));
+        const bool ex = ((q & 2) ? (dx >= 0.5) : (dx <= 0.5)) && ((q & 1) ? (dy >= 0.5) : (dy <= 0.5));
         w_in += w * 1.0 * 1.0;
-        s_rho += w * getp(a.p.rho, m) * 1.0;
+        if (o[F_RHO] >= 0) s_rho += w * v[o[F_RHO] * S] * 1.0;
         if (ex) {
             w_ex += w * 1.0 * 1.0;
-            s_coh += w * getp(a.p.cohesion, m) * 1.0;
-            const double fr = a.p.fric ? asin(a.p.fric[m]) * 180.0 / M_PI : 0.0;
-            s_fric += w * fr * 1.0;
-            s_ef += w * getp(a.p.eta_flow, m) * 1.0;
-            s_eta += w * getp(a.p.eta, m) * 1.0;
-            const double mu = a.p.mat_mu ? a.p.mat_mu[a.p.matid[m] - 1] : 1.0;
-            s_mu += w * (1 / mu) * 1.0;
-            s_sxy += w * getp(a.p.sxy, m) * 1.0;
-            s_pl += w * (a.p.pfailure ? (double)a.p.pfailure[m] : 0.0) * 1.0;
+            if (o[F_COH] >= 0) s_coh += w * v[o[F_COH] * S] * 1.0;
+            if (o[F_FRIC] >= 0) s_fric += w * v[o[F_FRIC] * S] * 1.0;
+            if (o[F_EFLOW] >= 0) s_ef += w * v[o[F_EFLOW] * S] * 1.0;
+            if (o[F_ETA] >= 0) s_eta += w * v[o[F_ETA] * S] * 1.0;
+            if (o[F_INVMU] >= 0) s_mu += w * v[o[F_INVMU] * S] * 1.0;
+            if (o[F_SXY] >= 0) s_sxy += w * v[o[F_SXY] * S] * 1.0;
+            if (o[F_PL] >= 0) s_pl += w * v[o[F_PL] * S] * 1.0;
         }
     }
     a.wtnodes[t] = w_in;
@@ -205,22 +380,22 @@ __global__ void __launch_bounds__(256) scatter_pressure_kernel(ScatterArgs a)
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int ny = a.ynum, nx = a.xnum;
     if (c >= (ny - 1) * (nx - 1)) return;
-    const int i = c % (ny - 1) + 1, j = c / (ny - 1) + 1;
+    const int* o = a.L.off;
+    const int64_t S = a.slot_stride;
     double w_c = 0.0, s_eta = 0.0, s_mu = 0.0, s_pl = 0.0, s_dil = 0.0, s_melt = 0.0, s_sxx = 0.0;
-    const double gx = a.ax.g[j - 1], hx = a.xstp_b[j - 1], gy = a.ay.g[i - 1], hy = a.ystp_b[i - 1];
-    for (int k = a.cstart[c]; k < a.cend[c]; ++k) {
-        const uint32_t m = a.vals_s[k];
-        const double dx = (a.x[m] - gx) / hx;
-        const double dy = (a.y[m] - gy) / hy;
+    int64_t e, ee;
+    cell_list(a, c, &e, &ee);
+    for (; e < ee; e += CGROUP) {
+        const double* v = a.eval + e;
+        const double dx = v[0], dy = v[S];
         const double w = (1.0 - fabs(0.5 - dx)) * (1.0 - fabs(0.5 - dy));
         w_c += w * 1.0 * 1.0;
-        s_eta += w * getp(a.p.eta, m) * 1.0;
-        const double mu = a.p.mat_mu ? a.p.mat_mu[a.p.matid[m] - 1] : 1.0;
-        s_mu += w * (1 / mu) * 1.0;
-        s_pl += w * (a.p.pfailure ? (double)a.p.pfailure[m] : 0.0) * 1.0;
-        s_dil += w * (a.p.dilatation ? (double)a.p.dilatation[m] : 0.0) * 1.0;
-        s_melt += w * getp(a.p.meltfrac, m) * 1.0;
-        s_sxx += w * getp(a.p.sxx, m) * 1.0;
+        if (o[F_ETA] >= 0) s_eta += w * v[o[F_ETA] * S] * 1.0;
+        if (o[F_INVMU] >= 0) s_mu += w * v[o[F_INVMU] * S] * 1.0;
+        if (o[F_PL] >= 0) s_pl += w * v[o[F_PL] * S] * 1.0;
+        if (o[F_DIL] >= 0) s_dil += w * v[o[F_DIL] * S] * 1.0;
+        if (o[F_MELT] >= 0) s_melt += w * v[o[F_MELT] * S] * 1.0;
+        if (o[F_SXX] >= 0) s_sxx += w * v[o[F_SXX] * S] * 1.0;
     }
     a.wtetan[c] = w_c;
     auto ar = [&](double* out, double s) { if (out) out[c] = (w_c != 0) ? s / w_c : out[c]; };
@@ -233,39 +408,36 @@ __global__ void __launch_bounds__(256) scatter_pressure_kernel(ScatterArgs a)
 }
 
 // vy nodes: inclusive rule on the vy grid (x remapped on gridx_vy, y basic).
+// Node (i,jv) takes the markers of basic cells rows i-1..i, columns jv-2..jv
+// whose vy-grid column is jv-1 or jv.
 __global__ void __launch_bounds__(256) scatter_vy_kernel(ScatterArgs a)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int ny = a.ynum, nx = a.xnum;
     if (t >= ny * (nx + 1)) return;
     const int i = t % ny + 1, jv = t / ny + 1;
-    int cells[6], cyi[6];
-    int q = 0;
-    for (int dj = -2; dj <= 0; ++dj)
-        for (int di = -1; di <= 0; ++di) {
-            const int ci = i + di, cj = jv + dj;
-            cyi[q] = ci;
-            cells[q++] = (ci >= 1 && ci <= ny - 1 && cj >= 1 && cj <= nx - 1) ? (cj - 1) * (ny - 1) + (ci - 1) : -1;
-        }
-    Merger<6> mg;
+    int cells[6];
+#pragma unroll
+    for (int l = 0; l < 6; ++l) {
+        const int ci = i - 1 + (l & 1), cj = jv - 2 + (l >> 1);
+        cells[l] = (ci >= 1 && ci <= ny - 1 && cj >= 1 && cj <= nx - 1) ? (cj - 1) * (ny - 1) + (ci - 1) : -1;
+    }
+    Merge<6> mg;
     mg.init(a, cells);
+    const int o_rho = a.L.off[F_RHO];
+    const int64_t S = a.slot_stride;
     double w_s = 0.0, s_rho = 0.0;
-    uint32_t m;
+    int64_t e;
     int l;
-    while ((l = mg.next(a, &m)) >= 0) {
-        const double xm = a.x[m], ym = a.y[m];
-        const int xnv = left_node_index(xm, a.avx.g, a.avx.n, a.avx.x0, a.avx.inv_h, a.avx.uniform);
+    while (mg.next(a, &e, &l)) {
+        const int xnv = a.exnv[e];
         if (xnv != jv && xnv != jv - 1) continue;
-        const int yn = cyi[l];
-        const double dx = (xm - a.gridx_vy[xnv - 1]) / a.xstp_vy[xnv - 1];
-        const double dy = (ym - a.ay.g[yn - 1]) / a.ystp_b[yn - 1];
-        double w;
-        if (yn == i && xnv == jv) w = (1.0 - dx) * (1.0 - dy);
-        else if (yn == i - 1 && xnv == jv) w = (1.0 - dx) * dy;
-        else if (yn == i && xnv == jv - 1) w = dx * (1.0 - dy);
-        else w = dx * dy;
+        const double* v = a.eval + e;
+        const double dx = v[2 * S], dy = v[S];
+        // odd l: the marker's cell row is i, even l: i-1
+        const double w = ((xnv == jv) ? (1.0 - dx) : dx) * ((l & 1) ? (1.0 - dy) : dy);
         w_s += w * 1.0 * 1.0;
-        s_rho += w * getp(a.p.rho, m) * 1.0;
+        if (o_rho >= 0) s_rho += w * v[o_rho * S] * 1.0;
     }
     a.wtnodes_vy[t] = w_s;
     if (a.f.rho1_vy) a.f.rho1_vy[t] = (w_s != 0) ? s_rho / w_s : a.f.rho1_vy[t];
@@ -333,6 +505,7 @@ int ebg_interpolate_marker_stokes_parameters_to_grid(ebg_ctx* ctx, const ebg_mar
         std::swap(S.vals_s, fresh.vals_s);
         std::swap(S.cstart, fresh.cstart);
         std::swap(S.cend, fresh.cend);
+        std::swap(S.rowbase, fresh.rowbase);
         std::swap(S.tmp, fresh.tmp);
         S.tmp_bytes = 0;
         const size_t cap = (size_t)std::max<int64_t>(n, 1);
@@ -342,6 +515,7 @@ int ebg_interpolate_marker_stokes_parameters_to_grid(ebg_ctx* ctx, const ebg_mar
         EBG_HIP(hipMalloc(&S.vals_s, cap * 4));
         EBG_HIP(hipMalloc(&S.cstart, (size_t)ncell * 4));
         EBG_HIP(hipMalloc(&S.cend, (size_t)ncell * 4));
+        EBG_HIP(hipMalloc(&S.rowbase, (size_t)((ncell + CGROUP - 1) / CGROUP + 1) * 4));
         S.cap = (int64_t)cap;
         S.ncell = ncell;
     }
@@ -384,6 +558,7 @@ int ebg_interpolate_marker_stokes_parameters_to_grid(ebg_ctx* ctx, const ebg_mar
     a.f = *fields;
     a.keys = S.keys;
     a.vals = S.vals;
+    a.keys_s = S.keys_s;
     a.vals_s = S.vals_s;
     a.cstart = S.cstart;
     a.cend = S.cend;
@@ -391,6 +566,26 @@ int ebg_interpolate_marker_stokes_parameters_to_grid(ebg_ctx* ctx, const ebg_mar
     a.wtetas = wts;
     a.wtetan = wte;
     a.wtnodes_vy = wtv;
+    // entry slots: the marker values some requested output consumes
+    {
+        const ebg_stokes_grid_fields& f = *fields;
+        const bool want[NFIELD] = {
+            props->rho && (f.rho1 || f.rho1_vy),
+            props->cohesion && f.cohesion_grid,
+            props->fric && f.fric_degrees_grid,
+            props->eta_flow && f.eta_flow,
+            props->eta && (f.etas1 || f.etan1),
+            f.mus1 || f.mun1,  // 1/mu is 1 without a material table
+            props->sxy && f.sxy1,
+            props->pfailure && (f.plastics || f.plasticn),
+            props->dilatation && f.dilatation_grid,
+            props->meltfrac && f.extractable_meltfrac_grid,
+            props->sxx && f.sxx1};
+        int slot = 3;
+        for (int q = 0; q < NFIELD; ++q) a.L.off[q] = want[q] ? slot++ : -1;
+        a.L.nslot = slot;
+    }
+    const int ngroup = (ncell + CGROUP - 1) / CGROUP;
     hipLaunchKernelGGL(fill_int_kernel, dim3(nbk(ncell)), dim3(256), 0, s, S.cstart, ncell, 0);
     hipLaunchKernelGGL(fill_int_kernel, dim3(nbk(ncell)), dim3(256), 0, s, S.cend, ncell, 0);
     if (n > 0) {
@@ -411,6 +606,38 @@ int ebg_interpolate_marker_stokes_parameters_to_grid(ebg_ctx* ctx, const ebg_mar
         hipLaunchKernelGGL(cell_ranges_kernel, dim3(nbk(n)), dim3(256), 0, s, S.keys_s, n, ncell,
                            S.cstart, S.cend);
     }
+    // rows of each 64-cell group, their offsets, and the total for sizing
+    hipLaunchKernelGGL(group_rows_kernel, dim3(nbk((int64_t)ngroup * CGROUP)), dim3(256), 0, s, S.cstart,
+                       S.cend, ncell, ngroup, S.rowbase);
+    hipLaunchKernelGGL(scan_rows_kernel, dim3(1), dim3(1024), 0, s, S.rowbase, ngroup);
+    if (!S.h_rows) EBG_HIP(hipHostMalloc((void**)&S.h_rows, sizeof(int)));
+    EBG_HIP(hipMemcpyAsync(S.h_rows, S.rowbase + ngroup, sizeof(int), hipMemcpyDeviceToHost, s));
+    EBG_HIP(hipStreamSynchronize(s));
+    const int64_t nent = (int64_t)*S.h_rows * CGROUP;
+    if (S.ecap < nent) {
+        if (S.emark) (void)hipFree(S.emark);
+        if (S.exnv) (void)hipFree(S.exnv);
+        S.emark = nullptr;
+        S.exnv = nullptr;
+        S.ecap = 0;
+        EBG_HIP(hipMalloc(&S.emark, (size_t)nent * 4));
+        EBG_HIP(hipMalloc(&S.exnv, (size_t)nent * 4));
+        S.ecap = nent;
+    }
+    if (S.evcap < nent * a.L.nslot) {
+        if (S.eval) (void)hipFree(S.eval);
+        S.eval = nullptr;
+        S.evcap = 0;
+        EBG_HIP(hipMalloc(&S.eval, (size_t)nent * a.L.nslot * 8));
+        S.evcap = nent * a.L.nslot;
+    }
+    a.rowbase = S.rowbase;
+    a.emark = S.emark;
+    a.exnv = S.exnv;
+    a.eval = S.eval;
+    a.slot_stride = nent;
+    if (nent > 0)
+        hipLaunchKernelGGL(scatter_entries_kernel, dim3(nbk((int64_t)ngroup * CGROUP)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(scatter_basic_kernel, dim3(nbk(nb)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(scatter_pressure_kernel, dim3(nbk(np)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(scatter_vy_kernel, dim3(nbk(nvy)), dim3(256), 0, s, a);
