@@ -276,6 +276,7 @@ def bench_stokes(ctx, nx, ny, max_iter):
         el = time.perf_counter() - t0
         out[name] = {"seconds": el, "iterations": st["iterations"], "relres": st["relres"],
                      "converged": st["converged"], "learned_accepted": st["learned_accepted"],
+                     "learned_evaluated": st["learned_evaluated"],
                      "setup_seconds": setup, "seconds_with_setup": el + setup}
     p, m, lm = out["plain_fp64_mg"], out["mixed_fp32_mg"], out["learned_mixed"]
     if p["converged"] and m["converged"]:

@@ -159,6 +159,11 @@ int mg_build(MGHierarchy<T>& H, int xnum, int ynum, double xsize, double ysize, 
         if ((rc = dev_zeros(H, (ny + 1) * nx, &l.vx))) return rc;
         if ((rc = dev_zeros(H, ny * (nx + 1), &l.vy))) return rc;
         if ((rc = dev_zeros(H, (ny - 1) * (nx - 1), &l.pr))) return rc;
+        if (k > 0) {
+            if ((rc = dev_zeros(H, (ny + 1) * nx, &l.vx_alt))) return rc;
+            if ((rc = dev_zeros(H, ny * (nx + 1), &l.vy_alt))) return rc;
+            if ((rc = dev_zeros(H, (ny - 1) * (nx - 1), &l.pr_alt))) return rc;
+        }
         if ((rc = dev_zeros(H, (ny + 1) * nx, &l.RX))) return rc;
         if ((rc = dev_zeros(H, ny * (nx + 1), &l.RY))) return rc;
         if ((rc = dev_zeros(H, (ny - 1) * (nx - 1), &l.RC))) return rc;
@@ -303,23 +308,13 @@ int mg_plan_tiles(MGHierarchy<T>& H, int tile, bool allow_single, bool fast)
     for (int k = 0; k < H.nlev(); ++k) {
         const MGLevelDev<T>& l = H.lev[k];
         typename MGHierarchy<T>::TilePlan p;
-        const size_t whole = tile_lds_bytes<T>(l.ynum + 1, l.xnum + 1);
-        if (allow_single && whole <= (size_t)max_lds) {
-            p.ti = l.ynum + 1;
-            p.tj = l.xnum + 1;
-            p.nti = p.ntj = 1;
-            p.colours = 1;
-            p.single = true;
-            p.lds = whole;
-        } else {
-            p.ti = p.tj = tile;
-            p.nti = (l.ynum + 1 + tile - 1) / tile;
-            p.ntj = (l.xnum + 1 + tile - 1) / tile;
-            p.colours = 4;
-            p.single = false;
-            p.lds = tile_lds_bytes<T>(tile, tile);
-        }
-        need = std::max(need, p.lds);
+        p.whole_lds = tile_lds_bytes<T>(l.ynum + 1, l.xnum + 1);
+        p.whole_ok = allow_single && p.whole_lds <= (size_t)max_lds;
+        p.ti = p.tj = tile;
+        p.nti = (l.ynum + 1 + tile - 1) / tile;
+        p.ntj = (l.xnum + 1 + tile - 1) / tile;
+        p.lds = tile_lds_bytes<T>(tile, tile);
+        need = std::max(need, p.whole_ok ? std::max(p.lds, p.whole_lds) : p.lds);
         H.tiles.push_back(p);
     }
     H.fast_div = fast;
@@ -336,36 +331,51 @@ void mg_smoother_tiled(MGHierarchy<T>& H, int k, int nsweeps, int local_sweeps, 
                        T theta_c, T prnorm, const T* prnorm_dev, const T* bc_d, hipStream_t s)
 {
     const bool normalize = (k > 0) || H.normalize_finest;
-    const MGLevelDev<T>& l = H.lev[k];
+    MGLevelDev<T>& l = H.lev[k];
     const auto& p = H.tiles[k];
-    const TileSpec ts{p.ti, p.tj, p.nti, p.ntj, p.colours};
+    const TileSpec ts{p.ti, p.tj, p.nti, p.ntj, 4};
     const int np = (l.ynum - 1) * (l.xnum - 1);
-    if (p.single) {
-        const TileSmoothArgs a{0, nsweeps, 1, normalize ? 1 : 0};
+    auto launch = [&](unsigned nblocks, unsigned nthreads, size_t lds, const TileSpec& spec,
+                      const TileSmoothArgs& a, T* ovx, T* ovy, T* opr) {
         if (H.fast_div)
-            hipLaunchKernelGGL((mg_tile_smooth_kernel<T, true>), dim3(1), dim3(1024), p.lds, s, l, ts, a,
-                               theta_s, theta_c, bc_d[0], bc_d[1], bc_d[2], bc_d[3], prnorm, prnorm_dev);
+            hipLaunchKernelGGL((mg_tile_smooth_kernel<T, true>), dim3(nblocks), dim3(nthreads), lds, s, l,
+                               spec, a, theta_s, theta_c, bc_d[0], bc_d[1], bc_d[2], bc_d[3], prnorm,
+                               prnorm_dev, ovx, ovy, opr);
         else
-            hipLaunchKernelGGL((mg_tile_smooth_kernel<T, false>), dim3(1), dim3(1024), p.lds, s, l, ts, a,
-                               theta_s, theta_c, bc_d[0], bc_d[1], bc_d[2], bc_d[3], prnorm, prnorm_dev);
+            hipLaunchKernelGGL((mg_tile_smooth_kernel<T, false>), dim3(nblocks), dim3(nthreads), lds, s, l,
+                               spec, a, theta_s, theta_c, bc_d[0], bc_d[1], bc_d[2], bc_d[3], prnorm,
+                               prnorm_dev, ovx, ovy, opr);
+    };
+    const bool additive = H.additive_tiles && k > 0 && l.vx_alt;
+    // one workgroup for the whole level (all sweeps, gauge shift and residuals
+    // in one launch) where the level fits its LDS; with additive tiles only
+    // for levels of at most 2 x 2 tiles, above which the all-tile launches
+    // are quicker than one workgroup's serial sweeps
+    if (p.whole_ok && (!additive || p.nti * p.ntj <= 4)) {
+        const TileSpec whole{l.ynum + 1, l.xnum + 1, 1, 1, 1};
+        launch(1, 1024, p.whole_lds, whole, TileSmoothArgs{0, nsweeps, 1, normalize ? 1 : 0}, l.vx, l.vy,
+               l.pr);
         return;
     }
     const int nthreads = std::min(1024, ((p.ti * p.tj + 63) / 64) * 64);
     for (int done = 0; done < nsweeps; done += local_sweeps) {
         const int m = std::min(local_sweeps, nsweeps - done);
+        if (additive) {
+            // every tile in one launch, reading the previous iterate (halos
+            // included) and writing the other buffer: deterministic
+            const TileSpec all{p.ti, p.tj, p.nti, p.ntj, 1};
+            launch((unsigned)(p.nti * p.ntj), nthreads, p.lds, all, TileSmoothArgs{0, m, 0, 0}, l.vx_alt,
+                   l.vy_alt, l.pr_alt);
+            std::swap(l.vx, l.vx_alt);
+            std::swap(l.vy, l.vy_alt);
+            std::swap(l.pr, l.pr_alt);
+            continue;
+        }
         for (int c = 0; c < 4; ++c) {
             const int ci = c & 1, cj = c >> 1;
             const int ni = (p.nti - ci + 1) / 2, nj = (p.ntj - cj + 1) / 2;
             if (ni <= 0 || nj <= 0) continue;
-            const TileSmoothArgs a{c, m, 0, 0};
-            if (H.fast_div)
-                hipLaunchKernelGGL((mg_tile_smooth_kernel<T, true>), dim3(ni * nj), dim3(nthreads), p.lds,
-                                   s, l, ts, a, theta_s, theta_c, bc_d[0], bc_d[1], bc_d[2], bc_d[3],
-                                   prnorm, prnorm_dev);
-            else
-                hipLaunchKernelGGL((mg_tile_smooth_kernel<T, false>), dim3(ni * nj), dim3(nthreads), p.lds,
-                                   s, l, ts, a, theta_s, theta_c, bc_d[0], bc_d[1], bc_d[2], bc_d[3],
-                                   prnorm, prnorm_dev);
+            launch((unsigned)(ni * nj), nthreads, p.lds, ts, TileSmoothArgs{c, m, 0, 0}, l.vx, l.vy, l.pr);
         }
     }
     if (normalize) {

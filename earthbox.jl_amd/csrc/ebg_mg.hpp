@@ -28,6 +28,9 @@ struct MGLevelDev {
     const T *xstp_b, *ystp_b, *xstp_vy, *ystp_vx;
     T *etas, *etan;
     T *vx, *vy, *pr;
+    // second solution buffers (levels below the finest): the additive tiled
+    // smoother writes every tile's update here, then the two are swapped
+    T *vx_alt = nullptr, *vy_alt = nullptr, *pr_alt = nullptr;
     T *RX, *RY, *RC;
     T *rx, *ry, *rc;
     // optional fixed-node masks (finest level of the Stokes preconditioner):
@@ -508,12 +511,14 @@ struct MGHierarchy {
     bool normalize_finest = true;       // pressure gauge p[1,1] = prnorm on level 0
     // LDS-tiled smoother plan per level (mg_plan_tiles)
     struct TilePlan {
-        int ti, tj, nti, ntj, colours;
-        size_t lds;
-        bool single;
+        int ti, tj, nti, ntj;  // the regular tiling of the level
+        size_t lds;            // LDS bytes of one regular tile
+        bool whole_ok;         // the whole level fits one workgroup's LDS ...
+        size_t whole_lds;      // ... with this many bytes
     };
     std::vector<TilePlan> tiles;
     bool fast_div = false;              // tiled smoother: FP32 hardware-reciprocal division
+    bool additive_tiles = false;        // tiled smoother below the finest level: all tiles per launch
     // optional direct solve of the coarsest level (mg_coarse_setup): the
     // row-major inverse of its operator and two vectors of coarse_n entries;
     // the memory belongs to the caller

@@ -13,7 +13,10 @@
 // Halo entries stay at their values from the start of the launch, so a
 // multi-tile launch is a Schwarz (block Gauss-Seidel) smoother.  Tiles are
 // launched in four colours ((bi & 1) + 2 (bj & 1)), so no tile reads an entry
-// that a concurrently running tile writes: results are deterministic.  When
+// that a concurrently running tile writes: results are deterministic.  The
+// additive variant launches every tile at once and writes a second buffer,
+// so all tiles read the previous iterate (block Jacobi between tiles, four
+// times fewer launches; also deterministic).  When
 // one tile covers the whole level the sweeps equal the global phased sweeps,
 // and the kernel also applies the pressure gauge shift and writes the
 // residuals, replacing four launches per smoothing call.
@@ -111,7 +114,8 @@ __global__ void __launch_bounds__(1024) mg_tile_smooth_kernel(MGLevelDev<T> l, T
                                                               TileSmoothArgs a, T theta_s,
                                                               T theta_c, T d_top, T d_bottom,
                                                               T d_left, T d_right, T prnorm,
-                                                              const T* prnorm_dev)
+                                                              const T* prnorm_dev, T* ovx, T* ovy,
+                                                              T* opr)
 {
     extern __shared__ __align__(16) unsigned char tile_smem[];
     T* sm = reinterpret_cast<T*>(tile_smem);
@@ -270,12 +274,13 @@ __global__ void __launch_bounds__(1024) mg_tile_smooth_kernel(MGLevelDev<T> l, T
             }
         }
     }
-    // ---- write back the owned velocities and pressures
+    // ---- write back the owned velocities and pressures (to the level's own
+    // arrays, or to its second buffers in an all-tiles launch)
     for (int e = threadIdx.x; e < nown; e += nt) {
         const int i = I0 + e % ts.ti, j = J0 + e / ts.ti;
-        if (tile_in_bounds(TF_VX, i, j, xnum, ynum)) MG_VX(l, i, j) = v.vx(i, j);
-        if (tile_in_bounds(TF_VY, i, j, xnum, ynum)) MG_VY(l, i, j) = v.vy(i, j);
-        if (tile_in_bounds(TF_PR, i, j, xnum, ynum)) MG_PR(l, i, j) = v.pr(i, j);
+        if (tile_in_bounds(TF_VX, i, j, xnum, ynum)) ovx[(size_t)(j - 1) * (ynum + 1) + (i - 1)] = v.vx(i, j);
+        if (tile_in_bounds(TF_VY, i, j, xnum, ynum)) ovy[(size_t)(j - 1) * ynum + (i - 1)] = v.vy(i, j);
+        if (tile_in_bounds(TF_PR, i, j, xnum, ynum)) opr[(size_t)(j - 1) * (ynum - 1) + (i - 1)] = v.pr(i, j);
     }
 }
 

@@ -175,7 +175,7 @@ int ebg_mg2d_smoother_tiled(ebg_ctx* ctx, int level, int nsweeps, int local_swee
               "ebg_mg2d_smoother_tiled: tile must be 0 (whole level) or in [4, 32]");
     int rc = mg_plan_tiles(m->H, tile == 0 ? 16 : tile, tile == 0, false);
     if (rc) return rc;
-    EBG_CHECK(tile != 0 || m->H.tiles[level].single, EBG_ERR_ARG,
+    EBG_CHECK(tile != 0 || m->H.tiles[level].whole_ok, EBG_ERR_ARG,
               "ebg_mg2d_smoother_tiled: level does not fit one LDS tile");
     const double one[4] = {1.0, 1.0, 1.0, 1.0};
     mg_smoother_tiled(m->H, level, nsweeps, local_sweeps, theta_s, theta_c, prnorm,

@@ -593,6 +593,7 @@ static int ensure_hierarchy(ebg_ctx* ctx, StokesState& st, std::unique_ptr<MGHie
         if (rc) return rc;
         key.tile = o.tile;
     }
+    H->additive_tiles = o.additive_tiles != 0;
     return EBG_OK;
 }
 
@@ -816,7 +817,7 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
     EBG_HIP(hipStreamSynchronize(s));
     const double bnorm = std::sqrt(bnorm2);
     if (!(bnorm > 0) || !std::isfinite(bnorm)) {
-        if (stats) { stats->iterations = 0; stats->relres = 0; stats->relres_initial = 0; }
+        if (stats) { stats->iterations = 0; stats->relres = 0; stats->relres_initial = 0; stats->learned_evaluated = 0; }
         hipLaunchKernelGGL(fill_kernel<double>, dim3(nb(n)), dim3(256), 0, s, x, n, 0.0);
         if (bnorm == 0) return EBG_OK;
         set_error("ebg_stokes_solve: right-hand side is not finite");
@@ -824,6 +825,11 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
     }
     int total = 0, restarts = 0, learned_dirs = 0;
     double relres = 1.0, relres0 = -1.0;
+    // learned 1: the judge's acceptance count after the last evaluation, the
+    // rejections in a row, and whether the network is still evaluated
+    double judged = 0.0;
+    int rejects = 0, evaluated = 0;
+    bool learned_on = (o.learned == 1 || o.learned == 2);
     while (true) {
         hipLaunchKernelGGL(residual_planar_kernel, dim3(nb(nc)), dim3(256), 0, s, st.op, x, b, r, LP);
         norm(r, st.nrm.as<double>());
@@ -848,6 +854,7 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
             double* vk = V + (int64_t)k * n;
             double* zk = Z + (int64_t)k * n;
             const bool learned_dir = pending;
+            bool judging = false;
             if (pending) {
                 // learned 3: the correction of the previous V-cycle output is
                 // an extra flexible direction (any z_k keeps A Z = V H valid)
@@ -857,9 +864,8 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
                 pending = false;
             } else {
                 apply_preconditioner(st, o, sm, vk, zk, s);
-                if ((o.learned == 1 || o.learned == 2) &&
-                    (rc = learned_step(ctx, st, vk, zk, o.learned == 1, s)))
-                    return rc;
+                if (learned_on && (rc = learned_step(ctx, st, vk, zk, o.learned == 1, s))) return rc;
+                judging = learned_on && o.learned == 1;
                 pending = (o.learned == 3);
             }
             hipLaunchKernelGGL((stokes_rows_kernel<false, LayoutPlanar>), dim3(nb(nc)), dim3(256), 0, s,
@@ -871,10 +877,19 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
             multidot(st, w, V, n, k + 1, n, hd + MAXK, 0, s);
             hipLaunchKernelGGL(update_kernel, dim3(nb(n)), dim3(256), 0, s, w, V, n, hd + MAXK, k + 1, n);
             norm(w, st.nrm.as<double>());
-            double hbuf[2 * MAXK], wn[3];
+            double hbuf[2 * MAXK], wn[3], acc = judged;
             EBG_HIP(hipMemcpyAsync(hbuf, hd, sizeof(double) * 2 * MAXK, hipMemcpyDeviceToHost, s));
             EBG_HIP(hipMemcpyAsync(wn, st.nrm.p, sizeof(wn), hipMemcpyDeviceToHost, s));
+            if (judging) EBG_HIP(hipMemcpyAsync(&acc, st.judge.p, 8, hipMemcpyDeviceToHost, s));
             EBG_HIP(hipStreamSynchronize(s));
+            if (judging) {
+                // the plain V-cycle is the fallback: after learned_patience
+                // rejections in a row the network is no longer evaluated
+                ++evaluated;
+                rejects = acc > judged ? 0 : rejects + 1;
+                judged = acc;
+                if (o.learned_patience > 0 && rejects >= o.learned_patience) learned_on = false;
+            }
             const double wn2 = wn[0];
             // a learned direction that lies (almost) in the current space would
             // turn rounding noise into the next basis vector: drop it
@@ -937,6 +952,7 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
         stats->relres = relres;
         stats->relres_initial = relres0;
         stats->learned_accepted = learned_dirs;
+        stats->learned_evaluated = evaluated;
         if (o.learned == 1) {
             double acc = 0;
             EBG_HIP(hipMemcpyAsync(&acc, st.judge.p, 8, hipMemcpyDeviceToHost, s));
@@ -1142,6 +1158,10 @@ int ebg_stokes_solve_opts_default(ebg_solve_opts* o)
     o->tile = 16;
     o->flexible = 1;
     o->coarse_unknowns = 6000;
+    // levels below the finest: all tiles per launch (a quarter of the
+    // launches of the four colours; a few more iterations, less time)
+    o->additive_tiles = 1;
+    o->learned_patience = 4;
     return EBG_OK;
 }
 
@@ -1177,6 +1197,9 @@ static int solve_impl(ebg_ctx* ctx, const double* R, double* S, const ebg_solve_
     EBG_CHECK(o.smoother == 0 || (o.local_sweeps >= 1 && o.tile >= 4 && o.tile <= 32), EBG_ERR_ARG,
               "ebg_stokes_solve: tiled smoother needs local_sweeps >= 1 and tile in [4, 32]");
     EBG_CHECK(o.learned >= 0 && o.learned <= 3, EBG_ERR_ARG, "ebg_stokes_solve: learned must be 0..3");
+    EBG_CHECK(o.additive_tiles == 0 || o.additive_tiles == 1, EBG_ERR_ARG,
+              "ebg_stokes_solve: additive_tiles must be 0 or 1");
+    EBG_CHECK(o.learned_patience >= 0, EBG_ERR_ARG, "ebg_stokes_solve: learned_patience must be >= 0");
     EBG_CHECK(o.learned == 0 || ctx->cnn, EBG_ERR_STATE,
               "ebg_stokes_solve: learned correction requested but no network (ebg_cnn_create)");
     const auto t0 = std::chrono::steady_clock::now();

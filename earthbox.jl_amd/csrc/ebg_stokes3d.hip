@@ -310,7 +310,7 @@ static int fgmres3(ebg_ctx* ctx, Stokes3State& st, const ebg_solve_opts& o, cons
     EBG_HIP(hipStreamSynchronize(s));
     const double bnorm = std::sqrt(bnorm2);
     if (!(bnorm > 0) || !std::isfinite(bnorm)) {
-        if (stats) { stats->iterations = 0; stats->relres = 0; stats->relres_initial = 0; }
+        if (stats) { stats->iterations = 0; stats->relres = 0; stats->relres_initial = 0; stats->learned_evaluated = 0; }
         mg3_set_bc(op_view(st, x, w), s);
         if (bnorm == 0) return EBG_OK;
         set_error("ebg_stokes3d_solve: right-hand side is not finite");
@@ -424,6 +424,7 @@ static int fgmres3(ebg_ctx* ctx, Stokes3State& st, const ebg_solve_opts& o, cons
         stats->relres = relres;
         stats->relres_initial = relres0;
         stats->learned_accepted = 0;
+        stats->learned_evaluated = 0;
     }
     return relres <= o.rtol ? EBG_OK : EBG_NOT_CONVERGED;
 }

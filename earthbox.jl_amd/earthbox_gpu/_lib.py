@@ -122,7 +122,8 @@ class SolveOpts(C.Structure):
                 ("sweep_multiplier", c_int), ("sweep_cap", c_int), ("theta_s", c_double),
                 ("theta_c", c_double), ("use_initial_guess", c_int), ("learned", c_int),
                 ("bc_d", c_double * 4), ("smoother", c_int), ("local_sweeps", c_int),
-                ("tile", c_int), ("flexible", c_int), ("coarse_unknowns", c_int)]
+                ("tile", c_int), ("flexible", c_int), ("coarse_unknowns", c_int),
+                ("additive_tiles", c_int), ("learned_patience", c_int)]
 
 
 class MGRunParams(C.Structure):
@@ -138,7 +139,7 @@ class SolveStats(C.Structure):
     """ebg_solve_stats (include/earthbox_gpu.h)."""
     _fields_ = [("iterations", c_int), ("restarts", c_int), ("relres", c_double),
                 ("relres_initial", c_double), ("seconds", c_double), ("status", c_int),
-                ("learned_accepted", c_int)]
+                ("learned_accepted", c_int), ("learned_evaluated", c_int)]
 
 
 class EBGError(RuntimeError):

@@ -41,6 +41,8 @@ struct SolveOpts
     tile::Cint
     flexible::Cint
     coarse_unknowns::Cint
+    additive_tiles::Cint
+    learned_patience::Cint
 end
 
 struct SolveStats
@@ -51,6 +53,7 @@ struct SolveStats
     seconds::Cdouble
     status::Cint
     learned_accepted::Cint
+    learned_evaluated::Cint
 end
 
 "Field order of ebg_mg_run_params (include/earthbox_gpu.h)."
@@ -165,7 +168,7 @@ function solve_system_gpu!(st::GPUState, model; mixed::Bool=true, rtol::Float64=
                     o[].theta_c, 0, 0,
                     (vc.btopx.array[2, 2], vc.bbottomx.array[2, 2], vc.blefty.array[2, 2],
                      vc.brighty.array[2, 2]), o[].smoother, o[].local_sweeps, o[].tile,
-                    o[].flexible, o[].coarse_unknowns)
+                    o[].flexible, o[].coarse_unknowns, o[].additive_tiles, o[].learned_patience)
     stats = Ref{SolveStats}()
     rc = check(ccall((:ebg_stokes_solve, LIB), Cint,
         (Ptr{Cvoid}, Ptr{Cvoid}, Ref{SolveOpts}, Ref{SolveStats}), st.ctx, dS, o, stats),

@@ -233,6 +233,13 @@ typedef struct {
                              and solves that level exactly (dense inverse of its operator, built
                              once per system with rocSOLVER); 0: smooth every level (no
                              reference counterpart) */
+    int additive_tiles;   /* 2-D tiled smoother on the levels below the finest: 1 (default)
+                             updates every tile in one launch from the previous iterate (block
+                             Jacobi between tiles, double-buffered, a quarter of the launches);
+                             0 launches the four tile colours in turn (block Gauss-Seidel) */
+    int learned_patience; /* learned 1: after this many consecutive rejections by the residual
+                             judge the solve falls back to the plain V-cycle for its remaining
+                             iterations (default 4; 0: judge every iteration) */
 } ebg_solve_opts;
 
 typedef struct {
@@ -244,6 +251,8 @@ typedef struct {
     int status;
     int learned_accepted; /* learned 1: corrections kept by the residual judge;
                              learned 3: learned Krylov directions */
+    int learned_evaluated;/* learned 1: corrections the judge compared (the network stops being
+                             evaluated after opts.learned_patience rejections in a row) */
 } ebg_solve_stats;
 
 int ebg_stokes_solve_opts_default(ebg_solve_opts *opts);

@@ -90,3 +90,28 @@ def test_learned_solve_matches_direct(gpu_ctx, case):
     assert st["converged"], st
     Sg = to_host(S)
     assert np.linalg.norm(Sg - Sd) <= 1e-8 * np.linalg.norm(Sd)
+
+
+@pytest.mark.parametrize("patience", [0, 3])
+def test_learned_fallback_after_rejections(gpu_ctx, patience):
+    """Mode 1 with an untrained network: patience 0 judges the correction at
+    every iteration; patience 3 stops evaluating it after three rejections in
+    a row and finishes on the plain V-cycle.  Both reach the direct solution."""
+    print(f"learned fallback patience {patience}", flush=True)
+    pr = random_case(**CASES[6])
+    Li, Lj, Lv, R = oracle.stokes_build(pr)
+    Sd = oracle.direct_solve(pr.N, Li, Lj, Lv, R)
+    sol = StokesSolverGPU(gpu_ctx)
+    sol.set_system(pr)
+    net = learned.LearnedCorrection(gpu_ctx, seed=7)
+    net.set_params(_params(8).astype(np.float32))
+    S, st = sol.solve_system(precision=1, rtol=1e-12, max_iterations=3000, learned=1,
+                             learned_patience=patience, bc_d=bc_mirror_factors(pr))
+    print(st, flush=True)
+    assert st["converged"], st
+    assert np.linalg.norm(to_host(S) - Sd) <= 1e-8 * np.linalg.norm(Sd)
+    if patience == 0:
+        assert st["learned_evaluated"] == st["iterations"]
+    else:
+        assert st["learned_evaluated"] - st["learned_accepted"] >= patience
+        assert st["learned_evaluated"] < st["iterations"]

@@ -47,7 +47,9 @@ def test_apply_rhs_residual_unpack_bit_exact(gpu_ctx, case):
 
 # default preconditioner (coarsest level solved directly) and the all-smoothed
 # hierarchy of the reference's sweep rule
-PRECOND = [dict(), dict(coarse_unknowns=0, finest_sweeps=4, sweep_cap=48)]
+# default (exact coarsest level, additive tiles below the finest level), the
+# four-colour tile launches, and smoothing on every level
+PRECOND = [dict(), dict(additive_tiles=0), dict(coarse_unknowns=0, finest_sweeps=4, sweep_cap=48)]
 
 
 @pytest.mark.parametrize("pc", PRECOND)
