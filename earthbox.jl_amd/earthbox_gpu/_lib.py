@@ -38,6 +38,7 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_rk4_advect": (c_int, [c_void_p, c_int64, c_int, c_double] + [c_void_p] * 13 + [c_int]),
     "ebg_calculate_spin": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "ebg_interpolate_marker_stokes_parameters_to_grid": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ebg_stokes_viscoelastic_rhs": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_stokes_set_system": (c_int, [c_void_p] + [c_void_p] * 16 + [c_int, c_double, c_double,
                                                                      c_double, c_double, c_int]),
     "ebg_stokes_apply": (c_int, [c_void_p, c_void_p, c_void_p]),
@@ -93,6 +94,19 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_stokes3d_precondition": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "ebg_stokes3d_solve": (c_int, [c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
 }
+
+
+class ViscoelasticIn(C.Structure):
+    _fields_ = [("etas1", c_void_p), ("mus1", c_void_p), ("sxy1", c_void_p), ("rho1", c_void_p),
+                ("etan1", c_void_p), ("mun1", c_void_p), ("sxx1", c_void_p), ("rho1_vy", c_void_p),
+                ("dilatation_grid", c_void_p), ("eii_plastic_pressure", c_void_p),
+                ("timestep", c_double), ("gravity_x", c_double), ("gravity_y", c_double),
+                ("iuse_interface_stabilization", c_int)]
+
+
+class ViscoelasticOut(C.Structure):
+    _fields_ = [("etas0", c_void_p), ("sxy0", c_void_p), ("etan0", c_void_p), ("sxx0", c_void_p),
+                ("RX1", c_void_p), ("RY1", c_void_p), ("RC1", c_void_p)]
 
 
 class MarkerStokesProps(C.Structure):

@@ -179,6 +179,55 @@ function solve_system_gpu!(st::GPUState, model; mixed::Bool=true, rtol::Float64=
     return S, stats[], rc
 end
 
+"Field order of ebg_viscoelastic_in / ebg_viscoelastic_out (include/earthbox_gpu.h)."
+struct ViscoelasticIn
+    etas1::Ptr{Cvoid}; mus1::Ptr{Cvoid}; sxy1::Ptr{Cvoid}; rho1::Ptr{Cvoid}
+    etan1::Ptr{Cvoid}; mun1::Ptr{Cvoid}; sxx1::Ptr{Cvoid}
+    rho1_vy::Ptr{Cvoid}
+    dilatation_grid::Ptr{Cvoid}; eii_plastic_pressure::Ptr{Cvoid}
+    timestep::Cdouble; gravity_x::Cdouble; gravity_y::Cdouble
+    iuse_interface_stabilization::Cint
+end
+struct ViscoelasticOut
+    etas0::Ptr{Cvoid}; sxy0::Ptr{Cvoid}; etan0::Ptr{Cvoid}; sxx0::Ptr{Cvoid}
+    RX1::Ptr{Cvoid}; RY1::Ptr{Cvoid}; RC1::Ptr{Cvoid}
+end
+
+"""
+    calc_viscoelastic_terms_and_rhs_gpu!(st, model)
+
+GPU replacement of the back-to-back calls
+StokesViscoelasticTerms.calc_viscoelastic_terms!(model) and
+StokesContinuityRhs.viscoelastic_rhs!(model) (StokesContinuitySolver.jl:189-190):
+fills etas0, sxy0, etan0, sxx0, RX1, RY1 and RC1 of model.stokes_continuity.
+"""
+function calc_viscoelastic_terms_and_rhs_gpu!(st::GPUState, model)
+    set_grid!(st, model)
+    a = model.stokes_continuity.arrays
+    up(k, x) = h2d!(st, k, x)
+    vin = ViscoelasticIn(up(:etas1, a.viscosity.etas1.array), up(:mus1, a.shear_modulus.mus1.array),
+        up(:sxy1, a.stress.sxy1.array), up(:rho1, a.density.rho1.array),
+        up(:etan1, a.viscosity.etan1.array), up(:mun1, a.shear_modulus.mun1.array),
+        up(:sxx1, a.stress.sxx1.array), up(:rho1_vy, a.density.rho1_vy.array),
+        up(:dil, a.plastic_def.dilatation_grid.array),
+        up(:eiip, a.strain_rate_and_spin.eii_plastic_pressure.array),
+        model.timestep.parameters.main_time_loop.timestep.value,
+        model.gravity.parameters.gravity_x.value, model.gravity.parameters.gravity_y.value,
+        model.stokes_continuity.parameters.build.iuse_interface_stabilization.value)
+    outs = (a.viscosity.etas0.array, a.stress.sxy0.array, a.viscosity.etan0.array, a.stress.sxx0.array,
+            a.rhs.RX1.array, a.rhs.RY1.array, a.rhs.RC1.array)
+    keys = (:etas0, :sxy0, :etan0, :sxx0, :RX1, :RY1, :RC1)
+    vout = ViscoelasticOut((dev!(st, k, x) for (k, x) in zip(keys, outs))...)
+    check(ccall((:ebg_stokes_viscoelastic_rhs, LIB), Cint,
+        (Ptr{Cvoid}, Ref{ViscoelasticIn}, Ref{ViscoelasticOut}), st.ctx, vin, vout),
+        "ebg_stokes_viscoelastic_rhs")
+    check(ccall((:ebg_ctx_synchronize, LIB), Cint, (Ptr{Cvoid},), st.ctx), "sync")
+    for (k, x) in zip(keys, outs)
+        d2h!(x, st.buffers[k])
+    end
+    return nothing
+end
+
 """
     advect_markers_gpu!(st, model, inside_flags)
 

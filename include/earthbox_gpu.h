@@ -154,6 +154,34 @@ int ebg_interpolate_marker_stokes_parameters_to_grid(ebg_ctx *ctx,
                                                      const ebg_marker_stokes_props *props,
                                                      ebg_stokes_grid_fields *fields);
 
+/* ---- viscoelastic terms and right-hand sides (K7) ----
+ * DEVICE grid arrays with the reference shapes.  Inputs: etas1, mus1, sxy1,
+ * rho1 (ynum,xnum); etan1, mun1, sxx1 (ynum-1,xnum-1); rho1_vy (ynum,xnum+1),
+ * read only with interface stabilization; dilatation_grid and
+ * eii_plastic_pressure (ynum-1,xnum-1), both NULL for RC1 = 0.  Outputs:
+ * etas0, sxy0 (ynum,xnum); etan0, sxx0 (ynum-1,xnum-1); RX1 (ynum+1,xnum),
+ * RY1 (ynum,xnum+1), RC1 (ynum-1,xnum-1), every entry written. */
+typedef struct {
+    const double *etas1, *mus1, *sxy1, *rho1;
+    const double *etan1, *mun1, *sxx1;
+    const double *rho1_vy;
+    const double *dilatation_grid, *eii_plastic_pressure;
+    double timestep, gravity_x, gravity_y;
+    int iuse_interface_stabilization;
+} ebg_viscoelastic_in;
+
+typedef struct {
+    double *etas0, *sxy0, *etan0, *sxx0;
+    double *RX1, *RY1, *RC1;
+} ebg_viscoelastic_out;
+
+/* Replaces StokesViscoelasticTerms.calc_viscoelastic_terms! followed by
+ * StokesContinuityRhs.viscoelastic_rhs! (solver/rhs/StokesViscoelasticTerms.jl:26-108,
+ * solver/rhs/StokesContinuityRhs.jl:30-190, called at StokesContinuitySolver.jl:189-190)
+ * on the context's grid; bit-identical except RC1, whose sine goes through the
+ * device math library. */
+int ebg_stokes_viscoelastic_rhs(ebg_ctx *ctx, const ebg_viscoelastic_in *in, ebg_viscoelastic_out *out);
+
 /* ---- Stokes-continuity system (assembled-operator conventions) ----
  * Replaces the inputs StokesBuildData gathers (solver/build_system/utils/
  * BuildStructs.jl:37-177).  Fields are DEVICE pointers with the reference

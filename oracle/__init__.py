@@ -174,6 +174,38 @@ def stokes_build(prob):
     return Li[:nnz].copy(), Lj[:nnz].copy(), Lv[:nnz].copy(), R
 
 
+class _RhsIn(C.Structure):
+    _fields_ = [("xnum", _I64), ("ynum", _I64), ("xstp_b", _P), ("ystp_b", _P), ("xstp_vy", _P),
+                ("ystp_vx", _P), ("etas1", _P), ("mus1", _P), ("sxy1", _P), ("etan1", _P),
+                ("mun1", _P), ("sxx1", _P), ("rho1", _P), ("rho1_vy", _P), ("dilatation", _P),
+                ("eii_plastic_pressure", _P), ("timestep", C.c_double), ("gravity_x", C.c_double),
+                ("gravity_y", C.c_double), ("iuse_interface_stabilization", _I64)]
+
+
+def stokes_viscoelastic_rhs(g, f: dict, timestep, gravity_x, gravity_y, iuse_interface_stabilization):
+    """K7 (StokesViscoelasticTerms.jl:26-108 then StokesContinuityRhs.jl:30-190):
+    f holds etas1, mus1, sxy1, rho1 (ynum, xnum), etan1, mun1, sxx1 (ynum-1,
+    xnum-1), rho1_vy (ynum, xnum+1) and optionally dilatation_grid and
+    eii_plastic_pressure (ynum-1, xnum-1).  Returns etas0, sxy0, etan0, sxx0,
+    RX1, RY1, RC1 (Fortran order)."""
+    L = lib()
+    L.ebo_stokes_rhs.restype = None
+    L.ebo_stokes_rhs.argtypes = [C.POINTER(_RhsIn)] + [_P] * 7
+    a = {k: (np.asfortranarray(v, dtype=np.float64) if v is not None else None) for k, v in f.items()}
+    ax = [np.ascontiguousarray(v, dtype=np.float64) for v in (g.xstp_b, g.ystp_b, g.xstp_vy, g.ystp_vx)]
+    r = _RhsIn(g.xnum, g.ynum, *[p(v) for v in ax], p(a["etas1"]), p(a["mus1"]), p(a["sxy1"]),
+               p(a["etan1"]), p(a["mun1"]), p(a["sxx1"]), p(a["rho1"]), p(a.get("rho1_vy")),
+               p(a.get("dilatation_grid")), p(a.get("eii_plastic_pressure")), float(timestep),
+               float(gravity_x), float(gravity_y), int(iuse_interface_stabilization))
+    ny, nx = g.ynum, g.xnum
+    out = {"etas0": np.zeros((ny, nx), order="F"), "sxy0": np.zeros((ny, nx), order="F"),
+           "etan0": np.zeros((ny - 1, nx - 1), order="F"), "sxx0": np.zeros((ny - 1, nx - 1), order="F"),
+           "RX1": np.zeros((ny + 1, nx), order="F"), "RY1": np.zeros((ny, nx + 1), order="F"),
+           "RC1": np.zeros((ny - 1, nx - 1), order="F")}
+    L.ebo_stokes_rhs(C.byref(r), *[p(out[k]) for k in ("etas0", "sxy0", "etan0", "sxx0", "RX1", "RY1", "RC1")])
+    return out
+
+
 def stokes_unpack(prob, S):
     """K10: (vx1, vy1, pr1) from the solution vector with ghost BCs applied."""
     L = lib()
