@@ -79,8 +79,10 @@ struct ebg_ctx {
     void* cnn = nullptr;  // learned preconditioner correction (ebg_cnn_create)
     void* mg3_ref = nullptr;   // reference-order 3-D multigrid (ebg_mg3d_create)
     void* stokes3d = nullptr;  // 3-D Stokes solve state (ebg_stokes3d_set_system)
+    void* plastic = nullptr;   // reduction scratch of ebg_update_nodes_for_plastic_yielding
 };
 
 namespace ebg {
 int grid_release(Grid2D& g);
+void plastic_release(ebg_ctx* ctx);
 }

@@ -1,0 +1,263 @@
+// Nodal plastic yielding: one step of the reference's Picard loop over nodes.
+//
+// Replaces PlasticFailureNodes.update_nodes_for_plastic_yielding
+// (rheology/plastic_failure/nodes/PlasticFailureNodes.jl:24-319) with
+// CheckYield.check_yield_is_applicable, YieldStress.calc_yield_stress,
+// FluidPressure.calculate_fluid_pressure (plastic_failure/utils/) and
+// MobileWall.in_mobile_wall_transient / in_plate_extension_transient
+// (utils/MobileWall.jl:106-122, 279-329).  One thread per basic node; the
+// yield count and the sum of squared yield errors are reduced in a fixed
+// order (per-block partials, then one block), so repeated calls agree.
+#include "ebg_common.hpp"
+
+#include <cmath>
+
+namespace ebg {
+
+namespace {
+
+constexpr int PL_THREADS = 256;
+constexpr int PL_MAX_BLOCKS = 1024;
+
+struct PlArgs {
+    int xnum, ynum;
+    const double *gridx_b, *gridy_b;
+    ebg_plastic_nodes_in in;
+    double *etavp_new, *plastic_strain_rate, *plastic_yield_new, *yield_error;
+    double* partial_err;       // [nblocks]
+    long long* partial_count;  // [nblocks]
+};
+
+// Julia's max / min on Float64: NaN wins and +0.0 > -0.0
+__device__ __forceinline__ double jl_max(double a, double b)
+{
+    if (a != a || b != b) return a + b;
+    if (a == b) return signbit(a) ? b : a;
+    return a > b ? a : b;
+}
+__device__ __forceinline__ double jl_min(double a, double b)
+{
+    if (a != a || b != b) return a + b;
+    if (a == b) return signbit(a) ? a : b;
+    return a < b ? a : b;
+}
+
+// PlasticFailureNodes.jl:206-248 (interpolate_from_pressure_grid_to_basic_grid)
+__device__ double pressure_to_basic(int i, int j, int ynum, int xnum, const double* __restrict__ a)
+{
+    const int ld = ynum - 1;
+#define PA(ii, jj) a[(size_t)((jj) - 1) * ld + ((ii) - 1)]
+    if (1 < i && i < ynum && 1 < j && j < xnum) return (PA(i - 1, j - 1) + PA(i, j - 1) + PA(i - 1, j) + PA(i, j)) / 4.0;
+    if (i == 1 && j == 1) return PA(i, j);
+    if (i == ynum && j == 1) return PA(i - 1, j);
+    if (i == ynum && j == xnum) return PA(i - 1, j - 1);
+    if (i == 1 && j == xnum) return PA(i, j - 1);
+    if (j == 1) return (PA(i, j) + PA(i - 1, j)) / 2.0;
+    if (j == xnum) return (PA(i - 1, j - 1) + PA(i, j - 1)) / 2.0;
+    if (i == 1) return (PA(i, j) + PA(i, j - 1)) / 2.0;
+    if (i == ynum) return (PA(i - 1, j) + PA(i - 1, j - 1)) / 2.0;
+#undef PA
+    return 0.0;
+}
+
+// YieldStress.calc_yield_stress: Drucker-Prager, optionally on the pressure
+// above the hydrostatic fluid pressure (density 1000, g 9.81) with the
+// tension cut-off cohesion * cos(phi) (stress_yield_tension_method2)
+__device__ double yield_stress(double cohesion, double sine_fric, double pressure, int use_fluid, double y,
+                               double y_sealevel)
+{
+    const double cosphi = cos(asin(sine_fric));
+    double s;
+    if (use_fluid == 0) {
+        s = cohesion * cosphi + sine_fric * pressure;
+    } else {
+        const double pf = 1000.0 * 9.81 * jl_max(0.0, y - y_sealevel);
+        if (pressure < pf) s = cohesion * cosphi;
+        else s = cohesion * cosphi + sine_fric * (pressure - pf);
+    }
+    return jl_max(s, 0.0);
+}
+
+__global__ void __launch_bounds__(PL_THREADS) plastic_nodes_kernel(PlArgs a)
+{
+    __shared__ double serr[PL_THREADS];
+    __shared__ long long scnt[PL_THREADS];
+    const int ny = a.ynum, nx = a.xnum;
+    const int64_t nn = (int64_t)nx * ny;
+    const ebg_plastic_nodes_in& in = a.in;
+    const double dt = in.timestep;
+    const double pi_term = M_PI / 180.0;
+    double err = 0.0;
+    long long cnt = 0;
+    for (int64_t t = (int64_t)blockIdx.x * PL_THREADS + threadIdx.x; t < nn; t += (int64_t)gridDim.x * PL_THREADS) {
+        const int i = (int)(t % ny) + 1, j = (int)(t / ny) + 1;
+        const double yield_marker = in.plastic_yield[t];
+        const double eta_old = in.etas1[t];
+        const double eta_flow = in.eta_flow[t];
+        const double mu = in.mus1[t];
+        const double cohesion = in.cohesion_grid[t];
+        const double fric = sin(in.fric_degrees_grid[t] * pi_term);
+        bool applicable = in.ntimestep > 0 && (cohesion > 0.0 || fric > 0.0);
+        const double xb = a.gridx_b[j - 1], yb = a.gridy_b[i - 1];
+        if (in.no_yielding_in_mobile_wall) {
+            const double xl = in.x_left_mobile_wall + in.velocity_internal_x * in.timesum;
+            const double xr = in.x_right_mobile_wall + in.velocity_internal_x * in.timesum;
+            if (xl <= xb && xb <= xr && in.y_top_mobile_wall <= yb && yb <= in.y_bottom_mobile_wall)
+                applicable = false;
+        }
+        if (in.no_yielding_in_plate_extension) {
+            const double xl = in.x_left_mobile_wall + in.velocity_internal_x * in.timesum;
+            const double xmin = xl - in.plate_extension_width, xmax = xl;
+            const double ymin = in.y_bottom_mobile_wall - in.plate_extension_thickness, ymax = in.y_bottom_mobile_wall;
+            if (xmin <= xb && xb <= xmax && ymin <= yb && yb <= ymax) applicable = false;
+        }
+        double ev = eta_flow, py = 0.0, psr = 0.0;
+        if (applicable) {
+            const double pressure = pressure_to_basic(i, j, ny, nx, in.pr1);
+            const double sxx = pressure_to_basic(i, j, ny, nx, in.sxx2);
+            const double sxy = in.sxy2[t];
+            const double sii = sqrt(sxy * sxy + sxx * sxx);
+            const double sii_e = sii * (mu * dt + eta_old) / eta_old;
+            const double sy = yield_stress(cohesion, fric, pressure, in.iuse_fluid_pressure_for_yield, yb, in.y_sealevel);
+            const bool at_yield = yield_marker > 0;
+            bool counted = false;
+            if (at_yield) counted = true;
+            if (sy < sii_e) {
+                double eta_new = dt * mu * sy / (sii_e - sy);
+                if (eta_new < eta_flow) {
+                    py = 1.0;
+                    eta_new = pow(eta_new, 1.0 - 0.0) * pow(eta_old, 0.0);
+                    eta_new = jl_max(eta_new, in.viscosity_min);
+                    eta_new = jl_min(eta_new, in.viscosity_max);
+                    ev = eta_new;
+                    psr = 0.5 * sii / eta_new - 0.5 * sii / eta_flow;
+                    counted = true;
+                }
+            }
+            if (counted) {
+                const double e = sii - sy;
+                a.yield_error[t] = e;
+                err += e * e;
+                ++cnt;
+            }
+        }
+        a.etavp_new[t] = ev;
+        a.plastic_yield_new[t] = py;
+        a.plastic_strain_rate[t] = psr;
+    }
+    serr[threadIdx.x] = err;
+    scnt[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int s = PL_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            serr[threadIdx.x] += serr[threadIdx.x + s];
+            scnt[threadIdx.x] += scnt[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.partial_err[blockIdx.x] = serr[0];
+        a.partial_count[blockIdx.x] = scnt[0];
+    }
+}
+
+// one block: the partials in block order, then sqrt(sum / count)
+__global__ void __launch_bounds__(PL_THREADS) plastic_finish_kernel(const double* __restrict__ perr,
+                                                                    const long long* __restrict__ pcnt, int nb,
+                                                                    double* __restrict__ result)
+{
+    __shared__ double serr[PL_THREADS];
+    __shared__ long long scnt[PL_THREADS];
+    double e = 0.0;
+    long long c = 0;
+    for (int k = threadIdx.x; k < nb; k += PL_THREADS) {
+        e += perr[k];
+        c += pcnt[k];
+    }
+    serr[threadIdx.x] = e;
+    scnt[threadIdx.x] = c;
+    __syncthreads();
+    for (int s = PL_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            serr[threadIdx.x] += serr[threadIdx.x + s];
+            scnt[threadIdx.x] += scnt[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        result[0] = (double)scnt[0];
+        result[1] = scnt[0] > 0 ? sqrt(serr[0] / (double)scnt[0]) : 0.0;
+    }
+}
+
+}  // namespace
+
+// per-context scratch: nb partial sums, nb partial counts, two results
+struct PlasticScratch {
+    double* buf = nullptr;
+    ~PlasticScratch()
+    {
+        if (buf) (void)hipFree(buf);
+    }
+};
+
+void plastic_release(ebg_ctx* ctx)
+{
+    if (ctx && ctx->plastic) {
+        delete (PlasticScratch*)ctx->plastic;
+        ctx->plastic = nullptr;
+    }
+}
+
+}  // namespace ebg
+
+using namespace ebg;
+
+extern "C" int ebg_update_nodes_for_plastic_yielding(ebg_ctx* ctx, const ebg_plastic_nodes_in* in,
+                                                     ebg_plastic_nodes_out* out)
+{
+    EBG_CHECK(ctx && ctx->has_grid, EBG_ERR_STATE, "ebg_update_nodes_for_plastic_yielding: no grid");
+    EBG_CHECK(in && out, EBG_ERR_ARG, "ebg_update_nodes_for_plastic_yielding: NULL arguments");
+    EBG_CHECK(in->plastic_yield && in->etas1 && in->eta_flow && in->mus1 && in->cohesion_grid &&
+                  in->fric_degrees_grid && in->sxy2 && in->sxx2 && in->pr1,
+              EBG_ERR_ARG, "ebg_update_nodes_for_plastic_yielding: NULL input field");
+    EBG_CHECK(out->etavp_new && out->plastic_strain_rate && out->plastic_yield_new && out->yield_error, EBG_ERR_ARG,
+              "ebg_update_nodes_for_plastic_yielding: NULL output field");
+    const Grid2D& g = ctx->grid;
+    hipStream_t s = ctx->stream;
+    const int64_t nn = g.xnum * g.ynum;
+    const int nb = (int)std::min<int64_t>(PL_MAX_BLOCKS, (nn + PL_THREADS - 1) / PL_THREADS);
+    if (!ctx->plastic) {
+        PlasticScratch* ps = new PlasticScratch();
+        if (hipMalloc(&ps->buf, sizeof(double) * (2 * PL_MAX_BLOCKS + 2)) != hipSuccess) {
+            delete ps;
+            set_error("ebg_update_nodes_for_plastic_yielding: hipMalloc failed");
+            return EBG_ERR_HIP;
+        }
+        ctx->plastic = ps;
+    }
+    double* scratch = ((PlasticScratch*)ctx->plastic)->buf;
+    PlArgs a{};
+    a.xnum = (int)g.xnum;
+    a.ynum = (int)g.ynum;
+    a.gridx_b = g.gridx_b;
+    a.gridy_b = g.gridy_b;
+    a.in = *in;
+    a.etavp_new = out->etavp_new;
+    a.plastic_strain_rate = out->plastic_strain_rate;
+    a.plastic_yield_new = out->plastic_yield_new;
+    a.yield_error = out->yield_error;
+    a.partial_err = scratch;
+    a.partial_count = (long long*)(scratch + PL_MAX_BLOCKS);
+    double* result = scratch + 2 * PL_MAX_BLOCKS;
+    hipLaunchKernelGGL(plastic_nodes_kernel, dim3(nb), dim3(PL_THREADS), 0, s, a);
+    hipLaunchKernelGGL(plastic_finish_kernel, dim3(1), dim3(PL_THREADS), 0, s, (const double*)a.partial_err,
+                       (const long long*)a.partial_count, nb, result);
+    EBG_HIP(hipGetLastError());
+    double h[2];
+    EBG_HIP(hipMemcpyAsync(h, result, sizeof(h), hipMemcpyDeviceToHost, s));
+    EBG_HIP(hipStreamSynchronize(s));
+    out->nnode_yield = (int64_t)h[0];
+    out->global_yield_error = h[1];
+    return EBG_OK;
+}

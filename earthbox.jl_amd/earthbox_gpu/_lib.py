@@ -39,6 +39,7 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_calculate_spin": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "ebg_interpolate_marker_stokes_parameters_to_grid": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_stokes_viscoelastic_rhs": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ebg_update_nodes_for_plastic_yielding": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_stokes_set_system": (c_int, [c_void_p] + [c_void_p] * 16 + [c_int, c_double, c_double,
                                                                      c_double, c_double, c_int]),
     "ebg_stokes_apply": (c_int, [c_void_p, c_void_p, c_void_p]),
@@ -107,6 +108,24 @@ class ViscoelasticIn(C.Structure):
 class ViscoelasticOut(C.Structure):
     _fields_ = [("etas0", c_void_p), ("sxy0", c_void_p), ("etan0", c_void_p), ("sxx0", c_void_p),
                 ("RX1", c_void_p), ("RY1", c_void_p), ("RC1", c_void_p)]
+
+
+class PlasticNodesIn(C.Structure):
+    _fields_ = [("plastic_yield", c_void_p), ("etas1", c_void_p), ("eta_flow", c_void_p), ("mus1", c_void_p),
+                ("cohesion_grid", c_void_p), ("fric_degrees_grid", c_void_p), ("sxy2", c_void_p),
+                ("sxx2", c_void_p), ("pr1", c_void_p), ("timestep", c_double), ("timesum", c_double),
+                ("ntimestep", c_int64), ("viscosity_min", c_double), ("viscosity_max", c_double),
+                ("iuse_fluid_pressure_for_yield", c_int), ("y_sealevel", c_double),
+                ("no_yielding_in_mobile_wall", c_int), ("no_yielding_in_plate_extension", c_int),
+                ("velocity_internal_x", c_double), ("x_left_mobile_wall", c_double),
+                ("x_right_mobile_wall", c_double), ("y_top_mobile_wall", c_double),
+                ("y_bottom_mobile_wall", c_double), ("plate_extension_thickness", c_double),
+                ("plate_extension_width", c_double)]
+
+
+class PlasticNodesOut(C.Structure):
+    _fields_ = [("etavp_new", c_void_p), ("plastic_strain_rate", c_void_p), ("plastic_yield_new", c_void_p),
+                ("yield_error", c_void_p), ("nnode_yield", c_int64), ("global_yield_error", c_double)]
 
 
 class MarkerStokesProps(C.Structure):

@@ -228,6 +228,70 @@ function calc_viscoelastic_terms_and_rhs_gpu!(st::GPUState, model)
     return nothing
 end
 
+"Field order of ebg_plastic_nodes_in / ebg_plastic_nodes_out (include/earthbox_gpu.h)."
+struct PlasticNodesIn
+    plastic_yield::Ptr{Cvoid}; etas1::Ptr{Cvoid}; eta_flow::Ptr{Cvoid}; mus1::Ptr{Cvoid}
+    cohesion_grid::Ptr{Cvoid}; fric_degrees_grid::Ptr{Cvoid}; sxy2::Ptr{Cvoid}
+    sxx2::Ptr{Cvoid}; pr1::Ptr{Cvoid}
+    timestep::Cdouble; timesum::Cdouble; ntimestep::Int64
+    viscosity_min::Cdouble; viscosity_max::Cdouble
+    iuse_fluid_pressure_for_yield::Cint; y_sealevel::Cdouble
+    no_yielding_in_mobile_wall::Cint; no_yielding_in_plate_extension::Cint
+    velocity_internal_x::Cdouble; x_left_mobile_wall::Cdouble; x_right_mobile_wall::Cdouble
+    y_top_mobile_wall::Cdouble; y_bottom_mobile_wall::Cdouble
+    plate_extension_thickness::Cdouble; plate_extension_width::Cdouble
+end
+mutable struct PlasticNodesOut
+    etavp_new::Ptr{Cvoid}; plastic_strain_rate::Ptr{Cvoid}; plastic_yield_new::Ptr{Cvoid}
+    yield_error::Ptr{Cvoid}; nnode_yield::Int64; global_yield_error::Cdouble
+end
+
+"""
+    update_nodes_for_plastic_yielding_gpu!(st, model, no_yielding_in_mobile_wall,
+        no_yielding_in_plate_extension=false) -> (etavp_new, plastic_strain_rate,
+        plastic_yield_new, nnode_yield, global_yield_error)
+
+GPU replacement of PlasticFailureNodes.update_nodes_for_plastic_yielding
+(rheology/plastic_failure/nodes/PlasticFailureNodes.jl:24-319); returns the
+reference's tuple, the three grids being the model's buffers, and updates
+plastic_def.yield_error.
+"""
+function update_nodes_for_plastic_yielding_gpu!(st::GPUState, model, no_yielding_in_mobile_wall::Bool,
+                                                no_yielding_in_plate_extension::Bool=false)
+    set_grid!(st, model)
+    a = model.stokes_continuity.arrays
+    pd = a.plastic_def
+    mw = model.geometry.parameters.mobile_wall
+    up(k, x) = h2d!(st, k, x)
+    pin = PlasticNodesIn(up(:pyield, pd.plastic_yield.array), up(:etas1, a.viscosity.etas1.array),
+        up(:eta_flow, a.viscosity.eta_flow.array), up(:mus1, a.shear_modulus.mus1.array),
+        up(:coh, pd.cohesion_grid.array), up(:fric, pd.fric_degrees_grid.array),
+        up(:sxy2, a.stress.sxy2.array), up(:sxx2, a.stress.sxx2.array), up(:pr1, a.pressure.pr1.array),
+        model.timestep.parameters.main_time_loop.timestep.value,
+        model.timestep.parameters.main_time_loop.timesum.value,
+        model.timestep.parameters.main_time_loop.ntimestep.value,
+        model.materials.parameters.viscosity_limits.viscosity_min.value,
+        model.materials.parameters.viscosity_limits.viscosity_max.value,
+        model.materials.parameters.stress_limits_yield.iuse_fluid_pressure_for_yield.value,
+        model.topography.parameters.sealevel.y_sealevel.value,
+        no_yielding_in_mobile_wall, no_yielding_in_plate_extension,
+        model.bcs.parameters.velocity.velocity_internal_x.value,
+        mw.x_left_mobile_wall.value, mw.x_right_mobile_wall.value, mw.y_top_mobile_wall.value,
+        mw.y_bottom_mobile_wall.value, mw.plate_extension_thickness.value, mw.plate_extension_width.value)
+    outs = (pd.etavp_buffer.array, pd.plastic_strain_rate_buffer.array, pd.plastic_yield_buffer.array)
+    keys = (:etavp_new, :psr, :pyield_new)
+    pout = PlasticNodesOut(dev!(st, keys[1], outs[1]), dev!(st, keys[2], outs[2]), dev!(st, keys[3], outs[3]),
+                           up(:yield_error, pd.yield_error.array), 0, 0.0)
+    check(ccall((:ebg_update_nodes_for_plastic_yielding, LIB), Cint,
+        (Ptr{Cvoid}, Ref{PlasticNodesIn}, Ref{PlasticNodesOut}), st.ctx, pin, pout),
+        "ebg_update_nodes_for_plastic_yielding")
+    for (k, x) in zip(keys, outs)
+        d2h!(x, st.buffers[k])
+    end
+    d2h!(pd.yield_error.array, st.buffers[:yield_error])
+    return (outs..., pout.nnode_yield, pout.global_yield_error)
+end
+
 """
     advect_markers_gpu!(st, model, inside_flags)
 

@@ -182,6 +182,40 @@ typedef struct {
  * device math library. */
 int ebg_stokes_viscoelastic_rhs(ebg_ctx *ctx, const ebg_viscoelastic_in *in, ebg_viscoelastic_out *out);
 
+/* ---- nodal plastic yielding (Picard loop over nodes) ----
+ * DEVICE grid arrays, reference shapes: plastic_yield, etas1, eta_flow, mus1,
+ * cohesion_grid, fric_degrees_grid, sxy2 (ynum,xnum); sxx2, pr1 (ynum-1,xnum-1).
+ * Scalars as the reference reads them from ModelData (timestep, timesum,
+ * ntimestep, viscosity limits, iuse_fluid_pressure_for_yield, y_sealevel,
+ * mobile-wall geometry) and the loop's two no-yielding switches. */
+typedef struct {
+    const double *plastic_yield, *etas1, *eta_flow, *mus1, *cohesion_grid, *fric_degrees_grid, *sxy2;
+    const double *sxx2, *pr1;
+    double timestep, timesum;
+    int64_t ntimestep;
+    double viscosity_min, viscosity_max;
+    int iuse_fluid_pressure_for_yield;
+    double y_sealevel;
+    int no_yielding_in_mobile_wall, no_yielding_in_plate_extension;
+    double velocity_internal_x, x_left_mobile_wall, x_right_mobile_wall, y_top_mobile_wall,
+        y_bottom_mobile_wall, plate_extension_thickness, plate_extension_width;
+} ebg_plastic_nodes_in;
+
+typedef struct {
+    double *etavp_new, *plastic_strain_rate, *plastic_yield_new; /* DEVICE (ynum,xnum), written */
+    double *yield_error;                                          /* DEVICE (ynum,xnum), updated */
+    int64_t nnode_yield;                                          /* host result */
+    double global_yield_error;                                    /* host result */
+} ebg_plastic_nodes_out;
+
+/* Replaces PlasticFailureNodes.update_nodes_for_plastic_yielding
+ * (rheology/plastic_failure/nodes/PlasticFailureNodes.jl:24-319) on the
+ * context's grid.  Per-node results follow the reference's expressions; the
+ * sine, arcsine and cosine go through the device math library, and the RMS
+ * of the yield errors is summed in a fixed tree order instead of node order. */
+int ebg_update_nodes_for_plastic_yielding(ebg_ctx *ctx, const ebg_plastic_nodes_in *in,
+                                          ebg_plastic_nodes_out *out);
+
 /* ---- Stokes-continuity system (assembled-operator conventions) ----
  * Replaces the inputs StokesBuildData gathers (solver/build_system/utils/
  * BuildStructs.jl:37-177).  Fields are DEVICE pointers with the reference

@@ -206,6 +206,51 @@ def stokes_viscoelastic_rhs(g, f: dict, timestep, gravity_x, gravity_y, iuse_int
     return out
 
 
+class _PlasticIn(C.Structure):
+    _fields_ = [("xnum", _I64), ("ynum", _I64), ("gridx_b", _P), ("gridy_b", _P),
+                ("plastic_yield", _P), ("etas1", _P), ("eta_flow", _P), ("mus1", _P),
+                ("cohesion_grid", _P), ("fric_degrees_grid", _P), ("sxy2", _P), ("sxx2", _P), ("pr1", _P),
+                ("timestep", C.c_double), ("timesum", C.c_double), ("ntimestep", _I64),
+                ("viscosity_min", C.c_double), ("viscosity_max", C.c_double),
+                ("iuse_fluid_pressure_for_yield", _I64), ("y_sealevel", C.c_double),
+                ("no_yielding_in_mobile_wall", _I64), ("no_yielding_in_plate_extension", _I64),
+                ("velocity_internal_x", C.c_double), ("x_left_mobile_wall", C.c_double),
+                ("x_right_mobile_wall", C.c_double), ("y_top_mobile_wall", C.c_double),
+                ("y_bottom_mobile_wall", C.c_double), ("plate_extension_thickness", C.c_double),
+                ("plate_extension_width", C.c_double)]
+
+
+PLASTIC_FIELDS = ("plastic_yield", "etas1", "eta_flow", "mus1", "cohesion_grid", "fric_degrees_grid", "sxy2",
+                  "sxx2", "pr1")
+
+
+def update_nodes_for_plastic_yielding(g, f: dict, par: dict, yield_error):
+    """PlasticFailureNodes.update_nodes_for_plastic_yielding
+    (rheology/plastic_failure/nodes/PlasticFailureNodes.jl:24-319).  f holds the
+    PLASTIC_FIELDS grid arrays, par the scalars of _PlasticIn (missing ones 0).
+    Returns (etavp_new, plastic_strain_rate, plastic_yield_new, yield_error,
+    nnode_yield, global_yield_error); yield_error is updated in a copy."""
+    L = lib()
+    L.ebo_update_nodes_for_plastic_yielding.restype = None
+    L.ebo_update_nodes_for_plastic_yielding.argtypes = [C.POINTER(_PlasticIn)] + [_P] * 4 + [
+        C.POINTER(_I64), C.POINTER(C.c_double)]
+    a = {k: np.asfortranarray(f[k], dtype=np.float64) for k in PLASTIC_FIELDS}
+    gx = np.ascontiguousarray(g.gridx_b, dtype=np.float64)
+    gy = np.ascontiguousarray(g.gridy_b, dtype=np.float64)
+    s = _PlasticIn(g.xnum, g.ynum, p(gx), p(gy), *[p(a[k]) for k in PLASTIC_FIELDS])
+    for k, _ in _PlasticIn._fields_[13:]:
+        setattr(s, k, par.get(k, 0))
+    ny, nx = g.ynum, g.xnum
+    ev = np.zeros((ny, nx), order="F")
+    psr = np.zeros((ny, nx), order="F")
+    pyn = np.zeros((ny, nx), order="F")
+    ye = np.asfortranarray(np.array(yield_error, dtype=np.float64, copy=True))
+    n = _I64(0)
+    err = C.c_double(0.0)
+    L.ebo_update_nodes_for_plastic_yielding(C.byref(s), p(ev), p(psr), p(pyn), p(ye), C.byref(n), C.byref(err))
+    return ev, psr, pyn, ye, int(n.value), float(err.value)
+
+
 def stokes_unpack(prob, S):
     """K10: (vx1, vy1, pr1) from the solution vector with ghost BCs applied."""
     L = lib()

@@ -1,0 +1,40 @@
+/*
+ * TEST INFRASTRUCTURE ONLY: plain-C restatement of the nodal plastic-yield
+ * update of the reference (rheology/plastic_failure/nodes/PlasticFailureNodes.jl
+ * :24-319 with plastic_failure/utils/{CheckYield,YieldStress,FluidPressure}.jl
+ * and utils/MobileWall.jl:106-122, 279-329).  Conventions as eb_oracle.h.
+ */
+#ifndef EB_PLASTIC_ORACLE_H
+#define EB_PLASTIC_ORACLE_H
+#include "eb_oracle.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int64_t xnum, ynum;
+    const double *gridx_b, *gridy_b;
+    const double *plastic_yield, *etas1, *eta_flow, *mus1, *cohesion_grid, *fric_degrees_grid, *sxy2; /* (ynum, xnum) */
+    const double *sxx2, *pr1;                                                                       /* (ynum-1, xnum-1) */
+    double timestep, timesum;
+    int64_t ntimestep;
+    double viscosity_min, viscosity_max;
+    int64_t iuse_fluid_pressure_for_yield;
+    double y_sealevel;
+    int64_t no_yielding_in_mobile_wall, no_yielding_in_plate_extension;
+    double velocity_internal_x, x_left_mobile_wall, x_right_mobile_wall, y_top_mobile_wall,
+        y_bottom_mobile_wall, plate_extension_thickness, plate_extension_width;
+} ebo_plastic_in;
+
+/* etavp_new, plastic_strain_rate, plastic_yield_new (ynum, xnum) out;
+ * yield_error (ynum, xnum) in/out; returns nnode_yield and global error. */
+void ebo_update_nodes_for_plastic_yielding(const ebo_plastic_in *in, double *etavp_new,
+                                           double *plastic_strain_rate, double *plastic_yield_new,
+                                           double *yield_error, int64_t *nnode_yield,
+                                           double *global_yield_error);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

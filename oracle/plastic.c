@@ -1,0 +1,142 @@
+/* TEST INFRASTRUCTURE ONLY: nodal plastic-yield update, restated from
+ * rheology/plastic_failure/nodes/PlasticFailureNodes.jl:24-319. */
+#include <math.h>
+
+#include "eb_plastic_oracle.h"
+
+/* Julia's max / min on Float64: NaN wins, and +0.0 > -0.0 */
+static double jl_max(double a, double b)
+{
+    if (a != a || b != b) return a + b;
+    if (a == b) return signbit(a) ? b : a;
+    return a > b ? a : b;
+}
+static double jl_min(double a, double b)
+{
+    if (a != a || b != b) return a + b;
+    if (a == b) return signbit(a) ? a : b;
+    return a < b ? a : b;
+}
+
+/* PlasticFailureNodes.jl:222-248 (interpolate_from_pressure_grid_to_basic_grid) */
+static double p2b(int64_t i, int64_t j, int64_t ynum, int64_t xnum, const double *a)
+{
+    const int64_t ld = ynum - 1;
+    if (1 < i && i < ynum && 1 < j && j < xnum)
+        return (EBO_M(a, ld, i - 1, j - 1) + EBO_M(a, ld, i, j - 1) + EBO_M(a, ld, i - 1, j) + EBO_M(a, ld, i, j)) / 4.0;
+    if (i == 1 && j == 1) return EBO_M(a, ld, i, j);
+    if (i == ynum && j == 1) return EBO_M(a, ld, i - 1, j);
+    if (i == ynum && j == xnum) return EBO_M(a, ld, i - 1, j - 1);
+    if (i == 1 && j == xnum) return EBO_M(a, ld, i, j - 1);
+    if (j == 1) return (EBO_M(a, ld, i, j) + EBO_M(a, ld, i - 1, j)) / 2.0;
+    if (j == xnum) return (EBO_M(a, ld, i - 1, j - 1) + EBO_M(a, ld, i, j - 1)) / 2.0;
+    if (i == 1) return (EBO_M(a, ld, i, j) + EBO_M(a, ld, i, j - 1)) / 2.0;
+    if (i == ynum) return (EBO_M(a, ld, i - 1, j) + EBO_M(a, ld, i - 1, j - 1)) / 2.0;
+    return 0.0;
+}
+
+/* YieldStress.jl calc_yield_stress with FluidPressure.calculate_fluid_pressure
+ * (density 1000, gravity 9.81) and stress_yield_tension_method2 */
+static double yield_stress(double cohesion, double sine_fric, double pressure, int64_t use_fluid, double y,
+                           double y_sealevel)
+{
+    const double phi = asin(sine_fric);
+    const double cosphi = cos(phi);
+    double s;
+    if (use_fluid == 0) {
+        s = cohesion * cosphi + sine_fric * pressure;
+    } else {
+        const double depth = jl_max(0.0, y - y_sealevel);
+        const double pf = 1000.0 * 9.81 * depth;
+        if (pressure < pf) s = cohesion * cosphi;
+        else s = cohesion * cosphi + sine_fric * (pressure - pf);
+    }
+    return jl_max(s, 0.0);
+}
+
+void ebo_update_nodes_for_plastic_yielding(const ebo_plastic_in *in, double *etavp_new,
+                                           double *plastic_strain_rate, double *plastic_yield_new,
+                                           double *yield_error, int64_t *nnode_yield_out,
+                                           double *global_yield_error_out)
+{
+    const int64_t xnum = in->xnum, ynum = in->ynum, ld = ynum;
+    const double dt = in->timestep;
+    const double weight = 0.0;
+    const double pi_term = M_PI / 180.0;
+    int64_t nnode_yield = 0;
+    double err_sum = 0.0;
+    for (int64_t k = 0; k < xnum * ynum; ++k) {
+        etavp_new[k] = in->eta_flow[k];
+        plastic_yield_new[k] = 0.0;
+        plastic_strain_rate[k] = 0.0;
+    }
+    for (int64_t j = 1; j <= xnum; ++j)
+        for (int64_t i = 1; i <= ynum; ++i) {
+            const double yield_marker = EBO_M(in->plastic_yield, ld, i, j);
+            const double eta_old = EBO_M(in->etas1, ld, i, j);
+            const double eta_flow = EBO_M(in->eta_flow, ld, i, j);
+            const double mu = EBO_M(in->mus1, ld, i, j);
+            const double cohesion = EBO_M(in->cohesion_grid, ld, i, j);
+            const double fric = sin(EBO_M(in->fric_degrees_grid, ld, i, j) * pi_term);
+            int applicable = in->ntimestep > 0 && (cohesion > 0.0 || fric > 0.0);
+            const double xb = EBO_V(in->gridx_b, j), yb = EBO_V(in->gridy_b, i);
+            if (in->no_yielding_in_mobile_wall) {
+                const double xl = in->x_left_mobile_wall + in->velocity_internal_x * in->timesum;
+                const double xr = in->x_right_mobile_wall + in->velocity_internal_x * in->timesum;
+                if (xl <= xb && xb <= xr && in->y_top_mobile_wall <= yb && yb <= in->y_bottom_mobile_wall)
+                    applicable = 0;
+            }
+            if (in->no_yielding_in_plate_extension) {
+                const double xl = in->x_left_mobile_wall + in->velocity_internal_x * in->timesum;
+                const double xmin = xl - in->plate_extension_width, xmax = xl;
+                const double ymin = in->y_bottom_mobile_wall - in->plate_extension_thickness;
+                const double ymax = in->y_bottom_mobile_wall;
+                if (xmin <= xb && xb <= xmax && ymin <= yb && yb <= ymax) applicable = 0;
+            }
+            double *ev = &EBO_M(etavp_new, ld, i, j), *py = &EBO_M(plastic_yield_new, ld, i, j);
+            if (!applicable) {
+                *ev = eta_flow;
+                *py = 0.0;
+                continue;
+            }
+            const double pressure = p2b(i, j, ynum, xnum, in->pr1);
+            const double sxx = p2b(i, j, ynum, xnum, in->sxx2);
+            const double sxy = EBO_M(in->sxy2, ld, i, j);
+            const double sii = sqrt(sxy * sxy + sxx * sxx);
+            const double sii_e = sii * (mu * dt + eta_old) / eta_old;
+            const double sy = yield_stress(cohesion, fric, pressure, in->iuse_fluid_pressure_for_yield, yb,
+                                           in->y_sealevel);
+            const int at_yield = yield_marker > 0;
+            if (at_yield) {
+                const double e = sii - sy;
+                err_sum = err_sum + e * e;
+                EBO_M(yield_error, ld, i, j) = e;
+                ++nnode_yield;
+            }
+            if (sy < sii_e) {
+                double eta_new = dt * mu * sy / (sii_e - sy);
+                if (eta_new < eta_flow) {
+                    *py = 1.0;
+                    eta_new = pow(eta_new, 1.0 - weight) * pow(eta_old, weight);
+                    eta_new = jl_max(eta_new, in->viscosity_min);
+                    eta_new = jl_min(eta_new, in->viscosity_max);
+                    *ev = eta_new;
+                    EBO_M(plastic_strain_rate, ld, i, j) = 0.5 * sii / eta_new - 0.5 * sii / eta_flow;
+                    if (!at_yield) {
+                        const double e = sii - sy;
+                        err_sum = err_sum + e * e;
+                        EBO_M(yield_error, ld, i, j) = e;
+                        ++nnode_yield;
+                    }
+                } else {
+                    *ev = eta_flow;
+                    *py = 0.0;
+                }
+            } else {
+                *ev = eta_flow;
+                *py = 0.0;
+            }
+        }
+    *nnode_yield_out = nnode_yield;
+    *global_yield_error_out = nnode_yield > 0 ? sqrt(err_sum / (double)nnode_yield) : 0.0;
+}

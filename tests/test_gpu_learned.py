@@ -115,3 +115,30 @@ def test_learned_fallback_after_rejections(gpu_ctx, patience):
     else:
         assert st["learned_evaluated"] - st["learned_accepted"] >= patience
         assert st["learned_evaluated"] < st["iterations"]
+
+
+def test_committed_network_gain_on_held_out_system(gpu_ctx):
+    """What the committed network gains: a held-out layered system (seed 41,
+    401 x 201 nodes) solved to 1e-8 with the FP32 V-cycle alone and with the
+    network judged by the FP64 residual (mode 1).  Both reach the tolerance;
+    the run records the iteration counts and the judge's verdicts.  With the
+    current network the judge accepts no correction, so the fallback makes
+    the learned solve take the plain solve's iterations."""
+    if not learned.WEIGHTS_FILE.exists():
+        pytest.skip("no committed network weights")
+    print("learned gain, held-out seed 41, 401 x 201", flush=True)
+    prob = workloads.layered_random(401, 201, 6.0e5, 2.4e5, seed=41)
+    sol = StokesSolverGPU(gpu_ctx)
+    sol.set_system(prob)
+    bcd = bc_mirror_factors(prob)
+    net = learned.LearnedCorrection(gpu_ctx)
+    net.load()
+    _, plain = sol.solve_system(precision=1, rtol=1e-8, max_iterations=2000, bc_d=bcd)
+    _, lrn = sol.solve_system(precision=1, rtol=1e-8, max_iterations=2000, learned=1, bc_d=bcd)
+    print(f"plain FP32 V-cycle: {plain['iterations']} iterations; with the network: "
+          f"{lrn['iterations']} iterations, {lrn['learned_accepted']} of "
+          f"{lrn['learned_evaluated']} corrections accepted", flush=True)
+    assert plain["converged"] and lrn["converged"]
+    assert lrn["learned_evaluated"] >= 1
+    if lrn["learned_accepted"] == 0:
+        assert lrn["iterations"] == plain["iterations"]
