@@ -40,6 +40,7 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_interpolate_marker_stokes_parameters_to_grid": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_stokes_viscoelastic_rhs": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_update_nodes_for_plastic_yielding": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ebg_grid_strain_rate_and_stress": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_stokes_set_system": (c_int, [c_void_p] + [c_void_p] * 16 + [c_int, c_double, c_double,
                                                                      c_double, c_double, c_int]),
     "ebg_stokes_apply": (c_int, [c_void_p, c_void_p, c_void_p]),
@@ -108,6 +109,17 @@ class ViscoelasticIn(C.Structure):
 class ViscoelasticOut(C.Structure):
     _fields_ = [("etas0", c_void_p), ("sxy0", c_void_p), ("etan0", c_void_p), ("sxx0", c_void_p),
                 ("RX1", c_void_p), ("RY1", c_void_p), ("RC1", c_void_p)]
+
+
+class StrainStressIn(C.Structure):
+    _fields_ = [("vx1", c_void_p), ("vy1", c_void_p), ("etas1", c_void_p), ("mus1", c_void_p),
+                ("sxy1", c_void_p), ("etan1", c_void_p), ("mun1", c_void_p), ("sxx1", c_void_p),
+                ("timestep", c_double)]
+
+
+class StrainStressOut(C.Structure):
+    _fields_ = [("exx", c_void_p), ("exy", c_void_p), ("eii", c_void_p), ("sxy2", c_void_p),
+                ("sxx2", c_void_p), ("dsxy", c_void_p), ("dsxx", c_void_p)]
 
 
 class PlasticNodesIn(C.Structure):

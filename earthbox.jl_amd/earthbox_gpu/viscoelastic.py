@@ -1,9 +1,12 @@
-"""Viscoelastic terms and right-hand sides of the Stokes system (host side).
+"""Viscoelastic terms before the Stokes solve and grid strain rates and
+stresses after it (host side).
 
 calc_viscoelastic_terms_and_rhs mirrors the reference's back-to-back calls
 StokesViscoelasticTerms.calc_viscoelastic_terms!(model) and
-StokesContinuityRhs.viscoelastic_rhs!(model) (StokesContinuitySolver.jl:189-190)
-on device arrays.
+StokesContinuityRhs.viscoelastic_rhs!(model) (StokesContinuitySolver.jl:189-190);
+update_grid_strain_rate_and_stress mirrors the grid part of
+stokes_solution_processing! (GridStrainRate.jl:33-159, GridStress.jl), on
+device arrays.
 """
 from __future__ import annotations
 
@@ -12,7 +15,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from ._lib import ViscoelasticIn, ViscoelasticOut, check, lib
+from ._lib import StrainStressIn, StrainStressOut, ViscoelasticIn, ViscoelasticOut, check, lib
 from .context import GPUContext
 from .device import empty, ptr
 
@@ -53,4 +56,33 @@ def calc_viscoelastic_terms_and_rhs(ctx: GPUContext, fields: dict, timestep: flo
         s = _shape(kind, nx, ny)
         setattr(o, k, ptr(out[k], np.float64, s[0] * s[1]))
     check(lib.ebg_stokes_viscoelastic_rhs(ctx.handle, C.byref(a), C.byref(o)), "ebg_stokes_viscoelastic_rhs")
+    return out
+
+
+STRAIN_IN = {"vx1": "vx", "vy1": "vy", "etas1": "b", "mus1": "b", "sxy1": "b", "etan1": "p", "mun1": "p",
+             "sxx1": "p"}
+STRAIN_OUT = {"exx": "p", "exy": "b", "eii": "p", "sxy2": "b", "sxx2": "p", "dsxy": "b", "dsxx": "p"}
+
+
+def update_grid_strain_rate_and_stress(ctx: GPUContext, fields: dict, timestep: float,
+                                       out: dict | None = None) -> dict:
+    """fields: device arrays keyed like STRAIN_IN; returns (or fills) the
+    outputs keyed like STRAIN_OUT."""
+    g = ctx.grid
+    if g is None:
+        raise RuntimeError("set_grid() must be called first")
+    nx, ny = g.xnum, g.ynum
+    a = StrainStressIn()
+    for k, kind in STRAIN_IN.items():
+        s = _shape(kind, nx, ny)
+        setattr(a, k, ptr(fields[k], np.float64, s[0] * s[1]))
+    a.timestep = float(timestep)
+    if out is None:
+        out = {k: empty(_shape(kind, nx, ny), np.float64) for k, kind in STRAIN_OUT.items()}
+    o = StrainStressOut()
+    for k, kind in STRAIN_OUT.items():
+        s = _shape(kind, nx, ny)
+        setattr(o, k, ptr(out[k], np.float64, s[0] * s[1]))
+    check(lib.ebg_grid_strain_rate_and_stress(ctx.handle, C.byref(a), C.byref(o)),
+          "ebg_grid_strain_rate_and_stress")
     return out

@@ -228,6 +228,49 @@ function calc_viscoelastic_terms_and_rhs_gpu!(st::GPUState, model)
     return nothing
 end
 
+"Field order of ebg_strain_stress_in / ebg_strain_stress_out (include/earthbox_gpu.h)."
+struct StrainStressIn
+    vx1::Ptr{Cvoid}; vy1::Ptr{Cvoid}
+    etas1::Ptr{Cvoid}; mus1::Ptr{Cvoid}; sxy1::Ptr{Cvoid}
+    etan1::Ptr{Cvoid}; mun1::Ptr{Cvoid}; sxx1::Ptr{Cvoid}
+    timestep::Cdouble
+end
+struct StrainStressOut
+    exx::Ptr{Cvoid}; exy::Ptr{Cvoid}; eii::Ptr{Cvoid}
+    sxy2::Ptr{Cvoid}; sxx2::Ptr{Cvoid}; dsxy::Ptr{Cvoid}; dsxx::Ptr{Cvoid}
+end
+
+"""
+    update_grid_strain_rate_and_stress_gpu!(st, model)
+
+GPU replacement of the grid part of stokes_solution_processing!
+(StokesContinuitySolver.jl:221, 273-274): GridStrainRate's exx, exy and eii,
+GridStress.forecast_viscoelastic_grid_stress! (sxy2, sxx2) and the stress
+change (dsxy, dsxx).  Spin: ebg_calculate_spin.
+"""
+function update_grid_strain_rate_and_stress_gpu!(st::GPUState, model)
+    set_grid!(st, model)
+    a = model.stokes_continuity.arrays
+    up(k, x) = h2d!(st, k, x)
+    sin_ = StrainStressIn(up(:vx1, a.staggered_grid_velocity.vx1.array), up(:vy1, a.staggered_grid_velocity.vy1.array),
+        up(:etas1, a.viscosity.etas1.array), up(:mus1, a.shear_modulus.mus1.array), up(:sxy1, a.stress.sxy1.array),
+        up(:etan1, a.viscosity.etan1.array), up(:mun1, a.shear_modulus.mun1.array), up(:sxx1, a.stress.sxx1.array),
+        model.timestep.parameters.main_time_loop.timestep.value)
+    sr = a.strain_rate_and_spin
+    outs = (sr.exx.array, sr.exy.array, sr.eii.array, a.stress.sxy2.array, a.stress.sxx2.array,
+            a.stress_change.dsxy.array, a.stress_change.dsxx.array)
+    keys = (:exx, :exy, :eii, :sxy2, :sxx2, :dsxy, :dsxx)
+    sout = StrainStressOut((dev!(st, k, x) for (k, x) in zip(keys, outs))...)
+    check(ccall((:ebg_grid_strain_rate_and_stress, LIB), Cint,
+        (Ptr{Cvoid}, Ref{StrainStressIn}, Ref{StrainStressOut}), st.ctx, sin_, sout),
+        "ebg_grid_strain_rate_and_stress")
+    check(ccall((:ebg_ctx_synchronize, LIB), Cint, (Ptr{Cvoid},), st.ctx), "sync")
+    for (k, x) in zip(keys, outs)
+        d2h!(x, st.buffers[k])
+    end
+    return nothing
+end
+
 "Field order of ebg_plastic_nodes_in / ebg_plastic_nodes_out (include/earthbox_gpu.h)."
 struct PlasticNodesIn
     plastic_yield::Ptr{Cvoid}; etas1::Ptr{Cvoid}; eta_flow::Ptr{Cvoid}; mus1::Ptr{Cvoid}

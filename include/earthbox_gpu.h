@@ -182,6 +182,30 @@ typedef struct {
  * device math library. */
 int ebg_stokes_viscoelastic_rhs(ebg_ctx *ctx, const ebg_viscoelastic_in *in, ebg_viscoelastic_out *out);
 
+/* ---- grid strain rate, forecast stress and stress change ----
+ * Replaces GridStrainRate.calculate_normal_strain_rate!, calculate_shear_strain_rate!
+ * and calculate_second_invariant! (stokes_continuity_solver/utils/strain_rate/
+ * GridStrainRate.jl:33-159), GridStress.forecast_viscoelastic_grid_stress! and
+ * calculate_deviatoric_grid_stress_change! (stokes_continuity_solver/utils/
+ * GridStress.jl), the grid part of stokes_solution_processing!
+ * (StokesContinuitySolver.jl:212-279); spin is ebg_calculate_spin.  DEVICE
+ * arrays: in vx1 (ynum+1,xnum), vy1 (ynum,xnum+1), etas1, mus1, sxy1
+ * (ynum,xnum), etan1, mun1, sxx1 (ynum-1,xnum-1); out exy, sxy2, dsxy
+ * (ynum,xnum), exx, eii, sxx2, dsxx (ynum-1,xnum-1).  Bit-identical. */
+typedef struct {
+    const double *vx1, *vy1;
+    const double *etas1, *mus1, *sxy1;
+    const double *etan1, *mun1, *sxx1;
+    double timestep;
+} ebg_strain_stress_in;
+
+typedef struct {
+    double *exx, *exy, *eii;
+    double *sxy2, *sxx2, *dsxy, *dsxx;
+} ebg_strain_stress_out;
+
+int ebg_grid_strain_rate_and_stress(ebg_ctx *ctx, const ebg_strain_stress_in *in, ebg_strain_stress_out *out);
+
 /* ---- nodal plastic yielding (Picard loop over nodes) ----
  * DEVICE grid arrays, reference shapes: plastic_yield, etas1, eta_flow, mus1,
  * cohesion_grid, fric_degrees_grid, sxy2 (ynum,xnum); sxx2, pr1 (ynum-1,xnum-1).

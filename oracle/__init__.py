@@ -251,6 +251,35 @@ def update_nodes_for_plastic_yielding(g, f: dict, par: dict, yield_error):
     return ev, psr, pyn, ye, int(n.value), float(err.value)
 
 
+class _StrainIn(C.Structure):
+    _fields_ = [("xnum", _I64), ("ynum", _I64), ("xstp_b", _P), ("ystp_b", _P), ("xstp_vy", _P),
+                ("ystp_vx", _P), ("vx1", _P), ("vy1", _P), ("etas1", _P), ("mus1", _P), ("sxy1", _P),
+                ("etan1", _P), ("mun1", _P), ("sxx1", _P), ("timestep", C.c_double)]
+
+
+def grid_strain_rate_and_stress(g, vx1, vy1, f: dict, timestep):
+    """GridStrainRate.jl:33-159 (exx, exy, eii), GridStress.forecast_viscoelastic_grid_stress!
+    (sxy2, sxx2) and calculate_deviatoric_grid_stress_change! (dsxy, dsxx).  f
+    holds etas1, mus1, sxy1 (ynum, xnum) and etan1, mun1, sxx1 (ynum-1, xnum-1)."""
+    L = lib()
+    L.ebo_grid_strain_rate_and_stress.restype = None
+    L.ebo_grid_strain_rate_and_stress.argtypes = [C.POINTER(_StrainIn)] + [_P] * 7
+    ax = [np.ascontiguousarray(v, dtype=np.float64) for v in (g.xstp_b, g.ystp_b, g.xstp_vy, g.ystp_vx)]
+    a = {k: np.asfortranarray(f[k], dtype=np.float64) for k in ("etas1", "mus1", "sxy1", "etan1", "mun1", "sxx1")}
+    v1 = np.asfortranarray(vx1, dtype=np.float64)
+    v2 = np.asfortranarray(vy1, dtype=np.float64)
+    s = _StrainIn(g.xnum, g.ynum, *[p(v) for v in ax], p(v1), p(v2), *[p(a[k]) for k in
+                  ("etas1", "mus1", "sxy1", "etan1", "mun1", "sxx1")], float(timestep))
+    ny, nx = g.ynum, g.xnum
+    out = {"exx": np.zeros((ny - 1, nx - 1), order="F"), "exy": np.zeros((ny, nx), order="F"),
+           "eii": np.zeros((ny - 1, nx - 1), order="F"), "sxy2": np.zeros((ny, nx), order="F"),
+           "sxx2": np.zeros((ny - 1, nx - 1), order="F"), "dsxy": np.zeros((ny, nx), order="F"),
+           "dsxx": np.zeros((ny - 1, nx - 1), order="F")}
+    L.ebo_grid_strain_rate_and_stress(C.byref(s), *[p(out[k]) for k in ("exx", "exy", "eii", "sxy2", "sxx2",
+                                                                      "dsxy", "dsxx")])
+    return out
+
+
 def stokes_unpack(prob, S):
     """K10: (vx1, vy1, pr1) from the solution vector with ghost BCs applied."""
     L = lib()

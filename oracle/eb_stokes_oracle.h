@@ -42,6 +42,20 @@ typedef struct {
     int64_t iuse_interface_stabilization;
 } ebo_rhs_in;
 
+/* Inputs of the grid strain rate and forecast stress (GridStrainRate.jl,
+ * GridStress.jl). */
+typedef struct {
+    int64_t xnum, ynum;
+    const double *xstp_b, *ystp_b, *xstp_vy, *ystp_vx;
+    const double *vx1, *vy1;                /* (ynum+1, xnum), (ynum, xnum+1) */
+    const double *etas1, *mus1, *sxy1;      /* (ynum, xnum) */
+    const double *etan1, *mun1, *sxx1;      /* (ynum-1, xnum-1) */
+    double timestep;
+} ebo_strain_in;
+
+void ebo_grid_strain_rate_and_stress(const ebo_strain_in *s, double *exx, double *exy, double *eii,
+                                     double *sxy2, double *sxx2, double *dsxy, double *dsxx);
+
 int64_t ebo_stokes_build(const ebo_stokes_in *s, int64_t *Li, int64_t *Lj, double *Lv, double *R);
 double ebo_pressure_scaling_coeff(double etan11, double xstpavg, double ystpavg);
 void ebo_stokes_rhs(const ebo_rhs_in *r, double *etas0, double *sxy0, double *etan0,
