@@ -192,6 +192,47 @@ __global__ void __launch_bounds__(PL_THREADS) plastic_finish_kernel(const double
 
 }  // namespace
 
+// HarmonicAverage.normal_viscosity_from_harmonic_avg_shear_viscosity! with
+// CheckPlasticYield.plastic_yielding (tolerance 0) and MathTools.harmonic4
+// (factors 0, summed in the reference's order)
+__global__ void harmonic_normal_viscosity_kernel(int xnum, int ynum, const double* __restrict__ etas1,
+                                                 const double* __restrict__ plastics,
+                                                 const double* __restrict__ plasticn,
+                                                 const double* __restrict__ pyield, double* __restrict__ etan1)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ny = ynum;
+    if (c >= (int64_t)(xnum - 1) * (ynum - 1)) return;
+    const int i = (int)(c % (ny - 1)) + 1, j = (int)(c / (ny - 1)) + 1;
+#define BA(a, ii, jj) (a)[(size_t)((jj) - 1) * ny + ((ii) - 1)]
+    bool y = plasticn[c] > 0.0;
+    if (BA(plastics, i, j) > 0.0 || BA(plastics, i, j + 1) > 0.0 || BA(plastics, i + 1, j) > 0.0 ||
+        BA(plastics, i + 1, j + 1) > 0.0)
+        y = true;
+    if (BA(pyield, i, j) > 0.0 || BA(pyield, i, j + 1) > 0.0 || BA(pyield, i + 1, j) > 0.0 ||
+        BA(pyield, i + 1, j + 1) > 0.0)
+        y = true;
+    if (!y) return;
+    const double vinv = 1.0 / BA(etas1, i, j) + 1.0 / BA(etas1, i, j + 1) + 1.0 / BA(etas1, i + 1, j) +
+                        1.0 / BA(etas1, i + 1, j + 1) + 0.0 + 0.0 + 0.0 + 0.0;
+    const double den = 4.0 + 0.0 + 0.0 + 0.0 + 0.0;
+    if (vinv != 0.0 && den != 0.0) etan1[c] = 1.0 / (vinv / den);
+#undef BA
+}
+
+// PlasticStrainRateInterpolation.interpolate_at_pressure_nodes!
+__global__ void plastic_rate_pressure_kernel(int xnum, int ynum, const double* __restrict__ eb,
+                                             double* __restrict__ ep)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ny = ynum;
+    if (c >= (int64_t)(xnum - 1) * (ynum - 1)) return;
+    const int i = (int)(c % (ny - 1)) + 1, j = (int)(c / (ny - 1)) + 1;
+#define BA(ii, jj) eb[(size_t)((jj) - 1) * ny + ((ii) - 1)]
+    ep[c] = (BA(i, j) + BA(i, j + 1) + BA(i + 1, j) + BA(i + 1, j + 1)) / 4.0;
+#undef BA
+}
+
 // per-context scratch: nb partial sums, nb partial counts, two results
 struct PlasticScratch {
     double* buf = nullptr;
@@ -259,5 +300,33 @@ extern "C" int ebg_update_nodes_for_plastic_yielding(ebg_ctx* ctx, const ebg_pla
     EBG_HIP(hipStreamSynchronize(s));
     out->nnode_yield = (int64_t)h[0];
     out->global_yield_error = h[1];
+    return EBG_OK;
+}
+
+extern "C" int ebg_harmonic_normal_viscosity(ebg_ctx* ctx, const double* etas1, const double* plastics,
+                                             const double* plasticn, const double* plastic_yield, double* etan1)
+{
+    EBG_CHECK(ctx && ctx->has_grid, EBG_ERR_STATE, "ebg_harmonic_normal_viscosity: no grid");
+    EBG_CHECK(etas1 && plastics && plasticn && plastic_yield && etan1, EBG_ERR_ARG,
+              "ebg_harmonic_normal_viscosity: NULL field");
+    const Grid2D& g = ctx->grid;
+    const int64_t np = (g.xnum - 1) * (g.ynum - 1);
+    hipLaunchKernelGGL(harmonic_normal_viscosity_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (int)g.xnum, (int)g.ynum, etas1, plastics, plasticn, plastic_yield, etan1);
+    EBG_HIP(hipGetLastError());
+    return EBG_OK;
+}
+
+extern "C" int ebg_plastic_strain_rate_at_pressure_nodes(ebg_ctx* ctx, const double* eii_plastic_basic,
+                                                         double* eii_plastic_pressure)
+{
+    EBG_CHECK(ctx && ctx->has_grid, EBG_ERR_STATE, "ebg_plastic_strain_rate_at_pressure_nodes: no grid");
+    EBG_CHECK(eii_plastic_basic && eii_plastic_pressure, EBG_ERR_ARG,
+              "ebg_plastic_strain_rate_at_pressure_nodes: NULL field");
+    const Grid2D& g = ctx->grid;
+    const int64_t np = (g.xnum - 1) * (g.ynum - 1);
+    hipLaunchKernelGGL(plastic_rate_pressure_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (int)g.xnum, (int)g.ynum, eii_plastic_basic, eii_plastic_pressure);
+    EBG_HIP(hipGetLastError());
     return EBG_OK;
 }

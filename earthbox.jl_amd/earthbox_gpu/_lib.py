@@ -41,6 +41,8 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_stokes_viscoelastic_rhs": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_update_nodes_for_plastic_yielding": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_grid_strain_rate_and_stress": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ebg_harmonic_normal_viscosity": (c_int, [c_void_p] * 6),
+    "ebg_plastic_strain_rate_at_pressure_nodes": (c_int, [c_void_p] * 3),
     "ebg_stokes_set_system": (c_int, [c_void_p] + [c_void_p] * 16 + [c_int, c_double, c_double,
                                                                      c_double, c_double, c_int]),
     "ebg_stokes_apply": (c_int, [c_void_p, c_void_p, c_void_p]),

@@ -1,0 +1,98 @@
+"""The Picard (global plasticity) loop over nodes, on device arrays.
+
+picard_loop_nodes mirrors PlasticityLoopNodes.picard_loop
+(global_plasticity_loop/plasticity_loop/plasticity_loop_nodes/PlasticityLoopNodes.jl:36-92):
+each iteration interpolates the viscoplastic viscosity to the pressure grid
+(after the first), forms the viscoelastic terms and right-hand sides, solves
+the Stokes system, processes the solution on the grid (strain rates, forecast
+stresses, change norms), updates the nodes for plastic yielding, copies the
+new viscosity, yield flags and plastic strain rate back when nodes yield, and
+stops by LoopTermination.check_stop_global_loop_nodes
+(global_plasticity_loop/utils/LoopTermination.jl:16-31).
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+import torch
+
+from .device import empty
+from .plastic import (interpolate_basic_viscoplastic_viscosity_to_pressure_grid,
+                      plastic_strain_rate_at_pressure_nodes, update_nodes_for_plastic_yielding)
+from .stokes import StokesSolverGPU, bc_mirror_factors, change_norms
+from .stokes_problem import StokesProblem
+from .viscoelastic import calc_viscoelastic_terms_and_rhs, update_grid_strain_rate_and_stress
+
+STATE = ("etas1", "eta_flow", "mus1", "sxy1", "rho1", "rho1_vy", "etan1", "mun1", "sxx1", "plastics",
+         "plasticn", "plastic_yield", "cohesion_grid", "fric_degrees_grid", "dilatation_grid",
+         "eii_plastic_pressure", "yield_error")
+
+
+def check_stop_global_loop_nodes(nnode_yield, iglobal, nglobal, global_yield_error, convergence_criterion,
+                                 tolerance) -> bool:
+    """LoopTermination.check_stop_global_loop_nodes (LoopTermination.jl:16-31)."""
+    return (nnode_yield == 0 or iglobal == nglobal or global_yield_error < tolerance
+            or convergence_criterion < tolerance)
+
+
+def picard_loop_nodes(ctx, template: StokesProblem, state: dict, plastic_params: dict, nglobal: int,
+                      tolerance: float, gravity_x: float = 0.0, iuse_harmonic_avg_normal_viscosity: int = 1,
+                      no_yielding_in_mobile_wall: bool = False, no_yielding_in_plate_extension: bool = False,
+                      solve_opts: dict | None = None, solver: StokesSolverGPU | None = None):
+    """template: the system's grid, velocity boundary conditions, pressure BC,
+    gravity and interface-stabilization flag (its fields are replaced each
+    iteration); state: device arrays keyed like STATE, updated in place as the
+    reference updates ModelData; plastic_params: the scalars of
+    update_nodes_for_plastic_yielding (timestep included).  Returns
+    (solution dict with vx1, vy1, pr1, exx, exy, eii, sxy2, sxx2, dsxy, dsxx;
+    per-iteration log)."""
+    g = template.grid
+    solver = solver or StokesSolverGPU(ctx)
+    opts = dict(precision=1, rtol=1e-10, max_iterations=4000, bc_d=bc_mirror_factors(template))
+    opts.update(solve_opts or {})
+    dt = float(plastic_params["timestep"])
+    ny, nx = g.ynum, g.xnum
+    eii_basic = empty((ny, nx), np.float64)
+    vx_old = vy_old = None
+    log = []
+    sol = {}
+    for iglobal in range(1, nglobal + 1):
+        if iglobal > 1:
+            interpolate_basic_viscoplastic_viscosity_to_pressure_grid(
+                ctx, state["etas1"], state["plastics"], state["plasticn"], state["plastic_yield"], state["etan1"],
+                iuse_harmonic_avg_normal_viscosity)
+        # StokesContinuitySolver.solve_viscoelastic_stokes_continuity_equations!
+        ve = calc_viscoelastic_terms_and_rhs(ctx, state, dt, gravity_x, template.gravity_y,
+                                             template.iuse_interface_stabilization)
+        etan11 = float(ve["etan0"][0, 0].item())
+        prob = dataclasses.replace(template, pscale=2.0 * etan11 / (g.xstpavg + g.ystpavg), timestep=dt)
+        solver.set_system(prob, fields={"etas0": ve["etas0"], "etan0": ve["etan0"], "rho1_vy": state["rho1_vy"],
+                                        "RX1": ve["RX1"], "RY1": ve["RY1"], "RC1": ve["RC1"]})
+        S, st = solver.solve_system(**opts)
+        vx1, vy1, pr1 = solver.process_stokes_solution(S)
+        # stokes_solution_processing! on the grid
+        ss = update_grid_strain_rate_and_stress(ctx, {"vx1": vx1, "vy1": vy1, **state}, dt)
+        dvxy_rel_L2 = change_norms(ctx, [vx1, vy1], [vx_old, vy_old])[1] if vx_old is not None else 1e38
+        vx_old, vy_old = vx1.clone(), vy1.clone()
+        # PlasticFailureNodes.update_nodes_for_plastic_yielding
+        fields = {k: state[k] for k in ("plastic_yield", "etas1", "eta_flow", "mus1", "cohesion_grid",
+                                        "fric_degrees_grid")}
+        fields.update(sxy2=ss["sxy2"], sxx2=ss["sxx2"], pr1=pr1)
+        out, nnode_yield, global_yield_error = update_nodes_for_plastic_yielding(
+            ctx, fields, state["yield_error"], plastic_params, no_yielding_in_mobile_wall,
+            no_yielding_in_plate_extension)
+        if nnode_yield > 0:
+            state["etas1"].copy_(out["etavp_new"])
+            state["plastic_yield"].copy_(out["plastic_yield_new"])
+            eii_basic.copy_(out["plastic_strain_rate"])
+            plastic_strain_rate_at_pressure_nodes(ctx, eii_basic, state["eii_plastic_pressure"])
+        log.append({"iglobal": iglobal, "iterations": st["iterations"], "relres": st["relres"],
+                    "nnode_yield": nnode_yield, "global_yield_error": global_yield_error,
+                    "dvxy_rel_L2": dvxy_rel_L2})
+        sol = {"vx1": vx1, "vy1": vy1, "pr1": pr1, **ss}
+        if check_stop_global_loop_nodes(nnode_yield, iglobal, nglobal, global_yield_error, dvxy_rel_L2,
+                                        tolerance):
+            break
+    torch.cuda.synchronize()
+    return sol, log

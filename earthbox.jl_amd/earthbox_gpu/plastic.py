@@ -49,3 +49,28 @@ def update_nodes_for_plastic_yielding(ctx: GPUContext, fields: dict, yield_error
     check(lib.ebg_update_nodes_for_plastic_yielding(ctx.handle, C.byref(a), C.byref(o)),
           "ebg_update_nodes_for_plastic_yielding")
     return out, int(o.nnode_yield), float(o.global_yield_error)
+
+
+def interpolate_basic_viscoplastic_viscosity_to_pressure_grid(ctx: GPUContext, etas1, plastics, plasticn,
+                                                              plastic_yield, etan1,
+                                                              iuse_harmonic_avg_normal_viscosity: int = 1):
+    """Interpolation.interpolate_basic_viscoplastic_viscosity_to_pressure_grid!
+    (Interpolation.jl:97-109): etan1 updated in place when the harmonic option is on."""
+    if iuse_harmonic_avg_normal_viscosity != 1:
+        return etan1
+    g = ctx.grid
+    nb, npr = g.xnum * g.ynum, (g.xnum - 1) * (g.ynum - 1)
+    check(lib.ebg_harmonic_normal_viscosity(ctx.handle, ptr(etas1, np.float64, nb), ptr(plastics, np.float64, nb),
+                                            ptr(plasticn, np.float64, npr), ptr(plastic_yield, np.float64, nb),
+                                            ptr(etan1, np.float64, npr)), "ebg_harmonic_normal_viscosity")
+    return etan1
+
+
+def plastic_strain_rate_at_pressure_nodes(ctx: GPUContext, eii_plastic_basic, eii_plastic_pressure):
+    """PlasticStrainRateInterpolation.interpolate_at_pressure_nodes!."""
+    g = ctx.grid
+    nb, npr = g.xnum * g.ynum, (g.xnum - 1) * (g.ynum - 1)
+    check(lib.ebg_plastic_strain_rate_at_pressure_nodes(ctx.handle, ptr(eii_plastic_basic, np.float64, nb),
+                                                        ptr(eii_plastic_pressure, np.float64, npr)),
+          "ebg_plastic_strain_rate_at_pressure_nodes")
+    return eii_plastic_pressure

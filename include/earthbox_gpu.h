@@ -240,6 +240,22 @@ typedef struct {
 int ebg_update_nodes_for_plastic_yielding(ebg_ctx *ctx, const ebg_plastic_nodes_in *in,
                                           ebg_plastic_nodes_out *out);
 
+/* Replaces Interpolation.interpolate_basic_viscoplastic_viscosity_to_pressure_grid!
+ * (interpolation/Interpolation.jl:97-109) with iuse_harmonic_avg_normal_viscosity = 1,
+ * i.e. HarmonicAverage.normal_viscosity_from_harmonic_avg_shear_viscosity!: etan1
+ * (ynum-1,xnum-1, DEVICE, updated) becomes the harmonic mean of the four etas1
+ * around a cell where plasticn, or plastics or plastic_yield at one of its
+ * corners, is positive (MathTools.harmonic4, utils/MathTools.jl:42-62).
+ * Bit-identical. */
+int ebg_harmonic_normal_viscosity(ebg_ctx *ctx, const double *etas1, const double *plastics,
+                                  const double *plasticn, const double *plastic_yield, double *etan1);
+/* Replaces PlasticStrainRateInterpolation.interpolate_at_pressure_nodes!
+ * (interpolation/utils/stokes_funcs/PlasticStrainRateInterpolation.jl:18-41):
+ * eii_plastic_pressure (ynum-1,xnum-1) = mean of the four eii_plastic_basic
+ * corners.  Bit-identical. */
+int ebg_plastic_strain_rate_at_pressure_nodes(ebg_ctx *ctx, const double *eii_plastic_basic,
+                                              double *eii_plastic_pressure);
+
 /* ---- Stokes-continuity system (assembled-operator conventions) ----
  * Replaces the inputs StokesBuildData gathers (solver/build_system/utils/
  * BuildStructs.jl:37-177).  Fields are DEVICE pointers with the reference

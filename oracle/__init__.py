@@ -280,6 +280,28 @@ def grid_strain_rate_and_stress(g, vx1, vy1, f: dict, timestep):
     return out
 
 
+def harmonic_normal_viscosity(g, etas1, plastics, plasticn, plastic_yield, etan1):
+    """HarmonicAverage.normal_viscosity_from_harmonic_avg_shear_viscosity!: etan1
+    (ynum-1, xnum-1) updated in a copy where a neighbouring node yields."""
+    L = lib()
+    L.ebo_harmonic_normal_viscosity.restype = None
+    L.ebo_harmonic_normal_viscosity.argtypes = [_I64, _I64] + [_P] * 5
+    a = [np.asfortranarray(v, dtype=np.float64) for v in (etas1, plastics, plasticn, plastic_yield)]
+    out = np.asfortranarray(np.array(etan1, dtype=np.float64, copy=True))
+    L.ebo_harmonic_normal_viscosity(g.xnum, g.ynum, *[p(v) for v in a], p(out))
+    return out
+
+
+def plastic_strain_rate_at_pressure_nodes(g, eii_basic):
+    L = lib()
+    L.ebo_plastic_strain_rate_at_pressure_nodes.restype = None
+    L.ebo_plastic_strain_rate_at_pressure_nodes.argtypes = [_I64, _I64, _P, _P]
+    a = np.asfortranarray(eii_basic, dtype=np.float64)
+    out = np.zeros((g.ynum - 1, g.xnum - 1), order="F")
+    L.ebo_plastic_strain_rate_at_pressure_nodes(g.xnum, g.ynum, p(a), p(out))
+    return out
+
+
 def stokes_unpack(prob, S):
     """K10: (vx1, vy1, pr1) from the solution vector with ghost BCs applied."""
     L = lib()

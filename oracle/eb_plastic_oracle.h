@@ -34,6 +34,11 @@ void ebo_update_nodes_for_plastic_yielding(const ebo_plastic_in *in, double *eta
                                            double *yield_error, int64_t *nnode_yield,
                                            double *global_yield_error);
 
+void ebo_harmonic_normal_viscosity(int64_t xnum, int64_t ynum, const double *etas1, const double *plastics,
+                                   const double *plasticn, const double *plastic_yield, double *etan1);
+void ebo_plastic_strain_rate_at_pressure_nodes(int64_t xnum, int64_t ynum, const double *eii_basic,
+                                               double *eii_pressure);
+
 #ifdef __cplusplus
 }
 #endif

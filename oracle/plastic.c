@@ -140,3 +140,42 @@ void ebo_update_nodes_for_plastic_yielding(const ebo_plastic_in *in, double *eta
     *nnode_yield_out = nnode_yield;
     *global_yield_error_out = nnode_yield > 0 ? sqrt(err_sum / (double)nnode_yield) : 0.0;
 }
+
+/* HarmonicAverage.normal_viscosity_from_harmonic_avg_shear_viscosity!
+ * (interpolation/utils/stokes_funcs/HarmonicAverage.jl) with
+ * CheckPlasticYield.plastic_yielding (tolerance 0) and MathTools.harmonic4
+ * (utils/MathTools.jl:42-62, factors 0) */
+void ebo_harmonic_normal_viscosity(int64_t xnum, int64_t ynum, const double *etas1, const double *plastics,
+                                   const double *plasticn, const double *plastic_yield, double *etan1)
+{
+    const int64_t ny = ynum;
+    for (int64_t i = 1; i <= ynum - 1; ++i)
+        for (int64_t j = 1; j <= xnum - 1; ++j) {
+            int y = EBO_M(plasticn, ny - 1, i, j) > 0.0;
+            if (EBO_M(plastics, ny, i, j) > 0.0 || EBO_M(plastics, ny, i, j + 1) > 0.0 ||
+                EBO_M(plastics, ny, i + 1, j) > 0.0 || EBO_M(plastics, ny, i + 1, j + 1) > 0.0)
+                y = 1;
+            if (EBO_M(plastic_yield, ny, i, j) > 0.0 || EBO_M(plastic_yield, ny, i, j + 1) > 0.0 ||
+                EBO_M(plastic_yield, ny, i + 1, j) > 0.0 || EBO_M(plastic_yield, ny, i + 1, j + 1) > 0.0)
+                y = 1;
+            if (!y) continue;
+            const double vinv = 1.0 / EBO_M(etas1, ny, i, j) + 1.0 / EBO_M(etas1, ny, i, j + 1) +
+                                1.0 / EBO_M(etas1, ny, i + 1, j) + 1.0 / EBO_M(etas1, ny, i + 1, j + 1) + 0.0 + 0.0 +
+                                0.0 + 0.0;
+            const double den = 4.0 + 0.0 + 0.0 + 0.0 + 0.0;
+            if (vinv != 0.0 && den != 0.0) EBO_M(etan1, ny - 1, i, j) = 1.0 / (vinv / den);
+        }
+}
+
+/* PlasticStrainRateInterpolation.interpolate_at_pressure_nodes!
+ * (interpolation/utils/stokes_funcs/PlasticStrainRateInterpolation.jl:18-41) */
+void ebo_plastic_strain_rate_at_pressure_nodes(int64_t xnum, int64_t ynum, const double *eii_basic,
+                                               double *eii_pressure)
+{
+    const int64_t ny = ynum;
+    for (int64_t j = 1; j <= xnum - 1; ++j)
+        for (int64_t i = 1; i <= ynum - 1; ++i)
+            EBO_M(eii_pressure, ny - 1, i, j) = (EBO_M(eii_basic, ny, i, j) + EBO_M(eii_basic, ny, i, j + 1) +
+                                                 EBO_M(eii_basic, ny, i + 1, j) + EBO_M(eii_basic, ny, i + 1, j + 1)) /
+                                                4.0;
+}

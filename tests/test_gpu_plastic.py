@@ -10,7 +10,8 @@ import pytest
 
 import oracle
 from earthbox_gpu.device import to_device, to_host
-from earthbox_gpu.plastic import update_nodes_for_plastic_yielding
+from earthbox_gpu.plastic import (interpolate_basic_viscoplastic_viscosity_to_pressure_grid,
+                                  plastic_strain_rate_at_pressure_nodes, update_nodes_for_plastic_yielding)
 from plastic_cases import PARAMS, plastic_fields
 
 pytestmark = pytest.mark.gpu
@@ -47,3 +48,26 @@ def test_no_yielding_before_the_first_step(gpu_ctx):
     np.testing.assert_array_equal(to_host(out["etavp_new"]), f["eta_flow"])
     np.testing.assert_array_equal(to_host(dye), ye)
     assert not to_host(out["plastic_yield_new"]).any()
+
+
+@pytest.mark.parametrize("nx,ny,seed", [(12, 9, 6), (301, 151, 7)])
+def test_picard_grid_updates_bit_exact(gpu_ctx, nx, ny, seed):
+    print(f"harmonic viscosity and plastic strain rate nx={nx} ny={ny}", flush=True)
+    g, f, _ = plastic_fields(nx, ny, seed)
+    rng = np.random.default_rng(seed)
+    plastics = np.asfortranarray((rng.random((ny, nx)) < 0.1).astype(float))
+    plasticn = np.asfortranarray((rng.random((ny - 1, nx - 1)) < 0.1).astype(float))
+    pyield = np.asfortranarray((rng.random((ny, nx)) < 0.1).astype(float))
+    etan1 = np.asfortranarray(rng.uniform(1e20, 1e22, (ny - 1, nx - 1)))
+    eii = np.asfortranarray(rng.uniform(0, 1e-14, (ny, nx)))
+    exp_eta = oracle.harmonic_normal_viscosity(g, f["etas1"], plastics, plasticn, pyield, etan1)
+    exp_eii = oracle.plastic_strain_rate_at_pressure_nodes(g, eii)
+    gpu_ctx.set_grid(g)
+    d_eta = to_device(etan1)
+    interpolate_basic_viscoplastic_viscosity_to_pressure_grid(gpu_ctx, to_device(f["etas1"]), to_device(plastics),
+                                                              to_device(plasticn), to_device(pyield), d_eta)
+    d_ep = to_device(np.zeros((ny - 1, nx - 1), order="F"))
+    plastic_strain_rate_at_pressure_nodes(gpu_ctx, to_device(eii), d_ep)
+    gpu_ctx.synchronize()
+    assert np.array_equal(to_host(d_eta), exp_eta)
+    assert np.array_equal(to_host(d_ep), exp_eii)

@@ -143,3 +143,41 @@ def test_yield_count_and_rms_error():
     assert n == counted.sum() and n > 0
     assert not changed[~counted].any()
     np.testing.assert_allclose(err, np.sqrt(np.mean(ye2[counted] ** 2)), rtol=1e-12)
+
+
+def test_harmonic_normal_viscosity_where_nodes_yield():
+    """Cells beside a yielding node take the harmonic mean of their four basic
+    viscosities (1/eta = mean of 1/eta_k; four equal values give that value);
+    the others keep etan1."""
+    g, f, _ = plastic_fields(12, 9, 6)
+    ny, nx = g.ynum, g.xnum
+    rng = np.random.default_rng(3)
+    plastics = np.asfortranarray((rng.random((ny, nx)) < 0.1).astype(float))
+    plasticn = np.asfortranarray((rng.random((ny - 1, nx - 1)) < 0.1).astype(float))
+    pyield = np.asfortranarray((rng.random((ny, nx)) < 0.1).astype(float))
+    etan1 = np.asfortranarray(rng.uniform(1e20, 1e22, (ny - 1, nx - 1)))
+    out = oracle.harmonic_normal_viscosity(g, f["etas1"], plastics, plasticn, pyield, etan1)
+    e = f["etas1"]
+    for i in range(ny - 1):
+        for j in range(nx - 1):
+            near = (plasticn[i, j] > 0 or plastics[i:i + 2, j:j + 2].any() or pyield[i:i + 2, j:j + 2].any())
+            if near:
+                np.testing.assert_allclose(1.0 / out[i, j], np.mean(1.0 / e[i:i + 2, j:j + 2]), rtol=1e-14)
+            else:
+                assert out[i, j] == etan1[i, j]
+    flat = np.asfortranarray(np.full((ny, nx), 3e21))
+    out2 = oracle.harmonic_normal_viscosity(g, flat, np.ones((ny, nx), order="F"), plasticn, pyield, etan1)
+    np.testing.assert_allclose(out2, 3e21, rtol=1e-15)
+
+
+def test_plastic_strain_rate_at_pressure_nodes_is_the_cell_centre_value():
+    """A bilinear field on the basic nodes averages to its value at the cell
+    centre (mean of the four corner coordinates)."""
+    g, _, _ = plastic_fields(14, 10, 7)
+    X, Y = np.meshgrid(g.gridx_b, g.gridy_b)
+    eii = np.asfortranarray(1e-15 + 2e-20 * X - 3e-20 * Y + 1e-25 * X * Y)
+    out = oracle.plastic_strain_rate_at_pressure_nodes(g, eii)
+    xc = 0.5 * (g.gridx_b[:-1] + g.gridx_b[1:])
+    yc = 0.5 * (g.gridy_b[:-1] + g.gridy_b[1:])
+    Xc, Yc = np.meshgrid(xc, yc)
+    np.testing.assert_allclose(out, 1e-15 + 2e-20 * Xc - 3e-20 * Yc + 1e-25 * Xc * Yc, rtol=1e-12)
