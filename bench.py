@@ -47,6 +47,7 @@ def parse():
     ap.add_argument("--stokes3d-n", type=int, default=257)
     ap.add_argument("--stokes3d-max-iter", type=int, default=300)
     ap.add_argument("--no-sinker", action="store_true")
+    ap.add_argument("--no-picard", action="store_true")
     return ap.parse_args()
 
 
@@ -311,6 +312,46 @@ def bench_scatter(ctx, g, x, y, reps=3):
     return {"ms": el * 1e3, "Mmarkers_per_s": n / el / 1e6, "markers": n}
 
 
+def bench_picard(ctx, nglobal=10, tol=1e-4):
+    """The Picard loop over nodes on the synthetic 2001 x 801 visco-elasto-plastic
+    rift state (four layers, cohesion and friction, +-0.5 cm/yr extension, one
+    100-kyr step): every Picard iteration forms the viscoelastic right-hand
+    side, solves to 1e-8 in FP64 (set-up of the preconditioner for the new
+    viscosity included), processes the solution on the grid and updates the
+    nodes for plastic yielding, until LoopTermination stops the loop."""
+    import torch
+    from earthbox_gpu import learned, workloads
+    from earthbox_gpu.device import to_device
+    from earthbox_gpu.picard import picard_loop_nodes
+    tmpl, state, par = workloads.rift2d(2001, 801)
+    ctx.set_grid(tmpl.grid)
+    out = {"system": "rift2d 2001x801 nodes (500 x 160 km), 4 layers, plastic, +-0.5 cm/yr",
+           "picard_tolerance": tol, "nglobal": nglobal, "solve_rtol": 1e-8}
+    net = learned.LearnedCorrection(ctx)
+    if learned.WEIGHTS_FILE.exists():
+        net.load()
+    legs = (("mixed_fp32_mg", dict(precision=1)), ("plain_fp64_mg", dict(precision=0)),
+            ("learned_mixed", dict(precision=1, learned=1)))
+    for name, kw in legs:
+        opts = dict(rtol=1e-8, **kw)
+        picard_loop_nodes(ctx, tmpl, {k: to_device(v) for k, v in state.items()}, par, 1, tol, solve_opts=opts)
+        ds = {k: to_device(v) for k, v in state.items()}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, log = picard_loop_nodes(ctx, tmpl, ds, par, nglobal, tol, solve_opts=opts)
+        el = time.perf_counter() - t0
+        out[name] = {"seconds": el, "picard_iterations": len(log),
+                     "krylov_iterations": [r["iterations"] for r in log],
+                     "max_relres": max(r["relres"] for r in log),
+                     "nnode_yield": [r["nnode_yield"] for r in log],
+                     "final_dvxy_rel_L2": log[-1]["dvxy_rel_L2"],
+                     "final_global_yield_error": log[-1]["global_yield_error"]}
+    p, m, lm = out["plain_fp64_mg"], out["mixed_fp32_mg"], out["learned_mixed"]
+    out["speedup_mixed_vs_plain"] = p["seconds"] / m["seconds"]
+    out["speedup_learned_mixed_vs_plain"] = p["seconds"] / lm["seconds"]
+    return out
+
+
 def cpu_baseline(g, x, y, vx1, vy1, esp, dt, sample_markers: int):
     """Oracle RK4 restatement timed on this host (test-infrastructure checker
     used only as the reported CPU baseline)."""
@@ -411,6 +452,8 @@ def main():
         out["stokes3d"] = bench_stokes3d(ctx, args.stokes3d_n, args.stokes3d_max_iter)
     if not args.no_sinker:
         out["sinker"] = bench_sinker(ctx)
+    if not args.no_picard:
+        out["picard"] = bench_picard(ctx)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, x, y, vx1, vy1, to_host_np(desp), dt, 2_000_000)
         if not args.no_sinker:

@@ -332,3 +332,68 @@ def stokes_sinker2d(n: int = 97, size: float = 1.0e5, viscosity_block: float | N
     out["dims"] = (n, n, size, size)
     out["pressure_bc"] = P["pressure_bc"]
     return out
+
+
+RIFT_LAYERS = (  # bottom depth (m), flow viscosity (Pa s), density (kg/m3)
+    (2.0e4, 1e24, 2700.0),   # upper crust
+    (3.5e4, 1e21, 2900.0),   # lower crust
+    (1.0e5, 1e23, 3300.0),   # lithospheric mantle
+    (np.inf, 1e19, 3250.0),  # asthenosphere
+)
+
+
+def rift2d(xnum: int = 2001, ynum: int = 801, xsize: float = 5.0e5, ysize: float = 1.6e5,
+           seed: int = 5, extension: float = 0.5 / 100 / 3.15576e7, gravity_y: float = 9.81,
+           timestep: float = 3.15576e12):
+    """Synthetic visco-elasto-plastic rift state on 2001 x 801 nodes over
+    500 x 160 km: four layers (upper crust, lower crust, lithospheric mantle,
+    asthenosphere) with flow viscosity clipped to [1e18, 1e25], shear modulus
+    3e10, cohesion 20 MPa and friction 30 degrees (5 MPa in a seeded weak
+    patch of the upper crust), the sides pulled apart at +-0.5 cm/yr with
+    the outflow replaced through the bottom, free slip elsewhere.  Returns
+    (template StokesProblem, grid state for the Picard loop as numpy arrays,
+    plastic-yield scalars)."""
+    from .grid import uniform_grid
+    from .stokes_problem import StokesProblem, free_slip_bcs
+    g = uniform_grid(xnum, ynum, xsize, ysize)
+    rng = np.random.default_rng(seed)
+
+    def layer_props(Y):
+        eta = np.empty_like(Y)
+        rho = np.empty_like(Y)
+        top = 0.0
+        for bottom, e, r in RIFT_LAYERS:
+            m = (Y >= top) & (Y < bottom)
+            eta[m] = e
+            rho[m] = r
+            top = bottom
+        return eta, rho
+    Xb, Yb = np.meshgrid(g.gridx_b, g.gridy_b)
+    Xp, Yp = np.meshgrid(g.gridx_pr, g.gridy_pr)
+    Xv, Yv = np.meshgrid(g.gridx_vy, g.gridy_b)
+    eta_b, rho_b = layer_props(Yb)
+    eta_p, _ = layer_props(Yp)
+    _, rho_v = layer_props(Yv)
+    # log-normal perturbation of the flow viscosity, +-0.3 decades, seeded
+    eta_b = np.clip(eta_b * 10 ** (0.3 * rng.standard_normal(eta_b.shape)), 1e18, 1e25)
+    eta_p = np.clip(eta_p * 10 ** (0.3 * rng.standard_normal(eta_p.shape)), 1e18, 1e25)
+    xc = xsize / 2 + 1e4 * (rng.random() - 0.5)
+    weak = (np.abs(Xb - xc) < 5e3) & (Yb > 5e3) & (Yb < 1.5e4)
+    F = np.asfortranarray
+    ny, nx = ynum, xnum
+    state = {"etas1": F(eta_b), "eta_flow": F(eta_b.copy()), "rho1": F(rho_b), "rho1_vy": F(rho_v),
+             "etan1": F(eta_p), "mus1": F(np.full((ny, nx), 3e10)), "mun1": F(np.full((ny - 1, nx - 1), 3e10)),
+             "sxy1": F(np.zeros((ny, nx))), "sxx1": F(np.zeros((ny - 1, nx - 1))),
+             "plastics": F(np.zeros((ny, nx))), "plasticn": F(np.zeros((ny - 1, nx - 1))),
+             "plastic_yield": F(np.zeros((ny, nx))), "cohesion_grid": F(np.where(weak, 5e6, 2e7)),
+             "fric_degrees_grid": F(np.full((ny, nx), 30.0)), "dilatation_grid": F(np.zeros((ny - 1, nx - 1))),
+             "eii_plastic_pressure": F(np.zeros((ny - 1, nx - 1))), "yield_error": F(np.zeros((ny, nx)))}
+    bcs = free_slip_bcs(g, vx_left=-extension, vx_right=extension,
+                        vy_bottom=-2.0 * extension * ysize / xsize)
+    RX1, RY1, RC1 = _rhs_viscous(g, rho_b, rho_v, gravity_y, 0)
+    template = StokesProblem(grid=g, etas0=state["etas1"], etan0=state["etan1"], rho1_vy=state["rho1_vy"],
+                             RX1=RX1, RY1=RY1, RC1=RC1, bcs=bcs, pressure_bc_mode=0, pressure_bc=0.0,
+                             timestep=timestep, gravity_y=gravity_y, iuse_interface_stabilization=0)
+    params = dict(timestep=timestep, timesum=timestep, ntimestep=1, viscosity_min=1e18, viscosity_max=1e25,
+                  iuse_fluid_pressure_for_yield=0)
+    return template, state, params
