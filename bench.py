@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--stokes3d-max-iter", type=int, default=300)
     ap.add_argument("--no-sinker", action="store_true")
     ap.add_argument("--no-picard", action="store_true")
+    ap.add_argument("--no-grid-kernels", action="store_true")
     return ap.parse_args()
 
 
@@ -312,6 +313,68 @@ def bench_scatter(ctx, g, x, y, reps=3):
     return {"ms": el * 1e3, "Mmarkers_per_s": n / el / 1e6, "markers": n}
 
 
+def bench_grid_kernels(ctx, reps=20):
+    """Per-call time of the grid kernels around the Stokes solve on the
+    2001 x 801 rift state, with HIP events on the library's stream, and the
+    achieved bandwidth on each call's algorithmic bytes (every input array
+    read once, every output written once; FP64)."""
+    import numpy as np
+    import torch
+    from earthbox_gpu import workloads
+    from earthbox_gpu.device import to_device
+    from earthbox_gpu.plastic import (interpolate_basic_viscoplastic_viscosity_to_pressure_grid,
+                                      plastic_strain_rate_at_pressure_nodes, update_nodes_for_plastic_yielding)
+    from earthbox_gpu.viscoelastic import calc_viscoelastic_terms_and_rhs, update_grid_strain_rate_and_stress
+    tmpl, state, par = workloads.rift2d(2001, 801)
+    g = tmpl.grid
+    ctx.set_grid(g)
+    nb, npr = g.xnum * g.ynum, (g.xnum - 1) * (g.ynum - 1)
+    nvx, nvy = (g.ynum + 1) * g.xnum, g.ynum * (g.xnum + 1)
+    ds = {k: to_device(v) for k, v in state.items()}
+    rng = np.random.default_rng(1)
+    vx1 = to_device(rng.uniform(-1e-9, 1e-9, (g.ynum + 1, g.xnum)))
+    vy1 = to_device(rng.uniform(-1e-9, 1e-9, (g.ynum, g.xnum + 1)))
+    ve = calc_viscoelastic_terms_and_rhs(ctx, ds, par["timestep"], 0.0, tmpl.gravity_y, 0)
+    ss = update_grid_strain_rate_and_stress(ctx, {"vx1": vx1, "vy1": vy1, **ds}, par["timestep"])
+    pr1 = to_device(rng.uniform(0, 5e8, (g.ynum - 1, g.xnum - 1)))
+    pf = {k: ds[k] for k in ("plastic_yield", "etas1", "eta_flow", "mus1", "cohesion_grid", "fric_degrees_grid")}
+    pf.update(sxy2=ss["sxy2"], sxx2=ss["sxx2"], pr1=pr1)
+    pout = update_nodes_for_plastic_yielding(ctx, pf, ds["yield_error"], par)[0]
+    calls = {
+        # inputs: 4 basic + 3 pressure + vy density + 2 pressure; outputs: 2 basic + 2 pressure + RX1, RY1, RC1
+        "viscoelastic_rhs": (lambda: calc_viscoelastic_terms_and_rhs(ctx, ds, par["timestep"], 0.0, tmpl.gravity_y,
+                                                                     0, out=ve),
+                             8 * (4 * nb + 5 * npr + nvy + 2 * nb + 3 * npr + nvx + nvy)),
+        # inputs: vx1, vy1, 3 basic, 3 pressure; outputs: exy, sxy2, dsxy, exx, eii, sxx2, dsxx
+        "strain_rate_and_stress": (lambda: update_grid_strain_rate_and_stress(ctx, {"vx1": vx1, "vy1": vy1, **ds},
+                                                                              par["timestep"], out=ss),
+                                   8 * (nvx + nvy + 3 * nb + 3 * npr + 3 * nb + 4 * npr)),
+        # inputs: 7 basic + 2 pressure (+ yield_error read and written); outputs: 3 basic
+        "plastic_yielding": (lambda: update_nodes_for_plastic_yielding(ctx, pf, ds["yield_error"], par, out=pout),
+                             8 * (7 * nb + 2 * npr + 2 * nb + 3 * nb)),
+        "harmonic_normal_viscosity": (lambda: interpolate_basic_viscoplastic_viscosity_to_pressure_grid(
+            ctx, ds["etas1"], ds["plastics"], ds["plasticn"], ds["plastic_yield"], ds["etan1"]),
+            8 * (3 * nb + 2 * npr + npr)),
+        "plastic_rate_at_pressure_nodes": (lambda: plastic_strain_rate_at_pressure_nodes(
+            ctx, pout["plastic_strain_rate"], ds["eii_plastic_pressure"]), 8 * (nb + npr)),
+    }
+    stream = torch.cuda.current_stream()
+    out = {"system": "rift2d 2001x801 nodes", "nodes": nb}
+    for name, (fn, nbytes) in calls.items():
+        fn()
+        torch.cuda.synchronize()
+        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s0.record(stream)
+        for _ in range(reps):
+            fn()
+        s1.record(stream)
+        torch.cuda.synchronize()
+        ms = s0.elapsed_time(s1) / reps
+        out[name] = {"ms": ms, "algorithmic_bytes": nbytes, "GB_per_s": nbytes / (ms * 1e-3) / 1e9,
+                     "frac_of_8TBs": nbytes / (ms * 1e-3) / 8e12}
+    return out
+
+
 def bench_picard(ctx, nglobal=10, tol=1e-4):
     """The Picard loop over nodes on the synthetic 2001 x 801 visco-elasto-plastic
     rift state (four layers, cohesion and friction, +-0.5 cm/yr extension, one
@@ -454,6 +517,8 @@ def main():
         out["sinker"] = bench_sinker(ctx)
     if not args.no_picard:
         out["picard"] = bench_picard(ctx)
+    if not args.no_grid_kernels:
+        out["grid_kernels"] = bench_grid_kernels(ctx)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(g, x, y, vx1, vy1, to_host_np(desp), dt, 2_000_000)
         if not args.no_sinker:
