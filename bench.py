@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--no-sinker", action="store_true")
     ap.add_argument("--no-picard", action="store_true")
     ap.add_argument("--no-grid-kernels", action="store_true")
+    ap.add_argument("--no-marker-gathers", action="store_true")
     return ap.parse_args()
 
 
@@ -438,6 +439,46 @@ def cpu_baseline(g, x, y, vx1, vy1, esp, dt, sample_markers: int):
                       f"workload (serial C restatement, {platform.processor() or 'host'} core)"}
 
 
+def bench_marker_gathers(ctx, g, dx, dy, reps=10):
+    """Per-call time of the fused grid -> marker gathers after the solve
+    (marker_exx, marker_exy, marker_sr_ratio, marker_pr) on the RK4 workload's
+    markers, with HIP events on the library's stream.  Algorithmic bytes: per
+    marker x, y, xn, yn, eta, matid read and four FP64 outputs written (66 B),
+    plus each of the seven grid arrays read once."""
+    import numpy as np
+    import torch
+    from earthbox_gpu.device import to_device
+    from earthbox_gpu.marker_gather import update_markers_after_stokes
+    ctx.set_grid(g)
+    n = int(dx.numel())
+    xn = torch.zeros(n, dtype=torch.int32, device=dx.device)
+    yn = torch.zeros(n, dtype=torch.int32, device=dx.device)
+    ctx.calc_upper_left_basic_grid_cell_indices_for_markers(dx, dy, None, xn, yn)
+    rng = np.random.default_rng(3)
+    nb, npr = g.xnum * g.ynum, (g.xnum - 1) * (g.ynum - 1)
+    pshape, bshape = (g.ynum - 1, g.xnum - 1), (g.ynum, g.xnum)
+    f = {k: to_device(rng.uniform(-1e7, 1e7, pshape)) for k in ("exx", "pr1", "sxx2", "dsxx")}
+    f.update({k: to_device(rng.uniform(-1e7, 1e7, bshape)) for k in ("exy", "sxy2", "dsxy")})
+    m = {"x": dx, "y": dy, "marker_xn": xn, "marker_yn": yn,
+         "eta": torch.full((n,), 1e21, dtype=torch.float64, device=dx.device),
+         "matid": torch.ones(n, dtype=torch.int16, device=dx.device),
+         "mat_mu": to_device(np.array([3e10]))}
+    out = update_markers_after_stokes(ctx, m, f, 3.15576e12)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s0.record(stream)
+    for _ in range(reps):
+        update_markers_after_stokes(ctx, m, f, 3.15576e12, out)
+    s1.record(stream)
+    torch.cuda.synchronize()
+    ms = s0.elapsed_time(s1) / reps
+    nbytes = 66 * n + 8 * (4 * npr + 3 * nb)
+    return {"kernel": "marker_gather_kernel", "markers": n, "ms": ms, "Mmarkers_per_s": n / (ms * 1e-3) / 1e6,
+            "algorithmic_bytes": nbytes, "GB_per_s": nbytes / (ms * 1e-3) / 1e9,
+            "frac_of_8TBs": nbytes / (ms * 1e-3) / 8e12}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -507,6 +548,8 @@ def main():
                      "kernel": "rk4_advect_kernel", "kernel_ms": kern_ms,
                      "bytes_per_unit": RK4_BYTES_PER_MARKER},
     }
+    if not args.no_marker_gathers:
+        out["marker_gathers"] = bench_marker_gathers(ctx, g, dx, dy)
     if not args.no_stokes:
         out["stokes"] = bench_stokes(ctx, nx, ny, args.stokes_max_iter)
     if not args.no_scatter:

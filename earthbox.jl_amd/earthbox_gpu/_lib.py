@@ -42,6 +42,7 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_update_nodes_for_plastic_yielding": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_grid_strain_rate_and_stress": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_harmonic_normal_viscosity": (c_int, [c_void_p] * 6),
+    "ebg_marker_strain_rate_and_pressure": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_plastic_strain_rate_at_pressure_nodes": (c_int, [c_void_p] * 3),
     "ebg_stokes_set_system": (c_int, [c_void_p] + [c_void_p] * 16 + [c_int, c_double, c_double,
                                                                      c_double, c_double, c_int]),
@@ -122,6 +123,18 @@ class StrainStressIn(C.Structure):
 class StrainStressOut(C.Structure):
     _fields_ = [("exx", c_void_p), ("exy", c_void_p), ("eii", c_void_p), ("sxy2", c_void_p),
                 ("sxx2", c_void_p), ("dsxy", c_void_p), ("dsxx", c_void_p)]
+
+
+class MarkerGatherIn(C.Structure):
+    _fields_ = ([("marknum", c_int64)] +
+                [(k, c_void_p) for k in ("x", "y", "marker_xn", "marker_yn", "inside_flags", "exx", "exy",
+                                         "pr1", "sxx2", "sxy2", "dsxx", "dsxy", "eta", "matid", "mat_mu")] +
+                [("nmat", c_int64), ("timestep", c_double)])
+
+
+class MarkerGatherOut(C.Structure):
+    _fields_ = [("marker_exx", c_void_p), ("marker_exy", c_void_p), ("marker_pr", c_void_p),
+                ("marker_sr_ratio", c_void_p)]
 
 
 class PlasticNodesIn(C.Structure):

@@ -1,0 +1,89 @@
+"""Grid -> marker gathers after the Stokes solve: the marker steps of
+StokesContinuitySolver.execute_post_stokes_solver_steps!
+(stokes_continuity_solver/StokesContinuitySolver.jl:272-279) on the GPU.
+
+``update_markers_after_stokes`` computes, in one pass over the markers, what
+MarkerStrainRate.calculate_marker_strain_rate! (marker_exx, marker_exy),
+MarkerStrainRateRatio.calculate_marker_strain_rate_ratio! (marker_sr_ratio)
+and MarkerPressure.calculate_marker_pressure! (marker_pr) write; the three
+named functions below run the same kernel for their own outputs.  Results are
+bit-identical to the reference loops; markers outside the grid keep their
+values.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import MarkerGatherIn, MarkerGatherOut, check, lib
+from .context import GPUContext
+from .device import ptr
+
+GRID_PRESSURE = ("exx", "pr1", "sxx2", "dsxx")
+GRID_BASIC = ("exy", "sxy2", "dsxy")
+OUTPUTS = ("marker_exx", "marker_exy", "marker_pr", "marker_sr_ratio")
+
+
+def update_markers_after_stokes(ctx: GPUContext, markers: dict, fields: dict, timestep: float = 0.0,
+                                out: dict | None = None, outputs=OUTPUTS) -> dict:
+    """markers: device vectors x, y, marker_xn, marker_yn (int32, 1-based, as
+    the scatter stores them), optional inside_flags (int8), and for the ratio
+    eta, matid (int16) and mat_mu.  fields: device grid arrays exx, pr1, sxx2,
+    dsxx (ynum-1, xnum-1) and exy, sxy2, dsxy (ynum, xnum), as far as the
+    requested outputs need them.  out: device vectors to update (created,
+    zero-filled, for the requested outputs when missing)."""
+    g = ctx.grid
+    if g is None:
+        raise RuntimeError("set_grid() must be called first")
+    nx, ny = g.xnum, g.ynum
+    n = int(markers["x"].numel())
+    a = MarkerGatherIn()
+    a.marknum = n
+    for k in ("x", "y", "eta"):
+        setattr(a, k, ptr(markers.get(k), np.float64, n))
+    a.marker_xn = ptr(markers["marker_xn"], np.int32, n)
+    a.marker_yn = ptr(markers["marker_yn"], np.int32, n)
+    a.inside_flags = ptr(markers.get("inside_flags"), np.int8, n)
+    a.matid = ptr(markers.get("matid"), np.int16, n)
+    mat_mu = markers.get("mat_mu")
+    a.mat_mu = ptr(mat_mu, np.float64)
+    a.nmat = 0 if mat_mu is None else int(mat_mu.numel())
+    for k in GRID_PRESSURE:
+        setattr(a, k, ptr(fields.get(k), np.float64, (nx - 1) * (ny - 1)))
+    for k in GRID_BASIC:
+        setattr(a, k, ptr(fields.get(k), np.float64, nx * ny))
+    a.timestep = float(timestep)
+    if out is None:
+        out = {}
+    dev = markers["x"].device
+    o = MarkerGatherOut()
+    for k in outputs:
+        if k not in OUTPUTS:
+            raise KeyError(k)
+        if k not in out:
+            out[k] = torch.zeros(n, dtype=torch.float64, device=dev)
+        setattr(o, k, ptr(out[k], np.float64, n))
+    check(lib.ebg_marker_strain_rate_and_pressure(ctx.handle, C.byref(a), C.byref(o)),
+          "ebg_marker_strain_rate_and_pressure")
+    return out
+
+
+def calculate_marker_strain_rate(ctx: GPUContext, markers: dict, fields: dict, out: dict | None = None) -> dict:
+    """MarkerStrainRate.calculate_marker_strain_rate! (MarkerStrainRate.jl:23-69)."""
+    return update_markers_after_stokes(ctx, markers, fields, 0.0, out, ("marker_exx", "marker_exy"))
+
+
+def calculate_marker_strain_rate_ratio(ctx: GPUContext, markers: dict, fields: dict, timestep: float,
+                                       out: dict | None = None) -> dict:
+    """MarkerStrainRateRatio.calculate_marker_strain_rate_ratio!
+    (MarkerStrainRateRatio.jl:71-188); the marker strain rates it reads are
+    recomputed from exx and exy in-register, as the reference's preceding step
+    would have stored them."""
+    return update_markers_after_stokes(ctx, markers, fields, timestep, out, ("marker_sr_ratio",))
+
+
+def calculate_marker_pressure(ctx: GPUContext, markers: dict, fields: dict, out: dict | None = None) -> dict:
+    """MarkerPressure.calculate_marker_pressure! (MarkerPressure.jl:17-49)."""
+    return update_markers_after_stokes(ctx, markers, fields, 0.0, out, ("marker_pr",))

@@ -376,6 +376,60 @@ function advect_markers_gpu!(st::GPUState, model, inside_flags::Vector{Int8})
     return nothing
 end
 
+"Field order of ebg_marker_gather_in / ebg_marker_gather_out (include/earthbox_gpu.h)."
+struct MarkerGatherIn
+    marknum::Int64
+    x::Ptr{Cvoid}; y::Ptr{Cvoid}; marker_xn::Ptr{Cvoid}; marker_yn::Ptr{Cvoid}; inside_flags::Ptr{Cvoid}
+    exx::Ptr{Cvoid}; exy::Ptr{Cvoid}; pr1::Ptr{Cvoid}
+    sxx2::Ptr{Cvoid}; sxy2::Ptr{Cvoid}; dsxx::Ptr{Cvoid}; dsxy::Ptr{Cvoid}
+    eta::Ptr{Cvoid}; matid::Ptr{Cvoid}; mat_mu::Ptr{Cvoid}
+    nmat::Int64
+    timestep::Cdouble
+end
+struct MarkerGatherOut
+    marker_exx::Ptr{Cvoid}; marker_exy::Ptr{Cvoid}; marker_pr::Ptr{Cvoid}; marker_sr_ratio::Ptr{Cvoid}
+end
+
+"""
+    update_markers_after_stokes_gpu!(st, model, inside_flags)
+
+GPU replacement of the marker steps of execute_post_stokes_solver_steps!
+(StokesContinuitySolver.jl:272-279): MarkerStrainRate.calculate_marker_strain_rate!,
+MarkerStrainRateRatio.calculate_marker_strain_rate_ratio! and
+MarkerPressure.calculate_marker_pressure!, in one pass.  The grid steps before
+them (update_grid_stress!, calculate_grid_stress_change!) are
+update_grid_strain_rate_and_stress_gpu!.
+"""
+function update_markers_after_stokes_gpu!(st::GPUState, model, inside_flags::Vector{Int8})
+    set_grid!(st, model)
+    m = model.markers.arrays
+    sc = model.stokes_continuity.arrays
+    n = model.markers.parameters.distribution.marknum.value
+    mat_mu = model.materials.arrays.mat_mu.array
+    up(k, x) = h2d!(st, k, x)
+    gin = MarkerGatherIn(n, up(:mx, m.location.marker_x.array), up(:my, m.location.marker_y.array),
+        up(:xn, m.grid_marker_relationship.marker_xn.array), up(:yn, m.grid_marker_relationship.marker_yn.array),
+        up(:flags, inside_flags),
+        up(:exx, sc.strain_rate_and_spin.exx.array), up(:exy, sc.strain_rate_and_spin.exy.array),
+        up(:pr1, sc.pressure.pr1.array), up(:sxx2, sc.stress.sxx2.array), up(:sxy2, sc.stress.sxy2.array),
+        up(:dsxx, sc.stress_change.dsxx.array), up(:dsxy, sc.stress_change.dsxy.array),
+        up(:meta, m.rheology.marker_eta.array), up(:matid, m.material.marker_matid.array), up(:mat_mu, mat_mu),
+        length(mat_mu), model.timestep.parameters.main_time_loop.timestep.value)
+    outs = (m.strain.marker_exx.array, m.strain.marker_exy.array, m.pressure.marker_pr.array,
+            m.strain.marker_sr_ratio.array)
+    keys = (:mexx, :mexy, :mpr, :msr)
+    # outside markers keep their values, so the outputs travel both ways
+    gout = MarkerGatherOut((h2d!(st, k, x) for (k, x) in zip(keys, outs))...)
+    check(ccall((:ebg_marker_strain_rate_and_pressure, LIB), Cint,
+        (Ptr{Cvoid}, Ref{MarkerGatherIn}, Ref{MarkerGatherOut}), st.ctx, gin, gout),
+        "ebg_marker_strain_rate_and_pressure")
+    check(ccall((:ebg_ctx_synchronize, LIB), Cint, (Ptr{Cvoid},), st.ctx), "sync")
+    for (k, x) in zip(keys, outs)
+        d2h!(x, st.buffers[k])
+    end
+    return nothing
+end
+
 """
     run_multigrid_solver_gpu!(st, multigrid_data) -> (cycles, converged)
 

@@ -256,6 +256,50 @@ int ebg_harmonic_normal_viscosity(ebg_ctx *ctx, const double *etas1, const doubl
 int ebg_plastic_strain_rate_at_pressure_nodes(ebg_ctx *ctx, const double *eii_plastic_basic,
                                               double *eii_plastic_pressure);
 
+/* ---- grid -> marker gathers after the Stokes solve ----
+ * Per-marker DEVICE arrays (length marknum): x, y; marker_xn/yn (1-based basic
+ * cell, as the scatter stores them); inside_flags (NULL: computed from the
+ * grid limits); eta and matid (Int16, 1-based, with mat_mu of nmat shear
+ * moduli), read only for the strain-rate ratio.  DEVICE grid arrays:
+ * exy, sxy2, dsxy (ynum,xnum); exx, pr1, sxx2, dsxx (ynum-1,xnum-1). */
+typedef struct {
+    int64_t marknum;
+    const double *x, *y;
+    const int32_t *marker_xn, *marker_yn;
+    const int8_t *inside_flags;
+    const double *exx, *exy, *pr1;
+    const double *sxx2, *sxy2, *dsxx, *dsxy;
+    const double *eta;
+    const int16_t *matid;
+    const double *mat_mu;
+    int64_t nmat;
+    double timestep;
+} ebg_marker_gather_in;
+
+/* Per-marker DEVICE outputs; each may be NULL (not computed).  Markers whose
+ * flag is not 1 keep their values. */
+typedef struct {
+    double *marker_exx, *marker_exy, *marker_pr, *marker_sr_ratio;
+} ebg_marker_gather_out;
+
+/* Replaces, in one pass over the markers, the marker steps of
+ * StokesContinuitySolver.execute_post_stokes_solver_steps!
+ * (stokes_continuity_solver/StokesContinuitySolver.jl:272-279):
+ *   MarkerStrainRate.calculate_marker_strain_rate!
+ *     (stokes_continuity_solver/utils/strain_rate/MarkerStrainRate.jl:23-69),
+ *   MarkerStrainRateRatio.calculate_marker_strain_rate_ratio!
+ *     (stokes_continuity_solver/utils/strain_rate/MarkerStrainRateRatio.jl:71-188),
+ *   MarkerPressure.calculate_marker_pressure!
+ *     (stokes_continuity_solver/utils/MarkerPressure.jl:17-49),
+ * with the normalized distances of
+ * MarkerGridMapping.calc_normalized_distances_from_upper_left_basic_node!
+ * (interpolation/mapping/MarkerGridMapping.jl:185-218) formed in-register.
+ * marker_exx, marker_exy and marker_pr need exx, exy and pr1; marker_sr_ratio
+ * needs exx, exy, sxx2, sxy2, dsxx, dsxy, eta, matid and mat_mu.  FP64 in the
+ * reference's operand order with correctly rounded division: bit-identical. */
+int ebg_marker_strain_rate_and_pressure(ebg_ctx *ctx, const ebg_marker_gather_in *in,
+                                        ebg_marker_gather_out *out);
+
 /* ---- Stokes-continuity system (assembled-operator conventions) ----
  * Replaces the inputs StokesBuildData gathers (solver/build_system/utils/
  * BuildStructs.jl:37-177).  Fields are DEVICE pointers with the reference

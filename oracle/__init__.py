@@ -280,6 +280,54 @@ def grid_strain_rate_and_stress(g, vx1, vy1, f: dict, timestep):
     return out
 
 
+class _GatherIn(C.Structure):
+    _fields_ = ([("xnum", _I64), ("ynum", _I64), ("marknum", _I64)] +
+                [(k, _P) for k in ("gridx_b", "gridy_b", "xstp_b", "ystp_b", "gridx_vy", "xstp_vy",
+                                   "gridy_vx", "ystp_vx", "x", "y", "xn", "yn", "inside", "exx", "exy",
+                                   "pr1", "sxx2", "sxy2", "dsxx", "dsxy", "eta", "matid", "mat_mu")] +
+                [("timestep", C.c_double)])
+
+
+GATHER_OUT = ("marker_dx", "marker_dy", "marker_exx", "marker_exy", "marker_sr_ratio", "marker_pr")
+
+
+def marker_strain_rate_and_pressure(g, m: dict, f: dict, timestep=0.0, prev: dict | None = None):
+    """MarkerGridMapping.calc_normalized_distances_from_upper_left_basic_node!,
+    MarkerStrainRate.calculate_marker_strain_rate!, MarkerStrainRateRatio.
+    calculate_marker_strain_rate_ratio! (when f holds sxx2) and MarkerPressure.
+    calculate_marker_pressure!, in the reference's order.  m holds x, y, xn, yn
+    (1-based Int32), inside (Int8) and, for the ratio, eta, matid (Int16) and
+    mat_mu; f holds the grid arrays.  prev: marker arrays the loops start from
+    (zeros by default); outside markers keep them."""
+    L = lib()
+    L.ebo_marker_strain_rate_and_pressure.restype = None
+    L.ebo_marker_strain_rate_and_pressure.argtypes = [C.POINTER(_GatherIn)] + [_P] * 6
+    keep = []
+
+    def arr(v, dt):
+        if v is None:
+            return None
+        a = np.asfortranarray(v, dtype=dt)
+        keep.append(a)
+        return p(a)
+
+    n = int(np.asarray(m["x"]).size)
+    ax = [arr(getattr(g, k), np.float64) for k in ("gridx_b", "gridy_b", "xstp_b", "ystp_b", "gridx_vy",
+                                                  "xstp_vy", "gridy_vx", "ystp_vx")]
+    ratio = "sxx2" in f
+    s = _GatherIn(g.xnum, g.ynum, n, *ax, arr(m["x"], np.float64), arr(m["y"], np.float64),
+                  arr(m["xn"], np.int32), arr(m["yn"], np.int32), arr(m["inside"], np.int8),
+                  arr(f["exx"], np.float64), arr(f["exy"], np.float64), arr(f["pr1"], np.float64),
+                  *[arr(f.get(k) if ratio else None, np.float64) for k in ("sxx2", "sxy2", "dsxx", "dsxy")],
+                  arr(m.get("eta") if ratio else None, np.float64),
+                  arr(m.get("matid") if ratio else None, np.int16),
+                  arr(m.get("mat_mu") if ratio else None, np.float64), float(timestep))
+    prev = prev or {}
+    out = {k: np.array(prev[k], dtype=np.float64, copy=True) if k in prev else np.zeros(n) for k in GATHER_OUT}
+    L.ebo_marker_strain_rate_and_pressure(C.byref(s), *[p(out[k]) for k in GATHER_OUT])
+    return out
+
+
 def harmonic_normal_viscosity(g, etas1, plastics, plasticn, plastic_yield, etan1):
     """HarmonicAverage.normal_viscosity_from_harmonic_avg_shear_viscosity!: etan1
     (ynum-1, xnum-1) updated in a copy where a neighbouring node yields."""

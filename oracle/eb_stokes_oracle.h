@@ -56,6 +56,27 @@ typedef struct {
 void ebo_grid_strain_rate_and_stress(const ebo_strain_in *s, double *exx, double *exy, double *eii,
                                      double *sxy2, double *sxx2, double *dsxy, double *dsxx);
 
+/* Inputs of the grid -> marker gathers after the solve (MarkerStrainRate.jl,
+ * MarkerStrainRateRatio.jl, MarkerPressure.jl); sxx2 == NULL skips the ratio. */
+typedef struct {
+    int64_t xnum, ynum, marknum;
+    const double *gridx_b, *gridy_b, *xstp_b, *ystp_b;
+    const double *gridx_vy, *xstp_vy, *gridy_vx, *ystp_vx;
+    const double *x, *y;
+    const int32_t *xn, *yn;
+    const int8_t *inside;
+    const double *exx, *exy, *pr1;          /* (ynum-1,xnum-1), (ynum,xnum), (ynum-1,xnum-1) */
+    const double *sxx2, *sxy2, *dsxx, *dsxy;
+    const double *eta;
+    const int16_t *matid;
+    const double *mat_mu;
+    double timestep;
+} ebo_gather_in;
+
+void ebo_marker_strain_rate_and_pressure(const ebo_gather_in *s, double *marker_dx, double *marker_dy,
+                                         double *marker_exx, double *marker_exy, double *marker_sr_ratio,
+                                         double *marker_pr);
+
 int64_t ebo_stokes_build(const ebo_stokes_in *s, int64_t *Li, int64_t *Lj, double *Lv, double *R);
 double ebo_pressure_scaling_coeff(double etan11, double xstpavg, double ystpavg);
 void ebo_stokes_rhs(const ebo_rhs_in *r, double *etas0, double *sxy0, double *etan0,
