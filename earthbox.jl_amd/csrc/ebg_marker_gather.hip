@@ -6,8 +6,11 @@
 // (stokes_continuity_solver/utils/strain_rate/MarkerStrainRate.jl:23-69),
 // MarkerStrainRateRatio.calculate_marker_strain_rate_ratio!
 // (stokes_continuity_solver/utils/strain_rate/MarkerStrainRateRatio.jl:71-188)
-// and MarkerPressure.calculate_marker_pressure!
-// (stokes_continuity_solver/utils/MarkerPressure.jl:17-49).  One thread owns
+// MarkerPressure.calculate_marker_pressure!
+// (stokes_continuity_solver/utils/MarkerPressure.jl:17-49) and
+// StressCorrection.apply_marker_stress_correction_for_remaining_grid_stress!
+// (stokes_continuity_solver/subgrid_stress/stress_correction/StressCorrection.jl:27-83).
+// One thread owns
 // one marker: it forms the basic-cell distances (MarkerGridMapping.jl:211-214),
 // the pressure-cell indices and distances (MarkerGridMapping.jl:311-350) and
 // the bilinear sums (interpolation/utils/GridToMarker.jl:190-226) in registers,
@@ -30,7 +33,7 @@ struct GatherArgs {
     const double *gridx_vy, *xstp_vy, *gridy_vx, *ystp_vx;
     ebg_marker_gather_in in;
     ebg_marker_gather_out out;
-    int want_ratio;
+    int want_ratio, want_stress;
 };
 
 // MarkerGridMapping.upr_left_index_pressure: the staggered node that starts
@@ -95,6 +98,10 @@ __global__ void __launch_bounds__(256) marker_gather_kernel(GatherArgs a)
             const double sxy = bilinear(a.in.sxy2, brow, yn, xn, dy, dx);
             const double dsxx = bilinear(a.in.dsxx, prow, ypr, xpr, dypr, dxpr);
             const double dsxy = bilinear(a.in.dsxy, brow, yn, xn, dy, dx);
+            if (a.want_stress) {
+                a.out.marker_sxy[m] = a.out.marker_sxy[m] + dsxy;
+                a.out.marker_sxx[m] = a.out.marker_sxx[m] + dsxx;
+            }
             const int id = a.in.matid[m];
             const double mu = (id >= 1 && id <= a.in.nmat) ? a.in.mat_mu[id - 1] : __builtin_nan("");
             const double eta = a.in.eta[m];
@@ -110,6 +117,10 @@ __global__ void __launch_bounds__(256) marker_gather_kernel(GatherArgs a)
         }
     }
     if (a.out.marker_pr) a.out.marker_pr[m] = bilinear(a.in.pr1, prow, ypr, xpr, dypr, dxpr);
+    if (a.want_stress && !a.want_ratio) {  // StressCorrection.jl:61-78
+        a.out.marker_sxy[m] = a.out.marker_sxy[m] + bilinear(a.in.dsxy, brow, yn, xn, dy, dx);
+        a.out.marker_sxx[m] = a.out.marker_sxx[m] + bilinear(a.in.dsxx, prow, ypr, xpr, dypr, dxpr);
+    }
 }
 
 }  // namespace
@@ -125,6 +136,10 @@ extern "C" int ebg_marker_strain_rate_and_pressure(ebg_ctx* ctx, const ebg_marke
     EBG_CHECK(in && out, EBG_ERR_ARG, "ebg_marker_strain_rate_and_pressure: NULL arguments");
     EBG_CHECK(in->marknum >= 0, EBG_ERR_ARG, "ebg_marker_strain_rate_and_pressure: marknum < 0");
     const bool want_strain = out->marker_exx || out->marker_exy || out->marker_sr_ratio;
+    const bool want_stress = out->marker_sxx || out->marker_sxy;
+    EBG_CHECK(!want_stress || (out->marker_sxx && out->marker_sxy && in->dsxx && in->dsxy), EBG_ERR_ARG,
+              "ebg_marker_strain_rate_and_pressure: the stress correction needs marker_sxx, marker_sxy, "
+              "dsxx and dsxy");
     EBG_CHECK(!want_strain || (in->exx && in->exy), EBG_ERR_ARG,
               "ebg_marker_strain_rate_and_pressure: strain-rate outputs need exx and exy");
     EBG_CHECK(!out->marker_pr || in->pr1, EBG_ERR_ARG,
@@ -135,7 +150,7 @@ extern "C" int ebg_marker_strain_rate_and_pressure(ebg_ctx* ctx, const ebg_marke
               EBG_ERR_ARG,
               "ebg_marker_strain_rate_and_pressure: marker_sr_ratio needs sxx2, sxy2, dsxx, dsxy, eta, "
               "matid and mat_mu");
-    if (in->marknum == 0 || !(want_strain || out->marker_pr)) return EBG_OK;
+    if (in->marknum == 0 || !(want_strain || want_stress || out->marker_pr)) return EBG_OK;
     EBG_CHECK(in->x && in->y && in->marker_xn && in->marker_yn, EBG_ERR_ARG,
               "ebg_marker_strain_rate_and_pressure: NULL marker arrays");
     const Grid2D& g = ctx->grid;
@@ -158,6 +173,7 @@ extern "C" int ebg_marker_strain_rate_and_pressure(ebg_ctx* ctx, const ebg_marke
     a.out = *out;
     if (!want_strain) a.in.exx = nullptr;  // pressure only
     a.want_ratio = out->marker_sr_ratio != nullptr;
+    a.want_stress = want_stress;
     const unsigned blocks = (unsigned)((in->marknum + 255) / 256);
     hipLaunchKernelGGL(marker_gather_kernel, dim3(blocks), dim3(256), 0, ctx->stream, a);
     EBG_HIP(hipGetLastError());

@@ -134,7 +134,7 @@ class MarkerGatherIn(C.Structure):
 
 class MarkerGatherOut(C.Structure):
     _fields_ = [("marker_exx", c_void_p), ("marker_exy", c_void_p), ("marker_pr", c_void_p),
-                ("marker_sr_ratio", c_void_p)]
+                ("marker_sr_ratio", c_void_p), ("marker_sxx", c_void_p), ("marker_sxy", c_void_p)]
 
 
 class PlasticNodesIn(C.Structure):

@@ -24,6 +24,7 @@ from .device import ptr
 GRID_PRESSURE = ("exx", "pr1", "sxx2", "dsxx")
 GRID_BASIC = ("exy", "sxy2", "dsxy")
 OUTPUTS = ("marker_exx", "marker_exy", "marker_pr", "marker_sr_ratio")
+STRESS = ("marker_sxx", "marker_sxy")  # updated in place, so they must be given in ``out``
 
 
 def update_markers_after_stokes(ctx: GPUContext, markers: dict, fields: dict, timestep: float = 0.0,
@@ -33,7 +34,10 @@ def update_markers_after_stokes(ctx: GPUContext, markers: dict, fields: dict, ti
     eta, matid (int16) and mat_mu.  fields: device grid arrays exx, pr1, sxx2,
     dsxx (ynum-1, xnum-1) and exy, sxy2, dsxy (ynum, xnum), as far as the
     requested outputs need them.  out: device vectors to update (created,
-    zero-filled, for the requested outputs when missing)."""
+    zero-filled, for the requested outputs when missing).  Asking for
+    marker_sxx and marker_sxy (given in out) adds the remaining grid stress
+    change dsxx, dsxy at each marker, as StressCorrection.
+    apply_marker_stress_correction_for_remaining_grid_stress! does."""
     g = ctx.grid
     if g is None:
         raise RuntimeError("set_grid() must be called first")
@@ -60,8 +64,10 @@ def update_markers_after_stokes(ctx: GPUContext, markers: dict, fields: dict, ti
     dev = markers["x"].device
     o = MarkerGatherOut()
     for k in outputs:
-        if k not in OUTPUTS:
+        if k not in OUTPUTS + STRESS:
             raise KeyError(k)
+        if k in STRESS and k not in out:
+            raise KeyError(f"{k} is updated in place and must be given in out")
         if k not in out:
             out[k] = torch.zeros(n, dtype=torch.float64, device=dev)
         setattr(o, k, ptr(out[k], np.float64, n))
@@ -82,6 +88,14 @@ def calculate_marker_strain_rate_ratio(ctx: GPUContext, markers: dict, fields: d
     recomputed from exx and exy in-register, as the reference's preceding step
     would have stored them."""
     return update_markers_after_stokes(ctx, markers, fields, timestep, out, ("marker_sr_ratio",))
+
+
+def apply_marker_stress_correction_for_remaining_grid_stress(ctx: GPUContext, markers: dict, fields: dict,
+                                                             out: dict) -> dict:
+    """StressCorrection.apply_marker_stress_correction_for_remaining_grid_stress!
+    (subgrid_stress/stress_correction/StressCorrection.jl:27-83): out holds the
+    device vectors marker_sxx and marker_sxy, updated in place."""
+    return update_markers_after_stokes(ctx, markers, fields, 0.0, out, STRESS)
 
 
 def calculate_marker_pressure(ctx: GPUContext, markers: dict, fields: dict, out: dict | None = None) -> dict:

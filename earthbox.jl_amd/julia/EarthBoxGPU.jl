@@ -388,6 +388,7 @@ struct MarkerGatherIn
 end
 struct MarkerGatherOut
     marker_exx::Ptr{Cvoid}; marker_exy::Ptr{Cvoid}; marker_pr::Ptr{Cvoid}; marker_sr_ratio::Ptr{Cvoid}
+    marker_sxx::Ptr{Cvoid}; marker_sxy::Ptr{Cvoid}
 end
 
 """
@@ -398,9 +399,13 @@ GPU replacement of the marker steps of execute_post_stokes_solver_steps!
 MarkerStrainRateRatio.calculate_marker_strain_rate_ratio! and
 MarkerPressure.calculate_marker_pressure!, in one pass.  The grid steps before
 them (update_grid_stress!, calculate_grid_stress_change!) are
-update_grid_strain_rate_and_stress_gpu!.
+update_grid_strain_rate_and_stress_gpu!.  With correct_stress=true the same
+pass also runs StressCorrection.apply_marker_stress_correction_for_remaining_grid_stress!
+(the whole of update_marker_stress_for_grid_and_subgrid_changes! when
+subgrid_diff_coef_stress is 0).
 """
-function update_markers_after_stokes_gpu!(st::GPUState, model, inside_flags::Vector{Int8})
+function update_markers_after_stokes_gpu!(st::GPUState, model, inside_flags::Vector{Int8};
+                                          correct_stress::Bool=false)
     set_grid!(st, model)
     m = model.markers.arrays
     sc = model.stokes_continuity.arrays
@@ -418,8 +423,13 @@ function update_markers_after_stokes_gpu!(st::GPUState, model, inside_flags::Vec
     outs = (m.strain.marker_exx.array, m.strain.marker_exy.array, m.pressure.marker_pr.array,
             m.strain.marker_sr_ratio.array)
     keys = (:mexx, :mexy, :mpr, :msr)
+    if correct_stress
+        outs = (outs..., m.stress.marker_sxx.array, m.stress.marker_sxy.array)
+        keys = (keys..., :sxx, :sxy)
+    end
     # outside markers keep their values, so the outputs travel both ways
-    gout = MarkerGatherOut((h2d!(st, k, x) for (k, x) in zip(keys, outs))...)
+    ptrs = [h2d!(st, k, x) for (k, x) in zip(keys, outs)]
+    gout = MarkerGatherOut(ptrs..., ntuple(_ -> C_NULL, 6 - length(ptrs))...)
     check(ccall((:ebg_marker_strain_rate_and_pressure, LIB), Cint,
         (Ptr{Cvoid}, Ref{MarkerGatherIn}, Ref{MarkerGatherOut}), st.ctx, gin, gout),
         "ebg_marker_strain_rate_and_pressure")

@@ -277,9 +277,11 @@ typedef struct {
 } ebg_marker_gather_in;
 
 /* Per-marker DEVICE outputs; each may be NULL (not computed).  Markers whose
- * flag is not 1 keep their values. */
+ * flag is not 1 keep their values.  marker_sxx and marker_sxy are updated in
+ * place: the remaining grid stress change dsxx, dsxy at the marker is added. */
 typedef struct {
     double *marker_exx, *marker_exy, *marker_pr, *marker_sr_ratio;
+    double *marker_sxx, *marker_sxy;
 } ebg_marker_gather_out;
 
 /* Replaces, in one pass over the markers, the marker steps of
@@ -291,11 +293,17 @@ typedef struct {
  *     (stokes_continuity_solver/utils/strain_rate/MarkerStrainRateRatio.jl:71-188),
  *   MarkerPressure.calculate_marker_pressure!
  *     (stokes_continuity_solver/utils/MarkerPressure.jl:17-49),
+ * then StressCorrection.apply_marker_stress_correction_for_remaining_grid_stress!
+ *   (stokes_continuity_solver/subgrid_stress/stress_correction/StressCorrection.jl:27-83)
+ *   when marker_sxx and marker_sxy are given (the step the reference runs
+ *   alone when subgrid_diff_coef_stress is 0; the subgrid stress update it runs
+ *   first otherwise is not part of this call),
  * with the normalized distances of
  * MarkerGridMapping.calc_normalized_distances_from_upper_left_basic_node!
  * (interpolation/mapping/MarkerGridMapping.jl:185-218) formed in-register.
  * marker_exx, marker_exy and marker_pr need exx, exy and pr1; marker_sr_ratio
- * needs exx, exy, sxx2, sxy2, dsxx, dsxy, eta, matid and mat_mu.  FP64 in the
+ * needs exx, exy, sxx2, sxy2, dsxx, dsxy, eta, matid and mat_mu; marker_sxx
+ * and marker_sxy need dsxx and dsxy.  FP64 in the
  * reference's operand order with correctly rounded division: bit-identical. */
 int ebg_marker_strain_rate_and_pressure(ebg_ctx *ctx, const ebg_marker_gather_in *in,
                                         ebg_marker_gather_out *out);

@@ -298,10 +298,12 @@ def marker_strain_rate_and_pressure(g, m: dict, f: dict, timestep=0.0, prev: dic
     calculate_marker_pressure!, in the reference's order.  m holds x, y, xn, yn
     (1-based Int32), inside (Int8) and, for the ratio, eta, matid (Int16) and
     mat_mu; f holds the grid arrays.  prev: marker arrays the loops start from
-    (zeros by default); outside markers keep them."""
+    (zeros by default); outside markers keep them.  With marker_sxx and
+    marker_sxy in m, StressCorrection.apply_marker_stress_correction_for_
+    remaining_grid_stress! follows and the corrected stresses are returned."""
     L = lib()
     L.ebo_marker_strain_rate_and_pressure.restype = None
-    L.ebo_marker_strain_rate_and_pressure.argtypes = [C.POINTER(_GatherIn)] + [_P] * 6
+    L.ebo_marker_strain_rate_and_pressure.argtypes = [C.POINTER(_GatherIn)] + [_P] * 8
     keep = []
 
     def arr(v, dt):
@@ -315,16 +317,22 @@ def marker_strain_rate_and_pressure(g, m: dict, f: dict, timestep=0.0, prev: dic
     ax = [arr(getattr(g, k), np.float64) for k in ("gridx_b", "gridy_b", "xstp_b", "ystp_b", "gridx_vy",
                                                   "xstp_vy", "gridy_vx", "ystp_vx")]
     ratio = "sxx2" in f
+    stress = "marker_sxx" in m
     s = _GatherIn(g.xnum, g.ynum, n, *ax, arr(m["x"], np.float64), arr(m["y"], np.float64),
                   arr(m["xn"], np.int32), arr(m["yn"], np.int32), arr(m["inside"], np.int8),
                   arr(f["exx"], np.float64), arr(f["exy"], np.float64), arr(f["pr1"], np.float64),
-                  *[arr(f.get(k) if ratio else None, np.float64) for k in ("sxx2", "sxy2", "dsxx", "dsxy")],
+                  arr(f.get("sxx2") if ratio else None, np.float64), arr(f.get("sxy2") if ratio else None, np.float64),
+                  *[arr(f.get(k) if (ratio or stress) else None, np.float64) for k in ("dsxx", "dsxy")],
                   arr(m.get("eta") if ratio else None, np.float64),
                   arr(m.get("matid") if ratio else None, np.int16),
                   arr(m.get("mat_mu") if ratio else None, np.float64), float(timestep))
     prev = prev or {}
     out = {k: np.array(prev[k], dtype=np.float64, copy=True) if k in prev else np.zeros(n) for k in GATHER_OUT}
-    L.ebo_marker_strain_rate_and_pressure(C.byref(s), *[p(out[k]) for k in GATHER_OUT])
+    for k in ("marker_sxx", "marker_sxy"):
+        if stress:
+            out[k] = np.array(m[k], dtype=np.float64, copy=True)
+    L.ebo_marker_strain_rate_and_pressure(C.byref(s), *[p(out[k]) for k in GATHER_OUT],
+                                          *[p(out.get(k)) for k in ("marker_sxx", "marker_sxy")])
     return out
 
 

@@ -75,7 +75,7 @@ typedef struct {
 
 void ebo_marker_strain_rate_and_pressure(const ebo_gather_in *s, double *marker_dx, double *marker_dy,
                                          double *marker_exx, double *marker_exy, double *marker_sr_ratio,
-                                         double *marker_pr);
+                                         double *marker_pr, double *marker_sxx, double *marker_sxy);
 
 int64_t ebo_stokes_build(const ebo_stokes_in *s, int64_t *Li, int64_t *Lj, double *Lv, double *R);
 double ebo_pressure_scaling_coeff(double etan11, double xstpavg, double ystpavg);

@@ -8,6 +8,8 @@
  *     (stokes_continuity_solver/utils/strain_rate/MarkerStrainRateRatio.jl:71-188),
  *   MarkerPressure.calculate_marker_pressure!
  *     (stokes_continuity_solver/utils/MarkerPressure.jl:17-49),
+ *   StressCorrection.apply_marker_stress_correction_for_remaining_grid_stress!
+ *     (stokes_continuity_solver/subgrid_stress/stress_correction/StressCorrection.jl:27-83),
  * with MarkerGridMapping.upr_left_index_pressure / upr_left_dist_pressure
  * (MarkerGridMapping.jl:311-350) and GridToMarker.get_marker_value /
  * calc_marker_weights (interpolation/utils/GridToMarker.jl:190-226).  Arrays
@@ -59,7 +61,7 @@ static double marker_strain_rate_ratio(double dt, double sxx, double sxy, double
 
 void ebo_marker_strain_rate_and_pressure(const ebo_gather_in *s, double *marker_dx, double *marker_dy,
                                          double *marker_exx, double *marker_exy, double *marker_sr_ratio,
-                                         double *marker_pr)
+                                         double *marker_pr, double *marker_sxx, double *marker_sxy)
 {
     const int64_t xnum = s->xnum, ynum = s->ynum;
     for (int64_t m = 0; m < s->marknum; ++m)  /* calc_normalized_distances_from_upper_left_basic_node! */
@@ -104,4 +106,15 @@ void ebo_marker_strain_rate_and_pressure(const ebo_gather_in *s, double *marker_
             const double dxpr = upr_left_dist_pressure(xpr, s->x[m], s->gridx_vy, s->xstp_vy);
             marker_pr[m] = get_marker_value(ypr, xpr, dypr, dxpr, s->pr1, ynum - 1);
         }
+    if (marker_sxx && marker_sxy)
+        for (int64_t m = 0; m < s->marknum; ++m)  /* apply_marker_stress_correction_for_remaining_grid_stress! */
+            if (s->inside[m] == 1) {
+                const double sxy = marker_sxy[m], sxx = marker_sxx[m];
+                marker_sxy[m] = sxy + get_marker_value(s->yn[m], s->xn[m], marker_dy[m], marker_dx[m], s->dsxy, ynum);
+                const int32_t ypr = upr_left_index_pressure(ynum, s->yn[m], s->y[m], s->gridy_vx);
+                const int32_t xpr = upr_left_index_pressure(xnum, s->xn[m], s->x[m], s->gridx_vy);
+                const double dypr = upr_left_dist_pressure(ypr, s->y[m], s->gridy_vx, s->ystp_vx);
+                const double dxpr = upr_left_dist_pressure(xpr, s->x[m], s->gridx_vy, s->xstp_vy);
+                marker_sxx[m] = sxx + get_marker_value(ypr, xpr, dypr, dxpr, s->dsxx, ynum - 1);
+            }
 }

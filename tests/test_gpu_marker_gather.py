@@ -7,7 +7,8 @@ import torch
 
 import oracle
 from earthbox_gpu.device import to_device, to_host
-from earthbox_gpu.marker_gather import (OUTPUTS, calculate_marker_pressure, calculate_marker_strain_rate,
+from earthbox_gpu.marker_gather import (OUTPUTS, STRESS, apply_marker_stress_correction_for_remaining_grid_stress,
+                                        calculate_marker_pressure, calculate_marker_strain_rate,
                                         update_markers_after_stokes)
 from gather_cases import BASIC, PRESSURE, gather_case
 
@@ -44,6 +45,15 @@ def test_marker_gathers_bit_exact(gpu_ctx, nx, ny, n, seed, flags):
     gpu_ctx.synchronize()
     for k in OUTPUTS:
         assert np.array_equal(to_host(out[k]), exp[k]), k
+    # again with the stress correction fused into the same pass
+    rng = np.random.default_rng(seed)
+    s0 = {"marker_sxx": 1e6 * rng.standard_normal(n), "marker_sxy": 1e6 * rng.standard_normal(n)}
+    exp = oracle.marker_strain_rate_and_pressure(g, dict(m, **s0), f, dt, prev=prev)
+    out = {k: to_device(v) for k, v in list(prev.items()) + list(s0.items())}
+    update_markers_after_stokes(gpu_ctx, dm, df, dt, out, OUTPUTS + STRESS)
+    gpu_ctx.synchronize()
+    for k in OUTPUTS + STRESS:
+        assert np.array_equal(to_host(out[k]), exp[k]), k
 
 
 def test_marker_gathers_subsets_bit_exact(gpu_ctx):
@@ -58,7 +68,14 @@ def test_marker_gathers_subsets_bit_exact(gpu_ctx):
     pr = calculate_marker_pressure(gpu_ctx, dm, {"pr1": df["pr1"]})
     sr = calculate_marker_strain_rate(gpu_ctx, dm, {"exx": df["exx"], "exy": df["exy"]})
     ra = update_markers_after_stokes(gpu_ctx, dm, df, dt, outputs=("marker_sr_ratio",))
+    rng = np.random.default_rng(9)
+    s0 = {"marker_sxx": rng.standard_normal(m["x"].size), "marker_sxy": rng.standard_normal(m["x"].size)}
+    st = {k: to_device(v) for k, v in s0.items()}
+    apply_marker_stress_correction_for_remaining_grid_stress(gpu_ctx, dm, {"dsxx": df["dsxx"], "dsxy": df["dsxy"]}, st)
+    exp_s = oracle.marker_strain_rate_and_pressure(g, dict(m, **s0), f, dt)
     gpu_ctx.synchronize()
+    assert np.array_equal(to_host(st["marker_sxx"]), exp_s["marker_sxx"])
+    assert np.array_equal(to_host(st["marker_sxy"]), exp_s["marker_sxy"])
     assert set(pr) == {"marker_pr"} and set(sr) == {"marker_exx", "marker_exy"}
     assert np.array_equal(to_host(pr["marker_pr"]), exp["marker_pr"])
     assert np.array_equal(to_host(sr["marker_exx"]), exp["marker_exx"])

@@ -120,3 +120,26 @@ def test_ratio_defaults_to_one_without_strain_and_outside_markers_keep_values():
     for k in oracle.GATHER_OUT:
         assert np.all(r[k][~ins] == -9.0), k
     assert (~ins).sum() >= 2
+
+
+def test_stress_correction_adds_affine_change_and_skips_outside_markers():
+    """StressCorrection: marker_sxx += dsxx and marker_sxy += dsxy at the
+    marker; affine stress changes on either grid are recovered exactly up to
+    rounding, and markers outside the grid are left alone."""
+    g, m, f = gather_case(23, 17, 4000, 8)
+    rng = np.random.default_rng(8)
+    sxx0, sxy0 = 1e6 * rng.standard_normal(4000), 1e6 * rng.standard_normal(4000)
+    cx, cy = (2.0e5, 3.0, -1.0), (-1.0e5, -2.0, 4.0)
+    h = dict(f)
+    Y, X = node_coords(g, "pressure")
+    h["dsxx"] = np.asfortranarray(cx[0] + cx[1] * X + cx[2] * Y)
+    Y, X = node_coords(g, "basic")
+    h["dsxy"] = np.asfortranarray(cy[0] + cy[1] * X + cy[2] * Y)
+    r = oracle.marker_strain_rate_and_pressure(g, dict(m, marker_sxx=sxx0, marker_sxy=sxy0), h, 1.0)
+    ins = m["inside"] == 1
+    ex = cx[0] + cx[1] * m["x"] + cx[2] * m["y"]
+    ey = cy[0] + cy[1] * m["x"] + cy[2] * m["y"]
+    np.testing.assert_allclose(r["marker_sxx"][ins] - sxx0[ins], ex[ins], rtol=0, atol=1e-9 * 1e6)
+    np.testing.assert_allclose(r["marker_sxy"][ins] - sxy0[ins], ey[ins], rtol=0, atol=1e-9 * 1e6)
+    assert np.array_equal(r["marker_sxx"][~ins], sxx0[~ins])
+    assert np.array_equal(r["marker_sxy"][~ins], sxy0[~ins])
