@@ -4,18 +4,33 @@
   assembled system they vanish (to rounding) at every node, except where the
   assembled system replaces the momentum equation (internal velocity zone
   rows, which the build places at node (i+1, j+1) of the zone's cell indices)
-  and, with interface stabilization, the y rows (the build adds the
-  stabilization terms; the residual formula does not).  A dropped term or a
+  and, with interface stabilization, the y rows, where the residual equals
+  exactly the stabilization terms the build added (the residual formula has
+  none; the terms themselves are pinned in
+  test_oracle_stokes_stabilization.py).  A dropped term or a
   wrong sign would leave O(1) residuals at interior nodes.
 - Norms (MathTools.jl:64-84): against numpy norms of the difference, and the
   1e38 convention for a zero reference vector.
 - Time step (Update.jl:43-98): closed form for known maxima.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
 import oracle
 from stokes_cases import CASES, random_case
+
+
+def _pack(pr, vx1, vy1, pr1):
+    """S in the reference's unknown order (StokesBuildManager.jl:339-431)."""
+    nx, ny = pr.grid.xnum, pr.grid.ynum
+    S = np.empty(pr.N)
+    for j in range(1, nx):
+        for i in range(1, ny):
+            c = (j - 1) * (ny - 1) + i
+            S[3 * c - 3], S[3 * c - 2], S[3 * c - 1] = vx1[i, j], vy1[i, j], pr1[i - 1, j - 1]
+    return S
 
 
 def _solved(case):
@@ -30,7 +45,21 @@ def test_residuals_of_direct_solution_vanish(case):
     pr, (vx, vy, p) = _solved(case)
     rx, ry, rc = oracle.stokes_grid_residuals(pr, vx, vy, p)
     if pr.iuse_interface_stabilization:
-        ry = ry * 0.0  # the build's stabilization terms are not in YStokes.jl
+        # YStokes.jl has no stabilization terms, so at the y rows the grid
+        # residual of the stabilized solution is what the build added: the
+        # stabilization coefficients applied to S minus their right-hand-side
+        # part (the folded boundary values)
+        Li, Lj, Lv, R = oracle.stokes_build(pr)
+        Lp = oracle.stokes_build(dataclasses.replace(pr, iuse_interface_stabilization=0))
+        S = _pack(pr, vx, vy, p)
+        T = oracle.coo_spmv(pr.N, Li, Lj, Lv - Lp[2], S) - (R - Lp[3])
+        nx, ny = pr.grid.xnum, pr.grid.ynum
+        added = np.zeros_like(ry)
+        for j in range(1, nx):
+            for i in range(1, ny):
+                added[i, j] = T[3 * ((j - 1) * (ny - 1) + i) - 2]
+        assert np.abs(added).max() > 1e-3 * np.abs(ry).max()
+        ry = ry - added
     assert np.abs(rx).max() <= 1e-11 * np.abs(pr.RX1).max()
     assert np.abs(ry).max() <= 1e-11 * max(np.abs(pr.RY1).max(), 1.0)
     assert np.abs(rc).max() <= 1e-9
