@@ -3,8 +3,10 @@
 
 Workload (1001 x 501 nodes over 1500 x 600 km, 6 x 6 markers per cell =
 18,000,000 markers, FP64): one step = one fused RK4 gather-advect pass of all
-markers (velocity + spin at 4 stages, position update) with inputs resident in
-HBM.  Prints one JSON line.
+markers (velocity at 4 stages, position update) with inputs resident in HBM,
+in the tolerance arithmetic by default (positions within 1e-10 of the domain
+size; the exact, bit-identical kernel is timed beside it).  Prints one JSON
+line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -24,11 +26,13 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 RK4_BYTES_PER_MARKER = 32.0  # read x, y; write x, y (FP64)
-# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
-# workload (18M markers, 1001 x 501 nodes), each counter scaled by its reading
-# for a plain 512 MiB copy: 385.4 MB read + 288.0 MB written.
-RK4_MEASURED_TRAFFIC_BYTES = 673.4e6
-RK4_TRAFFIC_WORKLOAD = (1001, 501, 6)
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the
+# 1001 x 501-node, 6 x 6 markers-per-cell workload (18M markers), each counter
+# scaled by its reading for a plain 512 MiB copy, keyed by kernel and workload.
+RK4_MEASURED_TRAFFIC = {
+    ("rk4_fast_kernel", 1001, 501, 6): 640.1e6,     # 352.3 MB read + 287.8 MB written
+    ("rk4_advect_kernel", 1001, 501, 6): 641.3e6,   # 353.4 MB read + 288.0 MB written
+}
 
 
 def parse():
@@ -40,6 +44,10 @@ def parse():
     ap.add_argument("--xnum", type=int, default=1001)
     ap.add_argument("--ynum", type=int, default=501)
     ap.add_argument("--markers-per-cell", type=int, default=6)
+    ap.add_argument("--rk4-arithmetic", choices=("tolerance", "exact"), default="tolerance")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run three plain 512 MiB device copies (the reference kernel for "
+                         "scaling the FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--no-stokes", action="store_true")
     ap.add_argument("--no-scatter", action="store_true")
     ap.add_argument("--stokes-max-iter", type=int, default=3000)
@@ -416,27 +424,54 @@ def bench_picard(ctx, nglobal=10, tol=1e-4):
     return out
 
 
+def _host_cpu():
+    """CPU model, logical CPUs and the OpenMP thread setting of this host."""
+    model = platform.processor() or "unknown"
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "sched_affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(g, x, y, vx1, vy1, esp, dt, sample_markers: int):
-    """Oracle RK4 restatement timed on this host (test-infrastructure checker
-    used only as the reported CPU baseline)."""
-    import numpy as np
+    """The oracle's RK4 restatement (threaded over markers as the reference's
+    Threads.@threads loop, VelocityAndSpin.jl:77) timed on this host: on all
+    the cores this process may use, and on one core.  Test-infrastructure
+    checker used only as the reported CPU baseline."""
     import oracle
+    host = _host_cpu()
+    all_cores = int(os.environ.get("OMP_NUM_THREADS") or host["sched_affinity"] or os.cpu_count() or 1)
     xs, ys = x[:sample_markers].copy(), y[:sample_markers].copy()
     flags = oracle.inside_flags(xs, ys, g.limits)
     xn, yn = oracle.cell_indices_basic(xs, ys, flags, g)
-    t0 = time.perf_counter()
-    reps = 0
-    while True:
-        oracle.advect_markers_and_rotate_stress_tensor(4, dt, g, xs, ys, xn, yn, flags,
-                                                       vx1, vy1, esp)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el > 10.0:
-            break
-    rate = reps * sample_markers / el / 1e6
-    return {"value": rate, "unit": "Mmarkers/s", "cores": 1, "kind": "port",
-            "sample": f"{reps} x RK4 advect of the first {sample_markers} markers of the same "
-                      f"workload (serial C restatement, {platform.processor() or 'host'} core)"}
+
+    def rate(threads, budget):
+        oracle.set_threads(threads)
+        oracle.advect_markers_and_rotate_stress_tensor(4, dt, g, xs, ys, xn, yn, flags, vx1, vy1, esp)
+        times = []
+        t_end = time.perf_counter() + budget
+        while time.perf_counter() < t_end or len(times) < 3:
+            t0 = time.perf_counter()
+            oracle.advect_markers_and_rotate_stress_tensor(4, dt, g, xs, ys, xn, yn, flags,
+                                                           vx1, vy1, esp)
+            times.append(time.perf_counter() - t0)
+        return sample_markers / float(np.median(times)) / 1e6, len(times)
+
+    import numpy as np
+    r_all, n_all = rate(all_cores, 8.0)
+    r_one, n_one = rate(1, 8.0)
+    oracle.set_threads(all_cores)
+    return {"value": r_all, "unit": "Mmarkers/s", "cores": all_cores, "kind": "port",
+            "single_core_value": r_one, "host": host,
+            "sample": f"RK4 advect of the first {sample_markers} markers of the same workload "
+                      f"(oracle C restatement, OpenMP over markers): median of {n_all} runs on "
+                      f"{all_cores} threads, {n_one} runs on 1 thread"}
 
 
 def bench_marker_gathers(ctx, g, dx, dy, reps=10):
@@ -487,6 +522,15 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
+    if args.calibrate_copy:
+        a = torch.ones(64 * 2 ** 20, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": a.numel() * 8, "bytes_written": b.numel() * 8,
+                          "dispatches": 3}))
+        return
     import earthbox_gpu
     from earthbox_gpu import workloads
     from earthbox_gpu.device import empty, to_device
@@ -509,24 +553,41 @@ def main():
         ctx.advect_markers_and_rotate_stress_tensor(4, dt, None, dx, dy, None, None,
                                                     dvx1, dvy1, desp)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s, e in evs:
-        s.record(stream)
-        step()
-        e.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = float(np.mean([s.elapsed_time(e) for s, e in evs]))
+    def timed(mode):
+        """W untimed steps, then K steps each bracketed by HIP events on the
+        library's stream (torch's current stream); wall time over the K."""
+        ctx.set_rk4_arithmetic(mode)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s, e in evs:
+            s.record(stream)
+            step()
+            e.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return wall, float(np.mean([s.elapsed_time(e) for s, e in evs]))
+
+    # the exact (bit-identical) kernel, reported beside the headline
+    wall_x, kern_x = timed(ctx.RK4_EXACT)
+    ex_ms = wall_x * 1e3 / args.steps
+    rk4_exact = {"kernel": "rk4_advect_kernel", "arithmetic": "exact (bit-identical to the oracle)",
+                 "ms_per_step": ex_ms, "Mmarkers_per_s": n / (ex_ms * 1e-3) / 1e6, "kernel_ms": kern_x,
+                 "traffic": RK4_MEASURED_TRAFFIC.get(("rk4_advect_kernel", nx, ny, nm)),
+                 "frac_of_8TBs": RK4_BYTES_PER_MARKER * n / (kern_x * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    mode = ctx.RK4_TOLERANCE if args.rk4_arithmetic == "tolerance" else ctx.RK4_EXACT
+    fast = mode == ctx.RK4_TOLERANCE and ctx.rk4_tolerance_kernel_available()
+    wall, kern_ms = timed(mode)
     ms_per_step = wall * 1e3 / args.steps
     value = n / (ms_per_step * 1e-3) / 1e6
     achieved = RK4_BYTES_PER_MARKER * n / (kern_ms * 1e-3) / 1e9
+    kname = "rk4_fast_kernel" if fast else "rk4_advect_kernel"
+    traffic = RK4_MEASURED_TRAFFIC.get((kname, nx, ny, nm))
     out = {
         "metric": "Stokes time-to-1e-8 rel. residual (learned+mixed vs FP64 MG); Mmarkers/s RK4",
         "value": value,
@@ -541,12 +602,14 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"rk4_advect {nx}x{ny} nodes, {nm}x{nm} markers/cell ({n} markers)",
-                   "markers": n, "xnum": nx, "ynum": ny},
+                   "markers": n, "xnum": nx, "ynum": ny,
+                   "arithmetic": ("tolerance: FMA, nominal uniform coordinates; positions within 1e-10 "
+                                  "of the domain size of the exact kernel" if fast else "exact")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (RK4_MEASURED_TRAFFIC_BYTES if (nx, ny, nm) == RK4_TRAFFIC_WORKLOAD else None),
-                     "kernel": "rk4_advect_kernel", "kernel_ms": kern_ms,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": kname, "kernel_ms": kern_ms,
                      "bytes_per_unit": RK4_BYTES_PER_MARKER},
+        "rk4_exact": rk4_exact,
     }
     if not args.no_marker_gathers:
         out["marker_gathers"] = bench_marker_gathers(ctx, g, dx, dy)

@@ -159,6 +159,7 @@ __device__ __forceinline__ StageWeights stage_weights(const AxisCell& cx, const 
     return w;
 }
 
+template <bool SPIN>
 __device__ __forceinline__ void point_values(const RK4Args& a, double xc, double yc, int xi,
                                              int yi, double& vx, double& vy, double& sp)
 {
@@ -167,7 +168,7 @@ __device__ __forceinline__ void point_values(const RK4Args& a, double xc, double
     const StageWeights w = stage_weights(cx, cy, xc, yc);
     vx = marker_value(yi + (w.ry ? 1 : 0), xi, w.dyv, w.dxb, a.vx1, a.ynum + 1);
     vy = marker_value(yi, xi + (w.rx ? 1 : 0), w.dyb, w.dxy, a.vy1, a.ynum);
-    sp = marker_value(yi, xi, w.dyb, w.dxb, a.esp, a.ynum);
+    if (SPIN) sp = marker_value(yi, xi, w.dyb, w.dxb, a.esp, a.ynum);
 }
 
 // (v1 + 2 v2 + 2 v3 + v4)/6 in the reference's order (RungeKutta.jl:36-47),
@@ -183,8 +184,9 @@ __device__ __forceinline__ double rk4_sixth(double s)
 // (MarkersLocation.jl:36-41) and stress rotation (MarkersStressRotation.jl:60-74)
 // fused per marker.  F: RK_RK4 (runge_kutta_order_max 4, else one stage),
 // RK_XN (stage-1 cells given), RK_STRESS (rotate sxx, sxy), RK_UNIFORM (both
-// axes uniform).
-enum { RK_RK4 = 1, RK_XN = 2, RK_STRESS = 4, RK_UNIFORM = 8 };
+// axes uniform), RK_SPIN (marker spin wanted; the stress rotation implies
+// it).  Without spin outputs the spin field is not gathered.
+enum { RK_RK4 = 1, RK_XN = 2, RK_STRESS = 4, RK_UNIFORM = 8, RK_SPIN = 16 };
 
 template <int F>
 __global__ void __launch_bounds__(256) rk4_advect_kernel(RK4Args a)
@@ -192,6 +194,7 @@ __global__ void __launch_bounds__(256) rk4_advect_kernel(RK4Args a)
     constexpr int NST = (F & RK_RK4) ? 4 : 1;
     constexpr bool XN = (F & RK_XN) != 0, STRESS = (F & RK_STRESS) != 0;
     constexpr bool UNI = (F & RK_UNIFORM) != 0;
+    constexpr bool SPIN = (F & (RK_SPIN | RK_STRESS)) != 0;
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= a.marknum) return;
     const double x0 = a.x[m];
@@ -214,28 +217,28 @@ __global__ void __launch_bounds__(256) rk4_advect_kernel(RK4Args a)
         xi = stage_cell<UNI>(x0, a.ax);
         yi = stage_cell<UNI>(y0, a.ay);
     }
-    double vx, vy, vs;
-    point_values(a, x0, y0, xi, yi, vx, vy, vs);
+    double vx, vy, vs = 0.0;
+    point_values<SPIN>(a, x0, y0, xi, yi, vx, vy, vs);
     if (NST == 4) {
         double ax = vx, ay = vy, as = vs;   // running ((v1 + 2 v2) + 2 v3) + v4
         double xc = x0 + a.half_dt * vx;
         double yc = y0 + a.half_dt * vy;
-        point_values(a, xc, yc, stage_cell<UNI>(xc, a.ax), stage_cell<UNI>(yc, a.ay), vx, vy, vs);
+        point_values<SPIN>(a, xc, yc, stage_cell<UNI>(xc, a.ax), stage_cell<UNI>(yc, a.ay), vx, vy, vs);
         ax = ax + 2.0 * vx;
         ay = ay + 2.0 * vy;
-        as = as + 2.0 * vs;
+        if (SPIN) as = as + 2.0 * vs;
         xc = x0 + a.half_dt * vx;
         yc = y0 + a.half_dt * vy;
-        point_values(a, xc, yc, stage_cell<UNI>(xc, a.ax), stage_cell<UNI>(yc, a.ay), vx, vy, vs);
+        point_values<SPIN>(a, xc, yc, stage_cell<UNI>(xc, a.ax), stage_cell<UNI>(yc, a.ay), vx, vy, vs);
         ax = ax + 2.0 * vx;
         ay = ay + 2.0 * vy;
-        as = as + 2.0 * vs;
+        if (SPIN) as = as + 2.0 * vs;
         xc = x0 + a.dt * vx;
         yc = y0 + a.dt * vy;
-        point_values(a, xc, yc, stage_cell<UNI>(xc, a.ax), stage_cell<UNI>(yc, a.ay), vx, vy, vs);
+        point_values<SPIN>(a, xc, yc, stage_cell<UNI>(xc, a.ax), stage_cell<UNI>(yc, a.ay), vx, vy, vs);
         svx = rk4_sixth(ax + vx);
         svy = rk4_sixth(ay + vy);
-        ssp = rk4_sixth(as + vs);
+        ssp = SPIN ? rk4_sixth(as + vs) : 0.0;
     } else {
         svx = vx;
         svy = vy;
@@ -243,7 +246,7 @@ __global__ void __launch_bounds__(256) rk4_advect_kernel(RK4Args a)
     }
     if (a.mvx) a.mvx[m] = svx;
     if (a.mvy) a.mvy[m] = svy;
-    if (a.mspin) a.mspin[m] = ssp;
+    if (SPIN && a.mspin) a.mspin[m] = ssp;
     if (a.update_location) {
         a.x[m] = x0 + a.dt * svx;
         a.y[m] = y0 + a.dt * svy;
@@ -259,6 +262,175 @@ __global__ void __launch_bounds__(256) rk4_advect_kernel(RK4Args a)
         a.sxx[m] = sxx_old * (c2 * c2 - s2 * s2) - sxy_old * s1;
     }
 }
+
+// ---------------------------------------------------------------------------
+// Tolerance-mode RK4 on nominally uniform axes (both axes uniform to 1e-9 of
+// a step, checked when the grid is set).  Same stages, remaps, clamps and
+// outputs as rk4_advect_kernel, with three changes of arithmetic:
+//  - the cell search is one FMA: t = (x - x0)/h, and the half-cell index
+//    k = clamp(floor(2t), 0, 2n-3) gives the basic cell (k >> 1) and the
+//    velocity-grid remap (k & 1, the side of the cell centre) at once;
+//  - the normalized distances come from the nominal node positions, so the
+//    basic one is t - c and the staggered one differs from it by +-1/2;
+//  - the bilinear value is the nested linear interpolation in FMAs, and the
+//    RK4 sum is multiplied by 1/6.
+// Spin is gathered only when an output needs it (SPIN).  The bilinear field
+// is continuous, so a different cell choice at a node or centre (where the
+// nominal and the stored coordinates round differently) changes the value by
+// rounding only.
+enum { RKF_RK4 = 1, RKF_XN = 2, RKF_STRESS = 4, RKF_SPIN = 8 };
+
+struct RK4FastArgs {
+    int64_t marknum;
+    double dt, half_dt;
+    double *x, *y;
+    const int32_t *xn, *yn;
+    const int8_t* flags;
+    const double *vx1, *vy1, *esp;
+    double *mvx, *mvy, *mspin;
+    double *sxx, *sxy;
+    int update_location;
+    int xnum, ynum;
+    double inv_hx, nx0, inv_hy, ny0;   // t = x * inv_h + n0 = (x - x0) / h
+    double kmax_x, kmax_y;             // 2n - 3: last half cell
+    double xmin, xmax, ymin, ymax;
+};
+
+struct FastCell {
+    int c;        // 0-based basic cell
+    int r;        // 1: right of (x) / below (y) the cell centre
+    double db;    // distance from the basic node, in cells
+    double ds;    // distance from the staggered node the remap picks
+};
+
+__device__ __forceinline__ FastCell fast_cell(double x, double inv_h, double n0, double kmax)
+{
+    FastCell f;
+    const double t = __builtin_fma(x, inv_h, n0);
+    const int k = (int)fmin(fmax(t + t, 0.0), kmax);   // NaN -> 0
+    f.c = k >> 1;
+    f.r = k & 1;
+    f.db = t - (double)f.c;
+    f.ds = f.db + (f.r ? -0.5 : 0.5);
+    return f;
+}
+
+// Cell from a given 1-based basic index (the stage-1 marker_xn/yn, clamped
+// to the grid): the remap side follows from the nominal distance.
+__device__ __forceinline__ FastCell fast_cell_given(double x, int c1, double inv_h, double n0, int n)
+{
+    FastCell f;
+    const double t = __builtin_fma(x, inv_h, n0);
+    f.c = min(max(c1 - 1, 0), n - 2);
+    f.db = t - (double)f.c;
+    f.r = f.db > 0.5;
+    f.ds = f.db + (f.r ? -0.5 : 0.5);
+    return f;
+}
+
+// nested linear interpolation: p points at the upper-left node, column-major
+__device__ __forceinline__ double lerp2(const double* __restrict__ p, int ld, double dy, double dx)
+{
+    const double p00 = p[0], p10 = p[1], p01 = p[ld], p11 = p[ld + 1];
+    const double a = __builtin_fma(dy, p10 - p00, p00);
+    const double b = __builtin_fma(dy, p11 - p01, p01);
+    return __builtin_fma(dx, b - a, a);
+}
+
+template <bool SPIN>
+__device__ __forceinline__ void fast_values(const RK4FastArgs& a, const FastCell& cx, const FastCell& cy,
+                                            double& vx, double& vy, double& sp)
+{
+    const int ldx = a.ynum + 1, ldy = a.ynum;
+    // vx grid: basic x cell, y remapped; vy grid: x remapped, basic y cell
+    vx = lerp2(a.vx1 + (unsigned)(cx.c * ldx + cy.c + cy.r), ldx, cy.ds, cx.db);
+    vy = lerp2(a.vy1 + (unsigned)((cx.c + cx.r) * ldy + cy.c), ldy, cy.db, cx.ds);
+    if (SPIN) sp = lerp2(a.esp + (unsigned)(cx.c * ldy + cy.c), ldy, cy.db, cx.db);
+}
+
+template <int F>
+__global__ void __launch_bounds__(256) rk4_fast_kernel(RK4FastArgs a)
+{
+    constexpr bool RK4 = (F & RKF_RK4) != 0, XN = (F & RKF_XN) != 0;
+    constexpr bool STRESS = (F & RKF_STRESS) != 0;
+    constexpr bool SPIN = (F & (RKF_SPIN | RKF_STRESS)) != 0;   // the rotation needs it
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= a.marknum) return;
+    const double x0 = a.x[m];
+    const double y0 = a.y[m];
+    bool inside;
+    if (a.flags) inside = (a.flags[m] == 1);
+    else inside = (a.xmin <= x0 && x0 <= a.xmax && a.ymin <= y0 && y0 <= a.ymax);
+    if (!inside) {  // VelocityAndSpin.jl:120-126; location and stress untouched
+        if (a.mvx) a.mvx[m] = 0.0;
+        if (a.mvy) a.mvy[m] = 0.0;
+        if (a.mspin) a.mspin[m] = 0.0;
+        return;
+    }
+    FastCell cx, cy;
+    if (XN) {
+        cx = fast_cell_given(x0, a.xn[m], a.inv_hx, a.nx0, a.xnum);
+        cy = fast_cell_given(y0, a.yn[m], a.inv_hy, a.ny0, a.ynum);
+    } else {
+        cx = fast_cell(x0, a.inv_hx, a.nx0, a.kmax_x);
+        cy = fast_cell(y0, a.inv_hy, a.ny0, a.kmax_y);
+    }
+    double vx, vy, vs = 0.0;
+    fast_values<SPIN>(a, cx, cy, vx, vy, vs);
+    double svx = vx, svy = vy, ssp = vs;
+    if (RK4) {
+        double ax = vx, ay = vy, as = vs;
+        double xc = __builtin_fma(a.half_dt, vx, x0);
+        double yc = __builtin_fma(a.half_dt, vy, y0);
+        fast_values<SPIN>(a, fast_cell(xc, a.inv_hx, a.nx0, a.kmax_x),
+                          fast_cell(yc, a.inv_hy, a.ny0, a.kmax_y), vx, vy, vs);
+        ax = __builtin_fma(2.0, vx, ax);
+        ay = __builtin_fma(2.0, vy, ay);
+        if (SPIN) as = __builtin_fma(2.0, vs, as);
+        xc = __builtin_fma(a.half_dt, vx, x0);
+        yc = __builtin_fma(a.half_dt, vy, y0);
+        fast_values<SPIN>(a, fast_cell(xc, a.inv_hx, a.nx0, a.kmax_x),
+                          fast_cell(yc, a.inv_hy, a.ny0, a.kmax_y), vx, vy, vs);
+        ax = __builtin_fma(2.0, vx, ax);
+        ay = __builtin_fma(2.0, vy, ay);
+        if (SPIN) as = __builtin_fma(2.0, vs, as);
+        xc = __builtin_fma(a.dt, vx, x0);
+        yc = __builtin_fma(a.dt, vy, y0);
+        fast_values<SPIN>(a, fast_cell(xc, a.inv_hx, a.nx0, a.kmax_x),
+                          fast_cell(yc, a.inv_hy, a.ny0, a.kmax_y), vx, vy, vs);
+        constexpr double r6 = 1.0 / 6.0;
+        svx = (ax + vx) * r6;
+        svy = (ay + vy) * r6;
+        if (SPIN) ssp = (as + vs) * r6;
+    }
+    if (a.mvx) a.mvx[m] = svx;
+    if (a.mvy) a.mvy[m] = svy;
+    if (SPIN && a.mspin) a.mspin[m] = ssp;
+    if (a.update_location) {
+        a.x[m] = __builtin_fma(a.dt, svx, x0);
+        a.y[m] = __builtin_fma(a.dt, svy, y0);
+    }
+    if (STRESS) {
+        const double t1 = 2.0 * ssp * a.dt;
+        const double t2 = ssp * a.dt;
+        double s1, c1, s2, c2;
+        sincos(t1, &s1, &c1);
+        sincos(t2, &s2, &c2);
+        const double sxy_old = a.sxy[m], sxx_old = a.sxx[m];
+        a.sxy[m] = sxx_old * s1 + sxy_old * c1;
+        a.sxx[m] = sxx_old * (c2 * c2 - s2 * s2) - sxy_old * s1;
+    }
+}
+
+static const void* const rk4_fast_kernels[16] = {
+    (const void*)rk4_fast_kernel<0>,  (const void*)rk4_fast_kernel<1>,
+    (const void*)rk4_fast_kernel<2>,  (const void*)rk4_fast_kernel<3>,
+    (const void*)rk4_fast_kernel<4>,  (const void*)rk4_fast_kernel<5>,
+    (const void*)rk4_fast_kernel<6>,  (const void*)rk4_fast_kernel<7>,
+    (const void*)rk4_fast_kernel<8>,  (const void*)rk4_fast_kernel<9>,
+    (const void*)rk4_fast_kernel<10>, (const void*)rk4_fast_kernel<11>,
+    (const void*)rk4_fast_kernel<12>, (const void*)rk4_fast_kernel<13>,
+    (const void*)rk4_fast_kernel<14>, (const void*)rk4_fast_kernel<15>};
 
 __global__ void inside_flags_kernel(int64_t n, const double* __restrict__ x,
                                     const double* __restrict__ y, double xmin, double xmax,
@@ -306,15 +478,14 @@ static AxisDesc axis_y(const Grid2D& g)
 
 static unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
-static const void* const rk4_kernels[16] = {
-    (const void*)rk4_advect_kernel<0>,  (const void*)rk4_advect_kernel<1>,
-    (const void*)rk4_advect_kernel<2>,  (const void*)rk4_advect_kernel<3>,
-    (const void*)rk4_advect_kernel<4>,  (const void*)rk4_advect_kernel<5>,
-    (const void*)rk4_advect_kernel<6>,  (const void*)rk4_advect_kernel<7>,
-    (const void*)rk4_advect_kernel<8>,  (const void*)rk4_advect_kernel<9>,
-    (const void*)rk4_advect_kernel<10>, (const void*)rk4_advect_kernel<11>,
-    (const void*)rk4_advect_kernel<12>, (const void*)rk4_advect_kernel<13>,
-    (const void*)rk4_advect_kernel<14>, (const void*)rk4_advect_kernel<15>};
+#define EBG_RK4_K(f) (const void*)rk4_advect_kernel<f>
+#define EBG_RK4_K8(b) EBG_RK4_K(b), EBG_RK4_K(b + 1), EBG_RK4_K(b + 2), EBG_RK4_K(b + 3), \
+    EBG_RK4_K(b + 4), EBG_RK4_K(b + 5), EBG_RK4_K(b + 6), EBG_RK4_K(b + 7)
+// rk4_advect_kernel<F> for every flag combination F in [0, 32)
+static const void* const rk4_kernels[32] = {EBG_RK4_K8(0), EBG_RK4_K8(8), EBG_RK4_K8(16),
+                                            EBG_RK4_K8(24)};
+#undef EBG_RK4_K8
+#undef EBG_RK4_K
 
 }  // namespace ebg
 
@@ -369,6 +540,45 @@ int ebg_rk4_advect(ebg_ctx* ctx, int64_t marknum, int order, double timestep,
               "ebg_rk4_advect: marker_sxx and marker_sxy must both be given or both NULL");
     EBG_CHECK(std::isfinite(timestep), EBG_ERR_ARG, "ebg_rk4_advect: timestep not finite");
     const Grid2D& g = ctx->grid;
+    if (ctx->rk4_arithmetic == EBG_RK4_TOLERANCE && g.nominal_x && g.nominal_y) {
+        RK4FastArgs a;
+        a.marknum = marknum;
+        a.dt = timestep;
+        a.half_dt = timestep / 2.0;
+        a.x = marker_x;
+        a.y = marker_y;
+        a.xn = marker_xn;
+        a.yn = marker_yn;
+        a.flags = inside_flags;
+        a.vx1 = vx1;
+        a.vy1 = vy1;
+        a.esp = esp;
+        a.mvx = marker_vx;
+        a.mvy = marker_vy;
+        a.mspin = marker_spin;
+        a.sxx = marker_sxx;
+        a.sxy = marker_sxy;
+        a.update_location = update_location;
+        a.xnum = (int)g.xnum;
+        a.ynum = (int)g.ynum;
+        a.inv_hx = g.inv_hx;
+        a.nx0 = -g.x0 * g.inv_hx;
+        a.inv_hy = g.inv_hy;
+        a.ny0 = -g.y0 * g.inv_hy;
+        a.kmax_x = (double)(2 * g.xnum - 3);
+        a.kmax_y = (double)(2 * g.ynum - 3);
+        a.xmin = g.xmin;
+        a.xmax = g.xmax;
+        a.ymin = g.ymin;
+        a.ymax = g.ymax;
+        const int f = (order == 4 ? RKF_RK4 : 0) | (marker_xn ? RKF_XN : 0) |
+                      (marker_sxx ? RKF_STRESS : 0) | (marker_spin ? RKF_SPIN : 0);
+        void* args[] = {&a};
+        EBG_HIP(hipLaunchKernel(rk4_fast_kernels[f], dim3(blocks_for(marknum, 256)), dim3(256), args,
+                                0, ctx->stream));
+        EBG_HIP(hipGetLastError());
+        return EBG_OK;
+    }
     RK4Args a;
     a.marknum = marknum;
     a.dt = timestep;
@@ -396,12 +606,28 @@ int ebg_rk4_advect(ebg_ctx* ctx, int64_t marknum, int order, double timestep,
     a.ymin = g.ymin;
     a.ymax = g.ymax;
     const int f = (order == 4 ? RK_RK4 : 0) | (marker_xn ? RK_XN : 0) |
-                  (marker_sxx ? RK_STRESS : 0) | (g.uniform_x && g.uniform_y ? RK_UNIFORM : 0);
+                  (marker_sxx ? RK_STRESS : 0) | (g.uniform_x && g.uniform_y ? RK_UNIFORM : 0) |
+                  (marker_spin ? RK_SPIN : 0);
     void* args[] = {&a};
     EBG_HIP(hipLaunchKernel(rk4_kernels[f], dim3(blocks_for(marknum, 256)), dim3(256), args, 0,
                             ctx->stream));
     EBG_HIP(hipGetLastError());
     return EBG_OK;
+}
+
+int ebg_rk4_set_arithmetic(ebg_ctx* ctx, int mode)
+{
+    EBG_CHECK(ctx != nullptr, EBG_ERR_ARG, "ebg_rk4_set_arithmetic: ctx is NULL");
+    EBG_CHECK(mode == EBG_RK4_EXACT || mode == EBG_RK4_TOLERANCE, EBG_ERR_ARG,
+              "ebg_rk4_set_arithmetic: mode must be EBG_RK4_EXACT or EBG_RK4_TOLERANCE");
+    ctx->rk4_arithmetic = mode;
+    return EBG_OK;
+}
+
+int ebg_rk4_tolerance_kernel_available(ebg_ctx* ctx)
+{
+    if (!ctx || !ctx->has_grid) return -1;
+    return (ctx->grid.nominal_x && ctx->grid.nominal_y) ? 1 : 0;
 }
 
 int ebg_calculate_spin(ebg_ctx* ctx, const double* vx1, const double* vy1, double* esp)

@@ -54,6 +54,9 @@ struct Grid2D {
     // uniform-axis hints for O(1) cell search (exactness restored by fix-up)
     double x0 = 0, y0 = 0, inv_hx = 0, inv_hy = 0;
     int uniform_x = 0, uniform_y = 0;
+    // every node within 1e-9 of a step of x0 + (i-1)*h: the tolerance-mode
+    // RK4 may then use the nominal coordinates for its weights
+    int nominal_x = 0, nominal_y = 0;
     double *gridx_b = nullptr, *gridy_b = nullptr;
     double *xstp_b = nullptr, *ystp_b = nullptr;
     double *gridx_vy = nullptr, *xstp_vy = nullptr;
@@ -80,6 +83,7 @@ struct ebg_ctx {
     void* mg3_ref = nullptr;   // reference-order 3-D multigrid (ebg_mg3d_create)
     void* stokes3d = nullptr;  // 3-D Stokes solve state (ebg_stokes3d_set_system)
     void* plastic = nullptr;   // reduction scratch of ebg_update_nodes_for_plastic_yielding
+    int rk4_arithmetic = EBG_RK4_EXACT;   // ebg_rk4_set_arithmetic
 };
 
 namespace ebg {

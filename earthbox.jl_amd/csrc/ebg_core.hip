@@ -75,6 +75,22 @@ static int axis_uniform(const double* g, int64_t n, double* x0, double* inv_h)
     return 1;
 }
 
+// Basic nodes within 1e-9 of a step of x0 + (i-1)*h and staggered nodes
+// within 1e-9 of a step of x0 + (i-3/2)*h (including both ghost nodes): the
+// tolerance-mode RK4 then computes cells and weights from these nominal
+// positions, which moves a weight by at most 1e-9 of the cell.
+static int axis_nominal(const double* g, int64_t n, const double* stag)
+{
+    const double h = (g[n - 1] - g[0]) / (double)(n - 1);
+    if (!(h > 0)) return 0;
+    const double tol = 1e-9 * h;
+    for (int64_t i = 0; i < n; ++i)
+        if (!(std::fabs(g[i] - (g[0] + (double)i * h)) <= tol)) return 0;
+    for (int64_t i = 0; i <= n; ++i)
+        if (!(std::fabs(stag[i] - (g[0] + ((double)i - 0.5) * h)) <= tol)) return 0;
+    return 1;
+}
+
 int mg3ref_release(ebg_ctx* ctx);    // ebg_mg3d.hip
 int stokes3d_release(ebg_ctx* ctx);  // ebg_stokes3d.hip
 
@@ -224,6 +240,8 @@ int ebg_grid2d_set(ebg_ctx* ctx, int64_t xnum, int64_t ynum,
     g.ymax = gridy_b[ynum - 1];
     g.uniform_x = axis_uniform(gridx_b, xnum, &g.x0, &g.inv_hx);
     g.uniform_y = axis_uniform(gridy_b, ynum, &g.y0, &g.inv_hy);
+    g.nominal_x = g.uniform_x && axis_nominal(gridx_b, xnum, gridx_vy);
+    g.nominal_y = g.uniform_y && axis_nominal(gridy_b, ynum, gridy_vx);
     static uint64_t generations = 0;
     g.generation = ++generations;
     ctx->has_grid = true;

@@ -124,6 +124,17 @@ class GPUContext:
                   inside_flags, vx1, vy1, esp, marker_vx, marker_vy, marker_spin,
                   marker_sxx, marker_sxy, 1)
 
+    RK4_EXACT, RK4_TOLERANCE = 0, 1
+
+    def set_rk4_arithmetic(self, mode: int) -> None:
+        """RK4_EXACT: the reference's FP64 evaluation order, bit-identical;
+        RK4_TOLERANCE: FMAs and nominal uniform coordinates (fast kernel, used
+        when both axes are uniform to 1e-9 of a step)."""
+        check(lib.ebg_rk4_set_arithmetic(self._h, int(mode)), "ebg_rk4_set_arithmetic")
+
+    def rk4_tolerance_kernel_available(self) -> bool:
+        return lib.ebg_rk4_tolerance_kernel_available(self._h) == 1
+
     def calculate_spin(self, vx1, vy1, esp) -> None:
         """GridStrainRate.calculate_spin! (GridStrainRate.jl:161-182)."""
         g = self._need_grid()

@@ -97,8 +97,15 @@ int ebg_marker_cell_indices(ebg_ctx *ctx, int64_t marknum, const double *marker_
  *   recompute them from marker_x/y (identical result, 8 B/marker less traffic).
  * inside_flags: NULL means recompute from the grid limits.
  * marker_vx/vy/spin: optional outputs (NULL = not written).
- * Arithmetic is FP64 with the reference's evaluation order (no contraction), so
- * velocities, spins and positions are bit-identical to the reference loops. */
+ * Arithmetic (ebg_rk4_set_arithmetic):
+ *   EBG_RK4_EXACT (default): FP64 with the reference's evaluation order (no
+ *     contraction, correctly rounded divisions), so velocities, spins and
+ *     positions are bit-identical to the reference loops;
+ *   EBG_RK4_TOLERANCE: FP64 with fused multiply-adds and the axes' nominal
+ *     uniform coordinates (reciprocal step instead of division), used when
+ *     both axes are uniform to 1e-9 of a step (else the exact kernel runs).
+ *     Positions agree with the exact mode to far below 1e-10 of the domain
+ *     size; the stage-1 cells given in marker_xn/yn are used as given. */
 int ebg_rk4_advect(ebg_ctx *ctx, int64_t marknum, int order, double timestep,
                    double *marker_x, double *marker_y,
                    const int32_t *marker_xn, const int32_t *marker_yn,
@@ -106,6 +113,15 @@ int ebg_rk4_advect(ebg_ctx *ctx, int64_t marknum, int order, double timestep,
                    const double *vx1, const double *vy1, const double *esp,
                    double *marker_vx, double *marker_vy, double *marker_spin,
                    double *marker_sxx, double *marker_sxy, int update_location);
+
+/* Selects the arithmetic of ebg_rk4_advect on this context (no counterpart
+ * in the reference, whose RK4 has one arithmetic). */
+enum { EBG_RK4_EXACT = 0, EBG_RK4_TOLERANCE = 1 };
+int ebg_rk4_set_arithmetic(ebg_ctx *ctx, int mode);
+
+/* 1 when the current grid lets EBG_RK4_TOLERANCE use its fast kernel (both
+ * axes uniform to 1e-9 of a step), else 0; -1 without a grid. */
+int ebg_rk4_tolerance_kernel_available(ebg_ctx *ctx);
 
 /* Replaces GridStrainRate.calculate_spin!
  * (stokes_continuity_solver/utils/strain_rate/GridStrainRate.jl:161-182). */

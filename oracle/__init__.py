@@ -29,6 +29,7 @@ _SIGS = {
     "ebo_update_marker_location": (None, [_I64, _D, _P, _P, _P, _P, _P]),
     "ebo_rotate_marker_stress": (None, [_I64, _D, _P, _P, _P, _P]),
     "ebo_calculate_spin": (None, [_I64, _I64, _P, _P, _P, _P, _P]),
+    "ebo_set_threads": (None, [C.c_int]),
 }
 
 _lib = None
@@ -120,6 +121,11 @@ def advect_markers_and_rotate_stress_tensor(order, timestep, g, x, y, xn, yn, fl
         sxx, sxy = sxx.copy(), sxy.copy()
         lib().ebo_rotate_marker_stress(x.size, timestep, p(flags), p(msp), p(sxx), p(sxy))
     return x2, y2, mvx, mvy, msp, sxx, sxy
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's marker loops (RK4, location update)."""
+    lib().ebo_set_threads(int(n))
 
 
 def calculate_spin(g, vx1, vy1) -> np.ndarray:

@@ -7,6 +7,7 @@
 #include "eb_oracle.h"
 #include "eb_mg3d_oracle.h"
 #include <math.h>
+#include <omp.h>
 
 /* Initialization.jl:27-43, Spacing.jl:6-18 and :124-133,
  * VxGridTools.jl:40-99 (vxgrid_y_or_z_coordinates_loop!, _spacing_loop!),
@@ -127,6 +128,9 @@ void ebo_rk4_velocity_and_spin(int64_t marknum, int order, double timestep,
                                const double *vx1, const double *vy1, const double *esp,
                                double *marker_vx, double *marker_vy, double *marker_spin)
 {
+    /* markers are independent; threaded over markers as the reference's
+     * Threads.@threads loop (VelocityAndSpin.jl:77), ebo_set_threads sets the count */
+#pragma omp parallel for schedule(static)
     for (int64_t m = 0; m < marknum; ++m) {
         if (flags[m] != 1) {          /* VelocityAndSpin.jl:120-126 */
             marker_vx[m] = 0.0;
@@ -192,10 +196,18 @@ void ebo_rk4_velocity_and_spin(int64_t marknum, int order, double timestep,
     }
 }
 
+/* OpenMP thread count of the threaded marker loops (the CPU baseline runs
+ * them on one core and on all cores). */
+void ebo_set_threads(int n)
+{
+    omp_set_num_threads(n > 0 ? n : 1);
+}
+
 /* MarkersLocation.jl:11-41 */
 void ebo_update_marker_location(int64_t marknum, double timestep, const int8_t *flags,
                                 const double *mvx, const double *mvy, double *x, double *y)
 {
+#pragma omp parallel for schedule(static)
     for (int64_t m = 0; m < marknum; ++m) {
         if (flags[m] == 1) {
             x[m] = x[m] + timestep * mvx[m];

@@ -82,6 +82,7 @@ void ebo_rk4_velocity_and_spin(int64_t marknum, int order, double timestep,
                                double *marker_vx, double *marker_vy, double *marker_spin);
 
 /* x += dt*vx, y += dt*vy for inside markers (location/MarkersLocation.jl:11-41). */
+void ebo_set_threads(int n);
 void ebo_update_marker_location(int64_t marknum, double timestep, const int8_t *flags,
                                 const double *mvx, const double *mvy, double *x, double *y);
 
