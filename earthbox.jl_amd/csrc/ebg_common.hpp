@@ -89,4 +89,5 @@ struct ebg_ctx {
 namespace ebg {
 int grid_release(Grid2D& g);
 void plastic_release(ebg_ctx* ctx);
+int plastic_scratch_setup(int device);   // ebg_plastic.hip
 }

@@ -118,6 +118,13 @@ int ebg_ctx_create(ebg_ctx** out, int device)
         return EBG_ERR_HIP;
     }
     c->own_stream = true;
+    // device address of the allocation-free entry point's scratch, once per device
+    int rc = plastic_scratch_setup(device);
+    if (rc != EBG_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
     *out = c;
     return EBG_OK;
 }

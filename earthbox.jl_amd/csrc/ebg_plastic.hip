@@ -10,6 +10,9 @@
 // order (per-block partials, then one block), so repeated calls agree.
 #include "ebg_common.hpp"
 
+#include <algorithm>
+#include <mutex>
+
 #include <cmath>
 
 namespace ebg {
@@ -254,30 +257,54 @@ void plastic_release(ebg_ctx* ctx)
 
 using namespace ebg;
 
-extern "C" int ebg_update_nodes_for_plastic_yielding(ebg_ctx* ctx, const ebg_plastic_nodes_in* in,
-                                                     ebg_plastic_nodes_out* out)
+namespace ebg {
+
+// Scratch of the allocation-free variant (ebg_update_nodes_for_plastic_yielding_noalloc):
+// PL_MAX_BLOCKS partial sums of squared yield errors, as many partial counts
+// (int64 in double-sized slots) and the two results (count, RMS): 2,050
+// doubles, 16.4 KB per device, held until the process ends.  The node loop is
+// grid-strided over at most PL_MAX_BLOCKS blocks, so this size covers any grid.
+constexpr int PL_SCRATCH_DOUBLES = 2 * PL_MAX_BLOCKS + 2;
+static __device__ __align__(256) double scratch[PL_SCRATCH_DOUBLES];
+
+// Device address of `scratch` per device, taken once per device with that
+// device current (each device holds its own copy of the array at its own
+// address; in host code the array's name is not a device address).
+constexpr int PL_MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static double* g_scratch_ptr[PL_MAX_DEVICES] = {};
+
+int plastic_scratch_setup(int device)
 {
-    EBG_CHECK(ctx && ctx->has_grid, EBG_ERR_STATE, "ebg_update_nodes_for_plastic_yielding: no grid");
-    EBG_CHECK(in && out, EBG_ERR_ARG, "ebg_update_nodes_for_plastic_yielding: NULL arguments");
-    EBG_CHECK(in->plastic_yield && in->etas1 && in->eta_flow && in->mus1 && in->cohesion_grid &&
-                  in->fric_degrees_grid && in->sxy2 && in->sxx2 && in->pr1,
-              EBG_ERR_ARG, "ebg_update_nodes_for_plastic_yielding: NULL input field");
-    EBG_CHECK(out->etavp_new && out->plastic_strain_rate && out->plastic_yield_new && out->yield_error, EBG_ERR_ARG,
-              "ebg_update_nodes_for_plastic_yielding: NULL output field");
+    EBG_CHECK(device >= 0 && device < PL_MAX_DEVICES, EBG_ERR_ARG, "plastic scratch: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return EBG_OK;
+    void* p = nullptr;
+    EBG_HIP(hipSetDevice(device));
+    EBG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<double*>(p);
+    return EBG_OK;
+}
+
+static double* plastic_scratch_ptr(int device)
+{
+    if (device < 0 || device >= PL_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch_ptr[device];
+}
+
+// The node pass and the fixed-order reduction on buf (PL_SCRATCH_DOUBLES
+// doubles: partial sums, partial counts, results), then the count and RMS back
+// to the host.
+static int plastic_yielding_run(ebg_ctx* ctx, const char* name, const ebg_plastic_nodes_in* in,
+                                ebg_plastic_nodes_out* out, double* buf)
+{
     const Grid2D& g = ctx->grid;
     hipStream_t s = ctx->stream;
     const int64_t nn = g.xnum * g.ynum;
     const int nb = (int)std::min<int64_t>(PL_MAX_BLOCKS, (nn + PL_THREADS - 1) / PL_THREADS);
-    if (!ctx->plastic) {
-        PlasticScratch* ps = new PlasticScratch();
-        if (hipMalloc(&ps->buf, sizeof(double) * (2 * PL_MAX_BLOCKS + 2)) != hipSuccess) {
-            delete ps;
-            set_error("ebg_update_nodes_for_plastic_yielding: hipMalloc failed");
-            return EBG_ERR_HIP;
-        }
-        ctx->plastic = ps;
-    }
-    double* scratch = ((PlasticScratch*)ctx->plastic)->buf;
+    EBG_CHECK(nb >= 1 && 2 * nb + 2 <= PL_SCRATCH_DOUBLES, EBG_ERR_ARG,
+              std::string(name) + ": reduction does not fit the scratch");
     PlArgs a{};
     a.xnum = (int)g.xnum;
     a.ynum = (int)g.ynum;
@@ -288,9 +315,9 @@ extern "C" int ebg_update_nodes_for_plastic_yielding(ebg_ctx* ctx, const ebg_pla
     a.plastic_strain_rate = out->plastic_strain_rate;
     a.plastic_yield_new = out->plastic_yield_new;
     a.yield_error = out->yield_error;
-    a.partial_err = scratch;
-    a.partial_count = (long long*)(scratch + PL_MAX_BLOCKS);
-    double* result = scratch + 2 * PL_MAX_BLOCKS;
+    a.partial_err = buf;
+    a.partial_count = (long long*)(buf + PL_MAX_BLOCKS);
+    double* result = buf + 2 * PL_MAX_BLOCKS;
     hipLaunchKernelGGL(plastic_nodes_kernel, dim3(nb), dim3(PL_THREADS), 0, s, a);
     hipLaunchKernelGGL(plastic_finish_kernel, dim3(1), dim3(PL_THREADS), 0, s, (const double*)a.partial_err,
                        (const long long*)a.partial_count, nb, result);
@@ -301,6 +328,51 @@ extern "C" int ebg_update_nodes_for_plastic_yielding(ebg_ctx* ctx, const ebg_pla
     out->nnode_yield = (int64_t)h[0];
     out->global_yield_error = h[1];
     return EBG_OK;
+}
+
+static int plastic_check_args(ebg_ctx* ctx, const char* name, const ebg_plastic_nodes_in* in,
+                              const ebg_plastic_nodes_out* out)
+{
+    EBG_CHECK(ctx && ctx->has_grid, EBG_ERR_STATE, std::string(name) + ": no grid");
+    EBG_CHECK(in && out, EBG_ERR_ARG, std::string(name) + ": NULL arguments");
+    EBG_CHECK(in->plastic_yield && in->etas1 && in->eta_flow && in->mus1 && in->cohesion_grid &&
+                  in->fric_degrees_grid && in->sxy2 && in->sxx2 && in->pr1,
+              EBG_ERR_ARG, std::string(name) + ": NULL input field");
+    EBG_CHECK(out->etavp_new && out->plastic_strain_rate && out->plastic_yield_new && out->yield_error, EBG_ERR_ARG,
+              std::string(name) + ": NULL output field");
+    return EBG_OK;
+}
+
+}  // namespace ebg
+
+extern "C" int ebg_update_nodes_for_plastic_yielding(ebg_ctx* ctx, const ebg_plastic_nodes_in* in,
+                                                     ebg_plastic_nodes_out* out)
+{
+    static const char* name = "ebg_update_nodes_for_plastic_yielding";
+    int rc = plastic_check_args(ctx, name, in, out);
+    if (rc) return rc;
+    if (!ctx->plastic) {
+        PlasticScratch* ps = new PlasticScratch();
+        if (hipMalloc(&ps->buf, sizeof(double) * PL_SCRATCH_DOUBLES) != hipSuccess) {
+            delete ps;
+            set_error("ebg_update_nodes_for_plastic_yielding: hipMalloc failed");
+            return EBG_ERR_HIP;
+        }
+        ctx->plastic = ps;
+    }
+    return plastic_yielding_run(ctx, name, in, out, ((PlasticScratch*)ctx->plastic)->buf);
+}
+
+extern "C" int ebg_update_nodes_for_plastic_yielding_noalloc(ebg_ctx* ctx, const ebg_plastic_nodes_in* in,
+                                                             ebg_plastic_nodes_out* out)
+{
+    static const char* name = "ebg_update_nodes_for_plastic_yielding_noalloc";
+    int rc = plastic_check_args(ctx, name, in, out);
+    if (rc) return rc;
+    double* buf = plastic_scratch_ptr(ctx->device);
+    EBG_CHECK(buf != nullptr, EBG_ERR_STATE,
+              std::string(name) + ": scratch address not set up for this device (ebg_ctx_create failed?)");
+    return plastic_yielding_run(ctx, name, in, out, buf);
 }
 
 extern "C" int ebg_harmonic_normal_viscosity(ebg_ctx* ctx, const double* etas1, const double* plastics,

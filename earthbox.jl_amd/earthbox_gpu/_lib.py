@@ -42,6 +42,7 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_interpolate_marker_stokes_parameters_to_grid": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_stokes_viscoelastic_rhs": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_update_nodes_for_plastic_yielding": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ebg_update_nodes_for_plastic_yielding_noalloc": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_grid_strain_rate_and_stress": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_harmonic_normal_viscosity": (c_int, [c_void_p] * 6),
     "ebg_marker_strain_rate_and_pressure": (c_int, [c_void_p, c_void_p, c_void_p]),

@@ -22,12 +22,15 @@ SCALARS = tuple(k for k, _ in PlasticNodesIn._fields_[9:])
 
 def update_nodes_for_plastic_yielding(ctx: GPUContext, fields: dict, yield_error, params: dict,
                                       no_yielding_in_mobile_wall: bool = False,
-                                      no_yielding_in_plate_extension: bool = False, out: dict | None = None):
+                                      no_yielding_in_plate_extension: bool = False, out: dict | None = None,
+                                      noalloc: bool = False):
     """fields: device arrays keyed like FIELDS; yield_error: device (ynum, xnum),
     updated in place; params: the remaining scalars of PlasticNodesIn (missing
     ones 0).  Returns (out dict with etavp_new, plastic_strain_rate and
     plastic_yield_new, nnode_yield, global_yield_error), as the reference's
-    tuple."""
+    tuple.  noalloc: use the allocation-free entry point (its reduction
+    scratch is a static device array; pass `out` to allocate nothing here
+    either)."""
     g = ctx.grid
     if g is None:
         raise RuntimeError("set_grid() must be called first")
@@ -46,8 +49,8 @@ def update_nodes_for_plastic_yielding(ctx: GPUContext, fields: dict, yield_error
     for k in ("etavp_new", "plastic_strain_rate", "plastic_yield_new"):
         setattr(o, k, ptr(out[k], np.float64, nb))
     o.yield_error = ptr(yield_error, np.float64, nb)
-    check(lib.ebg_update_nodes_for_plastic_yielding(ctx.handle, C.byref(a), C.byref(o)),
-          "ebg_update_nodes_for_plastic_yielding")
+    fn = "ebg_update_nodes_for_plastic_yielding" + ("_noalloc" if noalloc else "")
+    check(getattr(lib, fn)(ctx.handle, C.byref(a), C.byref(o)), fn)
     return out, int(o.nnode_yield), float(o.global_yield_error)
 
 

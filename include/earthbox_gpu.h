@@ -256,6 +256,21 @@ typedef struct {
 int ebg_update_nodes_for_plastic_yielding(ebg_ctx *ctx, const ebg_plastic_nodes_in *in,
                                           ebg_plastic_nodes_out *out);
 
+/* Allocation-free variant of ebg_update_nodes_for_plastic_yielding (no
+ * counterpart in the reference): for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same kernels, same results bit for bit.  Its reduction
+ * scratch (partial sums and counts of at most 1024 blocks and the two results,
+ * 2,050 doubles = 16.4 KB) is a static device array that each GPU holds until
+ * the process ends; its device address is taken once per device by
+ * ebg_ctx_create.  The node loop is grid-strided, so any grid fits the
+ * scratch.  The call makes no hipMalloc/hipFree and calls no allocating
+ * library routine; it still copies the count and RMS back to the host and
+ * synchronizes the context's stream.  Every call on one GPU shares that
+ * scratch: calls that overlap on one GPU (other streams or threads) race. */
+int ebg_update_nodes_for_plastic_yielding_noalloc(ebg_ctx *ctx, const ebg_plastic_nodes_in *in,
+                                                  ebg_plastic_nodes_out *out);
+
 /* Replaces Interpolation.interpolate_basic_viscoplastic_viscosity_to_pressure_grid!
  * (interpolation/Interpolation.jl:97-109) with iuse_harmonic_avg_normal_viscosity = 1,
  * i.e. HarmonicAverage.normal_viscosity_from_harmonic_avg_shear_viscosity!: etan1
