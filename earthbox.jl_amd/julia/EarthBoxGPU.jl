@@ -6,8 +6,9 @@ EarthBox.jl.  It keeps the reference's entry points as the seams:
 
 - `solve_system_gpu!(model, S)`        beside SystemSolver.solve_system_optimized
                                         (solver/system_solver/SystemSolver.jl:67-91)
-- `interpolate_markers_gpu!(model)`    beside Interpolation.interpolate_marker_stokes_parameters_to_grid!
-                                        (interpolation/Interpolation.jl:87-95)
+- `interpolate_markers_gpu!(model, inside_flags)` beside Interpolation.initialize_bilinear_interpolation!
+                                        + interpolate_marker_stokes_parameters_to_grid!
+                                        (interpolation/Interpolation.jl:61-95)
 - `advect_markers_gpu!(model)`         beside Advection.advect_markers_and_rotate_stress_tensor!
                                         (advection/Advection.jl:109-114)
 - `run_multigrid_solver_gpu!(multigrid_data)` beside MultigridSolver.run_multigrid_solver
@@ -373,6 +374,84 @@ function advect_markers_gpu!(st::GPUState, model, inside_flags::Vector{Int8})
     d2h!(m.advection.marker_spin.array, dmsp)
     d2h!(m.stress.marker_sxx.array, dsxx)
     d2h!(m.stress.marker_sxy.array, dsxy)
+    return nothing
+end
+
+"Field order of ebg_marker_stokes_props (include/earthbox_gpu.h)."
+struct MarkerStokesProps
+    marknum::Int64
+    x::Ptr{Cvoid}; y::Ptr{Cvoid}; inside_flags::Ptr{Cvoid}
+    rho::Ptr{Cvoid}; cohesion::Ptr{Cvoid}; fric::Ptr{Cvoid}; eta_flow::Ptr{Cvoid}; eta::Ptr{Cvoid}
+    sxy::Ptr{Cvoid}; sxx::Ptr{Cvoid}; meltfrac::Ptr{Cvoid}
+    pfailure::Ptr{Cvoid}; dilatation::Ptr{Cvoid}
+    matid::Ptr{Cvoid}; mat_mu::Ptr{Cvoid}
+    nmat::Int64
+end
+
+"Field order of ebg_stokes_grid_fields (include/earthbox_gpu.h)."
+struct StokesGridFields
+    rho1::Ptr{Cvoid}; rho1_vy::Ptr{Cvoid}; etas1::Ptr{Cvoid}; etan1::Ptr{Cvoid}; eta_flow::Ptr{Cvoid}
+    mus1::Ptr{Cvoid}; mun1::Ptr{Cvoid}; sxy1::Ptr{Cvoid}; sxx1::Ptr{Cvoid}
+    plastics::Ptr{Cvoid}; plasticn::Ptr{Cvoid}; cohesion_grid::Ptr{Cvoid}; fric_degrees_grid::Ptr{Cvoid}
+    dilatation_grid::Ptr{Cvoid}; extractable_meltfrac_grid::Ptr{Cvoid}
+    wtnodes::Ptr{Cvoid}; wtetas::Ptr{Cvoid}; wtetan::Ptr{Cvoid}; wtnodes_vy::Ptr{Cvoid}
+    marker_xn::Ptr{Cvoid}; marker_yn::Ptr{Cvoid}
+end
+
+"""
+    interpolate_markers_gpu!(st, model, inside_flags)
+
+GPU replacement of Interpolation.initialize_bilinear_interpolation! followed by
+Interpolation.interpolate_marker_stokes_parameters_to_grid!
+(interpolation/Interpolation.jl:61-95), called from TimeLoop.jl:219,226.  It
+reads the marker arrays BilinearNumerator.jl:205-300 reads, writes the Stokes
+transport grids (rho1, rho1_vy, etas1, etan1, eta_flow, mus1, mun1, sxy1,
+sxx1, plastics, plasticn, cohesion_grid, fric_degrees_grid, dilatation_grid,
+extractable_meltfrac_grid), the weight sums (wtnodes, wtetas, wtetan,
+wtnodes_vy) and marker_xn/yn.  The grids travel both ways: nodes without
+marker weight keep their previous value (BackupTransportArrays.jl:33-104).
+The per-marker distances and weights the reference stores
+(marker_dx/dy, marker_wtfor*node) are not written: the GPU keeps them in
+registers.  Node sums follow the serial marker order, so results equal the
+reference's loops bit for bit (the friction angle's asin aside).
+"""
+function interpolate_markers_gpu!(st::GPUState, model, inside_flags::Vector{Int8})
+    set_grid!(st, model)
+    m = model.markers.arrays
+    sc = model.stokes_continuity.arrays
+    iw = model.interpolation.arrays.grid_weights
+    n = model.markers.parameters.distribution.marknum.value
+    mat_mu = model.materials.arrays.mat_mu.array
+    up(k, x) = h2d!(st, k, x)
+    props = MarkerStokesProps(n, up(:mx, m.location.marker_x.array), up(:my, m.location.marker_y.array),
+        up(:flags, inside_flags),
+        up(:mrho, m.material.marker_rho.array), up(:mcoh, m.rheology.marker_cohesion.array),
+        up(:mfric, m.rheology.marker_fric.array), up(:metaf, m.rheology.marker_eta_flow.array),
+        up(:meta, m.rheology.marker_eta.array), up(:msxy, m.stress.marker_sxy.array),
+        up(:msxx, m.stress.marker_sxx.array), up(:mmelt, m.melt.marker_extractable_meltfrac.array),
+        up(:mpf, m.rheology.marker_pfailure.array), up(:mdil, m.rheology.marker_dilatation_angle.array),
+        up(:matid, m.material.marker_matid.array), up(:mat_mu, mat_mu), length(mat_mu))
+    grids = (sc.density.rho1.array, sc.density.rho1_vy.array, sc.viscosity.etas1.array,
+             sc.viscosity.etan1.array, sc.viscosity.eta_flow.array, sc.shear_modulus.mus1.array,
+             sc.shear_modulus.mun1.array, sc.stress.sxy1.array, sc.stress.sxx1.array,
+             sc.plastic_def.plastics.array, sc.plastic_def.plasticn.array, sc.plastic_def.cohesion_grid.array,
+             sc.plastic_def.fric_degrees_grid.array, sc.plastic_def.dilatation_grid.array,
+             sc.plastic_def.extractable_meltfrac_grid.array,
+             iw.wtnodes.array, iw.wtetas.array, iw.wtetan.array, iw.wtnodes_vy.array,
+             m.grid_marker_relationship.marker_xn.array, m.grid_marker_relationship.marker_yn.array)
+    keys = (:rho1, :rho1_vy, :etas1, :etan1, :eta_flow, :mus1, :mun1, :sxy1, :sxx1, :plastics, :plasticn,
+            :cohesion_grid, :fric_degrees_grid, :dilatation_grid, :meltfrac_grid,
+            :wtnodes, :wtetas, :wtetan, :wtnodes_vy, :xn, :yn)
+    # previous values are the backup where no marker weight lands
+    ptrs = [h2d!(st, k, x) for (k, x) in zip(keys, grids)]
+    f = StokesGridFields(ptrs...)
+    check(ccall((:ebg_interpolate_marker_stokes_parameters_to_grid, LIB), Cint,
+        (Ptr{Cvoid}, Ref{MarkerStokesProps}, Ref{StokesGridFields}), st.ctx, props, f),
+        "ebg_interpolate_marker_stokes_parameters_to_grid")
+    check(ccall((:ebg_ctx_synchronize, LIB), Cint, (Ptr{Cvoid},), st.ctx), "sync")
+    for (k, x) in zip(keys, grids)
+        d2h!(x, st.buffers[k])
+    end
     return nothing
 end
 

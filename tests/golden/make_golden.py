@@ -14,7 +14,7 @@ GPU tests fixed expected outputs.
 
 stokes_sinker3d_97_stats.json is not written by this script: it holds the
 final-solution statistics the reference's authors recorded in the source of
-its 97^3 Stokes sinker demo (multigrid_tests/stokes_sinker/StokesSinker.jl:58-62),
+its 97^3 Stokes sinker demo (multigrid_tests/stokes_sinker/StokesSinker.jl:60-64),
 transcribed as data."""
 import pathlib
 import sys

@@ -326,7 +326,7 @@ def _sinker_stats_error(parts):
 def test_sinker97_solve_matches_reference_statistics(gpu_ctx, precision):
     """The reference's 97^3 Stokes sinker (contrast 1e6) solved to 1e-10 by
     GMRES(200) with the V-cycle preconditioner reproduces the final-solution
-    statistics the reference's authors recorded (StokesSinker.jl:58-62):
+    statistics the reference's authors recorded (StokesSinker.jl:60-64):
     velocity extrema, pressure maximum and mean within 5e-8 relative,
     pressure minimum within 5e-8 of the pressure range (measured 7e-9 and
     2e-8).  This pins the 3-D operator, ghost passes and the demo's inputs
