@@ -28,7 +28,8 @@ STRESS = ("marker_sxx", "marker_sxy")  # updated in place, so they must be given
 
 
 def update_markers_after_stokes(ctx: GPUContext, markers: dict, fields: dict, timestep: float = 0.0,
-                                out: dict | None = None, outputs=OUTPUTS) -> dict:
+                                out: dict | None = None, outputs=OUTPUTS,
+                                subgrid_diff_coef_stress: float = 0.0) -> dict:
     """markers: device vectors x, y, marker_xn, marker_yn (int32, 1-based, as
     the scatter stores them), optional inside_flags (int8), and for the ratio
     eta, matid (int16) and mat_mu.  fields: device grid arrays exx, pr1, sxx2,
@@ -37,7 +38,13 @@ def update_markers_after_stokes(ctx: GPUContext, markers: dict, fields: dict, ti
     zero-filled, for the requested outputs when missing).  Asking for
     marker_sxx and marker_sxy (given in out) adds the remaining grid stress
     change dsxx, dsxy at each marker, as StressCorrection.
-    apply_marker_stress_correction_for_remaining_grid_stress! does."""
+    apply_marker_stress_correction_for_remaining_grid_stress! does.  That is
+    the whole marker stress update only when subgrid_diff_coef_stress is 0:
+    with a positive coefficient the reference first runs update_subgrid_stress!
+    (StokesContinuitySolver.jl:365-381), so the correction is refused."""
+    if subgrid_diff_coef_stress > 0 and any(k in STRESS for k in outputs):
+        raise ValueError("the marker stress correction alone is the reference's stress update only when "
+                         f"subgrid_diff_coef_stress is 0 (got {subgrid_diff_coef_stress})")
     g = ctx.grid
     if g is None:
         raise RuntimeError("set_grid() must be called first")

@@ -20,7 +20,7 @@ import torch
 from .device import empty
 from .plastic import (interpolate_basic_viscoplastic_viscosity_to_pressure_grid,
                       plastic_strain_rate_at_pressure_nodes, update_nodes_for_plastic_yielding)
-from .stokes import StokesSolverGPU, bc_mirror_factors, change_norms
+from .stokes import StokesSolverGPU, bc_mirror_factors, calculate_new_timestep, change_norms
 from .stokes_problem import StokesProblem
 from .viscoelastic import calc_viscoelastic_terms_and_rhs, update_grid_strain_rate_and_stress
 
@@ -39,22 +39,35 @@ def check_stop_global_loop_nodes(nnode_yield, iglobal, nglobal, global_yield_err
 def picard_loop_nodes(ctx, template: StokesProblem, state: dict, plastic_params: dict, nglobal: int,
                       tolerance: float, gravity_x: float = 0.0, iuse_harmonic_avg_normal_viscosity: int = 1,
                       no_yielding_in_mobile_wall: bool = False, no_yielding_in_plate_extension: bool = False,
-                      solve_opts: dict | None = None, solver: StokesSolverGPU | None = None):
+                      solve_opts: dict | None = None, solver: StokesSolverGPU | None = None,
+                      velocity_old: tuple | None = None, iupdate_timestep: int = 0,
+                      marker_cell_displ_max: float = 0.5):
     """template: the system's grid, velocity boundary conditions, pressure BC,
     gravity and interface-stabilization flag (its fields are replaced each
     iteration); state: device arrays keyed like STATE, updated in place as the
     reference updates ModelData; plastic_params: the scalars of
-    update_nodes_for_plastic_yielding (timestep included).  Returns
-    (solution dict with vx1, vy1, pr1, exx, exy, eii, sxy2, sxx2, dsxy, dsxx;
-    per-iteration log)."""
+    update_nodes_for_plastic_yielding (timestep included).
+    velocity_old: (vx1, vy1) of the previous time step, which
+    backup_stokes_arrays! leaves for calculate_stokes_norms!
+    (StokesContinuitySolver.jl:228-236); None on the first time step (zeros,
+    so the first change norm is 1).  The change norm is computed on every
+    iteration against the previous solution.
+    iupdate_timestep = 1: after every solve the time step is limited by the
+    displacement limit (TimeStep.adjust_time_step_using_displacement_limit!,
+    StokesContinuitySolver.jl:223-226, Update.jl:43-61) before the stress
+    forecast and the yielding, and the new step carries into the next
+    iteration's right-hand side.
+    Returns (solution dict with vx1, vy1, pr1, exx, exy, eii, sxy2, sxx2,
+    dsxy, dsxx and the final timestep; per-iteration log)."""
     g = template.grid
     solver = solver or StokesSolverGPU(ctx)
     opts = dict(precision=1, rtol=1e-10, max_iterations=4000, bc_d=bc_mirror_factors(template))
     opts.update(solve_opts or {})
     dt = float(plastic_params["timestep"])
+    plastic_params = dict(plastic_params)
     ny, nx = g.ynum, g.xnum
     eii_basic = empty((ny, nx), np.float64)
-    vx_old = vy_old = None
+    vx_old, vy_old = velocity_old if velocity_old is not None else (None, None)
     log = []
     sol = {}
     for iglobal in range(1, nglobal + 1):
@@ -71,9 +84,13 @@ def picard_loop_nodes(ctx, template: StokesProblem, state: dict, plastic_params:
                                         "RX1": ve["RX1"], "RY1": ve["RY1"], "RC1": ve["RC1"]})
         S, st = solver.solve_system(**opts)
         vx1, vy1, pr1 = solver.process_stokes_solution(S)
-        # stokes_solution_processing! on the grid
+        # stokes_solution_processing! on the grid: the time step limit (the
+        # strain rates do not depend on it), then the stress forecast with it
+        if iupdate_timestep == 1:
+            dt, _ = calculate_new_timestep(ctx, vx1, vy1, dt, marker_cell_displ_max)
+            plastic_params["timestep"] = dt
         ss = update_grid_strain_rate_and_stress(ctx, {"vx1": vx1, "vy1": vy1, **state}, dt)
-        dvxy_rel_L2 = change_norms(ctx, [vx1, vy1], [vx_old, vy_old])[1] if vx_old is not None else 1e38
+        dvxy_rel_L2 = change_norms(ctx, [vx1, vy1], [vx_old, vy_old])[1]
         vx_old, vy_old = vx1.clone(), vy1.clone()
         # PlasticFailureNodes.update_nodes_for_plastic_yielding
         fields = {k: state[k] for k in ("plastic_yield", "etas1", "eta_flow", "mus1", "cohesion_grid",
@@ -89,8 +106,8 @@ def picard_loop_nodes(ctx, template: StokesProblem, state: dict, plastic_params:
             plastic_strain_rate_at_pressure_nodes(ctx, eii_basic, state["eii_plastic_pressure"])
         log.append({"iglobal": iglobal, "iterations": st["iterations"], "relres": st["relres"],
                     "nnode_yield": nnode_yield, "global_yield_error": global_yield_error,
-                    "dvxy_rel_L2": dvxy_rel_L2})
-        sol = {"vx1": vx1, "vy1": vy1, "pr1": pr1, **ss}
+                    "dvxy_rel_L2": dvxy_rel_L2, "timestep": dt})
+        sol = {"vx1": vx1, "vy1": vy1, "pr1": pr1, "timestep": dt, **ss}
         if check_stop_global_loop_nodes(nnode_yield, iglobal, nglobal, global_yield_error, dvxy_rel_L2,
                                         tolerance):
             break

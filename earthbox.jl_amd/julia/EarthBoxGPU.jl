@@ -485,6 +485,15 @@ subgrid_diff_coef_stress is 0).
 """
 function update_markers_after_stokes_gpu!(st::GPUState, model, inside_flags::Vector{Int8};
                                           correct_stress::Bool=false)
+    if correct_stress
+        # update_marker_stress_for_grid_and_subgrid_changes! (StokesContinuitySolver.jl:365-381)
+        # runs update_subgrid_stress! before the correction when the coefficient is positive;
+        # the fused pass adds the grid stress change without that step
+        coef = model.markers.parameters.subgrid_diffusion.subgrid_diff_coef_stress.value
+        coef > 0 && error("update_markers_after_stokes_gpu!: correct_stress=true needs " *
+                          "subgrid_diff_coef_stress == 0 (got $coef); run the sub-grid stress " *
+                          "diffusion and the correction on the CPU path")
+    end
     set_grid!(st, model)
     m = model.markers.arrays
     sc = model.stokes_continuity.arrays

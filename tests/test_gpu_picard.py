@@ -2,7 +2,10 @@
 oracle's restatements with a sparse direct solve, on a 41 x 41 falling block
 with a cohesion that the stresses around the block exceed.  The loop, the
 yield counts and the stopping step agree; viscosities and velocities agree to
-the iterative solve's tolerance at every node whose yield flag matches."""
+the iterative solve's tolerance at every node whose yield flag matches.  The
+change norm compares with the previous step's velocity (zeros on the first
+step), and the displacement-limited time step is recomputed after every solve
+when iupdate_timestep is 1."""
 import dataclasses
 
 import numpy as np
@@ -43,10 +46,12 @@ def _setup(n=41):
     return g, s, template
 
 
-def _cpu_loop(g, s, template, nglobal, tol):
+def _cpu_loop(g, s, template, nglobal, tol, old=None, iupdate_timestep=0, displ_max=0.5):
+    """The loop composed from the oracle; old: the previous step's velocity
+    as one concatenated vector (None: zeros, the first time step)."""
     s = {k: v.copy(order="F") for k, v in s.items()}
     dt = PAR["timestep"]
-    old = None
+    par = dict(PAR)
     log = []
     for it in range(1, nglobal + 1):
         if it > 1:
@@ -59,18 +64,22 @@ def _cpu_loop(g, s, template, nglobal, tol):
         Li, Lj, Lv, R = oracle.stokes_build(prob)
         S = oracle.direct_solve(prob.N, Li, Lj, Lv, R)
         vx1, vy1, pr1 = oracle.stokes_unpack(prob, S)
+        if iupdate_timestep:
+            dt, _ = oracle.calculate_new_timestep(dt, displ_max, g.xstpavg, g.ystpavg, vx1, vy1)
+            par["timestep"] = dt
         ss = oracle.grid_strain_rate_and_stress(g, vx1, vy1, s, dt)
         v = np.concatenate([vx1.ravel(order="F"), vy1.ravel(order="F")])
-        dv = oracle.get_L2_norms(v, old)[1] if old is not None else 1e38
+        dv = oracle.get_L2_norms(v, old if old is not None else np.zeros_like(v))[1]
         old = v
         f = {k: s[k] for k in oracle.PLASTIC_FIELDS if k in s}
         f.update(sxy2=ss["sxy2"], sxx2=ss["sxx2"], pr1=pr1)
-        ev, psr, pyn, ye, n, err = oracle.update_nodes_for_plastic_yielding(g, f, PAR, s["yield_error"])
+        ev, psr, pyn, ye, n, err = oracle.update_nodes_for_plastic_yielding(g, f, par, s["yield_error"])
         s["yield_error"] = ye
         if n > 0:
             s["etas1"], s["plastic_yield"] = ev, pyn
             s["eii_plastic_pressure"] = oracle.plastic_strain_rate_at_pressure_nodes(g, psr)
-        log.append({"iglobal": it, "nnode_yield": n, "global_yield_error": err, "dvxy_rel_L2": dv})
+        log.append({"iglobal": it, "nnode_yield": n, "global_yield_error": err, "dvxy_rel_L2": dv,
+                    "timestep": dt})
         if check_stop_global_loop_nodes(n, it, nglobal, err, dv, tol):
             break
     return s, {"vx1": vx1, "vy1": vy1, "pr1": pr1}, log
@@ -97,3 +106,50 @@ def test_picard_loop_nodes_matches_oracle_loop(gpu_ctx):
     for k in ("vx1", "vy1"):
         ref = csol[k]
         assert np.linalg.norm(to_host(gsol[k]) - ref) <= 1e-5 * np.linalg.norm(ref)
+
+
+def test_picard_stops_at_first_iteration_when_prior_velocity_converged(gpu_ctx):
+    """The change norm of the first iteration compares with the previous time
+    step's velocity (backup_stokes_arrays!), so a loop that starts from a
+    converged state and that velocity stops after one solve, as the
+    reference does."""
+    print("picard loop restarted from a converged state", flush=True)
+    g, s, template = _setup()
+    nglobal, tol = 20, 1e-2
+    cs, csol, clog = _cpu_loop(g, s, template, nglobal, tol)
+    assert clog[-1]["dvxy_rel_L2"] < tol, "the first loop must stop on the change norm"
+    vold = np.concatenate([csol["vx1"].ravel(order="F"), csol["vy1"].ravel(order="F")])
+    cs2, _, clog2 = _cpu_loop(g, cs, template, nglobal, tol, old=vold)
+    gpu_ctx.set_grid(g)
+    ds = {k: to_device(v) for k, v in s.items()}
+    gsol, glog = picard_loop_nodes(gpu_ctx, template, ds, PAR, nglobal, tol, solve_opts=dict(rtol=1e-11))
+    gsol2, glog2 = picard_loop_nodes(gpu_ctx, template, ds, PAR, nglobal, tol, solve_opts=dict(rtol=1e-11),
+                                     velocity_old=(gsol["vx1"], gsol["vy1"]))
+    print("cpu", [f"{r['dvxy_rel_L2']:.3g}" for r in clog2], "gpu", [f"{r['dvxy_rel_L2']:.3g}" for r in glog2],
+          flush=True)
+    assert glog[0]["dvxy_rel_L2"] == pytest.approx(1.0) and clog[0]["dvxy_rel_L2"] == pytest.approx(1.0)
+    assert len(clog2) == 1 and len(glog2) == 1
+    assert glog2[0]["dvxy_rel_L2"] < tol
+    assert glog2[0]["dvxy_rel_L2"] == pytest.approx(clog2[0]["dvxy_rel_L2"], rel=0.1, abs=1e-6)
+
+
+def test_picard_updates_timestep_after_each_solve(gpu_ctx):
+    """iupdate_timestep = 1: the displacement limit shortens the step after
+    each solve, and the shortened step drives the stress forecast, the
+    yielding and the next right-hand side, in both loops alike."""
+    print("picard loop with the displacement-limited time step", flush=True)
+    g, s, template = _setup()
+    nglobal, tol, displ = 4, 1e-3, 0.05
+    cs, csol, clog = _cpu_loop(g, s, template, nglobal, tol, iupdate_timestep=1, displ_max=displ)
+    gpu_ctx.set_grid(g)
+    ds = {k: to_device(v) for k, v in s.items()}
+    gsol, glog = picard_loop_nodes(gpu_ctx, template, ds, PAR, nglobal, tol, solve_opts=dict(rtol=1e-11),
+                                   iupdate_timestep=1, marker_cell_displ_max=displ)
+    print("cpu dt", [r["timestep"] for r in clog], "gpu dt", [r["timestep"] for r in glog], flush=True)
+    assert clog[0]["timestep"] < PAR["timestep"], "the limit must bind in this test"
+    assert len(glog) == len(clog)
+    for a, b in zip(glog, clog):
+        assert a["timestep"] == pytest.approx(b["timestep"], rel=1e-6)
+        assert abs(a["nnode_yield"] - b["nnode_yield"]) <= max(2, b["nnode_yield"] // 100)
+        assert a["global_yield_error"] == pytest.approx(b["global_yield_error"], rel=1e-3)
+    assert gsol["timestep"] == glog[-1]["timestep"]

@@ -143,3 +143,16 @@ def test_stress_correction_adds_affine_change_and_skips_outside_markers():
     np.testing.assert_allclose(r["marker_sxy"][ins] - sxy0[ins], ey[ins], rtol=0, atol=1e-9 * 1e6)
     assert np.array_equal(r["marker_sxx"][~ins], sxx0[~ins])
     assert np.array_equal(r["marker_sxy"][~ins], sxy0[~ins])
+
+
+def test_stress_correction_refused_with_subgrid_diffusion():
+    """With a positive sub-grid stress diffusion coefficient the reference runs
+    update_subgrid_stress! before the stress correction
+    (StokesContinuitySolver.jl:365-381); the host wrapper refuses the fused
+    correction then (checked before any device work)."""
+    import types
+    import pytest
+    from earthbox_gpu.marker_gather import STRESS, update_markers_after_stokes
+    with pytest.raises(ValueError, match="subgrid_diff_coef_stress"):
+        update_markers_after_stokes(types.SimpleNamespace(grid=None), {}, {}, 1.0, out={}, outputs=STRESS,
+                                    subgrid_diff_coef_stress=1.0)
