@@ -43,6 +43,9 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_stokes_viscoelastic_rhs": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_update_nodes_for_plastic_yielding": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_update_nodes_for_plastic_yielding_noalloc": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ebg_update_marker_rheology": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ebg_calc_marker_viscoplastic_viscosity": (c_int, [c_void_p, c_int64] + [c_void_p] * 10),
+    "ebg_update_marker_strain": (c_int, [c_void_p, c_int64, c_void_p, c_double, c_double] + [c_void_p] * 10),
     "ebg_grid_strain_rate_and_stress": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_harmonic_normal_viscosity": (c_int, [c_void_p] * 6),
     "ebg_marker_strain_rate_and_pressure": (c_int, [c_void_p, c_void_p, c_void_p]),
@@ -156,6 +159,24 @@ class PlasticNodesIn(C.Structure):
 class PlasticNodesOut(C.Structure):
     _fields_ = [("etavp_new", c_void_p), ("plastic_strain_rate", c_void_p), ("plastic_yield_new", c_void_p),
                 ("yield_error", c_void_p), ("nnode_yield", c_int64), ("global_yield_error", c_double)]
+
+
+class MarkerRheologyIn(C.Structure):
+    """ebg_marker_rheology_in (include/earthbox_gpu.h)."""
+    _fields_ = [("marknum", c_int64), ("inside_flags", c_void_p), ("matid", c_void_p),
+                ("strain_plastic", c_void_p), ("fric_ini", c_void_p), ("melt_damage", c_void_p),
+                ("sxx", c_void_p), ("sxy", c_void_p), ("exx", c_void_p), ("exy", c_void_p), ("TK", c_void_p),
+                ("pr", c_void_p), ("GII", c_void_p), ("sr_ratio", c_void_p), ("nmat", c_int64),
+                ("mat_flow_type", c_void_p), ("mat_flow", c_void_p), ("mat_plastic", c_void_p),
+                ("mat_mu", c_void_p), ("timestep", c_double), ("temperature_top", c_double),
+                ("viscosity_min", c_double), ("viscosity_max", c_double), ("powerlaw_stress_min", c_double),
+                ("gas_constant", c_double), ("iuse_viscous_strain_soft", c_int), ("vsoftfac", c_double),
+                ("itype_global", c_int), ("ntimestep", c_int64), ("stages", c_int)]
+
+
+class MarkerRheologyOut(C.Structure):
+    _fields_ = [("cohesion", c_void_p), ("fric", c_void_p), ("eta_flow", c_void_p), ("preexp", c_void_p),
+                ("eta", c_void_p)]
 
 
 class MarkerStokesProps(C.Structure):

@@ -287,6 +287,92 @@ int ebg_harmonic_normal_viscosity(ebg_ctx *ctx, const double *etas1, const doubl
 int ebg_plastic_strain_rate_at_pressure_nodes(ebg_ctx *ctx, const double *eii_plastic_basic,
                                               double *eii_plastic_pressure);
 
+/* ---- marker rheology ----
+ * Per-marker DEVICE arrays (length marknum); inside_flags NULL = every marker
+ * inside.  Material tables are DEVICE, column-major with leading dimension
+ * nmat, 1-based matid as the reference's materials.arrays: mat_flow_type
+ * (Int64[nmat]), mat_flow (nmat x 12: 1 viscosity, 2 dislocation
+ * pre-exponential, 3 stress exponent, 4 activation energy kJ/mol, 5
+ * activation volume, 6 Peierls pre-exponential, 7-8 Peierls exponents m and
+ * n, 9 Peierls stress MPa, 10 diffusion pre-exponential, 11 diffusion
+ * activation energy, 12 diffusion activation volume), mat_plastic (nmat x 6:
+ * cohesion initial/final, sine of friction initial/final, strain
+ * initial/final), mat_mu (nmat).  `stages` selects which of the three
+ * reference steps run, in this order, in one pass over the markers:
+ *   EBG_RHEO_FAILURE: FailureProperties.calculate_failure_properties_loop!
+ *     (rheology/plastic_failure/failure_properties/FailureProperties.jl):
+ *     cohesion and sine of friction with linear strain weakening of the
+ *     plastic strain, optional melt damage (melt_damage NULL = off);
+ *     reads strain_plastic, fric_ini; writes cohesion, fric.
+ *   EBG_RHEO_FLOW: CompositeViscosity.update_marker_flow_viscosity!
+ *     (rheology/flow_viscosity/update/types/composite_viscosity/CompositeViscosity.jl):
+ *     dislocation (power-law), Peierls and diffusion creep with the
+ *     visco-elastic stress forecast found by bisection (20 steps, 1 Pa),
+ *     optional viscous strain softening (ViscousStrainSoftening.jl), the
+ *     result clipped to [viscosity_min, viscosity_max]; reads sxx, sxy,
+ *     exx, exy, TK, pr, GII, sr_ratio; writes eta_flow, preexp.
+ *   EBG_RHEO_VISCOPLASTIC: ViscoplasticViscosity.update_marker_viscoplastic_viscosity!
+ *     (rheology/viscoplastic_viscosity/ViscoplasticViscosity.jl:21-61):
+ *     eta = eta_flow for itype_global 0, or for itype_global 1 at
+ *     ntimestep 0 or where eta is 0; reads and writes eta.
+ * Same expressions and order as the reference; exp and pow are the device
+ * library's, so values agree with the serial loops to rounding. */
+enum { EBG_RHEO_FAILURE = 1, EBG_RHEO_FLOW = 2, EBG_RHEO_VISCOPLASTIC = 4 };
+typedef struct {
+    int64_t marknum;
+    const int8_t *inside_flags;
+    const int16_t *matid;
+    const double *strain_plastic, *fric_ini, *melt_damage;
+    const double *sxx, *sxy, *exx, *exy, *TK, *pr, *GII, *sr_ratio;
+    int64_t nmat;
+    const int64_t *mat_flow_type;
+    const double *mat_flow, *mat_plastic, *mat_mu;
+    double timestep, temperature_top, viscosity_min, viscosity_max, powerlaw_stress_min, gas_constant;
+    int iuse_viscous_strain_soft;
+    double vsoftfac;
+    int itype_global;
+    int64_t ntimestep;
+    int stages;
+} ebg_marker_rheology_in;
+
+typedef struct {
+    double *cohesion, *fric;     /* EBG_RHEO_FAILURE */
+    double *eta_flow, *preexp;   /* EBG_RHEO_FLOW */
+    double *eta;                 /* EBG_RHEO_VISCOPLASTIC (in/out) */
+} ebg_marker_rheology_out;
+
+/* Replaces Rheology.update_marker_rheology! (rheology/Rheology.jl:36-56, the
+ * failure properties and the composite flow viscosity) and
+ * Rheology.update_marker_viscoplastic_viscosity! (:70-80) with node-based
+ * plasticity, for the composite-viscosity flow law (TimeLoop.jl:222-223). */
+int ebg_update_marker_rheology(ebg_ctx *ctx, const ebg_marker_rheology_in *in, ebg_marker_rheology_out *out);
+
+/* Replaces GridViscoplasticityToMarkers.calc_marker_viscoplastic_viscosity!
+ * (rheology/plastic_failure/nodes/GridViscoplasticityToMarkers.jl), run after
+ * the Picard loop over nodes (TimeLoop.jl:240): where a corner of the
+ * marker's basic cell yields, eta becomes the yield-weighted harmonic average
+ * of etas1 if below eta_flow (and pfailure 1), else eta_flow.  marker_xn/yn
+ * are the 1-based basic cells the scatter stored; the distances are
+ * recomputed as MarkerGridMapping.jl:185-218 does.  DEVICE: plastic_yield,
+ * etas1 (ynum,xnum).  Bit-identical to the serial loop. */
+int ebg_calc_marker_viscoplastic_viscosity(ebg_ctx *ctx, int64_t marknum, const double *marker_x,
+                                           const double *marker_y, const int32_t *marker_xn,
+                                           const int32_t *marker_yn, const int8_t *inside_flags,
+                                           const double *plastic_yield, const double *etas1,
+                                           const double *marker_eta_flow, double *marker_eta,
+                                           float *marker_pfailure);
+
+/* Replaces Advection.update_marker_strain! (advection/Advection.jl:169-178),
+ * i.e. MarkersStrain.update_total_strain! and update_plastic_strain!
+ * (advection/strain/MarkersStrain.jl): marker_GII += dt |e|, and where
+ * pfailure is 1 the plastic strain rate 0.5 sii (1/eta - 1/eta_flow) (at
+ * least 0) and the plastic strain with healing (at least 0).  Bit-identical. */
+int ebg_update_marker_strain(ebg_ctx *ctx, int64_t marknum, const int8_t *inside_flags, double timestep,
+                             double plastic_healing_rate, const double *marker_exx, const double *marker_exy,
+                             const double *marker_sxx, const double *marker_sxy, const double *marker_eta,
+                             const double *marker_eta_flow, const float *marker_pfailure, double *marker_GII,
+                             double *marker_strain_plastic, double *marker_strain_rate_plastic);
+
 /* ---- grid -> marker gathers after the Stokes solve ----
  * Per-marker DEVICE arrays (length marknum): x, y; marker_xn/yn (1-based basic
  * cell, as the scatter stores them); inside_flags (NULL: computed from the

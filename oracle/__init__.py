@@ -520,3 +520,96 @@ def calculate_new_timestep(timestep_old, marker_cell_displ_max, xstpavg, ystpavg
     dt = limit(timestep_old, xstpavg, vmax(vx1))
     dt = limit(dt, ystpavg, vmax(vy1))
     return dt, dt != timestep_old
+
+
+# ---------------------------------------------------------------- marker rheology
+class _RheoIn(C.Structure):
+    _fields_ = [("marknum", _I64), ("flags", _P), ("matid", _P), ("strain_plastic", _P), ("fric_ini", _P),
+                ("melt_damage", _P), ("sxx", _P), ("sxy", _P), ("exx", _P), ("exy", _P), ("TK", _P),
+                ("pr", _P), ("GII", _P), ("sr_ratio", _P), ("nmat", _I64), ("mat_flow_type", _P),
+                ("mat_flow", _P), ("mat_plastic", _P), ("mat_mu", _P), ("timestep", _D),
+                ("temperature_top", _D), ("viscosity_min", _D), ("viscosity_max", _D),
+                ("powerlaw_stress_min", _D), ("gas_constant", _D), ("iuse_viscous_strain_soft", _I64),
+                ("vsoftfac", _D), ("itype_global", _I64), ("ntimestep", _I64)]
+
+
+RHEO_MARKER = ("strain_plastic", "fric_ini", "sxx", "sxy", "exx", "exy", "TK", "pr", "GII", "sr_ratio")
+
+
+def marker_rheology(m: dict, mats: dict, par: dict) -> dict:
+    """Rheology.update_marker_rheology! (failure properties, composite flow
+    viscosity) then update_marker_viscoplastic_viscosity! on copies.  m:
+    marker arrays (RHEO_MARKER, matid int16, flags int8, optional
+    melt_damage, eta); mats: mat_flow_type (nmat,), mat_flow (nmat, 12),
+    mat_plastic (nmat, 6), mat_mu (nmat,); par: the scalars of _RheoIn."""
+    L = lib()
+    for name in ("ebo_failure_properties", "ebo_flow_viscosity"):
+        getattr(L, name).restype = None
+        getattr(L, name).argtypes = [C.POINTER(_RheoIn), _P, _P]
+    L.ebo_viscoplastic_viscosity.restype = None
+    L.ebo_viscoplastic_viscosity.argtypes = [C.POINTER(_RheoIn), _P, _P]
+    n = m["matid"].size
+    keep = {k: np.ascontiguousarray(m[k], dtype=np.float64) for k in RHEO_MARKER}
+    md = m.get("melt_damage")
+    md = None if md is None else np.ascontiguousarray(md, dtype=np.float64)
+    flags = np.ascontiguousarray(m.get("flags", np.ones(n, np.int8)), dtype=np.int8)
+    mid = np.ascontiguousarray(m["matid"], dtype=np.int16)
+    mft = np.ascontiguousarray(mats["mat_flow_type"], dtype=np.int64)
+    mf = np.asfortranarray(mats["mat_flow"], dtype=np.float64)
+    mp = np.asfortranarray(mats["mat_plastic"], dtype=np.float64)
+    mmu = np.ascontiguousarray(mats["mat_mu"], dtype=np.float64)
+    s = _RheoIn(n, p(flags), p(mid), p(keep["strain_plastic"]), p(keep["fric_ini"]), p(md), p(keep["sxx"]),
+                p(keep["sxy"]), p(keep["exx"]), p(keep["exy"]), p(keep["TK"]), p(keep["pr"]), p(keep["GII"]),
+                p(keep["sr_ratio"]), mft.size, p(mft), p(mf), p(mp), p(mmu), par["timestep"],
+                par["temperature_top"], par["viscosity_min"], par["viscosity_max"], par["powerlaw_stress_min"],
+                par["gas_constant"], int(par.get("iuse_viscous_strain_soft", 0)), par.get("vsoftfac", 1.0),
+                int(par.get("itype_global", 1)), int(par.get("ntimestep", 0)))
+    out = {k: np.zeros(n) for k in ("cohesion", "fric", "eta_flow", "preexp")}
+    for k in out:   # outside markers keep their values
+        if k in m:
+            out[k] = np.array(m[k], dtype=np.float64)
+    out["eta"] = np.array(m.get("eta", np.zeros(n)), dtype=np.float64)
+    L.ebo_failure_properties(C.byref(s), p(out["cohesion"]), p(out["fric"]))
+    L.ebo_flow_viscosity(C.byref(s), p(out["eta_flow"]), p(out["preexp"]))
+    L.ebo_viscoplastic_viscosity(C.byref(s), p(out["eta_flow"]), p(out["eta"]))
+    return out
+
+
+def marker_viscoplastic_from_grid(g, m: dict, plastic_yield, etas1):
+    """GridViscoplasticityToMarkers.calc_marker_viscoplastic_viscosity! on
+    copies: (eta, pfailure).  m: x, y, flags, marker_xn/yn (1-based),
+    eta_flow, eta, pfailure (float32).  The distances are
+    MarkerGridMapping.jl:211-214's (x - gridx_b[xn]) / xstp_b[xn]."""
+    L = lib()
+    L.ebo_marker_viscoplastic_from_grid.restype = None
+    L.ebo_marker_viscoplastic_from_grid.argtypes = [_I64, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P]
+    n = m["x"].size
+    xn = np.ascontiguousarray(m["marker_xn"], dtype=np.int32)
+    yn = np.ascontiguousarray(m["marker_yn"], dtype=np.int32)
+    flags = np.ascontiguousarray(m["flags"], dtype=np.int8)
+    ix = np.clip(xn, 1, g.xnum - 1) - 1
+    iy = np.clip(yn, 1, g.ynum - 1) - 1
+    dx = np.ascontiguousarray((m["x"] - g.gridx_b[ix]) / g.xstp_b[ix])
+    dy = np.ascontiguousarray((m["y"] - g.gridy_b[iy]) / g.ystp_b[iy])
+    eta = np.array(m["eta"], dtype=np.float64)
+    pf = np.array(m["pfailure"], dtype=np.float32)
+    L.ebo_marker_viscoplastic_from_grid(n, p(flags), p(xn), p(yn), p(dx), p(dy), g.ynum,
+                                        p(np.asfortranarray(plastic_yield, dtype=np.float64)),
+                                        p(np.asfortranarray(etas1, dtype=np.float64)),
+                                        p(np.ascontiguousarray(m["eta_flow"], dtype=np.float64)), p(eta), p(pf))
+    return eta, pf
+
+
+def update_marker_strain(m: dict, timestep: float, healing: float):
+    """Advection.update_marker_strain! on copies: (GII, strain_plastic,
+    strain_rate_plastic)."""
+    L = lib()
+    L.ebo_update_marker_strain.restype = None
+    L.ebo_update_marker_strain.argtypes = [_I64, _P, _D, _D] + [_P] * 10
+    n = m["exx"].size
+    f = lambda k: np.ascontiguousarray(m[k], dtype=np.float64)
+    gii, ps, psr = (np.array(m[k], dtype=np.float64) for k in ("GII", "strain_plastic", "strain_rate_plastic"))
+    L.ebo_update_marker_strain(n, p(np.ascontiguousarray(m["flags"], dtype=np.int8)), timestep, healing,
+                               p(f("exx")), p(f("exy")), p(f("sxx")), p(f("sxy")), p(f("eta")), p(f("eta_flow")),
+                               p(np.ascontiguousarray(m["pfailure"], dtype=np.float32)), p(gii), p(ps), p(psr))
+    return gii, ps, psr
