@@ -13,6 +13,7 @@ stops by LoopTermination.check_stop_global_loop_nodes
 from __future__ import annotations
 
 import dataclasses
+import time
 
 import numpy as np
 import torch
@@ -80,9 +81,11 @@ def picard_loop_nodes(ctx, template: StokesProblem, state: dict, plastic_params:
                                              template.iuse_interface_stabilization)
         etan11 = float(ve["etan0"][0, 0].item())
         prob = dataclasses.replace(template, pscale=2.0 * etan11 / (g.xstpavg + g.ystpavg), timestep=dt)
+        t0 = time.perf_counter()
         solver.set_system(prob, fields={"etas0": ve["etas0"], "etan0": ve["etan0"], "rho1_vy": state["rho1_vy"],
                                         "RX1": ve["RX1"], "RY1": ve["RY1"], "RC1": ve["RC1"]})
-        S, st = solver.solve_system(**opts)
+        S, st = solver.solve_system(**opts)   # returns after the device finished (stats read back)
+        solve_seconds = time.perf_counter() - t0
         vx1, vy1, pr1 = solver.process_stokes_solution(S)
         # stokes_solution_processing! on the grid: the time step limit (the
         # strain rates do not depend on it), then the stress forecast with it
@@ -106,7 +109,8 @@ def picard_loop_nodes(ctx, template: StokesProblem, state: dict, plastic_params:
             plastic_strain_rate_at_pressure_nodes(ctx, eii_basic, state["eii_plastic_pressure"])
         log.append({"iglobal": iglobal, "iterations": st["iterations"], "relres": st["relres"],
                     "nnode_yield": nnode_yield, "global_yield_error": global_yield_error,
-                    "dvxy_rel_L2": dvxy_rel_L2, "timestep": dt})
+                    "dvxy_rel_L2": dvxy_rel_L2, "timestep": dt, "solve_seconds_with_setup": solve_seconds,
+                    "learned_accepted": st["learned_accepted"]})
         sol = {"vx1": vx1, "vy1": vy1, "pr1": pr1, "timestep": dt, **ss}
         if check_stop_global_loop_nodes(nnode_yield, iglobal, nglobal, global_yield_error, dvxy_rel_L2,
                                         tolerance):

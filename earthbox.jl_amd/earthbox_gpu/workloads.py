@@ -397,3 +397,95 @@ def rift2d(xnum: int = 2001, ynum: int = 801, xsize: float = 5.0e5, ysize: float
     params = dict(timestep=timestep, timesum=timestep, ntimestep=1, viscosity_min=1e18, viscosity_max=1e25,
                   iuse_fluid_pressure_for_yield=0)
     return template, state, params
+
+
+# Four creeping layers of the rifting model: dislocation creep (power law,
+# n = 3-3.5) with diffusion and Peierls creep in the lithospheric mantle, and
+# Drucker-Prager parameters with linear strain weakening (cohesion 20 -> 4
+# MPa, friction 30 -> 5 degrees over plastic strain 0 -> 1).  Columns follow
+# the reference's materials.arrays (1-based): mat_flow 1 viscosity, 2
+# A_dis (1/s/MPa^n), 3 n, 4 Ea (kJ/mol), 5 Va (J/MPa/mol), 6-9 Peierls
+# (A, m, n, sigma_p MPa), 10-12 diffusion (A, Ea, Va); mat_plastic cohesion
+# initial/final, sine friction initial/final, plastic strain initial/final.
+RIFT_MATERIALS = {
+    # name: (bottom depth m, density kg/m3 at 273 K, mat_flow row, mat_plastic row)
+    "upper_crust": (2.0e4, 2700.0, (0.0, 5.0e-6, 3.0, 190.0, 0.0, 0.0, 1.0, 2.0, 9100.0, 0.0, 0.0, 0.0),
+                    (2e7, 4e6, 0.5, 0.0872, 0.0, 1.0)),
+    "lower_crust": (3.5e4, 2900.0, (0.0, 3.3e-4, 3.2, 238.0, 0.0, 0.0, 1.0, 2.0, 9100.0, 0.0, 0.0, 0.0),
+                    (2e7, 4e6, 0.5, 0.0872, 0.0, 1.0)),
+    "lithospheric_mantle": (1.0e5, 3300.0,
+                            (0.0, 2.5e4, 3.5, 532.0, 8.0, 5.7e11, 1.0, 2.0, 9100.0, 1.5e-9, 300.0, 4.0),
+                            (2e7, 4e6, 0.6, 0.1, 0.0, 1.0)),
+    "asthenosphere": (np.inf, 3300.0, (0.0, 2.0e3, 3.5, 471.0, 4.0, 0.0, 1.0, 2.0, 9100.0, 0.0, 0.0, 0.0),
+                      (2e7, 4e6, 0.6, 0.1, 0.0, 1.0)),
+}
+
+
+def rift2d_markers(xnum: int = 2001, ynum: int = 801, nm: int = 5, xsize: float = 5.0e5, ysize: float = 1.6e5,
+                   seed: int = 5, extension: float = 0.5 / 100 / 3.15576e7, gravity_y: float = 9.81,
+                   timestep: float = 3.15576e12):
+    """Marker-based visco-elasto-plastic rifting model on xnum x ynum nodes over
+    xsize x ysize (2001 x 801 over 500 x 160 km, 5 x 5 markers per cell:
+    40M markers): four layers (upper crust 0-20 km, lower crust 20-35 km,
+    lithospheric mantle 35-100 km, asthenosphere) whose boundaries are
+    perturbed by a seeded +-1 km undulation; a seeded geotherm (273 K at the
+    surface, 823 K at the Moho, 1573 K at 100 km, then 0.4 K/km, plus a
+    seeded +-10 K perturbation); densities with thermal expansion 3e-5/K; the
+    creep laws and strain-weakening Drucker-Prager parameters of
+    RIFT_MATERIALS; shear modulus 3e10; a seeded 10 km wide weak seed of
+    plastic strain 0.5-1.0 in the upper crust; sides pulled apart at +-0.5
+    cm/yr with the outflow replaced through the bottom, free slip elsewhere.
+    Markers start stress-free with the background extension strain rate and
+    lithostatic pressure (the synthetic initial state).
+
+    Returns (template StokesProblem with the grid, boundary conditions and
+    gravity; marker arrays (numpy); material tables (numpy); scalars)."""
+    from .grid import uniform_grid
+    from .stokes_problem import StokesProblem, free_slip_bcs
+    g = uniform_grid(xnum, ynum, xsize, ysize)
+    rng = np.random.default_rng(seed)
+    x, y = marker_positions(g, nm, nm, seed)
+    n = x.size
+    names = list(RIFT_MATERIALS)
+    # layer boundaries with a seeded undulation (four Fourier modes, +-1 km)
+    k = np.arange(1, 5)
+    amp = rng.uniform(-250.0, 250.0, (3, 4))
+    ph = rng.uniform(0, 2 * np.pi, (3, 4))
+    matid = np.full(n, 4, dtype=np.int16)
+    for li in (2, 1, 0):
+        bottom = RIFT_MATERIALS[names[li]][0]
+        und = (amp[li][:, None] * np.sin(2 * np.pi * k[:, None] * x[None, :] / xsize + ph[li][:, None])).sum(0)
+        matid[y < bottom + und] = li + 1
+    # geotherm (piecewise linear) with a seeded perturbation
+    TK = np.interp(y, [0.0, 3.5e4, 1.0e5, 1e9], [273.0, 823.0, 1573.0, 1573.0 + 0.4e-3 * (1e9 - 1e5)])
+    TK += 10.0 * np.sin(2 * np.pi * x / xsize * 3 + rng.uniform(0, 2 * np.pi)) * np.exp(-y / 5e4)
+    rho0 = np.array([RIFT_MATERIALS[nm_][1] for nm_ in names])
+    rho = rho0[matid - 1] * (1.0 - 3e-5 * (TK - 273.0))
+    # weak seed of plastic strain in the upper crust
+    xc = xsize / 2 + 1e4 * (rng.random() - 0.5)
+    strain_plastic = np.where((np.abs(x - xc) < 5e3) & (y < 2.0e4), rng.uniform(0.5, 1.0, n), 0.0)
+    eb = 2.0 * extension / xsize
+    mats = {"mat_flow_type": np.ones(4, dtype=np.int64),
+            "mat_flow": np.asfortranarray([RIFT_MATERIALS[nm_][2] for nm_ in names]),
+            "mat_plastic": np.asfortranarray([RIFT_MATERIALS[nm_][3] for nm_ in names]),
+            "mat_mu": np.full(4, 3e10)}
+    markers = {"x": x, "y": y, "matid": matid, "TK": TK, "rho": rho,
+               "pr": 3000.0 * gravity_y * np.clip(y, 0.0, None),
+               "sxx": np.zeros(n), "sxy": np.zeros(n), "exx": np.full(n, eb), "exy": np.zeros(n),
+               "sr_ratio": np.ones(n), "GII": np.zeros(n), "strain_plastic": strain_plastic,
+               "strain_rate_plastic": np.zeros(n), "fric_ini": mats["mat_plastic"][matid - 1, 2].copy(),
+               "cohesion": np.zeros(n), "fric": np.zeros(n), "eta_flow": np.zeros(n), "preexp": np.zeros(n),
+               "eta": np.zeros(n), "pfailure": np.zeros(n, np.float32)}
+    ny, nx = ynum, xnum
+    bcs = free_slip_bcs(g, vx_left=-extension, vx_right=extension,
+                        vy_bottom=-2.0 * extension * ysize / xsize)
+    z = lambda s: np.zeros(s, order="F")
+    template = StokesProblem(grid=g, etas0=z((ny, nx)), etan0=z((ny - 1, nx - 1)), rho1_vy=z((ny, nx + 1)),
+                             RX1=z((ny + 1, nx)), RY1=z((ny, nx + 1)), RC1=z((ny - 1, nx - 1)), bcs=bcs,
+                             pressure_bc_mode=0, pressure_bc=0.0, timestep=timestep, gravity_y=gravity_y,
+                             iuse_interface_stabilization=0)
+    params = dict(timestep=timestep, temperature_top=273.0, viscosity_min=1e18, viscosity_max=1e25,
+                  powerlaw_stress_min=1e4, gas_constant=8.314, iuse_viscous_strain_soft=0, vsoftfac=1.0,
+                  itype_global=1, marker_cell_displ_max=0.5, plastic_healing_rate=0.0,
+                  iuse_fluid_pressure_for_yield=0, nglobal=10, tolerance_picard=1e-3)
+    return template, markers, mats, params
