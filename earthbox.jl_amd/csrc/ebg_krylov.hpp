@@ -11,11 +11,16 @@
 namespace ebg {
 
 constexpr int RED_BLOCKS = 512;  // partial sums per reduction
-constexpr int MAXK = 64;         // longest Krylov basis a multi-dot handles
+constexpr int MAXK = 256;        // longest Krylov basis a multi-dot handles (2-D restart <= MAXK - 2)
+constexpr int DOT_CHUNK = 8;     // basis vectors per pass of the multi-dot (register accumulators)
 
-// partial[b*k + q] = sum over block b's grid-stride slice of w * V[q]
-__global__ void multidot_kernel(const double* __restrict__ w, const double* __restrict__ V, int64_t ldv,
-                                int k, int64_t n, double* __restrict__ partial);
+// partial[b*k + q] = sum over block b's grid-stride slice of w * V[q] for
+// every q < k, DOT_CHUNK basis vectors per pass
+__global__ void multidot_chunks_kernel(const double* __restrict__ w, const double* __restrict__ V, int64_t ldv,
+                                       int k, int64_t n, double* __restrict__ partial);
+// out[q] (+)= sum_b partial[b*k + q], one wavefront per q, fixed order
+__global__ void reduce_columns_kernel(const double* __restrict__ partial, int nblocks, int k,
+                                      double* __restrict__ out, int accumulate);
 // out[q] (+)= sum_b partial[b*k + q], one block, fixed order
 __global__ void reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int k,
                                        double* __restrict__ out, int accumulate);

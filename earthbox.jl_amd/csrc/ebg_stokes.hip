@@ -147,29 +147,55 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// partial[b*(k)+q] = sum over this block's grid-stride slice of w * V[q]
-__global__ void __launch_bounds__(256) multidot_kernel(const double* __restrict__ w,
-                                                      const double* __restrict__ V, int64_t ldv,
-                                                      int k, int64_t n, double* __restrict__ partial)
+// partial[b*k + q] = block b's grid-stride share of <w, V[q]> for every q < k.
+// The basis is taken DOT_CHUNK vectors per pass, with the accumulators in
+// registers (a fixed-size, fully unrolled array), so w is re-read once per
+// chunk and the basis exactly once.
+__global__ void __launch_bounds__(256) multidot_chunks_kernel(const double* __restrict__ w,
+                                                             const double* __restrict__ V, int64_t ldv,
+                                                             int k, int64_t n, double* __restrict__ partial)
 {
-    __shared__ double sh[4][MAXK];
-    double acc[MAXK];
-    for (int q = 0; q < k; ++q) acc[q] = 0.0;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const double wv = w[t];
-        for (int q = 0; q < k; ++q) acc[q] += wv * V[(int64_t)q * ldv + t];
-    }
+    __shared__ double sh[4][DOT_CHUNK];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int q = 0; q < k; ++q) {
-        const double s = wave_sum(acc[q]);
-        if (lane == 0) sh[wid][q] = s;
+    for (int q0 = 0; q0 < k; q0 += DOT_CHUNK) {
+        const int kc = min(DOT_CHUNK, k - q0);
+        const double* Vc = V + (int64_t)q0 * ldv;
+        double acc[DOT_CHUNK];
+#pragma unroll
+        for (int q = 0; q < DOT_CHUNK; ++q) acc[q] = 0.0;
+        for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+             t += (int64_t)gridDim.x * blockDim.x) {
+            const double wv = w[t];
+#pragma unroll
+            for (int q = 0; q < DOT_CHUNK; ++q)
+                if (q < kc) acc[q] += wv * Vc[(int64_t)q * ldv + t];
+        }
+#pragma unroll
+        for (int q = 0; q < DOT_CHUNK; ++q) {
+            const double v = wave_sum(acc[q]);
+            if (lane == 0) sh[wid][q] = v;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < kc) {
+            const int q = threadIdx.x;
+            partial[(int64_t)blockIdx.x * k + q0 + q] = ((sh[0][q] + sh[1][q]) + sh[2][q]) + sh[3][q];
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x < k) {
-        const int q = threadIdx.x;
-        partial[(int64_t)blockIdx.x * k + q] = ((sh[0][q] + sh[1][q]) + sh[2][q]) + sh[3][q];
-    }
+}
+
+// out[q] = sum_b partial[b*k+q]: one wavefront per column q (four per block),
+// lanes take blocks b = lane, lane + 64, ..., then a fixed shuffle tree
+__global__ void __launch_bounds__(256) reduce_columns_kernel(const double* __restrict__ partial, int nblocks, int k,
+                                                            double* __restrict__ out, int accumulate)
+{
+    const int q = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (q >= k) return;   // whole wavefronts leave together
+    double v = 0.0;
+    for (int b = lane; b < nblocks; b += 64) v += partial[(int64_t)b * k + q];
+    v = wave_sum(v);
+    if (lane == 0) out[q] = accumulate ? out[q] + v : v;
 }
 
 // out[q] = sum_b partial[b*k+q] (one block, fixed order)
@@ -652,9 +678,9 @@ static void load_viscosity(StokesState& st, MGHierarchy<T>& H, hipStream_t s)
 static void multidot(StokesState& st, const double* w, const double* V, int64_t ldv, int k, int64_t n,
                      double* out, int accumulate, hipStream_t s)
 {
-    hipLaunchKernelGGL(multidot_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, w, V, ldv, k, n,
+    hipLaunchKernelGGL(multidot_chunks_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, w, V, ldv, k, n,
                        st.partial.as<double>());
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, st.partial.as<double>(),
+    hipLaunchKernelGGL(reduce_columns_kernel, dim3((k + 3) / 4), dim3(256), 0, s, st.partial.as<double>(),
                        RED_BLOCKS, k, out, accumulate);
 }
 
@@ -810,7 +836,7 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
     auto norm = [&](const double* v, double* out) {
         multidot(st, v, v, 0, 1, n, out, 0, s);
     };
-    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hh(MAXK + 1);
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), hbuf(2 * MAXK);
     double bnorm2 = 0;
     norm(b, st.nrm.as<double>());
     EBG_HIP(hipMemcpyAsync(&bnorm2, st.nrm.p, 8, hipMemcpyDeviceToHost, s));
@@ -870,18 +896,29 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
             }
             hipLaunchKernelGGL((stokes_rows_kernel<false, LayoutPlanar>), dim3(nb(nc)), dim3(256), 0, s,
                                st.op, zk, w, LP);
-            if (learned_dir) norm(w, st.nrm.as<double>() + 2);
-            // classical Gram-Schmidt, twice (CGS2)
+            // classical Gram-Schmidt; a second pass only when the first one
+            // cancelled more than half of the squared norm ("twice is enough")
+            norm(w, st.nrm.as<double>() + 1);
             multidot(st, w, V, n, k + 1, n, hd, 0, s);
             hipLaunchKernelGGL(update_kernel, dim3(nb(n)), dim3(256), 0, s, w, V, n, hd, k + 1, n);
-            multidot(st, w, V, n, k + 1, n, hd + MAXK, 0, s);
-            hipLaunchKernelGGL(update_kernel, dim3(nb(n)), dim3(256), 0, s, w, V, n, hd + MAXK, k + 1, n);
             norm(w, st.nrm.as<double>());
-            double hbuf[2 * MAXK], wn[3], acc = judged;
-            EBG_HIP(hipMemcpyAsync(hbuf, hd, sizeof(double) * 2 * MAXK, hipMemcpyDeviceToHost, s));
+            double wn[2], acc = judged;
+            EBG_HIP(hipMemcpyAsync(hbuf.data(), hd, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, s));
             EBG_HIP(hipMemcpyAsync(wn, st.nrm.p, sizeof(wn), hipMemcpyDeviceToHost, s));
             if (judging) EBG_HIP(hipMemcpyAsync(&acc, st.judge.p, 8, hipMemcpyDeviceToHost, s));
             EBG_HIP(hipStreamSynchronize(s));
+            double wn2 = wn[0];
+            for (int q = 0; q <= k; ++q) H[(size_t)k * (m + 1) + q] = hbuf[q];
+            if (!(wn2 > 0.5 * wn[1])) {
+                multidot(st, w, V, n, k + 1, n, hd + MAXK, 0, s);
+                hipLaunchKernelGGL(update_kernel, dim3(nb(n)), dim3(256), 0, s, w, V, n, hd + MAXK, k + 1, n);
+                norm(w, st.nrm.as<double>());
+                EBG_HIP(hipMemcpyAsync(hbuf.data() + MAXK, hd + MAXK, sizeof(double) * (k + 1),
+                                       hipMemcpyDeviceToHost, s));
+                EBG_HIP(hipMemcpyAsync(&wn2, st.nrm.p, 8, hipMemcpyDeviceToHost, s));
+                EBG_HIP(hipStreamSynchronize(s));
+                for (int q = 0; q <= k; ++q) H[(size_t)k * (m + 1) + q] += hbuf[MAXK + q];
+            }
             if (judging) {
                 // the plain V-cycle is the fallback: after learned_patience
                 // rejections in a row the network is no longer evaluated
@@ -890,17 +927,15 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
                 judged = acc;
                 if (o.learned_patience > 0 && rejects >= o.learned_patience) learned_on = false;
             }
-            const double wn2 = wn[0];
             // a learned direction that lies (almost) in the current space would
             // turn rounding noise into the next basis vector: drop it
             if (learned_dir) {
-                if (!(wn2 > 1e-4 * wn[2])) {
+                if (!(wn2 > 1e-4 * wn[1])) {
                     --k;
                     continue;
                 }
                 ++learned_dirs;
             }
-            for (int q = 0; q <= k; ++q) H[(size_t)k * (m + 1) + q] = hbuf[q] + hbuf[MAXK + q];
             const double hn = std::sqrt(wn2);
             H[(size_t)k * (m + 1) + k + 1] = hn;
             if (!std::isfinite(hn)) {
@@ -1189,7 +1224,7 @@ static int solve_impl(ebg_ctx* ctx, const double* R, double* S, const ebg_solve_
     ebg_solve_opts o;
     if (opts) o = *opts;
     else ebg_stokes_solve_opts_default(&o);
-    EBG_CHECK(o.restart >= 1 && o.restart < MAXK - 1, EBG_ERR_ARG, "ebg_stokes_solve: restart must be in [1, 62]");
+    EBG_CHECK(o.restart >= 1 && o.restart < MAXK - 1, EBG_ERR_ARG, "ebg_stokes_solve: restart must be in [1, 254]");
     EBG_CHECK(o.precision == 0 || o.precision == 1, EBG_ERR_ARG, "ebg_stokes_solve: precision must be 0 or 1");
     EBG_CHECK(o.rtol > 0 && o.max_iterations >= 0, EBG_ERR_ARG, "ebg_stokes_solve: bad tolerance");
     EBG_CHECK(o.finest_sweeps >= 1 && o.sweep_multiplier >= 1, EBG_ERR_ARG, "ebg_stokes_solve: bad sweeps");

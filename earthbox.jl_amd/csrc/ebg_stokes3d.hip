@@ -194,48 +194,15 @@ static void precondition3(Stokes3State& st, MG3Hierarchy<T>& H, const ebg_solve_
 // (m+1) vectors of the level layout (545 MB each at 257^3 in FP64), plus m
 // preconditioned vectors in the flexible mode
 constexpr int MAX_RESTART3 = 240;
-constexpr int DOT_CHUNK = 8;  // basis vectors per multi-dot pass (register accumulators)
-
-// partial[b*kc + q] = block b's grid-stride share of <w, V[q]>, q < kc <= 8;
-// accumulators stay in registers (a fixed-size, fully unrolled array)
-__global__ void __launch_bounds__(256) multidot8_kernel(const double* __restrict__ w, const double* __restrict__ V,
-                                                        int64_t ldv, int kc, int64_t n, double* __restrict__ partial)
-{
-    __shared__ double sh[4][DOT_CHUNK];
-    double acc[DOT_CHUNK];
-#pragma unroll
-    for (int q = 0; q < DOT_CHUNK; ++q) acc[q] = 0.0;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        const double wv = w[t];
-#pragma unroll
-        for (int q = 0; q < DOT_CHUNK; ++q)
-            if (q < kc) acc[q] += wv * V[(int64_t)q * ldv + t];
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int q = 0; q < DOT_CHUNK; ++q) {
-        double v = acc[q];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (lane == 0) sh[wid][q] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < kc) {
-        const int q = threadIdx.x;
-        partial[(int64_t)blockIdx.x * kc + q] = ((sh[0][q] + sh[1][q]) + sh[2][q]) + sh[3][q];
-    }
-}
-
-// out[q] = <w, V[q]> for q < k, in chunks of DOT_CHUNK vectors (fixed order)
+// out[q] = <w, V[q]> for q < k: DOT_CHUNK basis vectors per pass inside one
+// launch, then one fixed-order reduction per column
 static void multidot3(Stokes3State& st, const double* w, const double* V, int64_t ldv, int k, int64_t n, double* out,
                       hipStream_t s)
 {
-    for (int q0 = 0; q0 < k; q0 += DOT_CHUNK) {
-        const int kc = std::min(DOT_CHUNK, k - q0);
-        hipLaunchKernelGGL(multidot8_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, w, V + (int64_t)q0 * ldv, ldv, kc, n,
-                           st.partial.as<double>());
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, s, st.partial.as<double>(), RED_BLOCKS, kc,
-                           out + q0, 0);
-    }
+    hipLaunchKernelGGL(multidot_chunks_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, w, V, ldv, k, n,
+                       st.partial.as<double>());
+    hipLaunchKernelGGL(reduce_columns_kernel, dim3((k + 3) / 4), dim3(256), 0, s, st.partial.as<double>(), RED_BLOCKS,
+                       k, out, 0);
 }
 
 static int prepare3(ebg_ctx* ctx, Stokes3State& st, const ebg_solve_opts& o, std::vector<int>& sm)
@@ -270,7 +237,7 @@ static int fgmres3(ebg_ctx* ctx, Stokes3State& st, const ebg_solve_opts& o, cons
     int rc;
     if ((rc = st.b.ensure(n * 8)) || (rc = st.r.ensure(n * 8)) || (rc = st.w.ensure(n * 8)) ||
         (rc = st.V.ensure((size_t)(m + 1) * n * 8)) || (rc = st.Z.ensure((size_t)(flexible ? m : 1) * n * 8)) ||
-        (rc = st.partial.ensure((size_t)RED_BLOCKS * DOT_CHUNK * 8)) ||
+        (rc = st.partial.ensure((size_t)RED_BLOCKS * (MAX_RESTART3 + 1) * 8)) ||
         (rc = st.hdev.ensure((size_t)(MAX_RESTART3 + 1) * 8 * 2)) ||
         (rc = st.ydev.ensure((size_t)(MAX_RESTART3 + 1) * 8)) || (rc = st.nrm.ensure(64)))
         return rc;

@@ -24,6 +24,9 @@ MG_FIELDS = {"etas": 0, "etan": 1, "vx": 2, "vy": 3, "pr": 4, "RX": 5, "RY": 6, 
              "rx": 8, "ry": 9, "rc": 10}
 
 
+RESTART_2D = 100   # FGMRES restart of the 2-D solves (the C default, 30, suits the 3-D basis memory)
+
+
 def default_opts(**kw) -> SolveOpts:
     o = SolveOpts()
     check(lib.ebg_stokes_solve_opts_default(C.byref(o)), "ebg_stokes_solve_opts_default")
@@ -98,7 +101,7 @@ class StokesSolverGPU:
         """FGMRES + multigrid; returns (S, stats dict).  Raises on hard errors;
         stats["converged"] is False when the iteration budget ran out."""
         S = self.new_vector() if S is None else S
-        o = default_opts(**opts)
+        o = default_opts(**{"restart": RESTART_2D, **opts})
         st = SolveStats()
         rc = check(lib.ebg_stokes_solve(self.ctx.handle, ptr(S, np.float64, self.prob.N),
                                         C.byref(o), C.byref(st)), "ebg_stokes_solve")
@@ -109,7 +112,7 @@ class StokesSolverGPU:
     def solve_rhs(self, R: torch.Tensor, S: torch.Tensor | None = None, **opts):
         """solve_system with a caller right-hand side R (reference order)."""
         S = self.new_vector() if S is None else S
-        o = default_opts(**opts)
+        o = default_opts(**{"restart": RESTART_2D, **opts})
         st = SolveStats()
         rc = check(lib.ebg_stokes_solve_rhs(self.ctx.handle, ptr(R, np.float64, self.prob.N),
                                             ptr(S, np.float64, self.prob.N), C.byref(o), C.byref(st)),

@@ -175,7 +175,7 @@ class Stokes3DSolverGPU:
 
     # preconditioner settings measured best on the 3-D inclusion problems
     # (six sweeps on the finest level, pressure relaxation 1, coarse cap 64)
-    DEFAULTS = {"finest_sweeps": 6, "theta_c": 1.0, "sweep_cap": 64}
+    DEFAULTS = {"finest_sweeps": 6, "theta_c": 1.0, "sweep_cap": 64, "restart": 30}
 
     def solve(self, rhs: torch.Tensor, prnorm: float = 0.0, x: torch.Tensor | None = None, **opts):
         """Solve for rhs = [RX | RY | RZ | RC]; returns (x, stats dict)."""

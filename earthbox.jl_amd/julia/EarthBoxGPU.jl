@@ -164,7 +164,8 @@ function solve_system_gpu!(st::GPUState, model; mixed::Bool=true, rtol::Float64=
     dS = h2d!(st, :S, S)
     o = Ref{SolveOpts}()
     ccall((:ebg_stokes_solve_opts_default, LIB), Cint, (Ref{SolveOpts},), o)
-    o[] = SolveOpts(mixed ? 1 : 0, o[].restart, o[].max_iterations, rtol, o[].nlevels,
+    # restart 100: the basis holds the plastic systems without a restart
+    o[] = SolveOpts(mixed ? 1 : 0, 100, o[].max_iterations, rtol, o[].nlevels,
                     o[].finest_sweeps, o[].sweep_multiplier, o[].sweep_cap, o[].theta_s,
                     o[].theta_c, 0, 0,
                     (vc.btopx.array[2, 2], vc.bbottomx.array[2, 2], vc.blefty.array[2, 2],

@@ -476,7 +476,9 @@ int ebg_calculate_new_timestep(ebg_ctx *ctx, const double *vx1, const double *vy
 
 typedef struct {
     int precision;        /* 1: FP32 V-cycle preconditioner (mixed), 0: FP64 (plain) */
-    int restart;          /* FGMRES restart length (1..62), default 30 */
+    int restart;          /* FGMRES restart length (2-D 1..254, 3-D 1..240), default 30; the 2-D
+                             host wrappers use 100: the Krylov basis (2 m + 1 vectors, 7.7 GB at
+                             2001 x 801) holds the plastic systems without a restart */
     int max_iterations;   /* total FGMRES iterations */
     double rtol;          /* stop at ||R - L*S||_2 / ||R||_2 <= rtol (FP64) */
     int nlevels;          /* multigrid levels, 0 = reference rule (LevelManager.jl:441-457) */
