@@ -35,6 +35,11 @@ RK4_MEASURED_TRAFFIC = {
 }
 
 
+def progress(msg):
+    """One progress line on stderr (stdout carries only the JSON result)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +63,11 @@ def parse():
     ap.add_argument("--no-picard", action="store_true")
     ap.add_argument("--no-grid-kernels", action="store_true")
     ap.add_argument("--no-marker-gathers", action="store_true")
+    ap.add_argument("--no-config1", action="store_true")
+    ap.add_argument("--no-config5", action="store_true")
+    ap.add_argument("--config5-steps", type=int, default=20)
+    ap.add_argument("--stokes-amplitude", type=float, default=1.0,
+                    help="decades of the random log-viscosity perturbation of the 1001 x 501 system")
     return ap.parse_args()
 
 
@@ -248,20 +258,24 @@ def cpu_sinker_driver(cycles: int = 3):
             "kind": "port", "sample": f"{cycles} V-cycles of the 97^3 sinker (oracle C restatement)"}
 
 
-def bench_stokes(ctx, nx, ny, max_iter):
+def bench_stokes(ctx, nx, ny, max_iter, amplitude=1.0):
     """Time to ||R - L S|| / ||R|| <= 1e-8 (FP64) of the layered random
-    log-viscosity system (18-25, 1500 x 600 km) for three preconditioners of
-    the FP64 FGMRES: the FP64 V-cycle, the FP32 V-cycle, and the FP32 V-cycle
-    with the learned correction judged by the FP64 residual."""
+    log-viscosity system (18-25, 1500 x 600 km, perturbation of `amplitude`
+    decades) for three preconditioners of the FP64 FGMRES(100): the FP64
+    V-cycle, the FP32 V-cycle, and the FP32 V-cycle with the learned
+    correction judged by the FP64 residual.  Times include the per-system
+    set-up (new fields, viscosity restriction, coarsest-level inverse); the
+    solve alone is reported beside them."""
     import torch
     from earthbox_gpu import workloads
     from earthbox_gpu.stokes import StokesSolverGPU, bc_mirror_factors
-    prob = workloads.layered_random(nx, ny, 1.5e6 * (nx - 1) / 1000, 6.0e5 * (ny - 1) / 500, seed=2)
+    prob = workloads.layered_random(nx, ny, 1.5e6 * (nx - 1) / 1000, 6.0e5 * (ny - 1) / 500, seed=2,
+                                    amplitude=amplitude)
     sol = StokesSolverGPU(ctx)
     sol.set_system(prob)
     bcd = bc_mirror_factors(prob)
-    out = {"system": f"layered random log10(eta) in [18,25], {nx}x{ny} nodes, N={prob.N}",
-           "rtol": 1e-8}
+    out = {"system": f"layered random log10(eta) in [18,25] (+-{amplitude} decades), {nx}x{ny} nodes, "
+                     f"N={prob.N}", "rtol": 1e-8}
     from earthbox_gpu import learned
     net = learned.LearnedCorrection(ctx)
     if learned.WEIGHTS_FILE.exists():
@@ -291,9 +305,144 @@ def bench_stokes(ctx, nx, ny, max_iter):
                      "setup_seconds": setup, "seconds_with_setup": el + setup}
     p, m, lm = out["plain_fp64_mg"], out["mixed_fp32_mg"], out["learned_mixed"]
     if p["converged"] and m["converged"]:
-        out["speedup_mixed_vs_plain"] = p["seconds"] / m["seconds"]
+        out["speedup_mixed_vs_plain"] = p["seconds_with_setup"] / m["seconds_with_setup"]
+        out["speedup_mixed_vs_plain_solve_only"] = p["seconds"] / m["seconds"]
     if p["converged"] and lm["converged"]:
-        out["speedup_learned_mixed_vs_plain"] = p["seconds"] / lm["seconds"]
+        out["speedup_learned_mixed_vs_plain"] = p["seconds_with_setup"] / lm["seconds_with_setup"]
+        out["speedup_learned_mixed_vs_plain_solve_only"] = p["seconds"] / lm["seconds"]
+    return out
+
+
+def bench_config1(ctx):
+    """The 101 x 101 falling block (contrast 1e6): GPU time to 1e-8 and 1e-10
+    (set-up included) with each preconditioner, the forward error of each
+    solution against the FP64 sparse direct solve (SuperLU + 3 refinement
+    steps, standing in for MUMPS; velocity, pressure outside the block and
+    inside it), and that direct solve's time on one host core."""
+    import numpy as np
+    import torch
+    import oracle
+    from earthbox_gpu import workloads
+    from earthbox_gpu.device import to_host
+    from earthbox_gpu.stokes import StokesSolverGPU
+    pr = workloads.falling_block(101, 101)
+    Li, Lj, Lv, R = oracle.stokes_build(pr)
+    t0 = time.perf_counter()
+    Sd = oracle.direct_solve(pr.N, Li, Lj, Lv, R)
+    t_direct = time.perf_counter() - t0
+    ex, ey, ep = oracle.stokes_unpack(pr, Sd)
+    ev = np.concatenate([ex.ravel(), ey.ravel()])
+    blk = pr.etan0 > 1e26
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out = {"system": "falling block 101x101 (1e21 / 1e27 Pa s, 3200 / 3300 kg/m3), N=30,000",
+           "direct_solve_cpu": {"seconds": t_direct, "kind": "port",
+                                "method": "scipy SuperLU (COLAMD) + 3 FP64 refinement steps, 1 core"}}
+    sol = StokesSolverGPU(ctx)
+    for name, prec in (("mixed_fp32_mg", 1), ("plain_fp64_mg", 0)):
+        for rtol in (1e-8, 1e-10):
+            sol.set_system(pr)
+            sol.solve_system(precision=prec, rtol=rtol, max_iterations=2)   # build + warm
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sol.set_system(pr)
+            S, st = sol.solve_system(precision=prec, rtol=rtol, max_iterations=3000)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            vx, vy, p = oracle.stokes_unpack(pr, to_host(S))
+            v = np.concatenate([vx.ravel(), vy.ravel()])
+            out[f"{name}_rtol_{rtol:g}"] = {
+                "seconds_with_setup": el, "iterations": st["iterations"], "relres": st["relres"],
+                "converged": st["converged"], "forward_error_velocity": rel(v, ev),
+                "forward_error_pressure_outside_block": rel(p[~blk], ep[~blk]),
+                "forward_error_pressure_inside_block": rel(p[blk], ep[blk])}
+    return out
+
+
+def bench_config5(ctx, steps):
+    """The rifting model on 2001 x 801 nodes with 5 x 5 markers per cell (40M
+    markers; four creeping layers with a geotherm, strain-weakening
+    Drucker-Prager, +-0.5 cm/yr): `steps` time steps of the device driver
+    (rheology, scatter, Picard loop over nodes with up to 10 solves to 1e-8,
+    grid -> marker updates, RK4 with stress rotation, strain) for each solver
+    setting, from the same initial state.  Solve times include the set-up of
+    each new system."""
+    import numpy as np
+    import torch
+    from earthbox_gpu import learned, timeloop, workloads
+    tmpl, m, mats, par = workloads.rift2d_markers()
+    out = {"system": f"rift2d_markers 2001x801 nodes (500 x 160 km), {m['x'].size} markers, "
+                     f"{steps} time steps, <= {par['nglobal']} Picard iterations, solves to 1e-8",
+           "steps": steps}
+    net = learned.LearnedCorrection(ctx)
+    if learned.WEIGHTS_FILE.exists():
+        net.load()
+    legs = (("mixed_fp32_mg", dict(precision=1)), ("plain_fp64_mg", dict(precision=0)),
+            ("learned_mixed", dict(precision=1, learned=1)))
+    for name, kw in legs:
+        model = timeloop.upload(ctx, tmpl, m, mats, par)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        logs = []
+        for k in range(steps):
+            logs.append(timeloop.time_step(ctx, model, solve_opts=dict(rtol=1e-8, **kw)))
+            progress(f"config5 {name} step {k + 1}/{steps}: {logs[-1]['seconds']:.2f} s, "
+                     f"{logs[-1]['picard_iterations']} Picard, Krylov {logs[-1]['krylov_iterations']}")
+        el = time.perf_counter() - t0
+        solves = [t for lg in logs for t in lg["solve_seconds_with_setup"]]
+        its = [k for lg in logs for k in lg["krylov_iterations"]]
+        out[name] = {"seconds": el, "seconds_per_step": el / steps, "solves": len(solves),
+                     "solve_seconds_with_setup_total": float(np.sum(solves)),
+                     "solve_seconds_with_setup_mean": float(np.mean(solves)),
+                     "krylov_iterations_mean": float(np.mean(its)), "krylov_iterations_max": int(np.max(its)),
+                     "max_relres": max(lg["max_relres"] for lg in logs),
+                     "picard_iterations": [lg["picard_iterations"] for lg in logs],
+                     "learned_accepted": sum(lg["learned_accepted"] for lg in logs),
+                     "simulated_years": model.timesum / 3.15576e7}
+        del model
+        torch.cuda.empty_cache()
+    p, mx, lm = out["plain_fp64_mg"], out["mixed_fp32_mg"], out["learned_mixed"]
+    out["speedup_mixed_vs_plain"] = p["solve_seconds_with_setup_mean"] / mx["solve_seconds_with_setup_mean"]
+    out["speedup_learned_mixed_vs_plain"] = p["solve_seconds_with_setup_mean"] / lm["solve_seconds_with_setup_mean"]
+    return out
+
+
+def bench_rk4_config5(ctx, steps=10):
+    """RK4 gather-advect of 40M markers (5 x 5 per cell) on the rifting
+    model's 2001 x 801-node grid, both arithmetics; kernel time from HIP
+    events on the library's stream."""
+    import numpy as np
+    import torch
+    from earthbox_gpu import workloads
+    from earthbox_gpu.device import empty, to_device
+    from earthbox_gpu.grid import uniform_grid
+    g = uniform_grid(2001, 801, 5.0e5, 1.6e5)
+    x, y = workloads.marker_positions(g, 5, 5, 5)
+    vx1, vy1 = workloads.random_velocity_fields(g, 13, vmax=1e-9)
+    ctx.set_grid(g)
+    dvx1, dvy1 = to_device(vx1), to_device(vy1)
+    desp = empty((g.ynum, g.xnum), np.float64)
+    ctx.calculate_spin(dvx1, dvy1, desp)
+    dx, dy = to_device(x), to_device(y)
+    dt = 0.5 * g.xstpavg / 1e-9
+    n = x.size
+    out = {"markers": n, "grid": "2001x801 (500 x 160 km)"}
+    stream = torch.cuda.current_stream()
+    for name, mode in (("tolerance", ctx.RK4_TOLERANCE), ("exact", ctx.RK4_EXACT)):
+        ctx.set_rk4_arithmetic(mode)
+        ctx.advect_markers_and_rotate_stress_tensor(4, dt, None, dx, dy, None, None, dvx1, dvy1, desp)
+        torch.cuda.synchronize()
+        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s0.record(stream)
+        for _ in range(steps):
+            ctx.advect_markers_and_rotate_stress_tensor(4, dt, None, dx, dy, None, None, dvx1, dvy1, desp)
+        s1.record(stream)
+        torch.cuda.synchronize()
+        ms = s0.elapsed_time(s1) / steps
+        out[name] = {"kernel_ms": ms, "Mmarkers_per_s": n / (ms * 1e-3) / 1e6,
+                     "frac_of_8TBs": RK4_BYTES_PER_MARKER * n / (ms * 1e-3) / 8e12}
+    ctx.set_rk4_arithmetic(ctx.RK4_TOLERANCE)
+    del dx, dy
+    torch.cuda.empty_cache()
     return out
 
 
@@ -305,6 +454,7 @@ def bench_scatter(ctx, g, x, y, reps=3):
     from earthbox_gpu.device import to_device
     from earthbox_gpu.scatter import interpolate_marker_stokes_parameters_to_grid, new_grid_fields
     n = x.size
+    ctx.set_grid(g)
     rng = np.random.default_rng(5)
     m = {"x": to_device(x), "y": to_device(y),
          "rho": to_device(3000 + 300 * rng.random(n)),
@@ -611,21 +761,29 @@ def main():
                      "bytes_per_unit": RK4_BYTES_PER_MARKER},
         "rk4_exact": rk4_exact,
     }
-    if not args.no_marker_gathers:
-        out["marker_gathers"] = bench_marker_gathers(ctx, g, dx, dy)
-    if not args.no_stokes:
-        out["stokes"] = bench_stokes(ctx, nx, ny, args.stokes_max_iter)
-    if not args.no_scatter:
-        out["scatter"] = bench_scatter(ctx, g, x, y)
-    if not args.no_stokes3d:
-        out["stokes3d"] = bench_stokes3d(ctx, args.stokes3d_n, args.stokes3d_max_iter)
-    if not args.no_sinker:
-        out["sinker"] = bench_sinker(ctx)
-    if not args.no_picard:
-        out["picard"] = bench_picard(ctx)
-    if not args.no_grid_kernels:
-        out["grid_kernels"] = bench_grid_kernels(ctx)
+    progress(f"rk4 headline: {kern_ms:.3f} ms per launch ({kname})")
+    legs = [
+        ("marker_gathers", not args.no_marker_gathers, lambda: bench_marker_gathers(ctx, g, dx, dy)),
+        ("stokes", not args.no_stokes,
+         lambda: bench_stokes(ctx, nx, ny, args.stokes_max_iter, args.stokes_amplitude)),
+        ("config1", not args.no_config1, lambda: bench_config1(ctx)),
+        ("rk4_config5", not args.no_config5, lambda: bench_rk4_config5(ctx)),
+        ("config5", not args.no_config5, lambda: bench_config5(ctx, args.config5_steps)),
+        ("scatter", not args.no_scatter, lambda: bench_scatter(ctx, g, x, y)),
+        ("stokes3d", not args.no_stokes3d, lambda: bench_stokes3d(ctx, args.stokes3d_n, args.stokes3d_max_iter)),
+        ("sinker", not args.no_sinker, lambda: bench_sinker(ctx)),
+        ("picard", not args.no_picard, lambda: bench_picard(ctx)),
+        ("grid_kernels", not args.no_grid_kernels, lambda: bench_grid_kernels(ctx)),
+    ]
+    for name, on, fn in legs:
+        if not on:
+            continue
+        t0 = time.perf_counter()
+        progress(f"{name} ...")
+        out[name] = fn()
+        progress(f"{name} done in {time.perf_counter() - t0:.1f} s")
     if not args.no_cpu_baseline:
+        progress("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(g, x, y, vx1, vy1, to_host_np(desp), dt, 2_000_000)
         if not args.no_sinker:
             out["cpu_baseline"]["mg3d_97_reference_driver"] = cpu_sinker_driver()
