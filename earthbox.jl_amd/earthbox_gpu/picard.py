@@ -58,8 +58,9 @@ def picard_loop_nodes(ctx, template: StokesProblem, state: dict, plastic_params:
     StokesContinuitySolver.jl:223-226, Update.jl:43-61) before the stress
     forecast and the yielding, and the new step carries into the next
     iteration's right-hand side.
-    Returns (solution dict with vx1, vy1, pr1, exx, exy, eii, sxy2, sxx2,
-    dsxy, dsxx and the final timestep; per-iteration log)."""
+    Returns (solution dict with vx1, vy1, pr1, the last solution vector S,
+    exx, exy, eii, sxy2, sxx2, dsxy, dsxx and the final timestep;
+    per-iteration log)."""
     g = template.grid
     solver = solver or StokesSolverGPU(ctx)
     opts = dict(precision=1, rtol=1e-10, max_iterations=4000, bc_d=bc_mirror_factors(template))
@@ -111,7 +112,7 @@ def picard_loop_nodes(ctx, template: StokesProblem, state: dict, plastic_params:
                     "nnode_yield": nnode_yield, "global_yield_error": global_yield_error,
                     "dvxy_rel_L2": dvxy_rel_L2, "timestep": dt, "solve_seconds_with_setup": solve_seconds,
                     "learned_accepted": st["learned_accepted"]})
-        sol = {"vx1": vx1, "vy1": vy1, "pr1": pr1, "timestep": dt, **ss}
+        sol = {"vx1": vx1, "vy1": vy1, "pr1": pr1, "S": S, "timestep": dt, **ss}
         if check_stop_global_loop_nodes(nnode_yield, iglobal, nglobal, global_yield_error, dvxy_rel_L2,
                                         tolerance):
             break
