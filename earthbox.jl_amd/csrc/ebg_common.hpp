@@ -83,6 +83,8 @@ struct ebg_ctx {
     void* mg3_ref = nullptr;   // reference-order 3-D multigrid (ebg_mg3d_create)
     void* stokes3d = nullptr;  // 3-D Stokes solve state (ebg_stokes3d_set_system)
     void* plastic = nullptr;   // reduction scratch of ebg_update_nodes_for_plastic_yielding
+    void* heat = nullptr;      // heat-equation solve state (ebg_heat.hip)
+    void* subgrid = nullptr;   // per-marker buffers of the sub-grid diffusion (ebg_subgrid.hip)
     int rk4_arithmetic = EBG_RK4_EXACT;   // ebg_rk4_set_arithmetic
 };
 
@@ -90,4 +92,6 @@ namespace ebg {
 int grid_release(Grid2D& g);
 void plastic_release(ebg_ctx* ctx);
 int plastic_scratch_setup(int device);   // ebg_plastic.hip
+void heat_release(ebg_ctx* ctx);         // ebg_heat.hip
+void subgrid_release(ebg_ctx* ctx);      // ebg_subgrid.hip
 }

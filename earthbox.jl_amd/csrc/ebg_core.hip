@@ -145,6 +145,8 @@ int ebg_ctx_destroy(ebg_ctx* ctx)
     stokes3d_release(ctx);
     cnn_release_ctx(ctx);
     plastic_release(ctx);
+    heat_release(ctx);
+    subgrid_release(ctx);
     grid_release(ctx->grid);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;

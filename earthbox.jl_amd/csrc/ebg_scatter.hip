@@ -72,7 +72,11 @@ struct AxisD {
 // stored).  Values are stored as the reference's numerator loops use them:
 // the friction angle in degrees and the reciprocal shear modulus
 // (BilinearNumerator.jl:296-388).
-enum { F_RHO, F_COH, F_FRIC, F_EFLOW, F_ETA, F_INVMU, F_SXY, F_PL, F_DIL, F_MELT, F_SXX, NFIELD };
+enum { F_RHO, F_COH, F_FRIC, F_EFLOW, F_ETA, F_INVMU, F_SXY, F_PL, F_DIL, F_MELT, F_SXX, NSTOKES };
+// heat-transport slots (BilinearNumerator.jl:31-141): temperature,
+// conductivity, rho*Cp, heat production (material value plus the optional
+// serpentinization term) and the adiabatic term
+enum { F_TK = NSTOKES, F_KT, F_RHOCP, F_HR, F_HA, NFIELD };
 
 struct Layout {
     int nslot;
@@ -87,8 +91,12 @@ struct ScatterArgs {
     double xmin, xmax, ymin, ymax;
     AxisD ax, ay, avx;  // basic x, basic y, vy-grid x
     const double *xstp_b, *ystp_b, *gridx_vy, *xstp_vy;
+    const int8_t* inside_flags;
+    int32_t *out_xn, *out_yn;
     ebg_marker_stokes_props p;
     ebg_stokes_grid_fields f;
+    ebg_marker_heat_props h;
+    ebg_heat_grid_fields hf;
     const uint32_t *keys_s, *vals_s;
     const int *cstart, *cend;
     const int* rowbase;
@@ -103,7 +111,7 @@ struct ScatterArgs {
 
 __device__ __forceinline__ bool inside_marker(const ScatterArgs& a, int64_t m, double xm, double ym)
 {
-    if (a.p.inside_flags) return a.p.inside_flags[m] == 1;
+    if (a.inside_flags) return a.inside_flags[m] == 1;
     return a.xmin <= xm && xm <= a.xmax && a.ymin <= ym && ym <= a.ymax;
 }
 
@@ -117,8 +125,8 @@ __global__ void scatter_keys_kernel(ScatterArgs a)
         const int xn = left_node_index(xm, a.ax.g, a.ax.n, a.ax.x0, a.ax.inv_h, a.ax.uniform);
         const int yn = left_node_index(ym, a.ay.g, a.ay.n, a.ay.x0, a.ay.inv_h, a.ay.uniform);
         key = (uint32_t)((xn - 1) * (a.ynum - 1) + (yn - 1));
-        if (a.f.marker_xn) a.f.marker_xn[m] = xn;
-        if (a.f.marker_yn) a.f.marker_yn[m] = yn;
+        if (a.out_xn) a.out_xn[m] = xn;
+        if (a.out_yn) a.out_yn[m] = yn;
     }
     a.keys[m] = key;
     a.vals[m] = (uint32_t)m;
@@ -186,6 +194,7 @@ __global__ void __launch_bounds__(1024) scan_rows_kernel(int* __restrict__ rows,
 // 64 adjacent words.
 constexpr int EBATCH = 4;
 
+template <bool HEAT>
 __global__ void __launch_bounds__(256) scatter_entries_kernel(ScatterArgs a)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -213,8 +222,26 @@ __global__ void __launch_bounds__(256) scatter_entries_kernel(ScatterArgs a)
             ym[u] = Y[m[u]];
         }
         double pv[NFIELD][EBATCH];
+        if (HEAT) {
 #pragma unroll
-        for (int u = 0; u < EBATCH; ++u) {
+            for (int u = 0; u < EBATCH; ++u) {
+                const uint32_t mu_ = m[u];
+                pv[F_TK][u] = o[F_TK] >= 0 ? a.h.TK[mu_] : 0.0;
+                pv[F_KT][u] = o[F_KT] >= 0 ? a.h.kt[mu_] : 0.0;
+                pv[F_RHOCP][u] = o[F_RHOCP] >= 0 ? a.h.rhocp[mu_] : 0.0;
+                pv[F_HA][u] = o[F_HA] >= 0 ? a.h.ha[mu_] : 0.0;
+                double hr = 0.0;
+                if (o[F_HR] >= 0) {
+                    const int id = a.h.matid[mu_];
+                    hr = (id >= 1 && id <= a.h.nmat) ? a.h.mat_hr[id - 1] : __builtin_nan("");
+                    if (a.h.serpentinization_heat_production)
+                        hr = hr + (double)a.h.serpentinization_heat_production[mu_];
+                }
+                pv[F_HR][u] = hr;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < EBATCH && !HEAT; ++u) {
             const uint32_t mu_ = m[u];
             pv[F_RHO][u] = o[F_RHO] >= 0 ? a.p.rho[mu_] : 0.0;
             pv[F_COH][u] = o[F_COH] >= 0 ? a.p.cohesion[mu_] : 0.0;
@@ -236,6 +263,12 @@ __global__ void __launch_bounds__(256) scatter_entries_kernel(ScatterArgs a)
             double* v = a.eval + e;
             v[0] = (xm[u] - gx) / hx;
             v[S] = (ym[u] - gy) / hy;
+            if (HEAT) {  // basic-grid nodes only: no vy-grid column
+#pragma unroll
+                for (int q = F_TK; q < NFIELD; ++q)
+                    if (o[q] >= 0) v[o[q] * S] = pv[q][u];
+                continue;
+            }
             const int xnv = left_node_index(xm[u], a.avx.g, a.avx.n, a.avx.x0, a.avx.inv_h, a.avx.uniform);
             a.exnv[e] = xnv;
             v[2 * S] = (xm[u] - a.gridx_vy[xnv - 1]) / a.xstp_vy[xnv - 1];
@@ -443,6 +476,48 @@ __global__ void __launch_bounds__(256) scatter_vy_kernel(ScatterArgs a)
     if (a.f.rho1_vy) a.f.rho1_vy[t] = (w_s != 0) ? s_rho / w_s : a.f.rho1_vy[t];
 }
 
+// Basic nodes, heat transport properties: the inclusive rule only
+// (BilinearNumerator.jl:31-141, BilinearAverage.jl:10-50).
+__global__ void __launch_bounds__(256) scatter_heat_kernel(ScatterArgs a)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ny = a.ynum, nx = a.xnum;
+    if (t >= ny * nx) return;
+    const int i = t % ny + 1, j = t / ny + 1;
+    const int ci[4] = {i, i - 1, i, i - 1}, cj[4] = {j, j, j - 1, j - 1};
+    int cells[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        cells[q] = (ci[q] >= 1 && ci[q] <= ny - 1 && cj[q] >= 1 && cj[q] <= nx - 1)
+                       ? (cj[q] - 1) * (ny - 1) + (ci[q] - 1)
+                       : -1;
+    Merge<4> mg;
+    mg.init(a, cells);
+    const int* o = a.L.off;
+    const int64_t S = a.slot_stride;
+    double w_in = 0.0, s_tk = 0.0, s_kt = 0.0, s_rc = 0.0, s_hr = 0.0, s_ha = 0.0;
+    int64_t e;
+    int q;
+    while (mg.next(a, &e, &q)) {
+        const double* v = a.eval + e;
+        const double dx = v[0], dy = v[S];
+        const double w = ((q & 2) ? dx : (1.0 - dx)) * ((q & 1) ? dy : (1.0 - dy));
+        w_in += w * 1.0 * 1.0;
+        if (o[F_TK] >= 0) s_tk += w * v[o[F_TK] * S] * 1.0;
+        if (o[F_KT] >= 0) s_kt += w * v[o[F_KT] * S] * 1.0;
+        if (o[F_RHOCP] >= 0) s_rc += w * v[o[F_RHOCP] * S] * 1.0;
+        if (o[F_HR] >= 0) s_hr += w * v[o[F_HR] * S] * 1.0;
+        if (o[F_HA] >= 0) s_ha += w * v[o[F_HA] * S] * 1.0;
+    }
+    if (a.wtnodes) a.wtnodes[t] = w_in;
+    auto ar = [&](double* out, double s) { if (out) out[t] = (w_in != 0) ? s / w_in : out[t]; };
+    ar(a.hf.tk1, s_tk);
+    ar(a.hf.kt1, s_kt);
+    ar(a.hf.rhocp1, s_rc);
+    ar(a.hf.hr1, s_hr);
+    ar(a.hf.ha1, s_ha);
+}
+
 static int axis_uniform_h(const std::vector<double>& g, double* x0, double* inv_h)
 {
     const int64_t n = (int64_t)g.size();
@@ -472,31 +547,24 @@ int ebg_scatter_release(ebg_ctx* ctx)
     return EBG_OK;
 }
 
-int ebg_interpolate_marker_stokes_parameters_to_grid(ebg_ctx* ctx, const ebg_marker_stokes_props* props,
-                                                     ebg_stokes_grid_fields* fields)
+}  // extern "C"
+
+namespace ebg {
+namespace {
+
+// Binning shared by the Stokes and heat scatters: cell keys of the inside
+// markers, the stable radix sort, per-cell ranges, the sliced row layout and
+// the entry pass that gathers each marker's distances and the values of the
+// requested slots.  On return a is ready for the node kernels.
+int scatter_bin(ebg_ctx* ctx, int64_t n, const double* x, const double* y, const bool (&want)[NFIELD],
+                bool heat, ScatterArgs& a)
 {
-    EBG_CHECK(ctx && ctx->has_grid, EBG_ERR_STATE, "ebg_interpolate_marker_stokes_parameters_to_grid: no grid");
-    EBG_CHECK(props && fields, EBG_ERR_ARG, "ebg_interpolate_marker_stokes_parameters_to_grid: NULL arguments");
     const Grid2D& g = ctx->grid;
-    const int64_t n = props->marknum;
-    EBG_CHECK(n >= 0 && n < (int64_t)0xffffffffLL, EBG_ERR_ARG, "scatter: marknum out of range");
-    EBG_CHECK(n == 0 || (props->x && props->y), EBG_ERR_ARG, "scatter: NULL marker coordinates");
-    EBG_CHECK((props->mat_mu == nullptr) == (props->matid == nullptr), EBG_ERR_ARG,
-              "scatter: matid and mat_mu must be given together");
     const int nx = (int)g.xnum, ny = (int)g.ynum;
     const int ncell = (nx - 1) * (ny - 1);
     if (!ctx->scatter) ctx->scatter = new ScatterState();
     ScatterState& S = *(ScatterState*)ctx->scatter;
     hipStream_t s = ctx->stream;
-    if (props->mat_mu) {
-        // check_shear_modulus (BilinearNumerator.jl:155-160): zero modulus is an error
-        EBG_CHECK(props->nmat >= 1, EBG_ERR_ARG, "scatter: nmat must be >= 1");
-        std::vector<double> mu((size_t)props->nmat);
-        EBG_HIP(hipMemcpyAsync(mu.data(), props->mat_mu, mu.size() * 8, hipMemcpyDeviceToHost, s));
-        EBG_HIP(hipStreamSynchronize(s));
-        for (size_t k = 0; k < mu.size(); ++k)
-            EBG_CHECK(mu[k] != 0.0, EBG_ERR_ARG, "scatter: shear modulus of a material is zero");
-    }
     if (S.cap < n || S.ncell != ncell) {
         ScatterState fresh;
         std::swap(S.keys, fresh.keys);
@@ -519,28 +587,12 @@ int ebg_interpolate_marker_stokes_parameters_to_grid(ebg_ctx* ctx, const ebg_mar
         S.cap = (int64_t)cap;
         S.ncell = ncell;
     }
-    const int64_t nb = (int64_t)nx * ny, np = (int64_t)ncell, nvy = (int64_t)ny * (nx + 1);
-    double *wtn = fields->wtnodes, *wts = fields->wtetas, *wte = fields->wtetan, *wtv = fields->wtnodes_vy;
-    if (!wtn || !wts || !wte || !wtv) {
-        const int64_t need = 2 * nb + np + nvy;
-        if (S.wbuf_n < need) {
-            if (S.wbuf) (void)hipFree(S.wbuf);
-            S.wbuf = nullptr;
-            EBG_HIP(hipMalloc(&S.wbuf, need * 8));
-            S.wbuf_n = need;
-        }
-        if (!wtn) wtn = S.wbuf;
-        if (!wts) wts = S.wbuf + nb;
-        if (!wte) wte = S.wbuf + 2 * nb;
-        if (!wtv) wtv = S.wbuf + 2 * nb + np;
-    }
-    ScatterArgs a{};
     a.n = n;
     a.xnum = nx;
     a.ynum = ny;
     a.ncell = ncell;
-    a.x = props->x;
-    a.y = props->y;
+    a.x = x;
+    a.y = y;
     a.xmin = g.xmin;
     a.xmax = g.xmax;
     a.ymin = g.ymin;
@@ -554,37 +606,15 @@ int ebg_interpolate_marker_stokes_parameters_to_grid(ebg_ctx* ctx, const ebg_mar
     a.ystp_b = g.ystp_b;
     a.gridx_vy = g.gridx_vy;
     a.xstp_vy = g.xstp_vy;
-    a.p = *props;
-    a.f = *fields;
     a.keys = S.keys;
     a.vals = S.vals;
     a.keys_s = S.keys_s;
     a.vals_s = S.vals_s;
     a.cstart = S.cstart;
     a.cend = S.cend;
-    a.wtnodes = wtn;
-    a.wtetas = wts;
-    a.wtetan = wte;
-    a.wtnodes_vy = wtv;
-    // entry slots: the marker values some requested output consumes
-    {
-        const ebg_stokes_grid_fields& f = *fields;
-        const bool want[NFIELD] = {
-            props->rho && (f.rho1 || f.rho1_vy),
-            props->cohesion && f.cohesion_grid,
-            props->fric && f.fric_degrees_grid,
-            props->eta_flow && f.eta_flow,
-            props->eta && (f.etas1 || f.etan1),
-            f.mus1 || f.mun1,  // 1/mu is 1 without a material table
-            props->sxy && f.sxy1,
-            props->pfailure && (f.plastics || f.plasticn),
-            props->dilatation && f.dilatation_grid,
-            props->meltfrac && f.extractable_meltfrac_grid,
-            props->sxx && f.sxx1};
-        int slot = 3;
-        for (int q = 0; q < NFIELD; ++q) a.L.off[q] = want[q] ? slot++ : -1;
-        a.L.nslot = slot;
-    }
+    int slot = 3;  // slots 0..2: dx, dy, vy-grid dx
+    for (int q = 0; q < NFIELD; ++q) a.L.off[q] = want[q] ? slot++ : -1;
+    a.L.nslot = slot;
     const int ngroup = (ncell + CGROUP - 1) / CGROUP;
     hipLaunchKernelGGL(fill_int_kernel, dim3(nbk(ncell)), dim3(256), 0, s, S.cstart, ncell, 0);
     hipLaunchKernelGGL(fill_int_kernel, dim3(nbk(ncell)), dim3(256), 0, s, S.cend, ncell, 0);
@@ -636,11 +666,127 @@ int ebg_interpolate_marker_stokes_parameters_to_grid(ebg_ctx* ctx, const ebg_mar
     a.exnv = S.exnv;
     a.eval = S.eval;
     a.slot_stride = nent;
-    if (nent > 0)
-        hipLaunchKernelGGL(scatter_entries_kernel, dim3(nbk((int64_t)ngroup * CGROUP)), dim3(256), 0, s, a);
+    if (nent > 0) {
+        if (heat)
+            hipLaunchKernelGGL(scatter_entries_kernel<true>, dim3(nbk((int64_t)ngroup * CGROUP)), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL(scatter_entries_kernel<false>, dim3(nbk((int64_t)ngroup * CGROUP)), dim3(256), 0, s,
+                               a);
+    }
+    EBG_HIP(hipGetLastError());
+    return EBG_OK;
+}
+
+}  // namespace
+}  // namespace ebg
+
+extern "C" {
+
+int ebg_interpolate_marker_stokes_parameters_to_grid(ebg_ctx* ctx, const ebg_marker_stokes_props* props,
+                                                     ebg_stokes_grid_fields* fields)
+{
+    EBG_CHECK(ctx && ctx->has_grid, EBG_ERR_STATE, "ebg_interpolate_marker_stokes_parameters_to_grid: no grid");
+    EBG_CHECK(props && fields, EBG_ERR_ARG, "ebg_interpolate_marker_stokes_parameters_to_grid: NULL arguments");
+    const Grid2D& g = ctx->grid;
+    const int64_t n = props->marknum;
+    EBG_CHECK(n >= 0 && n < (int64_t)0xffffffffLL, EBG_ERR_ARG, "scatter: marknum out of range");
+    EBG_CHECK(n == 0 || (props->x && props->y), EBG_ERR_ARG, "scatter: NULL marker coordinates");
+    EBG_CHECK((props->mat_mu == nullptr) == (props->matid == nullptr), EBG_ERR_ARG,
+              "scatter: matid and mat_mu must be given together");
+    const int nx = (int)g.xnum, ny = (int)g.ynum;
+    const int ncell = (nx - 1) * (ny - 1);
+    hipStream_t s = ctx->stream;
+    if (props->mat_mu) {
+        // check_shear_modulus (BilinearNumerator.jl:155-160): zero modulus is an error
+        EBG_CHECK(props->nmat >= 1, EBG_ERR_ARG, "scatter: nmat must be >= 1");
+        std::vector<double> mu((size_t)props->nmat);
+        EBG_HIP(hipMemcpyAsync(mu.data(), props->mat_mu, mu.size() * 8, hipMemcpyDeviceToHost, s));
+        EBG_HIP(hipStreamSynchronize(s));
+        for (size_t k = 0; k < mu.size(); ++k)
+            EBG_CHECK(mu[k] != 0.0, EBG_ERR_ARG, "scatter: shear modulus of a material is zero");
+    }
+    if (!ctx->scatter) ctx->scatter = new ScatterState();
+    ScatterState& S = *(ScatterState*)ctx->scatter;
+    const int64_t nb = (int64_t)nx * ny, np = (int64_t)ncell, nvy = (int64_t)ny * (nx + 1);
+    double *wtn = fields->wtnodes, *wts = fields->wtetas, *wte = fields->wtetan, *wtv = fields->wtnodes_vy;
+    if (!wtn || !wts || !wte || !wtv) {
+        const int64_t need = 2 * nb + np + nvy;
+        if (S.wbuf_n < need) {
+            if (S.wbuf) (void)hipFree(S.wbuf);
+            S.wbuf = nullptr;
+            EBG_HIP(hipMalloc(&S.wbuf, need * 8));
+            S.wbuf_n = need;
+        }
+        if (!wtn) wtn = S.wbuf;
+        if (!wts) wts = S.wbuf + nb;
+        if (!wte) wte = S.wbuf + 2 * nb;
+        if (!wtv) wtv = S.wbuf + 2 * nb + np;
+    }
+    ScatterArgs a{};
+    a.inside_flags = props->inside_flags;
+    a.out_xn = fields->marker_xn;
+    a.out_yn = fields->marker_yn;
+    a.p = *props;
+    a.f = *fields;
+    a.wtnodes = wtn;
+    a.wtetas = wts;
+    a.wtetan = wte;
+    a.wtnodes_vy = wtv;
+    // entry slots: the marker values some requested output consumes
+    const ebg_stokes_grid_fields& f = *fields;
+    bool want[NFIELD] = {
+        props->rho && (f.rho1 || f.rho1_vy),
+        props->cohesion && f.cohesion_grid,
+        props->fric && f.fric_degrees_grid,
+        props->eta_flow && f.eta_flow,
+        props->eta && (f.etas1 || f.etan1),
+        f.mus1 || f.mun1,  // 1/mu is 1 without a material table
+        props->sxy && f.sxy1,
+        props->pfailure && (f.plastics || f.plasticn),
+        props->dilatation && f.dilatation_grid,
+        props->meltfrac && f.extractable_meltfrac_grid,
+        props->sxx && f.sxx1};
+    int rc = scatter_bin(ctx, n, props->x, props->y, want, false, a);
+    if (rc) return rc;
     hipLaunchKernelGGL(scatter_basic_kernel, dim3(nbk(nb)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(scatter_pressure_kernel, dim3(nbk(np)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(scatter_vy_kernel, dim3(nbk(nvy)), dim3(256), 0, s, a);
+    EBG_HIP(hipGetLastError());
+    return EBG_OK;
+}
+
+int ebg_interpolate_marker_heat_parameters_to_grid(ebg_ctx* ctx, const ebg_marker_heat_props* props,
+                                                   ebg_heat_grid_fields* fields)
+{
+    EBG_CHECK(ctx && ctx->has_grid, EBG_ERR_STATE, "ebg_interpolate_marker_heat_parameters_to_grid: no grid");
+    EBG_CHECK(props && fields, EBG_ERR_ARG, "ebg_interpolate_marker_heat_parameters_to_grid: NULL arguments");
+    const int64_t n = props->marknum;
+    EBG_CHECK(n >= 0 && n < (int64_t)0xffffffffLL, EBG_ERR_ARG, "heat scatter: marknum out of range");
+    EBG_CHECK(n == 0 || (props->x && props->y), EBG_ERR_ARG, "heat scatter: NULL marker coordinates");
+    const ebg_heat_grid_fields& f = *fields;
+    EBG_CHECK(!f.tk1 || props->TK, EBG_ERR_ARG, "heat scatter: tk1 needs TK");
+    EBG_CHECK(!f.kt1 || props->kt, EBG_ERR_ARG, "heat scatter: kt1 needs kt");
+    EBG_CHECK(!f.rhocp1 || props->rhocp, EBG_ERR_ARG, "heat scatter: rhocp1 needs rhocp");
+    EBG_CHECK(!f.ha1 || props->ha, EBG_ERR_ARG, "heat scatter: ha1 needs ha");
+    EBG_CHECK(!f.hr1 || (props->matid && props->mat_hr && props->nmat >= 1), EBG_ERR_ARG,
+              "heat scatter: hr1 needs matid, mat_hr and nmat >= 1");
+    const Grid2D& g = ctx->grid;
+    ScatterArgs a{};
+    a.inside_flags = props->inside_flags;
+    a.out_xn = fields->marker_xn;
+    a.out_yn = fields->marker_yn;
+    a.h = *props;
+    a.hf = *fields;
+    a.wtnodes = fields->wtnodes;
+    bool want[NFIELD] = {};
+    want[F_TK] = f.tk1 != nullptr;
+    want[F_KT] = f.kt1 != nullptr;
+    want[F_RHOCP] = f.rhocp1 != nullptr;
+    want[F_HR] = f.hr1 != nullptr;
+    want[F_HA] = f.ha1 != nullptr;
+    int rc = scatter_bin(ctx, n, props->x, props->y, want, true, a);
+    if (rc) return rc;
+    hipLaunchKernelGGL(scatter_heat_kernel, dim3(nbk(g.xnum * g.ynum)), dim3(256), 0, ctx->stream, a);
     EBG_HIP(hipGetLastError());
     return EBG_OK;
 }

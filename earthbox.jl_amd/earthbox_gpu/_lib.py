@@ -104,6 +104,18 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_stokes3d_apply": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_stokes3d_precondition": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "ebg_stokes3d_solve": (c_int, [c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p]),
+    "ebg_interpolate_marker_heat_parameters_to_grid": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ebg_apply_thermal_bc": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ebg_heat_rhs": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ebg_heat_system_apply": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_heat_solve": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_heat_residuals": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_run_conductive_heat_solver_loop": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_grid_to_marker_temperature": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "ebg_apply_sticky_temperature_correction": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_double,
+                                                        c_void_p]),
+    "ebg_subgrid_heat_diffusion_update": (c_int, [c_void_p] * 6),
+    "ebg_update_subgrid_stress": (c_int, [c_void_p, c_void_p, c_void_p]),
 }
 
 
@@ -177,6 +189,87 @@ class MarkerRheologyIn(C.Structure):
 class MarkerRheologyOut(C.Structure):
     _fields_ = [("cohesion", c_void_p), ("fric", c_void_p), ("eta_flow", c_void_p), ("preexp", c_void_p),
                 ("eta", c_void_p)]
+
+
+class MarkerHeatProps(C.Structure):
+    """ebg_marker_heat_props (include/earthbox_gpu.h)."""
+    _fields_ = [("marknum", c_int64), ("x", c_void_p), ("y", c_void_p), ("inside_flags", c_void_p),
+                ("TK", c_void_p), ("kt", c_void_p), ("rhocp", c_void_p), ("ha", c_void_p), ("matid", c_void_p),
+                ("mat_hr", c_void_p), ("nmat", c_int64), ("serpentinization_heat_production", c_void_p)]
+
+
+class HeatGridFields(C.Structure):
+    """ebg_heat_grid_fields."""
+    _fields_ = [("tk1", c_void_p), ("kt1", c_void_p), ("rhocp1", c_void_p), ("hr1", c_void_p), ("ha1", c_void_p),
+                ("wtnodes", c_void_p), ("marker_xn", c_void_p), ("marker_yn", c_void_p)]
+
+
+class ThermalBC(C.Structure):
+    """ebg_thermal_bc."""
+    _fields_ = [("btopt", c_void_p), ("bbottomt", c_void_p), ("bleftt", c_void_p), ("brightt", c_void_p)]
+
+
+class HeatRhsIn(C.Structure):
+    """ebg_heat_rhs_in."""
+    _fields_ = [(k, c_void_p) for k in ("hr1", "ha1", "rho1", "vx1", "vy1", "sxy2", "etas1", "sxx2", "etan1")] + [
+        ("gravity_x", c_double), ("gravity_y", c_double), ("iuse_adiabatic_heating", C.c_int32),
+        ("iuse_shear_heating", C.c_int32)]
+
+
+class HeatSystem(C.Structure):
+    """ebg_heat_system."""
+    _fields_ = [("RT1", c_void_p), ("tk0", c_void_p), ("rhocp1", c_void_p), ("kt1", c_void_p), ("bc", ThermalBC),
+                ("timestep", c_double)]
+
+
+class HeatSolveOpts(C.Structure):
+    """ebg_heat_solve_opts (zeros select the defaults)."""
+    _fields_ = [("restart", C.c_int32), ("max_iterations", C.c_int32), ("rtol", c_double),
+                ("pre_sweeps", C.c_int32), ("post_sweeps", C.c_int32)]
+
+
+class HeatLoopIn(C.Structure):
+    """ebg_heat_loop_in."""
+    _fields_ = [("RT1", c_void_p), ("rhocp1", c_void_p), ("kt1", c_void_p), ("bc", ThermalBC),
+                ("timestep", c_double), ("max_temp_change", c_double), ("max_substeps", C.c_int32),
+                ("tk1", c_void_p)]
+
+
+class HeatLoopOut(C.Structure):
+    """ebg_heat_loop_out."""
+    _fields_ = [("tk0", c_void_p), ("tk2", c_void_p), ("rest", c_void_p), ("dtk1", c_void_p)]
+
+
+class HeatLoopStats(C.Structure):
+    """ebg_heat_loop_stats."""
+    _fields_ = [("substeps", C.c_int32), ("solves", C.c_int32), ("iterations", C.c_int32),
+                ("not_converged", C.c_int32), ("last_relres", c_double), ("timestep_last", c_double),
+                ("timestep_sum", c_double)]
+
+
+class MarkerCells(C.Structure):
+    """ebg_marker_cells."""
+    _fields_ = [("marknum", c_int64), ("x", c_void_p), ("y", c_void_p), ("inside_flags", c_void_p),
+                ("marker_xn", c_void_p), ("marker_yn", c_void_p)]
+
+
+class SubgridHeatIn(C.Structure):
+    """ebg_subgrid_heat_in."""
+    _fields_ = [("markers", MarkerCells), ("tk1", c_void_p), ("kt1", c_void_p), ("rhocp1", c_void_p),
+                ("timestep", c_double), ("subgrid_diff_coef_temp", c_double)]
+
+
+class SubgridStressIn(C.Structure):
+    """ebg_subgrid_stress_in."""
+    _fields_ = [("markers", MarkerCells), ("matid", c_void_p), ("mat_mu", c_void_p), ("nmat", c_int64),
+                ("eta", c_void_p), ("sxy1", c_void_p), ("sxx1", c_void_p), ("timestep", c_double),
+                ("subgrid_diff_coef_stress", c_double)]
+
+
+class SubgridStressOut(C.Structure):
+    """ebg_subgrid_stress_out."""
+    _fields_ = [(k, c_void_p) for k in ("marker_sxy", "marker_sxx", "dsxy", "dsxx", "dsxyn", "dsxxn", "wtetas",
+                                        "wtetan")]
 
 
 class MarkerStokesProps(C.Structure):

@@ -17,9 +17,9 @@ import ctypes as C
 import numpy as np
 import torch
 
-from ._lib import MarkerGatherIn, MarkerGatherOut, check, lib
+from ._lib import MarkerGatherIn, MarkerGatherOut, SubgridStressIn, SubgridStressOut, check, lib
 from .context import GPUContext
-from .device import ptr
+from .device import empty, ptr
 
 GRID_PRESSURE = ("exx", "pr1", "sxx2", "dsxx")
 GRID_BASIC = ("exy", "sxy2", "dsxy")
@@ -108,3 +108,62 @@ def apply_marker_stress_correction_for_remaining_grid_stress(ctx: GPUContext, ma
 def calculate_marker_pressure(ctx: GPUContext, markers: dict, fields: dict, out: dict | None = None) -> dict:
     """MarkerPressure.calculate_marker_pressure! (MarkerPressure.jl:17-49)."""
     return update_markers_after_stokes(ctx, markers, fields, 0.0, out, ("marker_pr",))
+
+
+def update_subgrid_stress(ctx: GPUContext, markers: dict, fields: dict, timestep: float,
+                          subgrid_diff_coef_stress: float, out: dict) -> dict:
+    """SubGridStress.Update.update_subgrid_stress! (subgrid_stress/update/Update.jl:55-71).
+    markers: device x, y, marker_xn, marker_yn, optional inside_flags, eta,
+    matid (int16), mat_mu; fields: sxy1 (ny, nx) and sxx1 (ny-1, nx-1), the
+    pre-solve grid stresses, and dsxy, dsxx, updated in place to the remaining
+    changes.  out: device marker_sxy, marker_sxx (updated in place); dsxyn,
+    dsxxn, wtetas, wtetan are created when missing and returned in out."""
+    g = ctx.grid
+    if g is None:
+        raise RuntimeError("set_grid() must be called first")
+    nx, ny = g.xnum, g.ynum
+    n = int(markers["x"].numel())
+    a = SubgridStressIn()
+    m = a.markers
+    m.marknum = n
+    m.x = ptr(markers["x"], np.float64, n)
+    m.y = ptr(markers["y"], np.float64, n)
+    m.inside_flags = ptr(markers.get("inside_flags"), np.int8, n)
+    m.marker_xn = ptr(markers["marker_xn"], np.int32, n)
+    m.marker_yn = ptr(markers["marker_yn"], np.int32, n)
+    a.matid = ptr(markers["matid"], np.int16, n)
+    a.mat_mu = ptr(markers["mat_mu"], np.float64)
+    a.nmat = int(markers["mat_mu"].numel())
+    a.eta = ptr(markers["eta"], np.float64, n)
+    a.sxy1 = ptr(fields["sxy1"], np.float64, nx * ny)
+    a.sxx1 = ptr(fields["sxx1"], np.float64, (nx - 1) * (ny - 1))
+    a.timestep = float(timestep)
+    a.subgrid_diff_coef_stress = float(subgrid_diff_coef_stress)
+    dev = markers["x"].device
+    for k, shape in (("dsxyn", (ny, nx)), ("dsxxn", (ny - 1, nx - 1)), ("wtetas", (ny, nx)),
+                     ("wtetan", (ny - 1, nx - 1))):
+        if k not in out:
+            out[k] = empty(shape, np.float64, dev)
+    o = SubgridStressOut()
+    o.marker_sxy = ptr(out["marker_sxy"], np.float64, n)
+    o.marker_sxx = ptr(out["marker_sxx"], np.float64, n)
+    o.dsxy = ptr(fields["dsxy"], np.float64, nx * ny)
+    o.dsxx = ptr(fields["dsxx"], np.float64, (nx - 1) * (ny - 1))
+    o.dsxyn = ptr(out["dsxyn"], np.float64, nx * ny)
+    o.dsxxn = ptr(out["dsxxn"], np.float64, (nx - 1) * (ny - 1))
+    o.wtetas = ptr(out["wtetas"], np.float64, nx * ny)
+    o.wtetan = ptr(out["wtetan"], np.float64, (nx - 1) * (ny - 1))
+    check(lib.ebg_update_subgrid_stress(ctx.handle, C.byref(a), C.byref(o)), "ebg_update_subgrid_stress")
+    return out
+
+
+def update_marker_stress_for_grid_and_subgrid_changes(ctx: GPUContext, markers: dict, fields: dict,
+                                                      timestep: float, subgrid_diff_coef_stress: float,
+                                                      out: dict) -> dict:
+    """StokesContinuitySolver.update_marker_stress_for_grid_and_subgrid_changes!
+    (StokesContinuitySolver.jl:365-381): the sub-grid stress update when its
+    coefficient is positive, then the marker stress correction for the
+    remaining grid stress change."""
+    if subgrid_diff_coef_stress > 0:
+        update_subgrid_stress(ctx, markers, fields, timestep, subgrid_diff_coef_stress, out)
+    return update_markers_after_stokes(ctx, markers, fields, 0.0, out, STRESS)

@@ -170,6 +170,48 @@ int ebg_interpolate_marker_stokes_parameters_to_grid(ebg_ctx *ctx,
                                                      const ebg_marker_stokes_props *props,
                                                      ebg_stokes_grid_fields *fields);
 
+/* ---- marker -> grid heat-transport properties ----
+ * DEVICE marker arrays (length marknum): x, y; inside_flags optional (NULL:
+ * computed from the grid limits); TK, kt, rhocp, ha; matid (1-based) with the
+ * material table mat_hr[nmat] for the heat production, plus
+ * serpentinization_heat_production (Float32, optional: added when given,
+ * BilinearNumerator.jl:107-113).  A property may be NULL when its output is. */
+typedef struct {
+    int64_t marknum;
+    const double *x, *y;
+    const int8_t *inside_flags;
+    const double *TK, *kt, *rhocp, *ha;
+    const int16_t *matid;
+    const double *mat_hr;
+    int64_t nmat;
+    const float *serpentinization_heat_production;
+} ebg_marker_heat_props;
+
+/* DEVICE basic-grid arrays (ynum,xnum), in/out: on entry the backups kept where
+ * a node receives no marker weight, on exit the averages.  The reference backs
+ * up tk1 from the previous heat solution tk2 (BackupTransportArrays.jl:154-163),
+ * so tk1 must hold tk2's values on entry; kt1, rhocp1, hr1 and ha1 hold their
+ * previous values.  Any may be NULL (not computed).  wtnodes (optional out):
+ * the inclusive weight sums; marker_xn/yn (optional out): cell indices. */
+typedef struct {
+    double *tk1, *kt1, *rhocp1, *hr1, *ha1;
+    double *wtnodes;
+    int32_t *marker_xn, *marker_yn;
+} ebg_heat_grid_fields;
+
+/* Replaces MarkerToHeatArrays.interpolate_markers_to_transport_arrays!
+ * without its last step (interpolation/utils/MarkerToHeatArrays.jl:29-35:
+ * BackupTransportArrays.backup_and_clear_heat_transport_arrays!,
+ * BilinearNumerator.calc_marker_weight_sums_for_basic_grid_heat!
+ * (BilinearNumerator.jl:31-141) and BilinearAverage.calc_marker_average_at_basic_nodes_heat!
+ * (BilinearAverage.jl:10-50)); the thermal boundary conditions follow with
+ * ebg_apply_thermal_bc.  Node sums in serial marker order, bit-identical to
+ * the serial loops.  Used with only tk1 requested it is also the scatter of a
+ * per-marker quantity with the inclusive rule (the sub-grid temperature
+ * change, SubgridDiffusion.jl:129-176). */
+int ebg_interpolate_marker_heat_parameters_to_grid(ebg_ctx *ctx, const ebg_marker_heat_props *props,
+                                                   ebg_heat_grid_fields *fields);
+
 /* ---- viscoelastic terms and right-hand sides (K7) ----
  * DEVICE grid arrays with the reference shapes.  Inputs: etas1, mus1, sxy1,
  * rho1 (ynum,xnum); etan1, mun1, sxx1 (ynum-1,xnum-1); rho1_vy (ynum,xnum+1),
@@ -735,6 +777,173 @@ int ebg_stokes3d_precondition(ebg_ctx *ctx, const double *r, double *z, const eb
  * nodes set. */
 int ebg_stokes3d_solve(ebg_ctx *ctx, const double *rhs, double prnorm, double *x,
                        const ebg_solve_opts *opts, ebg_solve_stats *stats);
+
+/* ---- heat equation (basic grid) ----
+ * Thermal boundary conditions, DEVICE arrays in the reference's (n, 2)
+ * column-major layout (bcs.jl:8-27): btopt, bbottomt (xnum, 2),
+ * bleftt, brightt (ynum, 2); column 1 the value, column 2 the factor of the
+ * inner neighbour: tk[1,j] = btopt[j,1] + btopt[j,2]*tk[2,j]. */
+typedef struct {
+    const double *btopt, *bbottomt, *bleftt, *brightt;
+} ebg_thermal_bc;
+
+/* Replaces ApplyThermalBC.apply_thermal_bc_along_boundaries!
+ * (interpolation/utils/heat_funcs/ApplyThermalBC.jl:20-44): the top and
+ * bottom rows of the inner columns, then both side columns over every row,
+ * on tk1 (ynum,xnum) in place.  Bit-identical. */
+int ebg_apply_thermal_bc(ebg_ctx *ctx, const ebg_thermal_bc *bc, double *tk1);
+
+/* Inputs of the heat right-hand side, DEVICE arrays: hr1, ha1, rho1, sxy2,
+ * etas1 (ynum,xnum); vx1 (ynum+1,xnum); vy1 (ynum,xnum+1); sxx2, etan1
+ * (ynum-1,xnum-1).  Only hr1 is needed with both options off. */
+typedef struct {
+    const double *hr1, *ha1, *rho1, *vx1, *vy1, *sxy2, *etas1, *sxx2, *etan1;
+    double gravity_x, gravity_y;
+    int32_t iuse_adiabatic_heating, iuse_shear_heating;
+} ebg_heat_rhs_in;
+
+/* Replaces HeatSolver.calculate_heat_rhs_terms_grid! (heat_solver/HeatSolver.jl:107-118;
+ * HeatRhs.initialize_rhs_terms_array_for_heat!, add_adiabatic_terms_to_rhs_grid!,
+ * add_friction_terms_to_rhs_grid!, rhs/HeatRhs.jl:5-79): RT1 (ynum,xnum).
+ * Bit-identical, the squares evaluated as products. */
+int ebg_heat_rhs(ebg_ctx *ctx, const ebg_heat_rhs_in *in, double *RT1);
+
+/* One heat system (HeatBuildStructs.HeatBuildData, build_system/utils/BuildStructs.jl:70-96):
+ * DEVICE RT1, tk0 (old temperature), rhocp1, kt1 (ynum,xnum), the boundary
+ * conditions and the heat-loop time step (s). */
+typedef struct {
+    const double *RT1, *tk0, *rhocp1, *kt1;
+    ebg_thermal_bc bc;
+    double timestep;
+} ebg_heat_system;
+
+/* The rows of HeatBuildManager.build_system_of_equations
+ * (build_system/HeatBuildManager.jl:47-83, InternalNodes.jl:19-197,
+ * BoundaryNodes.jl:16-251) without assembling them: LT = L*T (optional) and
+ * the right-hand side R (optional), both (ynum,xnum) in the reference's
+ * unknown order itk = (j-1)*ynum + i.  Each row's triplets are accumulated
+ * from 0.0 in the order the reference emits them, so L*T equals the COO
+ * product bit for bit. */
+int ebg_heat_system_apply(ebg_ctx *ctx, const ebg_heat_system *sys, const double *T, double *LT, double *R);
+
+typedef struct {
+    int32_t restart;        /* FGMRES restart length, default 40 (at most 254) */
+    int32_t max_iterations; /* default 400 */
+    double rtol;            /* ||R - L*T|| / ||R|| target, default 1e-12 */
+    int32_t pre_sweeps, post_sweeps;  /* red-black Gauss-Seidel sweeps per level, default 2 and 2 */
+} ebg_heat_solve_opts;
+
+/* Replaces SystemSolver.discretize_and_solve_conductive_heat_equation
+ * (heat_solver/solver/SystemSolver.jl:32-66): build, parallel_direct_solver
+ * (the MUMPS seam) and reload_solu_temp.  tk2 (ynum,xnum) receives the
+ * solution of the system above by flexible GMRES (FP64) preconditioned with a
+ * geometric multigrid V-cycle, to ||R - L*T||/||R|| <= rtol.  opts and stats
+ * may be NULL.  Returns EBG_OK, EBG_NOT_CONVERGED (tk2 holds the last
+ * iterate) or an error. */
+int ebg_heat_solve(ebg_ctx *ctx, const ebg_heat_system *sys, double *tk2, const ebg_heat_solve_opts *opts,
+                   ebg_solve_stats *stats);
+
+/* Replaces HeatResiduals.compute_residuals_temp! (solver/residuals/HeatResiduals.jl:12-162):
+ * rest (ynum,xnum), 0 on the boundary.  Bit-identical. */
+int ebg_heat_residuals(ebg_ctx *ctx, const ebg_heat_system *sys, const double *tk2, double *rest);
+
+typedef struct {
+    const double *RT1, *rhocp1, *kt1;
+    ebg_thermal_bc bc;
+    double timestep;          /* model time step (s) */
+    double max_temp_change;   /* K */
+    int32_t max_substeps;     /* guard against a loop that cannot finish, 0: 100000 */
+    const double *tk1;        /* optional: the marker-interpolated temperature, for dtk1 */
+} ebg_heat_loop_in;
+
+typedef struct {
+    double *tk0;   /* in: the starting temperature (tk1), out: the final solution */
+    double *tk2;   /* out: the final solution */
+    double *rest;  /* out: residuals of the last solve */
+    double *dtk1;  /* optional out (needs tk1): tk2 - tk1, HeatSolver.calculate_grid_temp_change!
+                      (HeatSolver.jl:158-163) */
+} ebg_heat_loop_out;
+
+typedef struct {
+    int32_t substeps, solves, iterations, not_converged;
+    double last_relres, timestep_last, timestep_sum;
+} ebg_heat_loop_stats;
+
+/* Replaces HeatLoop.run_conductive_heat_solver_loop! (heat_solver/heat_loop/HeatLoop.jl:12-78):
+ * solves with the whole time step, and when the largest temperature change
+ * exceeds max_temp_change solves again with the step scaled by
+ * max_temp_change / change; repeats until the sub-steps add up to the time
+ * step, each starting from the previous solution.  Returns EBG_OK,
+ * EBG_NOT_CONVERGED (some solve missed rtol) or an error. */
+int ebg_run_conductive_heat_solver_loop(ebg_ctx *ctx, const ebg_heat_loop_in *in, ebg_heat_loop_out *out,
+                                        const ebg_heat_solve_opts *opts, ebg_heat_loop_stats *stats);
+
+/* ---- marker temperature and sub-grid diffusion ----
+ * DEVICE marker arrays (length marknum): x, y, the stored 1-based upper-left
+ * basic cells marker_xn/yn (Int32) and inside_flags (optional: NULL computes
+ * them from the grid limits). */
+typedef struct {
+    int64_t marknum;
+    const double *x, *y;
+    const int8_t *inside_flags;
+    const int32_t *marker_xn, *marker_yn;
+} ebg_marker_cells;
+
+/* mode 0 replaces GridTemperatureToMarkers.interpolate!
+ * (interpolation/utils/GridTemperatureToMarkers.jl:15-35): marker_TK = grid at
+ * the marker; mode 1 replaces MarkerTemperature.add_remaining_grid_change_to_marker_temperature!
+ * (heat_solver/marker_temperature/MarkerTemperature.jl:18-45): marker_TK +=
+ * grid at the marker (grid = dtk1).  Inside markers only; bit-identical. */
+int ebg_grid_to_marker_temperature(ebg_ctx *ctx, const ebg_marker_cells *m, const double *grid, int mode,
+                                   double *marker_TK);
+
+/* Replaces MarkerTemperature.apply_sticky_temperature_correction!
+ * (MarkerTemperature.jl:52-72): inside markers of the sticky-air or
+ * sticky-water material take temperature_top (an id of -1 matches nothing). */
+int ebg_apply_sticky_temperature_correction(ebg_ctx *ctx, const ebg_marker_cells *m, const int16_t *matid,
+                                            int matid_sticky_air, int matid_sticky_water, double temperature_top,
+                                            double *marker_TK);
+
+typedef struct {
+    ebg_marker_cells markers;
+    const double *tk1, *kt1, *rhocp1;   /* (ynum,xnum) */
+    double timestep, subgrid_diff_coef_temp;
+} ebg_subgrid_heat_in;
+
+/* Replaces SubgridDiffusion.subgrid_heat_diffusion_update! followed by
+ * remove_subgrid_effects_from_grid_changes! (heat_solver/subgrid_heat/SubgridDiffusion.jl:52-185):
+ * marker_TK += dT_sg (inside markers), dtkn = the inclusive average of dT_sg
+ * (0 where no weight), dtk1 -= dtkn; wtnodes (optional out) receives the
+ * weight sums.  Bit-identical except exp, from the device math library.
+ * The per-marker dT_sg buffer is the library's own (one per context). */
+int ebg_subgrid_heat_diffusion_update(ebg_ctx *ctx, const ebg_subgrid_heat_in *in, double *marker_TK,
+                                      double *dtkn, double *dtk1, double *wtnodes);
+
+typedef struct {
+    ebg_marker_cells markers;
+    const int16_t *matid;
+    const double *mat_mu;      /* [nmat], indexed by matid - 1 */
+    int64_t nmat;
+    const double *eta;         /* marker viscoplastic viscosity */
+    const double *sxy1;        /* (ynum,xnum) pre-solve grid stresses */
+    const double *sxx1;        /* (ynum-1,xnum-1) */
+    double timestep, subgrid_diff_coef_stress;
+} ebg_subgrid_stress_in;
+
+typedef struct {
+    double *marker_sxy, *marker_sxx;   /* in/out */
+    double *dsxy, *dsxx;               /* in: grid stress changes; out: the remaining changes */
+    double *dsxyn, *dsxxn;             /* out: sub-grid changes on the grid, 0 where no weight */
+    double *wtetas, *wtetan;           /* optional out: weight sums */
+} ebg_subgrid_stress_out;
+
+/* Replaces SubGridStress.Update.update_subgrid_stress!
+ * (stokes_continuity_solver/subgrid_stress/update/Update.jl:55-71;
+ * SubgridShearStress.jl:25-96, SubgridNormalStress.jl:24-89,
+ * DiffusionTerm.jl:13-29, RemainingStress.jl:16-23), which runs before the
+ * marker stress correction when subgrid_diff_coef_stress > 0
+ * (StokesContinuitySolver.jl:365-381).  Bit-identical except exp. */
+int ebg_update_subgrid_stress(ebg_ctx *ctx, const ebg_subgrid_stress_in *in, ebg_subgrid_stress_out *out);
 
 #ifdef __cplusplus
 }

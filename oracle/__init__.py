@@ -458,12 +458,18 @@ def scatter_stokes(g, m: dict, prev: dict):
              "fric_deg": "fric_degrees_grid", "dilatation": "dilatation_grid",
              "meltfrac": "extractable_meltfrac_grid", "wtnodes": "wtnodes", "wtetas": "wtetas",
              "wtetan": "wtetan", "wtnodes_vy": "wtnodes_vy"}
-    out = {}
-    for ok, gk in names.items():
-        a = prev.get(gk)
-        out[gk] = np.asfortranarray(a.copy()) if a is not None else None
     ny, nx = g.ynum, g.xnum
-    shapes = {"wtnodes": (ny, nx), "wtetas": (ny, nx), "wtetan": (ny - 1, nx - 1), "wtnodes_vy": (ny, nx + 1)}
+    basic, pres, vyg = (ny, nx), (ny - 1, nx - 1), (ny, nx + 1)
+    shape = {"rho1": basic, "rho1_vy": vyg, "etas1": basic, "etan1": pres, "eta_flow": basic, "mus1": basic,
+             "mun1": pres, "sxy1": basic, "sxx1": pres, "plastics": basic, "plasticn": pres,
+             "cohesion_grid": basic, "fric_degrees_grid": basic, "dilatation_grid": pres,
+             "extractable_meltfrac_grid": pres}
+    out = {}
+    for ok, gk in names.items():   # fields without a previous value start (and back up) at zero
+        a = prev.get(gk)
+        if gk in shape:
+            out[gk] = np.asfortranarray(a.copy()) if a is not None else np.zeros(shape[gk], order="F")
+    shapes = {"wtnodes": basic, "wtetas": basic, "wtetan": pres, "wtnodes_vy": vyg}
     for k, s in shapes.items():
         out[k] = np.zeros(s, order="F")
     xn = np.zeros(n, np.int32)
@@ -613,3 +619,281 @@ def update_marker_strain(m: dict, timestep: float, healing: float):
                                p(f("exx")), p(f("exy")), p(f("sxx")), p(f("sxy")), p(f("eta")), p(f("eta_flow")),
                                p(np.ascontiguousarray(m["pfailure"], dtype=np.float32)), p(gii), p(ps), p(psr))
     return gii, ps, psr
+
+
+# ---------------------------------------------------------------- heat path
+class _HeatGrid(C.Structure):
+    _fields_ = [("xnum", _I64), ("ynum", _I64)] + [(k, _P) for k in (
+        "gridx_b", "gridy_b", "xstp_b", "ystp_b", "xstp_pr", "ystp_pr", "gridx_vy", "xstp_vy", "gridy_vx",
+        "ystp_vx")]
+
+
+_HEAT_SIGS = {
+    "ebo_scatter_basic_inclusive": (None, [C.c_void_p, _I64, _P, _P, _P, _P, _P, C.c_int, _P, _P, _P]),
+    "ebo_marker_heat_production": (None, [_I64, _P, _P, _P, _P, _P]),
+    "ebo_apply_thermal_bc": (None, [_I64, _I64, _P, _P, _P, _P, _P]),
+    "ebo_heat_rhs": (None, [_I64, _I64] + [_P] * 9 + [_D, _D, C.c_int, C.c_int, _P]),
+    "ebo_heat_build": (_I64, [C.c_void_p, _P, _P, _P, _P, _P, _P, _P, _P, _D, _P, _P, _P, _P]),
+    "ebo_heat_residuals": (None, [C.c_void_p, _P, _P, _P, _P, _P, _D, _P]),
+    "ebo_grid_to_marker_temperature": (None, [C.c_void_p, _I64, _P, _P, _P, _P, _P, _P, C.c_int, _P]),
+    "ebo_subgrid_temperature_markers": (None, [C.c_void_p, _I64] + [_P] * 8 + [_D] * 4 + [_P, _P]),
+    "ebo_sticky_temperature_correction": (None, [_I64, _P, _P, C.c_int16, C.c_int16, _D, _P]),
+    "ebo_subgrid_stress_markers": (None, [C.c_void_p, _I64] + [_P] * 10 + [_D, _D] + [_P] * 4),
+}
+
+
+def _heat_lib():
+    L = lib()
+    for name, (res, args) in _HEAT_SIGS.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _fm(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+class _HG:
+    """Heat grid struct plus the arrays it points at (kept alive)."""
+
+    def __init__(self, g):
+        self.keep = {k: _f64(getattr(g, k)) for k in ("gridx_b", "gridy_b", "xstp_b", "ystp_b", "xstp_pr",
+                                                      "ystp_pr", "gridx_vy", "xstp_vy", "gridy_vx", "ystp_vx")}
+        self.s = _HeatGrid(g.xnum, g.ynum, *[p(self.keep[k]) for k in (
+            "gridx_b", "gridy_b", "xstp_b", "ystp_b", "xstp_pr", "ystp_pr", "gridx_vy", "xstp_vy", "gridy_vx",
+            "ystp_vx")])
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def _marker_cells(g, m):
+    flags = np.ascontiguousarray(m["flags"], dtype=np.int8)
+    if "marker_xn" in m:
+        xn = np.ascontiguousarray(m["marker_xn"], dtype=np.int32)
+        yn = np.ascontiguousarray(m["marker_yn"], dtype=np.int32)
+    else:
+        xn, yn = cell_indices_basic(m["x"], m["y"], flags, g)
+    return flags, xn, yn
+
+
+def scatter_basic_inclusive(g, m: dict, vals: list, backups: list):
+    """Inclusive basic-grid scatter of per-marker values (BilinearNumerator.jl:31-141
+    with the averages of BilinearAverage.jl:276-294).  backups: the values kept
+    where a node receives no weight.  Returns (averages, wtnodes)."""
+    L = _heat_lib()
+    hg = _HG(g)
+    flags, xn, yn = _marker_cells(g, m)
+    x, y = _f64(m["x"]), _f64(m["y"])
+    vals = [_f64(v) for v in vals]
+    outs = [np.array(b, dtype=np.float64, order="F", copy=True) for b in backups]
+    wt = np.zeros((g.ynum, g.xnum), order="F")
+    vp = (C.c_void_p * max(1, len(vals)))(*[p(v) for v in vals])
+    op = (C.c_void_p * max(1, len(outs)))(*[p(o) for o in outs])
+    L.ebo_scatter_basic_inclusive(hg.ref, x.size, p(x), p(y), p(flags), p(xn), p(yn), len(vals),
+                                  C.cast(vp, _P), C.cast(op, _P), p(wt))
+    return outs, wt
+
+
+HEAT_FIELDS = ("tk1", "kt1", "rhocp1", "hr1", "ha1")
+
+
+def marker_heat_production(m: dict, mat_hr, serp=None):
+    """Per-marker heat production (BilinearNumerator.jl:107-113)."""
+    L = _heat_lib()
+    flags = np.ascontiguousarray(m["flags"], dtype=np.int8)
+    n = flags.size
+    hr = np.empty(n)
+    L.ebo_marker_heat_production(n, p(flags), p(np.ascontiguousarray(m["matid"], dtype=np.int16)), p(_f64(mat_hr)),
+                                 p(None if serp is None else np.ascontiguousarray(serp, dtype=np.float32)), p(hr))
+    return hr
+
+
+def scatter_heat(g, m: dict, prev: dict, mat_hr, serp=None, bc: dict | None = None):
+    """MarkerToHeatArrays.interpolate_markers_to_transport_arrays!
+    (MarkerToHeatArrays.jl:29-35): backups (tk0 from tk2, the others from the
+    previous suffix-1 arrays, BackupTransportArrays.jl:154-163), the inclusive
+    scatter of TK, kt, rhocp, heat production and ha, the averages, then the
+    thermal boundary conditions on tk1 when bc is given.  prev keys: tk2,
+    kt1, rhocp1, hr1, ha1."""
+    hr = marker_heat_production(m, mat_hr, serp)
+    vals = [m["TK"], m["kt"], m["rhocp"], hr, m["ha"]]
+    backs = [prev["tk2"], prev["kt1"], prev["rhocp1"], prev["hr1"], prev["ha1"]]
+    outs, wt = scatter_basic_inclusive(g, m, vals, backs)
+    out = dict(zip(HEAT_FIELDS, outs))
+    out["wtnodes"] = wt
+    if bc is not None:
+        out["tk1"] = apply_thermal_bc(g, bc, out["tk1"])
+    return out
+
+
+def apply_thermal_bc(g, bc: dict, tk1):
+    L = _heat_lib()
+    t = np.array(tk1, dtype=np.float64, order="F", copy=True)
+    L.ebo_apply_thermal_bc(g.xnum, g.ynum, p(_fm(bc["btopt"])), p(_fm(bc["bbottomt"])), p(_fm(bc["bleftt"])),
+                           p(_fm(bc["brightt"])), p(t))
+    return t
+
+
+def heat_rhs(g, f: dict, gravity_x, gravity_y, iuse_adiabatic, iuse_shear):
+    """HeatSolver.calculate_heat_rhs_terms_grid! (HeatSolver.jl:107-118)."""
+    L = _heat_lib()
+    RT1 = np.empty((g.ynum, g.xnum), order="F")
+    z = lambda k, s: _fm(f[k]) if k in f else np.zeros(s, order="F")
+    ny, nx = g.ynum, g.xnum
+    L.ebo_heat_rhs(nx, ny, p(_fm(f["hr1"])), p(z("ha1", (ny, nx))), p(z("rho1", (ny, nx))),
+                   p(z("vx1", (ny + 1, nx))), p(z("vy1", (ny, nx + 1))), p(z("sxy2", (ny, nx))),
+                   p(z("etas1", (ny, nx))), p(z("sxx2", (ny - 1, nx - 1))), p(z("etan1", (ny - 1, nx - 1))),
+                   gravity_x, gravity_y, int(iuse_adiabatic), int(iuse_shear), p(RT1))
+    return RT1
+
+
+def heat_build(g, bc: dict, RT1, tk0, rhocp1, kt1, timestep):
+    """HeatBuildManager.build_system_of_equations: (N, Li, Lj, Lv, R), 1-based."""
+    L = _heat_lib()
+    hg = _HG(g)
+    N = g.xnum * g.ynum
+    Li = np.empty(5 * N, np.int64)
+    Lj = np.empty(5 * N, np.int64)
+    Lv = np.empty(5 * N)
+    R = np.empty(N)
+    nz = L.ebo_heat_build(hg.ref, p(_fm(bc["btopt"])), p(_fm(bc["bbottomt"])), p(_fm(bc["bleftt"])),
+                          p(_fm(bc["brightt"])), p(_fm(RT1)), p(_fm(tk0)), p(_fm(rhocp1)), p(_fm(kt1)),
+                          timestep, p(Li), p(Lj), p(Lv), p(R))
+    return N, Li[:nz].copy(), Lj[:nz].copy(), Lv[:nz].copy(), R
+
+
+def heat_solve(g, bc: dict, RT1, tk0, rhocp1, kt1, timestep):
+    """discretize_and_solve_conductive_heat_equation (SystemSolver.jl:32-66):
+    tk2 from the direct solve of the restated system (reload_solu_temp keeps
+    the column-major node order)."""
+    N, Li, Lj, Lv, R = heat_build(g, bc, RT1, tk0, rhocp1, kt1, timestep)
+    S = direct_solve(N, Li, Lj, Lv, R)
+    return np.asfortranarray(S.reshape((g.ynum, g.xnum), order="F"))
+
+
+def heat_residuals(g, RT1, rhocp1, tk0, tk2, kt1, timestep):
+    L = _heat_lib()
+    hg = _HG(g)
+    rest = np.empty((g.ynum, g.xnum), order="F")
+    L.ebo_heat_residuals(hg.ref, p(_fm(RT1)), p(_fm(rhocp1)), p(_fm(tk0)), p(_fm(tk2)), p(_fm(kt1)), timestep,
+                         p(rest))
+    return rest
+
+
+def heat_loop(g, bc: dict, RT1, tk1, rhocp1, kt1, timestep, max_temp_change):
+    """HeatLoop.run_conductive_heat_solver_loop! (heat_loop/HeatLoop.jl:12-78)
+    starting from tk0 = tk1 (HeatSolver.initialize_old_temperature_array!).
+    Returns (tk2, rest, log of (timestep_heat, dtkmax, resolved))."""
+    tk0 = np.array(tk1, order="F", copy=True)
+    dt_loop = timestep
+    total = 0.0
+    log = []
+    tk2 = rest = None
+    while total < timestep:
+        tk2 = heat_solve(g, bc, RT1, tk0, rhocp1, kt1, dt_loop)
+        rest = heat_residuals(g, RT1, rhocp1, tk0, tk2, kt1, dt_loop)
+        dtkmax = float(np.max(np.abs(tk2 - tk0)))
+        resolved = dtkmax > max_temp_change
+        if resolved:
+            dt_loop = dt_loop * max_temp_change / dtkmax
+            tk2 = heat_solve(g, bc, RT1, tk0, rhocp1, kt1, dt_loop)
+            rest = heat_residuals(g, RT1, rhocp1, tk0, tk2, kt1, dt_loop)
+        log.append((dt_loop, dtkmax, resolved))
+        total = total + dt_loop
+        dt_loop = min(dt_loop, timestep - total)
+        tk0 = tk2.copy(order="F")
+    return tk2, rest, log
+
+
+def grid_to_marker_temperature(g, m: dict, grid, mode: int, marker_TK):
+    """mode 0: GridTemperatureToMarkers.interpolate!; mode 1: add the grid
+    change (MarkerTemperature.add_remaining_grid_change_to_marker_temperature!)."""
+    L = _heat_lib()
+    hg = _HG(g)
+    flags, xn, yn = _marker_cells(g, m)
+    t = np.array(marker_TK, dtype=np.float64, copy=True)
+    L.ebo_grid_to_marker_temperature(hg.ref, t.size, p(_f64(m["x"])), p(_f64(m["y"])), p(flags), p(xn), p(yn),
+                                     p(_fm(grid)), int(mode), p(t))
+    return t
+
+
+def subgrid_temperature_markers(g, m: dict, tk1, kt1, rhocp1, timestep, coef):
+    L = _heat_lib()
+    hg = _HG(g)
+    flags, xn, yn = _marker_cells(g, m)
+    t = np.array(m["TK"], dtype=np.float64, copy=True)
+    dT = np.empty(t.size)
+    L.ebo_subgrid_temperature_markers(hg.ref, t.size, p(_f64(m["x"])), p(_f64(m["y"])), p(flags), p(xn), p(yn),
+                                      p(_fm(tk1)), p(_fm(kt1)), p(_fm(rhocp1)), timestep, coef, g.xstpavg,
+                                      g.ystpavg, p(t), p(dT))
+    return t, dT
+
+
+def sticky_temperature_correction(m: dict, matid_air, matid_water, temperature_top):
+    L = _heat_lib()
+    t = np.array(m["TK"], dtype=np.float64, copy=True)
+    L.ebo_sticky_temperature_correction(t.size, p(np.ascontiguousarray(m["flags"], dtype=np.int8)),
+                                        p(np.ascontiguousarray(m["matid"], dtype=np.int16)), int(matid_air),
+                                        int(matid_water), temperature_top, p(t))
+    return t
+
+
+def update_marker_temp_for_grid_and_subgrid_changes(g, bc: dict, m: dict, f: dict, timestep, max_temp_change,
+                                                    subgrid_coef, gravity_x=0.0, gravity_y=0.0,
+                                                    iuse_adiabatic=0, iuse_shear=0):
+    """HeatSolver.update_marker_temp_for_grid_and_subgrid_changes! (HeatSolver.jl:77-101):
+    f holds tk1, kt1, rhocp1, hr1 (plus ha1, rho1, vx1, vy1, sxy2, etas1, sxx2,
+    etan1 for the optional right-side terms).  Returns a dict with marker TK,
+    RT1, tk2, rest, dtk1 (remaining change), dtkn and the heat-loop log."""
+    RT1 = heat_rhs(g, f, gravity_x, gravity_y, iuse_adiabatic, iuse_shear)
+    tk2, rest, log = heat_loop(g, bc, RT1, f["tk1"], f["rhocp1"], f["kt1"], timestep, max_temp_change)
+    dtk1 = np.asfortranarray(tk2 - f["tk1"])
+    TK = np.array(m["TK"], dtype=np.float64, copy=True)
+    dtkn = None
+    if subgrid_coef > 0:
+        TK, dT = subgrid_temperature_markers(g, m, f["tk1"], f["kt1"], f["rhocp1"], timestep, subgrid_coef)
+        (dtkn,), _ = scatter_basic_inclusive(g, m, [dT], [np.zeros((g.ynum, g.xnum), order="F")])
+        dtk1 = np.asfortranarray(dtk1 - dtkn)
+    TK = grid_to_marker_temperature(g, m, dtk1, 1, TK)
+    return {"TK": TK, "RT1": RT1, "tk2": tk2, "rest": rest, "dtk1": dtk1, "dtkn": dtkn, "log": log}
+
+
+def subgrid_stress_markers(g, m: dict, sxy1, sxx1, timestep, coef):
+    L = _heat_lib()
+    hg = _HG(g)
+    flags, xn, yn = _marker_cells(g, m)
+    sxy = np.array(m["sxy"], dtype=np.float64, copy=True)
+    sxx = np.array(m["sxx"], dtype=np.float64, copy=True)
+    n = sxy.size
+    dxy, dxx = np.empty(n), np.empty(n)
+    L.ebo_subgrid_stress_markers(hg.ref, n, p(_f64(m["x"])), p(_f64(m["y"])), p(flags), p(xn), p(yn),
+                                 p(np.ascontiguousarray(m["matid"], dtype=np.int16)), p(_f64(m["mat_mu"])),
+                                 p(_f64(m["eta"])), p(_fm(sxy1)), p(_fm(sxx1)), timestep, coef, p(sxy), p(sxx),
+                                 p(dxy), p(dxx))
+    return sxy, sxx, dxy, dxx
+
+
+def update_subgrid_stress(g, m: dict, f: dict, timestep, coef):
+    """SubGridStress.Update.update_subgrid_stress! (subgrid_stress/update/Update.jl:55-71):
+    relaxed marker stresses, their exclusive (shear, basic grid) and central
+    (normal, pressure grid) scatter into dsxyn, dsxxn with zero where no weight
+    (BilinearAverage.jl:234-251), and the remaining changes dsxy - dsxyn,
+    dsxx - dsxxn (RemainingStress.jl:16-23)."""
+    sxy, sxx, dxy, dxx = subgrid_stress_markers(g, m, f["sxy1"], f["sxx1"], timestep, coef)
+    ny, nx = g.ynum, g.xnum
+    sm = {"x": m["x"], "y": m["y"], "inside_flags": np.ascontiguousarray(m["flags"], dtype=np.int8),
+          "sxy": dxy, "sxx": dxx}
+    prev = {"sxy1": np.zeros((ny, nx), order="F"), "sxx1": np.zeros((ny - 1, nx - 1), order="F")}
+    sc = scatter_stokes(g, sm, prev)
+    dsxyn, dsxxn = sc["sxy1"], sc["sxx1"]
+    return {"sxy": sxy, "sxx": sxx, "dsxyn": dsxyn, "dsxxn": dsxxn,
+            "dsxy": np.asfortranarray(f["dsxy"] - dsxyn), "dsxx": np.asfortranarray(f["dsxx"] - dsxxn),
+            "wtetas": sc["wtetas"], "wtetan": sc["wtetan"]}
