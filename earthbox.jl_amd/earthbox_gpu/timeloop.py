@@ -7,9 +7,16 @@ advection with stress rotation and the marker strain update.  Everything stays
 on the device between steps; only scalars (solver statistics, yield counts,
 the time step) come back to the host.
 
-Not in the loop (outside the accelerated path): the heat solver and the
-marker temperature update (marker temperatures stay as given), melting,
-surface processes, output and the transient boundary conditions.
+With thermal inputs (upload's thermal argument) the step also runs the heat
+path: the heat-field scatter with the thermal boundary conditions before the
+Stokes scatter (TimeLoop.jl:219-221), HeatSolver.update_marker_temp_for_grid_and_subgrid_changes!
+after the marker stress update (TimeLoop.jl:250-257) and the sticky
+temperature correction after advection (TimeLoop.jl:268-269).  Without them
+marker temperatures stay as given.
+
+Not in the loop (outside the accelerated path): the rock-property updates
+(conductivity, rho*Cp and the adiabatic term stay at their uploaded values),
+melting, surface processes, output and the transient boundary conditions.
 """
 from __future__ import annotations
 
@@ -19,7 +26,7 @@ import time
 import numpy as np
 import torch
 
-from . import rheology
+from . import heat, rheology
 from .device import empty, to_device
 from .marker_gather import OUTPUTS as GATHER_OUTPUTS, STRESS as GATHER_STRESS, update_markers_after_stokes
 from .picard import picard_loop_nodes
@@ -46,14 +53,24 @@ class DeviceModel:
     ntimestep: int = 0
     velocity_old: tuple | None = None
     solver: StokesSolverGPU | None = None
+    thermal_bc: dict | None = None   # device thermal boundary conditions; None: no heat path
 
 
-def upload(ctx, template: StokesProblem, markers: dict, mats: dict, params: dict) -> DeviceModel:
+def upload(ctx, template: StokesProblem, markers: dict, mats: dict, params: dict,
+           thermal: tuple | None = None) -> DeviceModel:
     """Copy a model (numpy arrays, as workloads.rift2d_markers returns it) to
-    the device and allocate the grid fields and marker work arrays."""
+    the device and allocate the grid fields and marker work arrays.  thermal:
+    (marker arrays kt, rhocp, ha; mats with mat_hr; thermal boundary
+    conditions; heat scalars), as workloads.rift2d_thermal returns them, turns
+    the heat path on."""
     g = template.grid
     ctx.set_grid(g)
     n = markers["x"].size
+    if thermal is not None:
+        therm, tmats, tbc, hpar = thermal
+        markers = {**markers, **therm}
+        mats = {**mats, **tmats}
+        params = {**params, **hpar}
     dm = {k: to_device(v) for k, v in markers.items()}
     dev = dm["x"].device
     dm["inside_flags"] = torch.empty(n, dtype=torch.int8, device=dev)
@@ -66,8 +83,16 @@ def upload(ctx, template: StokesProblem, markers: dict, mats: dict, params: dict
                    ("eii_plastic_pressure", (ny - 1, nx - 1)), ("esp", (ny, nx))):
         grid[k] = empty(shp, np.float64)
         grid[k].zero_()
+    bc = None
+    if thermal is not None:
+        # tk2 is the backup of tk1 (the previous heat solution); zero before
+        # the first step, where every node receives marker weight
+        for k in ("tk1", "tk2", "kt1", "rhocp1", "hr1", "ha1"):
+            grid[k] = empty((ny, nx), np.float64)
+            grid[k].zero_()
+        bc = {k: to_device(v) for k, v in tbc.items()}
     return DeviceModel(template=template, markers=dm, mats={k: to_device(v) for k, v in mats.items()},
-                       params=dict(params), grid=grid, timestep=float(params["timestep"]))
+                       params=dict(params), grid=grid, timestep=float(params["timestep"]), thermal_bc=bc)
 
 
 def time_step(ctx, model: DeviceModel, solve_opts: dict | None = None, rk4_arithmetic: int | None = None) -> dict:
@@ -82,6 +107,11 @@ def time_step(ctx, model: DeviceModel, solve_opts: dict | None = None, rk4_arith
     ctx.get_marker_inside_flags(m["x"], m["y"], out=m["inside_flags"])
     step = model.ntimestep + 1   # the reference's time-step counter of this step
     rpar = dict(p, timestep=model.timestep, ntimestep=step)
+    heat_on = model.thermal_bc is not None
+    if heat_on:   # Interpolation.interpolate_marker_heat_parameters_to_grid! (TimeLoop.jl:220)
+        gf["tk1"].copy_(gf["tk2"])
+        heat.interpolate_marker_heat_parameters_to_grid(
+            ctx, m, model.mats, {k: gf[k] for k in heat.HEAT_FIELDS}, model.thermal_bc)
     rheology.update_marker_rheology(ctx, m, model.mats, rpar, m)
     interpolate_marker_stokes_parameters_to_grid(ctx, {**m, "dilatation": None, "meltfrac": None},
                                                  {**gf, "marker_xn": m["marker_xn"], "marker_yn": m["marker_yn"]})
@@ -106,6 +136,19 @@ def time_step(ctx, model: DeviceModel, solve_opts: dict | None = None, rk4_arith
     out = {"marker_exx": m["exx"], "marker_exy": m["exy"], "marker_pr": m["pr"], "marker_sr_ratio": m["sr_ratio"],
            "marker_sxx": m["sxx"], "marker_sxy": m["sxy"]}
     update_markers_after_stokes(ctx, gm, fields, dt, out, GATHER_OUTPUTS + GATHER_STRESS)
+    hlog = {}
+    if heat_on:   # execute_heat_solver_steps! (TimeLoop.jl:250-257)
+        hf = {k: gf[k] for k in heat.HEAT_FIELDS + ("rho1", "etas1", "etan1")}
+        hf.update(vx1=vx1, vy1=vy1, sxy2=ss["sxy2"], sxx2=ss["sxx2"])
+        t_heat = time.perf_counter()
+        hr = heat.update_marker_temp_for_grid_and_subgrid_changes(
+            ctx, {**gm, "TK": m["TK"]}, hf, model.thermal_bc, dt, float(p["max_temp_change"]),
+            float(p.get("subgrid_diff_coef_temp", 0.0)), int(p.get("iuse_heat", 1)), 0.0,
+            float(model.template.gravity_y), int(p.get("iuse_adiabatic_heating", 0)),
+            int(p.get("iuse_shear_heating", 0)))
+        if hr:
+            gf["tk2"].copy_(hr["tk2"])
+            hlog = dict(hr["stats"], seconds=time.perf_counter() - t_heat)
     # advection steps (execute_advection_steps!, TimeLoop.jl:259-262)
     ctx.calculate_spin(vx1, vy1, gf["esp"])
     if rk4_arithmetic is not None:
@@ -114,6 +157,9 @@ def time_step(ctx, model: DeviceModel, solve_opts: dict | None = None, rk4_arith
                                                 m["marker_yn"], vx1, vy1, gf["esp"], None, None, None,
                                                 m["sxx"], m["sxy"])
     rheology.update_marker_strain(ctx, m, dt, p.get("plastic_healing_rate", 0.0))
+    if heat_on and int(p.get("iuse_sticky_correction", 0)) == 1:
+        heat.apply_sticky_temperature_correction(ctx, gm, m["TK"], int(p["matid_sticky_air"]),
+                                                 int(p["matid_sticky_water"]), float(p["temperature_top"]))
     # advance_model_time!
     model.timesum += dt
     model.timestep = dt
@@ -125,4 +171,4 @@ def time_step(ctx, model: DeviceModel, solve_opts: dict | None = None, rk4_arith
             "solve_seconds_with_setup": [r["solve_seconds_with_setup"] for r in plog],
             "max_relres": max(r["relres"] for r in plog), "nnode_yield": [r["nnode_yield"] for r in plog],
             "learned_accepted": sum(r["learned_accepted"] for r in plog),
-            "final_dvxy_rel_L2": plog[-1]["dvxy_rel_L2"]}
+            "final_dvxy_rel_L2": plog[-1]["dvxy_rel_L2"], "heat": hlog}

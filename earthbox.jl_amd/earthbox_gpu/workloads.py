@@ -489,3 +489,43 @@ def rift2d_markers(xnum: int = 2001, ynum: int = 801, nm: int = 5, xsize: float 
                   itype_global=1, marker_cell_displ_max=0.5, plastic_healing_rate=0.0,
                   iuse_fluid_pressure_for_yield=0, nglobal=10, tolerance_picard=1e-3)
     return template, markers, mats, params
+
+
+# thermal constants of the rift layers: conductivity W/m/K, rho*Cp J/m3/K,
+# radiogenic heat production W/m3, thermal expansivity 1/K
+RIFT_THERMAL = {
+    "upper_crust": (2.5, 2700.0 * 1000.0, 1.5e-6, 3e-5),
+    "lower_crust": (2.1, 2900.0 * 1000.0, 0.4e-6, 3e-5),
+    "lithospheric_mantle": (3.0, 3300.0 * 1000.0, 0.0, 3e-5),
+    "asthenosphere": (3.0, 3300.0 * 1000.0, 0.0, 3e-5),
+}
+
+
+def rift2d_thermal(template, markers: dict, max_temp_change: float = 20.0):
+    """Heat-equation inputs of the rifting model: per-marker conductivity,
+    rho*Cp and adiabatic term (expansivity x temperature) from RIFT_THERMAL
+    (held at their initial values: the temperature-dependent rock-property
+    update is not part of this path), the material heat production table,
+    thermal boundary conditions (273 K at the top, the unperturbed geotherm at
+    the bottom, 1597 K at 160 km, no heat flux through the sides) and the heat-solver
+    scalars.  Returns (marker arrays, mats, bc, params), numpy."""
+    g = template.grid
+    names = list(RIFT_THERMAL)
+    tab = np.array([RIFT_THERMAL[k] for k in names])
+    mid = markers["matid"].astype(np.int64) - 1
+    therm = {"kt": tab[mid, 0].copy(), "rhocp": tab[mid, 1].copy(), "ha": tab[mid, 3] * markers["TK"]}
+    nx, ny = g.xnum, g.ynum
+    t_bottom = float(np.interp(g.ysize, [0.0, 3.5e4, 1.0e5, 1e9],
+                               [273.0, 823.0, 1573.0, 1573.0 + 0.4e-3 * (1e9 - 1e5)]))
+    btopt = np.zeros((nx, 2), order="F")
+    bbottomt = np.zeros((nx, 2), order="F")
+    btopt[:, 0] = 273.0
+    bbottomt[:, 0] = t_bottom
+    bleftt = np.zeros((ny, 2), order="F")
+    brightt = np.zeros((ny, 2), order="F")
+    bleftt[:, 1] = 1.0
+    brightt[:, 1] = 1.0
+    bc = {"btopt": btopt, "bbottomt": bbottomt, "bleftt": bleftt, "brightt": brightt}
+    params = dict(iuse_heat=1, max_temp_change=max_temp_change, iuse_adiabatic_heating=0, iuse_shear_heating=1,
+                  subgrid_diff_coef_temp=0.0, temperature_top=273.0)
+    return therm, {"mat_hr": tab[:, 2].copy()}, bc, params

@@ -14,14 +14,19 @@ properties (the oracle at these sizes only where it finishes in seconds):
 - the rifting model on 2001 x 801 nodes with 40M markers: one Picard
   iteration after the marker rheology and scatter; the solve's residual
   recomputed by the COO-order residual kernel is <= 1e-8, and the nodal
-  yield count equals the oracle's on the same fields.
+  yield count equals the oracle's on the same fields;
+- the rifting model's heat step on 2001 x 801 nodes: the heat-field scatter
+  of its 40M markers bit-exact against the oracle's serial loops, the heat
+  solve's ||R - L T|| / ||R|| <= 1e-12 recomputed on the host from the
+  oracle's COO rows, and the adaptive heat loop's sub-steps summing to the
+  time step.
 """
 import numpy as np
 import pytest
 import torch
 
 import oracle
-from earthbox_gpu import learned, rheology, timeloop, workloads
+from earthbox_gpu import heat, learned, rheology, timeloop, workloads
 from earthbox_gpu.device import to_device, to_host
 from earthbox_gpu.grid import uniform_grid
 from earthbox_gpu.picard import picard_loop_nodes
@@ -153,4 +158,56 @@ def test_config5_picard_iteration_2001x801(gpu_ctx):
     assert n > 0 and log[0]["nnode_yield"] == n
     assert log[0]["global_yield_error"] == pytest.approx(err, rel=1e-9)
     del model, solver
+    torch.cuda.empty_cache()
+
+
+def test_config5_heat_step_2001x801(gpu_ctx):
+    print("rift model 2001 x 801, 40M markers: heat scatter, heat solve, heat loop", flush=True)
+    tmpl, m, mats, par = workloads.rift2d_markers()
+    therm, tmats, bc, hpar = workloads.rift2d_thermal(tmpl, m)
+    g = tmpl.grid
+    ny, nx = g.ynum, g.xnum
+    flags = oracle.inside_flags(m["x"], m["y"], g.limits)
+    hm = {"x": m["x"], "y": m["y"], "flags": flags, "TK": m["TK"], "matid": m["matid"], **therm}
+    zero = np.zeros((ny, nx), order="F")
+    exp = oracle.scatter_heat(g, hm, {"tk2": zero, "kt1": zero, "rhocp1": zero, "hr1": zero, "ha1": zero},
+                              tmats["mat_hr"], None, bc)
+    print("oracle scatter done", flush=True)
+    gpu_ctx.set_grid(g)
+    dm = {k: to_device(v) for k, v in hm.items() if k != "flags"}
+    dm["inside_flags"] = to_device(flags)
+    del m, hm
+    fields = {k: to_device(zero) for k in heat.HEAT_FIELDS}
+    dbc = {k: to_device(v) for k, v in bc.items()}
+    heat.interpolate_marker_heat_parameters_to_grid(gpu_ctx, dm, {"mat_hr": to_device(tmats["mat_hr"])}, fields, dbc)
+    gpu_ctx.synchronize()
+    for k in heat.HEAT_FIELDS:
+        assert np.array_equal(to_host(fields[k]), exp[k]), k
+    RT1 = heat.calculate_heat_rhs_terms_grid(gpu_ctx, fields)   # radiogenic heat only
+    dt = par["timestep"]
+    torch.cuda.synchronize()
+    tk2, st = heat.discretize_and_solve_conductive_heat_equation(gpu_ctx, RT1, fields["tk1"], fields["rhocp1"],
+                                                                 fields["kt1"], dbc, dt)
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    tk2, st = heat.discretize_and_solve_conductive_heat_equation(gpu_ctx, RT1, fields["tk1"], fields["rhocp1"],
+                                                                 fields["kt1"], dbc, dt, tk2)
+    t1.record()
+    torch.cuda.synchronize()
+    print(f"heat solve: {st}, {t0.elapsed_time(t1):.2f} ms", flush=True)
+    assert st["converged"] and st["iterations"] <= 40
+    N, Li, Lj, Lv, R = oracle.heat_build(g, bc, to_host(RT1), exp["tk1"], exp["rhocp1"], exp["kt1"], dt)
+    r = R - oracle.coo_spmv(N, Li, Lj, Lv, to_host(tk2).ravel(order="F"))
+    rel = np.linalg.norm(r) / np.linalg.norm(R)
+    print(f"host-recomputed relative residual {rel:.3e}", flush=True)
+    assert rel <= 2e-12
+    out, lst = heat.run_conductive_heat_solver_loop(gpu_ctx, RT1, fields["tk1"], fields["rhocp1"], fields["kt1"],
+                                                    dbc, 10 * dt, hpar["max_temp_change"])
+    torch.cuda.synchronize()
+    print(f"heat loop over 10 time steps: {lst}", flush=True)
+    assert lst["converged"] and lst["timestep_sum"] == pytest.approx(10 * dt, rel=1e-12)
+    chg = np.abs(to_host(out["dtk1"])).max()
+    assert 0 < chg and np.isfinite(chg)
+    del dm, fields, out
     torch.cuda.empty_cache()

@@ -4,7 +4,9 @@ marker updates, RK4 with stress rotation, marker strain) against the same
 steps composed from the oracle with a direct solve, on a small grid of the
 rifting workload: two steps; yield counts within 1 %, the time step to 1e-6,
 marker positions within 1e-10 of the domain size, marker viscosities and
-stresses to 1e-6 of their range."""
+stresses to 1e-6 of their range.  With the heat path on, marker and grid
+temperatures agree to 1e-6 K (the heat solve stands in for a direct solve
+at relative residual 1e-12)."""
 import numpy as np
 import pytest
 import torch
@@ -16,18 +18,24 @@ from timeloop_oracle import initial_grid, time_step
 pytestmark = pytest.mark.gpu
 
 
-def test_rift_time_steps_match_oracle(gpu_ctx):
-    print("rift model, two time steps on 101 x 41 nodes, 4 x 4 markers per cell", flush=True)
+@pytest.mark.parametrize("with_heat", [False, True])
+def test_rift_time_steps_match_oracle(gpu_ctx, with_heat):
+    print(f"rift model, two time steps on 101 x 41 nodes, 4 x 4 markers per cell, heat path {with_heat}",
+          flush=True)
     tmpl, m, mats, par = workloads.rift2d_markers(101, 41, nm=4, xsize=1.0e5, ysize=4.0e4, seed=7)
     par = dict(par, nglobal=6, tolerance_picard=1e-3)
-    model = timeloop.upload(gpu_ctx, tmpl, m, mats, par)
+    thermal = workloads.rift2d_thermal(tmpl, m, max_temp_change=20.0) if with_heat else None
+    model = timeloop.upload(gpu_ctx, tmpl, m, mats, par, thermal)
     mo = {k: v.copy() for k, v in m.items()}
+    if with_heat:
+        mo.update({k: v.copy() for k, v in thermal[0].items()})
     grid = initial_grid(tmpl.grid)
     dt, ts, vold = par["timestep"], 0.0, None
     L = max(tmpl.grid.xsize, tmpl.grid.ysize)
     for step in (1, 2):
         glog = timeloop.time_step(gpu_ctx, model, solve_opts=dict(rtol=1e-11))
-        dt_new, vx1, vy1, clog = time_step(tmpl, mo, mats, par, grid, dt, ts, step, vold)
+        dt_new, vx1, vy1, clog = time_step(tmpl, mo, mats, par, grid, dt, ts, step, vold,
+                                           thermal[1:] if with_heat else None)
         ts += dt_new
         dt = dt_new
         vold = np.concatenate([vx1.ravel(order="F"), vy1.ravel(order="F")])
@@ -43,6 +51,12 @@ def test_rift_time_steps_match_oracle(gpu_ctx):
         err = max(np.abs(gm["x"] - mo["x"]).max(), np.abs(gm["y"] - mo["y"]).max())
         print(f"  positions: max |d| = {err:.3e} m ({err / L:.2e} of the domain)", flush=True)
         assert err <= 1e-10 * L
+        if with_heat:
+            print(f"  heat: {glog['heat']}", flush=True)
+            dtk = np.abs(gm["TK"] - mo["TK"]).max()
+            print(f"  marker temperature: max |d| = {dtk:.3e} K", flush=True)
+            assert dtk <= 1e-6 and glog["heat"]["converged"]
+            assert np.abs(to_host(model.grid["tk2"]) - grid["tk2"]).max() <= 1e-6
         for k in ("eta_flow", "eta", "sxx", "sxy", "GII", "strain_plastic"):
             scale = np.abs(mo[k]).max() + 1e-300
             d = np.abs(gm[k] - mo[k])

@@ -18,6 +18,8 @@ def initial_grid(g):
     f = {k: np.zeros(grid_field_shape(k, nx, ny), order="F") for k in GRID_FIELDS}
     f.update(plastic_yield=np.zeros((ny, nx), order="F"), yield_error=np.zeros((ny, nx), order="F"),
              eii_plastic_pressure=np.zeros((ny - 1, nx - 1), order="F"))
+    for k in ("tk1", "tk2", "kt1", "rhocp1", "hr1", "ha1"):
+        f[k] = np.zeros((ny, nx), order="F")
     return f
 
 
@@ -54,10 +56,20 @@ def picard(g, template, s, par, dt, timesum, step, nglobal, tol, old):
     return vx1, vy1, pr1, log
 
 
-def time_step(template, m, mats, par, grid, dt, timesum, step, vold):
-    """m, grid: numpy dicts updated in place; returns (new dt, vx1, vy1, log)."""
+def time_step(template, m, mats, par, grid, dt, timesum, step, vold, thermal=None):
+    """m, grid: numpy dicts updated in place; returns (new dt, vx1, vy1, log).
+    thermal: (mats with mat_hr, thermal boundary conditions, heat scalars)
+    adds the heat path (m then holds kt, rhocp, ha; grid holds tk2)."""
     g = template.grid
     flags = oracle.inside_flags(m["x"], m["y"], g.limits)
+    if thermal is not None:   # heat-field scatter with fresh cells, then the boundary conditions
+        tmats, tbc, hpar = thermal
+        hm = {k: m[k] for k in ("x", "y", "TK", "kt", "rhocp", "ha", "matid")}
+        hm["flags"] = flags
+        prev = {"tk2": grid["tk2"], "kt1": grid["kt1"], "rhocp1": grid["rhocp1"], "hr1": grid["hr1"],
+                "ha1": grid["ha1"]}
+        grid.update({k: v for k, v in oracle.scatter_heat(g, hm, prev, tmats["mat_hr"], None, tbc).items()
+                     if k in ("tk1", "kt1", "rhocp1", "hr1", "ha1")})
     r = oracle.marker_rheology(dict(m, flags=flags), mats, dict(par, timestep=dt, ntimestep=step))
     m.update({k: r[k] for k in ("cohesion", "fric", "eta_flow", "preexp", "eta")})
     props = {k: m[k] for k in ("x", "y", "rho", "cohesion", "fric", "eta_flow", "eta", "sxy", "sxx", "pfailure",
@@ -84,6 +96,16 @@ def time_step(template, m, mats, par, grid, dt, timesum, step, vold):
     o = oracle.marker_strain_rate_and_pressure(g, gm, fg, dt_new, prev)
     m.update(exx=o["marker_exx"], exy=o["marker_exy"], sr_ratio=o["marker_sr_ratio"], pr=o["marker_pr"],
              sxx=o["marker_sxx"], sxy=o["marker_sxy"])
+    if thermal is not None:
+        hm = {"x": m["x"], "y": m["y"], "flags": flags, "marker_xn": m["marker_xn"], "marker_yn": m["marker_yn"],
+              "TK": m["TK"]}
+        hf = {k: grid[k] for k in ("tk1", "kt1", "rhocp1", "hr1", "ha1", "rho1", "etas1", "etan1")}
+        hf.update(vx1=vx1, vy1=vy1, sxy2=ss["sxy2"], sxx2=ss["sxx2"])
+        hr = oracle.update_marker_temp_for_grid_and_subgrid_changes(
+            g, tbc, hm, hf, dt_new, hpar["max_temp_change"], hpar.get("subgrid_diff_coef_temp", 0.0), 0.0,
+            template.gravity_y, hpar.get("iuse_adiabatic_heating", 0), hpar.get("iuse_shear_heating", 0))
+        m["TK"] = hr["TK"]
+        grid["tk2"] = hr["tk2"]
     esp = oracle.calculate_spin(g, vx1, vy1)
     x, y, _, _, _, sxx, sxy = oracle.advect_markers_and_rotate_stress_tensor(
         4, dt_new, g, m["x"], m["y"], m["marker_xn"], m["marker_yn"], flags, vx1, vy1, esp, m["sxx"], m["sxy"])
