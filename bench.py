@@ -66,6 +66,7 @@ def parse():
     ap.add_argument("--no-config1", action="store_true")
     ap.add_argument("--no-config5", action="store_true")
     ap.add_argument("--config5-steps", type=int, default=20)
+    ap.add_argument("--no-heat", action="store_true")
     ap.add_argument("--stokes-amplitude", type=float, default=1.0,
                     help="decades of the random log-viscosity perturbation of the 1001 x 501 system")
     return ap.parse_args()
@@ -403,6 +404,85 @@ def bench_config5(ctx, steps):
     p, mx, lm = out["plain_fp64_mg"], out["mixed_fp32_mg"], out["learned_mixed"]
     out["speedup_mixed_vs_plain"] = p["solve_seconds_with_setup_mean"] / mx["solve_seconds_with_setup_mean"]
     out["speedup_learned_mixed_vs_plain"] = p["solve_seconds_with_setup_mean"] / lm["solve_seconds_with_setup_mean"]
+    return out
+
+
+def bench_heat(ctx, reps=5):
+    """The rifting model's heat step on 2001 x 801 nodes with 40M markers:
+    the heat-field scatter with the thermal boundary conditions, one heat
+    solve (multigrid-preconditioned FGMRES to 1e-12) and the whole heat step
+    (right side with shear heating off, adaptive heat loop with a 20 K change
+    limit, marker temperature update) at the model time step; then the same
+    solve on the 501 x 201 version of the grid beside the oracle's sparse
+    direct solve of that system (SuperLU, one host core), the reference's
+    direct-solve seam."""
+    import numpy as np
+    import torch
+    import oracle
+    from earthbox_gpu import heat, workloads
+    from earthbox_gpu.device import to_device, to_host
+    out = {}
+    for nx, ny, nm, with_markers in ((2001, 801, 5, True), (501, 201, 5, False)):
+        tmpl, m, mats, par = workloads.rift2d_markers(nx, ny, nm)
+        therm, tmats, bc, hpar = workloads.rift2d_thermal(tmpl, m)
+        g = tmpl.grid
+        ctx.set_grid(g)
+        dm = {k: to_device(v) for k, v in ({"x": m["x"], "y": m["y"], "TK": m["TK"], "matid": m["matid"]} | therm).items()}
+        dm["inside_flags"] = torch.empty(m["x"].size, dtype=torch.int8, device=dm["x"].device)
+        ctx.get_marker_inside_flags(dm["x"], dm["y"], out=dm["inside_flags"])
+        dm["marker_xn"] = torch.zeros(m["x"].size, dtype=torch.int32, device=dm["x"].device)
+        dm["marker_yn"] = torch.zeros_like(dm["marker_xn"])
+        del m
+        z = np.zeros((ny, nx), order="F")
+        fields = {k: to_device(z) for k in heat.HEAT_FIELDS}
+        fields.update(marker_xn=dm["marker_xn"], marker_yn=dm["marker_yn"])
+        dbc = {k: to_device(v) for k, v in bc.items()}
+        dmats = {"mat_hr": to_device(tmats["mat_hr"])}
+        key = f"{nx}x{ny}"
+        r = {"markers": int(dm["x"].numel())}
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        heat.interpolate_marker_heat_parameters_to_grid(ctx, dm, dmats, fields, dbc)
+        if with_markers:
+            torch.cuda.synchronize()
+            ev[0].record()
+            for _ in range(reps):
+                heat.interpolate_marker_heat_parameters_to_grid(ctx, dm, dmats, fields, dbc)
+            ev[1].record()
+            torch.cuda.synchronize()
+            r["scatter_ms"] = ev[0].elapsed_time(ev[1]) / reps
+        RT1 = heat.calculate_heat_rhs_terms_grid(ctx, fields)
+        dt = par["timestep"]
+        tk2, st = heat.discretize_and_solve_conductive_heat_equation(ctx, RT1, fields["tk1"], fields["rhocp1"],
+                                                                     fields["kt1"], dbc, dt)
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(reps):
+            tk2, st = heat.discretize_and_solve_conductive_heat_equation(ctx, RT1, fields["tk1"], fields["rhocp1"],
+                                                                         fields["kt1"], dbc, dt, tk2)
+        ev[1].record()
+        torch.cuda.synchronize()
+        r.update(solve_ms=ev[0].elapsed_time(ev[1]) / reps, solve_iterations=st["iterations"],
+                 solve_relres=st["relres"])
+        if with_markers:
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                TK = dm["TK"].clone()
+                hr = heat.update_marker_temp_for_grid_and_subgrid_changes(
+                    ctx, {**dm, "TK": TK}, fields, dbc, dt, hpar["max_temp_change"], 0.0, 1)
+            torch.cuda.synchronize()
+            r["heat_step_ms"] = (time.perf_counter() - t0) / reps * 1e3
+            r["heat_loop"] = hr["stats"]
+        else:
+            Rt = {k: to_host(fields[k]) for k in ("tk1", "kt1", "rhocp1")}
+            N, Li, Lj, Lv, R = oracle.heat_build(g, bc, to_host(RT1), Rt["tk1"], Rt["rhocp1"], Rt["kt1"], dt)
+            t0 = time.perf_counter()
+            oracle.direct_solve(N, Li, Lj, Lv, R, refine=0)
+            r["oracle_direct_solve_ms"] = (time.perf_counter() - t0) * 1e3
+            r["speedup_vs_oracle_direct_solve"] = r["oracle_direct_solve_ms"] / r["solve_ms"]
+        progress(f"heat {key}: {r}")
+        out[key] = r
+        del dm, fields
+        torch.cuda.empty_cache()
     return out
 
 
@@ -769,6 +849,7 @@ def main():
         ("config1", not args.no_config1, lambda: bench_config1(ctx)),
         ("rk4_config5", not args.no_config5, lambda: bench_rk4_config5(ctx)),
         ("config5", not args.no_config5, lambda: bench_config5(ctx, args.config5_steps)),
+        ("heat", not args.no_heat, lambda: bench_heat(ctx)),
         ("scatter", not args.no_scatter, lambda: bench_scatter(ctx, g, x, y)),
         ("stokes3d", not args.no_stokes3d, lambda: bench_stokes3d(ctx, args.stokes3d_n, args.stokes3d_max_iter)),
         ("sinker", not args.no_sinker, lambda: bench_sinker(ctx)),

@@ -32,7 +32,7 @@ namespace ebg {
 
 namespace {
 
-constexpr int DENSE_MAX = 512;   // coarsest-level unknowns solved by the dense inverse
+constexpr int DENSE_MAX = 90;    // coarsest-level unknowns solved by the dense inverse (fits 64 KB of LDS)
 
 __host__ __device__ inline unsigned nbk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -252,28 +252,34 @@ __global__ void __launch_bounds__(256) heat_dense_kernel(HeatLevelDev L, double*
 }
 
 // In-place Gauss-Jordan inverse of a row diagonally dominant matrix (no
-// pivoting needed), one workgroup; thread q owns row q for the elimination
-// and column q for the pivot-row scaling.
+// pivoting needed) in one workgroup's LDS: the n x n column-major matrix is
+// loaded, inverted with a barrier per pivot step and written back.  Thread q
+// scales column q of the pivot row, then eliminates row q.
 __global__ void __launch_bounds__(1024) heat_gauss_jordan_kernel(double* __restrict__ A, int n)
 {
+    extern __shared__ double S[];
     const int tid = threadIdx.x, bs = blockDim.x;
+    const int nn = n * n;
+    for (int e = tid; e < nn; e += bs) S[e] = A[e];
+    __syncthreads();
     for (int k = 0; k < n; ++k) {
-        const double pinv = 1.0 / A[(int64_t)k * n + k];
+        const double pinv = 1.0 / S[k * n + k];
         __syncthreads();
         for (int c = tid; c < n; c += bs)
-            if (c != k) A[(int64_t)c * n + k] *= pinv;
+            if (c != k) S[c * n + k] *= pinv;
         __syncthreads();
         for (int r = tid; r < n; r += bs) {
             if (r == k) continue;
-            const double f = A[(int64_t)k * n + r];
+            const double f = S[k * n + r];
             for (int c = 0; c < n; ++c)
-                if (c != k) A[(int64_t)c * n + r] -= f * A[(int64_t)c * n + k];
-            A[(int64_t)k * n + r] = -f * pinv;
+                if (c != k) S[c * n + r] -= f * S[c * n + k];
+            S[k * n + r] = -f * pinv;
         }
         __syncthreads();
-        if (tid == 0) A[(int64_t)k * n + k] = pinv;
+        if (tid == 0) S[k * n + k] = pinv;
         __syncthreads();
     }
+    for (int e = tid; e < nn; e += bs) A[e] = S[e];
 }
 
 // x = Ainv b (column-major inverse)
@@ -464,7 +470,7 @@ int build_hierarchy(ebg_ctx* ctx, HeatState& H)
         if ((rc = upload(L->steps, packed.data(), packed.size() * 8, s))) return rc;
         const int64_t n = L->n;
         if ((rc = L->coef.ensure(5 * n * 8)) || (rc = L->r.ensure(n * 8))) return rc;
-        const bool last = n <= DENSE_MAX || (L->nx <= 3 && L->ny <= 3);
+        const bool last = n <= DENSE_MAX;
         if (!H.lev.empty()) {  // coarse levels own their node values and vectors
             if ((rc = L->kt.ensure(n * 8)) || (rc = L->rc.ensure(n * 8)) || (rc = L->x.ensure(n * 8)) ||
                 (rc = L->b.ensure(n * 8)) || (rc = L->b2.ensure((2 * L->nx + 2 * L->ny) * 8)))
@@ -616,7 +622,7 @@ int prepare_levels(ebg_ctx* ctx, HeatState& H, const ebg_heat_system& sys, std::
     const HeatLevelDev& dc = D.back();
     double* A = H.dense.as<double>();
     hipLaunchKernelGGL(heat_dense_kernel, dim3(nbk(dc.n)), dim3(256), 0, s, dc, A);
-    hipLaunchKernelGGL(heat_gauss_jordan_kernel, dim3(1), dim3(1024), 0, s, A, (int)dc.n);
+    hipLaunchKernelGGL(heat_gauss_jordan_kernel, dim3(1), dim3(128), (size_t)dc.n * dc.n * 8, s, A, (int)dc.n);
     EBG_HIP(hipGetLastError());
     return EBG_OK;
 }
