@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(256) scatter_entries_kernel(ScatterArgs a)
                     const int id = a.h.matid[mu_];
                     hr = (id >= 1 && id <= a.h.nmat) ? a.h.mat_hr[id - 1] : __builtin_nan("");
                     if (a.h.serpentinization_heat_production)
-                        hr = hr + (double)a.h.serpentinization_heat_production[mu_];
+                        hr = hr + a.h.serpentinization_heat_production[mu_];
                 }
                 pv[F_HR][u] = hr;
             }

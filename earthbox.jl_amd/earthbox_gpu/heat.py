@@ -61,7 +61,7 @@ def interpolate_marker_heat_parameters_to_grid(ctx: GPUContext, markers: dict, m
     """Interpolation.interpolate_marker_heat_parameters_to_grid!
     (interpolation/Interpolation.jl:77-85).  markers: device x, y, optional
     inside_flags, TK, kt, rhocp, ha, matid (int16) and optional
-    serpentinization_heat_production (float32); mats: mat_hr.  fields: device
+    serpentinization_heat_production; mats: mat_hr.  fields: device
     tk1, kt1, rhocp1, hr1, ha1 updated in place; tk1 must hold the previous
     heat solution tk2 on entry (the reference's backup of tk1 is tk2,
     BackupTransportArrays.jl:154-163).  Optional wtnodes, marker_xn,
@@ -81,7 +81,7 @@ def interpolate_marker_heat_parameters_to_grid(ctx: GPUContext, markers: dict, m
     mat_hr = mats.get("mat_hr")
     p.mat_hr = ptr(mat_hr, np.float64)
     p.nmat = 0 if mat_hr is None else int(mat_hr.numel())
-    p.serpentinization_heat_production = ptr(markers.get("serpentinization_heat_production"), np.float32, n)
+    p.serpentinization_heat_production = ptr(markers.get("serpentinization_heat_production"), np.float64, n)
     f = HeatGridFields()
     for k in HEAT_FIELDS + ("wtnodes",):
         setattr(f, k, ptr(fields.get(k), np.float64, nb))

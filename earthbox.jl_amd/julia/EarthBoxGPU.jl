@@ -13,6 +13,16 @@ EarthBox.jl.  It keeps the reference's entry points as the seams:
                                         (advection/Advection.jl:109-114)
 - `run_multigrid_solver_gpu!(multigrid_data)` beside MultigridSolver.run_multigrid_solver
                                         (multigrid/multigrid_solver/MultigridSolver.jl:113-135, 3-D)
+- `interpolate_marker_heat_parameters_to_grid_gpu!(model, inside_flags)` beside
+                                        Interpolation.interpolate_marker_heat_parameters_to_grid!
+                                        (interpolation/Interpolation.jl:77-85)
+- `update_marker_temp_for_grid_and_subgrid_changes_gpu!(model, inside_flags)` beside
+                                        HeatSolver.update_marker_temp_for_grid_and_subgrid_changes!
+                                        (heat_solver/HeatSolver.jl:77-101)
+- `apply_sticky_temperature_correction_gpu!(model, inside_flags, air, water)` beside
+                                        HeatSolver.apply_sticky_temperature_correction!
+- `update_subgrid_stress_gpu!(model, inside_flags)` beside SubGridStress.Update.update_subgrid_stress!
+                                        (subgrid_stress/update/Update.jl:55-71)
 
 Every call copies the needed ModelData arrays to device buffers owned by a
 `GPUState`, runs the kernels and copies results back, so ModelData stays the
@@ -480,20 +490,17 @@ MarkerStrainRateRatio.calculate_marker_strain_rate_ratio! and
 MarkerPressure.calculate_marker_pressure!, in one pass.  The grid steps before
 them (update_grid_stress!, calculate_grid_stress_change!) are
 update_grid_strain_rate_and_stress_gpu!.  With correct_stress=true the same
-pass also runs StressCorrection.apply_marker_stress_correction_for_remaining_grid_stress!
-(the whole of update_marker_stress_for_grid_and_subgrid_changes! when
-subgrid_diff_coef_stress is 0).
+pass also runs StressCorrection.apply_marker_stress_correction_for_remaining_grid_stress!,
+after update_subgrid_stress_gpu! when subgrid_diff_coef_stress > 0: together
+the whole of update_marker_stress_for_grid_and_subgrid_changes!.
 """
 function update_markers_after_stokes_gpu!(st::GPUState, model, inside_flags::Vector{Int8};
                                           correct_stress::Bool=false)
     if correct_stress
         # update_marker_stress_for_grid_and_subgrid_changes! (StokesContinuitySolver.jl:365-381)
-        # runs update_subgrid_stress! before the correction when the coefficient is positive;
-        # the fused pass adds the grid stress change without that step
+        # runs update_subgrid_stress! before the correction when the coefficient is positive
         coef = model.markers.parameters.subgrid_diffusion.subgrid_diff_coef_stress.value
-        coef > 0 && error("update_markers_after_stokes_gpu!: correct_stress=true needs " *
-                          "subgrid_diff_coef_stress == 0 (got $coef); run the sub-grid stress " *
-                          "diffusion and the correction on the CPU path")
+        coef > 0 && update_subgrid_stress_gpu!(st, model, inside_flags)
     end
     set_grid!(st, model)
     m = model.markers.arrays
@@ -582,6 +589,253 @@ function run_multigrid_solver_gpu!(st::GPUState, md)
         mr.resc_principle[c] = hist[4c]
     end
     return Int(cycles[]), conv[] != 0
+end
+
+# ---------------------------------------------------------------- heat path
+
+"Field order of ebg_marker_heat_props / ebg_heat_grid_fields / ebg_thermal_bc (include/earthbox_gpu.h)."
+struct MarkerHeatProps
+    marknum::Int64
+    x::Ptr{Cvoid}; y::Ptr{Cvoid}; inside_flags::Ptr{Cvoid}
+    TK::Ptr{Cvoid}; kt::Ptr{Cvoid}; rhocp::Ptr{Cvoid}; ha::Ptr{Cvoid}
+    matid::Ptr{Cvoid}; mat_hr::Ptr{Cvoid}; nmat::Int64
+    serpentinization_heat_production::Ptr{Cvoid}
+end
+struct HeatGridFields
+    tk1::Ptr{Cvoid}; kt1::Ptr{Cvoid}; rhocp1::Ptr{Cvoid}; hr1::Ptr{Cvoid}; ha1::Ptr{Cvoid}
+    wtnodes::Ptr{Cvoid}; marker_xn::Ptr{Cvoid}; marker_yn::Ptr{Cvoid}
+end
+struct ThermalBC
+    btopt::Ptr{Cvoid}; bbottomt::Ptr{Cvoid}; bleftt::Ptr{Cvoid}; brightt::Ptr{Cvoid}
+end
+
+function thermal_bc!(st::GPUState, model)
+    b = model.bcs.arrays.temperature
+    return ThermalBC(h2d!(st, :btopt, b.btopt.array), h2d!(st, :bbottomt, b.bbottomt.array),
+                     h2d!(st, :bleftt, b.bleftt.array), h2d!(st, :brightt, b.brightt.array))
+end
+
+sync!(st::GPUState) = check(ccall((:ebg_ctx_synchronize, LIB), Cint, (Ptr{Cvoid},), st.ctx), "sync")
+
+"""
+    interpolate_marker_heat_parameters_to_grid_gpu!(st, model, inside_flags)
+
+GPU replacement of Interpolation.interpolate_marker_heat_parameters_to_grid!
+(interpolation/Interpolation.jl:77-85, called at TimeLoop.jl:220): the heat
+backups (tk1 falls back to tk2, BackupTransportArrays.jl:154-163), the
+inclusive scatter of marker_TK, marker_kt, marker_rhocp, the heat production
+and marker_ha (BilinearNumerator.jl:31-141), the averages and the thermal
+boundary conditions on tk1 (ApplyThermalBC.jl:20-44).  Bit-identical.
+"""
+function interpolate_marker_heat_parameters_to_grid_gpu!(st::GPUState, model, inside_flags::Vector{Int8})
+    set_grid!(st, model)
+    m = model.markers.arrays
+    he = model.heat_equation.arrays
+    n = model.markers.parameters.distribution.marknum.value
+    mat_hr = model.materials.arrays.mat_hr.array
+    up(k, x) = h2d!(st, k, x)
+    serp = model.materials.parameters.serpentinization.iuse_serpentinization.value == 1 ?
+           up(:mserp, m.material.marker_serpentinization_heat_production.array) : C_NULL
+    props = MarkerHeatProps(n, up(:mx, m.location.marker_x.array), up(:my, m.location.marker_y.array),
+        up(:flags, inside_flags), up(:mTK, m.thermal.marker_TK.array), up(:mkt, m.thermal.marker_kt.array),
+        up(:mrhocp, m.thermal.marker_rhocp.array), up(:mha, m.thermal.marker_ha.array),
+        up(:matid, m.material.marker_matid.array), up(:mat_hr, mat_hr), length(mat_hr), serp)
+    # tk1 receives tk2 as its backup; the others back up their previous values
+    grids = (he.temperature.tk1.array, he.thermal_conductivity.kt1.array, he.rhocp.rhocp1.array,
+             he.radiogenic_production.hr1.array, he.adiabatic_production.ha1.array,
+             model.interpolation.arrays.grid_weights.wtnodes.array)
+    keys = (:tk1, :kt1, :rhocp1, :hr1, :ha1, :wtnodes)
+    ptrs = [h2d!(st, k, x) for (k, x) in zip(keys, grids)]
+    h2d!(st, :tk1, he.temperature.tk2.array)
+    f = HeatGridFields(ptrs..., C_NULL, C_NULL)
+    check(ccall((:ebg_interpolate_marker_heat_parameters_to_grid, LIB), Cint,
+        (Ptr{Cvoid}, Ref{MarkerHeatProps}, Ref{HeatGridFields}), st.ctx, props, f),
+        "ebg_interpolate_marker_heat_parameters_to_grid")
+    bc = thermal_bc!(st, model)
+    check(ccall((:ebg_apply_thermal_bc, LIB), Cint, (Ptr{Cvoid}, Ref{ThermalBC}, Ptr{Cvoid}),
+        st.ctx, bc, st.buffers[:tk1]), "ebg_apply_thermal_bc")
+    sync!(st)
+    for (k, x) in zip(keys, grids)
+        d2h!(x, st.buffers[k])
+    end
+    return nothing
+end
+
+"Field order of ebg_heat_rhs_in, ebg_heat_loop_in/out/stats, ebg_marker_cells, ebg_subgrid_heat_in."
+struct HeatRhsIn
+    hr1::Ptr{Cvoid}; ha1::Ptr{Cvoid}; rho1::Ptr{Cvoid}; vx1::Ptr{Cvoid}; vy1::Ptr{Cvoid}
+    sxy2::Ptr{Cvoid}; etas1::Ptr{Cvoid}; sxx2::Ptr{Cvoid}; etan1::Ptr{Cvoid}
+    gravity_x::Cdouble; gravity_y::Cdouble
+    iuse_adiabatic_heating::Int32; iuse_shear_heating::Int32
+end
+struct HeatLoopIn
+    RT1::Ptr{Cvoid}; rhocp1::Ptr{Cvoid}; kt1::Ptr{Cvoid}
+    bc::ThermalBC
+    timestep::Cdouble; max_temp_change::Cdouble; max_substeps::Int32
+    tk1::Ptr{Cvoid}
+end
+struct HeatLoopOut
+    tk0::Ptr{Cvoid}; tk2::Ptr{Cvoid}; rest::Ptr{Cvoid}; dtk1::Ptr{Cvoid}
+end
+mutable struct HeatLoopStats
+    substeps::Int32; solves::Int32; iterations::Int32; not_converged::Int32
+    last_relres::Cdouble; timestep_last::Cdouble; timestep_sum::Cdouble
+    HeatLoopStats() = new(0, 0, 0, 0, 0.0, 0.0, 0.0)
+end
+struct MarkerCells
+    marknum::Int64
+    x::Ptr{Cvoid}; y::Ptr{Cvoid}; inside_flags::Ptr{Cvoid}; marker_xn::Ptr{Cvoid}; marker_yn::Ptr{Cvoid}
+end
+struct SubgridHeatIn
+    markers::MarkerCells
+    tk1::Ptr{Cvoid}; kt1::Ptr{Cvoid}; rhocp1::Ptr{Cvoid}
+    timestep::Cdouble; subgrid_diff_coef_temp::Cdouble
+end
+
+function marker_cells!(st::GPUState, model, inside_flags)
+    m = model.markers.arrays
+    n = model.markers.parameters.distribution.marknum.value
+    return MarkerCells(n, h2d!(st, :mx, m.location.marker_x.array), h2d!(st, :my, m.location.marker_y.array),
+        h2d!(st, :flags, inside_flags), h2d!(st, :xn, m.grid_marker_relationship.marker_xn.array),
+        h2d!(st, :yn, m.grid_marker_relationship.marker_yn.array))
+end
+
+"""
+    update_marker_temp_for_grid_and_subgrid_changes_gpu!(st, model, inside_flags) -> stats
+
+GPU replacement of HeatSolver.update_marker_temp_for_grid_and_subgrid_changes!
+(heat_solver/HeatSolver.jl:77-101, TimeLoop.jl:250-257): the right-hand side
+(HeatRhs.jl), the adaptive heat loop around the heat solve (HeatLoop.jl:12-78;
+the direct solve of SystemSolver.jl:32-66 becomes a multigrid-preconditioned
+FGMRES to ||R - L T|| / ||R|| <= 1e-12), dtk1 = tk2 - tk1, the sub-grid
+diffusion when subgrid_diff_coef_temp > 0 (SubgridDiffusion.jl:52-185) and
+the marker temperature update (MarkerTemperature.jl:18-45).  Writes back RT1,
+tk0, tk2, rest, dtk1, dtkn and marker_TK; nothing when the heat solver is off
+(HeatSolver.use_heat_solver).
+"""
+function update_marker_temp_for_grid_and_subgrid_changes_gpu!(st::GPUState, model, inside_flags::Vector{Int8})
+    he = model.heat_equation.arrays
+    hp = model.heat_equation.parameters
+    dt = model.timestep.parameters.main_time_loop.timestep.value
+    limit = hp.temp_change_limit.max_temp_change.value
+    (dt > 0 && limit > 0 && hp.heat_options.iuse_heat.value == 1) || return nothing
+    set_grid!(st, model)
+    sc = model.stokes_continuity.arrays
+    gp = model.gravity.parameters
+    up(k, x) = h2d!(st, k, x)
+    rin = HeatRhsIn(up(:hr1, he.radiogenic_production.hr1.array), up(:ha1, he.adiabatic_production.ha1.array),
+        up(:rho1, sc.density.rho1.array), up(:vx1, sc.staggered_grid_velocity.vx1.array),
+        up(:vy1, sc.staggered_grid_velocity.vy1.array), up(:sxy2, sc.stress.sxy2.array),
+        up(:etas1, sc.viscosity.etas1.array), up(:sxx2, sc.stress.sxx2.array), up(:etan1, sc.viscosity.etan1.array),
+        gp.gravity_x.value, gp.gravity_y.value, Int32(hp.heat_options.iuse_adiabatic_heating.value),
+        Int32(hp.heat_options.iuse_shear_heating.value))
+    RT1 = up(:RT1, he.rhs.RT1.array)
+    check(ccall((:ebg_heat_rhs, LIB), Cint, (Ptr{Cvoid}, Ref{HeatRhsIn}, Ptr{Cvoid}), st.ctx, rin, RT1),
+          "ebg_heat_rhs")
+    tk1 = up(:tk1, he.temperature.tk1.array)
+    lin = HeatLoopIn(RT1, up(:rhocp1, he.rhocp.rhocp1.array), up(:kt1, he.thermal_conductivity.kt1.array),
+                     thermal_bc!(st, model), dt, limit, Int32(0), tk1)
+    # initialize_old_temperature_array!: tk0 starts as tk1
+    lout = HeatLoopOut(up(:tk0, he.temperature.tk1.array), up(:tk2, he.temperature.tk2.array),
+                       up(:rest, he.residuals.rest.array), up(:dtk1, he.temperature.dtk1.array))
+    stats = HeatLoopStats()
+    rc = check(ccall((:ebg_run_conductive_heat_solver_loop, LIB), Cint,
+        (Ptr{Cvoid}, Ref{HeatLoopIn}, Ref{HeatLoopOut}, Ptr{Cvoid}, Ref{HeatLoopStats}),
+        st.ctx, lin, lout, C_NULL, stats), "ebg_run_conductive_heat_solver_loop")
+    rc == 0 || @warn "heat solve missed its tolerance" stats.last_relres
+    cells = marker_cells!(st, model, inside_flags)
+    TK = up(:mTK, model.markers.arrays.thermal.marker_TK.array)
+    coef = model.markers.parameters.subgrid_diffusion.subgrid_diff_coef_temp.value
+    if coef > 0
+        sin = SubgridHeatIn(cells, tk1, st.buffers[:kt1], st.buffers[:rhocp1], dt, coef)
+        check(ccall((:ebg_subgrid_heat_diffusion_update, LIB), Cint,
+            (Ptr{Cvoid}, Ref{SubgridHeatIn}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+            st.ctx, sin, TK, up(:dtkn, he.temperature.dtkn.array), st.buffers[:dtk1],
+            up(:wtnodes, model.interpolation.arrays.grid_weights.wtnodes.array)),
+            "ebg_subgrid_heat_diffusion_update")
+    end
+    check(ccall((:ebg_grid_to_marker_temperature, LIB), Cint,
+        (Ptr{Cvoid}, Ref{MarkerCells}, Ptr{Cvoid}, Cint, Ptr{Cvoid}), st.ctx, cells, st.buffers[:dtk1], 1, TK),
+        "ebg_grid_to_marker_temperature")
+    sync!(st)
+    for (k, x) in ((:RT1, he.rhs.RT1.array), (:tk0, he.temperature.tk0.array), (:tk2, he.temperature.tk2.array),
+                   (:rest, he.residuals.rest.array), (:dtk1, he.temperature.dtk1.array),
+                   (:mTK, model.markers.arrays.thermal.marker_TK.array))
+        d2h!(x, st.buffers[k])
+    end
+    if coef > 0
+        d2h!(he.temperature.dtkn.array, st.buffers[:dtkn])
+        d2h!(model.interpolation.arrays.grid_weights.wtnodes.array, st.buffers[:wtnodes])
+    end
+    return stats
+end
+
+"""
+    apply_sticky_temperature_correction_gpu!(st, model, inside_flags, matid_sticky_air, matid_sticky_water)
+
+GPU replacement of MarkerTemperature.apply_sticky_temperature_correction!
+(MarkerTemperature.jl:52-72); the caller passes the ids of
+MaterialGroupIDs.get_sticky_material_ids(model).
+"""
+function apply_sticky_temperature_correction_gpu!(st::GPUState, model, inside_flags::Vector{Int8},
+                                                  matid_sticky_air::Integer, matid_sticky_water::Integer)
+    model.heat_equation.parameters.heat_options.iuse_sticky_correction.value == 1 || return nothing
+    set_grid!(st, model)
+    m = model.markers.arrays
+    cells = marker_cells!(st, model, inside_flags)
+    TK = h2d!(st, :mTK, m.thermal.marker_TK.array)
+    check(ccall((:ebg_apply_sticky_temperature_correction, LIB), Cint,
+        (Ptr{Cvoid}, Ref{MarkerCells}, Ptr{Cvoid}, Cint, Cint, Cdouble, Ptr{Cvoid}), st.ctx, cells,
+        h2d!(st, :matid, m.material.marker_matid.array), matid_sticky_air, matid_sticky_water,
+        model.bcs.parameters.temperature.temperature_top.value, TK), "ebg_apply_sticky_temperature_correction")
+    sync!(st)
+    d2h!(m.thermal.marker_TK.array, TK)
+    return nothing
+end
+
+"Field order of ebg_subgrid_stress_in / ebg_subgrid_stress_out."
+struct SubgridStressIn
+    markers::MarkerCells
+    matid::Ptr{Cvoid}; mat_mu::Ptr{Cvoid}; nmat::Int64; eta::Ptr{Cvoid}
+    sxy1::Ptr{Cvoid}; sxx1::Ptr{Cvoid}
+    timestep::Cdouble; subgrid_diff_coef_stress::Cdouble
+end
+struct SubgridStressOut
+    marker_sxy::Ptr{Cvoid}; marker_sxx::Ptr{Cvoid}; dsxy::Ptr{Cvoid}; dsxx::Ptr{Cvoid}
+    dsxyn::Ptr{Cvoid}; dsxxn::Ptr{Cvoid}; wtetas::Ptr{Cvoid}; wtetan::Ptr{Cvoid}
+end
+
+"""
+    update_subgrid_stress_gpu!(st, model, inside_flags)
+
+GPU replacement of SubGridStress.Update.update_subgrid_stress!
+(subgrid_stress/update/Update.jl:55-71): relaxed marker stresses, their
+scatter into dsxyn and dsxxn and the remaining changes dsxy, dsxx.
+"""
+function update_subgrid_stress_gpu!(st::GPUState, model, inside_flags::Vector{Int8})
+    set_grid!(st, model)
+    m = model.markers.arrays
+    sc = model.stokes_continuity.arrays
+    mat_mu = model.materials.arrays.mat_mu.array
+    up(k, x) = h2d!(st, k, x)
+    sin = SubgridStressIn(marker_cells!(st, model, inside_flags), up(:matid, m.material.marker_matid.array),
+        up(:mat_mu, mat_mu), length(mat_mu), up(:meta, m.rheology.marker_eta.array),
+        up(:sxy1, sc.stress.sxy1.array), up(:sxx1, sc.stress.sxx1.array),
+        model.timestep.parameters.main_time_loop.timestep.value,
+        model.markers.parameters.subgrid_diffusion.subgrid_diff_coef_stress.value)
+    iw = model.interpolation.arrays.grid_weights
+    outs = (m.stress.marker_sxy.array, m.stress.marker_sxx.array, sc.stress_change.dsxy.array,
+            sc.stress_change.dsxx.array, sc.stress_change.dsxyn.array, sc.stress_change.dsxxn.array,
+            iw.wtetas.array, iw.wtetan.array)
+    keys = (:sxy, :sxx, :dsxy, :dsxx, :dsxyn, :dsxxn, :wtetas, :wtetan)
+    sout = SubgridStressOut([h2d!(st, k, x) for (k, x) in zip(keys, outs)]...)
+    check(ccall((:ebg_update_subgrid_stress, LIB), Cint,
+        (Ptr{Cvoid}, Ref{SubgridStressIn}, Ref{SubgridStressOut}), st.ctx, sin, sout), "ebg_update_subgrid_stress")
+    sync!(st)
+    for (k, x) in zip(keys, outs)
+        d2h!(x, st.buffers[k])
+    end
+    return nothing
 end
 
 end # module

@@ -174,8 +174,8 @@ int ebg_interpolate_marker_stokes_parameters_to_grid(ebg_ctx *ctx,
  * DEVICE marker arrays (length marknum): x, y; inside_flags optional (NULL:
  * computed from the grid limits); TK, kt, rhocp, ha; matid (1-based) with the
  * material table mat_hr[nmat] for the heat production, plus
- * serpentinization_heat_production (Float32, optional: added when given,
- * BilinearNumerator.jl:107-113).  A property may be NULL when its output is. */
+ * serpentinization_heat_production (optional: added when given, as with
+ * iuse_serpentinization = 1, BilinearNumerator.jl:107-113).  A property may be NULL when its output is. */
 typedef struct {
     int64_t marknum;
     const double *x, *y;
@@ -184,7 +184,7 @@ typedef struct {
     const int16_t *matid;
     const double *mat_hr;
     int64_t nmat;
-    const float *serpentinization_heat_production;
+    const double *serpentinization_heat_production;
 } ebg_marker_heat_props;
 
 /* DEVICE basic-grid arrays (ynum,xnum), in/out: on entry the backups kept where

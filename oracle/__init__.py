@@ -712,7 +712,7 @@ def marker_heat_production(m: dict, mat_hr, serp=None):
     n = flags.size
     hr = np.empty(n)
     L.ebo_marker_heat_production(n, p(flags), p(np.ascontiguousarray(m["matid"], dtype=np.int16)), p(_f64(mat_hr)),
-                                 p(None if serp is None else np.ascontiguousarray(serp, dtype=np.float32)), p(hr))
+                                 p(None if serp is None else np.ascontiguousarray(serp, dtype=np.float64)), p(hr))
     return hr
 
 

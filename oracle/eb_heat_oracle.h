@@ -31,7 +31,7 @@ void ebo_scatter_basic_inclusive(const ebo_heat_grid *g, int64_t marknum, const 
 /* Heat-production value of each marker (BilinearNumerator.jl:107-113):
  * mat_hr[matid] plus the serpentinization heat production when serp != NULL. */
 void ebo_marker_heat_production(int64_t marknum, const int8_t *flags, const int16_t *matid,
-                                const double *mat_hr, const float *serp, double *hr);
+                                const double *mat_hr, const double *serp, double *hr);
 
 /* ApplyThermalBC.apply_thermal_bc_along_boundaries!
  * (interpolation/utils/heat_funcs/ApplyThermalBC.jl:20-44); btopt, bbottomt

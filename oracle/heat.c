@@ -87,7 +87,7 @@ void ebo_scatter_basic_inclusive(const ebo_heat_grid *g, int64_t M, const double
 }
 
 void ebo_marker_heat_production(int64_t M, const int8_t *flags, const int16_t *matid, const double *mat_hr,
-                                const float *serp, double *hr)
+                                const double *serp, double *hr)
 {
     for (int64_t m = 0; m < M; ++m) {
         if (flags[m] != 1) {
@@ -95,7 +95,7 @@ void ebo_marker_heat_production(int64_t M, const int8_t *flags, const int16_t *m
             continue;
         }
         const double r = EBO_V(mat_hr, matid[m]);
-        hr[m] = serp ? r + (double)serp[m] : r;
+        hr[m] = serp ? r + serp[m] : r;
     }
 }
 

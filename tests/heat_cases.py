@@ -69,7 +69,7 @@ def heat_markers(g, per_cell, seed, nmat=4, outside=2):
         "TK": 273.0 + 1300.0 * y / g.gridy_b[-1] + rng.uniform(-20, 20, n),
         "kt": rng.uniform(1.5, 4.0, n), "rhocp": rng.uniform(2.4e6, 3.6e6, n), "ha": rng.uniform(0, 0.05, n),
         "matid": rng.integers(1, nmat + 1, n).astype(np.int16),
-        "serp": rng.uniform(0, 1e-7, n).astype(np.float32),
+        "serp": rng.uniform(0, 1e-7, n),
         "eta": 10 ** rng.uniform(19, 23, n), "sxy": rng.uniform(-3e7, 3e7, n), "sxx": rng.uniform(-3e7, 3e7, n),
     }
 

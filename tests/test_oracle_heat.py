@@ -195,7 +195,7 @@ def test_heat_scatter_matches_the_pinned_inclusive_rule():
     empty = np.isin(m["marker_xn"], [5, 6]) & np.isin(m["marker_yn"], [7, 8])
     m["flags"][empty] = -1
     out = oracle.scatter_heat(g, m, prev, MAT_HR, m["serp"])
-    hr = MAT_HR[m["matid"] - 1] + m["serp"].astype(np.float64)
+    hr = MAT_HR[m["matid"] - 1] + m["serp"]
     for key, val in (("tk1", m["TK"]), ("kt1", m["kt"]), ("rhocp1", m["rhocp"]), ("hr1", hr), ("ha1", m["ha"])):
         st = oracle.scatter_stokes(g, {"x": m["x"], "y": m["y"], "inside_flags": m["flags"], "rho": val},
                                    {"rho1": prev["tk2" if key == "tk1" else key]})
