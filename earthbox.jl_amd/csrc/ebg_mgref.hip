@@ -21,13 +21,26 @@ struct MGRef {
     // by the first ebg_mg2d_run
     std::vector<std::array<double*, 2>> orig;
     double* partial = nullptr;
+    // the global level 0 of the multimulti driver: vx, vy, pr, RX, RY, RC,
+    // rx, ry, rc on the finest grid (allocated by the first multimulti run)
+    std::array<double*, 9> lev0{};
     ~MGRef()
     {
         for (auto& a : orig)
             for (double* q : a) (void)hipFree(q);
         if (partial) (void)hipFree(partial);
+        for (double* q : lev0)
+            if (q) (void)hipFree(q);
     }
 };
+
+unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+__global__ void mm_add_kernel(double* __restrict__ a, const double* __restrict__ b, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] += b[t];
+}
 
 // per-block sums of the squared scaled interior residuals of the finest level
 // (CalculateMeanResiduals.jl:130-172): part[3*b + {x, y, c}]
@@ -248,15 +261,20 @@ int ebg_mg2d_direct_solve(ebg_ctx* ctx, int level, const double* bc_d4)
     return rc;
 }
 
-// MultigridSolver.run_multigrid_solver for MultigridData2d with
-// use_multimulti = false (MultigridSolver.jl:113-135): viscosity restriction,
-// saved originals, viscosity continuation from constant viscosity
-// (MultigridDataManager.jl:321-373, ViscosityScaling.jl:26-39, 62-79,
-// 112-131: a jump every nvcycles_viscosity_jump cycles, no residual trigger),
-// V-cycles (MultigridVCycle.jl:197-267) with mean principal residuals
-// (CalculateMeanResiduals.jl:130-172) and the test of MultigridVCycle.jl:414-434, 454-468.
-int ebg_mg2d_run(ebg_ctx* ctx, const ebg_mg_run_params* prm, const double* bc_d4, int* cycles_done, int* converged,
-                 double* history)
+// MultigridSolver.run_multigrid_solver for MultigridData2d
+// (MultigridSolver.jl:113-135): viscosity restriction, saved originals,
+// viscosity continuation from constant viscosity (MultigridDataManager.jl:321-373,
+// ViscosityScaling.jl:26-39, 62-79), V-cycles (MultigridVCycle.jl:197-267)
+// with mean principal residuals (CalculateMeanResiduals.jl:130-172).
+// use_multimulti = false: a jump every nvcycles_viscosity_jump cycles
+// (ViscosityScaling.jl:112-131, no residual trigger in 2-D) and the test of
+// MultigridVCycle.jl:414-434, 454-468.  use_multimulti = true: level 0 (the
+// global solution, original viscosity and right-hand sides) and level 1's
+// correction problem (Smoother.jl:117-150, ViscosityScaling.jl:170-205,
+// 207-337) with the global-residual test (MultigridVCycle.jl:424-431, 488-504).
+static int mg2d_run_impl(ebg_ctx* ctx, const ebg_mg_run_params* prm, const double* bc_d4, bool multimulti,
+                         int* cycles_done, int* converged, double* history, int* global_updates,
+                         double* global_history, int global_capacity)
 {
     MGRef* m = get(ctx);
     EBG_CHECK(m, EBG_ERR_STATE, "ebg_mg2d_run: no hierarchy");
@@ -284,6 +302,53 @@ int ebg_mg2d_run(ebg_ctx* ctx, const ebg_mg_run_params* prm, const double* bc_d4
             for (int f = 0; f < 2; ++f) EBG_HIP(hipMalloc(&m->orig[k][f], (size_t)nvisc(k, f) * sizeof(double)));
         EBG_HIP(hipMalloc(&m->partial, (size_t)MG_RUN_BLOCKS * 3 * sizeof(double)));
     }
+    const double one[4] = {1.0, 1.0, 1.0, 1.0};
+    const double* bcd = bc_d4 ? bc_d4 : one;
+    MGLevelDev<double>& l1 = H.lev[0];
+    const int64_t ny = l1.ynum, nx = l1.xnum;
+    const int64_t nvx = (ny + 1) * nx, nvy = ny * (nx + 1), npr = (ny - 1) * (nx - 1);
+    const int64_t sizes[3] = {nvx, nvy, npr};
+    // level 0 view: its own solution and right-hand sides and residual arrays
+    MGLevelDev<double> l0 = l1;
+    auto global_residual = [&]() {
+        // calculate_residuals_without_smoothing, then level 1's right-hand sides
+        hipLaunchKernelGGL(mg_bc_kernel<double>, dim3(nblk(mg_bc_threads(l0.xnum, l0.ynum))), dim3(256), 0, s, l0,
+                           bcd[0], bcd[1], bcd[2], bcd[3]);
+        hipLaunchKernelGGL(mg_pressure_shift_kernel<double>, dim3(1), dim3(64), 0, s, l0.pr, (const double*)nullptr,
+                           p.pressure_bc, H.scratch);
+        hipLaunchKernelGGL(mg_pressure_add_kernel<double>, dim3(nblk(npr)), dim3(256), 0, s, l0.pr, (int)npr,
+                           (const double*)H.scratch);
+        hipLaunchKernelGGL(mg_residual_kernel<double>, dim3(nblk((ny + 1) * (nx + 1))), dim3(256), 0, s, l0);
+        const double* src[3] = {l0.rx, l0.ry, l0.rc};
+        double* dst[3] = {l1.RX, l1.RY, l1.RC};
+        for (int q = 0; q < 3; ++q)
+            (void)hipMemcpyAsync(dst[q], src[q], sizes[q] * sizeof(double), hipMemcpyDeviceToDevice, s);
+    };
+    if (multimulti) {
+        if (!m->lev0[0]) {
+            const int64_t sz[9] = {nvx, nvy, npr, nvx, nvy, npr, nvx, nvy, npr};
+            for (int q = 0; q < 9; ++q) EBG_HIP(hipMalloc(&m->lev0[q], (size_t)sz[q] * sizeof(double)));
+        }
+        l0.vx = m->lev0[0];
+        l0.vy = m->lev0[1];
+        l0.pr = m->lev0[2];
+        l0.RX = m->lev0[3];
+        l0.RY = m->lev0[4];
+        l0.RC = m->lev0[5];
+        l0.rx = m->lev0[6];
+        l0.ry = m->lev0[7];
+        l0.rc = m->lev0[8];
+        l0.fix_vx = l0.fix_vy = l0.fix_p = nullptr;
+        // the caller's initial guess and right-hand sides belong to level 0
+        // (initialize_pressure_and_rhs_parts!, MultigridDataManager.jl:376-421)
+        double* from[6] = {l1.vx, l1.vy, l1.pr, l1.RX, l1.RY, l1.RC};
+        double* to[6] = {l0.vx, l0.vy, l0.pr, l0.RX, l0.RY, l0.RC};
+        for (int q = 0; q < 6; ++q)
+            EBG_HIP(hipMemcpyAsync(to[q], from[q], sizes[q % 3] * sizeof(double), hipMemcpyDeviceToDevice, s));
+        mg_zero_solution(H, 0, s);
+        global_residual();   // update_rhs_parts_on_level1_using_global_level0_residuals
+        EBG_HIP(hipGetLastError());
+    }
     int rc;
     double vmin = INFINITY, vmax = -INFINITY;
     for (int f = 0; f < 2; ++f)
@@ -291,6 +356,7 @@ int ebg_mg2d_run(ebg_ctx* ctx, const ebg_mg_run_params* prm, const double* bc_d4
     EBG_CHECK(vmin > 0.0 && std::isfinite(vmax), EBG_ERR_ARG, "ebg_mg2d_run: viscosity must be positive and finite");
     ViscRamp ramp;
     ramp.init(vmin, vmax, p.nviscosity_jumps);
+    const double max_start = vmin;   // set_initial_viscosity_scaling_parameters
     mg_restrict_viscosity(H, s);
     double omin = INFINITY, omax = -INFINITY;
     for (int k = 0; k < nl; ++k)
@@ -299,6 +365,9 @@ int ebg_mg2d_run(ebg_ctx* ctx, const ebg_mg_run_params* prm, const double* bc_d4
                                    hipMemcpyDeviceToDevice, s));
             if ((rc = mg_device_range(m->orig[k][f], nvisc(k, f), m->partial, s, omin, omax))) return rc;
         }
+    // level 0 keeps the original viscosity (update_level0_viscosity_using_original!)
+    l0.etas = m->orig[0][0];
+    l0.etan = m->orig[0][1];
     // a negative restricted viscosity under a fractional power is the
     // reference's DomainError
     auto scale_all = [&]() -> int {
@@ -311,44 +380,107 @@ int ebg_mg2d_run(ebg_ctx* ctx, const ebg_mg_run_params* prm, const double* bc_d4
         return EBG_OK;
     };
     if (ramp.below_full() && (rc = scale_all())) return rc;   // rescale_viscosity!
-    const double one[4] = {1.0, 1.0, 1.0, 1.0};
     const auto sm = mg_smoothing_counts(nl, p.finest_sweeps, p.sweep_multiplier, p.sweep_cap);
-    const MGLevelDev<double>& l0 = H.lev[0];
-    const int64_t ny = l0.ynum, nx = l0.xnum;
     const double cnt[3] = {(double)((ny - 1) * (nx - 2)), (double)((ny - 2) * (nx - 1)), (double)((ny - 1) * (nx - 1))};
     double part[3 * MG_RUN_BLOCKS];
-    int jump = 1, done = 0, conv = 0;
-    for (int ivcycle = 1; ivcycle <= p.nvcycles; ++ivcycle) {
-        mg_vcycle(H, sm, p.theta_s, p.theta_c, p.relax_v, p.relax_p, p.pressure_bc, (const double*)nullptr,
-                  bc_d4 ? bc_d4 : one, s);
-        hipLaunchKernelGGL(mean_residual2d_partial_kernel, dim3(MG_RUN_BLOCKS), dim3(256), 0, s, l0, p.stokesscale,
+    auto mean_residuals = [&](const MGLevelDev<double>& l, double* res) -> int {
+        hipLaunchKernelGGL(mean_residual2d_partial_kernel, dim3(MG_RUN_BLOCKS), dim3(256), 0, s, l, p.stokesscale,
                            p.continscale, m->partial);
         EBG_HIP(hipMemcpyAsync(part, m->partial, sizeof(part), hipMemcpyDeviceToHost, s));
         EBG_HIP(hipStreamSynchronize(s));
-        double res[3];
         for (int q = 0; q < 3; ++q) {
             double sum = 0.0;
             for (int b = 0; b < MG_RUN_BLOCKS; ++b) sum += part[3 * b + q];
             res[q] = std::log10(std::sqrt(sum / cnt[q]));
-            if (history) history[3 * (ivcycle - 1) + q] = res[q];
         }
+        return EBG_OK;
+    };
+    int jump = 1, overstep = 0, iglobal = 1, done = 0, conv = 0;
+    double glast[3] = {0.0, 0.0, 0.0};
+    for (int ivcycle = 1; ivcycle <= p.nvcycles; ++ivcycle) {
+        mg_vcycle(H, sm, p.theta_s, p.theta_c, p.relax_v, p.relax_p, p.pressure_bc, (const double*)nullptr, bcd, s);
+        double res[3];
+        if ((rc = mean_residuals(l1, res))) return rc;
+        if (history)
+            for (int q = 0; q < 3; ++q) history[3 * (ivcycle - 1) + q] = res[q];
         done = ivcycle;
-        ++jump;   // rescale_viscosity_for_levels! (2-D)
-        if (jump > p.nvcycles_viscosity_jump && ramp.below_full()) {
+        ++jump;
+        if (!multimulti) {   // rescale_viscosity_for_levels! (2-D)
+            if (jump > p.nvcycles_viscosity_jump && ramp.below_full()) {
+                jump = 1;
+                ramp.jump();
+                if ((rc = scale_all())) return rc;
+            }
+            if (mg_max_principal_residual(res, 3, ivcycle) < p.convergence_criterion &&
+                ivcycle > p.nvcycles_viscosity_jump * p.nviscosity_jumps) {
+                conv = 1;
+                break;
+            }
+            continue;
+        }
+        // rescale_viscosity_for_levels_using_multimulti! (ViscosityScaling.jl:170-205)
+        if (jump > p.nvcycles_viscosity_jump) {
             jump = 1;
-            ramp.jump();
+            ramp.max_cur = ramp.max_cur * ramp.max_factor;
+            if (ramp.max_cur >= ramp.max_o) ++overstep;
+            if (ramp.max_cur > ramp.max_o) ramp.max_cur = ramp.max_o;
+            if (overstep > 1) {
+                ramp.max_cur = max_start;
+                overstep = 0;
+                double* l1s[3] = {l1.vx, l1.vy, l1.pr};
+                double* l0s[3] = {l0.vx, l0.vy, l0.pr};
+                for (int q = 0; q < 3; ++q)
+                    hipLaunchKernelGGL(mm_add_kernel, dim3(nblk(sizes[q])), dim3(256), 0, s, l0s[q],
+                                       (const double*)l1s[q], sizes[q]);
+                global_residual();
+                if ((rc = mean_residuals(l0, glast))) return rc;
+                if (global_history && iglobal <= global_capacity)
+                    for (int q = 0; q < 3; ++q) global_history[3 * (iglobal - 1) + q] = glast[q];
+                mg_zero_solution(H, 0, s);
+                ++iglobal;
+            }
             if ((rc = scale_all())) return rc;
         }
-        if (mg_max_principal_residual(res, 3, ivcycle) < p.convergence_criterion &&
-            ivcycle > p.nvcycles_viscosity_jump * p.nviscosity_jumps) {
+        double gmax = 1e38;   // calculate_max_global_residual_2d
+        if (iglobal - 1 > 1) {
+            gmax = 0.0;
+            for (int q = 0; q < 3; ++q) {
+                const double v = std::fabs(std::pow(10.0, glast[q]));
+                if (std::isnan(v)) { gmax = NAN; break; }
+                gmax = std::max(gmax, v);
+            }
+        }
+        if (gmax < p.convergence_criterion) {
             conv = 1;
             break;
         }
     }
+    if (multimulti) {   // the global solution is level 0's (print_final_solution_stats)
+        double* l1s[3] = {l1.vx, l1.vy, l1.pr};
+        const double* l0s[3] = {l0.vx, l0.vy, l0.pr};
+        for (int q = 0; q < 3; ++q)
+            EBG_HIP(hipMemcpyAsync(l1s[q], l0s[q], sizes[q] * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
     EBG_HIP(hipGetLastError());
     if (cycles_done) *cycles_done = done;
     if (converged) *converged = conv;
+    if (global_updates) *global_updates = iglobal - 1;
     return EBG_OK;
+}
+
+int ebg_mg2d_run(ebg_ctx* ctx, const ebg_mg_run_params* prm, const double* bc_d4, int* cycles_done, int* converged,
+                 double* history)
+{
+    return mg2d_run_impl(ctx, prm, bc_d4, false, cycles_done, converged, history, nullptr, nullptr, 0);
+}
+
+int ebg_mg2d_run_multimulti(ebg_ctx* ctx, const ebg_mg_run_params* prm, const double* bc_d4, int* cycles_done,
+                            int* converged, double* history, int* global_updates, double* global_history,
+                            int global_capacity)
+{
+    EBG_CHECK(global_capacity >= 0, EBG_ERR_ARG, "ebg_mg2d_run_multimulti: global_capacity < 0");
+    return mg2d_run_impl(ctx, prm, bc_d4, true, cycles_done, converged, history, global_updates, global_history,
+                         global_capacity);
 }
 
 }  // extern "C"

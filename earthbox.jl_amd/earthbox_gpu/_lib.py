@@ -97,6 +97,7 @@ SIGNATURES: dict[str, tuple] = {
                                  c_double, c_double, c_double]),
     "ebg_mg3d_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ebg_mg2d_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_mg2d_run_multimulti": (c_int, [c_void_p] * 8 + [c_int]),
     "ebg_mg2d_direct_solve": (c_int, [c_void_p, c_int, c_void_p]),
     "ebg_stokes3d_set_system": (c_int, [c_void_p, c_int64, c_int64, c_int64, c_double, c_double,
                                         c_double, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -688,6 +688,28 @@ typedef struct ebg_mg_run_params {
 int ebg_mg2d_run(ebg_ctx *ctx, const ebg_mg_run_params *params, const double *bc_d4, int *cycles_done,
                  int *converged, double *history);
 
+/* MultigridSolver.run_multigrid_solver for MultigridData2d with
+ * use_multimulti = true (MultigridSolver.jl:113-135), the driver of the
+ * reference's test/multigrid/run_stokes_sinker_2d_multimulti.jl.  The caller
+ * fills level 1 as for ebg_mg2d_run; those solution and right-hand-side
+ * arrays become the global level 0 (MultigridDataManager.jl:376-421), level 1
+ * then solves for corrections whose right-hand side is level 0's residual
+ * (Smoother.jl:117-150).  After each V-cycle the maximum viscosity grows by
+ * its factor every nvcycles_viscosity_jump cycles; at its second overstep of
+ * the full maximum level 0 takes level 1's correction, level 1's right-hand
+ * side becomes level 0's new residual, whose mean scaled residuals are the
+ * global residuals, level 1's solution is zeroed and the ramp restarts from
+ * constant viscosity (ViscosityScaling.jl:170-205, 207-337).  The run stops
+ * when the last global residuals fall below convergence_criterion, two
+ * global updates or more existing (MultigridVCycle.jl:424-431, 488-504).  On
+ * return level 1's vx, vy, pr hold level 0's (global) solution.  history as
+ * for ebg_mg2d_run; global_history (host, may be NULL) receives 3 doubles
+ * per global update, at most global_capacity updates; global_updates the
+ * number of updates. */
+int ebg_mg2d_run_multimulti(ebg_ctx *ctx, const ebg_mg_run_params *params, const double *bc_d4, int *cycles_done,
+                            int *converged, double *history, int *global_updates, double *global_history,
+                            int global_capacity);
+
 /* ---- the reference's experimental 3-D multigrid (FP64) ----
  * Replaces multigrid/multigrid_solver/MultigridSolver.jl:113-135 for
  * MultigridData3d and the level, smoother, restriction and prolongation

@@ -178,6 +178,107 @@ class Hierarchy:
         _sigs().ebo_mg_mean_residuals(C.byref(self.levels[0].cs()), stokesscale, continscale, p(out))
         return out
 
+    def level0_residual(self, lev0, pressure_bc):
+        """Smoother.calculate_residuals_without_smoothing on the global level 0
+        (Smoother.jl:142-150): ghost values, pressure normalization, residuals;
+        written into the level-1 right-hand sides
+        (update_rhs_parts_on_level1_using_global_level0_residuals, :117-131)."""
+        L = _sigs()
+        c = lev0.cs()
+        L.ebo_mg_set_bc(C.byref(c), *self.bc_d)
+        L.ebo_mg_normalize_pressure(C.byref(c), pressure_bc)
+        L.ebo_mg_residuals(C.byref(c))
+        f = self.levels[0]
+        f.RX[...] = lev0.rx
+        f.RY[...] = lev0.ry
+        f.RC[...] = lev0.rc
+
+    def run_multimulti(self, nvcycles, convergence_criterion, nviscosity_jumps=5.9999,
+                       nvcycles_viscosity_jump=3, stokesscale=1.0, continscale=1.0, finest_sweeps=2,
+                       multiplier=2, cap=None, theta_s=0.9, theta_c=0.3, relax_v=1.0, relax_p=1.0,
+                       pressure_bc=0.0):
+        """MultigridSolver.run_multigrid_solver for MultigridData2d with
+        use_multimulti = true (MultigridSolver.jl:113-135): level 0 holds the
+        global solution with the original viscosity and right-hand sides
+        (MultigridDataManager.jl:376-421 puts the initial guess and right-hand
+        sides there); the hierarchy from level 1 down solves for corrections
+        whose right-hand side is level 0's residual.  After each V-cycle
+        (MultigridVCycle.jl:197-267) the multimulti continuation
+        (ViscosityScaling.jl:170-205, 308-337): the maximum viscosity grows by
+        its factor every nvcycles_viscosity_jump cycles; on its second
+        overstep of the full maximum, level 0 takes level 1's correction
+        (:207-215), level 1's right-hand side becomes level 0's new residual,
+        whose mean scaled residuals are recorded as the global residuals
+        (:381-404), level 1's solution is zeroed and the ramp restarts from
+        constant viscosity.  The stopping test uses the last global residuals
+        once two global updates exist (MultigridVCycle.jl:424-431, 488-504).
+        The caller fills level 1 (self.levels[0]) as for run(); the global
+        solution is left in self.level0.  Returns (cycles run, converged,
+        principal history[cycles, 3], global history[updates, 3])."""
+        L = _sigs()
+        names = ("etan", "etas")
+        f0 = self.levels[0]
+        lev0 = Level(f0.xnum, f0.ynum, 1.0, 1.0)
+        lev0.g = f0.g
+        for k in ("etas", "etan", "vx", "vy", "pr", "RX", "RY", "RC"):
+            setattr(lev0, k, getattr(f0, k).copy(order="F"))
+        self.level0 = lev0
+        f0.zero_solution()
+        self.level0_residual(lev0, pressure_bc)
+        vmin = min(float(getattr(f0, n).min()) for n in names)
+        vmax = max(float(getattr(f0, n).max()) for n in names)
+        min_o, max_o = vmin, vmax
+        min_cur = max_cur = max_start = vmin
+        max_factor = 10.0 ** (np.log10(vmax / vmin) / nviscosity_jumps)
+        self.restrict_viscosity()
+        orig = [{n: getattr(lv, n).copy(order="F") for n in names} for lv in self.levels]
+        omin = min(float(a.min()) for o in orig for a in o.values())
+
+        def scale_all():
+            coef = np.log(max_cur / min_cur) / np.log(max_o / min_o)
+            if omin < 0 and coef != np.floor(coef):
+                raise ValueError("restricted viscosity is negative on a coarse level")
+            for lv, o in zip(self.levels, orig):
+                for n in names:
+                    L.ebo_mg_scale_viscosity(o[n].size, p(o[n]), p(getattr(lv, n)), min_o, max_o,
+                                             min_cur, max_cur)
+        if max_cur < max_o or min_cur > min_o:
+            scale_all()
+        sm = smoothing_counts(len(self.levels), finest_sweeps, multiplier, cap)
+        jump, overstep, hist, ghist, converged = 1, 0, [], [], False
+        for ivcycle in range(1, nvcycles + 1):
+            self.vcycle(sm, theta_s, theta_c, relax_v, relax_p, pressure_bc)
+            hist.append(self.mean_residuals(stokesscale, continscale))
+            jump += 1
+            if jump > nvcycles_viscosity_jump:
+                jump = 1
+                max_cur = max_cur * max_factor
+                if max_cur >= max_o:
+                    overstep += 1
+                if max_cur > max_o:
+                    max_cur = max_o
+                if overstep > 1:
+                    max_cur = max_start
+                    overstep = 0
+                    for k in ("vx", "vy", "pr"):
+                        getattr(lev0, k)[...] += getattr(f0, k)
+                    lev0.etas[...] = orig[0]["etas"]
+                    lev0.etan[...] = orig[0]["etan"]
+                    self.level0_residual(lev0, pressure_bc)
+                    out = np.zeros(3)
+                    L.ebo_mg_mean_residuals(C.byref(lev0.cs()), stokesscale, continscale, p(out))
+                    ghist.append(out)
+                    f0.zero_solution()
+                scale_all()
+            if len(ghist) > 1:
+                rmax = float(np.max(np.abs(10.0 ** ghist[-1])))
+            else:
+                rmax = 1e38
+            if rmax < convergence_criterion:
+                converged = True
+                break
+        return len(hist), converged, np.array(hist).reshape(-1, 3), np.array(ghist).reshape(-1, 3)
+
     def run(self, nvcycles, convergence_criterion, nviscosity_jumps=12.0, nvcycles_viscosity_jump=15,
             stokesscale=1.0, continscale=1.0, finest_sweeps=2, multiplier=2, cap=None,
             theta_s=0.9, theta_c=0.3, relax_v=1.0, relax_p=1.0, pressure_bc=0.0):

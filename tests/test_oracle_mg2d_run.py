@@ -10,7 +10,16 @@ What fixes each piece independently of its own formulas:
   - stopping rule: with a loose criterion the run stops at the first cycle
     past nvcycles_viscosity_jump * njumps (and never before the third);
   - fixed point: a converged run ends where plain V-cycles on the full
-    viscosity end.
+    viscosity end;
+  - multimulti: with one cycle per jump and 1.9999 jumps the maximum
+    viscosity oversteps the full maximum at the second jump of each ramp, so
+    global updates fall on cycles 3, 6, 9, ... (derived from the rule, not
+    from the code); the run cannot stop before the second update and stops
+    there under any criterion below the 1e38 placeholder; the recorded global
+    residuals are the mean residuals of level 0's own residual (recomputing
+    them after the run gives the last record bit for bit, the ghost and
+    pressure passes being idempotent); a converged run ends at the plain
+    V-cycle fixed point.
 """
 import numpy as np
 
@@ -92,4 +101,53 @@ def test_converged_run_reaches_the_plain_vcycle_fixed_point():
         P.vcycle(sm)
     for k in ("vx", "vy", "pr"):
         a, b = getattr(H.levels[0], k), getattr(P.levels[0], k)
+        assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max(), k
+
+
+def test_multimulti_global_update_schedule():
+    kw = dict(nviscosity_jumps=1.9999, nvcycles_viscosity_jump=1, finest_sweeps=2, cap=16)
+    H, _ = _sinker_hierarchy(17)
+    n, conv, hist, gh = H.run_multimulti(9, 0.0, **kw)
+    assert (n, conv) == (9, False) and gh.shape == (3, 3) and hist.shape == (9, 3)
+    for cycles, updates in ((2, 0), (3, 1), (5, 1), (6, 2)):
+        H, _ = _sinker_hierarchy(17)
+        _, _, _, gh = H.run_multimulti(cycles, 0.0, **kw)
+        assert gh.shape[0] == updates, (cycles, gh.shape)
+    H, _ = _sinker_hierarchy(17)
+    n, conv, _, gh = H.run_multimulti(50, 1e30, **kw)
+    assert (n, conv) == (6, True) and gh.shape[0] == 2
+    # the placeholder before two updates is 1e38: a criterion above it stops at once
+    H, _ = _sinker_hierarchy(17)
+    n, conv, _, gh = H.run_multimulti(50, 1e39, **kw)
+    assert (n, conv, gh.shape[0]) == (1, True, 0)
+
+
+def test_multimulti_global_residuals_are_level0_residuals():
+    H, S = _sinker_hierarchy(17)
+    sc = dict(stokesscale=S["stokesscale"], continscale=S["continscale"])
+    _, _, _, gh = H.run_multimulti(7, 0.0, nviscosity_jumps=1.9999, nvcycles_viscosity_jump=1, finest_sweeps=2,
+                                   cap=16, **sc)
+    lev0 = H.level0
+    rx = H.levels[0].RX.copy()
+    H.level0_residual(lev0, 0.0)
+    out = np.zeros(3)
+    mg._sigs().ebo_mg_mean_residuals(mg.C.byref(lev0.cs()), sc["stokesscale"], sc["continscale"], mg.p(out))
+    assert np.array_equal(out, gh[-1])
+    # level 1's right-hand side is level 0's residual
+    assert np.array_equal(H.levels[0].RX, rx)
+
+
+def test_multimulti_converges_to_the_plain_vcycle_fixed_point():
+    H, S = _sinker_hierarchy(33, eta_block=1e22)
+    n, conv, hist, gh = H.run_multimulti(600, 1e-9, nviscosity_jumps=5.9999, nvcycles_viscosity_jump=3,
+                                         stokesscale=S["stokesscale"], continscale=S["continscale"],
+                                         finest_sweeps=5, cap=48)
+    assert conv and n < 600 and gh.shape[0] >= 2
+    P, _ = _sinker_hierarchy(33, eta_block=1e22)
+    P.restrict_viscosity()
+    sm = mg.smoothing_counts(len(P.levels), 5, cap=48)
+    for _ in range(400):
+        P.vcycle(sm)
+    for k in ("vx", "vy", "pr"):
+        a, b = getattr(H.level0, k), getattr(P.levels[0], k)
         assert np.abs(a - b).max() <= 1e-6 * np.abs(b).max(), k
