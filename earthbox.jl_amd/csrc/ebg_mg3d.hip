@@ -361,13 +361,24 @@ struct MG3Ref {
     // reduction partials (4 * MG_RUN_BLOCKS); allocated by the first ebg_mg3d_run
     std::vector<std::array<double*, 4>> orig;
     double* partial = nullptr;
+    // the global level 0 of the multimulti driver: vx, vy, vz, pr, RX, RY, RZ,
+    // RC, rx, ry, rz, rc (allocated by the first multimulti run)
+    std::array<double*, 12> lev0{};
     ~MG3Ref()
     {
         for (auto& a : orig)
             for (double* p : a) (void)hipFree(p);
         if (partial) (void)hipFree(partial);
+        for (double* p : lev0)
+            if (p) (void)hipFree(p);
     }
 };
+
+__global__ void mm3_add_kernel(double* __restrict__ a, const double* __restrict__ b, int64_t n)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) a[t] += b[t];
+}
 
 // per-block sums of the squared scaled interior residuals of the finest level
 // (CalculateMeanResiduals.jl:14-70): part[4*b + {x, y, z, c}]
@@ -574,7 +585,9 @@ int ebg_mg3d_vcycles(ebg_ctx* ctx, int ncycles, int finest_sweeps, int multiplie
 // ViscosityScaling.jl:11-24, 41-60, 81-110), then V-cycles with mean principal residuals
 // (CalculateMeanResiduals.jl:14-70) and the convergence test
 // (MultigridVCycle.jl:79-135, 390-412, 436-452).
-int ebg_mg3d_run(ebg_ctx* ctx, const ebg_mg_run_params* prm, int* cycles_done, int* converged, double* history)
+static int mg3d_run_impl(ebg_ctx* ctx, const ebg_mg_run_params* prm, bool multimulti, int* cycles_done,
+                         int* converged, double* history, int* global_updates, double* global_history,
+                         int global_capacity)
 {
     MG3Ref* m = get3(ctx);
     EBG_CHECK(m, EBG_ERR_STATE, "ebg_mg3d_run: no hierarchy");
@@ -610,6 +623,37 @@ int ebg_mg3d_run(ebg_ctx* ctx, const ebg_mg_run_params* prm, int* cycles_done, i
             for (int f = 0; f < 4; ++f) EBG_HIP(hipMalloc(&m->orig[k][f], (size_t)nvisc(k, f) * sizeof(double)));
         EBG_HIP(hipMalloc(&m->partial, (size_t)MG_RUN_BLOCKS * 4 * sizeof(double)));
     }
+    MG3LevelDev<double>& l1 = H.lev[0];
+    const int64_t sizes[4] = {(int64_t)l1.nvx(), (int64_t)l1.nvy(), (int64_t)l1.nvz(), (int64_t)l1.npr()};
+    MG3LevelDev<double> l0v = l1;   // level 0 view
+    auto global_residual = [&]() {
+        // Smoother.calculate_residuals_without_smoothing on level 0, then
+        // level 1's right-hand sides (Smoother.jl:101-141)
+        mg3_set_bc(l0v, s);
+        hipLaunchKernelGGL(mg3_pressure_shift_kernel<double>, dim3(1), dim3(64), 0, s, l0v.pr, p.pressure_bc,
+                           H.scratch);
+        hipLaunchKernelGGL(mg3_pressure_add_kernel<double>, dim3(nb3(sizes[3])), dim3(256), 0, s, l0v.pr, sizes[3],
+                           (const double*)H.scratch);
+        residuals(l0v, s);
+        const double* src[4] = {l0v.rx, l0v.ry, l0v.rz, l0v.rc};
+        double* dst[4] = {l1.RX, l1.RY, l1.RZ, l1.RC};
+        for (int q = 0; q < 4; ++q)
+            (void)hipMemcpyAsync(dst[q], src[q], sizes[q] * sizeof(double), hipMemcpyDeviceToDevice, s);
+    };
+    if (multimulti) {
+        if (!m->lev0[0])
+            for (int q = 0; q < 12; ++q) EBG_HIP(hipMalloc(&m->lev0[q], (size_t)sizes[q % 4] * sizeof(double)));
+        double** v0[12] = {&l0v.vx, &l0v.vy, &l0v.vz, &l0v.pr, &l0v.RX, &l0v.RY, &l0v.RZ, &l0v.RC,
+                           &l0v.rx, &l0v.ry, &l0v.rz, &l0v.rc};
+        for (int q = 0; q < 12; ++q) *v0[q] = m->lev0[q];
+        // the caller's initial guess and right-hand sides belong to level 0
+        double* from[8] = {l1.vx, l1.vy, l1.vz, l1.pr, l1.RX, l1.RY, l1.RZ, l1.RC};
+        for (int q = 0; q < 8; ++q)
+            EBG_HIP(hipMemcpyAsync(*v0[q], from[q], sizes[q % 4] * sizeof(double), hipMemcpyDeviceToDevice, s));
+        mg3_zero_solution(H, 0, s);
+        global_residual();
+        EBG_HIP(hipGetLastError());
+    }
     int rc;
     // set_initial_viscosity_scaling_parameters: range of the finest level
     double vmin = INFINITY, vmax = -INFINITY;
@@ -618,6 +662,7 @@ int ebg_mg3d_run(ebg_ctx* ctx, const ebg_mg_run_params* prm, int* cycles_done, i
     EBG_CHECK(vmin > 0.0 && std::isfinite(vmax), EBG_ERR_ARG, "ebg_mg3d_run: viscosity must be positive and finite");
     ViscRamp ramp;
     ramp.init(vmin, vmax, p.nviscosity_jumps);
+    const double max_start = vmin;
     // interpolate_viscosity_to_coarser_levels! then save_original_viscosity_on_all_layers!
     mg3_restrict_viscosity(H, s);
     double omin = INFINITY, omax = -INFINITY;
@@ -627,6 +672,11 @@ int ebg_mg3d_run(ebg_ctx* ctx, const ebg_mg_run_params* prm, int* cycles_done, i
                                    hipMemcpyDeviceToDevice, s));
             if ((rc = mg_device_range(m->orig[k][f], nvisc(k, f), m->partial, s, omin, omax))) return rc;
         }
+    // level 0 keeps the original viscosity (update_level0_viscosity_using_original!)
+    l0v.etan = m->orig[0][0];
+    l0v.etaxy = m->orig[0][1];
+    l0v.etaxz = m->orig[0][2];
+    l0v.etayz = m->orig[0][3];
     // The restriction's boundary weights extrapolate, so a rough finest-level
     // field can restrict to negative coarse viscosity; the reference's power
     // law then raises a DomainError (negative base, fractional exponent),
@@ -642,26 +692,71 @@ int ebg_mg3d_run(ebg_ctx* ctx, const ebg_mg_run_params* prm, int* cycles_done, i
     };
     if (ramp.below_full() && (rc = scale_all())) return rc;   // rescale_viscosity!
     const auto sm = mg_smoothing_counts(nl, p.finest_sweeps, p.sweep_multiplier, p.sweep_cap);
-    const MG3LevelDev<double>& l0 = H.lev[0];
+    const MG3LevelDev<double>& l0 = H.lev[0];   // level 1 of the reference (the finest of the hierarchy)
     const int64_t ny = l0.ynum, nx = l0.xnum, nz = l0.znum;
     const double cnt[4] = {(double)((ny - 1) * (nx - 2) * (nz - 1)), (double)((ny - 2) * (nx - 1) * (nz - 1)),
                            (double)((ny - 1) * (nx - 1) * (nz - 2)), (double)((ny - 1) * (nx - 1) * (nz - 1))};
     double part[4 * MG_RUN_BLOCKS];
-    int jump = 1, done = 0, conv = 0;
-    for (int ivcycle = 1; ivcycle <= p.nvcycles; ++ivcycle) {
-        mg3_vcycle(H, sm, p.theta_s, p.theta_c, p.relax_v, p.relax_p, p.pressure_bc, s);
-        hipLaunchKernelGGL(mean_residual_partial_kernel, dim3(MG_RUN_BLOCKS), dim3(256), 0, s, l0,
+    auto mean_residuals = [&](const MG3LevelDev<double>& l, double* res) -> int {
+        hipLaunchKernelGGL(mean_residual_partial_kernel, dim3(MG_RUN_BLOCKS), dim3(256), 0, s, l,
                            1.0 / p.stokesscale, 1.0 / p.continscale, m->partial);
         EBG_HIP(hipMemcpyAsync(part, m->partial, sizeof(part), hipMemcpyDeviceToHost, s));
         EBG_HIP(hipStreamSynchronize(s));
-        double res[4];
         for (int q = 0; q < 4; ++q) {
             double sum = 0.0;
             for (int b = 0; b < MG_RUN_BLOCKS; ++b) sum += part[4 * b + q];
             res[q] = std::log10(std::sqrt(sum / cnt[q]));
-            if (history) history[4 * (ivcycle - 1) + q] = res[q];
         }
+        return EBG_OK;
+    };
+    int jump = 1, overstep = 0, iglobal = 1, done = 0, conv = 0;
+    double glast[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int ivcycle = 1; ivcycle <= p.nvcycles; ++ivcycle) {
+        mg3_vcycle(H, sm, p.theta_s, p.theta_c, p.relax_v, p.relax_p, p.pressure_bc, s);
+        double res[4];
+        if ((rc = mean_residuals(l0, res))) return rc;
+        if (history)
+            for (int q = 0; q < 4; ++q) history[4 * (ivcycle - 1) + q] = res[q];
         done = ivcycle;
+        if (multimulti) {   // rescale_viscosity_for_levels_using_multimulti! (ViscosityScaling.jl:133-168)
+            ++jump;
+            if (jump > p.nvcycles_viscosity_jump) {
+                jump = 1;
+                ramp.max_cur = ramp.max_cur * ramp.max_factor;
+                if (ramp.max_cur >= ramp.max_o) ++overstep;
+                if (ramp.max_cur > ramp.max_o) ramp.max_cur = ramp.max_o;
+                if (overstep > 1) {
+                    ramp.max_cur = max_start;
+                    overstep = 0;
+                    double* a1[4] = {l1.vx, l1.vy, l1.vz, l1.pr};
+                    double* a0[4] = {l0v.vx, l0v.vy, l0v.vz, l0v.pr};
+                    for (int q = 0; q < 4; ++q)
+                        hipLaunchKernelGGL(mm3_add_kernel, dim3(nb3(sizes[q])), dim3(256), 0, s, a0[q],
+                                           (const double*)a1[q], sizes[q]);
+                    global_residual();
+                    if ((rc = mean_residuals(l0v, glast))) return rc;
+                    if (global_history && iglobal <= global_capacity)
+                        for (int q = 0; q < 4; ++q) global_history[4 * (iglobal - 1) + q] = glast[q];
+                    mg3_zero_solution(H, 0, s);
+                    ++iglobal;
+                }
+                if ((rc = scale_all())) return rc;
+            }
+            double gmax = 1e38;   // calculate_max_global_residual_3d
+            if (iglobal - 1 > 1) {
+                gmax = 0.0;
+                for (int q = 0; q < 4; ++q) {
+                    const double v = std::fabs(std::pow(10.0, glast[q]));
+                    if (std::isnan(v)) { gmax = NAN; break; }
+                    gmax = std::max(gmax, v);
+                }
+            }
+            if (gmax < p.convergence_criterion) {
+                conv = 1;
+                break;
+            }
+            continue;
+        }
         // rescale_viscosity_for_levels! (3-D): restot sums the x-Stokes
         // residual four times, as the reference does
         ++jump;
@@ -679,10 +774,30 @@ int ebg_mg3d_run(ebg_ctx* ctx, const ebg_mg_run_params* prm, int* cycles_done, i
             break;
         }
     }
+    if (multimulti) {   // the global solution is level 0's
+        double* a1[4] = {l1.vx, l1.vy, l1.vz, l1.pr};
+        const double* a0[4] = {l0v.vx, l0v.vy, l0v.vz, l0v.pr};
+        for (int q = 0; q < 4; ++q)
+            EBG_HIP(hipMemcpyAsync(a1[q], a0[q], sizes[q] * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
     EBG_HIP(hipGetLastError());
     if (cycles_done) *cycles_done = done;
     if (converged) *converged = conv;
+    if (global_updates) *global_updates = iglobal - 1;
     return EBG_OK;
+}
+
+int ebg_mg3d_run(ebg_ctx* ctx, const ebg_mg_run_params* prm, int* cycles_done, int* converged, double* history)
+{
+    return mg3d_run_impl(ctx, prm, false, cycles_done, converged, history, nullptr, nullptr, 0);
+}
+
+int ebg_mg3d_run_multimulti(ebg_ctx* ctx, const ebg_mg_run_params* prm, int* cycles_done, int* converged,
+                            double* history, int* global_updates, double* global_history, int global_capacity)
+{
+    EBG_CHECK(global_capacity >= 0, EBG_ERR_ARG, "ebg_mg3d_run_multimulti: global_capacity < 0");
+    return mg3d_run_impl(ctx, prm, true, cycles_done, converged, history, global_updates, global_history,
+                         global_capacity);
 }
 
 }  // extern "C"

@@ -96,6 +96,7 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_mg3d_vcycles": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
                                  c_double, c_double, c_double]),
     "ebg_mg3d_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ebg_mg3d_run_multimulti": (c_int, [c_void_p] * 7 + [c_int]),
     "ebg_mg2d_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "ebg_mg2d_run_multimulti": (c_int, [c_void_p] * 8 + [c_int]),
     "ebg_mg2d_direct_solve": (c_int, [c_void_p, c_int, c_void_p]),

@@ -115,6 +115,29 @@ class Multigrid3DGPU:
                                hist.ctypes.data), "ebg_mg3d_run")
         return done.value, bool(conv.value), hist[:done.value].copy()
 
+    def run_multimulti(self, nvcycles, convergence_criterion, nviscosity_jumps=5.9999, nvcycles_viscosity_jump=3,
+                       stokesscale=1.0, continscale=1.0, finest_sweeps=2, multiplier=2, cap=0, theta_s=0.9,
+                       theta_c=0.3, relax_v=1.0, relax_p=1.0, pressure_bc=0.0):
+        """MultigridSolver.run_multigrid_solver with use_multimulti = true:
+        level 0 holds the global solution, level 1 solves for corrections.
+        Fill level 0 as for run(); on return it holds the global solution.
+        Returns (cycles run, converged, principal history[cycles, 4], global
+        history[updates, 4])."""
+        prm = MGRunParams(nvcycles=nvcycles, nvcycles_viscosity_jump=nvcycles_viscosity_jump,
+                          finest_sweeps=finest_sweeps, sweep_multiplier=multiplier, sweep_cap=cap,
+                          nviscosity_jumps=nviscosity_jumps,
+                          convergence_criterion=convergence_criterion, resmax=0.0,
+                          stokesscale=stokesscale, continscale=continscale, theta_s=theta_s,
+                          theta_c=theta_c, relax_v=relax_v, relax_p=relax_p, pressure_bc=pressure_bc)
+        hist = np.zeros((max(nvcycles, 1), 4))
+        cap_g = max(nvcycles, 1)
+        ghist = np.zeros((cap_g, 4))
+        done, conv, ng = C.c_int(), C.c_int(), C.c_int()
+        check(lib.ebg_mg3d_run_multimulti(self.ctx.handle, C.byref(prm), C.byref(done), C.byref(conv),
+                                          hist.ctypes.data, C.byref(ng), ghist.ctypes.data, cap_g),
+              "ebg_mg3d_run_multimulti")
+        return done.value, bool(conv.value), hist[:done.value].copy(), ghist[:ng.value].copy()
+
 
 class Stokes3DSolverGPU:
     """FGMRES(FP64) + V-cycle (FP32 or FP64) solve of the 3-D multigrid's system."""

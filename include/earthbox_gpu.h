@@ -769,6 +769,16 @@ int ebg_mg3d_vcycles(ebg_ctx *ctx, int ncycles, int finest_sweeps, int multiplie
 int ebg_mg3d_run(ebg_ctx *ctx, const ebg_mg_run_params *params, int *cycles_done, int *converged,
                  double *history);
 
+/* MultigridSolver.run_multigrid_solver for MultigridData3d with
+ * use_multimulti = true: the 3-D form of ebg_mg2d_run_multimulti
+ * (ViscosityScaling.jl:133-168, 217-265, Smoother.jl:101-141, the
+ * global-residual test of MultigridVCycle.jl:400-411, 470-486); resmax is not
+ * used.  On return level 1's vx, vy, vz, pr hold level 0's (global) solution;
+ * global_history receives 4 doubles per global update, at most
+ * global_capacity updates. */
+int ebg_mg3d_run_multimulti(ebg_ctx *ctx, const ebg_mg_run_params *params, int *cycles_done, int *converged,
+                            double *history, int *global_updates, double *global_history, int global_capacity);
+
 /* ---- 3-D Stokes solve on the 3-D multigrid's system ----
  * No assembled 3-D system exists in the reference: its 3-D Stokes path is the
  * multigrid of MultigridSolver.run_multigrid_solver (MultigridSolver.jl:113-135)

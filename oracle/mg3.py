@@ -242,6 +242,88 @@ class Hierarchy:
         _sigs().ebo_mg3_mean_residuals(C.byref(self.levels[0].cs()), stokesscale, continscale, p(out))
         return out
 
+    def level0_residual(self, lev0, pressure_bc):
+        """Smoother.calculate_residuals_without_smoothing on the global level 0
+        (Smoother.jl:133-141) written into the level-1 right-hand sides
+        (update_rhs_parts_on_level1_using_global_level0_residuals, :101-115)."""
+        L = _sigs()
+        c = lev0.cs()
+        L.ebo_mg3_set_bc(C.byref(c))
+        L.ebo_mg3_normalize_pressure(C.byref(c), pressure_bc)
+        L.ebo_mg3_residuals(C.byref(c))
+        f = self.levels[0]
+        for a, b in (("RX", "rx"), ("RY", "ry"), ("RZ", "rz"), ("RC", "rc")):
+            getattr(f, a)[...] = getattr(lev0, b)
+
+    def run_multimulti(self, nvcycles, convergence_criterion, nviscosity_jumps=5.9999, nvcycles_viscosity_jump=3,
+                       stokesscale=1.0, continscale=1.0, finest_sweeps=2, multiplier=2, cap=None, theta_s=0.9,
+                       theta_c=0.3, relax_v=1.0, relax_p=1.0, pressure_bc=0.0):
+        """MultigridSolver.run_multigrid_solver for MultigridData3d with
+        use_multimulti = true: as the 2-D restatement (oracle/mg.py
+        Hierarchy.run_multimulti) with the 3-D level 0 (ViscosityScaling.jl:133-168,
+        217-265, 353-381) and the global-residual test (MultigridVCycle.jl:400-411,
+        470-486).  Returns (cycles run, converged, principal history[cycles, 4],
+        global history[updates, 4]); the global solution is left in self.level0."""
+        L = _sigs()
+        names = ("etan", "etaxy", "etaxz", "etayz")
+        f0 = self.levels[0]
+        lev0 = Level(f0.xnum, f0.ynum, f0.znum, 1.0, 1.0, 1.0, axes=(f0.ax, f0.ay, f0.az))
+        for k in names + ("vx", "vy", "vz", "pr", "RX", "RY", "RZ", "RC"):
+            setattr(lev0, k, getattr(f0, k).copy(order="F"))
+        self.level0 = lev0
+        f0.zero_solution()
+        self.level0_residual(lev0, pressure_bc)
+        vmin = min(float(getattr(f0, n).min()) for n in names)
+        vmax = max(float(getattr(f0, n).max()) for n in names)
+        min_o, max_o = vmin, vmax
+        min_cur = max_cur = max_start = vmin
+        max_factor = 10.0 ** (np.log10(vmax / vmin) / nviscosity_jumps)
+        self.restrict_viscosity()
+        orig = [{n: getattr(lv, n).copy(order="F") for n in names} for lv in self.levels]
+        omin = min(float(a.min()) for o in orig for a in o.values())
+
+        def scale_all():
+            coef = np.log(max_cur / min_cur) / np.log(max_o / min_o)
+            if omin < 0 and coef != np.floor(coef):
+                raise ValueError("restricted viscosity is negative on a coarse level")
+            for lv, o in zip(self.levels, orig):
+                for n in names:
+                    L.ebo_mg_scale_viscosity(o[n].size, p(o[n]), p(getattr(lv, n)), min_o, max_o,
+                                             min_cur, max_cur)
+        if max_cur < max_o or min_cur > min_o:
+            scale_all()
+        sm = smoothing_counts(len(self.levels), finest_sweeps, multiplier, cap)
+        jump, overstep, hist, ghist, converged = 1, 0, [], [], False
+        for ivcycle in range(1, nvcycles + 1):
+            self.vcycle(sm, theta_s, theta_c, relax_v, relax_p, pressure_bc)
+            hist.append(self.mean_residuals(stokesscale, continscale))
+            jump += 1
+            if jump > nvcycles_viscosity_jump:
+                jump = 1
+                max_cur = max_cur * max_factor
+                if max_cur >= max_o:
+                    overstep += 1
+                if max_cur > max_o:
+                    max_cur = max_o
+                if overstep > 1:
+                    max_cur = max_start
+                    overstep = 0
+                    for k in ("vx", "vy", "vz", "pr"):
+                        getattr(lev0, k)[...] += getattr(f0, k)
+                    for n in names:
+                        getattr(lev0, n)[...] = orig[0][n]
+                    self.level0_residual(lev0, pressure_bc)
+                    out = np.zeros(4)
+                    L.ebo_mg3_mean_residuals(C.byref(lev0.cs()), stokesscale, continscale, p(out))
+                    ghist.append(out)
+                    f0.zero_solution()
+                scale_all()
+            rmax = float(np.max(np.abs(10.0 ** ghist[-1]))) if len(ghist) > 1 else 1e38
+            if rmax < convergence_criterion:
+                converged = True
+                break
+        return len(hist), converged, np.array(hist).reshape(-1, 4), np.array(ghist).reshape(-1, 4)
+
     def run(self, nvcycles, convergence_criterion, nviscosity_jumps=12.0, nvcycles_viscosity_jump=15,
             resmax=1e-6, stokesscale=1.0, continscale=1.0, finest_sweeps=2, multiplier=2, cap=None,
             theta_s=0.9, theta_c=0.3, relax_v=1.0, relax_p=1.0, pressure_bc=0.0):

@@ -186,6 +186,27 @@ def test_run_matches_oracle(gpu_ctx, case, criterion, prnorm):
         G.run(-1, criterion)
 
 
+@pytest.mark.parametrize("case,criterion", [(CASES[0], 1e-8), (CASES[1], 1e-6)])
+def test_run_multimulti_matches_oracle(gpu_ctx, case, criterion):
+    """Multimulti driver: same cycle count, stop flag and global updates as
+    the oracle, principal and global residual histories within 1e-6 (log10
+    units; the global residuals are level 0's residuals, a difference of
+    nearly equal terms), global solution to 1e-9 relative."""
+    print(f"mg3d multimulti {case}", flush=True)
+    H, G = _run_pair(gpu_ctx, **case, decades=2.0)
+    kw = dict(RUN_KW, nviscosity_jumps=2.9999, nvcycles_viscosity_jump=3)
+    kw.pop("resmax")
+    n_o, c_o, h_o, g_o = H.run_multimulti(300, criterion, **kw)
+    n_g, c_g, h_g, g_g = G.run_multimulti(300, criterion, **kw)
+    print(f"  oracle {n_o} cycles, {len(g_o)} updates; device {n_g} cycles, {len(g_g)} updates", flush=True)
+    assert (n_g, c_g, len(g_g)) == (n_o, c_o, len(g_o)) and c_o and len(g_o) >= 2
+    np.testing.assert_allclose(h_g, h_o, rtol=0, atol=1e-6)
+    np.testing.assert_allclose(g_g, g_o, rtol=0, atol=1e-6)
+    for k in SOL:
+        got, exp = to_host(G.get(0, k)), getattr(H.level0, k)
+        assert np.linalg.norm(got - exp) <= 1e-9 * np.linalg.norm(exp), k
+
+
 def test_run_negative_coarse_viscosity_is_an_error(gpu_ctx):
     """Both the oracle and the device refuse the fractional power of a
     negative restricted viscosity (the reference raises DomainError)."""

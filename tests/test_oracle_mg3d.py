@@ -348,3 +348,28 @@ def test_run_converges_to_direct_solve():
         m = masks[k]
         err = np.linalg.norm(getattr(L0, k)[m] - xs[k][m]) / np.linalg.norm(xs[k][m])
         assert err < 1e-8, (k, err)
+
+
+def test_multimulti_schedule_and_direct_solution():
+    """Multimulti (use_multimulti = true): with one cycle per jump and 1.9999
+    jumps the global updates fall on cycles 3, 6, 9 (the maximum oversteps
+    the full value at each ramp's second jump, derived from the rule), the
+    run cannot stop before two updates; a converged run leaves the direct
+    solution of the full-contrast system in level 0."""
+    kw = dict(nviscosity_jumps=1.9999, nvcycles_viscosity_jump=1, finest_sweeps=2)
+    for cycles, updates in ((2, 0), (3, 1), (6, 2), (9, 3)):
+        H = _hierarchy(seed=3)
+        n, conv, hist, gh = H.run_multimulti(cycles, 0.0, **kw)
+        assert (n, conv, gh.shape) == (cycles, False, (updates, 4)), (cycles, gh.shape)
+    H = _hierarchy(seed=3)
+    n, conv, _, gh = H.run_multimulti(50, 1e30, **kw)
+    assert (n, conv, gh.shape[0]) == (6, True, 2)
+    H = _hierarchy(seed=5)
+    xs, masks = _direct_solution(H.levels[0])
+    n, conv, hist, gh = H.run_multimulti(600, 1e-11, nviscosity_jumps=2.9999, nvcycles_viscosity_jump=3,
+                                         finest_sweeps=3)
+    assert conv and n < 600 and gh.shape[0] >= 2
+    for k in ("vx", "vy", "vz", "pr"):
+        m = masks[k]
+        err = np.linalg.norm(getattr(H.level0, k)[m] - xs[k][m]) / np.linalg.norm(xs[k][m])
+        assert err < 1e-8, (k, err)
