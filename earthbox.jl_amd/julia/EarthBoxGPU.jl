@@ -540,14 +540,20 @@ end
     run_multigrid_solver_gpu!(st, multigrid_data) -> (cycles, converged)
 
 GPU replacement of MultigridSolver.run_multigrid_solver for MultigridData3d
-with use_multimulti = false (multigrid_solver/MultigridSolver.jl:113-135).
-Uploads level_vector[1] (viscosity, right-hand sides, initial solution), runs
-the continuation and V-cycles on the device, and writes back the solution, the
-scaled viscosity and the principal mean residuals, as the reference leaves them.
+(multigrid_solver/MultigridSolver.jl:113-135).  Uploads level_vector[1]
+(viscosity, right-hand sides, initial solution), runs the continuation and
+V-cycles on the device, and writes back the solution, the scaled viscosity
+and the principal mean residuals, as the reference leaves them.  With
+use_multimulti the initial guess and right-hand sides come from level0, the
+device runs the multimulti driver (ebg_mg3d_run_multimulti) and the global
+solution and global mean residuals return to level0 and mean_residuals.
 """
 function run_multigrid_solver_gpu!(st::GPUState, md)
-    md.vcycle.use_multimulti && error("run_multigrid_solver_gpu!: multimulti is not supported")
+    mm = md.vcycle.use_multimulti
     lv = md.level_vector[1]
+    # with multimulti the initial guess and right-hand sides live on level 0
+    # (MultigridDataManager.jl:376-421); the viscosity is level 1's
+    src = mm ? md.level0 : lv
     geo = lv.grid.parameters.geometry
     nlev = Ref{Cint}(0)
     check(ccall((:ebg_mg3d_create, LIB), Cint,
@@ -558,8 +564,9 @@ function run_multigrid_solver_gpu!(st::GPUState, md)
     copy_field(id, dst, src) = check(ccall((:ebg_mg3d_copy_field, LIB), Cint,
         (Ptr{Cvoid}, Cint, Cint, Ptr{Cvoid}, Ptr{Cvoid}), st.ctx, 0, id, dst, src), "ebg_mg3d_copy_field")
     for (name, id) in pairs(ids)
-        hasproperty(lv, name) || continue
-        copy_field(id, C_NULL, h2d!(st, Symbol(:mg3_, name), getproperty(lv, name).array))
+        from = name in (:etan, :etaxy, :etaxz, :etayz) ? lv : src
+        hasproperty(from, name) || continue
+        copy_field(id, C_NULL, h2d!(st, Symbol(:mg3_, name), getproperty(from, name).array))
     end
     s = md.smoothing_iterations
     cap = length(s) > 1 ? maximum(s[2:end]) : 0
@@ -572,10 +579,20 @@ function run_multigrid_solver_gpu!(st::GPUState, md)
                         md.pressure_bc))
     cycles, conv = Ref{Cint}(0), Ref{Cint}(0)
     hist = zeros(Float64, 4 * max(md.vcycle.nvcycles, 1))
-    check(ccall((:ebg_mg3d_run, LIB), Cint, (Ptr{Cvoid}, Ref{MGRunParams}, Ref{Cint}, Ref{Cint}, Ptr{Float64}),
-                st.ctx, p, cycles, conv, hist), "ebg_mg3d_run")
+    nglobal = Ref{Cint}(0)
+    ghist = zeros(Float64, 4 * max(md.vcycle.nvcycles, 1))
+    if mm
+        check(ccall((:ebg_mg3d_run_multimulti, LIB), Cint,
+            (Ptr{Cvoid}, Ref{MGRunParams}, Ref{Cint}, Ref{Cint}, Ptr{Float64}, Ref{Cint}, Ptr{Float64}, Cint),
+            st.ctx, p, cycles, conv, hist, nglobal, ghist, max(md.vcycle.nvcycles, 1)), "ebg_mg3d_run_multimulti")
+    else
+        check(ccall((:ebg_mg3d_run, LIB), Cint, (Ptr{Cvoid}, Ref{MGRunParams}, Ref{Cint}, Ref{Cint}, Ptr{Float64}),
+                    st.ctx, p, cycles, conv, hist), "ebg_mg3d_run")
+    end
+    # the solution returns to level 1, or with multimulti to level 0 (the
+    # global solution, which the reference reports)
     for name in (:vx, :vy, :vz, :pr, :etan, :etaxy, :etaxz, :etayz)
-        a = getproperty(lv, name).array
+        a = getproperty(name in (:vx, :vy, :vz, :pr) ? src : lv, name).array
         dp = dev!(st, Symbol(:mg3_, name), a)
         copy_field(ids[name], dp, C_NULL)
         check(ccall((:ebg_ctx_synchronize, LIB), Cint, (Ptr{Cvoid},), st.ctx), "sync")
@@ -587,6 +604,12 @@ function run_multigrid_solver_gpu!(st::GPUState, md)
         mr.resy_principle[c] = hist[4c - 2]
         mr.resz_principle[c] = hist[4c - 1]
         mr.resc_principle[c] = hist[4c]
+    end
+    for c in 1:nglobal[]
+        mr.resx_global[c] = ghist[4c - 3]
+        mr.resy_global[c] = ghist[4c - 2]
+        mr.resz_global[c] = ghist[4c - 1]
+        mr.resc_global[c] = ghist[4c]
     end
     return Int(cycles[]), conv[] != 0
 end
