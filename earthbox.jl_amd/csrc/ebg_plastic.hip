@@ -9,6 +9,7 @@
 // yield count and the sum of squared yield errors are reduced in a fixed
 // order (per-block partials, then one block), so repeated calls agree.
 #include "ebg_common.hpp"
+#include "ebg_yield.hpp"
 
 #include <algorithm>
 #include <mutex>
@@ -31,20 +32,6 @@ struct PlArgs {
     long long* partial_count;  // [nblocks]
 };
 
-// Julia's max / min on Float64: NaN wins and +0.0 > -0.0
-__device__ __forceinline__ double jl_max(double a, double b)
-{
-    if (a != a || b != b) return a + b;
-    if (a == b) return signbit(a) ? b : a;
-    return a > b ? a : b;
-}
-__device__ __forceinline__ double jl_min(double a, double b)
-{
-    if (a != a || b != b) return a + b;
-    if (a == b) return signbit(a) ? a : b;
-    return a < b ? a : b;
-}
-
 // PlasticFailureNodes.jl:206-248 (interpolate_from_pressure_grid_to_basic_grid)
 __device__ double pressure_to_basic(int i, int j, int ynum, int xnum, const double* __restrict__ a)
 {
@@ -61,24 +48,6 @@ __device__ double pressure_to_basic(int i, int j, int ynum, int xnum, const doub
     if (i == ynum) return (PA(i - 1, j) + PA(i - 1, j - 1)) / 2.0;
 #undef PA
     return 0.0;
-}
-
-// YieldStress.calc_yield_stress: Drucker-Prager, optionally on the pressure
-// above the hydrostatic fluid pressure (density 1000, g 9.81) with the
-// tension cut-off cohesion * cos(phi) (stress_yield_tension_method2)
-__device__ double yield_stress(double cohesion, double sine_fric, double pressure, int use_fluid, double y,
-                               double y_sealevel)
-{
-    const double cosphi = cos(asin(sine_fric));
-    double s;
-    if (use_fluid == 0) {
-        s = cohesion * cosphi + sine_fric * pressure;
-    } else {
-        const double pf = 1000.0 * 9.81 * jl_max(0.0, y - y_sealevel);
-        if (pressure < pf) s = cohesion * cosphi;
-        else s = cohesion * cosphi + sine_fric * (pressure - pf);
-    }
-    return jl_max(s, 0.0);
 }
 
 __global__ void __launch_bounds__(PL_THREADS) plastic_nodes_kernel(PlArgs a)

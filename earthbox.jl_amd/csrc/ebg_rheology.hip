@@ -10,6 +10,7 @@
 // operand order (the library is compiled without contraction); exp and pow
 // are the device math library's.
 #include "ebg_common.hpp"
+#include "ebg_yield.hpp"
 
 #include <cmath>
 
@@ -314,6 +315,64 @@ __global__ void __launch_bounds__(256) marker_strain_kernel(StrainArgs a)
 
 unsigned blocks256(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+// MarkerPlasticity.update_plastic_yielding! (rheology/plastic_failure/markers/):
+// ViscoelasticForecast.jl:14-127 (forecast 0) and ElasticForecast.jl:14-110
+// (forecast 1) with YieldCheck.jl, Limits.jl, CheckYield.jl:3-12 and
+// ViscoelasticFactor.jl:19-30.  The squares of the reference's x^2.0 are
+// products.
+__global__ void __launch_bounds__(256) marker_plastic_yield_kernel(ebg_marker_plastic_in in,
+                                                                   ebg_marker_plastic_out out)
+{
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= in.marknum || in.inside_flags[m] != 1) return;
+    const int id = in.matid[m];
+    const double cohesion = in.cohesion[m], sfric = in.fric[m];
+    int flag = 0;
+    if (in.ntimestep > 0 && (cohesion > 0.0 || sfric > 0.0)) {
+        const double eta = out.eta[m];
+        const double mu = (id >= 1 && id <= in.nmat) ? in.mat_mu[id - 1] : __builtin_nan("");
+        const double sxx = out.sxx[m], sxy = out.sxy[m];
+        const double exx = in.exx[m], exy = in.exy[m], ratio = in.sr_ratio[m];
+        const double dt = in.timestep;
+        double sy = yield_stress(cohesion, sfric, in.pr[m], in.iuse_fluid_pressure_for_yield, in.y[m], in.y_sealevel);
+        sy = jl_min(jl_max(sy, in.yield_stress_min), in.yield_stress_max);
+        double fxx, fxy;
+        if (in.forecast == 0) {
+            const double z = eta / (eta + dt * mu);
+            fxx = sxx * z + 2.0 * eta * exx * ratio * (1.0 - z);
+            fxy = sxy * z + 2.0 * eta * exy * ratio * (1.0 - z);
+        } else {
+            fxx = sxx + 2.0 * mu * dt * exx * ratio;
+            fxy = sxy + 2.0 * mu * dt * exy * ratio;
+        }
+        const double sinv = sqrt(fxx * fxx + fxy * fxy);
+        bool yielding = sinv > sy;
+        if (yielding && in.no_yielding_in_mobile_wall &&
+            (id == in.matid_sticky_air || id == in.matid_mobile_wall || id == in.matid_plate_extension))
+            yielding = false;
+        if (yielding) {
+            flag = 1;
+            double eta_new;
+            if (in.forecast == 0) {
+                const double sold = sqrt(sxx * sxx + sxy * sxy);
+                if (sold != 0.0) {
+                    out.sxx[m] = sxx * sy / sold;
+                    out.sxy[m] = sxy * sy / sold;
+                } else {
+                    out.sxx[m] = sy;
+                    out.sxy[m] = sy;
+                }
+                const double eii = ratio * sqrt(exx * exx + exy * exy);
+                eta_new = sy / 2.0 / eii;
+            } else {
+                eta_new = mu * dt * sy / (sinv - sy);
+            }
+            out.eta[m] = jl_min(jl_max(eta_new, in.viscosity_min), in.viscosity_max);
+        }
+    }
+    if (out.pfailure[m] != 1.0f) out.pfailure[m] = (float)flag;
+}
+
 }  // namespace
 }  // namespace ebg
 
@@ -410,6 +469,27 @@ extern "C" int ebg_update_marker_strain(ebg_ctx* ctx, int64_t marknum, const int
                  marker_sxy, marker_eta, marker_eta_flow, marker_pfailure, marker_GII, marker_strain_plastic,
                  marker_strain_rate_plastic};
     hipLaunchKernelGGL(marker_strain_kernel, dim3(blocks256(marknum)), dim3(256), 0, ctx->stream, a);
+    EBG_HIP(hipGetLastError());
+    return EBG_OK;
+}
+
+extern "C" int ebg_update_marker_plastic_yielding(ebg_ctx* ctx, const ebg_marker_plastic_in* in,
+                                                  ebg_marker_plastic_out* out)
+{
+    EBG_CHECK(ctx != nullptr && in != nullptr && out != nullptr, EBG_ERR_ARG,
+              "ebg_update_marker_plastic_yielding: NULL arguments");
+    EBG_CHECK(in->marknum >= 0, EBG_ERR_ARG, "ebg_update_marker_plastic_yielding: marknum < 0");
+    EBG_CHECK(in->forecast == 0 || in->forecast == 1, EBG_ERR_ARG,
+              "ebg_update_marker_plastic_yielding: forecast must be 0 (viscoelastic) or 1 (elastic)");
+    if (in->marknum == 0) return EBG_OK;
+    EBG_CHECK(in->inside_flags && in->y && in->matid && in->mat_mu && in->nmat >= 1 && in->cohesion && in->fric &&
+                  in->exx && in->exy && in->sr_ratio && in->pr,
+              EBG_ERR_ARG, "ebg_update_marker_plastic_yielding: NULL input array");
+    EBG_CHECK(out->sxx && out->sxy && out->eta && out->pfailure, EBG_ERR_ARG,
+              "ebg_update_marker_plastic_yielding: NULL output array");
+    EBG_HIP(hipSetDevice(ctx->device));
+    hipLaunchKernelGGL(marker_plastic_yield_kernel, dim3(blocks256(in->marknum)), dim3(256), 0, ctx->stream, *in,
+                       *out);
     EBG_HIP(hipGetLastError());
     return EBG_OK;
 }

@@ -44,6 +44,7 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_update_nodes_for_plastic_yielding": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_update_nodes_for_plastic_yielding_noalloc": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_update_marker_rheology": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ebg_update_marker_plastic_yielding": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ebg_calc_marker_viscoplastic_viscosity": (c_int, [c_void_p, c_int64] + [c_void_p] * 10),
     "ebg_update_marker_strain": (c_int, [c_void_p, c_int64, c_void_p, c_double, c_double] + [c_void_p] * 10),
     "ebg_grid_strain_rate_and_stress": (c_int, [c_void_p, c_void_p, c_void_p]),
@@ -191,6 +192,22 @@ class MarkerRheologyIn(C.Structure):
 class MarkerRheologyOut(C.Structure):
     _fields_ = [("cohesion", c_void_p), ("fric", c_void_p), ("eta_flow", c_void_p), ("preexp", c_void_p),
                 ("eta", c_void_p)]
+
+
+class MarkerPlasticIn(C.Structure):
+    """ebg_marker_plastic_in."""
+    _fields_ = [("marknum", c_int64), ("inside_flags", c_void_p), ("y", c_void_p), ("matid", c_void_p),
+                ("mat_mu", c_void_p), ("nmat", c_int64), ("cohesion", c_void_p), ("fric", c_void_p),
+                ("exx", c_void_p), ("exy", c_void_p), ("sr_ratio", c_void_p), ("pr", c_void_p),
+                ("forecast", C.c_int32), ("ntimestep", C.c_int32), ("timestep", c_double), ("y_sealevel", c_double),
+                ("viscosity_min", c_double), ("viscosity_max", c_double), ("yield_stress_min", c_double),
+                ("yield_stress_max", c_double), ("iuse_fluid_pressure_for_yield", C.c_int32),
+                ("no_yielding_in_mobile_wall", C.c_int32), ("matid_sticky_air", C.c_int32),
+                ("matid_mobile_wall", C.c_int32), ("matid_plate_extension", C.c_int32)]
+
+
+class MarkerPlasticOut(C.Structure):
+    _fields_ = [("sxx", c_void_p), ("sxy", c_void_p), ("eta", c_void_p), ("pfailure", c_void_p)]
 
 
 class MarkerHeatProps(C.Structure):

@@ -9,6 +9,18 @@ stresses, change norms), updates the nodes for plastic yielding, copies the
 new viscosity, yield flags and plastic strain rate back when nodes yield, and
 stops by LoopTermination.check_stop_global_loop_nodes
 (global_plasticity_loop/utils/LoopTermination.jl:16-31).
+
+picard_loop_markers mirrors PlasticityLoopMarkers.picard_loop
+(plasticity_loop/plasticity_loop_markers/PlasticityLoopMarkers.jl:36-75):
+after the first iteration the markers yield
+(MarkerPlasticity.update_plastic_yielding!), their viscosity alone is
+scattered back to etas1 and etan1 with the previous values as backups
+(BackupTransportArrays.backup_and_clear_viscosity_transport_arrays!,
+MarkersToStokesArrays.interpolate_viscosity_to_transport_arrays!) and the
+harmonic normal viscosity follows; each solve is processed on the grid and
+gathered to the markers (strain rate, strain-rate ratio, pressure:
+StokesContinuitySolver.execute_post_stokes_solver_steps!), and the loop stops
+by LoopTermination.check_stop_global_loop_markers (LoopTermination.jl:3-14).
 """
 from __future__ import annotations
 
@@ -19,6 +31,9 @@ import numpy as np
 import torch
 
 from .device import empty
+from .marker_gather import OUTPUTS as GATHER_OUTPUTS, update_markers_after_stokes
+from .rheology import VISCOELASTIC_FORECAST, update_marker_plastic_yielding
+from .scatter import interpolate_marker_stokes_parameters_to_grid
 from .plastic import (interpolate_basic_viscoplastic_viscosity_to_pressure_grid,
                       plastic_strain_rate_at_pressure_nodes, update_nodes_for_plastic_yielding)
 from .stokes import StokesSolverGPU, bc_mirror_factors, calculate_new_timestep, change_norms
@@ -115,6 +130,92 @@ def picard_loop_nodes(ctx, template: StokesProblem, state: dict, plastic_params:
         sol = {"vx1": vx1, "vy1": vy1, "pr1": pr1, "S": S, "timestep": dt, **ss}
         if check_stop_global_loop_nodes(nnode_yield, iglobal, nglobal, global_yield_error, dvxy_rel_L2,
                                         tolerance):
+            break
+    torch.cuda.synchronize()
+    return sol, log
+
+
+def check_stop_global_loop_markers(iglobal, nglobal, convergence_criterion, tolerance) -> bool:
+    """LoopTermination.check_stop_global_loop_markers (LoopTermination.jl:3-14)."""
+    return iglobal == nglobal or convergence_criterion < tolerance
+
+
+MARKERS = ("x", "y", "inside_flags", "marker_xn", "marker_yn", "matid", "eta", "sxx", "sxy", "exx", "exy",
+           "sr_ratio", "pr", "cohesion", "fric", "pfailure")
+
+
+def picard_loop_markers(ctx, template: StokesProblem, state: dict, markers: dict, mat_mu, plastic_params: dict,
+                        nglobal: int, tolerance: float, gravity_x: float = 0.0,
+                        iuse_harmonic_avg_normal_viscosity: int = 1, forecast: int = VISCOELASTIC_FORECAST,
+                        matid_domains: tuple = (-1, -1, -1), solve_opts: dict | None = None,
+                        solver: StokesSolverGPU | None = None, velocity_old: tuple | None = None,
+                        iupdate_timestep: int = 0, marker_cell_displ_max: float = 0.5):
+    """template, state, solve_opts, solver, velocity_old, iupdate_timestep:
+    as picard_loop_nodes (state needs etas1, etan1, mus1, mun1, sxy1, sxx1,
+    rho1, rho1_vy and, for the harmonic normal viscosity, plastics, plasticn,
+    plastic_yield).  markers: device vectors keyed like MARKERS (marker_xn,
+    marker_yn as the scatter before the loop stored them); eta, sxx, sxy and
+    pfailure change where markers yield, and exx, exy, sr_ratio, pr hold the
+    last solve's gathers on return (on entry: the previous time step's, which
+    the first yielding reads).  mat_mu: device shear moduli by material.
+    plastic_params: the scalars of rheology.update_marker_plastic_yielding
+    (timestep included; ntimestep > 0 lets markers yield).  matid_domains:
+    (sticky air, mobile wall, plate extension) ids, used when
+    plastic_params["no_yielding_in_mobile_wall"] is 1.
+    Returns (solution dict as picard_loop_nodes; per-iteration log with the
+    number of markers whose pfailure is 1 after the yielding step, None on
+    the first iteration)."""
+    g = template.grid
+    solver = solver or StokesSolverGPU(ctx)
+    opts = dict(precision=1, rtol=1e-10, max_iterations=4000, bc_d=bc_mirror_factors(template))
+    opts.update(solve_opts or {})
+    dt = float(plastic_params["timestep"])
+    plastic_params = dict(plastic_params)
+    n = int(markers["x"].numel())
+    vx_old, vy_old = velocity_old if velocity_old is not None else (None, None)
+    scatter_markers = {k: markers[k] for k in ("x", "y", "inside_flags", "eta")}
+    gather_markers = {k: markers[k] for k in ("x", "y", "marker_xn", "marker_yn", "inside_flags", "eta", "matid")}
+    gather_markers["mat_mu"] = mat_mu
+    gather_out = {"marker_exx": markers["exx"], "marker_exy": markers["exy"], "marker_pr": markers["pr"],
+                  "marker_sr_ratio": markers["sr_ratio"]}
+    log = []
+    sol = {}
+    for iglobal in range(1, nglobal + 1):
+        nfailed = None
+        if iglobal > 1:
+            plastic_params["timestep"] = dt
+            update_marker_plastic_yielding(ctx, markers, mat_mu, plastic_params, forecast, *matid_domains)
+            nfailed = int((markers["pfailure"] == 1).sum().item()) if n else 0
+            # etas1/etan1 enter as the backups kept where a node receives no weight
+            interpolate_marker_stokes_parameters_to_grid(ctx, scatter_markers,
+                                                         {"etas1": state["etas1"], "etan1": state["etan1"]})
+            interpolate_basic_viscoplastic_viscosity_to_pressure_grid(
+                ctx, state["etas1"], state["plastics"], state["plasticn"], state["plastic_yield"], state["etan1"],
+                iuse_harmonic_avg_normal_viscosity)
+        ve = calc_viscoelastic_terms_and_rhs(ctx, state, dt, gravity_x, template.gravity_y,
+                                             template.iuse_interface_stabilization)
+        etan11 = float(ve["etan0"][0, 0].item())
+        prob = dataclasses.replace(template, pscale=2.0 * etan11 / (g.xstpavg + g.ystpavg), timestep=dt)
+        t0 = time.perf_counter()
+        solver.set_system(prob, fields={"etas0": ve["etas0"], "etan0": ve["etan0"], "rho1_vy": state["rho1_vy"],
+                                        "RX1": ve["RX1"], "RY1": ve["RY1"], "RC1": ve["RC1"]})
+        S, st = solver.solve_system(**opts)
+        solve_seconds = time.perf_counter() - t0
+        vx1, vy1, pr1 = solver.process_stokes_solution(S)
+        # stokes_solution_processing!: time step limit, grid stresses, marker gathers, norms
+        if iupdate_timestep == 1:
+            dt, _ = calculate_new_timestep(ctx, vx1, vy1, dt, marker_cell_displ_max)
+        ss = update_grid_strain_rate_and_stress(ctx, {"vx1": vx1, "vy1": vy1, **state}, dt)
+        fields = {"exx": ss["exx"], "exy": ss["exy"], "pr1": pr1, "sxx2": ss["sxx2"], "sxy2": ss["sxy2"],
+                  "dsxx": ss["dsxx"], "dsxy": ss["dsxy"]}
+        update_markers_after_stokes(ctx, gather_markers, fields, dt, gather_out, GATHER_OUTPUTS)
+        dvxy_rel_L2 = change_norms(ctx, [vx1, vy1], [vx_old, vy_old])[1]
+        vx_old, vy_old = vx1.clone(), vy1.clone()
+        log.append({"iglobal": iglobal, "iterations": st["iterations"], "relres": st["relres"],
+                    "nmarker_failed": nfailed, "dvxy_rel_L2": dvxy_rel_L2, "timestep": dt,
+                    "solve_seconds_with_setup": solve_seconds, "learned_accepted": st["learned_accepted"]})
+        sol = {"vx1": vx1, "vy1": vy1, "pr1": pr1, "S": S, "timestep": dt, **ss}
+        if check_stop_global_loop_markers(iglobal, nglobal, dvxy_rel_L2, tolerance):
             break
     torch.cuda.synchronize()
     return sol, log

@@ -4,7 +4,9 @@ update_marker_rheology mirrors Rheology.update_marker_rheology! (failure
 properties with strain weakening, FailureProperties.jl; composite-creep flow
 viscosity, CompositeViscosity.jl) and Rheology.update_marker_viscoplastic_viscosity!
 (ViscoplasticViscosity.jl:21-61), rheology/Rheology.jl:36-80.
-calc_marker_viscoplastic_viscosity mirrors
+update_marker_plastic_yielding mirrors MarkerPlasticity.update_plastic_yielding!
+(rheology/plastic_failure/markers/MarkerPlasticity.jl:34-43), the marker step
+of the Picard loop over markers.  calc_marker_viscoplastic_viscosity mirrors
 GridViscoplasticityToMarkers.calc_marker_viscoplastic_viscosity!, and
 update_marker_strain mirrors Advection.update_marker_strain!
 (advection/Advection.jl:169-178, MarkersStrain.jl).
@@ -15,7 +17,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import MarkerRheologyIn, MarkerRheologyOut, check, lib
+from ._lib import MarkerPlasticIn, MarkerPlasticOut, MarkerRheologyIn, MarkerRheologyOut, check, lib
 from .context import GPUContext
 from .device import ptr
 
@@ -54,6 +56,41 @@ def update_marker_rheology(ctx: GPUContext, markers: dict, mats: dict, params: d
         setattr(o, k, ptr(out.get(k), np.float64, n))
     check(lib.ebg_update_marker_rheology(ctx.handle, C.byref(a), C.byref(o)), "ebg_update_marker_rheology")
     return out
+
+
+PLASTIC_INPUTS = ("y", "cohesion", "fric", "exx", "exy", "sr_ratio", "pr")
+PLASTIC_SCALARS = ("ntimestep", "timestep", "y_sealevel", "viscosity_min", "viscosity_max", "yield_stress_min",
+                   "yield_stress_max", "iuse_fluid_pressure_for_yield", "no_yielding_in_mobile_wall")
+# itype_plasticity option ids (plastic_failure/markers/options/Options.jl)
+VISCOELASTIC_FORECAST, ELASTIC_FORECAST = 0, 1
+
+
+def update_marker_plastic_yielding(ctx: GPUContext, markers: dict, mat_mu, params: dict,
+                                   forecast: int = VISCOELASTIC_FORECAST, matid_sticky_air: int = -1,
+                                   matid_mobile_wall: int = -1, matid_plate_extension: int = -1) -> None:
+    """markers: device inside_flags (int8), matid (int16) and the PLASTIC_INPUTS
+    vectors; sxx, sxy, eta and pfailure (float32) updated in place.  mat_mu:
+    device shear moduli (nmat); params: PLASTIC_SCALARS.  Matching the
+    reference, only the viscoelastic forecast (0) corrects the stresses."""
+    n = int(markers["matid"].numel())
+    nmat = int(mat_mu.numel())
+    a = MarkerPlasticIn()
+    a.marknum = n
+    a.inside_flags = ptr(markers["inside_flags"], np.int8, n)
+    a.matid = ptr(markers["matid"], np.int16, n)
+    a.mat_mu = ptr(mat_mu, np.float64, nmat)
+    a.nmat = nmat
+    for k in PLASTIC_INPUTS:
+        setattr(a, k, ptr(markers[k], np.float64, n))
+    for k in PLASTIC_SCALARS:
+        setattr(a, k, params[k])
+    a.forecast = int(forecast)
+    a.matid_sticky_air, a.matid_mobile_wall, a.matid_plate_extension = (
+        int(matid_sticky_air), int(matid_mobile_wall), int(matid_plate_extension))
+    o = MarkerPlasticOut(ptr(markers["sxx"], np.float64, n), ptr(markers["sxy"], np.float64, n),
+                         ptr(markers["eta"], np.float64, n), ptr(markers["pfailure"], np.float32, n))
+    check(lib.ebg_update_marker_plastic_yielding(ctx.handle, C.byref(a), C.byref(o)),
+          "ebg_update_marker_plastic_yielding")
 
 
 def calc_marker_viscoplastic_viscosity(ctx: GPUContext, markers: dict, plastic_yield, etas1) -> None:

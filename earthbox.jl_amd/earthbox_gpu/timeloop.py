@@ -1,11 +1,11 @@
-"""Model time steps on device arrays for node-based plasticity, in the order of
-the reference's time loop (time_loop/TimeLoop.jl:163-273), restricted to the
-hot path: marker rheology, marker -> grid scatter, the Picard loop over nodes
-with the Stokes solve, grid -> marker viscoplastic viscosity, the
-displacement-limited time step, the post-solve grid and marker updates, RK4
-advection with stress rotation and the marker strain update.  Everything stays
-on the device between steps; only scalars (solver statistics, yield counts,
-the time step) come back to the host.
+"""Model time steps on device arrays, in the order of the reference's time
+loop (time_loop/TimeLoop.jl:163-273), restricted to the hot path: marker
+rheology, marker -> grid scatter, the Picard loop over nodes (itype_global 1;
+over markers with 0) with the Stokes solve, grid -> marker viscoplastic
+viscosity (node-based plasticity only), the displacement-limited time step,
+the post-solve grid and marker updates, RK4 advection with stress rotation and
+the marker strain update.  Everything stays on the device between steps; only
+scalars (solver statistics, yield counts, the time step) come back to the host.
 
 With thermal inputs (upload's thermal argument) the step also runs the heat
 path: the heat-field scatter with the thermal boundary conditions before the
@@ -29,7 +29,7 @@ import torch
 from . import heat, rheology
 from .device import empty, to_device
 from .marker_gather import OUTPUTS as GATHER_OUTPUTS, STRESS as GATHER_STRESS, update_markers_after_stokes
-from .picard import picard_loop_nodes
+from .picard import picard_loop_markers, picard_loop_nodes
 from .plastic import interpolate_basic_viscoplastic_viscosity_to_pressure_grid
 from .scatter import interpolate_marker_stokes_parameters_to_grid, new_grid_fields
 from .stokes import StokesSolverGPU, calculate_new_timestep
@@ -121,11 +121,23 @@ def time_step(ctx, model: DeviceModel, solve_opts: dict | None = None, rk4_arith
     plastic_params = dict(timestep=model.timestep, timesum=model.timesum, ntimestep=step,
                           viscosity_min=p["viscosity_min"], viscosity_max=p["viscosity_max"],
                           iuse_fluid_pressure_for_yield=p.get("iuse_fluid_pressure_for_yield", 0))
-    sol, plog = picard_loop_nodes(ctx, model.template, gf, plastic_params, int(p["nglobal"]),
-                                  float(p["tolerance_picard"]), solve_opts=solve_opts, solver=model.solver,
-                                  velocity_old=model.velocity_old)
+    node_based = int(p.get("itype_global", 1)) == 1
+    if node_based:
+        sol, plog = picard_loop_nodes(ctx, model.template, gf, plastic_params, int(p["nglobal"]),
+                                      float(p["tolerance_picard"]), solve_opts=solve_opts, solver=model.solver,
+                                      velocity_old=model.velocity_old)
+    else:   # PlasticityLoopMarkers.picard_loop (itype_global 0)
+        mpar = {k: p.get(k, d) for k, d in (("y_sealevel", 0.0), ("yield_stress_min", 0.0),
+                                             ("yield_stress_max", 1e32), ("no_yielding_in_mobile_wall", 0))}
+        mpar.update(plastic_params)
+        sol, plog = picard_loop_markers(ctx, model.template, gf, m, m["mat_mu"], mpar, int(p["nglobal"]),
+                                        float(p["tolerance_picard"]), forecast=int(p.get("itype_plasticity", 0)),
+                                        matid_domains=tuple(int(p.get(k, -1)) for k in (
+                                            "matid_sticky_air", "matid_mobile_wall", "matid_plate_extension")),
+                                        solve_opts=solve_opts, solver=model.solver, velocity_old=model.velocity_old)
     vx1, vy1, pr1 = sol["vx1"], sol["vy1"], sol["pr1"]
-    rheology.calc_marker_viscoplastic_viscosity(ctx, m, gf["plastic_yield"], gf["etas1"])
+    if node_based:   # interpolate_grid_viscoplastic_viscosity_to_markers! (node-based plasticity only)
+        rheology.calc_marker_viscoplastic_viscosity(ctx, m, gf["plastic_yield"], gf["etas1"])
     dt, _ = calculate_new_timestep(ctx, vx1, vy1, model.timestep, p["marker_cell_displ_max"])
     # execute_post_stokes_solver_steps! with the new step, then the marker stress update
     ss = update_grid_strain_rate_and_stress(ctx, {"vx1": vx1, "vy1": vy1, **gf}, dt)
@@ -169,6 +181,8 @@ def time_step(ctx, model: DeviceModel, solve_opts: dict | None = None, rk4_arith
     return {"ntimestep": model.ntimestep, "timestep": dt, "seconds": time.perf_counter() - t_step,
             "picard_iterations": len(plog), "krylov_iterations": [r["iterations"] for r in plog],
             "solve_seconds_with_setup": [r["solve_seconds_with_setup"] for r in plog],
-            "max_relres": max(r["relres"] for r in plog), "nnode_yield": [r["nnode_yield"] for r in plog],
+            "max_relres": max(r["relres"] for r in plog),
+            "nnode_yield": [r.get("nnode_yield") for r in plog],
+            "nmarker_failed": [r.get("nmarker_failed") for r in plog],
             "learned_accepted": sum(r["learned_accepted"] for r in plog),
             "final_dvxy_rel_L2": plog[-1]["dvxy_rel_L2"], "heat": hlog}

@@ -23,6 +23,13 @@ EarthBox.jl.  It keeps the reference's entry points as the seams:
                                         HeatSolver.apply_sticky_temperature_correction!
 - `update_subgrid_stress_gpu!(model, inside_flags)` beside SubGridStress.Update.update_subgrid_stress!
                                         (subgrid_stress/update/Update.jl:55-71)
+- `update_plastic_yielding_gpu!(model, inside_flags, no_yielding_in_mobile_wall)` beside
+                                        MarkerPlasticity.update_plastic_yielding!
+                                        (rheology/plastic_failure/markers/MarkerPlasticity.jl:34-43)
+- `interpolate_viscosity_to_transport_arrays_gpu!(model, inside_flags)` beside
+                                        backup_and_clear_viscosity_transport_arrays! +
+                                        MarkersToStokesArrays.interpolate_viscosity_to_transport_arrays!
+                                        (PlasticityLoopMarkers.jl:50-52)
 
 Every call copies the needed ModelData arrays to device buffers owned by a
 `GPUState`, runs the kernels and copies results back, so ModelData stays the
@@ -858,6 +865,100 @@ function update_subgrid_stress_gpu!(st::GPUState, model, inside_flags::Vector{In
     for (k, x) in zip(keys, outs)
         d2h!(x, st.buffers[k])
     end
+    return nothing
+end
+
+"Field order of ebg_marker_plastic_in / ebg_marker_plastic_out."
+struct MarkerPlasticIn
+    marknum::Int64
+    inside_flags::Ptr{Cvoid}; y::Ptr{Cvoid}; matid::Ptr{Cvoid}; mat_mu::Ptr{Cvoid}
+    nmat::Int64
+    cohesion::Ptr{Cvoid}; fric::Ptr{Cvoid}; exx::Ptr{Cvoid}; exy::Ptr{Cvoid}; sr_ratio::Ptr{Cvoid}; pr::Ptr{Cvoid}
+    forecast::Int32; ntimestep::Int32
+    timestep::Cdouble; y_sealevel::Cdouble
+    viscosity_min::Cdouble; viscosity_max::Cdouble; yield_stress_min::Cdouble; yield_stress_max::Cdouble
+    iuse_fluid_pressure_for_yield::Int32; no_yielding_in_mobile_wall::Int32
+    matid_sticky_air::Int32; matid_mobile_wall::Int32; matid_plate_extension::Int32
+end
+struct MarkerPlasticOut
+    sxx::Ptr{Cvoid}; sxy::Ptr{Cvoid}; eta::Ptr{Cvoid}; pfailure::Ptr{Cvoid}
+end
+
+"""
+    update_plastic_yielding_gpu!(st, model, inside_flags, no_yielding_in_mobile_wall)
+
+GPU replacement of MarkerPlasticity.update_plastic_yielding!
+(rheology/plastic_failure/markers/MarkerPlasticity.jl:34-43), the marker step
+of PlasticityLoopMarkers.picard_loop (PlasticityLoopMarkers.jl:51): the
+viscoelastic (itype_plasticity 0) or purely elastic (1) stress forecast,
+the yield test and the corrected marker viscosity (and, for the viscoelastic
+forecast, stresses); marker_pfailure keeps 1 once set.
+"""
+function update_plastic_yielding_gpu!(st::GPUState, model, inside_flags::Vector{Int8},
+                                      no_yielding_in_mobile_wall::Bool)
+    m = model.markers.arrays
+    mp = model.materials.parameters
+    tl = model.timestep.parameters.main_time_loop
+    mat_mu = model.materials.arrays.mat_mu.array
+    dom = model.materials.dicts.matid_domains
+    n = model.markers.parameters.distribution.marknum.value
+    up(k, x) = h2d!(st, k, x)
+    pin = MarkerPlasticIn(n, up(:flags, inside_flags), up(:my, m.location.marker_y.array),
+        up(:matid, m.material.marker_matid.array), up(:mat_mu, mat_mu), length(mat_mu),
+        up(:mcoh, m.rheology.marker_cohesion.array), up(:mfric, m.rheology.marker_fric.array),
+        up(:mexx, m.strain.marker_exx.array), up(:mexy, m.strain.marker_exy.array),
+        up(:msrr, m.strain.marker_sr_ratio.array), up(:mpr, m.pressure.marker_pr.array),
+        Int32(mp.material_description.itype_plasticity.value), Int32(tl.ntimestep.value), tl.timestep.value,
+        model.topography.parameters.sealevel.y_sealevel.value,
+        mp.viscosity_limits.viscosity_min.value, mp.viscosity_limits.viscosity_max.value,
+        mp.stress_limits_yield.yield_stress_min.value, mp.stress_limits_yield.yield_stress_max.value,
+        Int32(mp.stress_limits_yield.iuse_fluid_pressure_for_yield.value), Int32(no_yielding_in_mobile_wall),
+        Int32(dom["Atmosphere"]), Int32(dom["MobileWall"]), Int32(dom["PlateExtension"]))
+    outs = (m.stress.marker_sxx.array, m.stress.marker_sxy.array, m.rheology.marker_eta.array,
+            m.rheology.marker_pfailure.array)
+    keys = (:msxx, :msxy, :meta, :mpf)
+    pout = MarkerPlasticOut([h2d!(st, k, x) for (k, x) in zip(keys, outs)]...)
+    check(ccall((:ebg_update_marker_plastic_yielding, LIB), Cint,
+        (Ptr{Cvoid}, Ref{MarkerPlasticIn}, Ref{MarkerPlasticOut}), st.ctx, pin, pout),
+        "ebg_update_marker_plastic_yielding")
+    sync!(st)
+    for (k, x) in zip(keys, outs)
+        d2h!(x, st.buffers[k])
+    end
+    return nothing
+end
+
+"""
+    interpolate_viscosity_to_transport_arrays_gpu!(st, model, inside_flags)
+
+GPU replacement of BackupTransportArrays.backup_and_clear_viscosity_transport_arrays!
+followed by MarkersToStokesArrays.interpolate_viscosity_to_transport_arrays!
+(MarkersToStokesArrays.jl:60-64; BilinearNumerator.jl:470-533,
+BilinearAverage.jl:213-227), PlasticityLoopMarkers.jl:50,52: marker_eta alone
+scattered to etas1 (exclusive rule) and etan1 (central rule), the previous
+values kept (and copied to etas0, etan0) where a node receives no weight.
+The weight sums are recomputed from the same markers, so they equal the
+wtetas, wtetan of the full scatter.
+"""
+function interpolate_viscosity_to_transport_arrays_gpu!(st::GPUState, model, inside_flags::Vector{Int8})
+    set_grid!(st, model)
+    m = model.markers.arrays
+    v = model.stokes_continuity.arrays.viscosity
+    copyto!(v.etas0.array, v.etas1.array)
+    copyto!(v.etan0.array, v.etan1.array)
+    n = model.markers.parameters.distribution.marknum.value
+    up(k, x) = h2d!(st, k, x)
+    N = C_NULL
+    props = MarkerStokesProps(n, up(:mx, m.location.marker_x.array), up(:my, m.location.marker_y.array),
+        up(:flags, inside_flags), N, N, N, N, up(:meta, m.rheology.marker_eta.array), N, N, N, N, N, N, N, 0)
+    es, en = up(:etas1, v.etas1.array), up(:etan1, v.etan1.array)
+    f = StokesGridFields(N, N, es, en, N, N, N, N, N, N, N, N, N, N, N, N, N, N, N, N, N)
+    check(ccall((:ebg_interpolate_marker_stokes_parameters_to_grid, LIB), Cint,
+        (Ptr{Cvoid}, Ref{MarkerStokesProps}, Ref{StokesGridFields}), st.ctx, props, f),
+        "ebg_interpolate_marker_stokes_parameters_to_grid")
+    sync!(st)
+    d2h!(v.etas1.array, es)
+    d2h!(v.etan1.array, en)
     return nothing
 end
 

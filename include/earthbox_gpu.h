@@ -977,6 +977,42 @@ typedef struct {
  * (StokesContinuitySolver.jl:365-381).  Bit-identical except exp. */
 int ebg_update_subgrid_stress(ebg_ctx *ctx, const ebg_subgrid_stress_in *in, ebg_subgrid_stress_out *out);
 
+/* ---- marker plasticity (the Picard loop over markers) ----
+ * DEVICE marker arrays (length marknum): inside_flags, y, matid (1-based) with
+ * mat_mu[nmat]; marker cohesion, sine of the friction angle (fric), strain
+ * rates exx, exy, strain-rate ratio sr_ratio and pressure pr.  forecast: 0
+ * ViscoelasticStressForecast, 1 PureElasticStressForecast (itype_plasticity).
+ * Sticky-air, mobile-wall and plate-extension ids (-1: none) matter only with
+ * no_yielding_in_mobile_wall. */
+typedef struct {
+    int64_t marknum;
+    const int8_t *inside_flags;
+    const double *y;
+    const int16_t *matid;
+    const double *mat_mu;
+    int64_t nmat;
+    const double *cohesion, *fric, *exx, *exy, *sr_ratio, *pr;
+    int32_t forecast;
+    int32_t ntimestep;
+    double timestep, y_sealevel;
+    double viscosity_min, viscosity_max, yield_stress_min, yield_stress_max;
+    int32_t iuse_fluid_pressure_for_yield, no_yielding_in_mobile_wall;
+    int32_t matid_sticky_air, matid_mobile_wall, matid_plate_extension;
+} ebg_marker_plastic_in;
+
+typedef struct {
+    double *sxx, *sxy;   /* in/out (the viscoelastic forecast corrects them) */
+    double *eta;         /* in/out: marker viscoplastic viscosity */
+    float *pfailure;     /* in/out: 1 stays 1, else the yield flag */
+} ebg_marker_plastic_out;
+
+/* Replaces MarkerPlasticity.update_plastic_yielding!
+ * (rheology/plastic_failure/markers/MarkerPlasticity.jl:34-43; ViscoelasticForecast.jl:14-127,
+ * ElasticForecast.jl:14-110, YieldCheck.jl, Limits.jl), the marker step of the
+ * Picard loop over markers (PlasticityLoopMarkers.jl:48-53).  Bit-identical
+ * except asin/cos of the yield stress and the square roots. */
+int ebg_update_marker_plastic_yielding(ebg_ctx *ctx, const ebg_marker_plastic_in *in, ebg_marker_plastic_out *out);
+
 #ifdef __cplusplus
 }
 #endif

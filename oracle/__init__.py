@@ -897,3 +897,40 @@ def update_subgrid_stress(g, m: dict, f: dict, timestep, coef):
     return {"sxy": sxy, "sxx": sxx, "dsxyn": dsxyn, "dsxxn": dsxxn,
             "dsxy": np.asfortranarray(f["dsxy"] - dsxyn), "dsxx": np.asfortranarray(f["dsxx"] - dsxxn),
             "wtetas": sc["wtetas"], "wtetan": sc["wtetan"]}
+
+
+# ---------------------------------------------------------------- marker plasticity
+class _MarkerPlasticIn(C.Structure):
+    _fields_ = [("marknum", _I64), ("flags", _P), ("y", _P), ("matid", _P), ("mat_mu", _P), ("cohesion", _P),
+                ("fric", _P), ("exx", _P), ("exy", _P), ("sr_ratio", _P), ("pr", _P), ("forecast", _I64),
+                ("ntimestep", _I64), ("timestep", _D), ("y_sealevel", _D), ("viscosity_min", _D),
+                ("viscosity_max", _D), ("yield_stress_min", _D), ("yield_stress_max", _D),
+                ("iuse_fluid_pressure_for_yield", _I64), ("no_yielding_in_mobile_wall", _I64),
+                ("matid_sticky_air", _I64), ("matid_mobile_wall", _I64), ("matid_plate_extension", _I64)]
+
+
+MARKER_PLASTIC_SCALARS = ("forecast", "ntimestep", "timestep", "y_sealevel", "viscosity_min", "viscosity_max",
+                          "yield_stress_min", "yield_stress_max", "iuse_fluid_pressure_for_yield",
+                          "no_yielding_in_mobile_wall", "matid_sticky_air", "matid_mobile_wall",
+                          "matid_plate_extension")
+
+
+def update_marker_plastic_yielding(m: dict, par: dict):
+    """MarkerPlasticity.update_plastic_yielding! on copies: (sxx, sxy, eta,
+    pfailure).  m: flags, y, matid, mat_mu, cohesion, fric, exx, exy,
+    sr_ratio, pr, sxx, sxy, eta, pfailure; par: MARKER_PLASTIC_SCALARS."""
+    L = lib()
+    L.ebo_update_marker_plastic_yielding.restype = None
+    L.ebo_update_marker_plastic_yielding.argtypes = [C.POINTER(_MarkerPlasticIn), _P, _P, _P, _P]
+    f = lambda k: np.ascontiguousarray(m[k], dtype=np.float64)
+    keep = {k: f(k) for k in ("y", "cohesion", "fric", "exx", "exy", "sr_ratio", "pr")}
+    flags = np.ascontiguousarray(m["flags"], dtype=np.int8)
+    matid = np.ascontiguousarray(m["matid"], dtype=np.int16)
+    mu = np.ascontiguousarray(m["mat_mu"], dtype=np.float64)
+    a = _MarkerPlasticIn(flags.size, p(flags), p(keep["y"]), p(matid), p(mu),
+                         *[p(keep[k]) for k in ("cohesion", "fric", "exx", "exy", "sr_ratio", "pr")],
+                         *[par[k] for k in MARKER_PLASTIC_SCALARS])
+    sxx, sxy, eta = (np.array(m[k], dtype=np.float64) for k in ("sxx", "sxy", "eta"))
+    pf = np.array(m["pfailure"], dtype=np.float32)
+    L.ebo_update_marker_plastic_yielding(C.byref(a), p(sxx), p(sxy), p(eta), p(pf))
+    return sxx, sxy, eta, pf

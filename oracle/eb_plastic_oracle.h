@@ -39,6 +39,23 @@ void ebo_harmonic_normal_viscosity(int64_t xnum, int64_t ynum, const double *eta
 void ebo_plastic_strain_rate_at_pressure_nodes(int64_t xnum, int64_t ynum, const double *eii_basic,
                                                double *eii_pressure);
 
+/* Marker plastic yielding of the Picard loop over markers. */
+typedef struct {
+    int64_t marknum;
+    const int8_t *flags;
+    const double *y;
+    const int16_t *matid;
+    const double *mat_mu; /* 1-based by matid */
+    const double *cohesion, *fric, *exx, *exy, *sr_ratio, *pr;
+    int64_t forecast, ntimestep;
+    double timestep, y_sealevel, viscosity_min, viscosity_max, yield_stress_min, yield_stress_max;
+    int64_t iuse_fluid_pressure_for_yield, no_yielding_in_mobile_wall;
+    int64_t matid_sticky_air, matid_mobile_wall, matid_plate_extension;
+} ebo_marker_plastic_in;
+
+void ebo_update_marker_plastic_yielding(const ebo_marker_plastic_in *in, double *sxx, double *sxy, double *eta,
+                                        float *pfailure);
+
 #ifdef __cplusplus
 }
 #endif

@@ -179,3 +179,63 @@ void ebo_plastic_strain_rate_at_pressure_nodes(int64_t xnum, int64_t ynum, const
                                                  EBO_M(eii_basic, ny, i + 1, j) + EBO_M(eii_basic, ny, i + 1, j + 1)) /
                                                 4.0;
 }
+
+/* MarkerPlasticity.update_plastic_yielding! (rheology/plastic_failure/markers/):
+ * ViscoelasticForecast.jl:14-127 (forecast 0), ElasticForecast.jl:14-110
+ * (forecast 1), YieldCheck.jl:2-64, Limits.jl:2-21, CheckYield.jl:3-12,
+ * ViscoelasticFactor.jl:19-30; x^2.0 as x*x. */
+void ebo_update_marker_plastic_yielding(const ebo_marker_plastic_in *in, double *sxx, double *sxy, double *eta,
+                                        float *pfailure)
+{
+    for (int64_t m = 0; m < in->marknum; ++m) {
+        if (in->flags[m] != 1) continue;
+        const int16_t id = in->matid[m];
+        const double cohesion = in->cohesion[m], sfric = in->fric[m];
+        int flag = 0;
+        if (in->ntimestep > 0 && (cohesion > 0.0 || sfric > 0.0)) {
+            const double visc = eta[m], mu = EBO_V(in->mat_mu, id);
+            const double s_xx = sxx[m], s_xy = sxy[m];
+            const double exx = in->exx[m], exy = in->exy[m], ratio = in->sr_ratio[m], dt = in->timestep;
+            double sy = yield_stress(cohesion, sfric, in->pr[m], in->iuse_fluid_pressure_for_yield, in->y[m],
+                                     in->y_sealevel);
+            sy = jl_max(sy, in->yield_stress_min);
+            sy = jl_min(sy, in->yield_stress_max);
+            double fxx, fxy;
+            if (in->forecast == 0) {
+                const double z = visc / (visc + dt * mu);
+                fxx = s_xx * z + 2.0 * visc * exx * ratio * (1.0 - z);
+                fxy = s_xy * z + 2.0 * visc * exy * ratio * (1.0 - z);
+            } else {
+                fxx = s_xx + 2.0 * mu * dt * exx * ratio;
+                fxy = s_xy + 2.0 * mu * dt * exy * ratio;
+            }
+            const double sinv = sqrt(fxx * fxx + fxy * fxy);
+            int yielding;
+            if (in->no_yielding_in_mobile_wall)
+                yielding = sinv > sy && id != in->matid_sticky_air && id != in->matid_mobile_wall &&
+                           id != in->matid_plate_extension;
+            else
+                yielding = sinv > sy;
+            if (yielding) {
+                flag = 1;
+                double v;
+                if (in->forecast == 0) {
+                    const double sold = sqrt(s_xx * s_xx + s_xy * s_xy);
+                    if (sold != 0.0) {
+                        sxx[m] = s_xx * sy / sold;
+                        sxy[m] = s_xy * sy / sold;
+                    } else {
+                        sxx[m] = sy;
+                        sxy[m] = sy;
+                    }
+                    v = sy / 2.0 / (ratio * sqrt(exx * exx + exy * exy));
+                } else {
+                    v = mu * dt * sy / (sinv - sy);
+                }
+                v = jl_max(v, in->viscosity_min);
+                eta[m] = jl_min(v, in->viscosity_max);
+            }
+        }
+        if (pfailure[m] != 1.0f) pfailure[m] = (float)flag;
+    }
+}
