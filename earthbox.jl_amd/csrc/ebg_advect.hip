@@ -267,8 +267,8 @@ __global__ void __launch_bounds__(256) rk4_advect_kernel(RK4Args a)
 // Tolerance-mode RK4 on nominally uniform axes (both axes uniform to 1e-9 of
 // a step, checked when the grid is set).  Same stages, remaps, clamps and
 // outputs as rk4_advect_kernel, with three changes of arithmetic:
-//  - the cell search is one FMA: t = (x - x0)/h, and the half-cell index
-//    k = clamp(floor(2t), 0, 2n-3) gives the basic cell (k >> 1) and the
+//  - the cell search is one FMA: t2 = 2(x - x0)/h, and the half-cell index
+//    k = clamp(floor(t2), 0, 2n-3) gives the basic cell (k >> 1) and the
 //    velocity-grid remap (k & 1, the side of the cell centre) at once;
 //  - the normalized distances come from the nominal node positions, so the
 //    basic one is t - c and the staggered one differs from it by +-1/2;
@@ -277,7 +277,12 @@ __global__ void __launch_bounds__(256) rk4_advect_kernel(RK4Args a)
 // Spin is gathered only when an output needs it (SPIN).  The bilinear field
 // is continuous, so a different cell choice at a node or centre (where the
 // nominal and the stored coordinates round differently) changes the value by
-// rounding only.
+// rounding only.  The grids are read through buffer descriptors (32-bit
+// offsets, no 64-bit address arithmetic).  On the MI355X the kernel is bound
+// by the vector memory address unit: each 16-byte corner-pair gather takes it
+// a cycle per four lanes (16 gathers per marker).  Two markers per thread
+// measured no faster; a per-wavefront LDS window of the grids gained 7 % on
+// markers in their initial order and lost 27 % once they had drifted.
 enum { RKF_RK4 = 1, RKF_XN = 2, RKF_STRESS = 4, RKF_SPIN = 8 };
 
 struct RK4FastArgs {
@@ -291,8 +296,8 @@ struct RK4FastArgs {
     double *sxx, *sxy;
     int update_location;
     int xnum, ynum;
-    double inv_hx, nx0, inv_hy, ny0;   // t = x * inv_h + n0 = (x - x0) / h
-    double kmax_x, kmax_y;             // 2n - 3: last half cell
+    double inv_hx2, nx02, inv_hy2, ny02;   // t2 = x * inv_h2 + n02 = 2 (x - x0) / h
+    double kmax_x, kmax_y;                 // 2n - 3: last half cell
     double xmin, xmax, ymin, ymax;
 };
 
@@ -303,49 +308,87 @@ struct FastCell {
     double ds;    // distance from the staggered node the remap picks
 };
 
-__device__ __forceinline__ FastCell fast_cell(double x, double inv_h, double n0, double kmax)
+// +0.5 (r = 0) or -0.5 (r = 1) from the sign bit
+__device__ __forceinline__ double signed_half(int r)
+{
+    return __builtin_bit_cast(double, (unsigned long long)(0x3fe00000u | ((unsigned)r << 31)) << 32);
+}
+
+// t2 = 2t is exact (a doubled FMA), so db = t2/2 - c rounds as t - c
+__device__ __forceinline__ FastCell fast_cell(double x, double inv_h2, double n02, double kmax)
 {
     FastCell f;
-    const double t = __builtin_fma(x, inv_h, n0);
-    const int k = (int)fmin(fmax(t + t, 0.0), kmax);   // NaN -> 0
+    const double t2 = __builtin_fma(x, inv_h2, n02);
+    const int k = (int)fmin(fmax(t2, 0.0), kmax);   // NaN -> 0
     f.c = k >> 1;
     f.r = k & 1;
-    f.db = t - (double)f.c;
-    f.ds = f.db + (f.r ? -0.5 : 0.5);
+    f.db = __builtin_fma(t2, 0.5, -(double)f.c);
+    f.ds = f.db + signed_half(f.r);
     return f;
 }
 
 // Cell from a given 1-based basic index (the stage-1 marker_xn/yn, clamped
 // to the grid): the remap side follows from the nominal distance.
-__device__ __forceinline__ FastCell fast_cell_given(double x, int c1, double inv_h, double n0, int n)
+__device__ __forceinline__ FastCell fast_cell_given(double x, int c1, double inv_h2, double n02, int n)
 {
     FastCell f;
-    const double t = __builtin_fma(x, inv_h, n0);
+    const double t2 = __builtin_fma(x, inv_h2, n02);
     f.c = min(max(c1 - 1, 0), n - 2);
-    f.db = t - (double)f.c;
+    f.db = __builtin_fma(t2, 0.5, -(double)f.c);
     f.r = f.db > 0.5;
-    f.ds = f.db + (f.r ? -0.5 : 0.5);
+    f.ds = f.db + signed_half(f.r);
     return f;
 }
 
-// nested linear interpolation: p points at the upper-left node, column-major
-__device__ __forceinline__ double lerp2(const double* __restrict__ p, int ld, double dy, double dx)
+// Grid reads through buffer descriptors: 32-bit byte offsets (the launcher
+// keeps every grid below 2 GiB) and the column step as the scalar offset of
+// the second load, so no 64-bit address arithmetic per read.
+typedef unsigned int ebg_u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t grid_rsrc(const double* p, int64_t count)
 {
-    const double p00 = p[0], p10 = p[1], p01 = p[ld], p11 = p[ld + 1];
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, 0, (int)(count * 8), 0x00020000);
+}
+
+__device__ __forceinline__ double lo_f64(ebg_u32x4 v)
+{
+    return __builtin_bit_cast(double, ((unsigned long long)v.y << 32) | v.x);
+}
+__device__ __forceinline__ double hi_f64(ebg_u32x4 v)
+{
+    return __builtin_bit_cast(double, ((unsigned long long)v.w << 32) | v.z);
+}
+
+// nested linear interpolation from the four corner values
+__device__ __forceinline__ double lerp4(double p00, double p10, double p01, double p11, double dy, double dx)
+{
     const double a = __builtin_fma(dy, p10 - p00, p00);
     const double b = __builtin_fma(dy, p11 - p01, p01);
     return __builtin_fma(dx, b - a, a);
 }
 
+// the same at node idx (upper left) of a column-major grid with column step
+// ld8 bytes, read through its buffer descriptor
+__device__ __forceinline__ double lerp2(__amdgpu_buffer_rsrc_t r, int idx, int ld8, double dy, double dx)
+{
+    const ebg_u32x4 c0 = __builtin_amdgcn_raw_buffer_load_b128(r, idx << 3, 0, 0);
+    const ebg_u32x4 c1 = __builtin_amdgcn_raw_buffer_load_b128(r, idx << 3, ld8, 0);
+    return lerp4(lo_f64(c0), hi_f64(c0), lo_f64(c1), hi_f64(c1), dy, dx);
+}
+
+struct FastGrids {
+    __amdgpu_buffer_rsrc_t vx, vy, sp;
+    int ldx, ldy, ldx8, ldy8;
+};
+
 template <bool SPIN>
-__device__ __forceinline__ void fast_values(const RK4FastArgs& a, const FastCell& cx, const FastCell& cy,
+__device__ __forceinline__ void fast_values(const FastGrids& g, const FastCell& cx, const FastCell& cy,
                                             double& vx, double& vy, double& sp)
 {
-    const int ldx = a.ynum + 1, ldy = a.ynum;
     // vx grid: basic x cell, y remapped; vy grid: x remapped, basic y cell
-    vx = lerp2(a.vx1 + (unsigned)(cx.c * ldx + cy.c + cy.r), ldx, cy.ds, cx.db);
-    vy = lerp2(a.vy1 + (unsigned)((cx.c + cx.r) * ldy + cy.c), ldy, cy.db, cx.ds);
-    if (SPIN) sp = lerp2(a.esp + (unsigned)(cx.c * ldy + cy.c), ldy, cy.db, cx.db);
+    vx = lerp2(g.vx, __umul24(cx.c, g.ldx) + cy.c + cy.r, g.ldx8, cy.ds, cx.db);
+    vy = lerp2(g.vy, __umul24(cx.c + cx.r, g.ldy) + cy.c, g.ldy8, cy.db, cx.ds);
+    if (SPIN) sp = lerp2(g.sp, __umul24(cx.c, g.ldy) + cy.c, g.ldy8, cy.db, cx.db);
 }
 
 template <int F>
@@ -367,37 +410,45 @@ __global__ void __launch_bounds__(256) rk4_fast_kernel(RK4FastArgs a)
         if (a.mspin) a.mspin[m] = 0.0;
         return;
     }
+    FastGrids g;
+    g.ldx = a.ynum + 1;
+    g.ldy = a.ynum;
+    g.ldx8 = g.ldx * 8;
+    g.ldy8 = g.ldy * 8;
+    g.vx = grid_rsrc(a.vx1, (int64_t)g.ldx * a.xnum);
+    g.vy = grid_rsrc(a.vy1, (int64_t)g.ldy * (a.xnum + 1));
+    if (SPIN) g.sp = grid_rsrc(a.esp, (int64_t)g.ldy * a.xnum);
     FastCell cx, cy;
     if (XN) {
-        cx = fast_cell_given(x0, a.xn[m], a.inv_hx, a.nx0, a.xnum);
-        cy = fast_cell_given(y0, a.yn[m], a.inv_hy, a.ny0, a.ynum);
+        cx = fast_cell_given(x0, a.xn[m], a.inv_hx2, a.nx02, a.xnum);
+        cy = fast_cell_given(y0, a.yn[m], a.inv_hy2, a.ny02, a.ynum);
     } else {
-        cx = fast_cell(x0, a.inv_hx, a.nx0, a.kmax_x);
-        cy = fast_cell(y0, a.inv_hy, a.ny0, a.kmax_y);
+        cx = fast_cell(x0, a.inv_hx2, a.nx02, a.kmax_x);
+        cy = fast_cell(y0, a.inv_hy2, a.ny02, a.kmax_y);
     }
     double vx, vy, vs = 0.0;
-    fast_values<SPIN>(a, cx, cy, vx, vy, vs);
+    fast_values<SPIN>(g, cx, cy, vx, vy, vs);
     double svx = vx, svy = vy, ssp = vs;
     if (RK4) {
         double ax = vx, ay = vy, as = vs;
         double xc = __builtin_fma(a.half_dt, vx, x0);
         double yc = __builtin_fma(a.half_dt, vy, y0);
-        fast_values<SPIN>(a, fast_cell(xc, a.inv_hx, a.nx0, a.kmax_x),
-                          fast_cell(yc, a.inv_hy, a.ny0, a.kmax_y), vx, vy, vs);
+        fast_values<SPIN>(g, fast_cell(xc, a.inv_hx2, a.nx02, a.kmax_x),
+                          fast_cell(yc, a.inv_hy2, a.ny02, a.kmax_y), vx, vy, vs);
         ax = __builtin_fma(2.0, vx, ax);
         ay = __builtin_fma(2.0, vy, ay);
         if (SPIN) as = __builtin_fma(2.0, vs, as);
         xc = __builtin_fma(a.half_dt, vx, x0);
         yc = __builtin_fma(a.half_dt, vy, y0);
-        fast_values<SPIN>(a, fast_cell(xc, a.inv_hx, a.nx0, a.kmax_x),
-                          fast_cell(yc, a.inv_hy, a.ny0, a.kmax_y), vx, vy, vs);
+        fast_values<SPIN>(g, fast_cell(xc, a.inv_hx2, a.nx02, a.kmax_x),
+                          fast_cell(yc, a.inv_hy2, a.ny02, a.kmax_y), vx, vy, vs);
         ax = __builtin_fma(2.0, vx, ax);
         ay = __builtin_fma(2.0, vy, ay);
         if (SPIN) as = __builtin_fma(2.0, vs, as);
         xc = __builtin_fma(a.dt, vx, x0);
         yc = __builtin_fma(a.dt, vy, y0);
-        fast_values<SPIN>(a, fast_cell(xc, a.inv_hx, a.nx0, a.kmax_x),
-                          fast_cell(yc, a.inv_hy, a.ny0, a.kmax_y), vx, vy, vs);
+        fast_values<SPIN>(g, fast_cell(xc, a.inv_hx2, a.nx02, a.kmax_x),
+                          fast_cell(yc, a.inv_hy2, a.ny02, a.kmax_y), vx, vy, vs);
         constexpr double r6 = 1.0 / 6.0;
         svx = (ax + vx) * r6;
         svy = (ay + vy) * r6;
@@ -476,6 +527,13 @@ static AxisDesc axis_y(const Grid2D& g)
     return AxisDesc{g.gridy_b, (int)g.ynum, g.y0, g.inv_hy, g.uniform_y, g.cells_y};
 }
 
+// nominally uniform axes, and grids the fast kernel can address with 32-bit
+// byte offsets
+static bool fast_kernel_fits(const Grid2D& g)
+{
+    return g.nominal_x && g.nominal_y && (int64_t)(g.ynum + 1) * (g.xnum + 1) * 8 < ((int64_t)1 << 31);
+}
+
 static unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 #define EBG_RK4_K(f) (const void*)rk4_advect_kernel<f>
@@ -540,7 +598,7 @@ int ebg_rk4_advect(ebg_ctx* ctx, int64_t marknum, int order, double timestep,
               "ebg_rk4_advect: marker_sxx and marker_sxy must both be given or both NULL");
     EBG_CHECK(std::isfinite(timestep), EBG_ERR_ARG, "ebg_rk4_advect: timestep not finite");
     const Grid2D& g = ctx->grid;
-    if (ctx->rk4_arithmetic == EBG_RK4_TOLERANCE && g.nominal_x && g.nominal_y) {
+    if (ctx->rk4_arithmetic == EBG_RK4_TOLERANCE && fast_kernel_fits(g)) {
         RK4FastArgs a;
         a.marknum = marknum;
         a.dt = timestep;
@@ -561,10 +619,11 @@ int ebg_rk4_advect(ebg_ctx* ctx, int64_t marknum, int order, double timestep,
         a.update_location = update_location;
         a.xnum = (int)g.xnum;
         a.ynum = (int)g.ynum;
-        a.inv_hx = g.inv_hx;
-        a.nx0 = -g.x0 * g.inv_hx;
-        a.inv_hy = g.inv_hy;
-        a.ny0 = -g.y0 * g.inv_hy;
+        // doubled exactly: t2 = 2t
+        a.inv_hx2 = 2.0 * g.inv_hx;
+        a.nx02 = 2.0 * (-g.x0 * g.inv_hx);
+        a.inv_hy2 = 2.0 * g.inv_hy;
+        a.ny02 = 2.0 * (-g.y0 * g.inv_hy);
         a.kmax_x = (double)(2 * g.xnum - 3);
         a.kmax_y = (double)(2 * g.ynum - 3);
         a.xmin = g.xmin;
@@ -627,7 +686,7 @@ int ebg_rk4_set_arithmetic(ebg_ctx* ctx, int mode)
 int ebg_rk4_tolerance_kernel_available(ebg_ctx* ctx)
 {
     if (!ctx || !ctx->has_grid) return -1;
-    return (ctx->grid.nominal_x && ctx->grid.nominal_y) ? 1 : 0;
+    return fast_kernel_fits(ctx->grid) ? 1 : 0;
 }
 
 int ebg_calculate_spin(ebg_ctx* ctx, const double* vx1, const double* vy1, double* esp)

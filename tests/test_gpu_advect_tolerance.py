@@ -5,7 +5,9 @@ size of the oracle's; velocities and spins within 1e-9 of the largest grid
 velocity / spin (a swapped remap, corner or stage gives O(1) errors); markers
 outside the domain get zero velocity and keep their position and stress.  On
 a grid that is not uniform the mode falls back to the exact kernel, which is
-bit-identical to the oracle.
+bit-identical to the oracle.  Each thread advances two markers with their
+stages interleaved; a shuffled marker order gives bit-identical results to
+the spatially ordered one.
 """
 import numpy as np
 import pytest
@@ -151,3 +153,34 @@ def test_rk4_tolerance_full_size_sampled(tol_ctx):
     err = max(np.abs(gx[idx] - ex).max(), np.abs(gy[idx] - ey).max())
     print(f"18M markers: max position error {err:.3e} m", flush=True)
     assert err <= 1e-10 * 1.5e6
+
+
+@pytest.mark.parametrize("stress,courant", [(False, 0.5), (True, 0.5), (False, 3.0)])
+def test_rk4_tolerance_marker_order_independent(tol_ctx, stress, courant):
+    """Markers in the reference's order against the same markers shuffled,
+    unshuffled: every output bit-identical (a marker's result depends on
+    nothing but its own inputs, whichever marker shares its thread)."""
+    g, x, y, vx1, vy1, esp, _ = _case(203, 61, 5, 9)
+    dt = courant * g.xstpavg / 1e-9
+    print(f"rk4 tolerance marker order independence, stress={stress} courant={courant}", flush=True)
+    tol_ctx.set_grid(g)
+    perm = np.random.default_rng(4).permutation(x.size)
+    rng = np.random.default_rng(5)
+    sxx, sxy = rng.standard_normal(x.size) * 1e7, rng.standard_normal(x.size) * 1e7
+    outs = []
+    for order in (np.arange(x.size), perm):
+        dx, dy = to_device(x[order]), to_device(y[order])
+        dvx, dvy, dsp = (torch.zeros(x.size, dtype=torch.float64, device="cuda") for _ in range(3))
+        dsxx, dsxy = (to_device(sxx[order]), to_device(sxy[order])) if stress else (None, None)
+        tol_ctx.advect_markers_and_rotate_stress_tensor(4, dt, None, dx, dy, None, None, to_device(vx1),
+                                                        to_device(vy1), to_device(esp), dvx, dvy, dsp,
+                                                        dsxx, dsxy)
+        tol_ctx.synchronize()
+        r = {"x": to_host(dx), "y": to_host(dy), "vx": to_host(dvx), "vy": to_host(dvy), "spin": to_host(dsp)}
+        if stress:
+            r.update(sxx=to_host(dsxx), sxy=to_host(dsxy))
+        inv = np.empty_like(order)
+        inv[order] = np.arange(x.size)
+        outs.append({k: v[inv] for k, v in r.items()})
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k], equal_nan=True), k
