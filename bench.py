@@ -67,6 +67,7 @@ def parse():
     ap.add_argument("--no-config5", action="store_true")
     ap.add_argument("--config5-steps", type=int, default=20)
     ap.add_argument("--no-heat", action="store_true")
+    ap.add_argument("--no-cnn", action="store_true")
     ap.add_argument("--stokes-amplitude", type=float, default=1.0,
                     help="decades of the random log-viscosity perturbation of the 1001 x 501 system")
     return ap.parse_args()
@@ -614,6 +615,51 @@ def bench_grid_kernels(ctx, reps=20):
     return out
 
 
+def bench_cnn(ctx, reps=10):
+    """The learned correction's network on the MFMA units: a forward pass on a
+    512 x 1024 cell image (the size of the config-2 solve's inputs) and a
+    training step (forward, backward, weight and bias gradients) on a batch of
+    32 crops of 64 x 128 cells, timed with HIP events; achieved TFLOP/s on the
+    convolutions' multiply-adds (padded channels counted, as the MFMA units
+    compute them) against the MI355X's dense BF16 peak (2.5 PFLOP/s).  The
+    backward pass is counted as twice the forward (input and weight
+    gradients)."""
+    import numpy as np
+    import torch
+    from earthbox_gpu import learned
+    net = learned.LearnedCorrection(ctx, seed=5)
+    flop_px = sum(2 * 9 * cin * cout / 4 ** lv for cin, cout, _, _, lv, _ in learned.LAYERS)
+    rng = np.random.default_rng(3)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s0.record(stream)
+        for _ in range(reps):
+            fn()
+        s1.record(stream)
+        torch.cuda.synchronize()
+        return s0.elapsed_time(s1) / reps
+
+    x = torch.from_numpy(rng.standard_normal((1, 512, 1024, learned.N_IN)).astype(np.float32)).cuda()
+    ms_f = timed(lambda: net.forward(x))
+    flops_f = flop_px * 512 * 1024
+    xb = torch.from_numpy(rng.standard_normal((32, 64, 128, learned.N_IN)).astype(np.float32)).cuda()
+    tb = torch.from_numpy(rng.standard_normal((32, 64, 128, learned.N_OUT)).astype(np.float32)).cuda()
+    ms_t = timed(lambda: net.loss_and_grad(xb, tb))
+    flops_t = 3 * flop_px * 32 * 64 * 128
+    peak = 2500.0
+    return {"network": f"4-level U-Net, {len(learned.LAYERS)} 3x3 convolutions, 32 channels, BF16 MFMA "
+                       f"(v_mfma_f32_16x16x32_bf16 where 32 input channels, 16x16x16 otherwise)",
+            "forward_512x1024": {"ms": ms_f, "gflop": flops_f / 1e9, "tflops": flops_f / (ms_f * 1e-3) / 1e12,
+                                 "frac_of_bf16_peak": flops_f / (ms_f * 1e-3) / 1e12 / peak},
+            "train_step_32x64x128": {"ms": ms_t, "gflop": flops_t / 1e9,
+                                     "tflops": flops_t / (ms_t * 1e-3) / 1e12,
+                                     "frac_of_bf16_peak": flops_t / (ms_t * 1e-3) / 1e12 / peak}}
+
+
 def bench_picard(ctx, nglobal=10, tol=1e-4):
     """The Picard loop over nodes on the synthetic 2001 x 801 visco-elasto-plastic
     rift state (four layers, cohesion and friction, +-0.5 cm/yr extension, one
@@ -855,6 +901,7 @@ def main():
         ("sinker", not args.no_sinker, lambda: bench_sinker(ctx)),
         ("picard", not args.no_picard, lambda: bench_picard(ctx)),
         ("grid_kernels", not args.no_grid_kernels, lambda: bench_grid_kernels(ctx)),
+        ("cnn", not args.no_cnn, lambda: bench_cnn(ctx)),
     ]
     for name, on, fn in legs:
         if not on:

@@ -130,6 +130,81 @@ __global__ void __launch_bounds__(256) conv3x3_fwd_kernel(const uint16_t* __rest
         }
 }
 
+// The same for CIN = 32 on gfx950's double-K instruction
+// v_mfma_f32_16x16x32_bf16: one instruction per (tap, 16 output channels,
+// 16 pixels) covers all 32 input channels.  Operand layout (wave64): lane l
+// supplies A[l%16][8*(l/16) .. +7] (the weights w[tap][co][ci], 16 contiguous
+// bytes) and B[8*(l/16) .. +7][l%16] (x[pixel][ci], 16 contiguous bytes) and
+// receives D[4*(l/16) .. +3][l%16].  The block's weights (9 x COUT x 32 bf16,
+// 18 KB at COUT = 32) are staged once in LDS and shared by its four waves.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+template <int COUT>
+__global__ void __launch_bounds__(256) conv3x3_fwd32_kernel(const uint16_t* __restrict__ x, int B, int H, int W,
+                                                           const uint16_t* __restrict__ wt,
+                                                           const float* __restrict__ bias, float* __restrict__ y)
+{
+    constexpr int CIN = 32, MB = COUT / 16, NB = 4, WV = 9 * COUT * CIN / 8;   // 16-byte units
+    __shared__ uint4 wsh[WV];
+    const uint4* wg = reinterpret_cast<const uint4*>(wt);
+    for (int i = threadIdx.x; i < WV; i += 256) wsh[i] = wg[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t P = (int64_t)B * H * W;
+    const int64_t p0 = ((int64_t)blockIdx.x * 4 + wave) * 64;
+    if (p0 >= P) return;
+    const int col = lane & 15, kq = lane >> 4;
+    int pb[NB], ph[NB], pw[NB];
+    bool pv[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+        const int64_t p = p0 + nb * 16 + col;
+        pv[nb] = p < P;
+        const int64_t q = pv[nb] ? p : 0;
+        pw[nb] = (int)(q % W);
+        ph[nb] = (int)((q / W) % H);
+        pb[nb] = (int)(q / ((int64_t)W * H));
+    }
+    v4f acc[MB][NB];
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int tap = 0; tap < 9; ++tap) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        v8bf bop[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            const int hh = ph[nb] + dy, ww = pw[nb] + dx;
+            uint4 u = make_uint4(0u, 0u, 0u, 0u);
+            if (pv[nb] && hh >= 0 && hh < H && ww >= 0 && ww < W)
+                u = *reinterpret_cast<const uint4*>(x + (((int64_t)pb[nb] * H + hh) * W + ww) * CIN + kq * 8);
+            bop[nb] = __builtin_bit_cast(v8bf, u);
+        }
+#pragma unroll
+        for (int mb = 0; mb < MB; ++mb) {
+            const v8bf aop = __builtin_bit_cast(v8bf, wsh[(tap * COUT + mb * 16 + col) * (CIN / 8) + kq]);
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+                acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aop, bop[nb], acc[mb][nb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) {
+            if (!pv[nb]) continue;
+            const int co = mb * 16 + kq * 4;
+            const int64_t p = p0 + nb * 16 + col;
+            float4 o;
+            o.x = acc[mb][nb][0] + (bias ? bias[co + 0] : 0.f);
+            o.y = acc[mb][nb][1] + (bias ? bias[co + 1] : 0.f);
+            o.z = acc[mb][nb][2] + (bias ? bias[co + 2] : 0.f);
+            o.w = acc[mb][nb][3] + (bias ? bias[co + 3] : 0.f);
+            *reinterpret_cast<float4*>(y + p * COUT + co) = o;
+        }
+}
+
 // Weight gradient partial sums: part[chunk][tap][co][ci] =
 // sum over the chunk's pixels p of dy[p][co] * x[p + off(tap)][ci].
 // One wave per (chunk, tap); K = pixels in groups of 16.
@@ -190,30 +265,49 @@ __global__ void __launch_bounds__(64) conv3x3_wgrad_kernel(const uint16_t* __res
                 out[(size_t)(mb * 16 + kq * 4 + t) * CIN + nb * 16 + col] = acc[mb][nb][t];
 }
 
-// g[k] = sum over chunks of part[chunk][k] in chunk order (deterministic)
-__global__ void sum_chunks_kernel(const float* __restrict__ part, int nchunks, int n, float* __restrict__ g)
+// g[k] = sum over chunks of part[chunk][k]: a block takes 32 consecutive k;
+// its eight 32-lane groups sum every eighth chunk, then a fixed-order tree
+// over the groups in LDS (deterministic)
+__global__ void __launch_bounds__(256) sum_chunks_kernel(const float* __restrict__ part, int nchunks, int n,
+                                                         float* __restrict__ g)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    __shared__ float sh[256];
+    const int kl = threadIdx.x & 31, grp = threadIdx.x >> 5;
+    const int k = blockIdx.x * 32 + kl;
     float s = 0.f;
-    for (int c = 0; c < nchunks; ++c) s += part[(size_t)c * n + k];
-    g[k] = s;
+    if (k < n)
+        for (int c = grp; c < nchunks; c += 8) s += part[(size_t)c * n + k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int half = 4; half >= 1; half /= 2) {
+        if (grp < half) sh[threadIdx.x] += sh[threadIdx.x + half * 32];
+        __syncthreads();
+    }
+    if (grp == 0 && k < n) g[k] = sh[kl];
 }
 
-// bias gradient partials: part[chunk][co] = sum_p dpre[p][co] over the chunk
+// bias gradient partials: part[chunk][co] = sum_p dpre[p][co] over the chunk.
+// 256 threads: 256 / COUT pixel lanes per channel, each summing every
+// (256 / COUT)-th pixel of the chunk, then a fixed-order tree over the lanes
+// in LDS (deterministic).
 template <int COUT>
-__global__ void bias_grad_partial_kernel(const float* __restrict__ dpre, int64_t P, int chunk,
-                                         float* __restrict__ part)
+__global__ void __launch_bounds__(256) bias_grad_partial_kernel(const float* __restrict__ dpre, int64_t P,
+                                                                int chunk, float* __restrict__ part)
 {
-    const int co = threadIdx.x;
-    if (co >= COUT) return;
+    constexpr int G = 256 / COUT;
+    __shared__ float sh[256];
+    const int co = threadIdx.x % COUT, g = threadIdx.x / COUT;
     const int64_t p0 = (int64_t)blockIdx.x * chunk;
+    const int64_t pend = min(p0 + chunk, P);
     float s = 0.f;
-    for (int q = 0; q < chunk; ++q) {
-        const int64_t p = p0 + q;
-        if (p < P) s += dpre[p * COUT + co];
+    for (int64_t p = p0 + g; p < pend; p += G) s += dpre[p * COUT + co];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int half = G / 2; half >= 1; half /= 2) {
+        if (g < half) sh[threadIdx.x] += sh[threadIdx.x + half * COUT];
+        __syncthreads();
     }
-    part[(size_t)blockIdx.x * COUT + co] = s;
+    if (g == 0) part[(size_t)blockIdx.x * COUT + co] = sh[co];
 }
 
 // ---------------------------------------------------------------- pointwise
@@ -521,7 +615,7 @@ struct DevMem {
 enum { L_IN = 0, L_A0, L_E1A, L_E1B, L_E2A, L_E2B, L_E3A, L_E3B, L_D2A, L_D2B, L_D1A, L_D1B,
        L_D0A, L_D0B, L_OUT, NLAYERS };
 
-constexpr int WGRAD_CHUNK = 1024;
+constexpr int WGRAD_CHUNK = 256;
 
 struct CnnNet {
     std::vector<ConvLayer> layers;
@@ -579,8 +673,8 @@ static void conv_fwd(const ConvLayer& c, const uint16_t* x, int B, int H, int W,
     const int64_t P = (int64_t)B * H * W;
     const dim3 g(nblk(P, 256)), b(256);
     if (c.cin == 16 && c.cout == 32) hipLaunchKernelGGL((conv3x3_fwd_kernel<16, 32>), g, b, 0, s, x, B, H, W, wb, bias, y);
-    else if (c.cin == 32 && c.cout == 32) hipLaunchKernelGGL((conv3x3_fwd_kernel<32, 32>), g, b, 0, s, x, B, H, W, wb, bias, y);
-    else if (c.cin == 32 && c.cout == 16) hipLaunchKernelGGL((conv3x3_fwd_kernel<32, 16>), g, b, 0, s, x, B, H, W, wb, bias, y);
+    else if (c.cin == 32 && c.cout == 32) hipLaunchKernelGGL((conv3x3_fwd32_kernel<32>), g, b, 0, s, x, B, H, W, wb, bias, y);
+    else if (c.cin == 32 && c.cout == 16) hipLaunchKernelGGL((conv3x3_fwd32_kernel<16>), g, b, 0, s, x, B, H, W, wb, bias, y);
 }
 
 // input gradient: conv of dy (cout channels) with the flipped transpose -> cin channels
@@ -589,9 +683,9 @@ static void conv_bwd_data(const ConvLayer& c, const uint16_t* dy, int B, int H, 
 {
     const int64_t P = (int64_t)B * H * W;
     const dim3 g(nblk(P, 256)), b(256);
-    if (c.cout == 32 && c.cin == 32) hipLaunchKernelGGL((conv3x3_fwd_kernel<32, 32>), g, b, 0, s, dy, B, H, W, wT, (const float*)nullptr, dx);
+    if (c.cout == 32 && c.cin == 32) hipLaunchKernelGGL((conv3x3_fwd32_kernel<32>), g, b, 0, s, dy, B, H, W, wT, (const float*)nullptr, dx);
     else if (c.cout == 16 && c.cin == 32) hipLaunchKernelGGL((conv3x3_fwd_kernel<16, 32>), g, b, 0, s, dy, B, H, W, wT, (const float*)nullptr, dx);
-    else if (c.cout == 32 && c.cin == 16) hipLaunchKernelGGL((conv3x3_fwd_kernel<32, 16>), g, b, 0, s, dy, B, H, W, wT, (const float*)nullptr, dx);
+    else if (c.cout == 32 && c.cin == 16) hipLaunchKernelGGL((conv3x3_fwd32_kernel<16>), g, b, 0, s, dy, B, H, W, wT, (const float*)nullptr, dx);
 }
 
 static int conv_wgrad(CnnNet& n, const ConvLayer& c, const uint16_t* x, const uint16_t* dy_bf,
@@ -611,14 +705,14 @@ static int conv_wgrad(CnnNet& n, const ConvLayer& c, const uint16_t* x, const ui
     else
         hipLaunchKernelGGL((conv3x3_wgrad_kernel<32, 16>), g, dim3(64), 0, s, x, dy_bf, B, H, W, WGRAD_CHUNK, n.wpart.as<float>());
     float* gw = n.grads.as<float>() + c.w_off;
-    hipLaunchKernelGGL(sum_chunks_kernel, dim3(nblk((int64_t)wn)), dim3(256), 0, s, (const float*)n.wpart.as<float>(),
-                       nch, (int)wn, gw);
+    hipLaunchKernelGGL(sum_chunks_kernel, dim3((unsigned)((wn + 31) / 32)), dim3(256), 0, s,
+                       (const float*)n.wpart.as<float>(), nch, (int)wn, gw);
     if (c.cout == 32)
-        hipLaunchKernelGGL(bias_grad_partial_kernel<32>, dim3(nch), dim3(64), 0, s, dy_f32, P, WGRAD_CHUNK, n.bpart.as<float>());
+        hipLaunchKernelGGL(bias_grad_partial_kernel<32>, dim3(nch), dim3(256), 0, s, dy_f32, P, WGRAD_CHUNK, n.bpart.as<float>());
     else
-        hipLaunchKernelGGL(bias_grad_partial_kernel<16>, dim3(nch), dim3(64), 0, s, dy_f32, P, WGRAD_CHUNK, n.bpart.as<float>());
-    hipLaunchKernelGGL(sum_chunks_kernel, dim3(nblk(c.cout)), dim3(256), 0, s, (const float*)n.bpart.as<float>(), nch,
-                       c.cout, n.grads.as<float>() + c.b_off);
+        hipLaunchKernelGGL(bias_grad_partial_kernel<16>, dim3(nch), dim3(256), 0, s, dy_f32, P, WGRAD_CHUNK, n.bpart.as<float>());
+    hipLaunchKernelGGL(sum_chunks_kernel, dim3((unsigned)((c.cout + 31) / 32)), dim3(256), 0, s,
+                       (const float*)n.bpart.as<float>(), nch, c.cout, n.grads.as<float>() + c.b_off);
     return EBG_OK;
 }
 
