@@ -12,6 +12,15 @@ namespace ebg {
 
 constexpr int RED_BLOCKS = 512;  // partial sums per reduction
 constexpr int MAXK = 256;        // longest Krylov basis a multi-dot handles (2-D restart <= MAXK - 2)
+// Selective reorthogonalization (2-D and 3-D Krylov solves): a second
+// Gram-Schmidt pass when the first removed more than half of the norm (a
+// quarter of the squared norm left).  With 1/2 of the squared norm the second
+// pass ran at almost every step of the plastic rift solves: same iterations,
+// 7 % more solve time (3-D spheres at contrast 1e2-1e6: 5 %).  A looser 1/100
+// saved 24 % on the rift but took 119 instead of 80 iterations on the
+// 501 x 501 falling block (contrast 1e6) with the FP64 V-cycle, and 117
+// instead of 71 on the 101 x 101 block with the FP32 one.
+constexpr double REORTH_RATIO2 = 0.25;
 constexpr int DOT_CHUNK = 8;     // basis vectors per pass of the multi-dot (register accumulators)
 
 // partial[b*k + q] = sum over block b's grid-stride slice of w * V[q] for

@@ -675,14 +675,6 @@ static void load_viscosity(StokesState& st, MGHierarchy<T>& H, hipStream_t s)
 }
 
 // deterministic dot products of w against V[0..k-1] -> hdev[0..k-1]
-// Selective reorthogonalization: a second Gram-Schmidt pass when the first
-// removed more than half of the norm (a quarter of the squared norm left).
-// With 1/2 of the squared norm the second pass ran at almost every step of
-// the plastic rift solves: same iterations, 7 % more solve time.  A looser
-// 1/100 saved 24 % there but took 119 instead of 80 iterations on the
-// 501 x 501 falling block (contrast 1e6) with the FP64 V-cycle, and 117
-// instead of 71 on the 101 x 101 block with the FP32 one.
-constexpr double REORTH_RATIO2 = 0.25;
 
 static void multidot(StokesState& st, const double* w, const double* V, int64_t ldv, int k, int64_t n,
                      double* out, int accumulate, hipStream_t s)

@@ -322,7 +322,7 @@ static int fgmres3(ebg_ctx* ctx, Stokes3State& st, const ebg_solve_opts& o, cons
             EBG_HIP(hipStreamSynchronize(s));
             double wn2 = nn[0];
             for (int q = 0; q <= k; ++q) H[(size_t)k * (m + 1) + q] = hbuf[q];
-            if (!(wn2 > 0.5 * nn[1])) {
+            if (!(wn2 > REORTH_RATIO2 * nn[1])) {
                 multidot3(st, w, V, n, k + 1, n, hd + HOFF, s);
                 hipLaunchKernelGGL(update_kernel, dim3(nbk(n)), dim3(256), 0, s, w, V, n, hd + HOFF, k + 1, n);
                 norm2(w, st.nrm.as<double>());
