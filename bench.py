@@ -686,7 +686,7 @@ def bench_picard(ctx, nglobal=10, tol=1e-4):
         ds = {k: to_device(v) for k, v in state.items()}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        _, log = picard_loop_nodes(ctx, tmpl, ds, par, nglobal, tol, solve_opts=opts)
+        _, log = picard_loop_nodes(ctx, tmpl, ds, par, nglobal, tol, solve_opts=opts, warm_start=True)
         el = time.perf_counter() - t0
         out[name] = {"seconds": el, "picard_iterations": len(log),
                      "krylov_iterations": [r["iterations"] for r in log],

@@ -45,6 +45,17 @@ STATE = ("etas1", "eta_flow", "mus1", "sxy1", "rho1", "rho1_vy", "etan1", "mun1"
          "eii_plastic_pressure", "yield_error")
 
 
+def warm_start_vector(S_prev, pscale_prev: float, pscale: float):
+    """The previous solve's solution as the next solve's initial guess: its
+    scaled pressures (every third unknown, pr / pscale) rescaled to the new
+    pscale.  The reference solves directly, so the initial guess changes the
+    result only within the iterative solve's tolerance."""
+    S0 = S_prev.clone()
+    if pscale != pscale_prev:
+        S0[2::3] *= pscale_prev / pscale
+    return S0
+
+
 def check_stop_global_loop_nodes(nnode_yield, iglobal, nglobal, global_yield_error, convergence_criterion,
                                  tolerance) -> bool:
     """LoopTermination.check_stop_global_loop_nodes (LoopTermination.jl:16-31)."""
@@ -57,7 +68,8 @@ def picard_loop_nodes(ctx, template: StokesProblem, state: dict, plastic_params:
                       no_yielding_in_mobile_wall: bool = False, no_yielding_in_plate_extension: bool = False,
                       solve_opts: dict | None = None, solver: StokesSolverGPU | None = None,
                       velocity_old: tuple | None = None, iupdate_timestep: int = 0,
-                      marker_cell_displ_max: float = 0.5):
+                      marker_cell_displ_max: float = 0.5, warm_start: bool = False,
+                      initial_guess: tuple | None = None):
     """template: the system's grid, velocity boundary conditions, pressure BC,
     gravity and interface-stabilization flag (its fields are replaced each
     iteration); state: device arrays keyed like STATE, updated in place as the
@@ -73,8 +85,12 @@ def picard_loop_nodes(ctx, template: StokesProblem, state: dict, plastic_params:
     StokesContinuitySolver.jl:223-226, Update.jl:43-61) before the stress
     forecast and the yielding, and the new step carries into the next
     iteration's right-hand side.
-    Returns (solution dict with vx1, vy1, pr1, the last solution vector S,
-    exx, exy, eii, sxy2, sxx2, dsxy, dsxx and the final timestep;
+    warm_start: False (every solve from zero, default) or True: each solve
+    starts from the previous one's solution, the first from initial_guess
+    ((S, pscale) of an earlier solve, e.g. the previous time step's) when
+    given, see warm_start_vector.
+    Returns (solution dict with vx1, vy1, pr1, the last solution vector S and
+    its pscale, exx, exy, eii, sxy2, sxx2, dsxy, dsxx and the final timestep;
     per-iteration log)."""
     g = template.grid
     solver = solver or StokesSolverGPU(ctx)
@@ -100,7 +116,12 @@ def picard_loop_nodes(ctx, template: StokesProblem, state: dict, plastic_params:
         t0 = time.perf_counter()
         solver.set_system(prob, fields={"etas0": ve["etas0"], "etan0": ve["etan0"], "rho1_vy": state["rho1_vy"],
                                         "RX1": ve["RX1"], "RY1": ve["RY1"], "RC1": ve["RC1"]})
-        S, st = solver.solve_system(**opts)   # returns after the device finished (stats read back)
+        if warm_start and initial_guess is not None:
+            S, st = solver.solve_system(warm_start_vector(initial_guess[0], initial_guess[1], prob.pscale),
+                                        **{**opts, "use_initial_guess": 1})
+        else:
+            S, st = solver.solve_system(**opts)   # returns after the device finished (stats read back)
+        initial_guess = (S, prob.pscale)
         solve_seconds = time.perf_counter() - t0
         vx1, vy1, pr1 = solver.process_stokes_solution(S)
         # stokes_solution_processing! on the grid: the time step limit (the
@@ -127,7 +148,7 @@ def picard_loop_nodes(ctx, template: StokesProblem, state: dict, plastic_params:
                     "nnode_yield": nnode_yield, "global_yield_error": global_yield_error,
                     "dvxy_rel_L2": dvxy_rel_L2, "timestep": dt, "solve_seconds_with_setup": solve_seconds,
                     "learned_accepted": st["learned_accepted"]})
-        sol = {"vx1": vx1, "vy1": vy1, "pr1": pr1, "S": S, "timestep": dt, **ss}
+        sol = {"vx1": vx1, "vy1": vy1, "pr1": pr1, "S": S, "pscale": prob.pscale, "timestep": dt, **ss}
         if check_stop_global_loop_nodes(nnode_yield, iglobal, nglobal, global_yield_error, dvxy_rel_L2,
                                         tolerance):
             break
@@ -149,9 +170,10 @@ def picard_loop_markers(ctx, template: StokesProblem, state: dict, markers: dict
                         iuse_harmonic_avg_normal_viscosity: int = 1, forecast: int = VISCOELASTIC_FORECAST,
                         matid_domains: tuple = (-1, -1, -1), solve_opts: dict | None = None,
                         solver: StokesSolverGPU | None = None, velocity_old: tuple | None = None,
-                        iupdate_timestep: int = 0, marker_cell_displ_max: float = 0.5):
-    """template, state, solve_opts, solver, velocity_old, iupdate_timestep:
-    as picard_loop_nodes (state needs etas1, etan1, mus1, mun1, sxy1, sxx1,
+                        iupdate_timestep: int = 0, marker_cell_displ_max: float = 0.5,
+                        warm_start: bool = False, initial_guess: tuple | None = None):
+    """template, state, solve_opts, solver, velocity_old, iupdate_timestep,
+    warm_start, initial_guess: as picard_loop_nodes (state needs etas1, etan1, mus1, mun1, sxy1, sxx1,
     rho1, rho1_vy and, for the harmonic normal viscosity, plastics, plasticn,
     plastic_yield).  markers: device vectors keyed like MARKERS (marker_xn,
     marker_yn as the scatter before the loop stored them); eta, sxx, sxy and
@@ -199,7 +221,12 @@ def picard_loop_markers(ctx, template: StokesProblem, state: dict, markers: dict
         t0 = time.perf_counter()
         solver.set_system(prob, fields={"etas0": ve["etas0"], "etan0": ve["etan0"], "rho1_vy": state["rho1_vy"],
                                         "RX1": ve["RX1"], "RY1": ve["RY1"], "RC1": ve["RC1"]})
-        S, st = solver.solve_system(**opts)
+        if warm_start and initial_guess is not None:
+            S, st = solver.solve_system(warm_start_vector(initial_guess[0], initial_guess[1], prob.pscale),
+                                        **{**opts, "use_initial_guess": 1})
+        else:
+            S, st = solver.solve_system(**opts)
+        initial_guess = (S, prob.pscale)
         solve_seconds = time.perf_counter() - t0
         vx1, vy1, pr1 = solver.process_stokes_solution(S)
         # stokes_solution_processing!: time step limit, grid stresses, marker gathers, norms
@@ -214,7 +241,7 @@ def picard_loop_markers(ctx, template: StokesProblem, state: dict, markers: dict
         log.append({"iglobal": iglobal, "iterations": st["iterations"], "relres": st["relres"],
                     "nmarker_failed": nfailed, "dvxy_rel_L2": dvxy_rel_L2, "timestep": dt,
                     "solve_seconds_with_setup": solve_seconds, "learned_accepted": st["learned_accepted"]})
-        sol = {"vx1": vx1, "vy1": vy1, "pr1": pr1, "S": S, "timestep": dt, **ss}
+        sol = {"vx1": vx1, "vy1": vy1, "pr1": pr1, "S": S, "pscale": prob.pscale, "timestep": dt, **ss}
         if check_stop_global_loop_markers(iglobal, nglobal, dvxy_rel_L2, tolerance):
             break
     torch.cuda.synchronize()

@@ -54,6 +54,7 @@ class DeviceModel:
     velocity_old: tuple | None = None
     solver: StokesSolverGPU | None = None
     thermal_bc: dict | None = None   # device thermal boundary conditions; None: no heat path
+    last_solution: tuple | None = None   # (S, pscale) of the last Stokes solve (warm starts)
 
 
 def upload(ctx, template: StokesProblem, markers: dict, mats: dict, params: dict,
@@ -122,10 +123,12 @@ def time_step(ctx, model: DeviceModel, solve_opts: dict | None = None, rk4_arith
                           viscosity_min=p["viscosity_min"], viscosity_max=p["viscosity_max"],
                           iuse_fluid_pressure_for_yield=p.get("iuse_fluid_pressure_for_yield", 0))
     node_based = int(p.get("itype_global", 1)) == 1
+    warm = bool(int(p.get("warm_start", 1)))   # solves start from the previous solution
     if node_based:
         sol, plog = picard_loop_nodes(ctx, model.template, gf, plastic_params, int(p["nglobal"]),
                                       float(p["tolerance_picard"]), solve_opts=solve_opts, solver=model.solver,
-                                      velocity_old=model.velocity_old)
+                                      velocity_old=model.velocity_old, warm_start=warm,
+                                      initial_guess=model.last_solution)
     else:   # PlasticityLoopMarkers.picard_loop (itype_global 0)
         mpar = {k: p.get(k, d) for k, d in (("y_sealevel", 0.0), ("yield_stress_min", 0.0),
                                              ("yield_stress_max", 1e32), ("no_yielding_in_mobile_wall", 0))}
@@ -134,7 +137,8 @@ def time_step(ctx, model: DeviceModel, solve_opts: dict | None = None, rk4_arith
                                         float(p["tolerance_picard"]), forecast=int(p.get("itype_plasticity", 0)),
                                         matid_domains=tuple(int(p.get(k, -1)) for k in (
                                             "matid_sticky_air", "matid_mobile_wall", "matid_plate_extension")),
-                                        solve_opts=solve_opts, solver=model.solver, velocity_old=model.velocity_old)
+                                        solve_opts=solve_opts, solver=model.solver, velocity_old=model.velocity_old,
+                                        warm_start=warm, initial_guess=model.last_solution)
     vx1, vy1, pr1 = sol["vx1"], sol["vy1"], sol["pr1"]
     if node_based:   # interpolate_grid_viscoplastic_viscosity_to_markers! (node-based plasticity only)
         rheology.calc_marker_viscoplastic_viscosity(ctx, m, gf["plastic_yield"], gf["etas1"])
@@ -177,6 +181,7 @@ def time_step(ctx, model: DeviceModel, solve_opts: dict | None = None, rk4_arith
     model.timestep = dt
     model.ntimestep = step
     model.velocity_old = (vx1, vy1)
+    model.last_solution = (sol["S"], sol["pscale"])
     torch.cuda.synchronize()
     return {"ntimestep": model.ntimestep, "timestep": dt, "seconds": time.perf_counter() - t_step,
             "picard_iterations": len(plog), "krylov_iterations": [r["iterations"] for r in plog],

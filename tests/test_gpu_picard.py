@@ -153,3 +153,32 @@ def test_picard_updates_timestep_after_each_solve(gpu_ctx):
         assert abs(a["nnode_yield"] - b["nnode_yield"]) <= max(2, b["nnode_yield"] // 100)
         assert a["global_yield_error"] == pytest.approx(b["global_yield_error"], rel=1e-3)
     assert gsol["timestep"] == glog[-1]["timestep"]
+
+
+def test_picard_warm_start_matches_oracle_with_fewer_iterations(gpu_ctx):
+    """Each solve started from the previous one's solution (pressures
+    rescaled to the new pscale): the same loop, yield counts and stopping
+    step as the oracle-composed loop, the velocity within the solve's
+    tolerance, and fewer Krylov iterations than cold starts."""
+    print("picard loop over nodes with warm-started solves", flush=True)
+    g, s, template = _setup()
+    nglobal, tol = 6, 1e-2
+    cs, csol, clog = _cpu_loop(g, s, template, nglobal, tol)
+    gpu_ctx.set_grid(g)
+    runs = {}
+    for warm in (False, True):
+        ds = {k: to_device(v) for k, v in s.items()}
+        runs[warm] = picard_loop_nodes(gpu_ctx, template, ds, PAR, nglobal, tol, solve_opts=dict(rtol=1e-11),
+                                       warm_start=warm)
+    gsol, glog = runs[True]
+    cold = sum(r["iterations"] for r in runs[False][1])
+    warm_it = sum(r["iterations"] for r in glog)
+    print(f"krylov iterations cold {cold} warm {warm_it}", flush=True)
+    assert len(glog) == len(clog)
+    for a, b in zip(glog, clog):
+        assert abs(a["nnode_yield"] - b["nnode_yield"]) <= max(2, b["nnode_yield"] // 100)
+        assert a["global_yield_error"] == pytest.approx(b["global_yield_error"], rel=1e-3)
+    for k in ("vx1", "vy1"):
+        ref = csol[k]
+        assert np.linalg.norm(to_host(gsol[k]) - ref) <= 1e-5 * np.linalg.norm(ref)
+    assert warm_it < cold
