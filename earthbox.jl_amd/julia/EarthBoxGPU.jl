@@ -153,7 +153,8 @@ end
 GPU replacement of SystemSolver.solve_system_optimized: solves the assembled
 system without building the COO, returning S in the reference's unknown order.
 """
-function solve_system_gpu!(st::GPUState, model; mixed::Bool=true, rtol::Float64=1e-8)
+function solve_system_gpu!(st::GPUState, model; mixed::Bool=true, rtol::Float64=1e-8,
+                           initial_guess::Union{Nothing, Tuple{Vector{Float64}, Float64}}=nothing)
     set_grid!(st, model)
     sc = model.stokes_continuity
     bcs = model.bcs.arrays
@@ -177,14 +178,21 @@ function solve_system_gpu!(st::GPUState, model; mixed::Bool=true, rtol::Float64=
         model.gravity.parameters.gravity_y.value,
         sc.parameters.build.iuse_interface_stabilization.value), "ebg_stokes_set_system")
     N = sc.parameters.build.N.value
-    S = zeros(Float64, N)
+    # initial_guess = (S, pscale) of an earlier solve (the previous Picard
+    # iterate or time step): a warm start; its scaled pressures (every third
+    # unknown, p / pscale) are rescaled to this system's pscale
+    pscale = sc.parameters.build.pscale.value
+    S = initial_guess === nothing ? zeros(Float64, N) : copy(initial_guess[1])
+    if initial_guess !== nothing && initial_guess[2] != pscale
+        S[3:3:end] .*= initial_guess[2] / pscale
+    end
     dS = h2d!(st, :S, S)
     o = Ref{SolveOpts}()
     ccall((:ebg_stokes_solve_opts_default, LIB), Cint, (Ref{SolveOpts},), o)
     # restart 100: the basis holds the plastic systems without a restart
     o[] = SolveOpts(mixed ? 1 : 0, 100, o[].max_iterations, rtol, o[].nlevels,
                     o[].finest_sweeps, o[].sweep_multiplier, o[].sweep_cap, o[].theta_s,
-                    o[].theta_c, 0, 0,
+                    o[].theta_c, initial_guess === nothing ? 0 : 1, 0,
                     (vc.btopx.array[2, 2], vc.bbottomx.array[2, 2], vc.blefty.array[2, 2],
                      vc.brighty.array[2, 2]), o[].smoother, o[].local_sweeps, o[].tile,
                     o[].flexible, o[].coarse_unknowns, o[].additive_tiles, o[].learned_patience)
