@@ -2,11 +2,15 @@
 """Benchmark of the EarthBox per-timestep hot path on one MI355X.
 
 Workload (1001 x 501 nodes over 1500 x 600 km, 6 x 6 markers per cell =
-18,000,000 markers, FP64): one step = one fused RK4 gather-advect pass of all
-markers (velocity at 4 stages, position update) with inputs resident in HBM,
-in the tolerance arithmetic by default (positions within 1e-10 of the domain
-size; the exact, bit-identical kernel is timed beside it).  Prints one JSON
-line.
+18,000,000 markers, FP64): the layered random-viscosity Stokes system of that
+grid is solved first (FP64 FGMRES to 1e-8), and its velocity field with the
+time step of half a cell of displacement (the Courant limit 0.5) drives the
+markers.  One step = one fused RK4 gather-advect pass of all markers
+(velocity at 4 stages, position update) from the markers' initial positions:
+every timed step advects its own copy of those positions, prepared in HBM
+before the timed region.  Tolerance arithmetic by default (positions within
+1e-10 of the domain size; the exact, bit-identical kernel is timed beside it).
+Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
@@ -30,7 +34,7 @@ RK4_BYTES_PER_MARKER = 32.0  # read x, y; write x, y (FP64)
 # 1001 x 501-node, 6 x 6 markers-per-cell workload (18M markers), each counter
 # scaled by its reading for a plain 512 MiB copy, keyed by kernel and workload.
 RK4_MEASURED_TRAFFIC = {
-    ("rk4_fast_kernel", 1001, 501, 6): 640.1e6,     # 352.3 MB read + 287.8 MB written
+    ("rk4_tol_kernel", 1001, 501, 6): 640.4e6,      # 352.4 MB read + 288.0 MB written
     ("rk4_advect_kernel", 1001, 501, 6): 641.3e6,   # 353.4 MB read + 288.0 MB written
 }
 
@@ -313,6 +317,25 @@ def bench_stokes(ctx, nx, ny, max_iter, amplitude=1.0):
         out["speedup_learned_mixed_vs_plain"] = p["seconds_with_setup"] / lm["seconds_with_setup"]
         out["speedup_learned_mixed_vs_plain_solve_only"] = p["seconds"] / lm["seconds"]
     return out
+
+
+def config2_velocity(ctx, nx, ny):
+    """The velocity field of the layered random-viscosity system on the
+    benchmark grid (1.0-decade perturbation, FP64 FGMRES with the FP32 V-cycle
+    to 1e-8), unpacked with its ghost BCs, and the time step of half a cell of
+    displacement (Update.jl:43-61 with marker_cell_displ_max = 0.5)."""
+    from earthbox_gpu import workloads
+    from earthbox_gpu.stokes import StokesSolverGPU, bc_mirror_factors, calculate_new_timestep
+    prob = workloads.layered_random(nx, ny, 1.5e6 * (nx - 1) / 1000, 6.0e5 * (ny - 1) / 500, seed=2,
+                                    amplitude=1.0)
+    sol = StokesSolverGPU(ctx)
+    sol.set_system(prob)
+    S, st = sol.solve_system(rtol=1e-8, max_iterations=3000, bc_d=bc_mirror_factors(prob), precision=1)
+    vx1, vy1, _ = sol.process_stokes_solution(S)
+    dt, _ = calculate_new_timestep(ctx, vx1, vy1, 1.0e30, 0.5)
+    field = {"system": "layered random log10(eta) in [18,25] (+-1.0 decades)", "relres": st["relres"],
+             "iterations": st["iterations"], "dt_s": dt}
+    return prob.grid, vx1, vy1, dt, field
 
 
 def bench_config1(ctx):
@@ -813,40 +836,45 @@ def main():
     from earthbox_gpu.grid import uniform_grid
 
     nx, ny, nm = args.xnum, args.ynum, args.markers_per_cell
-    g = uniform_grid(nx, ny, 1.5e6 * (nx - 1) / 1000, 6.0e5 * (ny - 1) / 500)
-    x, y = workloads.marker_positions(g, nm, nm, 2)
-    vx1, vy1 = workloads.random_velocity_fields(g, 12, vmax=1e-9)
     ctx = earthbox_gpu.GPUContext(0)
+    progress("rk4 velocity field: config-2 Stokes solve ...")
+    g, dvx1, dvy1, dt, field = config2_velocity(ctx, nx, ny)
     ctx.set_grid(g)
-    dvx1, dvy1 = to_device(vx1), to_device(vy1)
+    x, y = workloads.marker_positions(g, nm, nm, 2)
     desp = empty((ny, nx), np.float64)
     ctx.calculate_spin(dvx1, dvy1, desp)
     dx, dy = to_device(x), to_device(y)
-    dt = 0.5 * g.xstpavg / 1e-9
     n = x.size
 
-    def step():
-        ctx.advect_markers_and_rotate_stress_tensor(4, dt, None, dx, dy, None, None,
-                                                    dvx1, dvy1, desp)
-
-    def timed(mode):
+    def timed(mode, vx1=dvx1, vy1=dvy1, sp=desp, step_dt=dt, fresh=True):
         """W untimed steps, then K steps each bracketed by HIP events on the
-        library's stream (torch's current stream); wall time over the K."""
+        library's stream (torch's current stream); wall time over the K.
+        fresh: step k advects its own copy of the initial positions (copies
+        made before the timed region); otherwise every step advances the same
+        arrays."""
         ctx.set_rk4_arithmetic(mode)
-        for _ in range(args.warmup):
-            step()
+        nsteps = args.warmup + args.steps
+        if fresh:
+            pos = [(dx.clone(), dy.clone()) for _ in range(nsteps)]
+        else:
+            pos = [(dx.clone(), dy.clone())] * nsteps
+        for k in range(args.warmup):
+            ctx.advect_markers_and_rotate_stress_tensor(4, step_dt, None, pos[k][0], pos[k][1], None, None,
+                                                        vx1, vy1, sp)
         torch.cuda.synchronize()
         stream = torch.cuda.current_stream()
         evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                for _ in range(args.steps)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for s, e in evs:
+        for k, (s, e) in enumerate(evs):
+            px, py = pos[args.warmup + k]
             s.record(stream)
-            step()
+            ctx.advect_markers_and_rotate_stress_tensor(4, step_dt, None, px, py, None, None, vx1, vy1, sp)
             e.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        del pos
         return wall, float(np.mean([s.elapsed_time(e) for s, e in evs]))
 
     # the exact (bit-identical) kernel, reported beside the headline
@@ -862,7 +890,7 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     value = n / (ms_per_step * 1e-3) / 1e6
     achieved = RK4_BYTES_PER_MARKER * n / (kern_ms * 1e-3) / 1e9
-    kname = "rk4_fast_kernel" if fast else "rk4_advect_kernel"
+    kname = "rk4_tol_kernel" if fast else "rk4_advect_kernel"
     traffic = RK4_MEASURED_TRAFFIC.get((kname, nx, ny, nm))
     out = {
         "metric": "Stokes time-to-1e-8 rel. residual (learned+mixed vs FP64 MG); Mmarkers/s RK4",
@@ -877,8 +905,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"rk4_advect {nx}x{ny} nodes, {nm}x{nm} markers/cell ({n} markers)",
-                   "markers": n, "xnum": nx, "ynum": ny,
+        "config": {"workload": f"rk4_advect {nx}x{ny} nodes, {nm}x{nm} markers/cell ({n} markers), "
+                               f"config-2 Stokes velocity, dt = 0.5 cell / max|v|, each step from the "
+                               f"initial positions",
+                   "markers": n, "xnum": nx, "ynum": ny, "velocity_field": field,
                    "arithmetic": ("tolerance: FMA, nominal uniform coordinates; positions within 1e-10 "
                                   "of the domain size of the exact kernel" if fast else "exact")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -888,6 +918,20 @@ def main():
         "rk4_exact": rk4_exact,
     }
     progress(f"rk4 headline: {kern_ms:.3f} ms per launch ({kname})")
+    # the same markers under the synthetic random field (smoothed white noise,
+    # |v| <= 1e-9 m/s, dt = 0.5 cell / 1e-9), K steps on the same arrays, as a
+    # harder case for the LDS tiles: markers of a wavefront drift apart
+    rvx, rvy = workloads.random_velocity_fields(g, 12, vmax=1e-9)
+    drvx, drvy = to_device(rvx), to_device(rvy)
+    rsp = empty((ny, nx), np.float64)
+    ctx.calculate_spin(drvx, drvy, rsp)
+    rdt = 0.5 * g.xstpavg / 1e-9
+    wall_r, kern_r = timed(mode, drvx, drvy, rsp, rdt, fresh=False)
+    out["rk4_random_field"] = {"kernel": kname, "kernel_ms": kern_r, "ms_per_step": wall_r * 1e3 / args.steps,
+                               "frac_of_8TBs": RK4_BYTES_PER_MARKER * n / (kern_r * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                               "field": "smoothed white noise, |v| <= 1e-9 m/s, dt = 0.5 cell / 1e-9 m/s, "
+                                        "K steps advancing the same arrays"}
+    ctx.set_grid(g)
     legs = [
         ("marker_gathers", not args.no_marker_gathers, lambda: bench_marker_gathers(ctx, g, dx, dy)),
         ("stokes", not args.no_stokes,
@@ -912,7 +956,8 @@ def main():
         progress(f"{name} done in {time.perf_counter() - t0:.1f} s")
     if not args.no_cpu_baseline:
         progress("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(g, x, y, vx1, vy1, to_host_np(desp), dt, 2_000_000)
+        out["cpu_baseline"] = cpu_baseline(g, x, y, to_host_np(dvx1), to_host_np(dvy1), to_host_np(desp), dt,
+                                           2_000_000)
         if not args.no_sinker:
             out["cpu_baseline"]["mg3d_97_reference_driver"] = cpu_sinker_driver()
     print(json.dumps(out), flush=True)

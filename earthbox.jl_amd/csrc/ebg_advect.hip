@@ -266,79 +266,40 @@ __global__ void __launch_bounds__(256) rk4_advect_kernel(RK4Args a)
 // ---------------------------------------------------------------------------
 // Tolerance-mode RK4 on nominally uniform axes (both axes uniform to 1e-9 of
 // a step, checked when the grid is set).  Same stages, remaps, clamps and
-// outputs as rk4_advect_kernel, with three changes of arithmetic:
-//  - the cell search is one FMA: t2 = 2(x - x0)/h, and the half-cell index
-//    k = clamp(floor(t2), 0, 2n-3) gives the basic cell (k >> 1) and the
-//    velocity-grid remap (k & 1, the side of the cell centre) at once;
-//  - the normalized distances come from the nominal node positions, so the
-//    basic one is t - c and the staggered one differs from it by +-1/2;
-//  - the bilinear value is the nested linear interpolation in FMAs, and the
-//    RK4 sum is multiplied by 1/6.
-// Spin is gathered only when an output needs it (SPIN).  The bilinear field
-// is continuous, so a different cell choice at a node or centre (where the
-// nominal and the stored coordinates round differently) changes the value by
-// rounding only.  The grids are read through buffer descriptors (32-bit
-// offsets, no 64-bit address arithmetic).  On the MI355X the kernel is bound
-// by the vector memory address unit: each 16-byte corner-pair gather takes it
-// a cycle per four lanes (16 gathers per marker).  Two markers per thread
-// measured no faster; a per-wavefront LDS window of the grids gained 7 % on
-// markers in their initial order and lost 27 % once they had drifted.
+// outputs as rk4_advect_kernel, with these changes of arithmetic:
+//  - positions are carried in cell coordinates t = (x - x0)/h (one FMA from
+//    the nominal origin and reciprocal step), and each stage point is one FMA
+//    from the marker's t: t + (h_stage/h) v;
+//  - the basic cell is floor(t) (clamped to the grid), its normalized
+//    distance t - floor(t); the velocity-grid remap takes the staggered node
+//    of the cell's far half when that distance is at least 1/2 (computed
+//    cells) or above 1/2 (the stage-1 cells a caller passes), and its
+//    distance differs from the basic one by -+1/2;
+//  - the bilinear value is the nested linear interpolation in FMAs from each
+//    node's value and its difference to the next node down the column, and
+//    the RK4 sum is multiplied by 1/6.
+// The bilinear field is continuous, so a different cell choice at a node or
+// centre (where nominal and stored coordinates round differently) changes the
+// value by rounding only; positions stay within 1e-10 of the domain size of
+// the exact kernel's.
+//
+// Grid reads.  Markers kept in their creation order put a wavefront's 64
+// markers in a few neighbouring cells of one column, so the 16 corner pairs a
+// marker reads (24 with spin) come from a small patch of nodes.  Each
+// wavefront takes the cells of its first and last marker (or, when some of
+// its markers lie outside that box, the least and greatest cells of all of
+// them), widens that box by one cell on every side, and copies the patch of
+// every grid it reads into its
+// own LDS tile as (value, difference to the next node down the column) pairs,
+// with one 16-byte load per node.  A stage point whose cell lies in the box
+// reads its two pairs per grid from LDS; any other point (a marker far from
+// its neighbours in memory, a stage leaving the box) computes the clamped cell
+// and reads global memory, with the same arithmetic, so every marker's result
+// is the same whether its reads hit the tile or not, and the same whatever
+// the marker order.  A box larger than the tile sends the whole wavefront to
+// global reads.  Global reads go through buffer descriptors (32-bit offsets;
+// reads past a grid's end return zero and are never used).
 enum { RKF_RK4 = 1, RKF_XN = 2, RKF_STRESS = 4, RKF_SPIN = 8 };
-
-struct RK4FastArgs {
-    int64_t marknum;
-    double dt, half_dt;
-    double *x, *y;
-    const int32_t *xn, *yn;
-    const int8_t* flags;
-    const double *vx1, *vy1, *esp;
-    double *mvx, *mvy, *mspin;
-    double *sxx, *sxy;
-    int update_location;
-    int xnum, ynum;
-    double inv_hx2, nx02, inv_hy2, ny02;   // t2 = x * inv_h2 + n02 = 2 (x - x0) / h
-    double kmax_x, kmax_y;                 // 2n - 3: last half cell
-    double xmin, xmax, ymin, ymax;
-};
-
-struct FastCell {
-    int c;        // 0-based basic cell
-    int r;        // 1: right of (x) / below (y) the cell centre
-    double db;    // distance from the basic node, in cells
-    double ds;    // distance from the staggered node the remap picks
-};
-
-// +0.5 (r = 0) or -0.5 (r = 1) from the sign bit
-__device__ __forceinline__ double signed_half(int r)
-{
-    return __builtin_bit_cast(double, (unsigned long long)(0x3fe00000u | ((unsigned)r << 31)) << 32);
-}
-
-// t2 = 2t is exact (a doubled FMA), so db = t2/2 - c rounds as t - c
-__device__ __forceinline__ FastCell fast_cell(double x, double inv_h2, double n02, double kmax)
-{
-    FastCell f;
-    const double t2 = __builtin_fma(x, inv_h2, n02);
-    const int k = (int)fmin(fmax(t2, 0.0), kmax);   // NaN -> 0
-    f.c = k >> 1;
-    f.r = k & 1;
-    f.db = __builtin_fma(t2, 0.5, -(double)f.c);
-    f.ds = f.db + signed_half(f.r);
-    return f;
-}
-
-// Cell from a given 1-based basic index (the stage-1 marker_xn/yn, clamped
-// to the grid): the remap side follows from the nominal distance.
-__device__ __forceinline__ FastCell fast_cell_given(double x, int c1, double inv_h2, double n02, int n)
-{
-    FastCell f;
-    const double t2 = __builtin_fma(x, inv_h2, n02);
-    f.c = min(max(c1 - 1, 0), n - 2);
-    f.db = __builtin_fma(t2, 0.5, -(double)f.c);
-    f.r = f.db > 0.5;
-    f.ds = f.db + signed_half(f.r);
-    return f;
-}
 
 // Grid reads through buffer descriptors: 32-bit byte offsets (the launcher
 // keeps every grid below 2 GiB) and the column step as the scalar offset of
@@ -359,58 +320,199 @@ __device__ __forceinline__ double hi_f64(ebg_u32x4 v)
     return __builtin_bit_cast(double, ((unsigned long long)v.w << 32) | v.z);
 }
 
-// nested linear interpolation from the four corner values
-__device__ __forceinline__ double lerp4(double p00, double p10, double p01, double p11, double dy, double dx)
+
+struct RK4TolArgs {
+    int64_t marknum;
+    double dt;
+    double *x, *y;
+    const int32_t *xn, *yn;
+    const int8_t* flags;
+    const double *vx1, *vy1, *esp;
+    double *mvx, *mvy, *mspin;
+    double *sxx, *sxy;
+    int update_location;
+    int xnum, ynum;
+    double ixh, tx0, iyh, ty0;        // cell coordinates: t = x * ixh + tx0
+    double hdx, hdy, ddx, ddy;        // stage steps in cells per unit velocity: dt/2/h, dt/h
+    double xmin, xmax, ymin, ymax;
+    int tiles;                        // 0: every read from global memory
+};
+
+constexpr int RKT_WAVES = 4;   // wavefronts per 256-thread block
+constexpr int RKT_TILE = 160;  // node pairs per grid per wavefront (2.5 KB)
+
+// Cell of a stage point: c, remap side r, basic and staggered distances.
+struct TolCell {
+    int c, r;
+    double db, ds;
+};
+
+__device__ __forceinline__ double half_toward(bool r)   // -1/2 when r, else +1/2
 {
-    const double a = __builtin_fma(dy, p10 - p00, p00);
-    const double b = __builtin_fma(dy, p11 - p01, p01);
+    return r ? -0.5 : 0.5;
+}
+
+// clamp(floor(t), 0, n-2), clamped in FP64 before the conversion (a NaN t
+// gives cell 0)
+__device__ __forceinline__ TolCell tol_cell_clamped(double t, int nm2)
+{
+    TolCell p;
+    p.c = (int)fmin(fmax(__builtin_floor(t), 0.0), (double)nm2);
+    p.db = t - (double)p.c;
+    p.r = p.db >= 0.5;
+    p.ds = p.db + half_toward(p.r);
+    return p;
+}
+
+// the caller's 1-based stage-1 cell, clamped to the grid
+__device__ __forceinline__ TolCell tol_cell_given(double t, int c1, int nm2)
+{
+    TolCell p;
+    p.c = min(max(c1 - 1, 0), nm2);
+    p.db = t - (double)p.c;
+    p.r = p.db > 0.5;
+    p.ds = p.db + half_toward(p.r);
+    return p;
+}
+
+// nested linear interpolation from two (value, difference down the column)
+// pairs: q0 at the upper-left node, q1 one column to the right
+__device__ __forceinline__ double lerp_pairs(double2 q0, double2 q1, double dy, double dx)
+{
+    const double a = __builtin_fma(dy, q0.y, q0.x);
+    const double b = __builtin_fma(dy, q1.y, q1.x);
     return __builtin_fma(dx, b - a, a);
 }
 
-// the same at node idx (upper left) of a column-major grid with column step
-// ld8 bytes, read through its buffer descriptor
-__device__ __forceinline__ double lerp2(__amdgpu_buffer_rsrc_t r, int idx, int ld8, double dy, double dx)
+// the same from global memory: node idx (upper left) of a column-major grid
+// with column step ld8 bytes, read through its buffer descriptor
+__device__ __forceinline__ double lerp_global(__amdgpu_buffer_rsrc_t r, int idx, int ld8, double dy, double dx)
 {
     const ebg_u32x4 c0 = __builtin_amdgcn_raw_buffer_load_b128(r, idx << 3, 0, 0);
     const ebg_u32x4 c1 = __builtin_amdgcn_raw_buffer_load_b128(r, idx << 3, ld8, 0);
-    return lerp4(lo_f64(c0), hi_f64(c0), lo_f64(c1), hi_f64(c1), dy, dx);
+    const double p00 = lo_f64(c0), p01 = lo_f64(c1);
+    return lerp_pairs(make_double2(p00, hi_f64(c0) - p00), make_double2(p01, hi_f64(c1) - p01), dy, dx);
 }
 
-struct FastGrids {
+struct TolGrids {
     __amdgpu_buffer_rsrc_t vx, vy, sp;
     int ldx, ldy, ldx8, ldy8;
 };
 
+// A wavefront's LDS tile: cells X0 .. X0+ncv-1 by Y0 .. Y0+nrv-1 (ncv = 0:
+// no tile), nodes X0 .. X0+ncv+1 by Y0 .. Y0+P-1 in columns of P pairs.
+struct TolTile {
+    const double2 *vx, *vy, *sp;
+    int X0, Y0, P;
+    unsigned ncv, nrv;
+    double X0d, Y0d, ncvd, nrvd;   // the same in FP64
+};
+
+// global reads at a clamped (or given) cell: the fallback of the tile
 template <bool SPIN>
-__device__ __forceinline__ void fast_values(const FastGrids& g, const FastCell& cx, const FastCell& cy,
-                                            double& vx, double& vy, double& sp)
+__device__ __forceinline__ void tol_values_global(const TolGrids& g, const TolCell& px, const TolCell& py,
+                                                  double& vx, double& vy, double& sp)
 {
-    // vx grid: basic x cell, y remapped; vy grid: x remapped, basic y cell
-    vx = lerp2(g.vx, __umul24(cx.c, g.ldx) + cy.c + cy.r, g.ldx8, cy.ds, cx.db);
-    vy = lerp2(g.vy, __umul24(cx.c + cx.r, g.ldy) + cy.c, g.ldy8, cy.db, cx.ds);
-    if (SPIN) sp = lerp2(g.sp, __umul24(cx.c, g.ldy) + cy.c, g.ldy8, cy.db, cx.db);
+    vx = lerp_global(g.vx, __umul24(px.c, g.ldx) + py.c + py.r, g.ldx8, py.ds, px.db);
+    vy = lerp_global(g.vy, __umul24(px.c + px.r, g.ldy) + py.c, g.ldy8, py.db, px.ds);
+    if (SPIN) sp = lerp_global(g.sp, __umul24(px.c, g.ldy) + py.c, g.ldy8, py.db, px.db);
+}
+
+// tile reads at tile cell (lc, lr)
+template <bool SPIN>
+__device__ __forceinline__ void tol_values_tile(const TolTile& w, int lc, int lr, const TolCell& px,
+                                                const TolCell& py, double& vx, double& vy, double& sp)
+{
+    int i = lc * w.P + lr + py.r;   // vx: basic x cell, y remapped
+    vx = lerp_pairs(w.vx[i], w.vx[i + w.P], py.ds, px.db);
+    i = (lc + px.r) * w.P + lr;      // vy: x remapped, basic y cell
+    vy = lerp_pairs(w.vy[i], w.vy[i + w.P], py.db, px.ds);
+    if (SPIN) {
+        i = lc * w.P + lr;
+        sp = lerp_pairs(w.sp[i], w.sp[i + w.P], py.db, px.db);
+    }
+}
+
+// velocity (and spin) at the stage point with cell coordinates (tx, ty).
+// The tile test is done on floor(t) in FP64 (false for NaN), so the integer
+// conversion only sees cells of the box; any other point takes the clamped
+// cell, which equals floor(t) for every cell of the grid.
+template <bool SPIN>
+__device__ __forceinline__ void tol_values(const TolTile& w, const TolGrids& g, double tx, double ty, int nxm2,
+                                           int nym2, double& vx, double& vy, double& sp)
+{
+    const double fx = __builtin_floor(tx), fy = __builtin_floor(ty);
+    const double ox = fx - w.X0d, oy = fy - w.Y0d;
+    const bool in = (ox >= 0.0) & (ox < w.ncvd) & (oy >= 0.0) & (oy < w.nrvd);
+    if (in) {
+        TolCell px, py;
+        px.db = tx - fx;
+        py.db = ty - fy;
+        px.r = px.db >= 0.5;
+        py.r = py.db >= 0.5;
+        px.ds = px.db + half_toward(px.r);
+        py.ds = py.db + half_toward(py.r);
+        tol_values_tile<SPIN>(w, (int)ox, (int)oy, px, py, vx, vy, sp);
+    } else {
+        tol_values_global<SPIN>(g, tol_cell_clamped(tx, nxm2), tol_cell_clamped(ty, nym2), vx, vy, sp);
+    }
+}
+
+__device__ __forceinline__ bool tol_in_tile(const TolTile& w, int cx, int cy)
+{
+    return ((unsigned)(cx - w.X0) < w.ncv) & ((unsigned)(cy - w.Y0) < w.nrv);
+}
+
+// copy nodes (X0 + col, Y0 + row), col < nc, row < P, of one grid into a
+// tile as (value, next value down the column - value)
+__device__ __forceinline__ void tol_fill(double2* t, __amdgpu_buffer_rsrc_t r, int ld, int X0, int Y0, int P,
+                                         int ne, int lane)
+{
+    const float rP = __builtin_amdgcn_rcpf((float)P);
+    for (int e = lane; e < ne; e += 64) {
+        const int col = (int)(((float)e + 0.5f) * rP);   // floor(e / P): e < 2^7, |error| < 1/(2P)
+        const int row = e - col * P;
+        const ebg_u32x4 q = __builtin_amdgcn_raw_buffer_load_b128(r, ((X0 + col) * ld + Y0 + row) << 3, 0, 0);
+        t[e] = make_double2(lo_f64(q), hi_f64(q) - lo_f64(q));
+    }
+}
+
+// wavefront-level ordering of the tile writes before the reads of other lanes
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <int F>
-__global__ void __launch_bounds__(256) rk4_fast_kernel(RK4FastArgs a)
+__global__ void __launch_bounds__(256) rk4_tol_kernel(RK4TolArgs a)
 {
     constexpr bool RK4 = (F & RKF_RK4) != 0, XN = (F & RKF_XN) != 0;
     constexpr bool STRESS = (F & RKF_STRESS) != 0;
     constexpr bool SPIN = (F & (RKF_SPIN | RKF_STRESS)) != 0;   // the rotation needs it
+    constexpr int NG = SPIN ? 3 : 2;
+    __shared__ double2 tiles[RKT_WAVES][NG][RKT_TILE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= a.marknum) return;
-    const double x0 = a.x[m];
-    const double y0 = a.y[m];
-    bool inside;
-    if (a.flags) inside = (a.flags[m] == 1);
-    else inside = (a.xmin <= x0 && x0 <= a.xmax && a.ymin <= y0 && y0 <= a.ymax);
-    if (!inside) {  // VelocityAndSpin.jl:120-126; location and stress untouched
-        if (a.mvx) a.mvx[m] = 0.0;
-        if (a.mvy) a.mvy[m] = 0.0;
-        if (a.mspin) a.mspin[m] = 0.0;
-        return;
+    const bool live = m < a.marknum;
+    double x0 = 0.0, y0 = 0.0;
+    if (live) {
+        x0 = a.x[m];
+        y0 = a.y[m];
     }
-    FastGrids g;
+    bool inside = false;
+    if (live) {
+        if (a.flags) inside = (a.flags[m] == 1);
+        else inside = (a.xmin <= x0) & (x0 <= a.xmax) & (a.ymin <= y0) & (y0 <= a.ymax);
+        if (!inside) {  // VelocityAndSpin.jl:120-126; location and stress untouched
+            if (a.mvx) a.mvx[m] = 0.0;
+            if (a.mvy) a.mvy[m] = 0.0;
+            if (a.mspin) a.mspin[m] = 0.0;
+        }
+    }
+    const int nxm2 = a.xnum - 2, nym2 = a.ynum - 2;
+    TolGrids g;
     g.ldx = a.ynum + 1;
     g.ldy = a.ynum;
     g.ldx8 = g.ldx * 8;
@@ -418,37 +520,72 @@ __global__ void __launch_bounds__(256) rk4_fast_kernel(RK4FastArgs a)
     g.vx = grid_rsrc(a.vx1, (int64_t)g.ldx * a.xnum);
     g.vy = grid_rsrc(a.vy1, (int64_t)g.ldy * (a.xnum + 1));
     if (SPIN) g.sp = grid_rsrc(a.esp, (int64_t)g.ldy * a.xnum);
-    FastCell cx, cy;
-    if (XN) {
-        cx = fast_cell_given(x0, a.xn[m], a.inv_hx2, a.nx02, a.xnum);
-        cy = fast_cell_given(y0, a.yn[m], a.inv_hy2, a.ny02, a.ynum);
-    } else {
-        cx = fast_cell(x0, a.inv_hx2, a.nx02, a.kmax_x);
-        cy = fast_cell(y0, a.inv_hy2, a.ny02, a.kmax_y);
+    const double tx = __builtin_fma(x0, a.ixh, a.tx0);
+    const double ty = __builtin_fma(y0, a.iyh, a.ty0);
+    // the tile box: the cells of the wavefront's first and last marker; when
+    // a marker inside the domain lies outside that box, the least and
+    // greatest cells of all its markers inside the domain.  Widened by one
+    // cell and clipped to the grid.
+    const int ix = (int)fmin(fmax(tx, -2.0), 1.0e9), iy = (int)fmin(fmax(ty, -2.0), 1.0e9);
+    const int fx = __builtin_amdgcn_readlane(ix, 0), lx = __builtin_amdgcn_readlane(ix, 63);
+    const int fy = __builtin_amdgcn_readlane(iy, 0), ly = __builtin_amdgcn_readlane(iy, 63);
+    int bx0 = min(fx, lx), bx1 = max(fx, lx), by0 = min(fy, ly), by1 = max(fy, ly);
+    const bool outside_box = inside & ((ix < bx0) | (ix > bx1) | (iy < by0) | (iy > by1));
+    if (__ballot(outside_box) != 0) {
+        bx0 = __reduce_min_sync(~0ull, inside ? ix : 0x7fffffff);
+        bx1 = __reduce_max_sync(~0ull, inside ? ix : -1);
+        by0 = __reduce_min_sync(~0ull, inside ? iy : 0x7fffffff);
+        by1 = __reduce_max_sync(~0ull, inside ? iy : -1);
     }
+    TolTile w;
+    w.X0 = max(bx0 - 1, 0);
+    w.Y0 = max(by0 - 1, 0);
+    const int ncv = min(bx1 + 1, nxm2) - w.X0 + 1;
+    const int nrv = min(by1 + 1, nym2) - w.Y0 + 1;
+    w.P = nrv + 2;
+    const bool on = a.tiles && ncv >= 1 && nrv >= 1 && ncv <= RKT_TILE && nrv <= RKT_TILE &&
+                    (ncv + 2) * w.P <= RKT_TILE;
+    w.ncv = on ? (unsigned)ncv : 0u;
+    w.nrv = on ? (unsigned)nrv : 0u;
+    w.X0d = (double)w.X0;
+    w.Y0d = (double)w.Y0;
+    w.ncvd = (double)w.ncv;
+    w.nrvd = (double)w.nrv;
+    w.vx = tiles[wave][0];
+    w.vy = tiles[wave][1];
+    w.sp = tiles[wave][NG - 1];
+    if (on) {
+        const int ne = (ncv + 2) * w.P;
+        tol_fill(tiles[wave][0], g.vx, g.ldx, w.X0, w.Y0, w.P, ne, lane);
+        tol_fill(tiles[wave][1], g.vy, g.ldy, w.X0, w.Y0, w.P, ne, lane);
+        if (SPIN) tol_fill(tiles[wave][NG - 1], g.sp, g.ldy, w.X0, w.Y0, w.P, ne, lane);
+    }
+    wave_lds_sync();
+    if (!inside) return;
     double vx, vy, vs = 0.0;
-    fast_values<SPIN>(g, cx, cy, vx, vy, vs);
+    if (XN) {
+        const TolCell px = tol_cell_given(tx, a.xn[m], nxm2);
+        const TolCell py = tol_cell_given(ty, a.yn[m], nym2);
+        if (tol_in_tile(w, px.c, py.c)) tol_values_tile<SPIN>(w, px.c - w.X0, py.c - w.Y0, px, py, vx, vy, vs);
+        else tol_values_global<SPIN>(g, px, py, vx, vy, vs);
+    } else {
+        tol_values<SPIN>(w, g, tx, ty, nxm2, nym2, vx, vy, vs);
+    }
     double svx = vx, svy = vy, ssp = vs;
     if (RK4) {
         double ax = vx, ay = vy, as = vs;
-        double xc = __builtin_fma(a.half_dt, vx, x0);
-        double yc = __builtin_fma(a.half_dt, vy, y0);
-        fast_values<SPIN>(g, fast_cell(xc, a.inv_hx2, a.nx02, a.kmax_x),
-                          fast_cell(yc, a.inv_hy2, a.ny02, a.kmax_y), vx, vy, vs);
+        tol_values<SPIN>(w, g, __builtin_fma(a.hdx, vx, tx), __builtin_fma(a.hdy, vy, ty), nxm2, nym2, vx, vy,
+                         vs);
         ax = __builtin_fma(2.0, vx, ax);
         ay = __builtin_fma(2.0, vy, ay);
         if (SPIN) as = __builtin_fma(2.0, vs, as);
-        xc = __builtin_fma(a.half_dt, vx, x0);
-        yc = __builtin_fma(a.half_dt, vy, y0);
-        fast_values<SPIN>(g, fast_cell(xc, a.inv_hx2, a.nx02, a.kmax_x),
-                          fast_cell(yc, a.inv_hy2, a.ny02, a.kmax_y), vx, vy, vs);
+        tol_values<SPIN>(w, g, __builtin_fma(a.hdx, vx, tx), __builtin_fma(a.hdy, vy, ty), nxm2, nym2, vx, vy,
+                         vs);
         ax = __builtin_fma(2.0, vx, ax);
         ay = __builtin_fma(2.0, vy, ay);
         if (SPIN) as = __builtin_fma(2.0, vs, as);
-        xc = __builtin_fma(a.dt, vx, x0);
-        yc = __builtin_fma(a.dt, vy, y0);
-        fast_values<SPIN>(g, fast_cell(xc, a.inv_hx2, a.nx02, a.kmax_x),
-                          fast_cell(yc, a.inv_hy2, a.ny02, a.kmax_y), vx, vy, vs);
+        tol_values<SPIN>(w, g, __builtin_fma(a.ddx, vx, tx), __builtin_fma(a.ddy, vy, ty), nxm2, nym2, vx, vy,
+                         vs);
         constexpr double r6 = 1.0 / 6.0;
         svx = (ax + vx) * r6;
         svy = (ay + vy) * r6;
@@ -473,15 +610,15 @@ __global__ void __launch_bounds__(256) rk4_fast_kernel(RK4FastArgs a)
     }
 }
 
-static const void* const rk4_fast_kernels[16] = {
-    (const void*)rk4_fast_kernel<0>,  (const void*)rk4_fast_kernel<1>,
-    (const void*)rk4_fast_kernel<2>,  (const void*)rk4_fast_kernel<3>,
-    (const void*)rk4_fast_kernel<4>,  (const void*)rk4_fast_kernel<5>,
-    (const void*)rk4_fast_kernel<6>,  (const void*)rk4_fast_kernel<7>,
-    (const void*)rk4_fast_kernel<8>,  (const void*)rk4_fast_kernel<9>,
-    (const void*)rk4_fast_kernel<10>, (const void*)rk4_fast_kernel<11>,
-    (const void*)rk4_fast_kernel<12>, (const void*)rk4_fast_kernel<13>,
-    (const void*)rk4_fast_kernel<14>, (const void*)rk4_fast_kernel<15>};
+static const void* const rk4_tol_kernels[16] = {
+    (const void*)rk4_tol_kernel<0>,  (const void*)rk4_tol_kernel<1>,
+    (const void*)rk4_tol_kernel<2>,  (const void*)rk4_tol_kernel<3>,
+    (const void*)rk4_tol_kernel<4>,  (const void*)rk4_tol_kernel<5>,
+    (const void*)rk4_tol_kernel<6>,  (const void*)rk4_tol_kernel<7>,
+    (const void*)rk4_tol_kernel<8>,  (const void*)rk4_tol_kernel<9>,
+    (const void*)rk4_tol_kernel<10>, (const void*)rk4_tol_kernel<11>,
+    (const void*)rk4_tol_kernel<12>, (const void*)rk4_tol_kernel<13>,
+    (const void*)rk4_tol_kernel<14>, (const void*)rk4_tol_kernel<15>};
 
 __global__ void inside_flags_kernel(int64_t n, const double* __restrict__ x,
                                     const double* __restrict__ y, double xmin, double xmax,
@@ -599,10 +736,9 @@ int ebg_rk4_advect(ebg_ctx* ctx, int64_t marknum, int order, double timestep,
     EBG_CHECK(std::isfinite(timestep), EBG_ERR_ARG, "ebg_rk4_advect: timestep not finite");
     const Grid2D& g = ctx->grid;
     if (ctx->rk4_arithmetic == EBG_RK4_TOLERANCE && fast_kernel_fits(g)) {
-        RK4FastArgs a;
+        RK4TolArgs a;
         a.marknum = marknum;
         a.dt = timestep;
-        a.half_dt = timestep / 2.0;
         a.x = marker_x;
         a.y = marker_y;
         a.xn = marker_xn;
@@ -619,22 +755,24 @@ int ebg_rk4_advect(ebg_ctx* ctx, int64_t marknum, int order, double timestep,
         a.update_location = update_location;
         a.xnum = (int)g.xnum;
         a.ynum = (int)g.ynum;
-        // doubled exactly: t2 = 2t
-        a.inv_hx2 = 2.0 * g.inv_hx;
-        a.nx02 = 2.0 * (-g.x0 * g.inv_hx);
-        a.inv_hy2 = 2.0 * g.inv_hy;
-        a.ny02 = 2.0 * (-g.y0 * g.inv_hy);
-        a.kmax_x = (double)(2 * g.xnum - 3);
-        a.kmax_y = (double)(2 * g.ynum - 3);
+        a.ixh = g.inv_hx;
+        a.tx0 = -g.x0 * g.inv_hx;
+        a.iyh = g.inv_hy;
+        a.ty0 = -g.y0 * g.inv_hy;
+        a.hdx = (timestep / 2.0) * g.inv_hx;
+        a.hdy = (timestep / 2.0) * g.inv_hy;
+        a.ddx = timestep * g.inv_hx;
+        a.ddy = timestep * g.inv_hy;
         a.xmin = g.xmin;
         a.xmax = g.xmax;
         a.ymin = g.ymin;
         a.ymax = g.ymax;
+        a.tiles = ctx->rk4_tiles;
         const int f = (order == 4 ? RKF_RK4 : 0) | (marker_xn ? RKF_XN : 0) |
                       (marker_sxx ? RKF_STRESS : 0) | (marker_spin ? RKF_SPIN : 0);
         void* args[] = {&a};
-        EBG_HIP(hipLaunchKernel(rk4_fast_kernels[f], dim3(blocks_for(marknum, 256)), dim3(256), args,
-                                0, ctx->stream));
+        EBG_HIP(hipLaunchKernel(rk4_tol_kernels[f], dim3(blocks_for(marknum, 256)), dim3(256), args, 0,
+                                ctx->stream));
         EBG_HIP(hipGetLastError());
         return EBG_OK;
     }
@@ -680,6 +818,14 @@ int ebg_rk4_set_arithmetic(ebg_ctx* ctx, int mode)
     EBG_CHECK(mode == EBG_RK4_EXACT || mode == EBG_RK4_TOLERANCE, EBG_ERR_ARG,
               "ebg_rk4_set_arithmetic: mode must be EBG_RK4_EXACT or EBG_RK4_TOLERANCE");
     ctx->rk4_arithmetic = mode;
+    return EBG_OK;
+}
+
+int ebg_rk4_set_tiling(ebg_ctx* ctx, int on)
+{
+    EBG_CHECK(ctx != nullptr, EBG_ERR_ARG, "ebg_rk4_set_tiling: ctx is NULL");
+    EBG_CHECK(on == 0 || on == 1, EBG_ERR_ARG, "ebg_rk4_set_tiling: on must be 0 or 1");
+    ctx->rk4_tiles = on;
     return EBG_OK;
 }
 

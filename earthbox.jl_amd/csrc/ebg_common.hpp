@@ -86,6 +86,7 @@ struct ebg_ctx {
     void* heat = nullptr;      // heat-equation solve state (ebg_heat.hip)
     void* subgrid = nullptr;   // per-marker buffers of the sub-grid diffusion (ebg_subgrid.hip)
     int rk4_arithmetic = EBG_RK4_EXACT;   // ebg_rk4_set_arithmetic
+    int rk4_tiles = 1;                     // ebg_rk4_set_tiling
 };
 
 namespace ebg {

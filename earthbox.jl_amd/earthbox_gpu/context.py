@@ -128,9 +128,16 @@ class GPUContext:
 
     def set_rk4_arithmetic(self, mode: int) -> None:
         """RK4_EXACT: the reference's FP64 evaluation order, bit-identical;
-        RK4_TOLERANCE: FMAs and nominal uniform coordinates (fast kernel, used
-        when both axes are uniform to 1e-9 of a step)."""
+        RK4_TOLERANCE: FMAs in nominal uniform cell coordinates with the grid
+        patch of each wavefront staged in LDS (used when both axes are uniform
+        to 1e-9 of a step)."""
         check(lib.ebg_rk4_set_arithmetic(self._h, int(mode)), "ebg_rk4_set_arithmetic")
+
+    def set_rk4_tiling(self, on: bool) -> None:
+        """Tolerance arithmetic only: True (default) serves the stage reads of
+        each wavefront from an LDS copy of its grid patch, False reads global
+        memory; the results are identical."""
+        check(lib.ebg_rk4_set_tiling(self._h, int(bool(on))), "ebg_rk4_set_tiling")
 
     def rk4_tolerance_kernel_available(self) -> bool:
         return lib.ebg_rk4_tolerance_kernel_available(self._h) == 1

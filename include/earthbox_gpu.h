@@ -101,11 +101,13 @@ int ebg_marker_cell_indices(ebg_ctx *ctx, int64_t marknum, const double *marker_
  *   EBG_RK4_EXACT (default): FP64 with the reference's evaluation order (no
  *     contraction, correctly rounded divisions), so velocities, spins and
  *     positions are bit-identical to the reference loops;
- *   EBG_RK4_TOLERANCE: FP64 with fused multiply-adds and the axes' nominal
- *     uniform coordinates (reciprocal step instead of division), used when
- *     both axes are uniform to 1e-9 of a step (else the exact kernel runs).
- *     Positions agree with the exact mode to far below 1e-10 of the domain
- *     size; the stage-1 cells given in marker_xn/yn are used as given. */
+ *   EBG_RK4_TOLERANCE: FP64 with fused multiply-adds in cell coordinates
+ *     from the axes' nominal uniform origin and step (reciprocal step
+ *     instead of division), used when both axes are uniform to 1e-9 of a
+ *     step (else the exact kernel runs); each wavefront's patch of the
+ *     grids is staged in LDS (ebg_rk4_set_tiling).  Positions agree with the
+ *     exact mode to far below 1e-10 of the domain size; the stage-1 cells
+ *     given in marker_xn/yn are used as given. */
 int ebg_rk4_advect(ebg_ctx *ctx, int64_t marknum, int order, double timestep,
                    double *marker_x, double *marker_y,
                    const int32_t *marker_xn, const int32_t *marker_yn,
@@ -118,6 +120,13 @@ int ebg_rk4_advect(ebg_ctx *ctx, int64_t marknum, int order, double timestep,
  * in the reference, whose RK4 has one arithmetic). */
 enum { EBG_RK4_EXACT = 0, EBG_RK4_TOLERANCE = 1 };
 int ebg_rk4_set_arithmetic(ebg_ctx *ctx, int mode);
+
+/* EBG_RK4_TOLERANCE only: on = 1 (default) stages each wavefront's patch of
+ * the velocity (and spin) grids in LDS and serves the stage reads from it;
+ * on = 0 reads every stage value from global memory.  Both give identical
+ * results.  No counterpart in the reference (a tuning switch of this GPU
+ * kernel). */
+int ebg_rk4_set_tiling(ebg_ctx *ctx, int on);
 
 /* 1 when the current grid lets EBG_RK4_TOLERANCE use its fast kernel (both
  * axes uniform to 1e-9 of a step), else 0; -1 without a grid. */

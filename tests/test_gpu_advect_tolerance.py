@@ -5,9 +5,11 @@ size of the oracle's; velocities and spins within 1e-9 of the largest grid
 velocity / spin (a swapped remap, corner or stage gives O(1) errors); markers
 outside the domain get zero velocity and keep their position and stress.  On
 a grid that is not uniform the mode falls back to the exact kernel, which is
-bit-identical to the oracle.  Each thread advances two markers with their
-stages interleaved; a shuffled marker order gives bit-identical results to
-the spatially ordered one.
+bit-identical to the oracle.  The kernel stages each wavefront's grid patch
+in LDS when its markers sit in a few neighbouring cells and reads global
+memory otherwise: a shuffled marker order (every wavefront on global reads)
+and the kernel with its LDS tiles switched off give results bit-identical to
+the spatially ordered markers on tiles.
 """
 import numpy as np
 import pytest
@@ -184,3 +186,28 @@ def test_rk4_tolerance_marker_order_independent(tol_ctx, stress, courant):
         outs.append({k: v[inv] for k, v in r.items()})
     for k in outs[0]:
         assert np.array_equal(outs[0][k], outs[1][k], equal_nan=True), k
+
+
+@pytest.mark.parametrize("given_cells,stress,outputs,courant",
+                         [(False, False, False, 0.5), (True, False, True, 0.5), (False, True, True, 0.5),
+                          (True, True, True, 1.5), (False, False, True, 3.0)])
+def test_rk4_tolerance_lds_tiles_bit_identical(tol_ctx, given_cells, stress, outputs, courant):
+    """The LDS-tiled kernel against the same kernel reading every stage value
+    from global memory: all outputs bit-identical, in the reference's marker
+    order (tiles used; stage cells more than a cell away read global memory
+    per lane at Courant 1.5 and 3), with markers off the grid, on nodes and
+    on the domain edge, on grids whose node counts are not multiples of the
+    tile shape."""
+    g, x, y, vx1, vy1, esp, _ = _case(203, 61, 5, 11)
+    dt = courant * g.xstpavg / 1e-9
+    print(f"rk4 tiles vs global given={given_cells} stress={stress} courant={courant}", flush=True)
+    res = []
+    try:
+        for tiles in (True, False):
+            tol_ctx.set_rk4_tiling(tiles)
+            got, _, _ = _run(tol_ctx, g, x, y, vx1, vy1, esp, dt, given_cells, stress, outputs)
+            res.append(got)
+    finally:
+        tol_ctx.set_rk4_tiling(True)
+    for k in res[0]:
+        assert np.array_equal(res[0][k], res[1][k], equal_nan=True), k
