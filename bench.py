@@ -566,14 +566,27 @@ def bench_scatter(ctx, g, x, y, reps=3):
          "matid": to_device(rng.integers(1, 4, n).astype(np.int16)),
          "mat_mu": to_device(np.array([3e10, 6e10, 1e10]))}
     f = new_grid_fields(g.xnum, g.ynum, 1.0)
-    interpolate_marker_stokes_parameters_to_grid(ctx, m, f)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
+    # algorithmic bytes: x, y, rho, eta (8 B each) and matid (2 B) per marker,
+    # and the grid outputs written once (rho1, etas1, mus1, wtnodes, wtetas;
+    # etan1, mun1, wtetan; rho1_vy, wtnodes_vy)
+    nb, npr, nvy = g.xnum * g.ynum, (g.xnum - 1) * (g.ynum - 1), g.ynum * (g.xnum + 1)
+    alg = 34 * n + 8 * (5 * nb + 3 * npr + 2 * nvy)
+    out = {"markers": n, "algorithmic_bytes": alg}
+    for name, order in (("serial_order", ctx.SCATTER_SERIAL_ORDER), ("cell_order", ctx.SCATTER_CELL_ORDER)):
+        ctx.set_scatter_order(order)
         interpolate_marker_stokes_parameters_to_grid(ctx, m, f)
-    torch.cuda.synchronize()
-    el = (time.perf_counter() - t0) / reps
-    return {"ms": el * 1e3, "Mmarkers_per_s": n / el / 1e6, "markers": n}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            interpolate_marker_stokes_parameters_to_grid(ctx, m, f)
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / reps
+        out[name] = {"ms": el * 1e3, "Mmarkers_per_s": n / el / 1e6,
+                     "frac_of_8TBs": alg / el / 1e9 / HBM_PEAK_GBS}
+    ctx.set_scatter_order(ctx.SCATTER_SERIAL_ORDER)
+    out["ms"] = out["serial_order"]["ms"]
+    out["speedup_cell_vs_serial"] = out["serial_order"]["ms"] / out["cell_order"]["ms"]
+    return out
 
 
 def bench_grid_kernels(ctx, reps=20):

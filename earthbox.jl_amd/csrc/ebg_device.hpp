@@ -37,4 +37,34 @@ __device__ __forceinline__ int left_node_index(double x, const double* __restric
     return c;
 }
 
+// Correctly rounded a/b given rb = RN(1/b): q0 = RN(a*rb) is within 1.5 ulp
+// of a/b, the first remainder step makes it faithful, and Markstein's theorem
+// (P. Markstein, IBM J. Res. Dev. 34(1), 1990; Muller et al., Handbook of
+// Floating-Point Arithmetic, "Newton-Raphson-based division with an FMA") makes
+// the second step return RN(a/b): with rb within half an ulp of 1/b and q
+// within one ulp of a/b, r = a - b*q is exact and RN(q + r*rb) = RN(a/b).
+// The theorem needs no underflow or overflow in r and q: callers use it only
+// for numerators that pass div_in_range() and divisors in [2^-100, 2^100]
+// (ebg_grid2d_set rejects grid steps outside that range), and take the plain
+// IEEE division otherwise, so the result always equals a / b bit for bit.
+// The divisors are positive, so a / b has the sign of a; copying that sign
+// onto q also turns the +0 the steps return for a = -0 into -0.
+__device__ __forceinline__ double rn_div(double a, double b, double rb)
+{
+    double q = a * rb;
+    double r = __builtin_fma(-b, q, a);
+    q = __builtin_fma(r, rb, q);
+    r = __builtin_fma(-b, q, a);
+    q = __builtin_fma(r, rb, q);
+    return __builtin_copysign(q, a);
+}
+
+// Numerators for which rn_div is exact with divisors and reciprocals between
+// 2^-100 and 2^100 (grid steps): zero, or magnitude in [2^-900, 2^900].
+__device__ __forceinline__ bool div_in_range(double a)
+{
+    const double m = fabs(a);
+    return m == 0.0 || (m >= 0x1p-900 && m <= 0x1p900);
+}
+
 }  // namespace ebg

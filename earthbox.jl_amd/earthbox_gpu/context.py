@@ -139,6 +139,15 @@ class GPUContext:
         memory; the results are identical."""
         check(lib.ebg_rk4_set_tiling(self._h, int(bool(on))), "ebg_rk4_set_tiling")
 
+    SCATTER_SERIAL_ORDER, SCATTER_CELL_ORDER = 0, 1
+
+    def set_scatter_order(self, order: int) -> None:
+        """SCATTER_SERIAL_ORDER (default): node sums in the reference's serial
+        marker order, bit-identical; SCATTER_CELL_ORDER: per-cell partial sums
+        in marker order added per node in a fixed order (deterministic,
+        within a few ulps of the sum of the absolute terms)."""
+        check(lib.ebg_scatter_set_order(self._h, int(order)), "ebg_scatter_set_order")
+
     def rk4_tolerance_kernel_available(self) -> bool:
         return lib.ebg_rk4_tolerance_kernel_available(self._h) == 1
 

@@ -179,6 +179,18 @@ int ebg_interpolate_marker_stokes_parameters_to_grid(ebg_ctx *ctx,
                                                      const ebg_marker_stokes_props *props,
                                                      ebg_stokes_grid_fields *fields);
 
+/* Summation order of both scatters on this context (no counterpart in the
+ * reference, whose serial loops have one order):
+ *   EBG_SCATTER_SERIAL_ORDER (default): every node sum in the reference's
+ *     serial marker order, bit-identical to the serial loops;
+ *   EBG_SCATTER_CELL_ORDER: each cell's markers summed privately in marker
+ *     order into per-corner partial sums, each node sum the fixed-order sum
+ *     of its neighbouring cells' partials; deterministic, no floating-point
+ *     atomics, within a few ulps of the sum of the absolute terms of the
+ *     serial order (pressure nodes bit-identical), several times faster. */
+enum { EBG_SCATTER_SERIAL_ORDER = 0, EBG_SCATTER_CELL_ORDER = 1 };
+int ebg_scatter_set_order(ebg_ctx *ctx, int order);
+
 /* ---- marker -> grid heat-transport properties ----
  * DEVICE marker arrays (length marknum): x, y; inside_flags optional (NULL:
  * computed from the grid limits); TK, kt, rhocp, ha; matid (1-based) with the
