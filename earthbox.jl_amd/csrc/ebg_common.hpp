@@ -87,6 +87,7 @@ struct ebg_ctx {
     void* subgrid = nullptr;   // per-marker buffers of the sub-grid diffusion (ebg_subgrid.hip)
     int rk4_arithmetic = EBG_RK4_EXACT;   // ebg_rk4_set_arithmetic
     int rk4_tiles = 1;                     // ebg_rk4_set_tiling
+    int stokes_graphs = 1;                 // ebg_stokes_set_graphs
 };
 
 namespace ebg {

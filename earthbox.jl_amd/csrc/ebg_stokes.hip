@@ -486,6 +486,60 @@ __global__ void __launch_bounds__(256) maxmin2_kernel(const double* __restrict__
 }
 
 // ------------------------------------------------------------------ state
+// A captured V-cycle.  The launch sequence of the tiled V-cycle depends only
+// on the hierarchy (its device buffers, tiling and coarse solve) and the
+// smoother settings, not on the vector being preconditioned (the right-hand
+// side is packed into, and the correction unpacked from, the finest level's
+// own arrays outside the graph).  It is captured once into a HIP graph on a
+// private stream and replayed on the context's stream, which removes the host
+// launch gaps between its ~70 short kernels.  The key records everything the
+// captured launches read from the host; any change re-captures.  The additive
+// smoother swaps each level's two solution buffers after every launch, an
+// even number of times per V-cycle, so the host's buffer assignment after a
+// V-cycle equals the one before it and a replay leaves the correction where
+// the unpacking reads it.
+struct VcycleGraph {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap = nullptr;
+    ~VcycleGraph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+};
+
+template <typename T>
+static std::vector<uint64_t> vcycle_key(const MGHierarchy<T>& H, const std::vector<int>& sm, const ebg_solve_opts& o,
+                                        const T* prn, const T (&bcd)[4])
+{
+    std::vector<uint64_t> k;
+    auto put = [&](const void* p, size_t bytes) {
+        const size_t w = (bytes + 7) / 8;
+        const size_t at = k.size();
+        k.resize(at + w, 0);
+        std::memcpy(k.data() + at, p, bytes);
+    };
+    auto u = [&](uint64_t v) { k.push_back(v); };
+    u((uint64_t)(uintptr_t)&H);
+    for (const auto& l : H.lev) put(&l, sizeof(l));     // ints and pointers, no padding
+    for (const auto& x : H.xfer) put(&x, sizeof(x));    // pointers only
+    for (const auto& t : H.tiles) {
+        u((uint64_t)t.ti); u((uint64_t)t.tj); u((uint64_t)t.nti); u((uint64_t)t.ntj);
+        u((uint64_t)t.lds); u((uint64_t)t.whole_ok); u((uint64_t)t.whole_lds);
+    }
+    u((uint64_t)H.fast_div); u((uint64_t)H.additive_tiles); u((uint64_t)H.normalize_finest);
+    u((uint64_t)(uintptr_t)H.coarse_inv); u((uint64_t)(uintptr_t)H.coarse_b);
+    u((uint64_t)(uintptr_t)H.coarse_x); u((uint64_t)H.coarse_n); u((uint64_t)(uintptr_t)H.scratch);
+    for (int v : sm) u((uint64_t)(int64_t)v);
+    u((uint64_t)o.local_sweeps);
+    const T th[2] = {(T)o.theta_s, (T)o.theta_c};
+    put(th, sizeof(th));
+    put(bcd, sizeof(bcd));
+    u((uint64_t)(uintptr_t)prn);
+    return k;
+}
+
 struct StokesState {
     int xnum = 0, ynum = 0, nc = 0;
     StokesOpDev op{};
@@ -518,7 +572,62 @@ struct StokesState {
     }
     // learned correction: candidate vector, judge residuals and acceptance count
     DevBuf zc, jr0, jr1, judge, pair_e, pair_r, pair_z;
+    // captured V-cycles per precision (ebg_stokes_set_graphs); a failed
+    // capture switches them off for this state
+    bool use_graphs = true, graph_broken = false;
+    VcycleGraph vg_f, vg_d;
 };
+
+template <typename T> static VcycleGraph& vgraph(StokesState& st);
+template <> VcycleGraph& vgraph<float>(StokesState& st) { return st.vg_f; }
+template <> VcycleGraph& vgraph<double>(StokesState& st) { return st.vg_d; }
+
+// the tiled V-cycle, replayed from its captured graph when graphs are on;
+// a capture that fails turns graphs off for this state and runs eagerly
+template <typename T>
+static void vcycle_tiled_graph(StokesState& st, MGHierarchy<T>& H, const ebg_solve_opts& o, const std::vector<int>& sm,
+                               const T* prn, const T (&bcd)[4], hipStream_t s)
+{
+    auto run = [&](hipStream_t q) {
+        mg_vcycle_tiled(H, sm, o.local_sweeps, (T)o.theta_s, (T)o.theta_c, T(1), T(1), T(0), prn, bcd, q);
+    };
+    if (!st.use_graphs) {
+        run(s);
+        return;
+    }
+    VcycleGraph& g = vgraph<T>(st);
+    std::vector<uint64_t> key = vcycle_key(H, sm, o, prn, bcd);
+    if (!g.exec || key != g.key) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        g.key.clear();
+        bool ok = g.cap || hipStreamCreateWithFlags(&g.cap, hipStreamNonBlocking) == hipSuccess;
+        hipGraph_t graph = nullptr;
+        ok = ok && hipStreamBeginCapture(g.cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
+        if (ok) {
+            run(g.cap);
+            ok = hipStreamEndCapture(g.cap, &graph) == hipSuccess && graph;
+        }
+        ok = ok && hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        if (!ok || vcycle_key(H, sm, o, prn, bcd) != key) {   // capture failed or moved the buffers
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+            st.use_graphs = false;
+            st.graph_broken = true;
+            run(s);
+            return;
+        }
+        g.key = std::move(key);
+    }
+    if (hipGraphLaunch(g.exec, s) != hipSuccess) {
+        (void)hipGetLastError();
+        st.use_graphs = false;
+        st.graph_broken = true;
+        run(s);
+    }
+}
 
 // The system's operator points into the context's grid arrays, so a system
 // is usable only while the grid it was set with is current.
@@ -556,8 +665,7 @@ static void precondition(StokesState& st, MGHierarchy<T>& H, T* prn, const ebg_s
         hipLaunchKernelGGL(set_fixed_kernel<T>, dim3(nb(st.nc)), dim3(256), 0, s, st.op, r, l0, st.nc);
     const T bcd[4] = {(T)o.bc_d[0], (T)o.bc_d[1], (T)o.bc_d[2], (T)o.bc_d[3]};
     if (o.smoother == 1)
-        mg_vcycle_tiled(H, sm, o.local_sweeps, (T)o.theta_s, (T)o.theta_c, T(1), T(1), T(0),
-                        (const T*)prn, bcd, s);
+        vcycle_tiled_graph(st, H, o, sm, (const T*)prn, bcd, s);
     else
         mg_vcycle(H, sm, (T)o.theta_s, (T)o.theta_c, T(1), T(1), T(0), (const T*)prn, bcd, s);
     hipLaunchKernelGGL(unpack_corr_kernel<T>, dim3(nb(st.nc)), dim3(256), 0, s, st.op, r, l0, z, st.nc);
@@ -816,6 +924,7 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
         return rc;
     std::vector<int> sm;
     const bool f32 = (o.precision == 1);
+    st.use_graphs = ctx->stokes_graphs != 0 && !st.graph_broken;
     if ((rc = prepare_preconditioner(ctx, st, o, sm))) return rc;
     if (o.learned && ((rc = st.zc.ensure(n * 8)) || (rc = st.jr0.ensure(n * 8)) || (rc = st.jr1.ensure(n * 8)) ||
                       (rc = st.judge.ensure(64))))
@@ -1005,6 +1114,14 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
 using namespace ebg;
 
 extern "C" {
+
+int ebg_stokes_set_graphs(ebg_ctx* ctx, int on)
+{
+    EBG_CHECK(ctx != nullptr, EBG_ERR_ARG, "ebg_stokes_set_graphs: ctx is NULL");
+    EBG_CHECK(on == 0 || on == 1, EBG_ERR_ARG, "ebg_stokes_set_graphs: on must be 0 or 1");
+    ctx->stokes_graphs = on;
+    return EBG_OK;
+}
 
 int ebg_stokes_release(ebg_ctx* ctx)
 {

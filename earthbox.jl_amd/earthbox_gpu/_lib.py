@@ -39,6 +39,7 @@ SIGNATURES: dict[str, tuple] = {
     "ebg_rk4_set_arithmetic": (c_int, [c_void_p, c_int]),
     "ebg_rk4_set_tiling": (c_int, [c_void_p, c_int]),
     "ebg_scatter_set_order": (c_int, [c_void_p, c_int]),
+    "ebg_stokes_set_graphs": (c_int, [c_void_p, c_int]),
     "ebg_rk4_tolerance_kernel_available": (c_int, [c_void_p]),
     "ebg_calculate_spin": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "ebg_interpolate_marker_stokes_parameters_to_grid": (c_int, [c_void_p, c_void_p, c_void_p]),

@@ -148,6 +148,12 @@ class GPUContext:
         within a few ulps of the sum of the absolute terms)."""
         check(lib.ebg_scatter_set_order(self._h, int(order)), "ebg_scatter_set_order")
 
+    def set_stokes_graphs(self, on: bool) -> None:
+        """True (default): the 2-D preconditioner's V-cycle is replayed from a
+        captured HIP graph; False launches its kernels one by one (identical
+        results)."""
+        check(lib.ebg_stokes_set_graphs(self._h, int(bool(on))), "ebg_stokes_set_graphs")
+
     def rk4_tolerance_kernel_available(self) -> bool:
         return lib.ebg_rk4_tolerance_kernel_available(self._h) == 1
 

@@ -592,6 +592,13 @@ typedef struct {
 } ebg_solve_stats;
 
 int ebg_stokes_solve_opts_default(ebg_solve_opts *opts);
+/* on = 1 (default): the tiled V-cycle of the 2-D preconditioner is captured
+ * once into a HIP graph (per hierarchy, precision and smoother settings) and
+ * replayed for each preconditioner application; on = 0 launches its kernels
+ * one by one.  Identical results.  No counterpart in the reference (a launch
+ * strategy of this GPU solver). */
+int ebg_stokes_set_graphs(ebg_ctx *ctx, int on);
+
 /* Replaces SystemSolver.solve_system_optimized / ParallelSolver.parallel_direct_solver
  * (solver/system_solver/SystemSolver.jl:67-91, parallel_solver/ParallelSolver.jl:52-126).
  * S (device, reference order, length 3*(xnum-1)*(ynum-1)) receives the solution.
