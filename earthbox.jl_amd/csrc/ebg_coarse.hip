@@ -468,14 +468,14 @@ int mg_coarse_setup(const MGHierarchy<T>& H, int k, const double* bc_d, double* 
 }
 
 template <typename T>
-void mg_coarse_solve(MGHierarchy<T>& H, const T* bc_d, hipStream_t s)
+void mg_coarse_solve(MGHierarchy<T>& H, const T* bc_d, hipStream_t s, bool zeroed)
 {
     const int k = H.nlev() - 1;
     MGLevelDev<T>& l = H.lev[k];
     const UnknownMap m = unknown_map(l.xnum, l.ynum);
     const int n = H.coarse_n;
     const unsigned nbn = (unsigned)((n + 255) / 256);
-    mg_zero_solution(H, k, s);
+    if (!zeroed) mg_zero_solution(H, k, s);
     hipLaunchKernelGGL(coarse_pack_kernel<T>, dim3(nbn), dim3(256), 0, s, l, m, n, H.coarse_b);
     hipLaunchKernelGGL(coarse_gemv_kernel<T>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, H.coarse_inv,
                        (const T*)H.coarse_b, H.coarse_x, n);
@@ -488,7 +488,7 @@ template int mg_coarse_setup<float>(const MGHierarchy<float>&, int, const double
                                     float*, void*, hipStream_t);
 template int mg_coarse_setup<double>(const MGHierarchy<double>&, int, const double*, double*, double*, int*,
                                      int*, double*, void*, hipStream_t);
-template void mg_coarse_solve<float>(MGHierarchy<float>&, const float*, hipStream_t);
-template void mg_coarse_solve<double>(MGHierarchy<double>&, const double*, hipStream_t);
+template void mg_coarse_solve<float>(MGHierarchy<float>&, const float*, hipStream_t, bool);
+template void mg_coarse_solve<double>(MGHierarchy<double>&, const double*, hipStream_t, bool);
 
 }  // namespace ebg

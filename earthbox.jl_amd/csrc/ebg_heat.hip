@@ -670,9 +670,9 @@ void multidot(HeatState& H, const double* w, const double* V, int64_t ldv, int k
               hipStream_t s)
 {
     hipLaunchKernelGGL(multidot_chunks_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, w, V, ldv, k, n,
-                       H.partial.as<double>());
+                       H.partial.as<double>(), (const int*)nullptr);
     hipLaunchKernelGGL(reduce_columns_kernel, dim3((k + 3) / 4), dim3(256), 0, s, H.partial.as<double>(),
-                       RED_BLOCKS, k, out, 0);
+                       RED_BLOCKS, k, out, 0, (const int*)nullptr);
 }
 
 ebg_heat_solve_opts default_opts()

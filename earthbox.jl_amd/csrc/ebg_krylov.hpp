@@ -23,16 +23,16 @@ constexpr int MAXK = 256;        // longest Krylov basis a multi-dot handles (2-
 constexpr double REORTH_RATIO2 = 0.25;
 constexpr int DOT_CHUNK = 8;     // basis vectors per pass of the multi-dot (register accumulators)
 
-// partial[b*k + q] = sum over block b's grid-stride slice of w * V[q] for
-// every q < k, DOT_CHUNK basis vectors per pass
+// partial[q*RED_BLOCKS + b] = sum over block b's grid-stride slice of w * V[q]
+// for every q < k, DOT_CHUNK basis vectors per pass; a non-null gate holding
+// 0 skips the launch's work (device-side condition)
 __global__ void multidot_chunks_kernel(const double* __restrict__ w, const double* __restrict__ V, int64_t ldv,
-                                       int k, int64_t n, double* __restrict__ partial);
-// out[q] (+)= sum_b partial[b*k + q], one wavefront per q, fixed order
+                                       int k, int64_t n, double* __restrict__ partial,
+                                       const int* __restrict__ gate);
+// out[q] (+)= sum_b partial[q*nblocks + b], one wavefront per q, fixed order;
+// gate as above
 __global__ void reduce_columns_kernel(const double* __restrict__ partial, int nblocks, int k,
-                                      double* __restrict__ out, int accumulate);
-// out[q] (+)= sum_b partial[b*k + q], one block, fixed order
-__global__ void reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int k,
-                                       double* __restrict__ out, int accumulate);
+                                      double* __restrict__ out, int accumulate, const int* __restrict__ gate);
 // w -= sum_q h[q] V[q]
 __global__ void update_kernel(double* __restrict__ w, const double* __restrict__ V, int64_t ldv,
                               const double* __restrict__ h, int k, int64_t n);

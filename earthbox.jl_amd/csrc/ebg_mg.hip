@@ -336,15 +336,15 @@ void mg_smoother_tiled(MGHierarchy<T>& H, int k, int nsweeps, int local_sweeps, 
     const TileSpec ts{p.ti, p.tj, p.nti, p.ntj, 4};
     const int np = (l.ynum - 1) * (l.xnum - 1);
     auto launch = [&](unsigned nblocks, unsigned nthreads, size_t lds, const TileSpec& spec,
-                      const TileSmoothArgs& a, T* ovx, T* ovy, T* opr) {
+                      const TileSmoothArgs& a, T* ovx, T* ovy, T* opr, T* shift) {
         if (H.fast_div)
             hipLaunchKernelGGL((mg_tile_smooth_kernel<T, true>), dim3(nblocks), dim3(nthreads), lds, s, l,
                                spec, a, theta_s, theta_c, bc_d[0], bc_d[1], bc_d[2], bc_d[3], prnorm,
-                               prnorm_dev, ovx, ovy, opr);
+                               prnorm_dev, ovx, ovy, opr, shift);
         else
             hipLaunchKernelGGL((mg_tile_smooth_kernel<T, false>), dim3(nblocks), dim3(nthreads), lds, s, l,
                                spec, a, theta_s, theta_c, bc_d[0], bc_d[1], bc_d[2], bc_d[3], prnorm,
-                               prnorm_dev, ovx, ovy, opr);
+                               prnorm_dev, ovx, ovy, opr, shift);
     };
     const bool additive = H.additive_tiles && k > 0 && l.vx_alt;
     // one workgroup for the whole level (all sweeps, gauge shift and residuals
@@ -354,18 +354,21 @@ void mg_smoother_tiled(MGHierarchy<T>& H, int k, int nsweeps, int local_sweeps, 
     if (p.whole_ok && (!additive || p.nti * p.ntj <= 4)) {
         const TileSpec whole{l.ynum + 1, l.xnum + 1, 1, 1, 1};
         launch(1, 1024, p.whole_lds, whole, TileSmoothArgs{0, nsweeps, 1, normalize ? 1 : 0}, l.vx, l.vy,
-               l.pr);
+               l.pr, (T*)nullptr);
         return;
     }
     const int nthreads = std::min(1024, ((p.ti * p.tj + 63) / 64) * 64);
     for (int done = 0; done < nsweeps; done += local_sweeps) {
         const int m = std::min(local_sweeps, nsweeps - done);
+        // the gauge shift prnorm - p(1,1) is taken by the block owning node
+        // (1,1) in the last pass (no later launch of the pass writes it)
+        T* shift = (normalize && done + m >= nsweeps) ? H.scratch : (T*)nullptr;
         if (additive) {
             // every tile in one launch, reading the previous iterate (halos
             // included) and writing the other buffer: deterministic
             const TileSpec all{p.ti, p.tj, p.nti, p.ntj, 1};
             launch((unsigned)(p.nti * p.ntj), nthreads, p.lds, all, TileSmoothArgs{0, m, 0, 0}, l.vx_alt,
-                   l.vy_alt, l.pr_alt);
+                   l.vy_alt, l.pr_alt, shift);
             std::swap(l.vx, l.vx_alt);
             std::swap(l.vy, l.vy_alt);
             std::swap(l.pr, l.pr_alt);
@@ -375,15 +378,13 @@ void mg_smoother_tiled(MGHierarchy<T>& H, int k, int nsweeps, int local_sweeps, 
             const int ci = c & 1, cj = c >> 1;
             const int ni = (p.nti - ci + 1) / 2, nj = (p.ntj - cj + 1) / 2;
             if (ni <= 0 || nj <= 0) continue;
-            launch((unsigned)(ni * nj), nthreads, p.lds, ts, TileSmoothArgs{c, m, 0, 0}, l.vx, l.vy, l.pr);
+            launch((unsigned)(ni * nj), nthreads, p.lds, ts, TileSmoothArgs{c, m, 0, 0}, l.vx, l.vy, l.pr,
+                   c == 0 ? shift : (T*)nullptr);
         }
     }
-    if (normalize) {
-        hipLaunchKernelGGL(mg_pressure_shift_kernel<T>, dim3(1), dim3(64), 0, s, l.pr, prnorm_dev, prnorm,
-                           H.scratch);
+    if (normalize)
         hipLaunchKernelGGL(mg_pressure_add_kernel<T>, dim3(nblk(np)), dim3(256), 0, s, l.pr, np,
                            H.scratch);
-    }
     hipLaunchKernelGGL(mg_residual_kernel<T>, dim3(nblk((int64_t)(l.ynum + 1) * (l.xnum + 1))),
                        dim3(256), 0, s, l);
 }
@@ -397,24 +398,20 @@ void mg_vcycle_tiled(MGHierarchy<T>& H, const std::vector<int>& sm, int local_sw
 {
     const int nl = H.nlev();
     const bool direct = H.coarse_inv && nl > 1;   // exact solve replaces the coarsest smoothing
+    // levels below the finest start from zero: the fused restriction into a
+    // level clears its solution (level 0 is cleared by the caller)
     for (int n = 0; n < nl; ++n) {
         if (direct && n == nl - 1) {
-            mg_coarse_solve(H, bc_d, s);
+            mg_coarse_solve(H, bc_d, s, true);
             break;
         }
-        if (n > 0) mg_zero_solution(H, n, s);
         mg_smoother_tiled(H, n, sm[n], local_sweeps, theta_s, theta_c, prnorm, prnorm_dev, bc_d, s);
         if (n < nl - 1) {
             const MGLevelDev<T>& f = H.lev[n];
             const MGLevelDev<T>& c = H.lev[n + 1];
-            const TransferDev<T>& t = H.xfer[n];
-            hipLaunchKernelGGL((mg_restrict_kernel<T, 0>), dim3(nblk((int64_t)(c.ynum + 1) * c.xnum)),
-                               dim3(256), 0, s, f, c, t.vx_y, t.vx_x);
-            hipLaunchKernelGGL((mg_restrict_kernel<T, 1>), dim3(nblk((int64_t)c.ynum * (c.xnum + 1))),
-                               dim3(256), 0, s, f, c, t.vy_y, t.vy_x);
-            hipLaunchKernelGGL((mg_restrict_kernel<T, 2>),
-                               dim3(nblk((int64_t)(c.ynum - 1) * (c.xnum - 1))), dim3(256), 0, s, f, c,
-                               t.pr_y, t.pr_x);
+            const int64_t nall = (int64_t)(c.ynum + 1) * c.xnum + (int64_t)c.ynum * (c.xnum + 1) +
+                                 (int64_t)(c.ynum - 1) * (c.xnum - 1);
+            hipLaunchKernelGGL(mg_restrict_zero_kernel<T>, dim3(nblk(nall)), dim3(256), 0, s, f, c, H.xfer[n]);
         }
     }
     for (int n = nl - 1; n >= 0; --n) {

@@ -297,10 +297,9 @@ __global__ void __launch_bounds__(256) mg_residual_kernel(MGLevelDev<T> l)
 // (scaled by etan for the continuity residual).  Loop order j-outer, i-inner
 // matches Restriction.jl:444-463.
 template <typename T, int KIND>  // 0 vx, 1 vy, 2 pr
-__global__ void __launch_bounds__(256) mg_restrict_kernel(MGLevelDev<T> f, MGLevelDev<T> c,
-                                                           AxisMapDev<T> my, AxisMapDev<T> mx)
+__device__ __forceinline__ void mg_restrict_at(const MGLevelDev<T>& f, const MGLevelDev<T>& c,
+                                               const AxisMapDev<T>& my, const AxisMapDev<T>& mx, int t)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int yc = c.ynum, xc = c.xnum;
     int nyc, nxc;
     if (KIND == 0) { nyc = yc + 1; nxc = xc; }
@@ -337,6 +336,39 @@ __global__ void __launch_bounds__(256) mg_restrict_kernel(MGLevelDev<T> f, MGLev
     if (KIND == 0) c.RX[(size_t)(jc - 1) * (yc + 1) + (ic - 1)] = (interior && den != T(0)) ? num / den : T(0);
     else if (KIND == 1) c.RY[(size_t)(jc - 1) * yc + (ic - 1)] = (interior && den != T(0)) ? num / den : T(0);
     else c.RC[(size_t)(jc - 1) * (yc - 1) + (ic - 1)] = (den != T(0)) ? num / den / MG_EN(c, ic, jc) : T(0);
+}
+template <typename T, int KIND>
+__global__ void __launch_bounds__(256) mg_restrict_kernel(MGLevelDev<T> f, MGLevelDev<T> c,
+                                                           AxisMapDev<T> my, AxisMapDev<T> mx)
+{
+    mg_restrict_at<T, KIND>(f, c, my, mx, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The three restrictions of mg_restrict_kernel in one launch (threads over
+// the coarse vx, then vy, then pressure grids, the same arithmetic per
+// entry), each thread also clearing its entry of the coarse solution, which
+// the V-cycle's smoothing of that level starts from.
+template <typename T>
+__global__ void __launch_bounds__(256) mg_restrict_zero_kernel(MGLevelDev<T> f, MGLevelDev<T> c, TransferDev<T> x)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nvx = (c.ynum + 1) * c.xnum, nvy = c.ynum * (c.xnum + 1), np = (c.ynum - 1) * (c.xnum - 1);
+    if (t < nvx) {
+        mg_restrict_at<T, 0>(f, c, x.vx_y, x.vx_x, t);
+        c.vx[t] = T(0);
+        return;
+    }
+    t -= nvx;
+    if (t < nvy) {
+        mg_restrict_at<T, 1>(f, c, x.vy_y, x.vy_x, t);
+        c.vy[t] = T(0);
+        return;
+    }
+    t -= nvy;
+    if (t < np) {
+        mg_restrict_at<T, 2>(f, c, x.pr_y, x.pr_x, t);
+        c.pr[t] = T(0);
+    }
 }
 
 // Gather viscosity restriction; loop order i-outer, j-inner as
@@ -581,7 +613,8 @@ int mg_coarse_setup(const MGHierarchy<T>& H, int k, const double* bc_d, double* 
 // x = inverse * b on the coarsest level: gathers RX, RY, RC, writes vx, vy, pr
 // (zero elsewhere) and refreshes the ghosts.
 template <typename T>
-void mg_coarse_solve(MGHierarchy<T>& H, const T* bc_d, hipStream_t s);
+// zeroed: the coarsest solution is already cleared (fused restriction)
+void mg_coarse_solve(MGHierarchy<T>& H, const T* bc_d, hipStream_t s, bool zeroed = false);
 
 // ---- pieces of the reference's stand-alone multigrid driver (2-D and 3-D) ----
 // Viscosity continuation: the initial range and step factor of

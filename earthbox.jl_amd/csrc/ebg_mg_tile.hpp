@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(1024) mg_tile_smooth_kernel(MGLevelDev<T> l, T
                                                               T theta_c, T d_top, T d_bottom,
                                                               T d_left, T d_right, T prnorm,
                                                               const T* prnorm_dev, T* ovx, T* ovy,
-                                                              T* opr)
+                                                              T* opr, T* shift)
 {
     extern __shared__ __align__(16) unsigned char tile_smem[];
     T* sm = reinterpret_cast<T*>(tile_smem);
@@ -274,6 +274,10 @@ __global__ void __launch_bounds__(1024) mg_tile_smooth_kernel(MGLevelDev<T> l, T
             }
         }
     }
+    // gauge shift of the level's pressures, prnorm - p(1,1), for the
+    // pressure update launched after the smoothing (block of node (1,1))
+    if (shift && I0 == 1 && J0 == 1 && threadIdx.x == 0)
+        shift[0] = (prnorm_dev ? prnorm_dev[0] : prnorm) - v.pr(1, 1);
     // ---- write back the owned velocities and pressures (to the level's own
     // arrays, or to its second buffers in an all-tiles launch)
     for (int e = threadIdx.x; e < nown; e += nt) {

@@ -147,14 +147,17 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// partial[b*k + q] = block b's grid-stride share of <w, V[q]> for every q < k.
-// The basis is taken DOT_CHUNK vectors per pass, with the accumulators in
-// registers (a fixed-size, fully unrolled array), so w is re-read once per
-// chunk and the basis exactly once.
+// partial[q*gridDim.x + b] = block b's grid-stride share of <w, V[q]> for
+// every q < k.  The basis is taken DOT_CHUNK vectors per pass, with the
+// accumulators in registers (a fixed-size, fully unrolled array), so w is
+// re-read once per chunk and the basis exactly once.  A non-null gate that
+// holds 0 makes the whole grid return at once.
 __global__ void __launch_bounds__(256) multidot_chunks_kernel(const double* __restrict__ w,
                                                              const double* __restrict__ V, int64_t ldv,
-                                                             int k, int64_t n, double* __restrict__ partial)
+                                                             int k, int64_t n, double* __restrict__ partial,
+                                                             const int* __restrict__ gate)
 {
+    if (gate && *gate == 0) return;
     __shared__ double sh[4][DOT_CHUNK];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int q0 = 0; q0 < k; q0 += DOT_CHUNK) {
@@ -163,8 +166,27 @@ __global__ void __launch_bounds__(256) multidot_chunks_kernel(const double* __re
         double acc[DOT_CHUNK];
 #pragma unroll
         for (int q = 0; q < DOT_CHUNK; ++q) acc[q] = 0.0;
-        for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-             t += (int64_t)gridDim.x * blockDim.x) {
+        // two slice elements per step (their loads in flight together), added
+        // in slice order
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        for (; t + stride < n; t += 2 * stride) {
+            const double w0 = w[t], w1 = w[t + stride];
+            double v0[DOT_CHUNK], v1[DOT_CHUNK];
+#pragma unroll
+            for (int q = 0; q < DOT_CHUNK; ++q)
+                if (q < kc) {
+                    v0[q] = Vc[(int64_t)q * ldv + t];
+                    v1[q] = Vc[(int64_t)q * ldv + t + stride];
+                }
+#pragma unroll
+            for (int q = 0; q < DOT_CHUNK; ++q)
+                if (q < kc) {
+                    acc[q] += w0 * v0[q];
+                    acc[q] += w1 * v1[q];
+                }
+        }
+        if (t < n) {
             const double wv = w[t];
 #pragma unroll
             for (int q = 0; q < DOT_CHUNK; ++q)
@@ -178,43 +200,34 @@ __global__ void __launch_bounds__(256) multidot_chunks_kernel(const double* __re
         __syncthreads();
         if ((int)threadIdx.x < kc) {
             const int q = threadIdx.x;
-            partial[(int64_t)blockIdx.x * k + q0 + q] = ((sh[0][q] + sh[1][q]) + sh[2][q]) + sh[3][q];
+            partial[(int64_t)(q0 + q) * gridDim.x + blockIdx.x] = ((sh[0][q] + sh[1][q]) + sh[2][q]) + sh[3][q];
         }
         __syncthreads();
     }
 }
 
-// out[q] = sum_b partial[b*k+q]: one wavefront per column q (four per block),
-// lanes take blocks b = lane, lane + 64, ..., then a fixed shuffle tree
-__global__ void __launch_bounds__(256) reduce_columns_kernel(const double* __restrict__ partial, int nblocks, int k,
-                                                            double* __restrict__ out, int accumulate)
+// lane's fixed-order share of sum_b col[b], then the shuffle tree (lane 0
+// holds the total)
+__device__ __forceinline__ double column_sum(const double* __restrict__ col, int nblocks, int lane)
 {
+    double v = 0.0;
+    for (int b = lane; b < nblocks; b += 64) v += col[b];
+    return wave_sum(v);
+}
+
+// out[q] = sum_b partial[q*nblocks + b]: one wavefront per column q (four per
+// block), lanes take blocks b = lane, lane + 64, ..., then a fixed shuffle
+// tree.  Gate as in the multi-dot.
+__global__ void __launch_bounds__(256) reduce_columns_kernel(const double* __restrict__ partial, int nblocks, int k,
+                                                            double* __restrict__ out, int accumulate,
+                                                            const int* __restrict__ gate)
+{
+    if (gate && *gate == 0) return;
     const int q = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (q >= k) return;   // whole wavefronts leave together
-    double v = 0.0;
-    for (int b = lane; b < nblocks; b += 64) v += partial[(int64_t)b * k + q];
-    v = wave_sum(v);
+    const double v = column_sum(partial + (int64_t)q * nblocks, nblocks, lane);
     if (lane == 0) out[q] = accumulate ? out[q] + v : v;
-}
-
-// out[q] = sum_b partial[b*k+q] (one block, fixed order)
-__global__ void reduce_partials_kernel(const double* __restrict__ partial, int nblocks, int k,
-                                       double* __restrict__ out, int accumulate)
-{
-    __shared__ double sh[256];
-    for (int q = 0; q < k; ++q) {
-        double s = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partial[(int64_t)b * k + q];
-        sh[threadIdx.x] = s;
-        __syncthreads();
-        for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-            if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[q] = accumulate ? out[q] + sh[0] : sh[0];
-        __syncthreads();
-    }
 }
 
 // w -= sum_q h[q] V[q]
@@ -248,12 +261,107 @@ __global__ void scale_kernel(double* __restrict__ out, const double* __restrict_
     if (t < n) out[t] = in[t] / nrm[0];
 }
 
+// ---- Arnoldi step of the 2-D solve on the device ----
+// w -= sum_q h[q] V[q] over the multi-dot's grid-stride slices (one launch of
+// RED_BLOCKS blocks), and partial[b] = block b's share of |w|^2 after the
+// update, in the slices and summation order of a one-column multi-dot of w
+// with itself.  Gate as in the multi-dot.
+__global__ void __launch_bounds__(256) update_norm_kernel(double* __restrict__ w, const double* __restrict__ V,
+                                                         int64_t ldv, const double* __restrict__ h, int k,
+                                                         int64_t n, double* __restrict__ partial,
+                                                         const int* __restrict__ gate)
+{
+    if (gate && *gate == 0) return;
+    __shared__ double sh[4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double acc = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        double v = w[t];
+        for (int q = 0; q < k; ++q) v -= h[q] * V[(int64_t)q * ldv + t];
+        w[t] = v;
+        acc += v * v;
+    }
+    const double v = wave_sum(acc);
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// Device state of one restart cycle: the Hessenberg columns (leading
+// dimension ldh), the Givens cosines and sines, the rotated right side g, the
+// scalars wn2 (|w|^2 after the first pass) and hn (the new basis norm), the
+// second-pass gate, and the per-step status in host-mapped memory:
+// status[2j] = |g[j+1]| (the residual norm estimate), status[2j+1] = hn.
+struct ArnoldiDev {
+    double* H;
+    int ldh;
+    double *cs, *sn, *g;
+    double *wn2, *hn;
+    int* gate;
+    double* status;
+};
+
+// one wavefront: |w|^2 after the first Gram-Schmidt pass and the decision on
+// the second ("twice is enough": only when the first pass left no more than
+// REORTH_RATIO2 of the squared norm); hd[k+1] holds |w|^2 before the pass
+__global__ void __launch_bounds__(64) arnoldi_decide_kernel(const double* __restrict__ npart, int nblocks,
+                                                           const double* __restrict__ hd, int k, ArnoldiDev a)
+{
+    const double v = column_sum(npart, nblocks, threadIdx.x);
+    if (threadIdx.x == 0) {
+        a.wn2[0] = v;
+        a.gate[0] = (v > REORTH_RATIO2 * hd[k + 1]) ? 0 : 1;
+    }
+}
+
+// one wavefront: Hessenberg column k (both passes' coefficients), the new
+// basis norm, the earlier rotations applied to the column, the new rotation,
+// the rotated right side and the step's status
+__global__ void __launch_bounds__(64) arnoldi_finish_kernel(const double* __restrict__ npart, int nblocks,
+                                                           const double* __restrict__ hd,
+                                                           const double* __restrict__ hd2, int k, ArnoldiDev a)
+{
+    const bool second = a.gate[0] != 0;
+    double wn2 = second ? column_sum(npart, nblocks, threadIdx.x) : 0.0;
+    if (threadIdx.x != 0) return;
+    if (!second) wn2 = a.wn2[0];
+    double* col = a.H + (int64_t)k * a.ldh;
+    for (int q = 0; q <= k; ++q) col[q] = second ? hd[q] + hd2[q] : hd[q];
+    const double hn = sqrt(wn2);
+    a.hn[0] = hn;
+    col[k + 1] = hn;
+    for (int q = 0; q < k; ++q) {
+        const double t = a.cs[q] * col[q] + a.sn[q] * col[q + 1];
+        col[q + 1] = -a.sn[q] * col[q] + a.cs[q] * col[q + 1];
+        col[q] = t;
+    }
+    const double d = hypot(col[k], col[k + 1]);
+    a.cs[k] = col[k] / d;
+    a.sn[k] = col[k + 1] / d;
+    col[k] = d;
+    col[k + 1] = 0.0;
+    a.g[k + 1] = -a.sn[k] * a.g[k];
+    a.g[k] = a.cs[k] * a.g[k];
+    a.status[2 * k] = fabs(a.g[k + 1]);
+    a.status[2 * k + 1] = hn;
+}
+
+// v /= hn[0] in place, skipped for a zero (or NaN) norm
+__global__ void scale_inplace_kernel(double* v, const double* hn, int64_t n)
+{
+    const double h = hn[0];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n && h > 0) v[t] = v[t] / h;
+}
+
 // ---- preconditioner packing ----
 template <typename T>
 __global__ void pack_rhs_kernel(StokesOpDev s, const double* __restrict__ r, MGLevelDev<T> l,
                                 T* prnorm_dev, int nc)
 {
-    // covers the vx grid; RX/RY/RC of level 0 (MG scaling: RC = r_p / pscale)
+    // covers the vx grid; RX/RY/RC of level 0 (MG scaling: RC = r_p / pscale),
+    // and the level's solution cleared for the V-cycle
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int xnum = s.xnum, ynum = s.ynum, ni = ynum + 1;
     if (t >= ni * (xnum + 1)) return;
@@ -261,10 +369,19 @@ __global__ void pack_rhs_kernel(StokesOpDev s, const double* __restrict__ r, MGL
     const int i = I - 1, j = J - 1;
     const bool cellv = (i >= 1 && i <= ynum - 1 && j >= 1 && j <= xnum - 1);
     const int c = (j - 1) * (ynum - 1) + (i - 1);
-    if (J <= xnum) l.RX[(size_t)(J - 1) * (ynum + 1) + (I - 1)] = cellv ? (T)r[c] : T(0);
-    if (I <= ynum) l.RY[(size_t)(J - 1) * ynum + (I - 1)] = cellv ? (T)r[nc + c] : T(0);
+    if (J <= xnum) {
+        const size_t k = (size_t)(J - 1) * (ynum + 1) + (I - 1);
+        l.RX[k] = cellv ? (T)r[c] : T(0);
+        l.vx[k] = T(0);
+    }
+    if (I <= ynum) {
+        const size_t k = (size_t)(J - 1) * ynum + (I - 1);
+        l.RY[k] = cellv ? (T)r[nc + c] : T(0);
+        l.vy[k] = T(0);
+    }
     if (I <= ynum - 1 && J <= xnum - 1) {
         const int cp = (J - 1) * (ynum - 1) + (I - 1);
+        l.pr[cp] = T(0);
         l.RC[cp] = (T)(r[2 * (int64_t)nc + cp] / s.pscale);
         if (cp == 0)
             prnorm_dev[0] = (s.pressure_bc_mode == 0) ? (T)(s.pscale * (r[2 * (int64_t)nc] / kbond(s))) : T(0);
@@ -569,6 +686,7 @@ struct StokesState {
     ~StokesState()
     {
         if (solver) (void)rocblas_destroy_handle((rocblas_handle)solver);
+        release_status();
     }
     // learned correction: candidate vector, judge residuals and acceptance count
     DevBuf zc, jr0, jr1, judge, pair_e, pair_r, pair_z;
@@ -576,6 +694,31 @@ struct StokesState {
     // capture switches them off for this state
     bool use_graphs = true, graph_broken = false;
     VcycleGraph vg_f, vg_d;
+    // device Arnoldi state (plain and mixed solves): Hessenberg columns,
+    // rotations and right side, the second-pass gate, the per-step status in
+    // host-mapped memory, and the events the host waits on one step behind
+    DevBuf arn, gate;
+    double* status_h = nullptr;
+    double* status_d = nullptr;
+    hipEvent_t step_done[2] = {nullptr, nullptr};
+    int ensure_status()
+    {
+        if (!status_h) {
+            EBG_HIP(hipHostMalloc((void**)&status_h, sizeof(double) * 2 * MAXK,
+                                  hipHostMallocMapped | hipHostMallocCoherent));
+            EBG_HIP(hipHostGetDevicePointer((void**)&status_d, status_h, 0));
+        }
+        for (hipEvent_t& e : step_done)
+            if (!e) EBG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return EBG_OK;
+    }
+    void release_status()
+    {
+        for (hipEvent_t& e : step_done)
+            if (e) (void)hipEventDestroy(e);
+        if (status_h) (void)hipHostFree(status_h);
+        status_h = status_d = nullptr;
+    }
 };
 
 template <typename T> static VcycleGraph& vgraph(StokesState& st);
@@ -660,7 +803,6 @@ static void precondition(StokesState& st, MGHierarchy<T>& H, T* prn, const ebg_s
         hipLaunchKernelGGL(compat_rc_finish_kernel<T>, dim3(1), dim3(256), 0, s, l0,
                            (const double*)st.partial.as<double>(), COMPAT_BLOCKS);
     }
-    mg_zero_solution(H, 0, s);
     if (l0.fix_vx || l0.fix_p)
         hipLaunchKernelGGL(set_fixed_kernel<T>, dim3(nb(st.nc)), dim3(256), 0, s, st.op, r, l0, st.nc);
     const T bcd[4] = {(T)o.bc_d[0], (T)o.bc_d[1], (T)o.bc_d[2], (T)o.bc_d[3]};
@@ -788,9 +930,9 @@ static void multidot(StokesState& st, const double* w, const double* V, int64_t 
                      double* out, int accumulate, hipStream_t s)
 {
     hipLaunchKernelGGL(multidot_chunks_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, w, V, ldv, k, n,
-                       st.partial.as<double>());
+                       st.partial.as<double>(), (const int*)nullptr);
     hipLaunchKernelGGL(reduce_columns_kernel, dim3((k + 3) / 4), dim3(256), 0, s, st.partial.as<double>(),
-                       RED_BLOCKS, k, out, accumulate);
+                       RED_BLOCKS, k, out, accumulate, (const int*)nullptr);
 }
 
 // Direct solve of the coarsest level when it is small enough: the inverse is
@@ -836,7 +978,7 @@ static int prepare_preconditioner(ebg_ctx* ctx, StokesState& st, const ebg_solve
 {
     hipStream_t s = ctx->stream;
     int rc;
-    if ((rc = st.partial.ensure((size_t)RED_BLOCKS * MAXK * 8))) return rc;
+    if ((rc = st.partial.ensure((size_t)RED_BLOCKS * (MAXK + 1) * 8))) return rc;
     if (o.precision == 1) {
         if ((rc = ensure_hierarchy(ctx, st, st.Hf, st.prnorm_f, st.key_f, o))) return rc;
         sm = mg_smoothing_counts(st.Hf->nlev(), o.finest_sweeps, o.sweep_multiplier, o.sweep_cap);
@@ -908,6 +1050,95 @@ static int learned_step(ebg_ctx* ctx, StokesState& st, const double* v, double* 
     return EBG_OK;
 }
 
+// One restart cycle of the Arnoldi process with the Hessenberg columns, the
+// Givens rotations and the residual estimate on the device (plain and mixed
+// solves).  Step j writes A z_j straight into the basis slot j + 1, takes
+// |w|^2 along with the Gram-Schmidt dots, fuses the update with the new norm
+// and decides the second pass on the device.  The host enqueues step j + 1
+// before it waits for step j's status (written by the step's last kernel into
+// host-mapped memory), so the device never idles on the host.  A step
+// enqueued past the converged one writes only basis slot j + 2, column j + 1
+// of H, rotation j + 1 and g[j + 1..j + 2], none of which the solution update
+// over the first j + 1 columns reads.  On return kcols is the number of
+// columns to combine; g[0] holds the cycle's initial residual norm.
+static int arnoldi_cycle_device(StokesState& st, const ebg_solve_opts& o, const std::vector<int>& sm, int64_t n,
+                                double bnorm, int& total, int& kcols, hipStream_t s)
+{
+    const int m = o.restart;
+    const int nc = st.nc;
+    LayoutPlanar LP{nc};
+    double* V = st.V.as<double>();
+    double* Z = st.Z.as<double>();
+    double* hd = st.hdev.as<double>();
+    double* part = st.partial.as<double>();
+    double* npart = part + (size_t)RED_BLOCKS * MAXK;
+    ArnoldiDev a;
+    a.H = st.arn.as<double>();
+    a.ldh = m + 1;
+    a.cs = a.H + (size_t)(m + 1) * m;
+    a.sn = a.cs + m;
+    a.g = a.sn + m;
+    a.wn2 = st.nrm.as<double>() + 2;
+    a.hn = st.nrm.as<double>() + 3;
+    a.gate = st.gate.as<int>();
+    a.status = st.status_d;
+    const volatile double* status = st.status_h;
+    // 1: converged at step j, 0: go on, -1: non-finite basis norm
+    auto check = [&](int j) {
+        const double res = status[2 * j], hn = status[2 * j + 1];
+        if (!std::isfinite(hn)) {
+            set_error("ebg_stokes_solve: preconditioned Krylov vector is non-finite "
+                      "(multigrid preconditioner diverged)");
+            return -1;
+        }
+        return (res / bnorm <= o.rtol || hn == 0) ? 1 : 0;
+    };
+    int k = 0;
+    for (; k < m && total < o.max_iterations; ++k) {
+        double* vk = V + (int64_t)k * n;
+        double* zk = Z + (int64_t)k * n;
+        double* w = V + (int64_t)(k + 1) * n;
+        apply_preconditioner(st, o, sm, vk, zk, s);
+        hipLaunchKernelGGL((stokes_rows_kernel<false, LayoutPlanar>), dim3(nb(nc)), dim3(256), 0, s, st.op, zk, w,
+                           LP);
+        // classical Gram-Schmidt: <w, V_q> for q <= k and |w|^2 in one pass
+        multidot(st, w, V, n, k + 2, n, hd, 0, s);
+        hipLaunchKernelGGL(update_norm_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, w, V, n, (const double*)hd, k + 1,
+                           n, npart, (const int*)nullptr);
+        hipLaunchKernelGGL(arnoldi_decide_kernel, dim3(1), dim3(64), 0, s, (const double*)npart, RED_BLOCKS,
+                           (const double*)hd, k, a);
+        // the second pass runs only where the gate says so
+        hipLaunchKernelGGL(multidot_chunks_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, (const double*)w,
+                           (const double*)V, n, k + 1, n, part, (const int*)a.gate);
+        hipLaunchKernelGGL(reduce_columns_kernel, dim3((k + 4) / 4), dim3(256), 0, s, (const double*)part,
+                           RED_BLOCKS, k + 1, hd + MAXK, 0, (const int*)a.gate);
+        hipLaunchKernelGGL(update_norm_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, w, V, n,
+                           (const double*)(hd + MAXK), k + 1, n, npart, (const int*)a.gate);
+        hipLaunchKernelGGL(arnoldi_finish_kernel, dim3(1), dim3(64), 0, s, (const double*)npart, RED_BLOCKS,
+                           (const double*)hd, (const double*)(hd + MAXK), k, a);
+        hipLaunchKernelGGL(scale_inplace_kernel, dim3(nb(n)), dim3(256), 0, s, w, (const double*)a.hn, n);
+        EBG_HIP(hipGetLastError());
+        EBG_HIP(hipEventRecord(st.step_done[k & 1], s));
+        ++total;
+        if (k > 0) {
+            EBG_HIP(hipEventSynchronize(st.step_done[(k - 1) & 1]));
+            const int c = check(k - 1);
+            if (c < 0) return EBG_ERR_NUMERIC;
+            if (c) {            // step k was enqueued past convergence
+                --total;
+                kcols = k;
+                return EBG_OK;
+            }
+        }
+    }
+    if (k > 0) {
+        EBG_HIP(hipEventSynchronize(st.step_done[(k - 1) & 1]));
+        if (check(k - 1) < 0) return EBG_ERR_NUMERIC;
+    }
+    kcols = k;
+    return EBG_OK;
+}
+
 // rhs: right-hand side in planar layout, or NULL for the system's R
 static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_solve_stats* stats,
                   const double* rhs)
@@ -919,8 +1150,12 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
     int rc;
     if ((rc = st.b.ensure(n * 8)) || (rc = st.x.ensure(n * 8)) || (rc = st.r.ensure(n * 8)) ||
         (rc = st.w.ensure(n * 8)) || (rc = st.V.ensure((size_t)(m + 1) * n * 8)) ||
-        (rc = st.Z.ensure((size_t)m * n * 8)) || (rc = st.partial.ensure((size_t)RED_BLOCKS * MAXK * 8)) ||
+        (rc = st.Z.ensure((size_t)m * n * 8)) || (rc = st.partial.ensure((size_t)RED_BLOCKS * (MAXK + 1) * 8)) ||
         (rc = st.hdev.ensure(MAXK * 8 * 2)) || (rc = st.ydev.ensure(MAXK * 8)) || (rc = st.nrm.ensure(64)))
+        return rc;
+    const bool dev_arnoldi = (o.learned == 0);
+    if (dev_arnoldi && ((rc = st.arn.ensure(((size_t)(m + 1) * m + 3 * (size_t)m + 1) * 8)) ||
+                        (rc = st.gate.ensure(64)) || (rc = st.ensure_status())))
         return rc;
     std::vector<int> sm;
     const bool f32 = (o.precision == 1);
@@ -986,7 +1221,18 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
         g[0] = beta;
         int k = 0;
         bool done = false, pending = false;
-        for (; k < m && total < o.max_iterations; ++k) {
+        if (dev_arnoldi) {
+            double* gd = st.arn.as<double>() + (size_t)(m + 1) * m + 2 * (size_t)m;
+            EBG_HIP(hipMemcpyAsync(gd, &beta, 8, hipMemcpyHostToDevice, s));
+            if ((rc = arnoldi_cycle_device(st, o, sm, n, bnorm, total, k, s))) return rc;
+            if (k > 0) {
+                EBG_HIP(hipMemcpyAsync(H.data(), st.arn.p, sizeof(double) * (size_t)(m + 1) * k,
+                                       hipMemcpyDeviceToHost, s));
+                EBG_HIP(hipMemcpyAsync(g.data(), gd, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+                EBG_HIP(hipStreamSynchronize(s));
+            }
+        }
+        for (; !dev_arnoldi && k < m && total < o.max_iterations; ++k) {
             double* vk = V + (int64_t)k * n;
             double* zk = Z + (int64_t)k * n;
             const bool learned_dir = pending;

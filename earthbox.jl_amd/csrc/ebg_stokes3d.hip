@@ -200,9 +200,9 @@ static void multidot3(Stokes3State& st, const double* w, const double* V, int64_
                       hipStream_t s)
 {
     hipLaunchKernelGGL(multidot_chunks_kernel, dim3(RED_BLOCKS), dim3(256), 0, s, w, V, ldv, k, n,
-                       st.partial.as<double>());
+                       st.partial.as<double>(), (const int*)nullptr);
     hipLaunchKernelGGL(reduce_columns_kernel, dim3((k + 3) / 4), dim3(256), 0, s, st.partial.as<double>(), RED_BLOCKS,
-                       k, out, 0);
+                       k, out, 0, (const int*)nullptr);
 }
 
 static int prepare3(ebg_ctx* ctx, Stokes3State& st, const ebg_solve_opts& o, std::vector<int>& sm)

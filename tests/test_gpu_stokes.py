@@ -431,3 +431,57 @@ def test_captured_vcycle_bit_identical(gpu_ctx, precision, pc):
     for (a, ia, ra), (b, ib, rb) in zip(out[0], out[1]):
         assert ia == ib and ra == rb
         assert np.array_equal(a, b)
+
+
+def test_arnoldi_status_one_step_behind(gpu_ctx):
+    """The host reads each Arnoldi step's status one step late, with the next
+    step already queued: the reported count is still the first step that
+    meets the tolerance (one fewer iteration does not converge and stops at
+    exactly that count), the step queued past convergence leaves the solution
+    untouched (bit-identical to a solve capped at the converged count), and
+    the reported residuals equal the ones recomputed from the solutions."""
+    print("lagged Arnoldi status", flush=True)
+    pr = workloads.layered_random(129, 65, 1.5e6 * 128 / 1000, 6.0e5 * 64 / 500, seed=5, amplitude=0.5)
+    sol = StokesSolverGPU(gpu_ctx)
+    sol.set_system(pr)
+    kw = dict(precision=1, rtol=1e-8, restart=100, bc_d=bc_mirror_factors(pr))
+    S, st = sol.solve_system(**kw)
+    print(f"  {st['iterations']} iterations, {st['restarts']} cycles", flush=True)
+    assert st["converged"], st
+    n = st["iterations"]
+    S1, st1 = sol.solve_system(max_iterations=n - 1, **kw)
+    assert not st1["converged"] and st1["iterations"] == n - 1 and st1["relres"] > 1e-8
+    S2, st2 = sol.solve_system(max_iterations=n, **kw)
+    assert st2["converged"] and st2["iterations"] == n
+    assert np.array_equal(to_host(S2), to_host(S))
+    Li, Lj, Lv, R = oracle.stokes_build(pr)
+    import scipy.sparse as sp
+    L = sp.csr_matrix((Lv, (Li - 1, Lj - 1)), shape=(pr.N, pr.N))
+    for x, s in ((to_host(S), st), (to_host(S1), st1)):
+        rel = np.linalg.norm(R - L @ x) / np.linalg.norm(R)
+        assert abs(rel - s["relres"]) <= 1e-6 * s["relres"], (rel, s["relres"])
+
+
+def test_arnoldi_short_restart_cycles(gpu_ctx):
+    """Restart cycles of 10 steps with the device-side Hessenberg matrix and
+    rotations: each cycle minimises the residual over its space, so the true
+    residual after 10, 20, 30 and 40 iterations never grows (up to rounding);
+    every capped solve stops at exactly its cap, and the residual it reports
+    is the one recomputed from its solution."""
+    print("Arnoldi restart cycles of 10", flush=True)
+    pr = workloads.layered_random(129, 65, 1.5e6 * 128 / 1000, 6.0e5 * 64 / 500, seed=5, amplitude=0.5)
+    sol = StokesSolverGPU(gpu_ctx)
+    sol.set_system(pr)
+    Li, Lj, Lv, R = oracle.stokes_build(pr)
+    import scipy.sparse as sp
+    L = sp.csr_matrix((Lv, (Li - 1, Lj - 1)), shape=(pr.N, pr.N))
+    prev = np.inf
+    for cap in (10, 20, 30, 40):
+        S, st = sol.solve_system(precision=1, rtol=1e-14, restart=10, max_iterations=cap,
+                                 bc_d=bc_mirror_factors(pr))
+        rel = np.linalg.norm(R - L @ to_host(S)) / np.linalg.norm(R)
+        print(f"  {cap} iterations: relres {st['relres']:.3e}", flush=True)
+        assert st["iterations"] == cap and st["restarts"] == cap // 10
+        assert abs(rel - st["relres"]) <= 1e-6 * st["relres"]
+        assert st["relres"] <= prev * (1 + 1e-9)
+        prev = st["relres"]
