@@ -159,15 +159,28 @@ __global__ void __launch_bounds__(1024) mg_tile_smooth_kernel(MGLevelDev<T> l, T
         syb[e] = (k >= 0 && k <= ynum - 2) ? l.ystp_b[k] : T(1);
         syv[e] = (k >= 0 && k <= ynum - 1) ? l.ystp_vx[k] : T(1);
     }
-    // ---- load the window
-    for (int e = threadIdx.x; e < v.ww; e += nt) {
-        const int i = v.i0 + e % v.w, j = v.j0 + e / v.w;
+    // ---- load the window: two entries per thread per round, every field's
+    // load issued before the first LDS store (one wait per round)
+    for (int e0 = threadIdx.x; e0 < v.ww; e0 += 2 * nt) {
+        T val[2][TF_N];
 #pragma unroll
-        for (int f = 0; f < TF_N; ++f) {
-            T val = T(0);
-            if (tile_in_bounds(f, i, j, xnum, ynum))
-                val = tile_field_ptr(l, f)[(size_t)(j - 1) * tile_field_ld(f, ynum) + (i - 1)];
-            sm[f * v.ww + e] = val;
+        for (int u = 0; u < 2; ++u) {
+            const int e = e0 + u * nt;
+            const int i = v.i0 + e % v.w, j = v.j0 + e / v.w;
+#pragma unroll
+            for (int f = 0; f < TF_N; ++f) {
+                val[u][f] = T(0);
+                if (e < v.ww && tile_in_bounds(f, i, j, xnum, ynum))
+                    val[u][f] = tile_field_ptr(l, f)[(size_t)(j - 1) * tile_field_ld(f, ynum) + (i - 1)];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = e0 + u * nt;
+            if (e < v.ww) {
+#pragma unroll
+                for (int f = 0; f < TF_N; ++f) sm[f * v.ww + e] = val[u][f];
+            }
         }
     }
     __syncthreads();
