@@ -162,8 +162,8 @@ int mg_build(MGHierarchy<T>& H, int xnum, int ynum, double xsize, double ysize, 
         if (k > 0) {
             if ((rc = dev_zeros(H, (ny + 1) * nx, &l.vx_alt))) return rc;
             if ((rc = dev_zeros(H, ny * (nx + 1), &l.vy_alt))) return rc;
-            if ((rc = dev_zeros(H, (ny - 1) * (nx - 1), &l.pr_alt))) return rc;
         }
+        if ((rc = dev_zeros(H, (ny - 1) * (nx - 1), &l.pr_alt))) return rc;
         if ((rc = dev_zeros(H, (ny + 1) * nx, &l.RX))) return rc;
         if ((rc = dev_zeros(H, ny * (nx + 1), &l.RY))) return rc;
         if ((rc = dev_zeros(H, (ny - 1) * (nx - 1), &l.RC))) return rc;
@@ -382,11 +382,16 @@ void mg_smoother_tiled(MGHierarchy<T>& H, int k, int nsweeps, int local_sweeps, 
                    c == 0 ? shift : (T*)nullptr);
         }
     }
-    if (normalize)
-        hipLaunchKernelGGL(mg_pressure_add_kernel<T>, dim3(nblk(np)), dim3(256), 0, s, l.pr, np,
-                           H.scratch);
-    hipLaunchKernelGGL(mg_residual_kernel<T>, dim3(nblk((int64_t)(l.ynum + 1) * (l.xnum + 1))),
-                       dim3(256), 0, s, l);
+    const unsigned nres = nblk((int64_t)(l.ynum + 1) * (l.xnum + 1));
+    if (normalize) {
+        // shift and residuals in one pass, the shifted pressures in the
+        // second buffer
+        hipLaunchKernelGGL(mg_shift_residual_kernel<T>, dim3(nres), dim3(256), 0, s, l, (const T*)H.scratch,
+                           l.pr_alt);
+        std::swap(l.pr, l.pr_alt);
+        return;
+    }
+    hipLaunchKernelGGL(mg_residual_kernel<T>, dim3(nres), dim3(256), 0, s, l);
 }
 
 // Same level order and transfers as mg_vcycle; the smoothing calls use the

@@ -266,31 +266,64 @@ __global__ void mg_pressure_add_kernel(T* pr, int n, const T* scratch)
     if (t < n) pr[t] += scratch[0];
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) mg_residual_kernel(MGLevelDev<T> l)
+// Level residuals at entry t of the (ynum+1) x (xnum+1) index space (the vx
+// grid; vy and p handled by the same thread where their ranges overlap)
+template <typename T, class V>
+__device__ __forceinline__ void mg_residual_at(const MGLevelDev<T>& l, const V& v, int t)
 {
-    // covers the vx grid (ynum+1) x xnum; vy and p handled by the same thread
-    // where their index ranges overlap
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int xnum = l.xnum, ynum = l.ynum;
     const int ni = ynum + 1;
     if (t >= ni * (xnum + 1)) return;
     const int i = 1 + t % ni, j = 1 + t / ni;
     if (j <= xnum) {
         T r = T(0);
-        if (i >= 2 && i <= ynum && j >= 2 && j <= xnum - 1) r = mg_vx_residual(l, i, j, (T*)nullptr);
+        if (i >= 2 && i <= ynum && j >= 2 && j <= xnum - 1) r = vx_residual_v<T>(v, i, j, (T*)nullptr);
         if (l.fix_vx && l.fix_vx[(size_t)(j - 1) * (ynum + 1) + (i - 1)]) r = T(0);
         l.rx[(size_t)(j - 1) * (ynum + 1) + (i - 1)] = r;
     }
     if (i <= ynum) {
         T r = T(0);
-        if (i >= 2 && i <= ynum - 1 && j >= 2 && j <= xnum) r = mg_vy_residual(l, i, j, (T*)nullptr);
+        if (i >= 2 && i <= ynum - 1 && j >= 2 && j <= xnum) r = vy_residual_v<T>(v, i, j, (T*)nullptr);
         if (l.fix_vy && l.fix_vy[(size_t)(j - 1) * ynum + (i - 1)]) r = T(0);
         l.ry[(size_t)(j - 1) * ynum + (i - 1)] = r;
     }
     if (i <= ynum - 1 && j <= xnum - 1) {
         const size_t k = (size_t)(j - 1) * (ynum - 1) + (i - 1);
-        l.rc[k] = (l.fix_p && l.fix_p[k]) ? T(0) : mg_pr_residual(l, i, j);
+        l.rc[k] = (l.fix_p && l.fix_p[k]) ? T(0) : pr_residual_v<T>(v, i, j);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) mg_residual_kernel(MGLevelDev<T> l)
+{
+    mg_residual_at<T>(l, GlobalView<T>{l}, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The level arrays with a pending gauge shift added to every pressure read
+template <typename T>
+struct ShiftedView : GlobalView<T> {
+    T shift;
+    __device__ T pr(int i, int j) const { return MG_PR(this->l, i, j) + shift; }
+};
+
+// The gauge shift p += shift[0] followed by the residuals, in one pass: the
+// residual arithmetic reads p + shift (the values the shift would store) and
+// each thread writes its shifted pressure to pnew, which the caller swaps in
+// for p.  Same values, bit for bit, as the separate shift and residual passes.
+template <typename T>
+__global__ void __launch_bounds__(256) mg_shift_residual_kernel(MGLevelDev<T> l, const T* __restrict__ shift,
+                                                                T* __restrict__ pnew)
+{
+    const T s = shift[0];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    mg_residual_at<T>(l, ShiftedView<T>{{l}, s}, t);
+    const int ni = l.ynum + 1;
+    if (t < ni * (l.xnum + 1)) {
+        const int i = 1 + t % ni, j = 1 + t / ni;
+        if (i <= l.ynum - 1 && j <= l.xnum - 1) {
+            const size_t k = (size_t)(j - 1) * (l.ynum - 1) + (i - 1);
+            pnew[k] = l.pr[k] + s;
+        }
     }
 }
 

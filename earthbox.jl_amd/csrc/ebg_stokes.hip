@@ -317,34 +317,48 @@ __global__ void __launch_bounds__(64) arnoldi_decide_kernel(const double* __rest
 
 // one wavefront: Hessenberg column k (both passes' coefficients), the new
 // basis norm, the earlier rotations applied to the column, the new rotation,
-// the rotated right side and the step's status
+// the rotated right side and the step's status.  The column and the earlier
+// rotations are staged in LDS by all lanes; lane 0 runs the recurrence.
 __global__ void __launch_bounds__(64) arnoldi_finish_kernel(const double* __restrict__ npart, int nblocks,
                                                            const double* __restrict__ hd,
                                                            const double* __restrict__ hd2, int k, ArnoldiDev a)
 {
+    __shared__ double col[MAXK + 1], cs[MAXK], sn[MAXK];
     const bool second = a.gate[0] != 0;
     double wn2 = second ? column_sum(npart, nblocks, threadIdx.x) : 0.0;
-    if (threadIdx.x != 0) return;
-    if (!second) wn2 = a.wn2[0];
-    double* col = a.H + (int64_t)k * a.ldh;
-    for (int q = 0; q <= k; ++q) col[q] = second ? hd[q] + hd2[q] : hd[q];
-    const double hn = sqrt(wn2);
-    a.hn[0] = hn;
-    col[k + 1] = hn;
-    for (int q = 0; q < k; ++q) {
-        const double t = a.cs[q] * col[q] + a.sn[q] * col[q + 1];
-        col[q + 1] = -a.sn[q] * col[q] + a.cs[q] * col[q + 1];
-        col[q] = t;
+    for (int q = threadIdx.x; q <= k; q += 64) {
+        col[q] = second ? hd[q] + hd2[q] : hd[q];
+        if (q < k) {
+            cs[q] = a.cs[q];
+            sn[q] = a.sn[q];
+        }
     }
-    const double d = hypot(col[k], col[k + 1]);
-    a.cs[k] = col[k] / d;
-    a.sn[k] = col[k + 1] / d;
-    col[k] = d;
-    col[k + 1] = 0.0;
-    a.g[k + 1] = -a.sn[k] * a.g[k];
-    a.g[k] = a.cs[k] * a.g[k];
-    a.status[2 * k] = fabs(a.g[k + 1]);
-    a.status[2 * k + 1] = hn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (!second) wn2 = a.wn2[0];
+        const double hn = sqrt(wn2);
+        a.hn[0] = hn;
+        col[k + 1] = hn;
+        for (int q = 0; q < k; ++q) {
+            const double t = cs[q] * col[q] + sn[q] * col[q + 1];
+            col[q + 1] = -sn[q] * col[q] + cs[q] * col[q + 1];
+            col[q] = t;
+        }
+        const double d = hypot(col[k], col[k + 1]);
+        const double c = col[k] / d, sv = col[k + 1] / d;
+        a.cs[k] = c;
+        a.sn[k] = sv;
+        col[k] = d;
+        col[k + 1] = 0.0;
+        const double gk = a.g[k];
+        a.g[k + 1] = -sv * gk;
+        a.g[k] = c * gk;
+        a.status[2 * k] = fabs(-sv * gk);
+        a.status[2 * k + 1] = hn;
+    }
+    __syncthreads();
+    double* out = a.H + (int64_t)k * a.ldh;
+    for (int q = threadIdx.x; q <= k + 1; q += 64) out[q] = col[q];
 }
 
 // v /= hn[0] in place, skipped for a zero (or NaN) norm
