@@ -148,6 +148,20 @@ __global__ void __launch_bounds__(1024) mg_tile_smooth_kernel(MGLevelDev<T> l, T
     v.ystp_b = syb;
     v.ystp_vx = syv;
     const int nt = blockDim.x;
+    // (i, j) of entry e of the owned block and of the window: every thread
+    // starts at e = threadIdx.x, whose indices are computed once here (the
+    // divisions by the runtime extents then run only for later entries)
+    const int tid = threadIdx.x;
+    const int own_i = I0 + tid % ts.ti, own_j = J0 + tid / ts.ti;
+    const int win_i = v.i0 + tid % v.w, win_j = v.j0 + tid / v.w;
+    auto owned = [&](int e, int& i, int& j) {
+        if (e == tid) { i = own_i; j = own_j; }
+        else { i = I0 + e % ts.ti; j = J0 + e / ts.ti; }
+    };
+    auto windowed = [&](int e, int& i, int& j) {
+        if (e == tid) { i = win_i; j = win_j; }
+        else { i = v.i0 + e % v.w; j = v.j0 + e / v.w; }
+    };
     // ---- step windows: x entries k in [j0-1, j0+tj], y entries in [i0-1, i0+ti]
     for (int e = threadIdx.x; e < wj; e += nt) {
         const int k = v.j0 - 1 + e;
@@ -166,7 +180,8 @@ __global__ void __launch_bounds__(1024) mg_tile_smooth_kernel(MGLevelDev<T> l, T
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             const int e = e0 + u * nt;
-            const int i = v.i0 + e % v.w, j = v.j0 + e / v.w;
+            int i, j;
+            windowed(e, i, j);
 #pragma unroll
             for (int f = 0; f < TF_N; ++f) {
                 val[u][f] = T(0);
@@ -192,7 +207,8 @@ __global__ void __launch_bounds__(1024) mg_tile_smooth_kernel(MGLevelDev<T> l, T
         // read and written by different threads in this pass.
         const bool has_row1 = v.i0 <= 1, has_col1 = v.j0 <= 1;
         for (int e = threadIdx.x; e < v.ww; e += nt) {
-            const int i = v.i0 + e % v.w, j = v.j0 + e / v.w;
+            int i, j;
+            windowed(e, i, j);
             if (j >= 1 && j <= xnum) {
                 const bool wall = (j == 1 || j == xnum);
                 if (i == 1) {
@@ -225,7 +241,8 @@ __global__ void __launch_bounds__(1024) mg_tile_smooth_kernel(MGLevelDev<T> l, T
         __syncthreads();
         for (int color = 0; color <= 1; ++color) {
             for (int e = threadIdx.x; e < nown; e += nt) {
-                const int i = I0 + e % ts.ti, j = J0 + e / ts.ti;
+                int i, j;
+                owned(e, i, j);
                 if (i >= 2 && i <= ynum && j >= 2 && j <= xnum - 1 && ((i + j) & 1) == color &&
                     !(l.fix_vx && l.fix_vx[(size_t)(j - 1) * (ynum + 1) + (i - 1)])) {
                     T coef;
@@ -235,7 +252,8 @@ __global__ void __launch_bounds__(1024) mg_tile_smooth_kernel(MGLevelDev<T> l, T
             }
             __syncthreads();
             for (int e = threadIdx.x; e < nown; e += nt) {
-                const int i = I0 + e % ts.ti, j = J0 + e / ts.ti;
+                int i, j;
+                owned(e, i, j);
                 if (i >= 2 && i <= ynum - 1 && j >= 2 && j <= xnum && ((i + j) & 1) == color &&
                     !(l.fix_vy && l.fix_vy[(size_t)(j - 1) * ynum + (i - 1)])) {
                     T coef;
@@ -245,7 +263,8 @@ __global__ void __launch_bounds__(1024) mg_tile_smooth_kernel(MGLevelDev<T> l, T
             }
             __syncthreads();
             for (int e = threadIdx.x; e < nown; e += nt) {
-                const int i = I0 + e % ts.ti, j = J0 + e / ts.ti;
+                int i, j;
+                owned(e, i, j);
                 if (i <= ynum - 1 && j <= xnum - 1 && ((i + j) & 1) == color &&
                     !(l.fix_p && l.fix_p[(size_t)(j - 1) * (ynum - 1) + (i - 1)])) {
                     const T dR = pr_residual_v<T>(v, i, j);
@@ -261,14 +280,16 @@ __global__ void __launch_bounds__(1024) mg_tile_smooth_kernel(MGLevelDev<T> l, T
             const T shift = (prnorm_dev ? prnorm_dev[0] : prnorm) - v.pr(1, 1);
             __syncthreads();
             for (int e = threadIdx.x; e < nown; e += nt) {
-                const int i = I0 + e % ts.ti, j = J0 + e / ts.ti;
+                int i, j;
+                owned(e, i, j);
                 if (i <= ynum - 1 && j <= xnum - 1) v.at(TF_PR, i, j) += shift;
             }
             __syncthreads();
         }
         // residuals as mg_residual_kernel
         for (int e = threadIdx.x; e < nown; e += nt) {
-            const int i = I0 + e % ts.ti, j = J0 + e / ts.ti;
+            int i, j;
+                owned(e, i, j);
             if (j <= xnum && i <= ynum + 1) {
                 T r = T(0);
                 if (i >= 2 && i <= ynum && j >= 2 && j <= xnum - 1) r = vx_residual_v<T>(v, i, j, (T*)nullptr);
@@ -294,7 +315,8 @@ __global__ void __launch_bounds__(1024) mg_tile_smooth_kernel(MGLevelDev<T> l, T
     // ---- write back the owned velocities and pressures (to the level's own
     // arrays, or to its second buffers in an all-tiles launch)
     for (int e = threadIdx.x; e < nown; e += nt) {
-        const int i = I0 + e % ts.ti, j = J0 + e / ts.ti;
+        int i, j;
+                owned(e, i, j);
         if (tile_in_bounds(TF_VX, i, j, xnum, ynum)) ovx[(size_t)(j - 1) * (ynum + 1) + (i - 1)] = v.vx(i, j);
         if (tile_in_bounds(TF_VY, i, j, xnum, ynum)) ovy[(size_t)(j - 1) * ynum + (i - 1)] = v.vy(i, j);
         if (tile_in_bounds(TF_PR, i, j, xnum, ynum)) opr[(size_t)(j - 1) * (ynum - 1) + (i - 1)] = v.pr(i, j);
