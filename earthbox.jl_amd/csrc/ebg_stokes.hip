@@ -1229,6 +1229,11 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
             return EBG_ERR_NUMERIC;
         }
         if (relres <= o.rtol || total >= o.max_iterations) break;
+        // the network only within the first restart cycle: accepted
+        // corrections vary the preconditioner from step to step, and a solve
+        // that needs a restart finishes on the plain V-cycle (rift solves
+        // stalled for thousands of iterations otherwise)
+        if (restarts > 0) learned_on = false;
         EBG_HIP(hipMemcpyAsync(st.nrm.p, &beta, 8, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(scale_kernel, dim3(nb(n)), dim3(256), 0, s, V, r, st.nrm.as<double>(), n);
         std::fill(g.begin(), g.end(), 0.0);

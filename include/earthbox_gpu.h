@@ -575,7 +575,9 @@ typedef struct {
                              0 launches the four tile colours in turn (block Gauss-Seidel) */
     int learned_patience; /* learned 1: after this many consecutive rejections by the residual
                              judge the solve falls back to the plain V-cycle for its remaining
-                             iterations (default 4; 0: judge every iteration) */
+                             iterations (default 4; 0: judge every iteration).  Learned 1 and 2
+                             use the network only in the first restart cycle; a solve that
+                             restarts finishes on the plain V-cycle */
 } ebg_solve_opts;
 
 typedef struct {
