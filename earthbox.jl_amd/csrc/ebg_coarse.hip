@@ -363,8 +363,18 @@ __global__ void __launch_bounds__(256) coarse_gemv_kernel(const T* __restrict__ 
     const int lane = threadIdx.x & 63;
     if (row >= n) return;
     const T* a = inv + (size_t)row * n;
-    T acc = T(0);
-    for (int j = lane; j < n; j += 64) acc += a[j] * b[j];
+    // four independent partial sums per lane (four row segments of 64 in
+    // flight per step), added in a fixed order
+    T acc0 = T(0), acc1 = T(0), acc2 = T(0), acc3 = T(0);
+    int j = lane;
+    for (; j + 192 < n; j += 256) {
+        acc0 += a[j] * b[j];
+        acc1 += a[j + 64] * b[j + 64];
+        acc2 += a[j + 128] * b[j + 128];
+        acc3 += a[j + 192] * b[j + 192];
+    }
+    for (; j < n; j += 64) acc0 += a[j] * b[j];
+    T acc = (acc0 + acc1) + (acc2 + acc3);
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
     if (lane == 0) x[row] = acc;
 }
