@@ -173,8 +173,8 @@ __global__ void __launch_bounds__(1024) mg_tile_smooth_kernel(MGLevelDev<T> l, T
         syb[e] = (k >= 0 && k <= ynum - 2) ? l.ystp_b[k] : T(1);
         syv[e] = (k >= 0 && k <= ynum - 1) ? l.ystp_vx[k] : T(1);
     }
-    // ---- load the window: two entries per thread per round, every field's
-    // load issued before the first LDS store (one wait per round)
+    // ---- load the window: two entries per thread per pass, every field's
+    // load issued before the first LDS store (one wait per pass)
     for (int e0 = threadIdx.x; e0 < v.ww; e0 += 2 * nt) {
         T val[2][TF_N];
 #pragma unroll
