@@ -1074,9 +1074,11 @@ static int learned_step(ebg_ctx* ctx, StokesState& st, const double* v, double* 
 // enqueued past the converged one writes only basis slot j + 2, column j + 1
 // of H, rotation j + 1 and g[j + 1..j + 2], none of which the solution update
 // over the first j + 1 columns reads.  On return kcols is the number of
-// columns to combine; g[0] holds the cycle's initial residual norm.
+// columns to combine; g[0] holds the cycle's initial residual norm.  k0 > 0
+// continues a cycle whose first k0 steps ran on the host (their Hessenberg
+// columns, rotations and g[0..k0] already copied to the device state).
 static int arnoldi_cycle_device(StokesState& st, const ebg_solve_opts& o, const std::vector<int>& sm, int64_t n,
-                                double bnorm, int& total, int& kcols, hipStream_t s)
+                                double bnorm, int& total, int& kcols, hipStream_t s, int k0 = 0)
 {
     const int m = o.restart;
     const int nc = st.nc;
@@ -1107,7 +1109,7 @@ static int arnoldi_cycle_device(StokesState& st, const ebg_solve_opts& o, const 
         }
         return (res / bnorm <= o.rtol || hn == 0) ? 1 : 0;
     };
-    int k = 0;
+    int k = k0;
     for (; k < m && total < o.max_iterations; ++k) {
         double* vk = V + (int64_t)k * n;
         double* zk = Z + (int64_t)k * n;
@@ -1134,7 +1136,7 @@ static int arnoldi_cycle_device(StokesState& st, const ebg_solve_opts& o, const 
         EBG_HIP(hipGetLastError());
         EBG_HIP(hipEventRecord(st.step_done[k & 1], s));
         ++total;
-        if (k > 0) {
+        if (k > k0) {
             EBG_HIP(hipEventSynchronize(st.step_done[(k - 1) & 1]));
             const int c = check(k - 1);
             if (c < 0) return EBG_ERR_NUMERIC;
@@ -1145,7 +1147,7 @@ static int arnoldi_cycle_device(StokesState& st, const ebg_solve_opts& o, const 
             }
         }
     }
-    if (k > 0) {
+    if (k > k0) {
         EBG_HIP(hipEventSynchronize(st.step_done[(k - 1) & 1]));
         if (check(k - 1) < 0) return EBG_ERR_NUMERIC;
     }
@@ -1168,9 +1170,11 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
         (rc = st.hdev.ensure(MAXK * 8 * 2)) || (rc = st.ydev.ensure(MAXK * 8)) || (rc = st.nrm.ensure(64)))
         return rc;
     const bool dev_arnoldi = (o.learned == 0);
-    if (dev_arnoldi && ((rc = st.arn.ensure(((size_t)(m + 1) * m + 3 * (size_t)m + 1) * 8)) ||
-                        (rc = st.gate.ensure(64)) || (rc = st.ensure_status())))
+    if ((rc = st.arn.ensure(((size_t)(m + 1) * m + 3 * (size_t)m + 1) * 8)) || (rc = st.gate.ensure(64)) ||
+        (rc = st.ensure_status()))
         return rc;
+    double* const arn_cs = st.arn.as<double>() + (size_t)(m + 1) * m;
+    double* const arn_g = arn_cs + 2 * (size_t)m;
     std::vector<int> sm;
     const bool f32 = (o.precision == 1);
     st.use_graphs = ctx->stokes_graphs != 0 && !st.graph_broken;
@@ -1240,18 +1244,23 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
         g[0] = beta;
         int k = 0;
         bool done = false, pending = false;
-        if (dev_arnoldi) {
-            double* gd = st.arn.as<double>() + (size_t)(m + 1) * m + 2 * (size_t)m;
-            EBG_HIP(hipMemcpyAsync(gd, &beta, 8, hipMemcpyHostToDevice, s));
-            if ((rc = arnoldi_cycle_device(st, o, sm, n, bnorm, total, k, s))) return rc;
-            if (k > 0) {
-                EBG_HIP(hipMemcpyAsync(H.data(), st.arn.p, sizeof(double) * (size_t)(m + 1) * k,
+        // plain and mixed solves, and learned ones once the network is off
+        // (mode 3 keeps adding its directions), run the cycle on the device
+        const bool dev_cycle = dev_arnoldi || (!learned_on && o.learned != 3);
+        auto fetch_hg = [&](int kc) -> int {
+            if (kc > 0) {
+                EBG_HIP(hipMemcpyAsync(H.data(), st.arn.p, sizeof(double) * (size_t)(m + 1) * kc,
                                        hipMemcpyDeviceToHost, s));
-                EBG_HIP(hipMemcpyAsync(g.data(), gd, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+                EBG_HIP(hipMemcpyAsync(g.data(), arn_g, sizeof(double) * kc, hipMemcpyDeviceToHost, s));
                 EBG_HIP(hipStreamSynchronize(s));
             }
+            return EBG_OK;
+        };
+        if (dev_cycle) {
+            EBG_HIP(hipMemcpyAsync(arn_g, &beta, 8, hipMemcpyHostToDevice, s));
+            if ((rc = arnoldi_cycle_device(st, o, sm, n, bnorm, total, k, s)) || (rc = fetch_hg(k))) return rc;
         }
-        for (; !dev_arnoldi && k < m && total < o.max_iterations; ++k) {
+        for (; !dev_cycle && k < m && total < o.max_iterations; ++k) {
             double* vk = V + (int64_t)k * n;
             double* zk = Z + (int64_t)k * n;
             const bool learned_dir = pending;
@@ -1341,6 +1350,21 @@ static int fgmres(ebg_ctx* ctx, StokesState& st, const ebg_solve_opts& o, ebg_so
             if (std::fabs(g[k + 1]) / bnorm <= o.rtol || hn == 0) {
                 ++k;
                 done = true;
+                break;
+            }
+            // the network is off for the rest of the solve: the cycle's
+            // remaining steps run on the device, from the host's state
+            if (!learned_on && o.learned != 3 && k + 1 < m && total < o.max_iterations) {
+                double* ad = st.arn.as<double>();
+                EBG_HIP(hipMemcpyAsync(ad, H.data(), sizeof(double) * (size_t)(m + 1) * (k + 1),
+                                       hipMemcpyHostToDevice, s));
+                EBG_HIP(hipMemcpyAsync(arn_cs, cs.data(), sizeof(double) * (k + 1), hipMemcpyHostToDevice, s));
+                EBG_HIP(hipMemcpyAsync(arn_cs + m, sn.data(), sizeof(double) * (k + 1), hipMemcpyHostToDevice, s));
+                EBG_HIP(hipMemcpyAsync(arn_g, g.data(), sizeof(double) * (k + 2), hipMemcpyHostToDevice, s));
+                int kc = 0;
+                if ((rc = arnoldi_cycle_device(st, o, sm, n, bnorm, total, kc, s, k + 1)) || (rc = fetch_hg(kc)))
+                    return rc;
+                k = kc;
                 break;
             }
         }
